@@ -1,0 +1,233 @@
+"""GLR build + downstream benchmark on one MI355X.
+
+Workload (the plain-vs-mixed GPU build configuration): 2D 1024 x 1024 grid,
+n = 1,048,576; B an anisotropic Gaussian PSF (angle pi*x1, axes 3h / 1.5h,
+Mahalanobis <= 3, ~127 nnz/row); Z = M + 0.05 K; ell = 400 (q = 1) in
+64-column panels; Z^{-1} by blocked Chebyshev around the AMG V-cycle.
+A step is one mixed-precision lgh_glr_compute on inputs already resident in
+HBM.  Reported beside it: the plain FP64 build, the error of the mixed
+build against it, solve RHS/s with p50/p99 latency, the roofline of the
+dominant kernel (HIP events, algorithmic bytes), and the FP64 CPU oracle
+timed on a bounded sample of the same build.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "lgpsf-hessian_amd")
+sys.path.insert(0, ROOT)
+sys.path.insert(0, PKG_DIR)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+FP32_PEAK_TFS = 157.3
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location(
+        "lgpsf_hessian_amd", os.path.join(PKG_DIR, "__init__.py"),
+        submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["lgpsf_hessian_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(B, Z, M, lgh, ell, q, kept, budget_s):
+    """FP64 oracle (single thread) on a bounded sample: whole F sweeps of a
+    few sketch columns with the same Chebyshev-MG operator, scaled to the
+    (q+2) ell + kept columns of one build; the dense stages are not in the
+    sample (they are < 1% of the oracle's build time here)."""
+    import oracle as O
+    h = lgh.AMG(Z)
+    levels = [{"A": O.CSR.from_scipy(l["A"]),
+               "P": O.CSR.from_scipy(l["P"]) if l["P"] is not None else None,
+               "slo": l["slo"], "shi": l["shi"], "nu": l["nu"]} for l in h.levels()]
+    mg = O.MG(levels)
+    return mg, levels
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--side", type=int, default=1024)
+    ap.add_argument("--ell", type=int, default=400)
+    ap.add_argument("--mode", type=int, default=3)
+    ap.add_argument("--cpu-cols", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only (this build has no multi-GPU "
+              "path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import workloads as W
+    lgh = load_pkg()
+    t0 = time.time()
+    Bs, Zs, M = W.cfg2_problem(args.side)
+    n = Zs.shape[0]
+    log("problem n=%d nnz(B)=%d (%.1f/row) gen %.1fs" % (n, Bs.nnz, Bs.nnz / n,
+                                                           time.time() - t0))
+    lgh.init()
+    t0 = time.time()
+    prior = lgh.Prior.from_mat(Zs, M, blocked_mode=args.mode)
+    info = prior.info()
+    log("prior create %.1fs %s" % (time.time() - t0, info))
+    Bm = lgh.Mat.aij(Bs)
+    opts = lgh.glr_opts(ell=args.ell, q_power=1, seed=0)
+    mixed = lgh.xopts(mixed=True)
+    plain = lgh.xopts(mixed=False)
+
+    for w in range(args.warmup):
+        t0 = time.time()
+        g = lgh.GLR(Bm, prior, opts, x=mixed)
+        log("warmup build %d: %.2fs kept=%d" % (w, time.time() - t0, g.kept))
+        del g
+
+    lgh.prof_enable(True)
+    for k in lgh.PROF_KINDS:
+        lgh.prof_collect(k)
+    lgh.lib().lgh_hip_sync()
+    t0 = time.perf_counter()
+    reps = []
+    g = None
+    for s in range(args.steps):
+        g = lgh.GLR(Bm, prior, opts, x=mixed)
+        reps.append(g.report)
+    lgh.lib().lgh_hip_sync()
+    t_mixed = (time.perf_counter() - t0) / args.steps
+    lgh.prof_enable(False)
+    prof = {k: lgh.prof_collect(k) for k in lgh.PROF_KINDS}
+    log("mixed build %.3fs kept=%d report=%s" % (t_mixed, g.kept, reps[-1]))
+    for k, (c, ms, by, fl) in prof.items():
+        if c:
+            log("  %-12s launches %6d  %9.2f ms  avg %.3f ms  %7.1f GB/s  %6.2f TF/s"
+                % (k, c, ms, ms / c, by / ms / 1e6, fl / ms / 1e9))
+
+    # dominant kernel (by time) -> roofline
+    dom = max(prof, key=lambda k: prof[k][1])
+    c, ms, by, fl = prof[dom]
+    avg_ms = ms / c
+    if dom in ("gram", "gemm"):
+        ach = fl / c / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFS,
+                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFS, "traffic": None}
+    else:
+        ach = by / c / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
+    roof["kernel"] = dom
+    roof["avg_launch_ms"] = avg_ms
+    roof["launches"] = c
+    sp = prof["spmm_B"]
+    spmm_roof = None
+    if sp[0]:
+        a = sp[2] / sp[1] / 1e6
+        spmm_roof = {"kernel": "spmm_B", "achieved": a, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": a / HBM_PEAK_GBS,
+                     "avg_launch_ms": sp[1] / sp[0]}
+
+    extra = {}
+    if not args.no_plain:
+        t0 = time.perf_counter()
+        gp = lgh.GLR(Bm, prior, opts, x=plain)
+        lgh.lib().lgh_hip_sync()
+        t_plain = time.perf_counter() - t0
+        k = min(g.kept, gp.kept)
+        spec_err = float(np.max(np.abs(g.lam[:k] - gp.lam[:k]) / np.abs(gp.lam[:k])))
+        extra.update(plain_build_s=t_plain, kept_plain=gp.kept,
+                     mixed_vs_fp64_spectrum_rel=spec_err)
+        log("plain build %.3fs kept=%d; mixed-vs-fp64 spectrum rel err %.2e"
+            % (t_plain, gp.kept, spec_err))
+
+    # downstream: batched multi-shift solves (host arrays)
+    import oracle as O
+    rng = np.random.default_rng(0)
+    m = 64
+    R = O.randn_fill(99, 0, 0, n, m)
+    cs = 10.0 ** rng.uniform(-4, 1, size=m)
+    X = g.solve_block(cs, R)
+    t0 = time.perf_counter()
+    X = g.solve_block(cs, R)
+    t_blk = time.perf_counter() - t0
+    lat = []
+    for j in range(5):
+        t1 = time.perf_counter()
+        g.solve(cs[j], R[:, j])
+        lat.append(time.perf_counter() - t1)
+    extra.update(solve_rhs_per_s=m / t_blk, solve_block_m=m,
+                 solve_latency_p50_s=float(np.percentile(lat, 50)),
+                 solve_latency_p99_s=float(np.percentile(lat, 99)))
+    if not args.no_plain:
+        Xp = gp.solve_block(cs, R)
+        extra["mixed_vs_fp64_solve_rel"] = float(
+            np.max(np.linalg.norm(X - Xp, axis=0) / np.linalg.norm(Xp, axis=0)))
+    log("solve block m=%d: %.1f RHS/s; single-RHS p50 %.3fs" %
+        (m, m / t_blk, extra["solve_latency_p50_s"]))
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        mg, levels = cpu_baseline(Bs, Zs, M, lgh, args.ell, 1, g.kept, 0)
+        zs = O.ZSolver.mat(O.CSR.from_scipy(Zs), mode=3, mg=mg, lmin=info["lmin"],
+                           lmax=info["lmax"], nit=info["nit"], ksp_rtol=1e-12)
+        msq = np.sqrt(M)
+        Bc = O.CSR.from_scipy(Bs)
+        Xs = O.randn_fill(0, 0, 0, n, args.cpu_cols)
+        t0 = time.perf_counter()
+        Wk = Xs * msq[:, None]
+        T = zs.solve_block(Wk)
+        Wk = O.csr_mm(Bc, T)
+        T = zs.solve_block(Wk)
+        t_cols = time.perf_counter() - t0
+        cols_build = 3 * args.ell + g.kept
+        cpu = {"value": t_cols / args.cpu_cols * cols_build, "unit": "s/build",
+               "cores": 1, "kind": "port",
+               "sample": "%d full F-sweep column(s) of the FP64 oracle (same "
+                         "Chebyshev-MG operator) timed %.1fs, scaled to the "
+                         "%d F columns of one build (dense stages excluded)"
+                         % (args.cpu_cols, t_cols, cols_build)}
+        log("cpu baseline:", cpu)
+
+    line = {
+        "metric": "GLR build s + solve RHS/s, p99 latency; mixed vs FP64 plain at rel err <=1e-6",
+        "value": t_mixed,
+        "unit": "s/build",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mixed * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 sweeps + f64 final (mixed)",
+        "data": "synthetic",
+        "config": {"workload": "cfg2: 2D %dx%d grid n=%d, anisotropic PSF B "
+                               "(%.1f nnz/row), Z=M+0.05K, ell=%d q=1, w=64, "
+                               "Z^-1 = Chebyshev(%d) around AMG V-cycle"
+                               % (args.side, args.side, n, Bs.nnz / n, args.ell,
+                                  info["nit"]),
+                   "n": n, "ell": args.ell, "kept": g.kept},
+        "roofline": roof,
+        "spmm_roofline": spmm_roof,
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,92 @@
+/* lgh_hip.h — the thin C ABI of the gfx950 kernels and host set-up
+ * routines under the GLR engine (plain pointers and sizes; no PETSc or
+ * torch types).  Each kernel entry replaces one reference primitive:
+ *
+ *   lgh_hip_randn_fill   lgh_randn_at fills        glr_common_impl.h:441-443
+ *   lgh_hip_spmm         per-column MatMult(B)     glr_common_impl.h:325-332
+ *   lgh_hip_cheb_jacobi  lgh_zs_cheb_core + lgh_zs_rowscale_pc
+ *                                                  zsolve_impl.h:104-152
+ *   lgh_hip_gram         DGEMM("T","N") Gram / T   glr_common_impl.h:355, 481
+ *   lgh_hip_gemm_nn      DGEMM("N","N") / DTRSM    glr_common_impl.h:402, 567
+ *   lgh_amg_*            the PCGAMG hierarchy the reference harvests
+ *                                                  zsolve_impl.h:200-356
+ *
+ * Host arrays in and out (column-major n x m blocks with a leading
+ * dimension); the device data layout (panel matrices, see lgh_dev.hpp) is
+ * internal.  All functions return 0 on success; lgh_hip_last_error gives
+ * the message of the last failure.  Precision codes: 0 = FP64, 1 = FP32. */
+#ifndef LGH_HIP_H
+#define LGH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int         lgh_hip_init (void);
+const char *lgh_hip_last_error (void);
+int         lgh_hip_sync (void);
+/* bytes currently held by the engine's device allocator */
+int64_t     lgh_hip_bytes_in_use (void);
+void        lgh_hip_trim (void);
+
+/* live kernel timing: when enabled, every launch of the engine is bracketed
+ * by HIP events on the engine stream and credited with its algorithmic
+ * HBM bytes and flops.  kind: 0 B SpMM of the F sweep, 1 other SpMM,
+ * 2 fused Chebyshev step (Jacobi), 3 fused Chebyshev step (explicit Z),
+ * 4 residual, 5 Gram (MFMA), 6 GEMM (MFMA).  collect synchronizes and
+ * resets that kind's totals.  (No reference counterpart: measurement.) */
+void lgh_hip_prof_enable (int on);
+int  lgh_hip_prof_collect (int kind, int64_t *count, double *ms,
+                           double *bytes, double *flops);
+
+/* the device hashed-Gaussian arithmetic evaluated on the host (no GPU) */
+double lgh_hip_randn_at_host (uint64_t seed, int64_t i, int64_t j);
+
+/* out(r, c) = randn_at(seed, row0 + r, col0 + c) generated on the device
+ * with panel width W in precision prec, returned as FP64 */
+int lgh_hip_randn_fill (uint64_t seed, int64_t row0, int64_t col0, int nr,
+                        int nc, int W, int prec, double *out, int ld);
+
+/* Y = A X (A: n x m CSR) on ncols columns */
+int lgh_hip_spmm (int n, int m, const int *rp, const int *ci, const double *v,
+                  int prec_vals, int prec_vec, int W, int ncols,
+                  const double *X, int ldx, double *Y, int ldy);
+
+/* X = Chebyshev(nit) solve of A X = B with the Jacobi preconditioner on
+ * the interval [lmin, lmax], zero initial guess, one fused kernel per
+ * iteration */
+int lgh_hip_cheb_jacobi (int n, const int *rp, const int *ci, const double *v,
+                         int prec, int W, double lmin, double lmax, int nit,
+                         int ncols, const double *B, int ldb, double *X,
+                         int ldx);
+
+/* G (a x b) = X^T Y, X: n x a, Y: n x b */
+int lgh_hip_gram (int prec, int n, int a, const double *X, int ldx, int b,
+                  const double *Y, int ldy, double *G, int ldg);
+/* Y (n x b) = X C, X: n x a, C: a x b */
+int lgh_hip_gemm_nn (int prec, int n, int a, const double *X, int ldx, int b,
+                     const double *C, int ldc, double *Y, int ldy);
+
+/* ---- host AMG hierarchy (no GPU needed) ------------------------------ */
+typedef struct lgh_amg lgh_amg_t;
+int  lgh_amg_build (int n, const int *rp, const int *ci, const double *v,
+                    int smax_krylov, double smoother_top, int nu_force,
+                    lgh_amg_t **out);
+void lgh_amg_free (lgh_amg_t *h);
+int  lgh_amg_levels (const lgh_amg_t *h);
+/* level l (0 = coarsest): operator CSR, interpolation from l-1 (l > 0),
+ * smoother interval and degree; pointers are borrowed from h */
+int  lgh_amg_level (const lgh_amg_t *h, int l, int *n, int *nc,
+                    const int **rp, const int **ci, const double **v,
+                    const int **prp, const int **pci, const double **pv,
+                    double *slo, double *shi, int *nu);
+/* X = V-cycle(B) on the device (prec 0/1), ncols columns */
+int  lgh_amg_vcycle (const lgh_amg_t *h, int prec, int W, int ncols,
+                     const double *B, int ldb, double *X, int ldx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

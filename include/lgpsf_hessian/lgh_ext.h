@@ -1,0 +1,71 @@
+/* lgh_ext.h — extensions of the MI355X engine beyond the reference API
+ * (no counterpart in the reference): precision policy of the build,
+ * batched multi-shift downstream operations, and read-only introspection
+ * used by the parity tests and the benchmark. */
+#ifndef LGPSF_HESSIAN_LGH_EXT_H
+#define LGPSF_HESSIAN_LGH_EXT_H
+
+#include "lgpsf_hessian/glr.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LGH_PREC_F64 0
+#define LGH_PREC_F32 1
+
+/* Precision of each build stage.  FP64 everywhere reproduces the reference
+ * arithmetic ("plain"); the mixed policy runs the range-finder sweeps
+ * (sketch, power passes, their CholQR2 Grams) in FP32 and keeps the final
+ * sweep, the Rayleigh-Ritz projection, U, the Cholesky factors and the
+ * eigensolve in FP64. */
+typedef struct lgh_glr_xopts
+{
+  int prec_sweep;   /* sketch + power-pass F sweeps and their CholQR2      */
+  int prec_final;   /* last F sweep, T = Q^T F Q, U = Q C                  */
+  int prec_check;   /* residual-gate sweep (opts.check)                    */
+}
+lgh_glr_xopts_t;
+
+lgh_glr_xopts_t lgh_glr_xopts_default (void);   /* all LGH_PREC_F64 */
+lgh_glr_xopts_t lgh_glr_xopts_mixed (void);
+
+int lgh_glr_compute_x (Mat B, lgh_prior_t *prior, const lgh_glr_opts_t *opts,
+                       const lgh_glr_xopts_t *xopts, lgh_glr_t **glr,
+                       lgh_glr_report_t *report);
+
+/* Batched operations on m host columns (column-major, leading dimension
+ * ld), column j at shift c[j]:  x_j = H(c_j)^{-1} rhs_j  and
+ * x_j = G(c_j)^{-T} xi_j.  Same arithmetic per column as lgh_glr_solve /
+ * lgh_glr_sample. */
+int lgh_glr_solve_block (lgh_glr_t *glr, int m, const double *c,
+                         const double *rhs, int ldr, double *x, int ldx);
+int lgh_glr_sample_block (lgh_glr_t *glr, int m, const double *c,
+                          const double *xi, int ldr, double *x, int ldx);
+
+typedef struct lgh_prior_info
+{
+  int    is_mat;        /* 1: assembled Z, 0: callbacks                    */
+  int    blocked_mode;
+  int    nit;           /* blocked Chebyshev iterations                    */
+  double lmin, lmax;    /* margined preconditioned interval                */
+  int    levels;        /* AMG levels (0 when none)                        */
+  int    n_coarse;      /* coarsest AMG size                               */
+  int    bound_its;     /* iterations of the interval estimate             */
+}
+lgh_prior_info_t;
+
+int lgh_prior_get_info (const lgh_prior_t *prior, lgh_prior_info_t *info);
+
+/* Z-solves through the prior's tiers (host column-major blocks):
+ * which = 0 blocked tier (what the build uses), 1 vector (PCG) path.     */
+int lgh_prior_solve_host (lgh_prior_t *prior, int which, int prec, int m,
+                          const double *X, int ldx, double *Y, int ldy);
+
+/* copy of the kept U (n x kept, column-major) for tests */
+int lgh_glr_get_U (const lgh_glr_t *glr, double *U, int ldu);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

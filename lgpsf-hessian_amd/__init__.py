@@ -1,0 +1,608 @@
+"""lgpsf-hessian_amd — MI355X engine for lgpsf-hessian's GLR stage.
+
+Python bindings (ctypes) of liblgh.so: the reference's C API
+(lgh_prior_* / lgh_glr_*, include/lgpsf_hessian/*.h) over the PETSc-subset
+Mat/Vec types, and the thin lgh_hip_* kernel ABI (include/lgh_hip.h).
+The compute path is the HIP library; if it is missing or no GPU is
+visible the calls fail with an error instead of falling back.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblgh.so")
+_lib = None
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+vp = C.c_void_p
+
+PREC_F64 = 0
+PREC_F32 = 1
+
+
+class LGHError(RuntimeError):
+    pass
+
+
+def build(jobs=8):
+    """Compile liblgh.so for gfx950 (always recompiles what changed)."""
+    subprocess.run(["make", "-s", "-C", _HERE, "-j%d" % jobs], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LGHError("liblgh.so is not built (run build())")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+# ---- C structs ----------------------------------------------------------------
+
+class GlrOpts(C.Structure):
+    _fields_ = [("ell", C.c_int), ("q_power", C.c_int),
+                ("trunc_abs", C.c_double), ("trunc_rel", C.c_double),
+                ("treat", C.c_int), ("seed", C.c_ulong), ("backend", C.c_int),
+                ("nb", C.c_int), ("panel", C.c_int), ("grid2d", C.c_int),
+                ("check", C.c_int)]
+
+
+class GlrReport(C.Structure):
+    _fields_ = [("kept", C.c_int), ("lam_abs_max", C.c_double),
+                ("lam_raw_min", C.c_double), ("lam_raw_max", C.c_double),
+                ("next_abs", C.c_double), ("n_negative_raw", C.c_int),
+                ("resid_max", C.c_double), ("t_operator", C.c_double),
+                ("t_dense", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class GlrXOpts(C.Structure):
+    _fields_ = [("prec_sweep", C.c_int), ("prec_final", C.c_int),
+                ("prec_check", C.c_int)]
+
+
+class PriorMatOpts(C.Structure):
+    _fields_ = [("ksp_rtol", C.c_double), ("blocked_mode", C.c_int),
+                ("tile", C.c_int), ("cheb_rtol", C.c_double),
+                ("smax_krylov", C.c_int), ("smoother_top", C.c_double),
+                ("nu_force", C.c_int), ("verbose", C.c_int)]
+
+
+class PriorInfo(C.Structure):
+    _fields_ = [("is_mat", C.c_int), ("blocked_mode", C.c_int),
+                ("nit", C.c_int), ("lmin", C.c_double), ("lmax", C.c_double),
+                ("levels", C.c_int), ("n_coarse", C.c_int),
+                ("bound_its", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+VECFN = C.CFUNCTYPE(None, vp, vp, vp)
+MATFN = C.CFUNCTYPE(None, vp, vp, vp)
+
+
+class Callbacks(C.Structure):
+    _fields_ = [("applyZ", VECFN), ("applyZt", VECFN), ("solveZ", VECFN),
+                ("solveZt", VECFN), ("solveZ_blocked", MATFN),
+                ("solveZt_blocked", MATFN), ("ctx", vp)]
+
+
+# every exported function: name -> (restype, argtypes)
+_SIGS = {
+    # PETSc subset
+    "lgh_petsc_last_error": (C.c_char_p, []),
+    "VecCreateSeq": (C.c_int, [C.c_int, C.c_int, C.POINTER(vp)]),
+    "VecDestroy": (C.c_int, [C.POINTER(vp)]),
+    "VecGetArray": (C.c_int, [vp, C.POINTER(dp)]),
+    "VecRestoreArray": (C.c_int, [vp, C.POINTER(dp)]),
+    "VecGetLocalSize": (C.c_int, [vp, ip]),
+    "MatCreateSeqAIJWithArrays": (C.c_int, [C.c_int, C.c_int, C.c_int, ip, ip,
+                                            dp, C.POINTER(vp)]),
+    "MatCreateSeqDense": (C.c_int, [C.c_int, C.c_int, C.c_int, dp,
+                                    C.POINTER(vp)]),
+    "MatDestroy": (C.c_int, [C.POINTER(vp)]),
+    "MatDenseGetArray": (C.c_int, [vp, C.POINTER(dp)]),
+    "MatDenseRestoreArray": (C.c_int, [vp, C.POINTER(dp)]),
+    "MatGetSize": (C.c_int, [vp, ip, ip]),
+    # reference API
+    "lgh_prior_mat_opts_default": (PriorMatOpts, []),
+    "lgh_prior_create_mat": (C.c_int, [vp, vp, C.POINTER(PriorMatOpts),
+                                       C.POINTER(vp)]),
+    "lgh_prior_create_callbacks": (C.c_int, [vp, C.POINTER(Callbacks),
+                                             C.POINTER(vp)]),
+    "lgh_prior_create_ksp": (C.c_int, [vp, vp, C.POINTER(PriorMatOpts),
+                                       C.POINTER(vp)]),
+    "lgh_prior_destroy": (None, [vp]),
+    "lgh_prior_apply": (C.c_int, [vp, vp, vp]),
+    "lgh_glr_opts_default": (GlrOpts, []),
+    "lgh_glr_compute": (C.c_int, [vp, vp, C.POINTER(GlrOpts), C.POINTER(vp),
+                                  C.POINTER(GlrReport)]),
+    "lgh_glr_extend": (C.c_int, [vp, C.c_int, C.POINTER(GlrReport)]),
+    "lgh_glr_destroy": (None, [vp]),
+    "lgh_glr_solve": (C.c_int, [vp, C.c_double, vp, vp]),
+    "lgh_glr_apply": (C.c_int, [vp, C.c_double, vp, vp]),
+    "lgh_glr_sample": (C.c_int, [vp, C.c_double, vp, vp]),
+    "lgh_glr_factor": (C.c_int, [vp, C.c_double, C.c_int, vp, vp]),
+    "lgh_glr_logdet": (C.c_double, [vp, C.c_double]),
+    "lgh_glr_pow": (C.c_int, [vp, C.c_double, C.c_double, vp, vp]),
+    "lgh_glr_filter": (C.c_int, [vp, dp, C.c_double, vp, vp]),
+    "lgh_glr_eigs": (C.c_int, [vp, ip, C.POINTER(dp)]),
+    # extensions
+    "lgh_glr_xopts_default": (GlrXOpts, []),
+    "lgh_glr_xopts_mixed": (GlrXOpts, []),
+    "lgh_glr_compute_x": (C.c_int, [vp, vp, C.POINTER(GlrOpts),
+                                    C.POINTER(GlrXOpts), C.POINTER(vp),
+                                    C.POINTER(GlrReport)]),
+    "lgh_glr_solve_block": (C.c_int, [vp, C.c_int, dp, dp, C.c_int, dp,
+                                      C.c_int]),
+    "lgh_glr_sample_block": (C.c_int, [vp, C.c_int, dp, dp, C.c_int, dp,
+                                       C.c_int]),
+    "lgh_prior_get_info": (C.c_int, [vp, C.POINTER(PriorInfo)]),
+    "lgh_prior_solve_host": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp,
+                                       C.c_int, dp, C.c_int]),
+    "lgh_glr_get_U": (C.c_int, [vp, dp, C.c_int]),
+    # kernel ABI
+    "lgh_hip_init": (C.c_int, []),
+    "lgh_hip_last_error": (C.c_char_p, []),
+    "lgh_hip_sync": (C.c_int, []),
+    "lgh_hip_bytes_in_use": (C.c_int64, []),
+    "lgh_hip_trim": (None, []),
+    "lgh_hip_prof_enable": (None, [C.c_int]),
+    "lgh_hip_prof_collect": (C.c_int, [C.c_int, C.POINTER(C.c_int64), dp, dp, dp]),
+    "lgh_hip_randn_at_host": (C.c_double, [C.c_uint64, C.c_int64, C.c_int64]),
+    "lgh_hip_randn_fill": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.c_int,
+                                     C.c_int, C.c_int, C.c_int, dp, C.c_int]),
+    "lgh_hip_spmm": (C.c_int, [C.c_int, C.c_int, ip, ip, dp, C.c_int, C.c_int,
+                               C.c_int, C.c_int, dp, C.c_int, dp, C.c_int]),
+    "lgh_hip_cheb_jacobi": (C.c_int, [C.c_int, ip, ip, dp, C.c_int, C.c_int,
+                                      C.c_double, C.c_double, C.c_int, C.c_int,
+                                      dp, C.c_int, dp, C.c_int]),
+    "lgh_hip_gram": (C.c_int, [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int,
+                               dp, C.c_int, dp, C.c_int]),
+    "lgh_hip_gemm_nn": (C.c_int, [C.c_int, C.c_int, C.c_int, dp, C.c_int,
+                                  C.c_int, dp, C.c_int, dp, C.c_int]),
+    "lgh_amg_build": (C.c_int, [C.c_int, ip, ip, dp, C.c_int, C.c_double,
+                                C.c_int, C.POINTER(vp)]),
+    "lgh_amg_free": (None, [vp]),
+    "lgh_amg_levels": (C.c_int, [vp]),
+    "lgh_amg_level": (C.c_int, [vp, C.c_int, ip, ip, C.POINTER(ip),
+                                C.POINTER(ip), C.POINTER(dp), C.POINTER(ip),
+                                C.POINTER(ip), C.POINTER(dp), dp, dp, ip]),
+    "lgh_amg_vcycle": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, C.c_int,
+                                 dp, C.c_int]),
+}
+
+
+def _declare(L):
+    for name, (res, args) in _SIGS.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _d(a):
+    return a.ctypes.data_as(dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(ip)
+
+
+def _hip_check(rc, what):
+    if rc != 0:
+        msg = lib().lgh_hip_last_error().decode()
+        raise LGHError("%s failed (%d): %s" % (what, rc, msg))
+
+
+def _petsc_check(rc, what):
+    if rc != 0:
+        msg = lib().lgh_petsc_last_error().decode()
+        raise LGHError("%s failed (PETSc code %d): %s" % (what, rc, msg))
+
+
+def _fcol(a):
+    a = np.asarray(a, dtype=np.float64)
+    if a.ndim == 1:
+        a = a[:, None]
+    return np.asfortranarray(a)
+
+
+# ---- PETSc-subset objects -------------------------------------------------------
+
+class Vec:
+    def __init__(self, n_or_array):
+        L = lib()
+        h = vp()
+        if np.isscalar(n_or_array):
+            n = int(n_or_array)
+            _petsc_check(L.VecCreateSeq(0, n, C.byref(h)), "VecCreateSeq")
+            self.h, self.n = h, n
+        else:
+            a = np.asarray(n_or_array, dtype=np.float64).ravel()
+            _petsc_check(L.VecCreateSeq(0, len(a), C.byref(h)), "VecCreateSeq")
+            self.h, self.n = h, len(a)
+            self.set(a)
+
+    def array(self):
+        p = dp()
+        lib().VecGetArray(self.h, C.byref(p))
+        return np.ctypeslib.as_array(p, shape=(self.n,))
+
+    def set(self, a):
+        self.array()[:] = a
+
+    def get(self):
+        return self.array().copy()
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.VecDestroy(C.byref(self.h))
+            self.h = None
+
+
+class Mat:
+    def __init__(self, handle, keep=()):
+        self.h = handle
+        self._keep = keep
+
+    @classmethod
+    def aij(cls, A):
+        """AIJ Mat from a scipy sparse matrix (values copied)."""
+        import scipy.sparse as sp
+        A = sp.csr_matrix(A)
+        A.sort_indices()
+        rp = np.ascontiguousarray(A.indptr, dtype=np.int32)
+        ci = np.ascontiguousarray(A.indices, dtype=np.int32)
+        v = np.ascontiguousarray(A.data, dtype=np.float64)
+        h = vp()
+        _petsc_check(lib().MatCreateSeqAIJWithArrays(0, A.shape[0], A.shape[1],
+                                                     _i(rp), _i(ci), _d(v),
+                                                     C.byref(h)),
+                     "MatCreateSeqAIJWithArrays")
+        return cls(h)
+
+    @classmethod
+    def dense(cls, D):
+        D = np.asfortranarray(D, dtype=np.float64)
+        h = vp()
+        _petsc_check(lib().MatCreateSeqDense(0, D.shape[0], D.shape[1], _d(D),
+                                             C.byref(h)), "MatCreateSeqDense")
+        return cls(h, keep=(D,))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.MatDestroy(C.byref(self.h))
+            self.h = None
+
+
+# ---- reference API ----------------------------------------------------------------
+
+def prior_mat_opts(**kw):
+    o = lib().lgh_prior_mat_opts_default()
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def glr_opts(**kw):
+    o = lib().lgh_glr_opts_default()
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def xopts(mixed=False, **kw):
+    x = lib().lgh_glr_xopts_mixed() if mixed else lib().lgh_glr_xopts_default()
+    for k, v in kw.items():
+        setattr(x, k, v)
+    return x
+
+
+class Prior:
+    def __init__(self, h, keep=()):
+        self.h = h
+        self._keep = keep
+
+    @classmethod
+    def from_mat(cls, Z, mass, **opts):
+        Zm = Z if isinstance(Z, Mat) else Mat.aij(Z)
+        mv = mass if isinstance(mass, Vec) else Vec(mass)
+        o = prior_mat_opts(**opts)
+        h = vp()
+        _petsc_check(lib().lgh_prior_create_mat(Zm.h, mv.h, C.byref(o),
+                                                C.byref(h)),
+                     "lgh_prior_create_mat")
+        return cls(h, keep=(Zm, mv))
+
+    @classmethod
+    def diagonal_callbacks(cls, zdiag, mass):
+        """Prior with Z = diag(zdiag) supplied through the Vec callbacks
+        (the exact "fake solver" prior of tests/test_glr.cpp:67-85)."""
+        L = lib()
+        zd = np.asarray(zdiag, dtype=np.float64).copy()
+        n = len(zd)
+
+        def arr(h):
+            p = dp()
+            L.VecGetArray(h, C.byref(p))
+            return np.ctypeslib.as_array(p, shape=(n,))
+
+        def applyZ(x, y, ctx):
+            arr(y)[:] = arr(x) * zd
+
+        def solveZ(x, y, ctx):
+            arr(y)[:] = arr(x) / zd
+
+        cbs = (VECFN(applyZ), VECFN(solveZ))
+        cb = Callbacks(cbs[0], VECFN(), cbs[1], VECFN(), MATFN(), MATFN(), None)
+        mv = mass if isinstance(mass, Vec) else Vec(mass)
+        h = vp()
+        _petsc_check(L.lgh_prior_create_callbacks(mv.h, C.byref(cb), C.byref(h)),
+                     "lgh_prior_create_callbacks")
+        return cls(h, keep=(mv, cbs, cb, zd))
+
+    def info(self):
+        i = PriorInfo()
+        lib().lgh_prior_get_info(self.h, C.byref(i))
+        return i.as_dict()
+
+    def solve(self, X, which=0, prec=PREC_F64):
+        X = _fcol(X)
+        Y = np.zeros_like(X, order="F")
+        _petsc_check(lib().lgh_prior_solve_host(self.h, which, prec, X.shape[1],
+                                                _d(X), X.shape[0], _d(Y),
+                                                X.shape[0]),
+                     "lgh_prior_solve_host")
+        return Y
+
+    def apply(self, x):
+        xv, yv = Vec(x), Vec(len(x))
+        _petsc_check(lib().lgh_prior_apply(self.h, xv.h, yv.h), "lgh_prior_apply")
+        return yv.get()
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.lgh_prior_destroy(self.h)
+            self.h = None
+
+
+class GLR:
+    def __init__(self, B, prior, opts=None, x=None, **kw):
+        self.Bm = B if isinstance(B, Mat) else Mat.aij(B)
+        self.prior = prior
+        o = opts if opts is not None else glr_opts(**kw)
+        rep = GlrReport()
+        h = vp()
+        if x is None:
+            rc = lib().lgh_glr_compute(self.Bm.h, prior.h, C.byref(o),
+                                       C.byref(h), C.byref(rep))
+        else:
+            rc = lib().lgh_glr_compute_x(self.Bm.h, prior.h, C.byref(o),
+                                         C.byref(x), C.byref(h), C.byref(rep))
+        _petsc_check(rc, "lgh_glr_compute")
+        self.h = h
+        self.report = rep.as_dict()
+        n = C.c_int()
+        lib().MatGetSize(self.Bm.h, C.byref(n), None)
+        self.n = n.value
+
+    def extend(self, k_new):
+        rep = GlrReport()
+        _petsc_check(lib().lgh_glr_extend(self.h, k_new, C.byref(rep)),
+                     "lgh_glr_extend")
+        self.report = rep.as_dict()
+        return self.report
+
+    @property
+    def lam(self):
+        k = C.c_int()
+        p = dp()
+        lib().lgh_glr_eigs(self.h, C.byref(k), C.byref(p))
+        if k.value == 0:
+            return np.zeros(0)
+        return np.ctypeslib.as_array(p, shape=(k.value,)).copy()
+
+    @property
+    def kept(self):
+        return len(self.lam)
+
+    @property
+    def U(self):
+        k = self.kept
+        U = np.zeros((self.n, k), order="F")
+        if k:
+            _petsc_check(lib().lgh_glr_get_U(self.h, _d(U), self.n), "lgh_glr_get_U")
+        return U
+
+    def _vop(self, fn, name, *scal, v):
+        xv, yv = Vec(v), Vec(self.n)
+        _petsc_check(fn(self.h, *scal, xv.h, yv.h), name)
+        return yv.get()
+
+    def solve(self, c, rhs):
+        return self._vop(lib().lgh_glr_solve, "lgh_glr_solve", c, v=rhs)
+
+    def apply(self, c, v):
+        return self._vop(lib().lgh_glr_apply, "lgh_glr_apply", c, v=v)
+
+    def sample(self, c, xi):
+        return self._vop(lib().lgh_glr_sample, "lgh_glr_sample", c, v=xi)
+
+    def factor(self, c, which, v):
+        return self._vop(lib().lgh_glr_factor, "lgh_glr_factor", c, which, v=v)
+
+    def pow(self, c, p, v):
+        return self._vop(lib().lgh_glr_pow, "lgh_glr_pow", c, p, v=v)
+
+    def filter(self, f_lam, f_c, v):
+        f = np.ascontiguousarray(f_lam, dtype=np.float64)
+        return self._vop(lib().lgh_glr_filter, "lgh_glr_filter", _d(f), f_c, v=v)
+
+    def logdet(self, c):
+        return lib().lgh_glr_logdet(self.h, c)
+
+    def _block(self, fn, name, c, X):
+        X = _fcol(X)
+        m = X.shape[1]
+        cc = np.ascontiguousarray(np.broadcast_to(np.asarray(c, dtype=np.float64), (m,)))
+        Y = np.zeros_like(X, order="F")
+        _petsc_check(fn(self.h, m, _d(cc), _d(X), X.shape[0], _d(Y), X.shape[0]), name)
+        return Y
+
+    def solve_block(self, c, X):
+        return self._block(lib().lgh_glr_solve_block, "lgh_glr_solve_block", c, X)
+
+    def sample_block(self, c, X):
+        return self._block(lib().lgh_glr_sample_block, "lgh_glr_sample_block", c, X)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.lgh_glr_destroy(self.h)
+            self.h = None
+
+
+# ---- kernel ABI -------------------------------------------------------------------
+
+def init():
+    _hip_check(lib().lgh_hip_init(), "lgh_hip_init")
+
+
+PROF_KINDS = {"spmm_B": 0, "spmm": 1, "cheb_jacobi": 2, "cheb_z": 3,
+              "resid": 4, "gram": 5, "gemm": 6}
+
+
+def prof_enable(on=True):
+    lib().lgh_hip_prof_enable(1 if on else 0)
+
+
+def prof_collect(kind):
+    """(launches, total ms, algorithmic bytes, flops) since the last collect"""
+    c = C.c_int64()
+    ms, by, fl = C.c_double(), C.c_double(), C.c_double()
+    _hip_check(lib().lgh_hip_prof_collect(PROF_KINDS.get(kind, kind), C.byref(c),
+                                          C.byref(ms), C.byref(by), C.byref(fl)),
+               "lgh_hip_prof_collect")
+    return c.value, ms.value, by.value, fl.value
+
+
+def randn_at_host(seed, i, j):
+    return lib().lgh_hip_randn_at_host(seed, i, j)
+
+
+def randn_fill(seed, row0, col0, nr, nc, W=64, prec=PREC_F64):
+    out = np.zeros((nr, nc), order="F")
+    _hip_check(lib().lgh_hip_randn_fill(seed, row0, col0, nr, nc, W, prec,
+                                        _d(out), nr), "lgh_hip_randn_fill")
+    return out
+
+
+def _csr(A):
+    import scipy.sparse as sp
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    return (np.ascontiguousarray(A.indptr, dtype=np.int32),
+            np.ascontiguousarray(A.indices, dtype=np.int32),
+            np.ascontiguousarray(A.data, dtype=np.float64))
+
+
+def spmm(A, X, prec_vals=PREC_F64, prec_vec=PREC_F64, W=64):
+    rp, ci, v = _csr(A)
+    X = _fcol(X)
+    Y = np.zeros((A.shape[0], X.shape[1]), order="F")
+    _hip_check(lib().lgh_hip_spmm(A.shape[0], A.shape[1], _i(rp), _i(ci), _d(v),
+                                  prec_vals, prec_vec, W, X.shape[1], _d(X),
+                                  X.shape[0], _d(Y), A.shape[0]), "lgh_hip_spmm")
+    return Y
+
+
+def cheb_jacobi(A, lmin, lmax, nit, B, prec=PREC_F64, W=64):
+    rp, ci, v = _csr(A)
+    B = _fcol(B)
+    X = np.zeros_like(B, order="F")
+    _hip_check(lib().lgh_hip_cheb_jacobi(A.shape[0], _i(rp), _i(ci), _d(v), prec,
+                                         W, lmin, lmax, nit, B.shape[1], _d(B),
+                                         B.shape[0], _d(X), B.shape[0]),
+               "lgh_hip_cheb_jacobi")
+    return X
+
+
+def gram(X, Y, prec=PREC_F64):
+    X, Y = _fcol(X), _fcol(Y)
+    G = np.zeros((X.shape[1], Y.shape[1]), order="F")
+    _hip_check(lib().lgh_hip_gram(prec, X.shape[0], X.shape[1], _d(X), X.shape[0],
+                                  Y.shape[1], _d(Y), Y.shape[0], _d(G),
+                                  X.shape[1]), "lgh_hip_gram")
+    return G
+
+
+def gemm_nn(X, Cm, prec=PREC_F64):
+    X, Cm = _fcol(X), _fcol(Cm)
+    Y = np.zeros((X.shape[0], Cm.shape[1]), order="F")
+    _hip_check(lib().lgh_hip_gemm_nn(prec, X.shape[0], X.shape[1], _d(X),
+                                     X.shape[0], Cm.shape[1], _d(Cm),
+                                     Cm.shape[0], _d(Y), X.shape[0]),
+               "lgh_hip_gemm_nn")
+    return Y
+
+
+class AMG:
+    """Host-built hierarchy (lgh_amg_build); levels() exports it."""
+
+    def __init__(self, Z, smax_krylov=1, smoother_top=1.1, nu_force=0):
+        rp, ci, v = _csr(Z)
+        h = vp()
+        _hip_check(lib().lgh_amg_build(Z.shape[0], _i(rp), _i(ci), _d(v),
+                                       smax_krylov, smoother_top, nu_force,
+                                       C.byref(h)), "lgh_amg_build")
+        self.h = h
+
+    def levels(self):
+        import scipy.sparse as sp
+        L = lib()
+        out = []
+        for l in range(L.lgh_amg_levels(self.h)):
+            n, nc, nu = C.c_int(), C.c_int(), C.c_int()
+            rp, ci, prp, pci = ip(), ip(), ip(), ip()
+            v, pv = dp(), dp()
+            slo, shi = C.c_double(), C.c_double()
+            L.lgh_amg_level(self.h, l, C.byref(n), C.byref(nc), C.byref(rp),
+                            C.byref(ci), C.byref(v), C.byref(prp), C.byref(pci),
+                            C.byref(pv), C.byref(slo), C.byref(shi), C.byref(nu))
+            nn = n.value
+            r = np.ctypeslib.as_array(rp, shape=(nn + 1,)).copy()
+            nnz = int(r[-1])
+            A = sp.csr_matrix((np.ctypeslib.as_array(v, shape=(max(nnz, 1),))[:nnz].copy(),
+                               np.ctypeslib.as_array(ci, shape=(max(nnz, 1),))[:nnz].copy(), r),
+                              shape=(nn, nn))
+            P = None
+            if l > 0:
+                pr = np.ctypeslib.as_array(prp, shape=(nn + 1,)).copy()
+                pn = int(pr[-1])
+                P = sp.csr_matrix((np.ctypeslib.as_array(pv, shape=(pn,)).copy(),
+                                   np.ctypeslib.as_array(pci, shape=(pn,)).copy(), pr),
+                                  shape=(nn, nc.value))
+            out.append({"A": A, "P": P, "slo": slo.value, "shi": shi.value,
+                        "nu": nu.value})
+        return out
+
+    def vcycle(self, B, prec=PREC_F64, W=64):
+        B = _fcol(B)
+        X = np.zeros_like(B, order="F")
+        _hip_check(lib().lgh_amg_vcycle(self.h, prec, W, B.shape[1], _d(B),
+                                        B.shape[0], _d(X), B.shape[0]),
+                   "lgh_amg_vcycle")
+        return X
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.lgh_amg_free(self.h)
+            self.h = None

@@ -1,0 +1,319 @@
+// device.hip — context, stream, caching allocator, CSR upload, rocSOLVER
+// glue.  One process drives one GPU (device 0) on one non-blocking stream
+// at default priority; every kernel of the engine is enqueued on it.
+#include "lgh_dev.hpp"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <map>
+#include <mutex>
+#include <cstdio>
+#include <cstring>
+
+namespace lgh {
+
+static thread_local std::string g_err;
+void set_error (const std::string &m) { g_err = m; }
+const char *last_error () { return g_err.c_str (); }
+
+int
+hip_fail (hipError_t e, const char *what, const char *file, int line)
+{
+  char buf[512];
+  snprintf (buf, sizeof buf, "%s failed: %s (%s:%d)", what,
+            hipGetErrorString (e), file, line);
+  g_err = buf;
+  return 2;
+}
+
+static bool g_init = false;
+static hipStream_t g_stream = nullptr;
+static rocblas_handle g_rb = nullptr;
+static int *g_info_dev = nullptr;
+static std::mutex g_mu;
+
+int
+dev_init ()
+{
+  std::lock_guard<std::mutex> lk (g_mu);
+  if (g_init) return 0;
+  int ndev = 0;
+  LGH_HIP (hipGetDeviceCount (&ndev));
+  LGH_CHECK (ndev > 0, 3, "no HIP device visible: the engine needs a GPU");
+  LGH_HIP (hipSetDevice (0));
+  LGH_HIP (hipStreamCreateWithFlags (&g_stream, hipStreamNonBlocking));
+  if (rocblas_create_handle (&g_rb) != rocblas_status_success) {
+    set_error ("rocblas_create_handle failed");
+    return 4;
+  }
+  rocblas_set_stream (g_rb, g_stream);
+  LGH_HIP (hipMalloc (&g_info_dev, sizeof (int) * 4));
+  g_init = true;
+  return 0;
+}
+
+hipStream_t stream () { return g_stream; }
+void *rocblas_h () { return (void *) g_rb; }
+
+int
+dev_sync ()
+{
+  LGH_HIP (hipStreamSynchronize (g_stream));
+  return 0;
+}
+
+// ---- live kernel timing -----------------------------------------------------
+struct ProfRec { int kind; hipEvent_t a, b; double bytes, flops; };
+static int g_prof = 0;
+static thread_local int g_btag = 0;
+static std::vector<ProfRec> g_recs;
+static std::vector<hipEvent_t> g_evpool;
+
+static hipEvent_t
+ev_get ()
+{
+  if (!g_evpool.empty ()) { hipEvent_t e = g_evpool.back (); g_evpool.pop_back (); return e; }
+  hipEvent_t e = nullptr;
+  (void) hipEventCreate (&e);
+  return e;
+}
+
+void prof_enable (int on) { g_prof = on; }
+int prof_enabled () { return g_prof; }
+void prof_set_b_tag (int on) { g_btag = on; }
+int prof_b_tag () { return g_btag; }
+
+ProfScope::ProfScope (int k, double by, double fl)
+  : kind (k), bytes (by), flops (fl), active (g_prof), ev0 (nullptr)
+{
+  if (!active) return;
+  hipEvent_t a = ev_get ();
+  (void) hipEventRecord (a, g_stream);
+  ev0 = (void *) a;
+}
+
+ProfScope::~ProfScope ()
+{
+  if (!active) return;
+  hipEvent_t b = ev_get ();
+  (void) hipEventRecord (b, g_stream);
+  g_recs.push_back ({ kind, (hipEvent_t) ev0, b, bytes, flops });
+}
+
+int
+prof_collect (int kind, long *count, double *ms, double *bytes, double *flops)
+{
+  LGH_HIP (hipStreamSynchronize (g_stream));
+  long c = 0;
+  double t = 0, by = 0, fl = 0;
+  std::vector<ProfRec> keep;
+  for (auto &r : g_recs) {
+    if (r.kind != kind) { keep.push_back (r); continue; }
+    float e = 0.f;
+    LGH_HIP (hipEventElapsedTime (&e, r.a, r.b));
+    c++; t += e; by += r.bytes; fl += r.flops;
+    g_evpool.push_back (r.a);
+    g_evpool.push_back (r.b);
+  }
+  g_recs.swap (keep);
+  *count = c; *ms = t; *bytes = by; *flops = fl;
+  return 0;
+}
+
+// ---- caching allocator ----------------------------------------------------
+// Blocks are rounded up (256 B granularity below 1 MiB, 2 MiB above) and
+// recycled by exact rounded size.
+static std::multimap<size_t, void *> g_free;
+static std::map<void *, size_t> g_live;
+static size_t g_in_use = 0;
+
+static size_t
+round_up (size_t b)
+{
+  if (b < ((size_t) 1 << 20)) return (b + 255) & ~(size_t) 255;
+  const size_t g = (size_t) 2 << 20;
+  return (b + g - 1) / g * g;
+}
+
+void *
+dmalloc (size_t bytes)
+{
+  if (dev_init ()) return nullptr;
+  std::lock_guard<std::mutex> lk (g_mu);
+  size_t b = round_up (bytes ? bytes : 1);
+  auto it = g_free.find (b);
+  void *p = nullptr;
+  if (it != g_free.end ()) { p = it->second; g_free.erase (it); }
+  else {
+    hipError_t e = hipMalloc (&p, b);
+    if (e != hipSuccess) {
+      // release the cache and retry once
+      for (auto &kv : g_free) (void) hipFree (kv.second);
+      g_free.clear ();
+      e = hipMalloc (&p, b);
+      if (e != hipSuccess) {
+        char buf[256];
+        snprintf (buf, sizeof buf, "device allocation of %zu bytes failed: %s",
+                  b, hipGetErrorString (e));
+        g_err = buf;
+        return nullptr;
+      }
+    }
+  }
+  g_live[p] = b;
+  g_in_use += b;
+  return p;
+}
+
+void
+dfree (void *p)
+{
+  if (!p) return;
+  std::lock_guard<std::mutex> lk (g_mu);
+  auto it = g_live.find (p);
+  if (it == g_live.end ()) return;
+  g_in_use -= it->second;
+  g_free.insert ({ it->second, p });
+  g_live.erase (it);
+}
+
+void
+dtrim ()
+{
+  std::lock_guard<std::mutex> lk (g_mu);
+  if (g_stream) (void) hipStreamSynchronize (g_stream);
+  for (auto &kv : g_free) (void) hipFree (kv.second);
+  g_free.clear ();
+}
+
+size_t dbytes_in_use () { return g_in_use; }
+
+int
+h2d (void *dst, const void *src, size_t bytes)
+{
+  if (!bytes) return 0;
+  LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+  LGH_HIP (hipStreamSynchronize (g_stream));
+  return 0;
+}
+
+int
+d2h (void *dst, const void *src, size_t bytes)
+{
+  if (!bytes) return 0;
+  LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+  LGH_HIP (hipStreamSynchronize (g_stream));
+  return 0;
+}
+
+int
+dcopy (void *dst, const void *src, size_t bytes)
+{
+  if (!bytes) return 0;
+  LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
+  return 0;
+}
+
+// ---- CSR / panel matrices -------------------------------------------------
+DevCSR::~DevCSR ()
+{
+  dfree (rp); dfree (ci); dfree (v); dfree (dinv);
+}
+
+int
+csr_upload (int n, int m, const int *rp, const int *ci, const double *v,
+            int prec, int want_dinv, DevCSR **out)
+{
+  LGH_TRY (dev_init ());
+  DevCSR *A = new DevCSR ();
+  A->n = n; A->m = m; A->nnz = rp[n]; A->prec = prec;
+  const size_t vb = prec_bytes (prec);
+  A->rp = (int *) dmalloc (sizeof (int) * (size_t) (n + 1));
+  A->ci = (int *) dmalloc (sizeof (int) * (size_t) (A->nnz ? A->nnz : 1));
+  A->v = dmalloc (vb * (size_t) (A->nnz ? A->nnz : 1));
+  if (!A->rp || !A->ci || !A->v) { delete A; return 5; }
+  int rc = h2d (A->rp, rp, sizeof (int) * (size_t) (n + 1));
+  if (!rc) rc = h2d (A->ci, ci, sizeof (int) * (size_t) A->nnz);
+  std::vector<float> vf;
+  if (!rc) {
+    if (prec == F64) rc = h2d (A->v, v, sizeof (double) * (size_t) A->nnz);
+    else {
+      vf.resize ((size_t) A->nnz);
+      for (int64_t k = 0; k < A->nnz; k++) vf[k] = (float) v[k];
+      rc = h2d (A->v, vf.data (), sizeof (float) * (size_t) A->nnz);
+    }
+  }
+  if (!rc && want_dinv && n == m) {
+    std::vector<double> d ((size_t) n, 0.0);
+    for (int i = 0; i < n; i++)
+      for (int k = rp[i]; k < rp[i + 1]; k++)
+        if (ci[k] == i) d[i] += v[k];
+    for (int i = 0; i < n; i++) d[i] = 1.0 / d[i];
+    A->dinv = dmalloc (vb * (size_t) n);
+    if (!A->dinv) rc = 5;
+    else if (prec == F64) rc = h2d (A->dinv, d.data (), sizeof (double) * n);
+    else {
+      std::vector<float> df (d.begin (), d.end ());
+      rc = h2d (A->dinv, df.data (), sizeof (float) * n);
+    }
+  }
+  if (rc) { delete A; return rc; }
+  *out = A;
+  return 0;
+}
+
+int
+pmat_alloc (PMat &m, int nrows, int ncols, int W, int prec)
+{
+  m.nrows = nrows; m.ncols = ncols; m.W = W; m.prec = prec;
+  m.p = dmalloc (m.bytes () ? m.bytes () : 1);
+  if (!m.p) return 5;
+  return dzero (m.p, m.bytes ());
+}
+
+void
+pmat_free (PMat &m)
+{
+  dfree (m.p);
+  m.p = nullptr;
+}
+
+// ---- rocSOLVER ------------------------------------------------------------
+int
+sol_potrf (int upper, int n, double *A, int lda, int *info_host)
+{
+  rocblas_status s = rocsolver_dpotrf (g_rb, upper ? rocblas_fill_upper
+                                                   : rocblas_fill_lower,
+                                       n, A, lda, g_info_dev);
+  LGH_CHECK (s == rocblas_status_success, 6, "rocsolver_dpotrf failed");
+  return d2h (info_host, g_info_dev, sizeof (int));
+}
+
+int
+sol_trtri (int upper, int n, double *A, int lda, int *info_host)
+{
+  rocblas_status s = rocsolver_dtrtri (g_rb, upper ? rocblas_fill_upper
+                                                   : rocblas_fill_lower,
+                                       rocblas_diagonal_non_unit, n, A, lda,
+                                       g_info_dev);
+  LGH_CHECK (s == rocblas_status_success, 6, "rocsolver_dtrtri failed");
+  return d2h (info_host, g_info_dev, sizeof (int));
+}
+
+int
+sol_syevd (int n, double *A, int lda, double *w_dev, int *info_host)
+{
+  double *E = (double *) dmalloc (sizeof (double) * (size_t) (n > 0 ? n : 1));
+  if (!E) return 5;
+  rocblas_status s = rocsolver_dsyevd (g_rb, rocblas_evect_original,
+                                       rocblas_fill_upper, n, A, lda, w_dev,
+                                       E, g_info_dev);
+  int rc = 0;
+  if (s != rocblas_status_success) { set_error ("rocsolver_dsyevd failed"); rc = 6; }
+  if (!rc) rc = d2h (info_host, g_info_dev, sizeof (int));
+  dfree (E);
+  return rc;
+}
+
+}  // namespace lgh

@@ -1,0 +1,56 @@
+// engine.hpp — object layouts of lgh_prior / lgh_glr and the internal
+// device-side operations the API functions compose.
+#pragma once
+#include <lgpsf_hessian/lgpsf_hessian.h>
+
+#include "lgh_dev.hpp"
+#include "zsolve.hpp"
+#include "petsc_impl.hpp"
+
+#include <vector>
+
+struct lgh_prior {
+  int refs = 1;
+  Vec mass = nullptr, msqrt = nullptr, minvsqrt = nullptr, work = nullptr;
+  double *d_msqrt = nullptr, *d_minvsqrt = nullptr, *d_minv = nullptr;
+  lgh_prior_callbacks_t cb {};
+  int is_mat = 0;
+  // Mat path
+  Mat Z = nullptr;
+  lgh_prior_mat_opts_t o {};
+  lgh::AmgHierarchy amg;
+  int amg_ok = 0;
+  lgh::DevMG *mg[2] = { nullptr, nullptr };
+  double *d_zdinv = nullptr;            // fp64 1/diag(Z) (Jacobi PCG)
+  int nit = 0, bound_its = 0;
+  double lmin = 0, lmax = 0;
+  lgh::ChebWork cw[2];
+  int n = 0;
+};
+
+struct lgh_glr {
+  Mat B = nullptr;
+  lgh_prior_t *prior = nullptr;
+  lgh_glr_opts_t opts {};
+  lgh_glr_xopts_t x {};
+  int n = 0;
+  lgh::PMat U;                          // n x kept, fp64, W = 64
+  std::vector<double> lam;
+  int kept = 0;
+  // incremental-extension state: orthonormal Q (n x ell) and raw Q^T F Q
+  lgh::PMat Q;
+  std::vector<double> Araw;
+  int ell = 0;
+  int corrected = 0;
+  double floor = 0.0;
+};
+
+namespace lgh {
+// device-side prior operations on panel matrices (precision = X.prec)
+int prior_setup_device (lgh_prior_t *p, int prec);
+// trans: 1 selects the transposed callbacks (Z symmetric on the Mat path)
+int prior_solve_block (lgh_prior_t *p, int trans, const PMat &X, PMat &Y);
+int prior_solve_vec (lgh_prior_t *p, int trans, const PMat &X, PMat &Y);
+int prior_apply_Z (lgh_prior_t *p, int trans, const PMat &X, PMat &Y);
+void prior_ref (lgh_prior_t *p);
+}

@@ -1,0 +1,889 @@
+// glr.cpp — lgh_glr_compute / _extend / downstream operations on the GPU.
+//
+// Host code keeps the reference control flow (glr_common_impl.h:409-1104;
+// incremental extension of glr_scalapack_impl.h:689-915); every O(n)
+// array lives in HBM as a W = 64 panel matrix and only ell x ell blocks,
+// kept-sized coefficient vectors and scalars cross back to the host.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+using namespace lgh;
+
+#define API_FAIL(what)                                                        \
+  lgh_petsc_error (PETSC_ERR_LIB, "%s: %s", what, lgh::last_error ())
+
+extern "C" lgh_glr_opts_t
+lgh_glr_opts_default (void)
+{
+  // glr_common_impl.h:295-312
+  lgh_glr_opts_t o;
+  o.ell = 200;
+  o.q_power = 1;
+  o.trunc_abs = 0.;
+  o.trunc_rel = 1e-3;
+  o.treat = LGH_TREAT_FLIP;
+  o.seed = 271828UL;
+  o.backend = LGH_GLR_DEFAULT;
+  o.nb = 64;
+  o.panel = 64;
+  o.grid2d = 0;
+  o.check = 1;
+  return o;
+}
+
+extern "C" lgh_glr_xopts_t
+lgh_glr_xopts_default (void)
+{
+  lgh_glr_xopts_t x;
+  x.prec_sweep = LGH_PREC_F64;
+  x.prec_final = LGH_PREC_F64;
+  x.prec_check = LGH_PREC_F64;
+  return x;
+}
+
+extern "C" lgh_glr_xopts_t
+lgh_glr_xopts_mixed (void)
+{
+  lgh_glr_xopts_t x;
+  x.prec_sweep = LGH_PREC_F32;
+  x.prec_final = LGH_PREC_F64;
+  x.prec_check = LGH_PREC_F32;
+  return x;
+}
+
+static inline int npan (int c) { return (c + 63) / 64; }
+
+// ---- F sweep (glr_common_impl.h:316-336) -----------------------------------
+// Y = Msqrt Z^{-1} B Z^{-T} Msqrt X ; Wk is scratch of X's shape
+static int
+apply_F (lgh_glr_t *g, const PMat &X, PMat &Y, PMat &Wk)
+{
+  lgh_prior_t *p = g->prior;
+  DevCSR *dB = nullptr;
+  LGH_TRY (prior_setup_device (p, X.prec));
+  LGH_TRY (mat_device_csr (g->B, X.prec, 0, &dB));
+  LGH_TRY (k_diag_scale (X, p->d_msqrt, Wk));
+  LGH_TRY (prior_solve_block (p, 1, Wk, Y));
+  prof_set_b_tag (1);
+  int rc = k_spmm (*dB, 1.0, Y, 0.0, Wk);
+  prof_set_b_tag (0);
+  LGH_TRY (rc);
+  LGH_TRY (prior_solve_block (p, 0, Wk, Y));
+  LGH_TRY (k_diag_scale (Y, p->d_msqrt, Y));
+  return 0;
+}
+
+// ---- CholQR2 (glr_common_impl.h:340-407) -----------------------------------
+// X <- X L^{-T}, twice; Gram on the device (MFMA, fixed-order reduction),
+// Cholesky and the triangular inverse in FP64 (rocSOLVER), the triangular
+// update as an MFMA product with the explicit L^{-T}.  `passes` = 2 is the
+// reference's CholQR2.
+static int
+cholqr (PMat &X, int ncols, int passes, int *jittered)
+{
+  const int P = X.npanels (), ld = P * 64;
+  double *G = (double *) dmalloc (sizeof (double) * (size_t) ld * ld);
+  double *C = (double *) dmalloc (sizeof (double) * (size_t) ld * ld);
+  double *sh = (double *) dmalloc (sizeof (double) * 2);
+  PMat T;
+  int rc = (G && C && sh) ? 0 : 5;
+  if (!rc) rc = pmat_alloc (T, X.nrows, X.ncols, 64, X.prec);
+  for (int pass = 0; pass < passes && !rc; pass++) {
+    int info = 0;
+    for (int attempt = 0; attempt < 2 && !rc; attempt++) {
+      rc = k_gram (X, 0, P, X, 0, P, G, ld, 1);
+      if (!rc && attempt == 1) {
+        // re-formed Gram + 1e-12 trace/ell on the diagonal (:389-395)
+        double tr = 0.0;
+        rc = k_trace (G, ncols, ld, sh);
+        if (!rc) rc = d2h (&tr, sh, sizeof (double));
+        double s = 1e-12 * tr / ncols;
+        if (!rc) rc = h2d (sh + 1, &s, sizeof (double));
+        if (!rc) rc = k_add_diag (G, ncols, ld, sh + 1);
+        if (jittered) *jittered = 1;
+      }
+      if (!rc) rc = sol_potrf (0, ncols, G, ld, &info);
+      if (!rc && info == 0) break;
+    }
+    if (!rc && info != 0) {
+      set_error ("CholQR2: potrf failed twice");
+      rc = 11;
+    }
+    if (!rc) rc = sol_trtri (0, ncols, G, ld, &info);
+    if (!rc && info != 0) { set_error ("CholQR2: trtri failed"); rc = 11; }
+    if (!rc) rc = k_lower_t_pad (G, ncols, ld, C, ld, ld);
+    if (!rc) rc = k_gemm_nn (X, 0, P, C, ld, 1.0, 0.0, T, 0, P);
+    if (!rc) std::swap (X.p, T.p);
+  }
+  pmat_free (T);
+  dfree (G); dfree (C); dfree (sh);
+  return rc;
+}
+
+// panel CholQR2 of glr_scalapack_impl.h:110-148 on one 64-column panel q of
+// X holding bw valid columns (upper factor, jitter 1e-12 max(diag, 1))
+static int
+panel_cholqr (PMat &X, int q, int bw)
+{
+  double *G = (double *) dmalloc (sizeof (double) * 64 * 64);
+  double *C = (double *) dmalloc (sizeof (double) * 64 * 64);
+  int rc = (G && C) ? 0 : 5;
+  for (int pass = 0; pass < 2 && !rc; pass++) {
+    int info = 0;
+    rc = k_gram (X, q, 1, X, q, 1, G, 64, 1);
+    if (!rc) rc = sol_potrf (1, bw, G, 64, &info);
+    if (!rc && info != 0) {
+      std::vector<double> hg (64 * 64);
+      rc = k_gram (X, q, 1, X, q, 1, G, 64, 1);
+      if (!rc) rc = d2h (hg.data (), G, sizeof (double) * 64 * 64);
+      double dmax = 0.0;
+      for (int i = 0; i < bw; i++) dmax = std::max (dmax, hg[i + i * 64]);
+      for (int i = 0; i < bw; i++) hg[i + i * 64] += 1.e-12 * std::max (dmax, 1.);
+      if (!rc) rc = h2d (G, hg.data (), sizeof (double) * 64 * 64);
+      if (!rc) rc = sol_potrf (1, bw, G, 64, &info);
+      if (!rc && info != 0) { set_error ("panel CholQR: potrf failed twice"); rc = 11; }
+    }
+    if (!rc) rc = sol_trtri (1, bw, G, 64, &info);
+    if (!rc) rc = k_upper_pad (G, bw, 64, C, 64, 64);
+    if (!rc) rc = k_gemm_nn (X, q, 1, C, 64, 1.0, 0.0, X, q, 1);   // in place
+  }
+  dfree (G); dfree (C);
+  return rc;
+}
+
+// BCGS2 (glr_scalapack_impl.h:153-195) of the k_new columns of Yn against
+// the first `ncols_old` columns of Qold (may be null) and each other, in
+// 64-column panels
+static int
+bcgs2 (const PMat *Qold, int ncols_old, PMat &Yn, int k_new)
+{
+  const int Po = Qold ? npan (ncols_old) : 0, Pn = npan (k_new);
+  const int ldo = std::max (Po, 1) * 64, ldn = Pn * 64;
+  double *C1 = (double *) dmalloc (sizeof (double) * (size_t) ldo * 64);
+  double *C2 = (double *) dmalloc (sizeof (double) * (size_t) ldn * 64);
+  int rc = (C1 && C2) ? 0 : 5;
+  for (int t = 0; t < Pn && !rc; t++) {
+    const int bw = std::min (64, k_new - 64 * t);
+    for (int sweep = 0; sweep < 2 && !rc; sweep++) {
+      if (Po > 0) {
+        rc = k_gram (*Qold, 0, Po, Yn, t, 1, C1, ldo, 0);
+        if (!rc) rc = k_gemm_nn (*Qold, 0, Po, C1, ldo, -1.0, 1.0, Yn, t, 1);
+      }
+      if (!rc && t > 0) {
+        rc = k_gram (Yn, 0, t, Yn, t, 1, C2, ldn, 0);
+        if (!rc) rc = k_gemm_nn (Yn, 0, t, C2, ldn, -1.0, 1.0, Yn, t, 1);
+      }
+      if (!rc) rc = panel_cholqr (Yn, t, bw);
+    }
+  }
+  dfree (C1); dfree (C2);
+  return rc;
+}
+
+// selection + treatment (glr_common_impl.h:517-559) from ascending evals
+static void
+select_modes (lgh_glr_t *g, int ell, const std::vector<double> &ev,
+              lgh_glr_report_t *rep, std::vector<int> &order)
+{
+  order.resize (ell);
+  for (int i = 0; i < ell; i++) order[i] = i;
+  for (int i = 0; i < ell; i++)
+    for (int j = i + 1; j < ell; j++)
+      if (std::fabs (ev[order[j]]) > std::fabs (ev[order[i]])) std::swap (order[i], order[j]);
+  const double absmax = std::fabs (ev[order[0]]);
+  double cut = 0.;
+  if (g->opts.trunc_abs > 0.) cut = std::max (cut, g->opts.trunc_abs);
+  if (g->opts.trunc_rel > 0.) cut = std::max (cut, g->opts.trunc_rel * absmax);
+  int kept = 0;
+  while (kept < ell && std::fabs (ev[order[kept]]) > cut) kept++;
+  rep->kept = kept;
+  rep->lam_abs_max = absmax;
+  rep->next_abs = (kept < ell) ? std::fabs (ev[order[kept]]) : 0.;
+  rep->lam_raw_max = ev[ell - 1];
+  rep->lam_raw_min = ev[0];
+  rep->n_negative_raw = 0;
+  for (int i = 0; i < ell; i++) if (ev[i] < 0.) rep->n_negative_raw++;
+  g->lam.assign ((size_t) std::max (kept, 1), 0.0);
+  for (int j = 0; j < kept; j++) {
+    const double l = ev[order[j]];
+    switch (g->opts.treat) {
+    case LGH_TREAT_FLIP: g->lam[j] = std::fabs (l); break;
+    case LGH_TREAT_RELU: g->lam[j] = std::max (l, 0.); break;
+    default: g->lam[j] = l; break;
+    }
+  }
+  g->kept = kept;
+}
+
+// symmetrized eig of the raw projection A (ell x ell, host), selection,
+// U = [Qa | Qb] C.  Qb (may be null) holds columns ell_a..ell-1.
+static int
+rayleigh_ritz (lgh_glr_t *g, int ell, const std::vector<double> &Araw,
+               const PMat &Qa, int ell_a, const PMat *Qb,
+               lgh_glr_report_t *rep, std::vector<double> &evals,
+               std::vector<int> &order, double *t_dense)
+{
+  std::vector<double> T ((size_t) ell * ell);
+  for (int j = 0; j < ell; j++)
+    for (int i = 0; i < ell; i++) T[i + (size_t) j * ell] = Araw[i + (size_t) j * ell];
+  for (int j = 0; j < ell; j++)          // glr_common_impl.h:490-494
+    for (int i = 0; i < j; i++) {
+      const double avg = 0.5 * (T[i + (size_t) j * ell] + T[j + (size_t) i * ell]);
+      T[i + (size_t) j * ell] = avg;
+      T[j + (size_t) i * ell] = avg;
+    }
+  const double t0 = MPI_Wtime ();
+  double *dT = (double *) dmalloc (sizeof (double) * (size_t) ell * ell);
+  double *dw = (double *) dmalloc (sizeof (double) * (size_t) ell);
+  if (!dT || !dw) { dfree (dT); dfree (dw); return 5; }
+  int info = 0;
+  int rc = h2d (dT, T.data (), sizeof (double) * (size_t) ell * ell);
+  if (!rc) rc = sol_syevd (ell, dT, ell, dw, &info);
+  if (!rc && info != 0) { set_error ("syevd failed"); rc = 12; }
+  evals.resize (ell);
+  if (!rc) rc = d2h (evals.data (), dw, sizeof (double) * ell);
+  if (!rc) rc = d2h (T.data (), dT, sizeof (double) * (size_t) ell * ell);
+  dfree (dT); dfree (dw);
+  *t_dense += MPI_Wtime () - t0;
+  LGH_TRY (rc);
+  select_modes (g, ell, evals, rep, order);
+  const int k = g->kept;
+  pmat_free (g->U);
+  LGH_TRY (pmat_alloc (g->U, g->n, std::max (k, 1), 64, F64));
+  if (k == 0) return 0;
+  const int Pk = npan (k), Pa = npan (ell_a);
+  const int lda = Pa * 64, ldk = Pk * 64;
+  std::vector<double> Ca ((size_t) lda * ldk, 0.0), Cb;
+  for (int j = 0; j < k; j++)
+    for (int i = 0; i < ell_a; i++)
+      Ca[i + (size_t) j * lda] = T[i + (size_t) order[j] * ell];
+  double *dC = (double *) dmalloc (sizeof (double) * (size_t) lda * ldk);
+  if (!dC) return 5;
+  rc = h2d (dC, Ca.data (), sizeof (double) * Ca.size ());
+  if (!rc) rc = k_gemm_nn (Qa, 0, Pa, dC, lda, 1.0, 0.0, g->U, 0, Pk);
+  if (!rc) rc = dev_sync ();
+  dfree (dC);
+  LGH_TRY (rc);
+  if (Qb && ell > ell_a) {
+    const int nb = ell - ell_a, Pb = npan (nb), ldb = Pb * 64;
+    Cb.assign ((size_t) ldb * ldk, 0.0);
+    for (int j = 0; j < k; j++)
+      for (int i = 0; i < nb; i++)
+        Cb[i + (size_t) j * ldb] = T[ell_a + i + (size_t) order[j] * ell];
+    dC = (double *) dmalloc (sizeof (double) * (size_t) ldb * ldk);
+    if (!dC) return 5;
+    rc = h2d (dC, Cb.data (), sizeof (double) * Cb.size ());
+    if (!rc) rc = k_gemm_nn (*Qb, 0, Pb, dC, ldb, 1.0, 1.0, g->U, 0, Pk);
+    if (!rc) rc = dev_sync ();
+    dfree (dC);
+    LGH_TRY (rc);
+  }
+  return 0;
+}
+
+// residual gate (glr_common_impl.h:576-600): max_j ||F u_j - lam_raw_j u_j||
+static int
+residual_gate (lgh_glr_t *g, const std::vector<double> &evals,
+               const std::vector<int> &order, int prec,
+               lgh_glr_report_t *rep)
+{
+  const int k = g->kept;
+  if (!g->opts.check || k == 0) return 0;
+  PMat U, FU, Wk;
+  const int Pk = npan (k);
+  double *s = (double *) dmalloc (sizeof (double) * Pk * 64 * 2);
+  if (!s) return 5;
+  int rc = pmat_alloc (FU, g->n, k, 64, prec);
+  if (!rc) rc = pmat_alloc (Wk, g->n, k, 64, prec);
+  if (!rc && prec != F64) {
+    rc = pmat_alloc (U, g->n, k, 64, prec);
+    if (!rc) rc = k_copy_cols (g->U, 0, U, 0, k);
+  }
+  const PMat &Uin = (prec == F64) ? g->U : U;
+  const double t0 = MPI_Wtime ();
+  if (!rc) rc = apply_F (g, Uin, FU, Wk);
+  std::vector<double> lr ((size_t) Pk * 64, 0.0), r2 ((size_t) Pk * 64, 0.0);
+  for (int j = 0; j < k; j++) lr[j] = evals[order[j]];
+  if (!rc) rc = h2d (s, lr.data (), sizeof (double) * Pk * 64);
+  if (!rc) rc = k_colresid2 (FU, Uin, s, s + Pk * 64);
+  if (!rc) rc = d2h (r2.data (), s + Pk * 64, sizeof (double) * Pk * 64);
+  rep->t_operator += MPI_Wtime () - t0;
+  for (int j = 0; j < k && !rc; j++)
+    rep->resid_max = std::max (rep->resid_max, std::sqrt (r2[j]));
+  pmat_free (U); pmat_free (FU); pmat_free (Wk);
+  dfree (s);
+  return rc;
+}
+
+static int
+timed_F (lgh_glr_t *g, const PMat &X, PMat &Y, PMat &Wk, double *t_op)
+{
+  const double t0 = MPI_Wtime ();
+  LGH_TRY (apply_F (g, X, Y, Wk));
+  LGH_TRY (dev_sync ());
+  *t_op += MPI_Wtime () - t0;
+  return 0;
+}
+
+// the build (glr_common_impl.h:415-608) with the precision policy of
+// lgh_glr_xopts_t
+static int
+build (lgh_glr_t *g, lgh_glr_report_t *rep)
+{
+  const int n = g->n, ell = g->opts.ell, P = npan (ell);
+  const int ps = g->x.prec_sweep == LGH_PREC_F32 ? F32 : F64;
+  const int pf = g->x.prec_final == LGH_PREC_F32 ? F32 : F64;
+  const int pc = g->x.prec_check == LGH_PREC_F32 ? F32 : F64;
+  memset (rep, 0, sizeof (*rep));
+  PMat Q, Y, Wk;
+  LGH_TRY (pmat_alloc (Q, n, ell, 64, ps));
+  LGH_TRY (pmat_alloc (Y, n, ell, 64, ps));
+  LGH_TRY (pmat_alloc (Wk, n, ell, 64, ps));
+  int rc = k_randn (Q, g->opts.seed, 0, 0);                    // Omega
+  if (!rc) rc = timed_F (g, Q, Y, Wk, &rep->t_operator);
+  for (int p = 0; p < g->opts.q_power && !rc; p++) {
+    double t0 = MPI_Wtime ();
+    rc = cholqr (Y, ell, 2, nullptr);
+    if (!rc) rc = dev_sync ();
+    rep->t_dense += MPI_Wtime () - t0;
+    if (!rc) std::swap (Q.p, Y.p);                              // Q <- Y
+    if (!rc) rc = timed_F (g, Q, Y, Wk, &rep->t_operator);
+  }
+  if (!rc) {
+    double t0 = MPI_Wtime ();
+    rc = cholqr (Y, ell, 2, nullptr);
+    if (!rc) std::swap (Q.p, Y.p);
+    if (!rc && pf != ps) {
+      // precision change: carry Q to the final precision and restore its
+      // orthonormality there (one more CholQR2 in FP64)
+      PMat Qf, Yf, Wf;
+      rc = pmat_alloc (Qf, n, ell, 64, pf);
+      if (!rc) rc = pmat_alloc (Yf, n, ell, 64, pf);
+      if (!rc) rc = pmat_alloc (Wf, n, ell, 64, pf);
+      if (!rc) rc = k_copy_cols (Q, 0, Qf, 0, ell);
+      if (!rc) rc = cholqr (Qf, ell, 2, nullptr);
+      pmat_free (Q); pmat_free (Y); pmat_free (Wk);
+      Q = Qf; Y = Yf; Wk = Wf;
+    }
+    if (!rc) rc = dev_sync ();
+    rep->t_dense += MPI_Wtime () - t0;
+  }
+  if (!rc) rc = timed_F (g, Q, Y, Wk, &rep->t_operator);     // Y = F Q
+  pmat_free (Wk);
+  std::vector<double> evals, Araw ((size_t) ell * ell);
+  std::vector<int> order;
+  if (!rc) {
+    // T = Q^T Y (raw, kept for extension) — glr_common_impl.h:469-489
+    const int ld = P * 64;
+    double *dT = (double *) dmalloc (sizeof (double) * (size_t) ld * ld);
+    std::vector<double> hT ((size_t) ld * ld);
+    rc = dT ? 0 : 5;
+    if (!rc) rc = k_gram (Q, 0, P, Y, 0, P, dT, ld, 0);
+    if (!rc) rc = d2h (hT.data (), dT, sizeof (double) * hT.size ());
+    dfree (dT);
+    for (int j = 0; j < ell; j++)
+      for (int i = 0; i < ell; i++) Araw[i + (size_t) j * ell] = hT[i + (size_t) j * ld];
+  }
+  pmat_free (Y);
+  if (!rc) rc = rayleigh_ritz (g, ell, Araw, Q, ell, nullptr, rep, evals, order,
+                               &rep->t_dense);
+  if (!rc) rc = residual_gate (g, evals, order, pc, rep);
+  if (!rc) {
+    pmat_free (g->Q);
+    if (Q.prec != F64) {
+      PMat Q64;
+      rc = pmat_alloc (Q64, n, ell, 64, F64);
+      if (!rc) rc = k_copy_cols (Q, 0, Q64, 0, ell);
+      pmat_free (Q);
+      Q = Q64;
+    }
+    g->Q = Q;
+    Q.p = nullptr;
+    g->Araw = Araw;
+    g->ell = ell;
+  }
+  pmat_free (Q);
+  return rc;
+}
+
+extern "C" int
+lgh_glr_compute_x (Mat B, lgh_prior_t *prior, const lgh_glr_opts_t *opts,
+                   const lgh_glr_xopts_t *xopts, lgh_glr_t **glr,
+                   lgh_glr_report_t *report)
+{
+  PetscCheck (B != NULL && prior != NULL && glr != NULL, 0, PETSC_ERR_ARG_NULL,
+              "lgh_glr_compute: NULL argument");
+  lgh_glr_opts_t o = opts ? *opts : lgh_glr_opts_default ();
+  const int N = B->m;
+  // glr_common_impl.h:651-664
+  PetscCheck (o.ell > 0 && o.ell <= N, 0, PETSC_ERR_ARG_OUTOFRANGE,
+              "lgh_glr_compute: need 0 < ell (%d) <= N (%d)", o.ell, N);
+  if (o.backend == LGH_GLR_DEFAULT) o.backend = LGH_GLR_REPLICATED;
+  PetscCheck (o.backend != LGH_GLR_SCALAPACK, 0, PETSC_ERR_SUP,
+              "lgh_glr_compute: this build has no ScaLAPACK backend");
+  PetscCheck (B->m == B->n && B->m == prior->n, 0, PETSC_ERR_ARG_SIZ,
+              "lgh_glr_compute: B (%d x %d) and prior (%d) disagree", B->m,
+              B->n, prior->n);
+  PetscCheck (B->assembled, 0, PETSC_ERR_ARG_WRONGSTATE,
+              "lgh_glr_compute: B not assembled");
+  if (dev_init ()) return API_FAIL ("lgh_glr_compute");
+  lgh_glr_t *g = new lgh_glr_t ();
+  g->opts = o;
+  g->x = xopts ? *xopts : lgh_glr_xopts_default ();
+  g->n = N;
+  g->B = B;
+  PetscObjectReference ((PetscObject) B);
+  g->prior = prior;
+  prior_ref (prior);
+  lgh_glr_report_t rep;
+  if (build (g, &rep)) {
+    PetscErrorCode e = API_FAIL ("lgh_glr_compute");
+    lgh_glr_destroy (g);
+    return e;
+  }
+  if (report) *report = rep;
+  *glr = g;
+  return 0;
+}
+
+extern "C" int
+lgh_glr_compute (Mat B, lgh_prior_t *prior, const lgh_glr_opts_t *opts,
+                 lgh_glr_t **glr, lgh_glr_report_t *report)
+{
+  return lgh_glr_compute_x (B, prior, opts, nullptr, glr, report);
+}
+
+// incremental extension (glr_scalapack_impl.h:689-915), replicated T
+static int
+extend_incr (lgh_glr_t *g, int k_new, lgh_glr_report_t *rep)
+{
+  const int n = g->n, ell1 = g->ell, ell2 = ell1 + k_new;
+  const int ps = g->x.prec_sweep == LGH_PREC_F32 ? F32 : F64;
+  const int pf = g->x.prec_final == LGH_PREC_F32 ? F32 : F64;
+  const int pc = g->x.prec_check == LGH_PREC_F32 ? F32 : F64;
+  memset (rep, 0, sizeof (*rep));
+  PMat Qn, Yn, Wn;
+  LGH_TRY (pmat_alloc (Qn, n, k_new, 64, ps));
+  LGH_TRY (pmat_alloc (Yn, n, k_new, 64, ps));
+  LGH_TRY (pmat_alloc (Wn, n, k_new, 64, ps));
+  int rc = k_randn (Qn, g->opts.seed, 0, ell1);              // :726-729
+  if (!rc) rc = timed_F (g, Qn, Yn, Wn, &rep->t_operator);
+  for (int p = 0; p < g->opts.q_power && !rc; p++) {          // :739-747
+    double t0 = MPI_Wtime ();
+    rc = bcgs2 (nullptr, 0, Yn, k_new);
+    if (!rc) rc = dev_sync ();
+    rep->t_dense += MPI_Wtime () - t0;
+    if (!rc) std::swap (Qn.p, Yn.p);
+    if (!rc) rc = timed_F (g, Qn, Yn, Wn, &rep->t_operator);
+  }
+  if (!rc && ps != pf) {
+    PMat Yf, Qf, Wf;
+    rc = pmat_alloc (Yf, n, k_new, 64, pf);
+    if (!rc) rc = pmat_alloc (Qf, n, k_new, 64, pf);
+    if (!rc) rc = pmat_alloc (Wf, n, k_new, 64, pf);
+    if (!rc) rc = k_copy_cols (Yn, 0, Yf, 0, k_new);
+    pmat_free (Qn); pmat_free (Yn); pmat_free (Wn);
+    Qn = Qf; Yn = Yf; Wn = Wf;
+  }
+  if (!rc) {                                                  // :749-768
+    double t0 = MPI_Wtime ();
+    rc = bcgs2 (&g->Q, ell1, Yn, k_new);
+    if (!rc) rc = dev_sync ();
+    rep->t_dense += MPI_Wtime () - t0;
+    if (!rc) std::swap (Qn.p, Yn.p);
+  }
+  if (!rc) rc = timed_F (g, Qn, Yn, Wn, &rep->t_operator);   // :789
+  pmat_free (Wn);
+  // border Cnew = Q_all^T (F Q_new) (:792-814)
+  std::vector<double> A2 ((size_t) ell2 * ell2, 0.0);
+  if (!rc) {
+    const int P1 = npan (ell1), Pn = npan (k_new);
+    const int ld1 = P1 * 64, ldn = Pn * 64;
+    double *top = (double *) dmalloc (sizeof (double) * (size_t) ld1 * ldn);
+    double *bot = (double *) dmalloc (sizeof (double) * (size_t) ldn * ldn);
+    std::vector<double> ht ((size_t) ld1 * ldn), hb ((size_t) ldn * ldn);
+    rc = (top && bot) ? 0 : 5;
+    if (!rc) rc = k_gram (g->Q, 0, P1, Yn, 0, Pn, top, ld1, 0);
+    if (!rc) rc = k_gram (Qn, 0, Pn, Yn, 0, Pn, bot, ldn, 0);
+    if (!rc) rc = d2h (ht.data (), top, sizeof (double) * ht.size ());
+    if (!rc) rc = d2h (hb.data (), bot, sizeof (double) * hb.size ());
+    dfree (top); dfree (bot);
+    // Cn(i, jn): i < ell1 from top, i >= ell1 from bot
+    auto Cn = [&] (int i, int jn) {
+      return i < ell1 ? ht[i + (size_t) jn * ld1] : hb[(i - ell1) + (size_t) jn * ldn];
+    };
+    // bordered raw A (:819-840)
+    for (int j = 0; j < ell1; j++) {
+      for (int i = 0; i < ell1; i++)
+        A2[i + (size_t) j * ell2] = g->Araw[i + (size_t) j * ell1];
+      for (int i = ell1; i < ell2; i++) A2[i + (size_t) j * ell2] = Cn (j, i - ell1);
+    }
+    for (int j = ell1; j < ell2; j++)
+      for (int i = 0; i < ell2; i++) A2[i + (size_t) j * ell2] = Cn (i, j - ell1);
+  }
+  pmat_free (Yn);
+  PMat Qn64;
+  if (!rc && Qn.prec != F64) {
+    rc = pmat_alloc (Qn64, n, k_new, 64, F64);
+    if (!rc) rc = k_copy_cols (Qn, 0, Qn64, 0, k_new);
+    pmat_free (Qn);
+    Qn = Qn64;
+    Qn64.p = nullptr;
+  }
+  std::vector<double> evals;
+  std::vector<int> order;
+  const int old_ell = g->opts.ell;
+  if (!rc) {
+    g->opts.ell = ell2;
+    rc = rayleigh_ritz (g, ell2, A2, g->Q, ell1, &Qn, rep, evals, order,
+                        &rep->t_dense);
+  }
+  if (!rc) rc = residual_gate (g, evals, order, pc, rep);
+  if (!rc) {
+    PMat Q2;
+    rc = pmat_alloc (Q2, n, ell2, 64, F64);
+    if (!rc) rc = k_copy_cols (g->Q, 0, Q2, 0, ell1);
+    if (!rc) rc = k_copy_cols (Qn, 0, Q2, ell1, k_new);
+    if (!rc) rc = dev_sync ();
+    if (!rc) {
+      pmat_free (g->Q);
+      g->Q = Q2;
+      g->Araw = A2;
+      g->ell = ell2;
+    }
+    else pmat_free (Q2);
+  }
+  if (rc) g->opts.ell = old_ell;
+  pmat_free (Qn);
+  return rc;
+}
+
+extern "C" int
+lgh_glr_extend (lgh_glr_t *glr, int k_new, lgh_glr_report_t *report)
+{
+  // glr_common_impl.h:695-704
+  PetscCheck (!glr->corrected, 0, PETSC_ERR_ORDER,
+              "lgh_glr_extend: not available after lgh_glr_correct");
+  PetscCheck (k_new > 0, 0, PETSC_ERR_ARG_OUTOFRANGE,
+              "lgh_glr_extend: k_new must be positive");
+  PetscCheck (glr->opts.ell + k_new <= glr->n, 0, PETSC_ERR_ARG_OUTOFRANGE,
+              "lgh_glr_extend: extended ell (%d) exceeds N (%d)",
+              glr->opts.ell + k_new, glr->n);
+  lgh_glr_report_t rep;
+  if (extend_incr (glr, k_new, &rep)) return API_FAIL ("lgh_glr_extend");
+  if (report) *report = rep;
+  return 0;
+}
+
+extern "C" void
+lgh_glr_destroy (lgh_glr_t *g)
+{
+  if (g == NULL) return;
+  pmat_free (g->U);
+  pmat_free (g->Q);
+  MatDestroy (&g->B);
+  lgh_prior_destroy (g->prior);
+  delete g;
+}
+
+// ---- downstream -------------------------------------------------------------
+
+static int
+width_for (int m)
+{
+  int W = 1;
+  while (W < m && W < 64) W *= 2;
+  return W;
+}
+
+// Y = f_c X + U diag(f_lam - f_c) U^T X per column j (block master formula,
+// glr_common_impl.h:909-966).  X, Y: W = 64 fp64 panels of m columns;
+// fk[k + j*kept] = f_lam_k(j) - f_c(j), fc[j] = f_c(j).
+static int
+woodbury (lgh_glr_t *g, const PMat &X, int m, const std::vector<double> &fk,
+          const std::vector<double> &fc, PMat &Y)
+{
+  const int Pm = npan (m), ldm = Pm * 64;
+  double *dfc = (double *) dmalloc (sizeof (double) * ldm);
+  if (!dfc) return 5;
+  std::vector<double> hfc ((size_t) ldm, 0.0);
+  for (int j = 0; j < m; j++) hfc[j] = fc[j];
+  int rc = h2d (dfc, hfc.data (), sizeof (double) * ldm);
+  if (!rc) rc = k_colscale (dfc, X, Y);
+  dfree (dfc);
+  LGH_TRY (rc);
+  const int k = g->kept;
+  if (k == 0) return 0;
+  const int Pk = npan (k), ldk = Pk * 64;
+  double *W = (double *) dmalloc (sizeof (double) * (size_t) ldk * ldm);
+  if (!W) return 5;
+  std::vector<double> hw ((size_t) ldk * ldm);
+  rc = k_gram (g->U, 0, Pk, X, 0, Pm, W, ldk, 0);
+  if (!rc) rc = d2h (hw.data (), W, sizeof (double) * hw.size ());
+  if (!rc) {
+    for (int j = 0; j < ldm; j++)
+      for (int i = 0; i < ldk; i++) {
+        double &w = hw[i + (size_t) j * ldk];
+        w = (i < k && j < m) ? w * fk[i + (size_t) j * k] : 0.0;
+      }
+    rc = h2d (W, hw.data (), sizeof (double) * hw.size ());
+  }
+  if (!rc) rc = k_gemm_nn (g->U, 0, Pk, W, ldk, 1.0, 1.0, Y, 0, Pm);
+  if (!rc) rc = dev_sync ();
+  dfree (W);
+  return rc;
+}
+
+// block driver for the full-space operations: op selects
+//   0 solve   : x = Z^{-T} M^{1/2} (F+cI)^{-1} M^{1/2} Z^{-1} rhs
+//   3 GINVT   : x = Z^{-T} M^{1/2} (F+cI)^{-1/2} in   (== sample)
+//   2 GINV    : x = (F+cI)^{-1/2} M^{1/2} Z^{-1} in
+//   0x10 G    : x = Z M^{-1/2} (F+cI)^{1/2} in
+//   0x11 GT   : x = (F+cI)^{1/2} M^{-1/2} Z^T in
+//   0x20 pow  : x = (F+cI)^p in (prior-weighted frame)
+//   0x30 filter with caller values
+struct OpSpec {
+  int kind;
+  double p;
+  const double *f_lam;
+  double f_c;
+};
+
+static void
+coeffs (lgh_glr_t *g, const OpSpec &op, int m, const double *c,
+        std::vector<double> &fk, std::vector<double> &fc)
+{
+  const int k = g->kept;
+  fk.assign ((size_t) std::max (k, 1) * m, 0.0);
+  fc.assign (m, 0.0);
+  const double half = 0.5;
+  for (int j = 0; j < m; j++) {
+    const double cj = c ? c[j] : 0.0;
+    double f_c;
+    for (int i = 0; i < k; i++) {
+      const double l = g->lam[i];
+      double f;
+      switch (op.kind) {
+      case 0: f = 1. / (l + cj); break;                         // :1018-1019
+      case 2: case 3: f = 1. / std::pow (l + cj, half); break;  // :1079-1086
+      case 0x10: case 0x11: f = std::pow (l + cj, half); break; // :1063-1073
+      case 0x20: f = std::pow (l + cj, op.p); break;            // :983-984
+      default: f = op.f_lam[i]; break;
+      }
+      fk[i + (size_t) j * k] = f;
+    }
+    switch (op.kind) {
+    case 0: f_c = 1. / cj; break;
+    case 2: case 3: f_c = 1. / std::sqrt (cj); break;
+    case 0x10: case 0x11: f_c = std::sqrt (cj); break;
+    case 0x20: f_c = std::pow (cj, op.p); break;
+    default: f_c = op.f_c; break;
+    }
+    fc[j] = f_c;
+    for (int i = 0; i < k; i++) fk[i + (size_t) j * k] -= f_c;
+  }
+}
+
+static int
+run_op (lgh_glr_t *g, const OpSpec &op, int m, const double *c,
+        const double *in, int ldi, double *out, int ldo)
+{
+  lgh_prior_t *p = g->prior;
+  const int n = g->n, Wz = width_for (m);
+  LGH_TRY (prior_setup_device (p, F64));
+  std::vector<double> fk, fc;
+  coeffs (g, op, m, c, fk, fc);
+  PMat H, A, Bm, X64, Y64;
+  std::vector<double> hb ((size_t) n * m);
+  for (int j = 0; j < m; j++) memcpy (&hb[(size_t) j * n], in + (size_t) j * ldi, sizeof (double) * n);
+  int rc = pmat_alloc (H, n, m, 1, F64);
+  if (!rc) rc = pmat_alloc (A, n, m, Wz, F64);
+  if (!rc) rc = pmat_alloc (Bm, n, m, Wz, F64);
+  if (!rc) rc = pmat_alloc (X64, n, m, 64, F64);
+  if (!rc) rc = pmat_alloc (Y64, n, m, 64, F64);
+  if (!rc) rc = h2d (H.p, hb.data (), sizeof (double) * hb.size ());
+  if (!rc) rc = k_copy_cols (H, 0, A, 0, m);
+  // left part (before the filter): into X64
+  if (!rc) {
+    switch (op.kind) {
+    case 0: case 2:                    // M^{1/2} Z^{-1} in
+      rc = prior_solve_vec (p, 0, A, Bm);
+      if (!rc) rc = k_diag_scale (Bm, p->d_msqrt, Bm);
+      if (!rc) rc = k_copy_cols (Bm, 0, X64, 0, m);
+      break;
+    case 0x11:                         // M^{-1/2} Z^T in
+      rc = prior_apply_Z (p, 1, A, Bm);
+      if (!rc) rc = k_diag_scale (Bm, p->d_minvsqrt, Bm);
+      if (!rc) rc = k_copy_cols (Bm, 0, X64, 0, m);
+      break;
+    default:
+      rc = k_copy_cols (A, 0, X64, 0, m);
+      break;
+    }
+  }
+  if (!rc) rc = woodbury (g, X64, m, fk, fc, Y64);
+  if (!rc) rc = k_copy_cols (Y64, 0, A, 0, m);
+  if (!rc) {
+    switch (op.kind) {
+    case 0: case 3:                    // Z^{-T} M^{1/2} (.)
+      rc = k_diag_scale (A, p->d_msqrt, A);
+      if (!rc) rc = prior_solve_vec (p, 1, A, Bm);
+      if (!rc) rc = k_copy_cols (Bm, 0, H, 0, m);
+      break;
+    case 0x10:                         // Z M^{-1/2} (.)
+      rc = k_diag_scale (A, p->d_minvsqrt, A);
+      if (!rc) rc = prior_apply_Z (p, 0, A, Bm);
+      if (!rc) rc = k_copy_cols (Bm, 0, H, 0, m);
+      break;
+    default:
+      rc = k_copy_cols (A, 0, H, 0, m);
+      break;
+    }
+  }
+  if (!rc) rc = d2h (hb.data (), H.p, sizeof (double) * hb.size ());
+  pmat_free (H); pmat_free (A); pmat_free (Bm); pmat_free (X64); pmat_free (Y64);
+  LGH_TRY (rc);
+  for (int j = 0; j < m; j++) memcpy (out + (size_t) j * ldo, &hb[(size_t) j * n], sizeof (double) * n);
+  return 0;
+}
+
+static PetscErrorCode
+vec_op (lgh_glr_t *g, const OpSpec &op, double c, Vec in, Vec out,
+        const char *name)
+{
+  PetscCheck (in->n == g->n && out->n == g->n, 0, PETSC_ERR_ARG_SIZ,
+              "%s: vector size mismatch", name);
+  if (run_op (g, op, 1, &c, in->a, g->n, out->a, g->n)) return API_FAIL (name);
+  return 0;
+}
+
+extern "C" int
+lgh_glr_solve (lgh_glr_t *g, double c, Vec rhs, Vec x)
+{
+  PetscCheck (c > g->floor, 0, PETSC_ERR_ARG_OUTOFRANGE,
+              "lgh_glr_solve: shift c must exceed the spectrum floor %g", g->floor);
+  return vec_op (g, OpSpec { 0, 0, nullptr, 0 }, c, rhs, x, "lgh_glr_solve");
+}
+
+extern "C" int
+lgh_glr_factor (lgh_glr_t *g, double c, lgh_factor_t which, Vec in, Vec out)
+{
+  PetscCheck (c > g->floor, 0, PETSC_ERR_ARG_OUTOFRANGE,
+              "lgh_glr_factor: shift c must exceed the spectrum floor %g", g->floor);
+  int kind;
+  switch (which) {
+  case LGH_FACTOR_G: kind = 0x10; break;
+  case LGH_FACTOR_GT: kind = 0x11; break;
+  case LGH_FACTOR_GINV: kind = 2; break;
+  case LGH_FACTOR_GINVT: kind = 3; break;
+  default:
+    SETERRQ (0, PETSC_ERR_ARG_OUTOFRANGE, "lgh_glr_factor: unknown factor action");
+  }
+  return vec_op (g, OpSpec { kind, 0, nullptr, 0 }, c, in, out, "lgh_glr_factor");
+}
+
+extern "C" int
+lgh_glr_sample (lgh_glr_t *g, double c, Vec xi, Vec x)
+{
+  return lgh_glr_factor (g, c, LGH_FACTOR_GINVT, xi, x);     // :1099-1104
+}
+
+extern "C" int
+lgh_glr_pow (lgh_glr_t *g, double c, double p, Vec in, Vec out)
+{
+  PetscCheck (c > g->floor, 0, PETSC_ERR_ARG_OUTOFRANGE,
+              "lgh_glr_pow: shift c must exceed the spectrum floor %g", g->floor);
+  return vec_op (g, OpSpec { 0x20, p, nullptr, 0 }, c, in, out, "lgh_glr_pow");
+}
+
+extern "C" int
+lgh_glr_filter (lgh_glr_t *g, const double *f_lam, double f_c, Vec in, Vec out)
+{
+  return vec_op (g, OpSpec { 0x30, 0, f_lam, f_c }, 0.0, in, out,
+                 "lgh_glr_filter");
+}
+
+extern "C" double
+lgh_glr_logdet (lgh_glr_t *g, double c)
+{
+  double s = 0.;                                              // :988-997
+  for (int i = 0; i < g->kept; i++) s += std::log (g->lam[i] + c) - std::log (c);
+  return s;
+}
+
+extern "C" int
+lgh_glr_eigs (const lgh_glr_t *g, int *nkept, const double **lam)
+{
+  *nkept = g->kept;
+  *lam = g->lam.data ();
+  return 0;
+}
+
+extern "C" int
+lgh_glr_apply (lgh_glr_t *g, double c, Vec v, Vec y)
+{
+  // exact y = B v + c Z M^{-1} Z^T v (glr_common_impl.h:1043-1047)
+  PetscCheck (v->n == g->n && y->n == g->n, 0, PETSC_ERR_ARG_SIZ,
+              "lgh_glr_apply: vector size mismatch");
+  lgh_prior_t *p = g->prior;
+  PMat X, T, Yb;
+  DevCSR *dB = nullptr;
+  int rc = prior_setup_device (p, F64);
+  if (!rc) rc = mat_device_csr (g->B, F64, 0, &dB);
+  if (!rc) rc = pmat_alloc (X, g->n, 1, 1, F64);
+  if (!rc) rc = pmat_alloc (T, g->n, 1, 1, F64);
+  if (!rc) rc = pmat_alloc (Yb, g->n, 1, 1, F64);
+  if (!rc) rc = h2d (X.p, v->a, sizeof (double) * g->n);
+  if (!rc) rc = k_spmm (*dB, 1.0, X, 0.0, Yb);
+  if (!rc) rc = prior_apply_Z (p, 1, X, T);
+  if (!rc) rc = k_diag_scale (T, p->d_minv, T);
+  if (!rc) rc = prior_apply_Z (p, 0, T, X);
+  if (!rc) rc = k_axpby (c, X, 1.0, Yb);
+  if (!rc) rc = d2h (y->a, Yb.p, sizeof (double) * g->n);
+  pmat_free (X); pmat_free (T); pmat_free (Yb);
+  if (rc) return API_FAIL ("lgh_glr_apply");
+  return 0;
+}
+
+extern "C" int
+lgh_glr_solve_block (lgh_glr_t *g, int m, const double *c, const double *rhs,
+                     int ldr, double *x, int ldx)
+{
+  for (int j = 0; j < m; j++)
+    PetscCheck (c[j] > g->floor, 0, PETSC_ERR_ARG_OUTOFRANGE,
+                "lgh_glr_solve_block: shift c[%d] must exceed %g", j, g->floor);
+  if (run_op (g, OpSpec { 0, 0, nullptr, 0 }, m, c, rhs, ldr, x, ldx))
+    return API_FAIL ("lgh_glr_solve_block");
+  return 0;
+}
+
+extern "C" int
+lgh_glr_sample_block (lgh_glr_t *g, int m, const double *c, const double *xi,
+                      int ldr, double *x, int ldx)
+{
+  for (int j = 0; j < m; j++)
+    PetscCheck (c[j] > g->floor, 0, PETSC_ERR_ARG_OUTOFRANGE,
+                "lgh_glr_sample_block: shift c[%d] must exceed %g", j, g->floor);
+  if (run_op (g, OpSpec { 3, 0, nullptr, 0 }, m, c, xi, ldr, x, ldx))
+    return API_FAIL ("lgh_glr_sample_block");
+  return 0;
+}
+
+extern "C" int
+lgh_glr_get_U (const lgh_glr_t *g, double *U, int ldu)
+{
+  const int k = g->kept;
+  if (k == 0) return 0;
+  PMat H;
+  std::vector<double> h ((size_t) g->n * k);
+  int rc = pmat_alloc (H, g->n, k, 1, F64);
+  if (!rc) rc = k_copy_cols (g->U, 0, H, 0, k);
+  if (!rc) rc = d2h (h.data (), H.p, sizeof (double) * h.size ());
+  pmat_free (H);
+  if (rc) return API_FAIL ("lgh_glr_get_U");
+  for (int j = 0; j < k; j++) memcpy (U + (size_t) j * ldu, &h[(size_t) j * g->n], sizeof (double) * g->n);
+  return 0;
+}

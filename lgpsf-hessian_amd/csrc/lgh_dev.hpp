@@ -1,0 +1,174 @@
+// lgh_dev.hpp — internal device layer of the MI355X GLR engine.
+//
+// Data layout in HBM.  Every tall block (sketch, Q, Y, U, right-hand
+// sides) is a PANEL MATRIX: an nrows x ncols matrix split into column
+// panels of width W (W in {1,2,4,...,64}); panel p is an nrows x W
+// row-major slab, so element (i, j) lives at
+//     ((j / W) * nrows + i) * W + (j % W).
+// With W = 64 one wavefront owns one row of a panel (one lane per column):
+// every CSR neighbour gather is one coalesced 256 B (fp32) / 512 B (fp64)
+// load, and elementwise work streams contiguously.  Columns past ncols in
+// the last panel are padding and are kept at zero by every kernel.
+//
+// Sparse matrices are CSR with int32 row pointers / column indices and
+// fp64 or fp32 values.  Small dense matrices (Gram, T, eigenvectors,
+// coefficient blocks) are fp64 column-major.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+#include <vector>
+
+namespace lgh {
+
+enum Prec { F64 = 0, F32 = 1 };
+inline size_t prec_bytes (int p) { return p == F64 ? 8 : 4; }
+
+// ---- errors -------------------------------------------------------------
+void set_error (const std::string &msg);
+const char *last_error ();
+int hip_fail (hipError_t e, const char *what, const char *file, int line);
+#define LGH_HIP(expr)                                                        \
+  do {                                                                       \
+    hipError_t _e = (expr);                                                  \
+    if (_e != hipSuccess) return ::lgh::hip_fail (_e, #expr, __FILE__, __LINE__); \
+  } while (0)
+#define LGH_TRY(expr)                                                        \
+  do { int _rc = (expr); if (_rc) return _rc; } while (0)
+#define LGH_CHECK(cond, code, msg)                                           \
+  do { if (!(cond)) { ::lgh::set_error (msg); return (code); } } while (0)
+
+// ---- context ------------------------------------------------------------
+int  dev_init ();                 // idempotent; selects device 0
+hipStream_t stream ();
+int  dev_sync ();
+void *rocblas_h ();               // rocblas_handle bound to stream()
+
+// caching device allocator (no hipMalloc/hipFree churn inside builds)
+void *dmalloc (size_t bytes);     // nullptr on failure (error set)
+void  dfree (void *p);
+void  dtrim ();                   // release cached blocks
+size_t dbytes_in_use ();
+
+int h2d (void *dst, const void *src, size_t bytes);
+int d2h (void *dst, const void *src, size_t bytes);
+int dzero (void *dst, size_t bytes);   // fill kernel (graph-safe)
+int dcopy (void *dst, const void *src, size_t bytes);
+
+// ---- live kernel timing (HIP events on the engine stream) ---------------
+// Launchers bracket their kernel with events when profiling is enabled and
+// credit the launch's algorithmic HBM bytes / flops to a kernel class.
+enum ProfKind {
+  PK_SPMM_B = 0,      // B X of the F sweep
+  PK_SPMM = 1,        // other SpMMs (Z apply, P, P^T)
+  PK_CHEB_JAC = 2,    // fused Chebyshev step, Jacobi (smoothers, mode 4)
+  PK_CHEB_Z = 3,      // fused Chebyshev step, explicit preconditioned Z
+  PK_RESID = 4,
+  PK_GRAM = 5,
+  PK_GEMM = 6,
+  PK_OTHER = 7,
+  PK_N = 8
+};
+void prof_enable (int on);
+int  prof_enabled ();
+void prof_set_b_tag (int on);      // next SpMMs are the F sweep's B product
+int  prof_b_tag ();
+struct ProfScope {
+  int kind; double bytes, flops; int active; void *ev0;
+  ProfScope (int kind, double bytes, double flops);
+  ~ProfScope ();
+};
+// totals since the last collect (synchronizes the stream)
+int prof_collect (int kind, long *count, double *ms, double *bytes,
+                  double *flops);
+
+// ---- objects ------------------------------------------------------------
+struct DevCSR {
+  int n = 0, m = 0;               // rows, cols
+  int64_t nnz = 0;
+  int *rp = nullptr, *ci = nullptr;
+  void *v = nullptr;              // values, precision `prec`
+  int prec = F64;
+  void *dinv = nullptr;           // 1/diag (same precision), square only
+  ~DevCSR ();
+};
+// upload host CSR (values fp64 on the host) at precision prec
+int csr_upload (int n, int m, const int *rp, const int *ci, const double *v,
+                int prec, int want_dinv, DevCSR **out);
+
+struct PMat {
+  void *p = nullptr;
+  int nrows = 0, ncols = 0, W = 64, prec = F64;
+  int npanels () const { return (ncols + W - 1) / W; }
+  size_t elems () const { return (size_t) npanels () * nrows * W; }
+  size_t bytes () const { return elems () * prec_bytes (prec); }
+  void *panel (int q) const
+  { return (char *) p + (size_t) q * nrows * W * prec_bytes (prec); }
+};
+int  pmat_alloc (PMat &m, int nrows, int ncols, int W, int prec);  // zeroed
+void pmat_free (PMat &m);
+
+// ---- kernels (all on stream(); asynchronous) ----------------------------
+// hashed Gaussian block: element (i, j) = randn_at(seed, row0 + i, col0 + j)
+int k_randn (PMat &out, uint64_t seed, int64_t row0, int64_t col0);
+// Y = alpha A X + beta Y over all panels of X/Y (same W, prec of X/Y)
+int k_spmm (const DevCSR &A, double alpha, const PMat &X, double beta,
+            PMat &Y);
+// Y = diag(d) X  (d: n doubles, fp64 device vector); Y may alias X
+int k_diag_scale (const PMat &X, const double *d, PMat &Y);
+// Y = a X + b Y elementwise (same shape)
+int k_axpby (double a, const PMat &X, double b, PMat &Y);
+// Y = X converted to Y's precision / panel width (same nrows/ncols), with
+// column offsets: Y(:, yc0 + j) = X(:, xc0 + j), j < nc
+int k_copy_cols (const PMat &X, int xc0, PMat &Y, int yc0, int nc);
+// R = B - A X
+int k_resid (const DevCSR &A, const PMat &B, const PMat &X, PMat &R);
+// fused Chebyshev step (see kernels.hip):
+//   Zj = dinv_j R_j (jacobi) or Zsrc_j;  Dn = a D + b Z;
+//   X = X + Dn (x_mode 1) | X = Dn (x_mode 0);  Rn = R - A Dn
+// (D may be null when a == 0)
+int k_cheb_step (const DevCSR &A, int jacobi, const PMat *Zsrc, double a,
+                 double b, const PMat *D, const PMat &R, PMat &X, int x_mode,
+                 PMat &Dn, PMat &Rn);
+// X0 = Ainv B0 (dense n0 x n0 row-major inverse in X's precision)
+int k_dense_apply (const void *Ainv, int n0, const PMat &B, PMat &X);
+
+// column-wise reductions: out[j] = sum_i X(i,j) Y(i,j) (fp64, ncols)
+int k_coldot (const PMat &X, const PMat &Y, double *out_dev);
+// out[j] = sum_i (X(i,j) - s_j Y(i,j))^2
+int k_colresid2 (const PMat &X, const PMat &Y, const double *s_dev,
+                 double *out_dev);
+// per-column CG updates: X += alpha_j P, R -= alpha_j Q ; P = Z + beta_j P
+int k_cg_xr (const double *alpha, const PMat &P, const PMat &Q, PMat &X,
+             PMat &R);
+int k_cg_p (const double *beta, const PMat &Z, PMat &P);
+// Y(:, j) = s_j X(:, j)  (s: fp64 device, one per padded column)
+int k_colscale (const double *s, const PMat &X, PMat &Y);
+
+// ---- tall-skinny dense (W = 64 panel matrices) -------------------------
+// G (fp64, col-major, ldg) = X[:, xc] ^T Y[:, yc] over whole panels
+//   xp0/npx: X panels, yp0/npy: Y panels; G is (npx*64) x (npy*64)
+int k_gram (const PMat &X, int xp0, int npx, const PMat &Y, int yp0, int npy,
+            double *G, int ldg, int sym);
+// Y[:, panels yp0..yp0+npy) = alpha X[:, panels xp0..xp0+npx) C + beta Y
+//   C: fp64 col-major (npx*64) x (npy*64), ldc
+int k_gemm_nn (const PMat &X, int xp0, int npx, const double *C, int ldc,
+               double alpha, double beta, PMat &Y, int yp0, int npy);
+
+// ---- small dense helpers on device (fp64) ------------------------------
+int k_symmetrize (double *T, int n, int ld);
+int k_add_diag (double *A, int n, int ld, const double *shift_dev);
+// C = transpose(lower(Linv)) embedded in a zeroed (ldc x nc) block
+int k_lower_t_pad (const double *Linv, int n, int ldl, double *C, int ldc,
+                   int ccols);
+int k_upper_pad (const double *Rinv, int n, int ldr, double *C, int ldc,
+                 int ccols);
+int k_trace (const double *A, int n, int ld, double *out_dev);
+
+// rocSOLVER wrappers (device pointers, fp64)
+int sol_potrf (int upper, int n, double *A, int lda, int *info_host);
+int sol_trtri (int upper, int n, double *A, int lda, int *info_host);
+int sol_syevd (int n, double *A, int lda, double *w_dev, int *info_host);
+
+}  // namespace lgh

@@ -1,0 +1,150 @@
+// randn.hpp — the hashed Gaussian as a pure function of (seed, i, j), for
+// host and device: glr_common_impl.h:35-73 (lgh_randn / lgh_randn_at).
+//
+// The integer part (splitmix64 mix, xorshift64* pair, 53-bit uniforms) and
+// the IEEE operations (*, /, sqrt; correctly rounded on both the host and
+// gfx950, with FMA contraction disabled) are the reference's.  The one
+// libm call, log(q), is replaced by a double-double evaluation that is
+// correctly rounded except when the exact value lies within ~2^-100
+// relative of a rounding midpoint:
+//     q = 2^e m, m in [sqrt(1/2), sqrt(2)),  t = (m - 1) / (m + 1),
+//     log q = e ln2 + 2 atanh(t) = e ln2 + 2 t sum_k t^(2k) / (2k + 1),
+// summed by double-double Horner (24 terms, |t| <= 0.1716 so t^48 < 2^-120).
+// A draw therefore equals the host's bit for bit wherever the host libm log
+// is correctly rounded; the tests count the exceptions exhaustively.
+#pragma once
+#include <stdint.h>
+#include <math.h>
+
+#if defined(__HIPCC__)
+#define LGH_HD __host__ __device__
+#else
+#define LGH_HD
+#endif
+
+namespace lgh {
+
+struct dd { double hi, lo; };
+
+LGH_HD inline dd dd_two_sum (double a, double b)
+{
+#pragma clang fp contract(off)
+  double s = a + b;
+  double bb = s - a;
+  double e = (a - (s - bb)) + (b - bb);
+  return { s, e };
+}
+
+LGH_HD inline dd dd_quick (double a, double b)
+{
+#pragma clang fp contract(off)
+  double s = a + b;
+  double e = b - (s - a);
+  return { s, e };
+}
+
+LGH_HD inline dd dd_add (dd a, dd b)
+{
+#pragma clang fp contract(off)
+  dd s = dd_two_sum (a.hi, b.hi);
+  dd t = dd_two_sum (a.lo, b.lo);
+  s.lo += t.hi;
+  s = dd_quick (s.hi, s.lo);
+  s.lo += t.lo;
+  return dd_quick (s.hi, s.lo);
+}
+
+LGH_HD inline dd dd_mul (dd a, dd b)
+{
+#pragma clang fp contract(off)
+  double p = a.hi * b.hi;
+  double e = fma (a.hi, b.hi, -p);
+  e += a.hi * b.lo + a.lo * b.hi;
+  return dd_quick (p, e);
+}
+
+LGH_HD inline dd dd_mul_d (dd a, double b)
+{
+#pragma clang fp contract(off)
+  double p = a.hi * b;
+  double e = fma (a.hi, b, -p);
+  e += a.lo * b;
+  return dd_quick (p, e);
+}
+
+// 1/d as a double-double (d a small odd integer)
+LGH_HD inline dd dd_recip (double d)
+{
+#pragma clang fp contract(off)
+  double h = 1.0 / d;
+  double r = fma (-h, d, 1.0);         // exact remainder 1 - h d
+  return { h, r / d };
+}
+
+LGH_HD inline double log_dd (double q)
+{
+#pragma clang fp contract(off)
+  // q is a positive normal double (q >= 2^-104 for the polar draws)
+  union { double d; uint64_t u; } cv;
+  cv.d = q;
+  int e = (int) ((cv.u >> 52) & 0x7ff) - 1023;
+  cv.u = (cv.u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;   // [1,2)
+  double m = cv.d;
+  if (m > 1.4142135623730951) { m = 0.5 * m; e += 1; }              // exact
+  // t = (m - 1)/(m + 1) in double-double
+  double num = m - 1.0;                                             // exact
+  dd den = dd_two_sum (m, 1.0);
+  double th = num / den.hi;
+  // residual num - th*den exactly enough: num - th*den.hi - th*den.lo
+  double p = th * den.hi;
+  double pe = fma (th, den.hi, -p);
+  double r = ((num - p) - pe) - th * den.lo;
+  dd t = dd_quick (th, r / den.hi);
+  dd t2 = dd_mul (t, t);
+  dd S = dd_recip (49.0);
+  for (int k = 23; k >= 0; k--) {
+    S = dd_mul (S, t2);
+    S = dd_add (S, dd_recip (2.0 * k + 1.0));
+  }
+  dd L = dd_mul (t, S);
+  L.hi *= 2.0; L.lo *= 2.0;                                         // exact
+  // e ln2, ln2 = 0x1.62e42fefa39efp-1 + 0x1.abc9e3b39803fp-56
+  const double ln2_hi = 6.93147180559945286227e-01;
+  const double ln2_lo = 2.31904681384629955842e-17;
+  double de = (double) e;
+  double a = de * ln2_hi;
+  double ae = fma (de, ln2_hi, -a);
+  dd E = dd_quick (a, ae + de * ln2_lo);
+  dd res = dd_add (E, L);
+  return res.hi;
+}
+
+LGH_HD inline double randn_at (uint64_t seed, int64_t i, int64_t j)
+{
+#pragma clang fp contract(off)
+  uint64_t z = seed + 0x9E3779B97F4A7C15ULL * (uint64_t) (i + 1)
+               + 0xC2B2AE3D27D4EB4FULL * (uint64_t) (j + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z = z ^ (z >> 31);
+  uint64_t s = z ? z : 0x853C49E6748FEA9BULL;
+  for (;;) {
+    uint64_t x = s;
+    x ^= x >> 12; x ^= x << 25; x ^= x >> 27; s = x;
+    double u1 = (double) ((x * 2685821657736338717ULL) >> 11) / 9007199254740992.0;
+    x = s;
+    x ^= x >> 12; x ^= x << 25; x ^= x >> 27; s = x;
+    double u2 = (double) ((x * 2685821657736338717ULL) >> 11) / 9007199254740992.0;
+    u1 = 2. * u1 - 1.;
+    u2 = 2. * u2 - 1.;
+    double a = u1 * u1;
+    double b = u2 * u2;
+    double q = a + b;
+    if (q > 0. && q < 1.) {
+      double f = sqrt (-2. * log_dd (q) / q);
+      return u1 * f;
+    }
+  }
+}
+
+}  // namespace lgh

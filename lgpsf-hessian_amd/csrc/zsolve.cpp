@@ -1,0 +1,295 @@
+// zsolve.cpp — device Z-solve tiers (see zsolve.hpp).
+#include "zsolve.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace lgh {
+
+DevMG::~DevMG ()
+{
+  free_work ();
+  for (auto *a : A) delete a;
+  for (auto *a : P) delete a;
+  for (auto *a : PT) delete a;
+  dfree (cinv);
+}
+
+void
+DevMG::free_work ()
+{
+  for (auto *vv : { &R1, &R2, &D1, &D2, &Bl, &Xl })
+    for (auto &m : *vv) pmat_free (m);
+  R1.clear (); R2.clear (); D1.clear (); D2.clear (); Bl.clear (); Xl.clear ();
+  wnp = -1;
+}
+
+int
+mg_upload (const AmgHierarchy &h, int prec, DevMG **out)
+{
+  DevMG *g = new DevMG ();
+  g->prec = prec;
+  g->nl = (int) h.lev.size ();
+  g->A.assign (g->nl, nullptr);
+  g->P.assign (g->nl, nullptr);
+  g->PT.assign (g->nl, nullptr);
+  int rc = 0;
+  for (int l = 0; l < g->nl && !rc; l++) {
+    const AmgLevel &lv = h.lev[l];
+    g->n.push_back (lv.A.n);
+    g->slo.push_back (lv.slo);
+    g->shi.push_back (lv.shi);
+    g->nu.push_back (lv.nu);
+    rc = csr_upload (lv.A.n, lv.A.m, lv.A.rp.data (), lv.A.ci.data (),
+                     lv.A.v.data (), prec, 1, &g->A[l]);
+    if (!rc && l > 0) {
+      rc = csr_upload (lv.P.n, lv.P.m, lv.P.rp.data (), lv.P.ci.data (),
+                       lv.P.v.data (), prec, 0, &g->P[l]);
+      if (!rc) {
+        HostCSR T = host_transpose (lv.P);
+        rc = csr_upload (T.n, T.m, T.rp.data (), T.ci.data (), T.v.data (),
+                         prec, 0, &g->PT[l]);
+      }
+    }
+  }
+  if (!rc) {
+    g->n0 = h.lev[0].A.n;
+    const size_t cnt = (size_t) g->n0 * g->n0;
+    g->cinv = dmalloc (cnt * prec_bytes (prec));
+    if (!g->cinv) rc = 5;
+    else if (prec == F64) rc = h2d (g->cinv, h.coarse_inv.data (), cnt * 8);
+    else {
+      std::vector<float> f (h.coarse_inv.begin (), h.coarse_inv.end ());
+      rc = h2d (g->cinv, f.data (), cnt * 4);
+    }
+  }
+  if (rc) { delete g; return rc; }
+  *out = g;
+  return 0;
+}
+
+static int
+mg_work (DevMG &g, const PMat &like)
+{
+  const int np = like.npanels ();
+  if (g.wW == like.W && g.wnp == np) return 0;
+  g.free_work ();
+  g.R1.resize (g.nl); g.R2.resize (g.nl); g.D1.resize (g.nl);
+  g.D2.resize (g.nl); g.Bl.resize (g.nl); g.Xl.resize (g.nl);
+  const int nc = np * like.W;
+  for (int l = 0; l < g.nl; l++) {
+    if (l > 0) {
+      LGH_TRY (pmat_alloc (g.R1[l], g.n[l], nc, like.W, g.prec));
+      LGH_TRY (pmat_alloc (g.R2[l], g.n[l], nc, like.W, g.prec));
+      LGH_TRY (pmat_alloc (g.D1[l], g.n[l], nc, like.W, g.prec));
+      LGH_TRY (pmat_alloc (g.D2[l], g.n[l], nc, like.W, g.prec));
+    }
+    if (l < g.nl - 1) {
+      LGH_TRY (pmat_alloc (g.Bl[l], g.n[l], nc, like.W, g.prec));
+      LGH_TRY (pmat_alloc (g.Xl[l], g.n[l], nc, like.W, g.prec));
+    }
+  }
+  g.wW = like.W;
+  g.wnp = np;
+  return 0;
+}
+
+// Chebyshev recurrence scalars of zsolve_impl.h:113-138
+struct ChebCoef {
+  double theta, delta, sigma1, rho_old;
+  ChebCoef (double lmin, double lmax)
+  {
+    theta = 0.5 * (lmax + lmin);
+    delta = 0.5 * (lmax - lmin);
+    sigma1 = theta / delta;
+    rho_old = 1.0 / sigma1;
+  }
+  // (a, b) for step k (k = 1: a = 0, b = 1/theta)
+  void step (int k, double *a, double *b)
+  {
+    if (k == 1) { *a = 0.0; *b = 1.0 / theta; return; }
+    const double rho = 1.0 / (2.0 * sigma1 - rho_old);
+    *a = rho * rho_old;
+    *b = 2.0 * rho / delta;
+    rho_old = rho;
+  }
+};
+
+// correction-form smoothing of level l (zsolve_impl.h:385-404).
+// x_zero: X is zero on entry (pre-smoothing), so R = B without a product.
+static int
+mg_smooth (DevMG &g, int l, const PMat &B, PMat &X, int x_zero)
+{
+  const DevCSR &A = *g.A[l];
+  ChebCoef cc (g.slo[l], g.shi[l]);
+  PMat *Rin = &g.R1[l], *Rout = &g.R2[l];
+  PMat *Dcur = nullptr, *Dn = &g.D1[l];
+  const PMat *R0 = &B;
+  if (!x_zero) {
+    LGH_TRY (k_resid (A, B, X, g.R1[l]));
+    R0 = &g.R1[l];
+    Rout = &g.R2[l];
+  }
+  else Rout = &g.R1[l];
+  for (int k = 1; k <= g.nu[l]; k++) {
+    double a, b;
+    cc.step (k, &a, &b);
+    const PMat *Rsrc = (k == 1) ? R0 : Rin;
+    const int xmode = (k == 1) ? (x_zero ? 0 : 1) : 1;
+    LGH_TRY (k_cheb_step (A, 1, nullptr, a, b, Dcur, *Rsrc, X, xmode, *Dn, *Rout));
+    // rotate: the output residual/direction become the inputs
+    Rin = Rout;
+    Rout = (Rin == &g.R1[l]) ? &g.R2[l] : &g.R1[l];
+    Dcur = Dn;
+    Dn = (Dcur == &g.D1[l]) ? &g.D2[l] : &g.D1[l];
+  }
+  return 0;
+}
+
+// zsolve_impl.h:406-485 (X overwritten; zero initial guess)
+static int
+mg_run (DevMG &g, int l, const PMat &B, PMat &X)
+{
+  if (l == 0) return k_dense_apply (g.cinv, g.n0, B, X);
+  LGH_TRY (mg_smooth (g, l, B, X, 1));                           // pre
+  LGH_TRY (k_resid (*g.A[l], B, X, g.R1[l]));
+  LGH_TRY (k_spmm (*g.PT[l], 1.0, g.R1[l], 0.0, g.Bl[l - 1]));
+  LGH_TRY (mg_run (g, l - 1, g.Bl[l - 1], g.Xl[l - 1]));
+  LGH_TRY (k_spmm (*g.P[l], 1.0, g.Xl[l - 1], 1.0, X));
+  LGH_TRY (mg_smooth (g, l, B, X, 0));                           // post
+  return 0;
+}
+
+int
+mg_vcycle (DevMG &g, const PMat &B, PMat &X)
+{
+  LGH_CHECK (B.prec == g.prec && X.prec == g.prec, 8,
+             "mg_vcycle: precision mismatch");
+  LGH_TRY (mg_work (g, B));
+  return mg_run (g, g.nl - 1, B, X);
+}
+
+ChebWork::~ChebWork ()
+{
+  pmat_free (R[0]); pmat_free (R[1]); pmat_free (D[0]); pmat_free (D[1]);
+  pmat_free (Z);
+}
+
+int
+ChebWork::ensure (const PMat &like)
+{
+  if (W == like.W && np == like.npanels () && prec == like.prec
+      && nrows == like.nrows) return 0;
+  pmat_free (R[0]); pmat_free (R[1]); pmat_free (D[0]); pmat_free (D[1]);
+  pmat_free (Z);
+  const int nc = like.npanels () * like.W;
+  LGH_TRY (pmat_alloc (R[0], like.nrows, nc, like.W, like.prec));
+  LGH_TRY (pmat_alloc (R[1], like.nrows, nc, like.W, like.prec));
+  LGH_TRY (pmat_alloc (D[0], like.nrows, nc, like.W, like.prec));
+  LGH_TRY (pmat_alloc (D[1], like.nrows, nc, like.W, like.prec));
+  LGH_TRY (pmat_alloc (Z, like.nrows, nc, like.W, like.prec));
+  W = like.W; np = like.npanels (); prec = like.prec; nrows = like.nrows;
+  return 0;
+}
+
+int
+cheb_solve (const DevCSR &A, DevMG *mg, double lmin, double lmax, int nit,
+            const PMat &B, PMat &X, ChebWork &w)
+{
+  LGH_CHECK (nit >= 1, 8, "cheb_solve: nit < 1");
+  LGH_TRY (w.ensure (B));
+  ChebCoef cc (lmin, lmax);
+  const PMat *Rsrc = &B;
+  int ri = 0, di = 0;
+  for (int k = 1; k <= nit; k++) {
+    double a, b;
+    cc.step (k, &a, &b);
+    const PMat *Dcur = (k == 1) ? nullptr : &w.D[di ^ 1];
+    if (mg) {
+      LGH_TRY (mg_vcycle (*mg, *Rsrc, w.Z));
+      LGH_TRY (k_cheb_step (A, 0, &w.Z, a, b, Dcur, *Rsrc, X, k > 1, w.D[di],
+                            w.R[ri]));
+    }
+    else
+      LGH_TRY (k_cheb_step (A, 1, nullptr, a, b, Dcur, *Rsrc, X, k > 1,
+                            w.D[di], w.R[ri]));
+    Rsrc = &w.R[ri];
+    ri ^= 1;
+    di ^= 1;
+  }
+  return 0;
+}
+
+int
+pcg_solve (const DevCSR &A, DevMG *mg, const double *dinv64, const PMat &B,
+           PMat &X, double rtol, int maxit, PcgInfo *info)
+{
+  const int nc = B.npanels () * B.W;
+  PMat R, Z, P, Q;
+  int rc = 0;
+  std::vector<double> dp (nc), pq (nc), dn (nc), zn (nc), z0 (nc);
+  std::vector<double> al (nc), be (nc);
+  std::vector<int> done (nc, 0);
+  double *dev = (double *) dmalloc (sizeof (double) * 4 * (size_t) nc);
+  if (!dev) return 5;
+  double *d_a = dev, *d_b = dev + nc, *d_t1 = dev + 2 * nc, *d_t2 = dev + 3 * nc;
+  auto precond = [&] (const PMat &r, PMat &z) -> int {
+    if (mg) return mg_vcycle (*mg, r, z);
+    return k_diag_scale (r, dinv64, z);
+  };
+  if (info) *info = PcgInfo ();
+#define PCG_TRY(e) do { rc = (e); if (rc) goto out; } while (0)
+  PCG_TRY (pmat_alloc (R, B.nrows, nc, B.W, B.prec));
+  PCG_TRY (pmat_alloc (Z, B.nrows, nc, B.W, B.prec));
+  PCG_TRY (pmat_alloc (P, B.nrows, nc, B.W, B.prec));
+  PCG_TRY (pmat_alloc (Q, B.nrows, nc, B.W, B.prec));
+  PCG_TRY (dzero (X.p, X.bytes ()));
+  PCG_TRY (dcopy (R.p, B.p, B.bytes ()));
+  PCG_TRY (precond (R, Z));
+  PCG_TRY (k_coldot (R, Z, d_t1));
+  PCG_TRY (k_coldot (Z, Z, d_t2));
+  PCG_TRY (d2h (dp.data (), d_t1, sizeof (double) * nc));
+  PCG_TRY (d2h (z0.data (), d_t2, sizeof (double) * nc));
+  for (int j = 0; j < nc; j++) {
+    z0[j] = std::sqrt (z0[j]);
+    if (!(z0[j] > 0.0)) done[j] = 1;
+  }
+  PCG_TRY (dcopy (P.p, Z.p, Z.bytes ()));
+  {
+    int it = 0, all = 1;
+    for (int j = 0; j < nc; j++) all &= done[j];
+    while (!all && it < maxit) {
+      PCG_TRY (k_spmm (A, 1.0, P, 0.0, Q));
+      PCG_TRY (k_coldot (P, Q, d_t1));
+      PCG_TRY (d2h (pq.data (), d_t1, sizeof (double) * nc));
+      for (int j = 0; j < nc; j++) al[j] = done[j] ? 0.0 : dp[j] / pq[j];
+      PCG_TRY (h2d (d_a, al.data (), sizeof (double) * nc));
+      PCG_TRY (k_cg_xr (d_a, P, Q, X, R));
+      PCG_TRY (precond (R, Z));
+      PCG_TRY (k_coldot (R, Z, d_t1));
+      PCG_TRY (k_coldot (Z, Z, d_t2));
+      PCG_TRY (d2h (dn.data (), d_t1, sizeof (double) * nc));
+      PCG_TRY (d2h (zn.data (), d_t2, sizeof (double) * nc));
+      for (int j = 0; j < nc; j++) be[j] = done[j] ? 0.0 : dn[j] / dp[j];
+      if (info && !done[0]) { info->alpha.push_back (al[0]); info->beta.push_back (be[0]); }
+      it++;
+      all = 1;
+      for (int j = 0; j < nc; j++) {
+        if (!done[j] && std::sqrt (zn[j]) <= rtol * z0[j]) done[j] = 1;
+        all &= done[j];
+        if (!done[j]) dp[j] = dn[j];
+      }
+      if (all) break;
+      PCG_TRY (h2d (d_b, be.data (), sizeof (double) * nc));
+      PCG_TRY (k_cg_p (d_b, Z, P));
+    }
+    if (info) { info->its = it; info->converged = all; }
+  }
+out:
+#undef PCG_TRY
+  pmat_free (R); pmat_free (Z); pmat_free (P); pmat_free (Q);
+  dfree (dev);
+  return rc;
+}
+
+}  // namespace lgh

@@ -1,0 +1,52 @@
+// zsolve.hpp — device Z-solve tiers: fused Chebyshev-Jacobi, the block
+// V-cycle on the host-built hierarchy, Chebyshev around the V-cycle, and
+// block PCG (per-column coefficients) for the vector-solve path.
+#pragma once
+#include "lgh_dev.hpp"
+#include "amg.hpp"
+
+namespace lgh {
+
+struct DevMG {
+  int prec = F64;
+  int nl = 0;
+  std::vector<DevCSR *> A, P, PT;      // per PCMG level (P, PT null at 0)
+  std::vector<double> slo, shi;
+  std::vector<int> nu, n;
+  void *cinv = nullptr;                // dense row-major inverse of A_0
+  int n0 = 0;
+  // workspace for one (W, npanels) shape
+  int wW = 0, wnp = -1;
+  std::vector<PMat> R1, R2, D1, D2, Bl, Xl;
+  ~DevMG ();
+  void free_work ();
+};
+
+int mg_upload (const AmgHierarchy &h, int prec, DevMG **out);
+// X = V-cycle(B) at the finest level (zero initial guess)
+int mg_vcycle (DevMG &g, const PMat &B, PMat &X);
+
+struct ChebWork {
+  int W = 0, np = -1, prec = -1, nrows = 0;
+  PMat R[2], D[2], Z;
+  ~ChebWork ();
+  int ensure (const PMat &like);
+};
+
+// X = Chebyshev(nit) solve of A X = B, zero initial guess
+// (zsolve_impl.h:104-141 with the blocked-asolve contract of :629-633);
+// preconditioner: the V-cycle when mg, else Jacobi (A.dinv)
+int cheb_solve (const DevCSR &A, DevMG *mg, double lmin, double lmax,
+                int nit, const PMat &B, PMat &X, ChebWork &w);
+
+struct PcgInfo {
+  int its = 0;
+  int converged = 0;
+  std::vector<double> alpha, beta;     // column 0 history (Lanczos)
+};
+// per-column PCG to ||z|| <= rtol ||z0|| (preconditioned norm, as KSPCG),
+// preconditioner: V-cycle (mg) or Jacobi (dinv64: fp64 device vector)
+int pcg_solve (const DevCSR &A, DevMG *mg, const double *dinv64,
+               const PMat &B, PMat &X, double rtol, int maxit, PcgInfo *info);
+
+}  // namespace lgh

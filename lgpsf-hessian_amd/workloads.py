@@ -157,28 +157,31 @@ def prior_Z(side, dim, mass_coef, k_coef):
 
 def psf_B(side, dim, offsets_fn, value_fn, dtype=np.float64):
     """B rows from a list of integer offsets (same for every node, clipped at
-    the boundary) and a vectorized value function value_fn(x_i, x_j, d)
-    evaluated per offset; returns CSR with sorted columns."""
+    the boundary) and a vectorized value function value_fn(i, j, d) of the
+    row/column node indices and the offset vector, evaluated per offset.  Offsets are ordered so that column indices grow
+    with the offset index, which makes the CSR rows sorted by construction
+    (no COO sort)."""
     n = side ** dim
     X, h = grid_coords(side, dim)
     grid = np.indices((side,) * dim).reshape(dim, -1)[::-1].T  # (n, dim) ints
-    offs = offsets_fn(h)
-    rows_l, cols_l, vals_l = [], [], []
+    offs = np.asarray(offsets_fn(h))
     strides = np.array([side ** a for a in range(dim)])
-    for off in offs:
+    offs = offs[np.argsort((offs * strides).sum(1), kind="stable")]
+    vals = np.zeros((n, len(offs)), dtype=np.float64)
+    for k, off in enumerate(offs):
         tgt = grid + off[None, :]
         ok = np.all((tgt >= 0) & (tgt < side), axis=1)
         i = np.nonzero(ok)[0]
         j = (tgt[ok] * strides).sum(axis=1)
-        v = value_fn(X[i], X[j], off * h)
-        keep = v != 0.0
-        rows_l.append(i[keep]); cols_l.append(j[keep]); vals_l.append(v[keep])
-    r = np.concatenate(rows_l)
-    c = np.concatenate(cols_l)
-    v = np.concatenate(vals_l)
-    B = sp.csr_matrix((v.astype(dtype), (r, c)), shape=(n, n))
-    B.sort_indices()
-    return B
+        vals[i, k] = value_fn(i, j, np.broadcast_to(off * h, (len(i), dim)))
+    mask = vals != 0.0
+    rp = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(mask.sum(axis=1), out=rp[1:])
+    ii, kk = np.nonzero(mask)
+    ci = (ii + (offs[kk] * strides).sum(axis=1)).astype(np.int32)
+    v = vals[ii, kk].astype(dtype)
+    del vals, mask, ii, kk
+    return sp.csr_matrix((v, ci, rp.astype(np.int32)), shape=(n, n))
 
 
 def _ball_offsets(radius_cells, dim):
@@ -194,12 +197,12 @@ def cfg1_problem(side=128):
     (a(x_i) + a(x_j)) / 2, a(x) = 1 + 0.5 sin(2 pi x1) cos(2 pi x2)."""
     Z, M = prior_Z(side, 2, 1.0, 0.1)
 
-    def amp(x):
-        return 1.0 + 0.5 * np.sin(2 * np.pi * x[:, 0]) * np.cos(2 * np.pi * x[:, 1])
+    X, _ = grid_coords(side, 2)
+    a = 1.0 + 0.5 * np.sin(2 * np.pi * X[:, 0]) * np.cos(2 * np.pi * X[:, 1])
+    s = 2.0 / (side - 1)
 
-    def val(xi, xj, d):
-        s = 2.0 / (side - 1)
-        return 0.5 * (amp(xi) + amp(xj)) * np.exp(-(d ** 2).sum(1) / (2 * s * s))
+    def val(i, j, d):
+        return 0.5 * (a[i] + a[j]) * np.exp(-(d ** 2).sum(1) / (2 * s * s))
 
     B = psf_B(side, 2, lambda h: _ball_offsets(4.0, 2), val)
     return B, Z, M
@@ -212,17 +215,17 @@ def cfg2_problem(side=1024, dtype=np.float64):
     Z, M = prior_Z(side, 2, 1.0, 0.05)
     h = 1.0 / (side - 1)
 
-    def inv_sigma(x):
-        th = np.pi * x[:, 0]
-        c, s = np.cos(th), np.sin(th)
-        a, b = 1.0 / (3 * h) ** 2, 1.0 / (1.5 * h) ** 2
-        # R diag(a, b) R^T with R = [[c, -s], [s, c]]
-        return (a * c * c + b * s * s, (a - b) * c * s, a * s * s + b * c * c)
+    X, _ = grid_coords(side, 2)
+    th = np.pi * X[:, 0]
+    c, s = np.cos(th), np.sin(th)
+    a, b = 1.0 / (3 * h) ** 2, 1.0 / (1.5 * h) ** 2
+    # Sigma^{-1} = R diag(a, b) R^T with R = [[c, -s], [s, c]]
+    P_, Q_, R_ = a * c * c + b * s * s, (a - b) * c * s, a * s * s + b * c * c
 
-    def val(xi, xj, d):
-        out = np.zeros(len(xi))
-        for x in (xi, xj):
-            p, q, r = inv_sigma(x)
+    def val(i, j, d):
+        out = np.zeros(len(i))
+        for t in (i, j):
+            p, q, r = P_[t], Q_[t], R_[t]
             md2 = p * d[:, 0] ** 2 + 2 * q * d[:, 0] * d[:, 1] + r * d[:, 1] ** 2
             out += np.where(md2 <= 9.0, np.exp(-0.5 * md2), 0.0)
         return 0.5 * out
@@ -237,7 +240,7 @@ def cfg3_problem(side=128):
     Z, M = prior_Z(side, 3, 1.0, 0.1)
     s = 1.5 / (side - 1)
 
-    def val(xi, xj, d):
+    def val(i, j, d):
         return np.exp(-(d ** 2).sum(1) / (2 * s * s))
 
     B = psf_B(side, 3, lambda h: _ball_offsets(2.0, 3), val)

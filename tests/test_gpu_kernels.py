@@ -1,0 +1,93 @@
+"""Kernel-level parity of the gfx950 kernels through the lgh_hip_* C ABI
+against the FP64 oracle / numpy (run on an MI355X)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def dev(lgh):
+    lgh.init()
+    return lgh
+
+
+def rand_csr(n, m, density, seed):
+    A = sp.random(n, m, density=density, random_state=seed, format="csr")
+    A.data = np.random.default_rng(seed).standard_normal(A.nnz)
+    return A
+
+
+@pytest.mark.parametrize("Wd", [64, 8, 1])
+def test_randn_bitexact_vs_host_arithmetic(dev, oracle, Wd):
+    nr, nc = 1000, 70
+    got = dev.randn_fill(271828, 5, 3, nr, nc, W=Wd)
+    host = np.array([[dev.randn_at_host(271828, 5 + i, 3 + j) for j in range(nc)]
+                     for i in range(nr)])
+    assert np.array_equal(got, host)          # device == same algorithm on host
+    ref = oracle.randn_fill(271828, 5, 3, nr, nc)
+    bad = got != ref
+    assert bad.mean() < 2e-3
+    if bad.any():
+        assert np.max(np.abs(got[bad] - ref[bad]) / np.abs(ref[bad])) < 4e-16
+    f32 = dev.randn_fill(271828, 5, 3, nr, nc, W=Wd, prec=dev.PREC_F32)
+    assert np.array_equal(f32, got.astype(np.float32).astype(np.float64))
+
+
+@pytest.mark.parametrize("Wd", [64, 16, 1])
+def test_spmm(dev, Wd):
+    A = rand_csr(1003, 997, 0.01, 1)
+    X = np.random.default_rng(2).standard_normal((997, 70))
+    ref = A @ X
+    assert rel(dev.spmm(A, X, W=Wd), ref) < 1e-13
+    y32 = dev.spmm(A, X, prec_vals=dev.PREC_F32, prec_vec=dev.PREC_F32, W=Wd)
+    assert rel(y32, ref) < 1e-5
+
+
+def test_cheb_jacobi_matches_oracle(dev, oracle):
+    Z = W.tpm_Z()
+    n = Z.shape[0]
+    A = oracle.CSR.from_scipy(Z)
+    b = W.tg_probe(n)
+    B = np.stack([b * (1 + 0.1 * j) for j in range(11)], 1)
+    dinv = 1.0 / Z.diagonal()
+    lmin, lmax, nit = 0.1, 1.9, 37
+    ref = oracle.cheb_jacobi(A, dinv, lmin, lmax, nit, B)
+    for Wd in (64, 8):
+        assert rel(dev.cheb_jacobi(Z, lmin, lmax, nit, B, W=Wd), ref) < 1e-13
+    got32 = dev.cheb_jacobi(Z, lmin, lmax, nit, B, prec=dev.PREC_F32)
+    assert rel(got32, ref) < 1e-5
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-13), (1, 1e-5)])
+def test_gram_and_gemm(dev, prec, tol):
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((1003, 70))
+    Y = rng.standard_normal((1003, 130))
+    G = dev.gram(X, Y, prec=prec)
+    assert rel(G, X.T @ Y) < tol
+    Cm = rng.standard_normal((70, 130))
+    assert rel(dev.gemm_nn(X, Cm, prec=prec), X @ Cm) < tol
+    S = dev.gram(X, X, prec=prec)
+    assert rel(S, X.T @ X) < tol
+
+
+def test_vcycle_matches_oracle(dev, oracle):
+    Z, M = W.prior_Z(40, 2, 1.0, 0.1)
+    h = dev.AMG(Z)
+    levels = [{"A": oracle.CSR.from_scipy(l["A"]),
+               "P": oracle.CSR.from_scipy(l["P"]) if l["P"] is not None else None,
+               "slo": l["slo"], "shi": l["shi"], "nu": l["nu"]} for l in h.levels()]
+    mg = oracle.MG(levels)
+    B = np.random.default_rng(4).standard_normal((Z.shape[0], 5))
+    ref = mg.vcycle(B)
+    assert rel(h.vcycle(B), ref) < 1e-12
+    assert rel(h.vcycle(B, prec=dev.PREC_F32), ref) < 1e-5
+    assert rel(h.vcycle(B, W=4), ref) < 1e-12
