@@ -668,22 +668,26 @@ k_colscale (const double *s, const PMat &X, PMat &Y)
 // MFMA tall-skinny products (W = 64 panels)
 
 template <typename T> struct mfma;
+// Result layouts (probed on gfx950): f64 16x16x4 puts D(lane/16 + 4 r,
+// lane%16) in element r; f32 16x16x4 puts D(4 (lane/16) + r, lane%16).
 template <> struct mfma<double> {
   typedef d4 acc_t;
   static __device__ inline acc_t op (double a, double b, acc_t c)
   { return __builtin_amdgcn_mfma_f64_16x16x4f64 (a, b, c, 0, 0, 0); }
+  static __device__ inline int row (int lk, int r) { return lk + 4 * r; }
 };
 template <> struct mfma<float> {
   typedef f4 acc_t;
   static __device__ inline acc_t op (float a, float b, acc_t c)
   { return __builtin_amdgcn_mfma_f32_16x16x4f32 (a, b, c, 0, 0, 0); }
+  static __device__ inline int row (int lk, int r) { return 4 * lk + r; }
 };
 
 // Gram partials: block (chunk, pair) computes the 64 x 64 tile
 // X_pi^T Y_pj over rows [r0, r1).  Wave w owns rows 32*(w>>1).., cols
 // 32*(w&1).. of the tile as 2 x 2 MFMA 16 x 16 tiles.  MFMA 16x16x4
 // operand layout: A(i, k) at lane i + 16 k, B(k, j) at lane j + 16 k;
-// D(4 (lane>>4) + r, lane & 15) in accumulator element r.
+// result rows per mfma<T>::row.
 template <typename T>
 __global__ void __launch_bounds__ (TPB)
 gram_k (int n, const T *__restrict__ X, int xp0, const T *__restrict__ Y,
@@ -721,7 +725,7 @@ gram_k (int n, const T *__restrict__ X, int xp0, const T *__restrict__ Y,
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++)
       for (int r = 0; r < 4; r++) {
-        const int row = mb + 16 * a + 4 * lk + r;
+        const int row = mb + 16 * a + mfma<T>::row (lk, r);
         const int col = nb + 16 * b + li;
         tile[col * 64 + row] = (double) acc[a][b][r];
       }
@@ -838,7 +842,7 @@ gemm_nn_k (int n, const T *__restrict__ X, int xp0, int npx,
   T *Yq = Y + (size_t) (yp0 + q) * n * 64;
   for (int b = 0; b < 4; b++)
     for (int r = 0; r < 4; r++) {
-      const int row = r0 + 16 * wave + 4 * lk + r;
+      const int row = r0 + 16 * wave + mfma<T>::row (lk, r);
       const int col = 16 * b + li;
       if (row < n) {
         T *y = Yq + (size_t) row * 64 + col;

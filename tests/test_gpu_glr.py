@@ -185,3 +185,21 @@ def test_grid_extend_incremental(dev, grid, oracle):
     # leading (well-converged) modes agree
     k = min(5, g1.kept)
     assert np.max(np.abs(g2.lam[:k] - g1.lam[:k]) / np.abs(g1.lam[:k])) < 1e-6
+
+
+def test_c_consumer_of_public_api(tmp_path):
+    """tests/c/test_glr_api.cpp: unchanged-API C++ consumer (the shape of
+    tests/test_glr.cpp) linked against liblgh.so."""
+    import subprocess
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "tests", "c", "test_glr_api")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "c")], check=True)
+    files = []
+    for name in ("spd", "indef"):
+        p = tmp_path / ("eigs_%s.txt" % name)
+        np.savetxt(p, np.load(os.path.join(GOLDEN, "tg_eigs_%s.npy" % name)), fmt="%.17g")
+        files.append(str(p))
+    r = subprocess.run([exe] + files, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "PASS test_glr_api" in r.stdout
