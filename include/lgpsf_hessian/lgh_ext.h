@@ -14,16 +14,19 @@ extern "C" {
 #define LGH_PREC_F64 0
 #define LGH_PREC_F32 1
 
-/* Precision of each build stage.  FP64 everywhere reproduces the reference
- * arithmetic ("plain"); the mixed policy runs the range-finder sweeps
- * (sketch, power passes, their CholQR2 Grams) in FP32 and keeps the final
- * sweep, the Rayleigh-Ritz projection, U, the Cholesky factors and the
- * eigensolve in FP64. */
+/* Precision policy of the build.  All LGH_PREC_F64 reproduces the
+ * reference arithmetic ("plain").  The mixed policy (lgh_glr_xopts_mixed)
+ * runs the sketch sweep Y = F Omega entirely in FP32 (vectors, B, Z and
+ * the V-cycle: its rounding acts like a perturbation of the Gaussian
+ * sketch), runs every later sweep with FP64 vectors around an FP32 AMG
+ * V-cycle inside an FP64 Chebyshev iteration (FP64-accurate Z solves with
+ * the preconditioner work in FP32), keeps CholQR2, T = Q^T F Q, the
+ * eigensolve and U in FP64, and runs the residual-gate sweep in FP32. */
 typedef struct lgh_glr_xopts
 {
-  int prec_sweep;   /* sketch + power-pass F sweeps and their CholQR2      */
-  int prec_final;   /* last F sweep, T = Q^T F Q, U = Q C                  */
-  int prec_check;   /* residual-gate sweep (opts.check)                    */
+  int prec_sketch;   /* sketch sweep Y = F Omega                           */
+  int prec_precond;  /* V-cycle inside the later sweeps' Z solves          */
+  int prec_check;    /* residual-gate sweep (opts.check)                   */
 }
 lgh_glr_xopts_t;
 
@@ -58,7 +61,8 @@ lgh_prior_info_t;
 int lgh_prior_get_info (const lgh_prior_t *prior, lgh_prior_info_t *info);
 
 /* Z-solves through the prior's tiers (host column-major blocks):
- * which = 0 blocked tier (what the build uses), 1 vector (PCG) path.     */
+ * which = 0 blocked tier (what the build uses), 1 vector (PCG) path;
+ * prec = 0 FP64, 1 FP32, 2 FP64 vectors around the FP32 V-cycle.        */
 int lgh_prior_solve_host (lgh_prior_t *prior, int which, int prec, int m,
                           const double *X, int ldx, double *Y, int ldy);
 

@@ -65,7 +65,7 @@ class GlrReport(C.Structure):
 
 
 class GlrXOpts(C.Structure):
-    _fields_ = [("prec_sweep", C.c_int), ("prec_final", C.c_int),
+    _fields_ = [("prec_sketch", C.c_int), ("prec_precond", C.c_int),
                 ("prec_check", C.c_int)]
 
 

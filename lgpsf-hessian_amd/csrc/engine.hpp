@@ -25,6 +25,7 @@ struct lgh_prior {
   int nit = 0, bound_its = 0;
   double lmin = 0, lmax = 0;
   lgh::ChebWork cw[2];
+  lgh::ChebMixedWork cwm;
   int n = 0;
 };
 
@@ -49,7 +50,10 @@ namespace lgh {
 // device-side prior operations on panel matrices (precision = X.prec)
 int prior_setup_device (lgh_prior_t *p, int prec);
 // trans: 1 selects the transposed callbacks (Z symmetric on the Mat path)
-int prior_solve_block (lgh_prior_t *p, int trans, const PMat &X, PMat &Y);
+// pcp: preconditioner precision (F32 with FP64 X selects the mixed
+// Chebyshev around the FP32 V-cycle; -1: X's precision)
+int prior_solve_block (lgh_prior_t *p, int trans, const PMat &X, PMat &Y,
+                       int pcp = -1);
 int prior_solve_vec (lgh_prior_t *p, int trans, const PMat &X, PMat &Y);
 int prior_apply_Z (lgh_prior_t *p, int trans, const PMat &X, PMat &Y);
 void prior_ref (lgh_prior_t *p);

@@ -38,8 +38,8 @@ extern "C" lgh_glr_xopts_t
 lgh_glr_xopts_default (void)
 {
   lgh_glr_xopts_t x;
-  x.prec_sweep = LGH_PREC_F64;
-  x.prec_final = LGH_PREC_F64;
+  x.prec_sketch = LGH_PREC_F64;
+  x.prec_precond = LGH_PREC_F64;
   x.prec_check = LGH_PREC_F64;
   return x;
 }
@@ -48,30 +48,32 @@ extern "C" lgh_glr_xopts_t
 lgh_glr_xopts_mixed (void)
 {
   lgh_glr_xopts_t x;
-  x.prec_sweep = LGH_PREC_F32;
-  x.prec_final = LGH_PREC_F64;
+  x.prec_sketch = LGH_PREC_F32;
+  x.prec_precond = LGH_PREC_F32;
   x.prec_check = LGH_PREC_F32;
   return x;
 }
+
+static inline int to_prec (int p) { return p == LGH_PREC_F32 ? F32 : F64; }
 
 static inline int npan (int c) { return (c + 63) / 64; }
 
 // ---- F sweep (glr_common_impl.h:316-336) -----------------------------------
 // Y = Msqrt Z^{-1} B Z^{-T} Msqrt X ; Wk is scratch of X's shape
 static int
-apply_F (lgh_glr_t *g, const PMat &X, PMat &Y, PMat &Wk)
+apply_F (lgh_glr_t *g, const PMat &X, PMat &Y, PMat &Wk, int pcp = -1)
 {
   lgh_prior_t *p = g->prior;
   DevCSR *dB = nullptr;
   LGH_TRY (prior_setup_device (p, X.prec));
   LGH_TRY (mat_device_csr (g->B, X.prec, 0, &dB));
   LGH_TRY (k_diag_scale (X, p->d_msqrt, Wk));
-  LGH_TRY (prior_solve_block (p, 1, Wk, Y));
+  LGH_TRY (prior_solve_block (p, 1, Wk, Y, pcp));
   prof_set_b_tag (1);
   int rc = k_spmm (*dB, 1.0, Y, 0.0, Wk);
   prof_set_b_tag (0);
   LGH_TRY (rc);
-  LGH_TRY (prior_solve_block (p, 0, Wk, Y));
+  LGH_TRY (prior_solve_block (p, 0, Wk, Y, pcp));
   LGH_TRY (k_diag_scale (Y, p->d_msqrt, Y));
   return 0;
 }
@@ -319,59 +321,74 @@ residual_gate (lgh_glr_t *g, const std::vector<double> &evals,
 }
 
 static int
-timed_F (lgh_glr_t *g, const PMat &X, PMat &Y, PMat &Wk, double *t_op)
+timed_F (lgh_glr_t *g, const PMat &X, PMat &Y, PMat &Wk, double *t_op,
+         int pcp = -1)
 {
   const double t0 = MPI_Wtime ();
-  LGH_TRY (apply_F (g, X, Y, Wk));
+  LGH_TRY (apply_F (g, X, Y, Wk, pcp));
   LGH_TRY (dev_sync ());
   *t_op += MPI_Wtime () - t0;
   return 0;
 }
 
-// the build (glr_common_impl.h:415-608) with the precision policy of
-// lgh_glr_xopts_t
+// sketch sweep Y64 = F Omega(seed, columns col0..col0+nc-1) at precision
+// ps (FP32: vectors, B, Z and V-cycle all in FP32), result in FP64
+static int
+sketch_sweep (lgh_glr_t *g, int ps, int64_t col0, int nc, PMat &Y64,
+              double *t_op)
+{
+  const int n = g->n;
+  if (ps == F64) {
+    PMat Q, Wk;
+    LGH_TRY (pmat_alloc (Q, n, nc, 64, F64));
+    LGH_TRY (pmat_alloc (Wk, n, nc, 64, F64));
+    int rc = k_randn (Q, g->opts.seed, 0, col0);
+    if (!rc) rc = timed_F (g, Q, Y64, Wk, t_op, to_prec (g->x.prec_precond));
+    pmat_free (Q); pmat_free (Wk);
+    return rc;
+  }
+  PMat Q, Y, Wk;
+  LGH_TRY (pmat_alloc (Q, n, nc, 64, ps));
+  LGH_TRY (pmat_alloc (Y, n, nc, 64, ps));
+  LGH_TRY (pmat_alloc (Wk, n, nc, 64, ps));
+  int rc = k_randn (Q, g->opts.seed, 0, col0);
+  if (!rc) rc = timed_F (g, Q, Y, Wk, t_op, ps);
+  if (!rc) rc = k_copy_cols (Y, 0, Y64, 0, nc);
+  pmat_free (Q); pmat_free (Y); pmat_free (Wk);
+  return rc;
+}
+
+// the build (glr_common_impl.h:415-608) under the precision policy of
+// lgh_glr_xopts_t: every block after the sketch is FP64
 static int
 build (lgh_glr_t *g, lgh_glr_report_t *rep)
 {
   const int n = g->n, ell = g->opts.ell, P = npan (ell);
-  const int ps = g->x.prec_sweep == LGH_PREC_F32 ? F32 : F64;
-  const int pf = g->x.prec_final == LGH_PREC_F32 ? F32 : F64;
-  const int pc = g->x.prec_check == LGH_PREC_F32 ? F32 : F64;
+  const int ps = to_prec (g->x.prec_sketch);
+  const int pp = to_prec (g->x.prec_precond);
+  const int pc = to_prec (g->x.prec_check);
   memset (rep, 0, sizeof (*rep));
   PMat Q, Y, Wk;
-  LGH_TRY (pmat_alloc (Q, n, ell, 64, ps));
-  LGH_TRY (pmat_alloc (Y, n, ell, 64, ps));
-  LGH_TRY (pmat_alloc (Wk, n, ell, 64, ps));
-  int rc = k_randn (Q, g->opts.seed, 0, 0);                    // Omega
-  if (!rc) rc = timed_F (g, Q, Y, Wk, &rep->t_operator);
+  LGH_TRY (pmat_alloc (Q, n, ell, 64, F64));
+  LGH_TRY (pmat_alloc (Y, n, ell, 64, F64));
+  LGH_TRY (pmat_alloc (Wk, n, ell, 64, F64));
+  int rc = sketch_sweep (g, ps, 0, ell, Y, &rep->t_operator);   // F Omega
   for (int p = 0; p < g->opts.q_power && !rc; p++) {
     double t0 = MPI_Wtime ();
     rc = cholqr (Y, ell, 2, nullptr);
     if (!rc) rc = dev_sync ();
     rep->t_dense += MPI_Wtime () - t0;
     if (!rc) std::swap (Q.p, Y.p);                              // Q <- Y
-    if (!rc) rc = timed_F (g, Q, Y, Wk, &rep->t_operator);
+    if (!rc) rc = timed_F (g, Q, Y, Wk, &rep->t_operator, pp);
   }
   if (!rc) {
     double t0 = MPI_Wtime ();
     rc = cholqr (Y, ell, 2, nullptr);
     if (!rc) std::swap (Q.p, Y.p);
-    if (!rc && pf != ps) {
-      // precision change: carry Q to the final precision and restore its
-      // orthonormality there (one more CholQR2 in FP64)
-      PMat Qf, Yf, Wf;
-      rc = pmat_alloc (Qf, n, ell, 64, pf);
-      if (!rc) rc = pmat_alloc (Yf, n, ell, 64, pf);
-      if (!rc) rc = pmat_alloc (Wf, n, ell, 64, pf);
-      if (!rc) rc = k_copy_cols (Q, 0, Qf, 0, ell);
-      if (!rc) rc = cholqr (Qf, ell, 2, nullptr);
-      pmat_free (Q); pmat_free (Y); pmat_free (Wk);
-      Q = Qf; Y = Yf; Wk = Wf;
-    }
     if (!rc) rc = dev_sync ();
     rep->t_dense += MPI_Wtime () - t0;
   }
-  if (!rc) rc = timed_F (g, Q, Y, Wk, &rep->t_operator);     // Y = F Q
+  if (!rc) rc = timed_F (g, Q, Y, Wk, &rep->t_operator, pp);  // Y = F Q
   pmat_free (Wk);
   std::vector<double> evals, Araw ((size_t) ell * ell);
   std::vector<int> order;
@@ -393,13 +410,6 @@ build (lgh_glr_t *g, lgh_glr_report_t *rep)
   if (!rc) rc = residual_gate (g, evals, order, pc, rep);
   if (!rc) {
     pmat_free (g->Q);
-    if (Q.prec != F64) {
-      PMat Q64;
-      rc = pmat_alloc (Q64, n, ell, 64, F64);
-      if (!rc) rc = k_copy_cols (Q, 0, Q64, 0, ell);
-      pmat_free (Q);
-      Q = Q64;
-    }
     g->Q = Q;
     Q.p = nullptr;
     g->Araw = Araw;
@@ -461,32 +471,23 @@ static int
 extend_incr (lgh_glr_t *g, int k_new, lgh_glr_report_t *rep)
 {
   const int n = g->n, ell1 = g->ell, ell2 = ell1 + k_new;
-  const int ps = g->x.prec_sweep == LGH_PREC_F32 ? F32 : F64;
-  const int pf = g->x.prec_final == LGH_PREC_F32 ? F32 : F64;
-  const int pc = g->x.prec_check == LGH_PREC_F32 ? F32 : F64;
+  const int ps = to_prec (g->x.prec_sketch);
+  const int pp = to_prec (g->x.prec_precond);
+  const int pc = to_prec (g->x.prec_check);
   memset (rep, 0, sizeof (*rep));
   PMat Qn, Yn, Wn;
-  LGH_TRY (pmat_alloc (Qn, n, k_new, 64, ps));
-  LGH_TRY (pmat_alloc (Yn, n, k_new, 64, ps));
-  LGH_TRY (pmat_alloc (Wn, n, k_new, 64, ps));
-  int rc = k_randn (Qn, g->opts.seed, 0, ell1);              // :726-729
-  if (!rc) rc = timed_F (g, Qn, Yn, Wn, &rep->t_operator);
+  LGH_TRY (pmat_alloc (Qn, n, k_new, 64, F64));
+  LGH_TRY (pmat_alloc (Yn, n, k_new, 64, F64));
+  LGH_TRY (pmat_alloc (Wn, n, k_new, 64, F64));
+  // hashed columns continue the sequence past ell (:726-729)
+  int rc = sketch_sweep (g, ps, ell1, k_new, Yn, &rep->t_operator);
   for (int p = 0; p < g->opts.q_power && !rc; p++) {          // :739-747
     double t0 = MPI_Wtime ();
     rc = bcgs2 (nullptr, 0, Yn, k_new);
     if (!rc) rc = dev_sync ();
     rep->t_dense += MPI_Wtime () - t0;
     if (!rc) std::swap (Qn.p, Yn.p);
-    if (!rc) rc = timed_F (g, Qn, Yn, Wn, &rep->t_operator);
-  }
-  if (!rc && ps != pf) {
-    PMat Yf, Qf, Wf;
-    rc = pmat_alloc (Yf, n, k_new, 64, pf);
-    if (!rc) rc = pmat_alloc (Qf, n, k_new, 64, pf);
-    if (!rc) rc = pmat_alloc (Wf, n, k_new, 64, pf);
-    if (!rc) rc = k_copy_cols (Yn, 0, Yf, 0, k_new);
-    pmat_free (Qn); pmat_free (Yn); pmat_free (Wn);
-    Qn = Qf; Yn = Yf; Wn = Wf;
+    if (!rc) rc = timed_F (g, Qn, Yn, Wn, &rep->t_operator, pp);
   }
   if (!rc) {                                                  // :749-768
     double t0 = MPI_Wtime ();
@@ -495,7 +496,7 @@ extend_incr (lgh_glr_t *g, int k_new, lgh_glr_report_t *rep)
     rep->t_dense += MPI_Wtime () - t0;
     if (!rc) std::swap (Qn.p, Yn.p);
   }
-  if (!rc) rc = timed_F (g, Qn, Yn, Wn, &rep->t_operator);   // :789
+  if (!rc) rc = timed_F (g, Qn, Yn, Wn, &rep->t_operator, pp);   // :789
   pmat_free (Wn);
   // border Cnew = Q_all^T (F Q_new) (:792-814)
   std::vector<double> A2 ((size_t) ell2 * ell2, 0.0);
@@ -511,11 +512,10 @@ extend_incr (lgh_glr_t *g, int k_new, lgh_glr_report_t *rep)
     if (!rc) rc = d2h (ht.data (), top, sizeof (double) * ht.size ());
     if (!rc) rc = d2h (hb.data (), bot, sizeof (double) * hb.size ());
     dfree (top); dfree (bot);
-    // Cn(i, jn): i < ell1 from top, i >= ell1 from bot
     auto Cn = [&] (int i, int jn) {
       return i < ell1 ? ht[i + (size_t) jn * ld1] : hb[(i - ell1) + (size_t) jn * ldn];
     };
-    // bordered raw A (:819-840)
+    // bordered raw A (:819-840): new rows of old columns by symmetry of F
     for (int j = 0; j < ell1; j++) {
       for (int i = 0; i < ell1; i++)
         A2[i + (size_t) j * ell2] = g->Araw[i + (size_t) j * ell1];
@@ -525,14 +525,6 @@ extend_incr (lgh_glr_t *g, int k_new, lgh_glr_report_t *rep)
       for (int i = 0; i < ell2; i++) A2[i + (size_t) j * ell2] = Cn (i, j - ell1);
   }
   pmat_free (Yn);
-  PMat Qn64;
-  if (!rc && Qn.prec != F64) {
-    rc = pmat_alloc (Qn64, n, k_new, 64, F64);
-    if (!rc) rc = k_copy_cols (Qn, 0, Qn64, 0, k_new);
-    pmat_free (Qn);
-    Qn = Qn64;
-    Qn64.p = nullptr;
-  }
   std::vector<double> evals;
   std::vector<int> order;
   const int old_ell = g->opts.ell;

@@ -128,6 +128,184 @@ k_randn (PMat &out, uint64_t seed, int64_t row0, int64_t col0)
   return with_W (out.W, l);
 }
 
+
+// ---------------------------------------------------------------------------
+// W = 64 row kernels: one wavefront per row.  The row's CSR entries are
+// fetched 64 at a time with one coalesced load per array (lane k holds
+// entry k) and broadcast with v_readlane, so the neighbour gathers of a
+// row are independent loads that can all be in flight together.
+
+__device__ inline int bcast (int v, int e) { return __builtin_amdgcn_readlane (v, e); }
+__device__ inline float bcast (float v, int e)
+{ return __int_as_float (__builtin_amdgcn_readlane (__float_as_int (v), e)); }
+__device__ inline double bcast (double v, int e)
+{
+  const long long b = __double_as_longlong (v);
+  const int lo = __builtin_amdgcn_readlane ((int) (b & 0xffffffffLL), e);
+  const int hi = __builtin_amdgcn_readlane ((int) (b >> 32), e);
+  return __longlong_as_double (((long long) hi << 32) | (unsigned int) lo);
+}
+
+__device__ inline int uniform_row (int per_block_row)
+{
+  return __builtin_amdgcn_readfirstlane (per_block_row);
+}
+
+// Y[p] = alpha A X[p] + beta Y[p] for NP panels per block (B read once)
+template <typename TV, typename T, int NP>
+__global__ void __launch_bounds__ (TPB)
+spmm64_k (int n, int nx, const int *__restrict__ rp,
+          const int *__restrict__ ci, const TV *__restrict__ v, T alpha,
+          const T *__restrict__ X, T beta, T *__restrict__ Y, int npanels)
+{
+  const int lane = threadIdx.x & 63;
+  const int row = uniform_row (blockIdx.x * (TPB / 64) + (threadIdx.x >> 6));
+  if (row >= n) return;
+  const int p0 = blockIdx.y * NP;
+  const int nph = min (NP, npanels - p0);
+  const size_t xs = (size_t) nx * 64;
+  const T *Xb = X + (size_t) p0 * xs + lane;
+  T acc[NP];
+#pragma unroll
+  for (int p = 0; p < NP; p++) acc[p] = 0;
+  const int k0 = rp[row], k1 = rp[row + 1];
+  for (int kb = k0; kb < k1; kb += 64) {
+    const int kk = kb + lane;
+    int c = 0;
+    TV w = 0;
+    if (kk < k1) { c = ci[kk]; w = v[kk]; }
+    const int cnt = min (64, k1 - kb);
+    int e = 0;
+    for (; e + 4 <= cnt; e += 4) {
+      const int c0 = bcast (c, e), c1 = bcast (c, e + 1);
+      const int c2 = bcast (c, e + 2), c3 = bcast (c, e + 3);
+      const T w0 = (T) bcast (w, e), w1 = (T) bcast (w, e + 1);
+      const T w2 = (T) bcast (w, e + 2), w3 = (T) bcast (w, e + 3);
+#pragma unroll
+      for (int p = 0; p < NP; p++)
+        if (p < nph) {
+          const T *xp = Xb + p * xs;
+          const T x0 = xp[(size_t) c0 * 64], x1 = xp[(size_t) c1 * 64];
+          const T x2 = xp[(size_t) c2 * 64], x3 = xp[(size_t) c3 * 64];
+          acc[p] += w0 * x0;
+          acc[p] += w1 * x1;
+          acc[p] += w2 * x2;
+          acc[p] += w3 * x3;
+        }
+    }
+    for (; e < cnt; e++) {
+      const int c0 = bcast (c, e);
+      const T w0 = (T) bcast (w, e);
+#pragma unroll
+      for (int p = 0; p < NP; p++)
+        if (p < nph) acc[p] += w0 * Xb[p * xs + (size_t) c0 * 64];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NP; p++)
+    if (p < nph) {
+      T *y = Y + ((size_t) (p0 + p) * n + row) * 64 + lane;
+      *y = (beta == (T) 0) ? alpha * acc[p] : alpha * acc[p] + beta * *y;
+    }
+}
+
+// R = B - A X
+template <typename TV, typename T>
+__global__ void __launch_bounds__ (TPB)
+resid64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
+           const TV *__restrict__ v, const T *__restrict__ B,
+           const T *__restrict__ X, T *__restrict__ R)
+{
+  const int lane = threadIdx.x & 63;
+  const int row = uniform_row (blockIdx.x * (TPB / 64) + (threadIdx.x >> 6));
+  if (row >= n) return;
+  const size_t off = (size_t) blockIdx.y * n * 64 + lane;
+  T acc = 0;
+  const int k0 = rp[row], k1 = rp[row + 1];
+  for (int kb = k0; kb < k1; kb += 64) {
+    const int kk = kb + lane;
+    int c = 0;
+    TV w = 0;
+    if (kk < k1) { c = ci[kk]; w = v[kk]; }
+    const int cnt = min (64, k1 - kb);
+    int e = 0;
+    for (; e + 4 <= cnt; e += 4) {
+      const T x0 = X[off + (size_t) bcast (c, e) * 64];
+      const T x1 = X[off + (size_t) bcast (c, e + 1) * 64];
+      const T x2 = X[off + (size_t) bcast (c, e + 2) * 64];
+      const T x3 = X[off + (size_t) bcast (c, e + 3) * 64];
+      acc += (T) bcast (w, e) * x0;
+      acc += (T) bcast (w, e + 1) * x1;
+      acc += (T) bcast (w, e + 2) * x2;
+      acc += (T) bcast (w, e + 3) * x3;
+    }
+    for (; e < cnt; e++) acc += (T) bcast (w, e) * X[off + (size_t) bcast (c, e) * 64];
+  }
+  R[off + (size_t) row * 64] = B[off + (size_t) row * 64] - acc;
+}
+
+// fused Chebyshev step at W = 64 (same arithmetic as cheb_k)
+template <typename TV, typename T, bool JAC, bool HASD>
+__global__ void __launch_bounds__ (TPB)
+cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
+          const TV *__restrict__ v, const TV *__restrict__ dinv,
+          const T *__restrict__ Zs, T a, T b, const T *__restrict__ D,
+          const T *__restrict__ R, T *__restrict__ X, int xmode,
+          T *__restrict__ Dn, T *__restrict__ Rn)
+{
+  const int lane = threadIdx.x & 63;
+  const int row = uniform_row (blockIdx.x * (TPB / 64) + (threadIdx.x >> 6));
+  if (row >= n) return;
+  const size_t off = (size_t) blockIdx.y * n * 64 + lane;
+  auto dnew = [&] (int j, T dj) -> T {
+    const size_t q = off + (size_t) j * 64;
+    T z = JAC ? dj * R[q] : Zs[q];
+    T bz = b * z;
+    if (HASD) {
+      T ad = a * D[q];
+      return ad + bz;
+    }
+    return bz;
+  };
+  const T di = dnew (row, JAC ? (T) dinv[row] : (T) 0);
+  T acc = 0;
+  const int k0 = rp[row], k1 = rp[row + 1];
+  for (int kb = k0; kb < k1; kb += 64) {
+    const int kk = kb + lane;
+    int c = 0;
+    TV w = 0, dd = 0;
+    if (kk < k1) {
+      c = ci[kk];
+      w = v[kk];
+      if (JAC) dd = dinv[c];
+    }
+    const int cnt = min (64, k1 - kb);
+    int e = 0;
+    for (; e + 4 <= cnt; e += 4) {
+      const int c0 = bcast (c, e), c1 = bcast (c, e + 1);
+      const int c2 = bcast (c, e + 2), c3 = bcast (c, e + 3);
+      const T d0 = JAC ? (T) bcast (dd, e) : (T) 0;
+      const T d1 = JAC ? (T) bcast (dd, e + 1) : (T) 0;
+      const T d2 = JAC ? (T) bcast (dd, e + 2) : (T) 0;
+      const T d3 = JAC ? (T) bcast (dd, e + 3) : (T) 0;
+      const T y0 = dnew (c0, d0), y1 = dnew (c1, d1);
+      const T y2 = dnew (c2, d2), y3 = dnew (c3, d3);
+      acc += (T) bcast (w, e) * y0;
+      acc += (T) bcast (w, e + 1) * y1;
+      acc += (T) bcast (w, e + 2) * y2;
+      acc += (T) bcast (w, e + 3) * y3;
+    }
+    for (; e < cnt; e++) {
+      const T d0 = JAC ? (T) bcast (dd, e) : (T) 0;
+      acc += (T) bcast (w, e) * dnew (bcast (c, e), d0);
+    }
+  }
+  const size_t qi = off + (size_t) row * 64;
+  Dn[qi] = di;
+  X[qi] = xmode ? X[qi] + di : di;
+  Rn[qi] = R[qi] - acc;
+}
+
 // ---------------------------------------------------------------------------
 // SpMM: Y = alpha A X + beta Y
 
@@ -156,6 +334,24 @@ template <typename TV, typename T>
 static int
 spmm_t (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
 {
+  if (X.W == 64) {
+    const int np = X.npanels ();
+    const unsigned gx = grid1 (A.n, TPB / 64);
+#define LGH_SP64(NP)                                                          \
+    hipLaunchKernelGGL ((spmm64_k<TV, T, NP>), dim3 (gx, (np + NP - 1) / NP), \
+                        dim3 (TPB), 0, stream (), A.n, A.m, A.rp, A.ci,        \
+                        (const TV *) A.v, (T) alpha, (const T *) X.p,          \
+                        (T) beta, (T *) Y.p, np)
+    if (np >= 8) LGH_SP64 (8);
+    else if (np == 4) LGH_SP64 (4);
+    else if (np > 4) LGH_SP64 (8);
+    else if (np == 3) LGH_SP64 (4);
+    else if (np == 2) LGH_SP64 (2);
+    else LGH_SP64 (1);
+#undef LGH_SP64
+    LGH_LAUNCH_CHECK ();
+    return 0;
+  }
   auto l = [&] (auto wt) -> int {
       constexpr int W = decltype (wt)::value;
       constexpr int rows_per_block = (TPB / 64) * (64 / W);
@@ -218,6 +414,14 @@ template <typename TV, typename T>
 static int
 resid_t (const DevCSR &A, const PMat &B, const PMat &X, PMat &R)
 {
+  if (X.W == 64) {
+    dim3 g (grid1 (A.n, TPB / 64), X.npanels ());
+    hipLaunchKernelGGL ((resid64_k<TV, T>), g, dim3 (TPB), 0, stream (), A.n,
+                        A.rp, A.ci, (const TV *) A.v, (const T *) B.p,
+                        (const T *) X.p, (T *) R.p);
+    LGH_LAUNCH_CHECK ();
+    return 0;
+  }
   auto l = [&] (auto wt) -> int {
       constexpr int W = decltype (wt)::value;
       constexpr int rows_per_block = (TPB / 64) * (64 / W);
@@ -292,6 +496,22 @@ static int
 cheb_t (const DevCSR &A, int jac, const PMat *Zs, double a, double b,
         const PMat *D, const PMat &R, PMat &X, int xmode, PMat &Dn, PMat &Rn)
 {
+  if (R.W == 64) {
+    dim3 g (grid1 (A.n, TPB / 64), R.npanels ());
+    const T *zp = Zs ? (const T *) Zs->p : nullptr;
+    const T *dp = D ? (const T *) D->p : nullptr;
+#define LGH_CH64(J, H)                                                        \
+    hipLaunchKernelGGL ((cheb64_k<TV, T, J, H>), g, dim3 (TPB), 0, stream (), \
+                        A.n, A.rp, A.ci, (const TV *) A.v,                     \
+                        (const TV *) A.dinv, zp, (T) a, (T) b, dp,             \
+                        (const T *) R.p, (T *) X.p, xmode, (T *) Dn.p,         \
+                        (T *) Rn.p)
+    if (jac) { if (dp) LGH_CH64 (true, true); else LGH_CH64 (true, false); }
+    else { if (dp) LGH_CH64 (false, true); else LGH_CH64 (false, false); }
+#undef LGH_CH64
+    LGH_LAUNCH_CHECK ();
+    return 0;
+  }
   auto l = [&] (auto wt) -> int {
       constexpr int W = decltype (wt)::value;
       constexpr int rows_per_block = (TPB / 64) * (64 / W);
@@ -335,6 +555,90 @@ k_cheb_step (const DevCSR &A, int jacobi, const PMat *Zsrc, double a,
   return A.prec == F64
       ? cheb_t<double, float> (A, jacobi, Zsrc, a, b, D, R, X, x_mode, Dn, Rn)
       : cheb_t<float, float> (A, jacobi, Zsrc, a, b, D, R, X, x_mode, Dn, Rn);
+}
+
+
+// mixed-precision fused Chebyshev step (W = 64): FP64 residual R and
+// accumulator X, FP32 direction D and preconditioned residual Z.  The new
+// direction is rounded to FP32 once and that rounded value is what X and
+// R both see, so R stays the FP64 residual of X and the iteration reaches
+// FP64 accuracy with an FP32 preconditioner.  Also writes R in FP32 for
+// the next preconditioner application.
+template <bool HASD>
+__global__ void __launch_bounds__ (TPB)
+cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
+           const double *__restrict__ v, const float *__restrict__ Zs,
+           double a, double b, const float *__restrict__ D,
+           const double *__restrict__ R, double *__restrict__ X, int xmode,
+           float *__restrict__ Dn, double *__restrict__ Rn,
+           float *__restrict__ Rn32)
+{
+  const int lane = threadIdx.x & 63;
+  const int row = uniform_row (blockIdx.x * (TPB / 64) + (threadIdx.x >> 6));
+  if (row >= n) return;
+  const size_t off = (size_t) blockIdx.y * n * 64 + lane;
+  auto dnew = [&] (int j) -> double {
+    const size_t q = off + (size_t) j * 64;
+    double bz = b * (double) Zs[q];
+    double d = HASD ? a * (double) D[q] + bz : bz;
+    return (double) (float) d;
+  };
+  const double di = dnew (row);
+  double acc = 0;
+  const int k0 = rp[row], k1 = rp[row + 1];
+  for (int kb = k0; kb < k1; kb += 64) {
+    const int kk = kb + lane;
+    int c = 0;
+    double w = 0;
+    if (kk < k1) { c = ci[kk]; w = v[kk]; }
+    const int cnt = min (64, k1 - kb);
+    int e = 0;
+    for (; e + 4 <= cnt; e += 4) {
+      const double y0 = dnew (bcast (c, e)), y1 = dnew (bcast (c, e + 1));
+      const double y2 = dnew (bcast (c, e + 2)), y3 = dnew (bcast (c, e + 3));
+      acc += bcast (w, e) * y0;
+      acc += bcast (w, e + 1) * y1;
+      acc += bcast (w, e + 2) * y2;
+      acc += bcast (w, e + 3) * y3;
+    }
+    for (; e < cnt; e++) acc += bcast (w, e) * dnew (bcast (c, e));
+  }
+  const size_t qi = off + (size_t) row * 64;
+  Dn[qi] = (float) di;
+  X[qi] = xmode ? X[qi] + di : di;
+  const double rn = R[qi] - acc;
+  Rn[qi] = rn;
+  Rn32[qi] = (float) rn;
+}
+
+int
+k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
+                   const PMat *D32, const PMat &R, PMat &X, int x_mode,
+                   PMat &Dn32, PMat &Rn, PMat &Rn32)
+{
+  LGH_CHECK (A.prec == F64 && R.prec == F64 && X.prec == F64 && R.W == 64
+             && Z32.prec == F32 && Dn32.prec == F32 && Rn32.prec == F32
+             && Rn.prec == F64 && R.nrows == A.n, 8,
+             "k_cheb_step_mixed: shape/precision mismatch");
+  const double vec = (double) A.n * 64;
+  ProfScope ps (PK_CHEB_Z, R.npanels () * (csr_bytes (A)
+                + vec * (4 + (D32 ? 4 : 0) + 8 + (x_mode ? 8 : 0) + 4 + 8 + 8 + 4)),
+                R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * 64);
+  dim3 g (grid1 (A.n, TPB / 64), R.npanels ());
+  if (D32)
+    hipLaunchKernelGGL (cheb64m_k<true>, g, dim3 (TPB), 0, stream (), A.n,
+                        A.rp, A.ci, (const double *) A.v, (const float *) Z32.p,
+                        a, b, (const float *) D32->p, (const double *) R.p,
+                        (double *) X.p, x_mode, (float *) Dn32.p,
+                        (double *) Rn.p, (float *) Rn32.p);
+  else
+    hipLaunchKernelGGL (cheb64m_k<false>, g, dim3 (TPB), 0, stream (), A.n,
+                        A.rp, A.ci, (const double *) A.v, (const float *) Z32.p,
+                        a, b, nullptr, (const double *) R.p, (double *) X.p,
+                        x_mode, (float *) Dn32.p, (double *) Rn.p,
+                        (float *) Rn32.p);
+  LGH_LAUNCH_CHECK ();
+  return 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -131,6 +131,11 @@ int k_resid (const DevCSR &A, const PMat &B, const PMat &X, PMat &R);
 int k_cheb_step (const DevCSR &A, int jacobi, const PMat *Zsrc, double a,
                  double b, const PMat *D, const PMat &R, PMat &X, int x_mode,
                  PMat &Dn, PMat &Rn);
+// mixed step at W = 64: FP64 R, X (A values FP64), FP32 D and Z; also
+// writes the new residual in FP32 (Rn32) for the next preconditioner
+int k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
+                       const PMat *D32, const PMat &R, PMat &X, int x_mode,
+                       PMat &Dn32, PMat &Rn, PMat &Rn32);
 // X0 = Ainv B0 (dense n0 x n0 row-major inverse in X's precision)
 int k_dense_apply (const void *Ainv, int n0, const PMat &B, PMat &X);
 

@@ -274,7 +274,8 @@ cb_block (lgh_prior_t *p, int kind, const PMat &X, PMat &Y)
 }
 
 int
-lgh::prior_solve_block (lgh_prior_t *p, int trans, const PMat &X, PMat &Y)
+lgh::prior_solve_block (lgh_prior_t *p, int trans, const PMat &X, PMat &Y,
+                        int pcp)
 {
   if (!p->is_mat) return cb_block (p, trans ? 1 : 0, X, Y);
   LGH_TRY (prior_setup_device (p, X.prec));
@@ -282,6 +283,11 @@ lgh::prior_solve_block (lgh_prior_t *p, int trans, const PMat &X, PMat &Y)
   LGH_TRY (mat_device_csr (p->Z, X.prec, 1, &dz));
   const int mode = p->o.blocked_mode;
   if (mode < 2) return prior_solve_vec (p, trans, X, Y);
+  if (mode <= 3 && pcp == F32 && X.prec == F64 && X.W == 64) {
+    LGH_TRY (prior_setup_device (p, F32));
+    return cheb_solve_mixed (*dz, *p->mg[F32], p->lmin, p->lmax, p->nit, X, Y,
+                             p->cwm);
+  }
   return cheb_solve (*dz, mode <= 3 ? p->mg[X.prec] : nullptr, p->lmin,
                      p->lmax, p->nit, X, Y, p->cw[X.prec]);
 }
@@ -368,19 +374,23 @@ extern "C" int
 lgh_prior_solve_host (lgh_prior_t *p, int which, int prec, int m,
                       const double *X, int ldx, double *Y, int ldy)
 {
+  // prec: 0 FP64, 1 FP32, 2 FP64 vectors around the FP32 preconditioner
   const int n = p->n;
+  const int vprec = (prec == 1) ? F32 : F64;
+  const int pcp = (prec == 2) ? F32 : -1;
   std::vector<double> hx ((size_t) n * m);
   for (int j = 0; j < m; j++) memcpy (&hx[(size_t) j * n], X + (size_t) j * ldx, sizeof (double) * n);
   int W = 1;
   while (W < m && W < 64) W *= 2;
+  if (prec == 2) W = 64;
   PMat t, x, y;
-  int rc = prior_setup_device (p, prec);
+  int rc = prior_setup_device (p, vprec);
   if (!rc) rc = pmat_alloc (t, n, m, 1, F64);
-  if (!rc) rc = pmat_alloc (x, n, m, W, prec);
-  if (!rc) rc = pmat_alloc (y, n, m, W, prec);
+  if (!rc) rc = pmat_alloc (x, n, m, W, vprec);
+  if (!rc) rc = pmat_alloc (y, n, m, W, vprec);
   if (!rc) rc = h2d (t.p, hx.data (), sizeof (double) * n * m);
   if (!rc) rc = k_copy_cols (t, 0, x, 0, m);
-  if (!rc) rc = which == 0 ? prior_solve_block (p, 0, x, y) : prior_solve_vec (p, 0, x, y);
+  if (!rc) rc = which == 0 ? prior_solve_block (p, 0, x, y, pcp) : prior_solve_vec (p, 0, x, y);
   if (!rc) rc = k_copy_cols (y, 0, t, 0, m);
   if (!rc) rc = d2h (hx.data (), t.p, sizeof (double) * n * m);
   pmat_free (t); pmat_free (x); pmat_free (y);

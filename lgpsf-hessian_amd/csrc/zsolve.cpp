@@ -220,6 +220,55 @@ cheb_solve (const DevCSR &A, DevMG *mg, double lmin, double lmax, int nit,
   return 0;
 }
 
+ChebMixedWork::~ChebMixedWork ()
+{
+  pmat_free (R[0]); pmat_free (R[1]); pmat_free (D[0]); pmat_free (D[1]);
+  pmat_free (R32); pmat_free (Z32);
+}
+
+int
+ChebMixedWork::ensure (const PMat &like)
+{
+  if (np == like.npanels () && nrows == like.nrows) return 0;
+  pmat_free (R[0]); pmat_free (R[1]); pmat_free (D[0]); pmat_free (D[1]);
+  pmat_free (R32); pmat_free (Z32);
+  const int nc = like.npanels () * 64;
+  LGH_TRY (pmat_alloc (R[0], like.nrows, nc, 64, F64));
+  LGH_TRY (pmat_alloc (R[1], like.nrows, nc, 64, F64));
+  LGH_TRY (pmat_alloc (D[0], like.nrows, nc, 64, F32));
+  LGH_TRY (pmat_alloc (D[1], like.nrows, nc, 64, F32));
+  LGH_TRY (pmat_alloc (R32, like.nrows, nc, 64, F32));
+  LGH_TRY (pmat_alloc (Z32, like.nrows, nc, 64, F32));
+  np = like.npanels ();
+  nrows = like.nrows;
+  return 0;
+}
+
+int
+cheb_solve_mixed (const DevCSR &A64, DevMG &mg32, double lmin, double lmax,
+                  int nit, const PMat &B, PMat &X, ChebMixedWork &w)
+{
+  LGH_CHECK (nit >= 1 && B.W == 64 && B.prec == F64 && X.prec == F64
+             && mg32.prec == F32, 8, "cheb_solve_mixed: shape/precision");
+  LGH_TRY (w.ensure (B));
+  ChebCoef cc (lmin, lmax);
+  LGH_TRY (k_copy_cols (B, 0, w.R32, 0, B.npanels () * 64));
+  const PMat *Rsrc = &B;
+  int ri = 0, di = 0;
+  for (int k = 1; k <= nit; k++) {
+    double a, b;
+    cc.step (k, &a, &b);
+    const PMat *Dcur = (k == 1) ? nullptr : &w.D[di ^ 1];
+    LGH_TRY (mg_vcycle (mg32, w.R32, w.Z32));
+    LGH_TRY (k_cheb_step_mixed (A64, w.Z32, a, b, Dcur, *Rsrc, X, k > 1,
+                                w.D[di], w.R[ri], w.R32));
+    Rsrc = &w.R[ri];
+    ri ^= 1;
+    di ^= 1;
+  }
+  return 0;
+}
+
 int
 pcg_solve (const DevCSR &A, DevMG *mg, const double *dinv64, const PMat &B,
            PMat &X, double rtol, int maxit, PcgInfo *info)

@@ -39,6 +39,18 @@ struct ChebWork {
 int cheb_solve (const DevCSR &A, DevMG *mg, double lmin, double lmax,
                 int nit, const PMat &B, PMat &X, ChebWork &w);
 
+// FP64-accurate Chebyshev around an FP32 V-cycle (W = 64 panels):
+// X, R in FP64; D, Z and the preconditioner in FP32 (k_cheb_step_mixed)
+struct ChebMixedWork {
+  int np = -1, nrows = 0;
+  PMat R[2], D[2], R32, Z32;
+  ~ChebMixedWork ();
+  int ensure (const PMat &like);
+};
+int cheb_solve_mixed (const DevCSR &A64, DevMG &mg32, double lmin,
+                      double lmax, int nit, const PMat &B, PMat &X,
+                      ChebMixedWork &w);
+
 struct PcgInfo {
   int its = 0;
   int converged = 0;

@@ -147,6 +147,12 @@ def test_grid_blocked_solve_matches_oracle(dev, grid, oracle):
     assert rel(grid["prior"].solve(X, which=0), ref) < 1e-12
     # and it is Z^{-1} to the Chebyshev tolerance
     assert rel(grid["Z"] @ ref, X) < 1e-8
+    # FP32 throughout: FP32-level agreement
+    assert rel(grid["prior"].solve(X, which=0, prec=1), ref) < 1e-4
+    # FP64 Chebyshev around the FP32 V-cycle: the same FP64-accurate Z^{-1}
+    mixed = grid["prior"].solve(X, which=0, prec=2)
+    assert rel(grid["Z"] @ mixed, X) < 1e-8
+    assert rel(mixed, ref) < 1e-8
 
 
 @pytest.mark.parametrize("mixed", [False, True])
