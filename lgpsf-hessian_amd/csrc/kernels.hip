@@ -151,6 +151,39 @@ __device__ inline int uniform_row (int per_block_row)
   return __builtin_amdgcn_readfirstlane (per_block_row);
 }
 
+// XCD-aware row tiling.  Work-groups are dispatched round-robin over the 8
+// XCDs (each with its own L2); remapping block b to tile
+// (b % 8) * (grid / 8) + b / 8 gives every XCD one contiguous range of
+// rows, so the CSR neighbours of a row (the rows just before and after it)
+// are found in that XCD's L2.  Each block owns ROWS64 consecutive rows;
+// wave w takes rows w, w + 4, ... of the tile.  The grid is padded to a
+// multiple of 8 blocks.
+static constexpr int ROWS64 = 64;
+
+__device__ inline int xcd_tile ()
+{
+  const int nb = gridDim.x;              // multiple of 8
+  const int b = blockIdx.x;
+  return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+static inline unsigned
+grid_rows64 (int n)
+{
+  unsigned g = (unsigned) ((n + ROWS64 - 1) / ROWS64);
+  return (g + 7) / 8 * 8;
+}
+
+#define LGH_ROW_BEGIN(n)                                                       \
+  {                                                                            \
+    const int rbase_ = xcd_tile () * ROWS64 + (threadIdx.x >> 6);              \
+    for (int t_ = 0; t_ < ROWS64 / (TPB / 64); t_++) {                         \
+      const int row = uniform_row (rbase_ + t_ * (TPB / 64));                  \
+      if (row >= (n)) break;
+#define LGH_ROW_END                                                            \
+    }                                                                          \
+  }
+
 // Y[p] = alpha A X[p] + beta Y[p] for NP panels per block (B read once)
 template <typename TV, typename T, int NP>
 __global__ void __launch_bounds__ (TPB)
@@ -159,12 +192,11 @@ spmm64_k (int n, int nx, const int *__restrict__ rp,
           const T *__restrict__ X, T beta, T *__restrict__ Y, int npanels)
 {
   const int lane = threadIdx.x & 63;
-  const int row = uniform_row (blockIdx.x * (TPB / 64) + (threadIdx.x >> 6));
-  if (row >= n) return;
   const int p0 = blockIdx.y * NP;
   const int nph = min (NP, npanels - p0);
   const size_t xs = (size_t) nx * 64;
   const T *Xb = X + (size_t) p0 * xs + lane;
+  LGH_ROW_BEGIN (n)
   T acc[NP];
 #pragma unroll
   for (int p = 0; p < NP; p++) acc[p] = 0;
@@ -207,6 +239,7 @@ spmm64_k (int n, int nx, const int *__restrict__ rp,
       T *y = Y + ((size_t) (p0 + p) * n + row) * 64 + lane;
       *y = (beta == (T) 0) ? alpha * acc[p] : alpha * acc[p] + beta * *y;
     }
+  LGH_ROW_END
 }
 
 // R = B - A X
@@ -217,9 +250,8 @@ resid64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
            const T *__restrict__ X, T *__restrict__ R)
 {
   const int lane = threadIdx.x & 63;
-  const int row = uniform_row (blockIdx.x * (TPB / 64) + (threadIdx.x >> 6));
-  if (row >= n) return;
   const size_t off = (size_t) blockIdx.y * n * 64 + lane;
+  LGH_ROW_BEGIN (n)
   T acc = 0;
   const int k0 = rp[row], k1 = rp[row + 1];
   for (int kb = k0; kb < k1; kb += 64) {
@@ -242,6 +274,7 @@ resid64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
     for (; e < cnt; e++) acc += (T) bcast (w, e) * X[off + (size_t) bcast (c, e) * 64];
   }
   R[off + (size_t) row * 64] = B[off + (size_t) row * 64] - acc;
+  LGH_ROW_END
 }
 
 // fused Chebyshev step at W = 64 (same arithmetic as cheb_k)
@@ -254,8 +287,6 @@ cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
           T *__restrict__ Dn, T *__restrict__ Rn)
 {
   const int lane = threadIdx.x & 63;
-  const int row = uniform_row (blockIdx.x * (TPB / 64) + (threadIdx.x >> 6));
-  if (row >= n) return;
   const size_t off = (size_t) blockIdx.y * n * 64 + lane;
   auto dnew = [&] (int j, T dj) -> T {
     const size_t q = off + (size_t) j * 64;
@@ -267,6 +298,7 @@ cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
     }
     return bz;
   };
+  LGH_ROW_BEGIN (n)
   const T di = dnew (row, JAC ? (T) dinv[row] : (T) 0);
   T acc = 0;
   const int k0 = rp[row], k1 = rp[row + 1];
@@ -304,6 +336,7 @@ cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
   Dn[qi] = di;
   X[qi] = xmode ? X[qi] + di : di;
   Rn[qi] = R[qi] - acc;
+  LGH_ROW_END
 }
 
 // ---------------------------------------------------------------------------
@@ -336,7 +369,7 @@ spmm_t (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
 {
   if (X.W == 64) {
     const int np = X.npanels ();
-    const unsigned gx = grid1 (A.n, TPB / 64);
+    const unsigned gx = grid_rows64 (A.n);
 #define LGH_SP64(NP)                                                          \
     hipLaunchKernelGGL ((spmm64_k<TV, T, NP>), dim3 (gx, (np + NP - 1) / NP), \
                         dim3 (TPB), 0, stream (), A.n, A.m, A.rp, A.ci,        \
@@ -415,7 +448,7 @@ static int
 resid_t (const DevCSR &A, const PMat &B, const PMat &X, PMat &R)
 {
   if (X.W == 64) {
-    dim3 g (grid1 (A.n, TPB / 64), X.npanels ());
+    dim3 g (grid_rows64 (A.n), X.npanels ());
     hipLaunchKernelGGL ((resid64_k<TV, T>), g, dim3 (TPB), 0, stream (), A.n,
                         A.rp, A.ci, (const TV *) A.v, (const T *) B.p,
                         (const T *) X.p, (T *) R.p);
@@ -497,7 +530,7 @@ cheb_t (const DevCSR &A, int jac, const PMat *Zs, double a, double b,
         const PMat *D, const PMat &R, PMat &X, int xmode, PMat &Dn, PMat &Rn)
 {
   if (R.W == 64) {
-    dim3 g (grid1 (A.n, TPB / 64), R.npanels ());
+    dim3 g (grid_rows64 (A.n), R.npanels ());
     const T *zp = Zs ? (const T *) Zs->p : nullptr;
     const T *dp = D ? (const T *) D->p : nullptr;
 #define LGH_CH64(J, H)                                                        \
@@ -574,8 +607,6 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
            float *__restrict__ Rn32)
 {
   const int lane = threadIdx.x & 63;
-  const int row = uniform_row (blockIdx.x * (TPB / 64) + (threadIdx.x >> 6));
-  if (row >= n) return;
   const size_t off = (size_t) blockIdx.y * n * 64 + lane;
   auto dnew = [&] (int j) -> double {
     const size_t q = off + (size_t) j * 64;
@@ -583,6 +614,7 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
     double d = HASD ? a * (double) D[q] + bz : bz;
     return (double) (float) d;
   };
+  LGH_ROW_BEGIN (n)
   const double di = dnew (row);
   double acc = 0;
   const int k0 = rp[row], k1 = rp[row + 1];
@@ -609,6 +641,7 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
   const double rn = R[qi] - acc;
   Rn[qi] = rn;
   Rn32[qi] = (float) rn;
+  LGH_ROW_END
 }
 
 int
@@ -624,7 +657,7 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
   ProfScope ps (PK_CHEB_Z, R.npanels () * (csr_bytes (A)
                 + vec * (4 + (D32 ? 4 : 0) + 8 + (x_mode ? 8 : 0) + 4 + 8 + 8 + 4)),
                 R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * 64);
-  dim3 g (grid1 (A.n, TPB / 64), R.npanels ());
+  dim3 g (grid_rows64 (A.n), R.npanels ());
   if (D32)
     hipLaunchKernelGGL (cheb64m_k<true>, g, dim3 (TPB), 0, stream (), A.n,
                         A.rp, A.ci, (const double *) A.v, (const float *) Z32.p,

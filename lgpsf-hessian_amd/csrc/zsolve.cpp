@@ -117,8 +117,11 @@ struct ChebCoef {
 
 // correction-form smoothing of level l (zsolve_impl.h:385-404).
 // x_zero: X is zero on entry (pre-smoothing), so R = B without a product.
+// *Rout_final receives the buffer holding the recurrence residual B - A X
+// after the last step.
 static int
-mg_smooth (DevMG &g, int l, const PMat &B, PMat &X, int x_zero)
+mg_smooth (DevMG &g, int l, const PMat &B, PMat &X, int x_zero,
+           const PMat **Rfinal)
 {
   const DevCSR &A = *g.A[l];
   ChebCoef cc (g.slo[l], g.shi[l]);
@@ -143,6 +146,7 @@ mg_smooth (DevMG &g, int l, const PMat &B, PMat &X, int x_zero)
     Dcur = Dn;
     Dn = (Dcur == &g.D1[l]) ? &g.D2[l] : &g.D1[l];
   }
+  if (Rfinal) *Rfinal = (g.nu[l] >= 1) ? Rin : R0;
   return 0;
 }
 
@@ -151,12 +155,15 @@ static int
 mg_run (DevMG &g, int l, const PMat &B, PMat &X)
 {
   if (l == 0) return k_dense_apply (g.cinv, g.n0, B, X);
-  LGH_TRY (mg_smooth (g, l, B, X, 1));                           // pre
-  LGH_TRY (k_resid (*g.A[l], B, X, g.R1[l]));
-  LGH_TRY (k_spmm (*g.PT[l], 1.0, g.R1[l], 0.0, g.Bl[l - 1]));
+  // pre-smoothing; the residual B - A X it leaves (the Chebyshev
+  // recurrence keeps it) is restricted directly — the reference forms the
+  // same residual again with one more product (zsolve_impl.h:464-468)
+  const PMat *R = nullptr;
+  LGH_TRY (mg_smooth (g, l, B, X, 1, &R));
+  LGH_TRY (k_spmm (*g.PT[l], 1.0, *R, 0.0, g.Bl[l - 1]));
   LGH_TRY (mg_run (g, l - 1, g.Bl[l - 1], g.Xl[l - 1]));
   LGH_TRY (k_spmm (*g.P[l], 1.0, g.Xl[l - 1], 1.0, X));
-  LGH_TRY (mg_smooth (g, l, B, X, 0));                           // post
+  LGH_TRY (mg_smooth (g, l, B, X, 0, nullptr));                  // post
   return 0;
 }
 
