@@ -675,6 +675,143 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
 }
 
 // ---------------------------------------------------------------------------
+// degree-2 correction-form smoother (zsolve_impl.h:385-404 with nu = 2) as
+// two fused passes.  With X = X0 on entry, step 1 gives D1 = b1 dinv R,
+// R1 = R - A D1; step 2 gives D2 = a2 D1 + b2 dinv R1, R2 = R1 - A D2 and
+// the smoothed X = X0 + D1 + D2.  X is touched once (in step 2), D2 is
+// never stored, and R2 only when it is used (pre-smoothing: restriction).
+
+// step 1: D1 = b dinv Rin ; R1 = Rin - A D1
+template <typename TV, typename T>
+__global__ void __launch_bounds__ (TPB)
+sm1_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
+       const TV *__restrict__ v, const TV *__restrict__ dinv, T b,
+       const T *__restrict__ Rin, T *__restrict__ D1, T *__restrict__ R1)
+{
+  const int lane = threadIdx.x & 63;
+  const size_t off = (size_t) blockIdx.y * n * 64 + lane;
+  LGH_ROW_BEGIN (n)
+  const T di = b * ((T) dinv[row] * Rin[off + (size_t) row * 64]);
+  T acc = 0;
+  const int k0 = rp[row], k1 = rp[row + 1];
+  for (int kb = k0; kb < k1; kb += 64) {
+    const int kk = kb + lane;
+    int c = 0;
+    TV w = 0, dd = 0;
+    if (kk < k1) { c = ci[kk]; w = v[kk]; dd = dinv[c]; }
+    const int cnt = min (64, k1 - kb);
+    int e = 0;
+    for (; e + 4 <= cnt; e += 4) {
+      const T y0 = b * ((T) bcast (dd, e) * Rin[off + (size_t) bcast (c, e) * 64]);
+      const T y1 = b * ((T) bcast (dd, e + 1) * Rin[off + (size_t) bcast (c, e + 1) * 64]);
+      const T y2 = b * ((T) bcast (dd, e + 2) * Rin[off + (size_t) bcast (c, e + 2) * 64]);
+      const T y3 = b * ((T) bcast (dd, e + 3) * Rin[off + (size_t) bcast (c, e + 3) * 64]);
+      acc += (T) bcast (w, e) * y0;
+      acc += (T) bcast (w, e + 1) * y1;
+      acc += (T) bcast (w, e + 2) * y2;
+      acc += (T) bcast (w, e + 3) * y3;
+    }
+    for (; e < cnt; e++)
+      acc += (T) bcast (w, e)
+             * (b * ((T) bcast (dd, e) * Rin[off + (size_t) bcast (c, e) * 64]));
+  }
+  const size_t qi = off + (size_t) row * 64;
+  D1[qi] = di;
+  R1[qi] = Rin[qi] - acc;
+  LGH_ROW_END
+}
+
+// step 2: D2 = a D1 + b dinv R1 ; X = (XADD ? X : 0) + D1 + D2 ;
+// R2 = R1 - A D2 when WR
+template <typename TV, typename T, bool XADD, bool WR>
+__global__ void __launch_bounds__ (TPB)
+sm2_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
+       const TV *__restrict__ v, const TV *__restrict__ dinv, T a, T b,
+       const T *__restrict__ D1, const T *__restrict__ R1,
+       T *__restrict__ X, T *__restrict__ R2)
+{
+  const int lane = threadIdx.x & 63;
+  const size_t off = (size_t) blockIdx.y * n * 64 + lane;
+  auto d2 = [&] (int j, T dj) -> T {
+    const size_t q = off + (size_t) j * 64;
+    T bz = b * (dj * R1[q]);
+    T ad = a * D1[q];
+    return ad + bz;
+  };
+  LGH_ROW_BEGIN (n)
+  const size_t qi = off + (size_t) row * 64;
+  const T di = d2 (row, (T) dinv[row]);
+  const T d1 = D1[qi];
+  X[qi] = XADD ? X[qi] + d1 + di : d1 + di;
+  if (WR) {
+    T acc = 0;
+    const int k0 = rp[row], k1 = rp[row + 1];
+    for (int kb = k0; kb < k1; kb += 64) {
+      const int kk = kb + lane;
+      int c = 0;
+      TV w = 0, dd = 0;
+      if (kk < k1) { c = ci[kk]; w = v[kk]; dd = dinv[c]; }
+      const int cnt = min (64, k1 - kb);
+      int e = 0;
+      for (; e + 4 <= cnt; e += 4) {
+        const T y0 = d2 (bcast (c, e), (T) bcast (dd, e));
+        const T y1 = d2 (bcast (c, e + 1), (T) bcast (dd, e + 1));
+        const T y2 = d2 (bcast (c, e + 2), (T) bcast (dd, e + 2));
+        const T y3 = d2 (bcast (c, e + 3), (T) bcast (dd, e + 3));
+        acc += (T) bcast (w, e) * y0;
+        acc += (T) bcast (w, e + 1) * y1;
+        acc += (T) bcast (w, e + 2) * y2;
+        acc += (T) bcast (w, e + 3) * y3;
+      }
+      for (; e < cnt; e++) acc += (T) bcast (w, e) * d2 (bcast (c, e), (T) bcast (dd, e));
+    }
+    R2[qi] = R1[qi] - acc;
+  }
+  LGH_ROW_END
+}
+
+template <typename TV, typename T>
+static int
+smooth2_t (const DevCSR &A, double b1, double a2, double b2, const PMat &Rin,
+           PMat &D1, PMat &R1, PMat &X, int xadd, PMat *R2)
+{
+  dim3 g (grid_rows64 (A.n), Rin.npanels ());
+  hipLaunchKernelGGL ((sm1_k<TV, T>), g, dim3 (TPB), 0, stream (), A.n, A.rp,
+                      A.ci, (const TV *) A.v, (const TV *) A.dinv, (T) b1,
+                      (const T *) Rin.p, (T *) D1.p, (T *) R1.p);
+  LGH_LAUNCH_CHECK ();
+  T *r2 = R2 ? (T *) R2->p : nullptr;
+#define LGH_SM2(XA, W_)                                                       \
+  hipLaunchKernelGGL ((sm2_k<TV, T, XA, W_>), g, dim3 (TPB), 0, stream (),    \
+                      A.n, A.rp, A.ci, (const TV *) A.v, (const TV *) A.dinv, \
+                      (T) a2, (T) b2, (const T *) D1.p, (const T *) R1.p,      \
+                      (T *) X.p, r2)
+  if (xadd) { if (R2) LGH_SM2 (true, true); else LGH_SM2 (true, false); }
+  else { if (R2) LGH_SM2 (false, true); else LGH_SM2 (false, false); }
+#undef LGH_SM2
+  LGH_LAUNCH_CHECK ();
+  return 0;
+}
+
+int
+k_smooth2 (const DevCSR &A, double b1, double a2, double b2, const PMat &Rin,
+           PMat &D1, PMat &R1, PMat &X, int xadd, PMat *R2)
+{
+  LGH_CHECK (Rin.W == 64 && X.W == 64 && D1.W == 64 && R1.W == 64
+             && Rin.prec == X.prec && A.dinv && Rin.nrows == A.n, 8,
+             "k_smooth2: shape mismatch");
+  const double vec = (double) A.n * 64 * prec_bytes (Rin.prec);
+  const double streams = 3 + (2 + (xadd ? 1 : 0) + 1 + (R2 ? 1 : 0));
+  ProfScope ps (PK_CHEB_JAC, Rin.npanels () * (2 * csr_bytes (A) + streams * vec),
+                Rin.npanels () * (4.0 * (double) A.nnz + 10.0 * A.n) * 64);
+  if (Rin.prec == F64)
+    return A.prec == F64 ? smooth2_t<double, double> (A, b1, a2, b2, Rin, D1, R1, X, xadd, R2)
+                         : smooth2_t<float, double> (A, b1, a2, b2, Rin, D1, R1, X, xadd, R2);
+  return A.prec == F64 ? smooth2_t<double, float> (A, b1, a2, b2, Rin, D1, R1, X, xadd, R2)
+                       : smooth2_t<float, float> (A, b1, a2, b2, Rin, D1, R1, X, xadd, R2);
+}
+
+// ---------------------------------------------------------------------------
 // dense coarse apply X0 = Ainv B0 (Ainv row-major n0 x n0)
 
 template <typename T, int W>

@@ -136,6 +136,12 @@ int k_cheb_step (const DevCSR &A, int jacobi, const PMat *Zsrc, double a,
 int k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
                        const PMat *D32, const PMat &R, PMat &X, int x_mode,
                        PMat &Dn32, PMat &Rn, PMat &Rn32);
+// degree-2 Chebyshev-Jacobi smoother in two fused passes (W = 64):
+// D1 = b1 dinv Rin, R1 = Rin - A D1; D2 = a2 D1 + b2 dinv R1;
+// X = (xadd ? X : 0) + D1 + D2; R2 = R1 - A D2 when R2 != null
+int k_smooth2 (const DevCSR &A, double b1, double a2, double b2,
+               const PMat &Rin, PMat &D1, PMat &R1, PMat &X, int xadd,
+               PMat *R2);
 // X0 = Ainv B0 (dense n0 x n0 row-major inverse in X's precision)
 int k_dense_apply (const void *Ainv, int n0, const PMat &B, PMat &X);
 
