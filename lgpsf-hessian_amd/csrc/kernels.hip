@@ -32,6 +32,13 @@ grid1 (size_t work, int per_block)
 
 #define LGH_LAUNCH_CHECK() LGH_HIP (hipGetLastError ())
 
+static double
+csr_bytes (const DevCSR &A)
+{
+  return (double) A.nnz * (prec_bytes (A.prec) + 4) + 4.0 * (A.n + 1);
+}
+
+
 // W dispatch helper: f(std::integral_constant<int, W>)
 template <int N> using IC = std::integral_constant<int, N>;
 template <typename F>
@@ -242,54 +249,93 @@ spmm64_k (int n, int nx, const int *__restrict__ rp,
   LGH_ROW_END
 }
 
-// R = B - A X
-template <typename TV, typename T>
-__global__ void __launch_bounds__ (TPB)
-resid64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
-           const TV *__restrict__ v, const T *__restrict__ B,
-           const T *__restrict__ X, T *__restrict__ R)
+// ---------------------------------------------------------------------------
+// W = 64 row kernels over NP panels at once: a wavefront owns one row and
+// NP panels of it, so the row's CSR entries (and dinv at the neighbours)
+// are fetched once for all NP panels and the NP neighbour gathers of each
+// entry are independent loads in flight together.
+
+template <typename F>
+static int
+np_dispatch (int np, F &&f)
 {
-  const int lane = threadIdx.x & 63;
-  const size_t off = (size_t) blockIdx.y * n * 64 + lane;
-  LGH_ROW_BEGIN (n)
-  T acc = 0;
+  if (np >= 5) return f (IC<8> ());
+  if (np >= 3) return f (IC<4> ());
+  if (np == 2) return f (IC<2> ());
+  return f (IC<1> ());
+}
+
+template <typename TV, bool NEEDD, typename F>
+__device__ __forceinline__ void
+for_nbrs (int row, const int *__restrict__ rp, const int *__restrict__ ci,
+          const TV *__restrict__ v, const TV *__restrict__ dinv, int lane,
+          F &&f)
+{
   const int k0 = rp[row], k1 = rp[row + 1];
   for (int kb = k0; kb < k1; kb += 64) {
     const int kk = kb + lane;
     int c = 0;
-    TV w = 0;
-    if (kk < k1) { c = ci[kk]; w = v[kk]; }
-    const int cnt = min (64, k1 - kb);
-    int e = 0;
-    for (; e + 4 <= cnt; e += 4) {
-      const T x0 = X[off + (size_t) bcast (c, e) * 64];
-      const T x1 = X[off + (size_t) bcast (c, e + 1) * 64];
-      const T x2 = X[off + (size_t) bcast (c, e + 2) * 64];
-      const T x3 = X[off + (size_t) bcast (c, e + 3) * 64];
-      acc += (T) bcast (w, e) * x0;
-      acc += (T) bcast (w, e + 1) * x1;
-      acc += (T) bcast (w, e + 2) * x2;
-      acc += (T) bcast (w, e + 3) * x3;
+    TV w = 0, dd = 0;
+    if (kk < k1) {
+      c = ci[kk];
+      w = v[kk];
+      if (NEEDD) dd = dinv[c];
     }
-    for (; e < cnt; e++) acc += (T) bcast (w, e) * X[off + (size_t) bcast (c, e) * 64];
+    const int cnt = min (64, k1 - kb);
+    for (int e = 0; e < cnt; e++)
+      f (bcast (c, e), bcast (w, e), NEEDD ? bcast (dd, e) : (TV) 0);
   }
-  R[off + (size_t) row * 64] = B[off + (size_t) row * 64] - acc;
+}
+
+#define LGH_PANELS(NP)                                                        \
+  const int lane = threadIdx.x & 63;                                          \
+  const int p0 = blockIdx.y * (NP);                                           \
+  const int nph = min ((NP), npanels - p0);                                   \
+  const size_t ps = (size_t) n * 64;                                          \
+  const size_t base = (size_t) p0 * ps + lane;
+
+// R = B - A X
+template <typename TV, typename T, int NP>
+__global__ void __launch_bounds__ (TPB)
+resid64_k (int n, int npanels, const int *__restrict__ rp,
+           const int *__restrict__ ci, const TV *__restrict__ v,
+           const T *__restrict__ B, const T *__restrict__ X,
+           T *__restrict__ R)
+{
+  LGH_PANELS (NP)
+  LGH_ROW_BEGIN (n)
+  T acc[NP];
+#pragma unroll
+  for (int p = 0; p < NP; p++) acc[p] = 0;
+  for_nbrs<TV, false> (row, rp, ci, v, (const TV *) nullptr, lane,
+                       [&] (int c, TV w, TV) {
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+      if (p < nph) acc[p] += (T) w * X[base + p * ps + (size_t) c * 64];
+  });
+#pragma unroll
+  for (int p = 0; p < NP; p++)
+    if (p < nph) {
+      const size_t q = base + p * ps + (size_t) row * 64;
+      R[q] = B[q] - acc[p];
+    }
   LGH_ROW_END
 }
 
-// fused Chebyshev step at W = 64 (same arithmetic as cheb_k)
-template <typename TV, typename T, bool JAC, bool HASD>
+// fused Chebyshev step (zsolve_impl.h:104-141): Dn = a D + b Z with
+// Z = dinv R (JAC) or Zs; X (+)= Dn; Rn = R - A Dn (Dn recomputed at the
+// neighbours)
+template <typename TV, typename T, int NP, bool JAC, bool HASD>
 __global__ void __launch_bounds__ (TPB)
-cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
-          const TV *__restrict__ v, const TV *__restrict__ dinv,
-          const T *__restrict__ Zs, T a, T b, const T *__restrict__ D,
-          const T *__restrict__ R, T *__restrict__ X, int xmode,
-          T *__restrict__ Dn, T *__restrict__ Rn)
+cheb64_k (int n, int npanels, const int *__restrict__ rp,
+          const int *__restrict__ ci, const TV *__restrict__ v,
+          const TV *__restrict__ dinv, const T *__restrict__ Zs, T a, T b,
+          const T *__restrict__ D, const T *__restrict__ R,
+          T *__restrict__ X, int xmode, T *__restrict__ Dn,
+          T *__restrict__ Rn)
 {
-  const int lane = threadIdx.x & 63;
-  const size_t off = (size_t) blockIdx.y * n * 64 + lane;
-  auto dnew = [&] (int j, T dj) -> T {
-    const size_t q = off + (size_t) j * 64;
+  LGH_PANELS (NP)
+  auto dnew = [&] (size_t q, T dj) -> T {
     T z = JAC ? dj * R[q] : Zs[q];
     T bz = b * z;
     if (HASD) {
@@ -299,44 +345,228 @@ cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
     return bz;
   };
   LGH_ROW_BEGIN (n)
-  const T di = dnew (row, JAC ? (T) dinv[row] : (T) 0);
-  T acc = 0;
-  const int k0 = rp[row], k1 = rp[row + 1];
-  for (int kb = k0; kb < k1; kb += 64) {
-    const int kk = kb + lane;
-    int c = 0;
-    TV w = 0, dd = 0;
-    if (kk < k1) {
-      c = ci[kk];
-      w = v[kk];
-      if (JAC) dd = dinv[c];
+  T acc[NP];
+#pragma unroll
+  for (int p = 0; p < NP; p++) acc[p] = 0;
+  for_nbrs<TV, JAC> (row, rp, ci, v, dinv, lane, [&] (int c, TV w, TV dd) {
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+      if (p < nph) acc[p] += (T) w * dnew (base + p * ps + (size_t) c * 64, (T) dd);
+  });
+  const T dr = JAC ? (T) dinv[row] : (T) 0;
+#pragma unroll
+  for (int p = 0; p < NP; p++)
+    if (p < nph) {
+      const size_t qi = base + p * ps + (size_t) row * 64;
+      const T di = dnew (qi, dr);
+      Dn[qi] = di;
+      X[qi] = xmode ? X[qi] + di : di;
+      Rn[qi] = R[qi] - acc[p];
     }
-    const int cnt = min (64, k1 - kb);
-    int e = 0;
-    for (; e + 4 <= cnt; e += 4) {
-      const int c0 = bcast (c, e), c1 = bcast (c, e + 1);
-      const int c2 = bcast (c, e + 2), c3 = bcast (c, e + 3);
-      const T d0 = JAC ? (T) bcast (dd, e) : (T) 0;
-      const T d1 = JAC ? (T) bcast (dd, e + 1) : (T) 0;
-      const T d2 = JAC ? (T) bcast (dd, e + 2) : (T) 0;
-      const T d3 = JAC ? (T) bcast (dd, e + 3) : (T) 0;
-      const T y0 = dnew (c0, d0), y1 = dnew (c1, d1);
-      const T y2 = dnew (c2, d2), y3 = dnew (c3, d3);
-      acc += (T) bcast (w, e) * y0;
-      acc += (T) bcast (w, e + 1) * y1;
-      acc += (T) bcast (w, e + 2) * y2;
-      acc += (T) bcast (w, e + 3) * y3;
-    }
-    for (; e < cnt; e++) {
-      const T d0 = JAC ? (T) bcast (dd, e) : (T) 0;
-      acc += (T) bcast (w, e) * dnew (bcast (c, e), d0);
-    }
-  }
-  const size_t qi = off + (size_t) row * 64;
-  Dn[qi] = di;
-  X[qi] = xmode ? X[qi] + di : di;
-  Rn[qi] = R[qi] - acc;
   LGH_ROW_END
+}
+
+// mixed-precision fused Chebyshev step: FP64 residual R and accumulator X,
+// FP32 direction D and preconditioned residual Z.  The new direction is
+// rounded to FP32 once and that rounded value is what X and R both see,
+// so R stays the FP64 residual of X and the iteration reaches FP64
+// accuracy with an FP32 preconditioner.  Also writes R in FP32 for the
+// next preconditioner application.
+template <int NP, bool HASD>
+__global__ void __launch_bounds__ (TPB)
+cheb64m_k (int n, int npanels, const int *__restrict__ rp,
+           const int *__restrict__ ci, const double *__restrict__ v,
+           const float *__restrict__ Zs, double a, double b,
+           const float *__restrict__ D, const double *__restrict__ R,
+           double *__restrict__ X, int xmode, float *__restrict__ Dn,
+           double *__restrict__ Rn, float *__restrict__ Rn32)
+{
+  LGH_PANELS (NP)
+  auto dnew = [&] (size_t q) -> double {
+    double bz = b * (double) Zs[q];
+    double d = HASD ? a * (double) D[q] + bz : bz;
+    return (double) (float) d;
+  };
+  LGH_ROW_BEGIN (n)
+  double acc[NP];
+#pragma unroll
+  for (int p = 0; p < NP; p++) acc[p] = 0;
+  for_nbrs<double, false> (row, rp, ci, v, (const double *) nullptr, lane,
+                           [&] (int c, double w, double) {
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+      if (p < nph) acc[p] += w * dnew (base + p * ps + (size_t) c * 64);
+  });
+#pragma unroll
+  for (int p = 0; p < NP; p++)
+    if (p < nph) {
+      const size_t qi = base + p * ps + (size_t) row * 64;
+      const double di = dnew (qi);
+      Dn[qi] = (float) di;
+      X[qi] = xmode ? X[qi] + di : di;
+      const double rn = R[qi] - acc[p];
+      Rn[qi] = rn;
+      Rn32[qi] = (float) rn;
+    }
+  LGH_ROW_END
+}
+
+int
+k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
+                   const PMat *D32, const PMat &R, PMat &X, int x_mode,
+                   PMat &Dn32, PMat &Rn, PMat &Rn32)
+{
+  LGH_CHECK (A.prec == F64 && R.prec == F64 && X.prec == F64 && R.W == 64
+             && Z32.prec == F32 && Dn32.prec == F32 && Rn32.prec == F32
+             && Rn.prec == F64 && R.nrows == A.n, 8,
+             "k_cheb_step_mixed: shape/precision mismatch");
+  const double vec = (double) A.n * 64;
+  ProfScope ps (PK_CHEB_Z, R.npanels () * (csr_bytes (A)
+                + vec * (4 + (D32 ? 4 : 0) + 8 + (x_mode ? 8 : 0) + 4 + 8 + 8 + 4)),
+                R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * 64);
+  const int np = R.npanels ();
+  auto l = [&] (auto npt) -> int {
+    constexpr int NP = decltype (npt)::value;
+    dim3 g (grid_rows64 (A.n), (np + NP - 1) / NP);
+    if (D32)
+      hipLaunchKernelGGL ((cheb64m_k<NP, true>), g, dim3 (TPB), 0, stream (),
+                          A.n, np, A.rp, A.ci, (const double *) A.v,
+                          (const float *) Z32.p, a, b, (const float *) D32->p,
+                          (const double *) R.p, (double *) X.p, x_mode,
+                          (float *) Dn32.p, (double *) Rn.p, (float *) Rn32.p);
+    else
+      hipLaunchKernelGGL ((cheb64m_k<NP, false>), g, dim3 (TPB), 0, stream (),
+                          A.n, np, A.rp, A.ci, (const double *) A.v,
+                          (const float *) Z32.p, a, b, nullptr,
+                          (const double *) R.p, (double *) X.p, x_mode,
+                          (float *) Dn32.p, (double *) Rn.p, (float *) Rn32.p);
+    LGH_LAUNCH_CHECK ();
+    return 0;
+  };
+  return np_dispatch (np, l);
+}
+
+// degree-2 correction-form smoother (zsolve_impl.h:385-404 with nu = 2) in
+// two fused passes.  With X = X0 on entry, step 1 gives D1 = b1 dinv R,
+// R1 = R - A D1; step 2 gives D2 = a2 D1 + b2 dinv R1, R2 = R1 - A D2 and
+// the smoothed X = X0 + D1 + D2.  X is touched once (in step 2), D2 is
+// never stored, and R2 only when it is used (pre-smoothing: restriction).
+template <typename TV, typename T, int NP>
+__global__ void __launch_bounds__ (TPB)
+sm1_k (int n, int npanels, const int *__restrict__ rp,
+       const int *__restrict__ ci, const TV *__restrict__ v,
+       const TV *__restrict__ dinv, T b, const T *__restrict__ Rin,
+       T *__restrict__ D1, T *__restrict__ R1)
+{
+  LGH_PANELS (NP)
+  LGH_ROW_BEGIN (n)
+  T acc[NP];
+#pragma unroll
+  for (int p = 0; p < NP; p++) acc[p] = 0;
+  for_nbrs<TV, true> (row, rp, ci, v, dinv, lane, [&] (int c, TV w, TV dd) {
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+      if (p < nph)
+        acc[p] += (T) w * (b * ((T) dd * Rin[base + p * ps + (size_t) c * 64]));
+  });
+  const T dr = (T) dinv[row];
+#pragma unroll
+  for (int p = 0; p < NP; p++)
+    if (p < nph) {
+      const size_t qi = base + p * ps + (size_t) row * 64;
+      const T rin = Rin[qi];
+      D1[qi] = b * (dr * rin);
+      R1[qi] = rin - acc[p];
+    }
+  LGH_ROW_END
+}
+
+template <typename TV, typename T, int NP, bool XADD, bool WR>
+__global__ void __launch_bounds__ (TPB)
+sm2_k (int n, int npanels, const int *__restrict__ rp,
+       const int *__restrict__ ci, const TV *__restrict__ v,
+       const TV *__restrict__ dinv, T a, T b, const T *__restrict__ D1,
+       const T *__restrict__ R1, T *__restrict__ X, T *__restrict__ R2)
+{
+  LGH_PANELS (NP)
+  auto d2 = [&] (size_t q, T dj) -> T {
+    T bz = b * (dj * R1[q]);
+    T ad = a * D1[q];
+    return ad + bz;
+  };
+  LGH_ROW_BEGIN (n)
+  const T dr = (T) dinv[row];
+#pragma unroll
+  for (int p = 0; p < NP; p++)
+    if (p < nph) {
+      const size_t qi = base + p * ps + (size_t) row * 64;
+      const T di = d2 (qi, dr);
+      const T d1 = D1[qi];
+      X[qi] = XADD ? X[qi] + d1 + di : d1 + di;
+    }
+  if (WR) {
+    T acc[NP];
+#pragma unroll
+    for (int p = 0; p < NP; p++) acc[p] = 0;
+    for_nbrs<TV, true> (row, rp, ci, v, dinv, lane, [&] (int c, TV w, TV dd) {
+#pragma unroll
+      for (int p = 0; p < NP; p++)
+        if (p < nph) acc[p] += (T) w * d2 (base + p * ps + (size_t) c * 64, (T) dd);
+    });
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+      if (p < nph) {
+        const size_t qi = base + p * ps + (size_t) row * 64;
+        R2[qi] = R1[qi] - acc[p];
+      }
+  }
+  LGH_ROW_END
+}
+
+template <typename TV, typename T>
+static int
+smooth2_t (const DevCSR &A, double b1, double a2, double b2, const PMat &Rin,
+           PMat &D1, PMat &R1, PMat &X, int xadd, PMat *R2)
+{
+  const int np = Rin.npanels ();
+  auto l = [&] (auto npt) -> int {
+    constexpr int NP = decltype (npt)::value;
+    dim3 g (grid_rows64 (A.n), (np + NP - 1) / NP);
+    hipLaunchKernelGGL ((sm1_k<TV, T, NP>), g, dim3 (TPB), 0, stream (), A.n,
+                        np, A.rp, A.ci, (const TV *) A.v, (const TV *) A.dinv,
+                        (T) b1, (const T *) Rin.p, (T *) D1.p, (T *) R1.p);
+    LGH_LAUNCH_CHECK ();
+    T *r2 = R2 ? (T *) R2->p : nullptr;
+#define LGH_SM2(XA, W_)                                                       \
+    hipLaunchKernelGGL ((sm2_k<TV, T, NP, XA, W_>), g, dim3 (TPB), 0,         \
+                        stream (), A.n, np, A.rp, A.ci, (const TV *) A.v,      \
+                        (const TV *) A.dinv, (T) a2, (T) b2, (const T *) D1.p, \
+                        (const T *) R1.p, (T *) X.p, r2)
+    if (xadd) { if (R2) LGH_SM2 (true, true); else LGH_SM2 (true, false); }
+    else { if (R2) LGH_SM2 (false, true); else LGH_SM2 (false, false); }
+#undef LGH_SM2
+    LGH_LAUNCH_CHECK ();
+    return 0;
+  };
+  return np_dispatch (np, l);
+}
+
+int
+k_smooth2 (const DevCSR &A, double b1, double a2, double b2, const PMat &Rin,
+           PMat &D1, PMat &R1, PMat &X, int xadd, PMat *R2)
+{
+  LGH_CHECK (Rin.W == 64 && X.W == 64 && D1.W == 64 && R1.W == 64
+             && Rin.prec == X.prec && A.dinv && Rin.nrows == A.n, 8,
+             "k_smooth2: shape mismatch");
+  const double vec = (double) A.n * 64 * prec_bytes (Rin.prec);
+  const double streams = 3 + (2 + (xadd ? 1 : 0) + 1 + (R2 ? 1 : 0));
+  ProfScope ps (PK_CHEB_JAC, Rin.npanels () * (2 * csr_bytes (A) + streams * vec),
+                Rin.npanels () * (4.0 * (double) A.nnz + 10.0 * A.n) * 64);
+  if (Rin.prec == F64)
+    return A.prec == F64 ? smooth2_t<double, double> (A, b1, a2, b2, Rin, D1, R1, X, xadd, R2)
+                         : smooth2_t<float, double> (A, b1, a2, b2, Rin, D1, R1, X, xadd, R2);
+  return A.prec == F64 ? smooth2_t<double, float> (A, b1, a2, b2, Rin, D1, R1, X, xadd, R2)
+                       : smooth2_t<float, float> (A, b1, a2, b2, Rin, D1, R1, X, xadd, R2);
 }
 
 // ---------------------------------------------------------------------------
@@ -370,20 +600,47 @@ spmm_t (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
   if (X.W == 64) {
     const int np = X.npanels ();
     const unsigned gx = grid_rows64 (A.n);
+    auto launch = [&] (int NPsel) -> int {
 #define LGH_SP64(NP)                                                          \
-    hipLaunchKernelGGL ((spmm64_k<TV, T, NP>), dim3 (gx, (np + NP - 1) / NP), \
-                        dim3 (TPB), 0, stream (), A.n, A.m, A.rp, A.ci,        \
-                        (const TV *) A.v, (T) alpha, (const T *) X.p,          \
-                        (T) beta, (T *) Y.p, np)
-    if (np >= 8) LGH_SP64 (8);
-    else if (np == 4) LGH_SP64 (4);
-    else if (np > 4) LGH_SP64 (8);
-    else if (np == 3) LGH_SP64 (4);
-    else if (np == 2) LGH_SP64 (2);
-    else LGH_SP64 (1);
+      hipLaunchKernelGGL ((spmm64_k<TV, T, NP>), dim3 (gx, (np + NP - 1) / NP), \
+                          dim3 (TPB), 0, stream (), A.n, A.m, A.rp, A.ci,      \
+                          (const TV *) A.v, (T) alpha, (const T *) X.p,        \
+                          (T) beta, (T *) Y.p, np)
+      switch (NPsel) {
+      case 8: LGH_SP64 (8); break;
+      case 4: LGH_SP64 (4); break;
+      case 2: LGH_SP64 (2); break;
+      default: LGH_SP64 (1); break;
+      }
 #undef LGH_SP64
-    LGH_LAUNCH_CHECK ();
-    return 0;
+      LGH_LAUNCH_CHECK ();
+      return 0;
+    };
+    const int vp = vt<T>::prec;
+    int &tuned = const_cast<DevCSR &> (A).spmm_np[vp];
+    const int widest = np >= 5 ? 8 : np >= 3 ? 4 : np == 2 ? 2 : 1;
+    if (tuned == 0 && np >= 2 && beta == 0.0 && A.nnz >= 16 * (int64_t) A.n) {
+      // autotune once per (matrix, vector precision): panels per block trade
+      // CSR re-reads (fewer panels) against gather working set (more panels)
+      hipEvent_t e0, e1;
+      LGH_HIP (hipEventCreate (&e0));
+      LGH_HIP (hipEventCreate (&e1));
+      float best = 1e30f;
+      int best_np = widest;
+      for (int cand = 1; cand <= widest; cand *= 2) {
+        LGH_HIP (hipEventRecord (e0, stream ()));
+        LGH_TRY (launch (cand));
+        LGH_HIP (hipEventRecord (e1, stream ()));
+        LGH_HIP (hipEventSynchronize (e1));
+        float ms = 0.f;
+        LGH_HIP (hipEventElapsedTime (&ms, e0, e1));
+        if (ms < best) { best = ms; best_np = cand; }
+      }
+      (void) hipEventDestroy (e0);
+      (void) hipEventDestroy (e1);
+      tuned = best_np;
+    }
+    return launch (tuned ? tuned : widest);
   }
   auto l = [&] (auto wt) -> int {
       constexpr int W = decltype (wt)::value;
@@ -396,12 +653,6 @@ spmm_t (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
       return 0;
   };
   return with_W (X.W, l);
-}
-
-static double
-csr_bytes (const DevCSR &A)
-{
-  return (double) A.nnz * (prec_bytes (A.prec) + 4) + 4.0 * (A.n + 1);
 }
 
 int
@@ -448,12 +699,17 @@ static int
 resid_t (const DevCSR &A, const PMat &B, const PMat &X, PMat &R)
 {
   if (X.W == 64) {
-    dim3 g (grid_rows64 (A.n), X.npanels ());
-    hipLaunchKernelGGL ((resid64_k<TV, T>), g, dim3 (TPB), 0, stream (), A.n,
-                        A.rp, A.ci, (const TV *) A.v, (const T *) B.p,
-                        (const T *) X.p, (T *) R.p);
-    LGH_LAUNCH_CHECK ();
-    return 0;
+    const int np = X.npanels ();
+    auto l64 = [&] (auto npt) -> int {
+      constexpr int NP = decltype (npt)::value;
+      dim3 g (grid_rows64 (A.n), (np + NP - 1) / NP);
+      hipLaunchKernelGGL ((resid64_k<TV, T, NP>), g, dim3 (TPB), 0, stream (),
+                          A.n, np, A.rp, A.ci, (const TV *) A.v,
+                          (const T *) B.p, (const T *) X.p, (T *) R.p);
+      LGH_LAUNCH_CHECK ();
+      return 0;
+    };
+    return np_dispatch (np, l64);
   }
   auto l = [&] (auto wt) -> int {
       constexpr int W = decltype (wt)::value;
@@ -530,20 +786,25 @@ cheb_t (const DevCSR &A, int jac, const PMat *Zs, double a, double b,
         const PMat *D, const PMat &R, PMat &X, int xmode, PMat &Dn, PMat &Rn)
 {
   if (R.W == 64) {
-    dim3 g (grid_rows64 (A.n), R.npanels ());
+    const int np = R.npanels ();
     const T *zp = Zs ? (const T *) Zs->p : nullptr;
     const T *dp = D ? (const T *) D->p : nullptr;
+    auto l64 = [&] (auto npt) -> int {
+      constexpr int NP = decltype (npt)::value;
+      dim3 g (grid_rows64 (A.n), (np + NP - 1) / NP);
 #define LGH_CH64(J, H)                                                        \
-    hipLaunchKernelGGL ((cheb64_k<TV, T, J, H>), g, dim3 (TPB), 0, stream (), \
-                        A.n, A.rp, A.ci, (const TV *) A.v,                     \
-                        (const TV *) A.dinv, zp, (T) a, (T) b, dp,             \
-                        (const T *) R.p, (T *) X.p, xmode, (T *) Dn.p,         \
-                        (T *) Rn.p)
-    if (jac) { if (dp) LGH_CH64 (true, true); else LGH_CH64 (true, false); }
-    else { if (dp) LGH_CH64 (false, true); else LGH_CH64 (false, false); }
+      hipLaunchKernelGGL ((cheb64_k<TV, T, NP, J, H>), g, dim3 (TPB), 0,     \
+                          stream (), A.n, np, A.rp, A.ci, (const TV *) A.v,    \
+                          (const TV *) A.dinv, zp, (T) a, (T) b, dp,           \
+                          (const T *) R.p, (T *) X.p, xmode, (T *) Dn.p,       \
+                          (T *) Rn.p)
+      if (jac) { if (dp) LGH_CH64 (true, true); else LGH_CH64 (true, false); }
+      else { if (dp) LGH_CH64 (false, true); else LGH_CH64 (false, false); }
 #undef LGH_CH64
-    LGH_LAUNCH_CHECK ();
-    return 0;
+      LGH_LAUNCH_CHECK ();
+      return 0;
+    };
+    return np_dispatch (np, l64);
   }
   auto l = [&] (auto wt) -> int {
       constexpr int W = decltype (wt)::value;
@@ -591,225 +852,7 @@ k_cheb_step (const DevCSR &A, int jacobi, const PMat *Zsrc, double a,
 }
 
 
-// mixed-precision fused Chebyshev step (W = 64): FP64 residual R and
-// accumulator X, FP32 direction D and preconditioned residual Z.  The new
-// direction is rounded to FP32 once and that rounded value is what X and
-// R both see, so R stays the FP64 residual of X and the iteration reaches
-// FP64 accuracy with an FP32 preconditioner.  Also writes R in FP32 for
-// the next preconditioner application.
-template <bool HASD>
-__global__ void __launch_bounds__ (TPB)
-cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
-           const double *__restrict__ v, const float *__restrict__ Zs,
-           double a, double b, const float *__restrict__ D,
-           const double *__restrict__ R, double *__restrict__ X, int xmode,
-           float *__restrict__ Dn, double *__restrict__ Rn,
-           float *__restrict__ Rn32)
-{
-  const int lane = threadIdx.x & 63;
-  const size_t off = (size_t) blockIdx.y * n * 64 + lane;
-  auto dnew = [&] (int j) -> double {
-    const size_t q = off + (size_t) j * 64;
-    double bz = b * (double) Zs[q];
-    double d = HASD ? a * (double) D[q] + bz : bz;
-    return (double) (float) d;
-  };
-  LGH_ROW_BEGIN (n)
-  const double di = dnew (row);
-  double acc = 0;
-  const int k0 = rp[row], k1 = rp[row + 1];
-  for (int kb = k0; kb < k1; kb += 64) {
-    const int kk = kb + lane;
-    int c = 0;
-    double w = 0;
-    if (kk < k1) { c = ci[kk]; w = v[kk]; }
-    const int cnt = min (64, k1 - kb);
-    int e = 0;
-    for (; e + 4 <= cnt; e += 4) {
-      const double y0 = dnew (bcast (c, e)), y1 = dnew (bcast (c, e + 1));
-      const double y2 = dnew (bcast (c, e + 2)), y3 = dnew (bcast (c, e + 3));
-      acc += bcast (w, e) * y0;
-      acc += bcast (w, e + 1) * y1;
-      acc += bcast (w, e + 2) * y2;
-      acc += bcast (w, e + 3) * y3;
-    }
-    for (; e < cnt; e++) acc += bcast (w, e) * dnew (bcast (c, e));
-  }
-  const size_t qi = off + (size_t) row * 64;
-  Dn[qi] = (float) di;
-  X[qi] = xmode ? X[qi] + di : di;
-  const double rn = R[qi] - acc;
-  Rn[qi] = rn;
-  Rn32[qi] = (float) rn;
-  LGH_ROW_END
-}
 
-int
-k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
-                   const PMat *D32, const PMat &R, PMat &X, int x_mode,
-                   PMat &Dn32, PMat &Rn, PMat &Rn32)
-{
-  LGH_CHECK (A.prec == F64 && R.prec == F64 && X.prec == F64 && R.W == 64
-             && Z32.prec == F32 && Dn32.prec == F32 && Rn32.prec == F32
-             && Rn.prec == F64 && R.nrows == A.n, 8,
-             "k_cheb_step_mixed: shape/precision mismatch");
-  const double vec = (double) A.n * 64;
-  ProfScope ps (PK_CHEB_Z, R.npanels () * (csr_bytes (A)
-                + vec * (4 + (D32 ? 4 : 0) + 8 + (x_mode ? 8 : 0) + 4 + 8 + 8 + 4)),
-                R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * 64);
-  dim3 g (grid_rows64 (A.n), R.npanels ());
-  if (D32)
-    hipLaunchKernelGGL (cheb64m_k<true>, g, dim3 (TPB), 0, stream (), A.n,
-                        A.rp, A.ci, (const double *) A.v, (const float *) Z32.p,
-                        a, b, (const float *) D32->p, (const double *) R.p,
-                        (double *) X.p, x_mode, (float *) Dn32.p,
-                        (double *) Rn.p, (float *) Rn32.p);
-  else
-    hipLaunchKernelGGL (cheb64m_k<false>, g, dim3 (TPB), 0, stream (), A.n,
-                        A.rp, A.ci, (const double *) A.v, (const float *) Z32.p,
-                        a, b, nullptr, (const double *) R.p, (double *) X.p,
-                        x_mode, (float *) Dn32.p, (double *) Rn.p,
-                        (float *) Rn32.p);
-  LGH_LAUNCH_CHECK ();
-  return 0;
-}
-
-// ---------------------------------------------------------------------------
-// degree-2 correction-form smoother (zsolve_impl.h:385-404 with nu = 2) as
-// two fused passes.  With X = X0 on entry, step 1 gives D1 = b1 dinv R,
-// R1 = R - A D1; step 2 gives D2 = a2 D1 + b2 dinv R1, R2 = R1 - A D2 and
-// the smoothed X = X0 + D1 + D2.  X is touched once (in step 2), D2 is
-// never stored, and R2 only when it is used (pre-smoothing: restriction).
-
-// step 1: D1 = b dinv Rin ; R1 = Rin - A D1
-template <typename TV, typename T>
-__global__ void __launch_bounds__ (TPB)
-sm1_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
-       const TV *__restrict__ v, const TV *__restrict__ dinv, T b,
-       const T *__restrict__ Rin, T *__restrict__ D1, T *__restrict__ R1)
-{
-  const int lane = threadIdx.x & 63;
-  const size_t off = (size_t) blockIdx.y * n * 64 + lane;
-  LGH_ROW_BEGIN (n)
-  const T di = b * ((T) dinv[row] * Rin[off + (size_t) row * 64]);
-  T acc = 0;
-  const int k0 = rp[row], k1 = rp[row + 1];
-  for (int kb = k0; kb < k1; kb += 64) {
-    const int kk = kb + lane;
-    int c = 0;
-    TV w = 0, dd = 0;
-    if (kk < k1) { c = ci[kk]; w = v[kk]; dd = dinv[c]; }
-    const int cnt = min (64, k1 - kb);
-    int e = 0;
-    for (; e + 4 <= cnt; e += 4) {
-      const T y0 = b * ((T) bcast (dd, e) * Rin[off + (size_t) bcast (c, e) * 64]);
-      const T y1 = b * ((T) bcast (dd, e + 1) * Rin[off + (size_t) bcast (c, e + 1) * 64]);
-      const T y2 = b * ((T) bcast (dd, e + 2) * Rin[off + (size_t) bcast (c, e + 2) * 64]);
-      const T y3 = b * ((T) bcast (dd, e + 3) * Rin[off + (size_t) bcast (c, e + 3) * 64]);
-      acc += (T) bcast (w, e) * y0;
-      acc += (T) bcast (w, e + 1) * y1;
-      acc += (T) bcast (w, e + 2) * y2;
-      acc += (T) bcast (w, e + 3) * y3;
-    }
-    for (; e < cnt; e++)
-      acc += (T) bcast (w, e)
-             * (b * ((T) bcast (dd, e) * Rin[off + (size_t) bcast (c, e) * 64]));
-  }
-  const size_t qi = off + (size_t) row * 64;
-  D1[qi] = di;
-  R1[qi] = Rin[qi] - acc;
-  LGH_ROW_END
-}
-
-// step 2: D2 = a D1 + b dinv R1 ; X = (XADD ? X : 0) + D1 + D2 ;
-// R2 = R1 - A D2 when WR
-template <typename TV, typename T, bool XADD, bool WR>
-__global__ void __launch_bounds__ (TPB)
-sm2_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
-       const TV *__restrict__ v, const TV *__restrict__ dinv, T a, T b,
-       const T *__restrict__ D1, const T *__restrict__ R1,
-       T *__restrict__ X, T *__restrict__ R2)
-{
-  const int lane = threadIdx.x & 63;
-  const size_t off = (size_t) blockIdx.y * n * 64 + lane;
-  auto d2 = [&] (int j, T dj) -> T {
-    const size_t q = off + (size_t) j * 64;
-    T bz = b * (dj * R1[q]);
-    T ad = a * D1[q];
-    return ad + bz;
-  };
-  LGH_ROW_BEGIN (n)
-  const size_t qi = off + (size_t) row * 64;
-  const T di = d2 (row, (T) dinv[row]);
-  const T d1 = D1[qi];
-  X[qi] = XADD ? X[qi] + d1 + di : d1 + di;
-  if (WR) {
-    T acc = 0;
-    const int k0 = rp[row], k1 = rp[row + 1];
-    for (int kb = k0; kb < k1; kb += 64) {
-      const int kk = kb + lane;
-      int c = 0;
-      TV w = 0, dd = 0;
-      if (kk < k1) { c = ci[kk]; w = v[kk]; dd = dinv[c]; }
-      const int cnt = min (64, k1 - kb);
-      int e = 0;
-      for (; e + 4 <= cnt; e += 4) {
-        const T y0 = d2 (bcast (c, e), (T) bcast (dd, e));
-        const T y1 = d2 (bcast (c, e + 1), (T) bcast (dd, e + 1));
-        const T y2 = d2 (bcast (c, e + 2), (T) bcast (dd, e + 2));
-        const T y3 = d2 (bcast (c, e + 3), (T) bcast (dd, e + 3));
-        acc += (T) bcast (w, e) * y0;
-        acc += (T) bcast (w, e + 1) * y1;
-        acc += (T) bcast (w, e + 2) * y2;
-        acc += (T) bcast (w, e + 3) * y3;
-      }
-      for (; e < cnt; e++) acc += (T) bcast (w, e) * d2 (bcast (c, e), (T) bcast (dd, e));
-    }
-    R2[qi] = R1[qi] - acc;
-  }
-  LGH_ROW_END
-}
-
-template <typename TV, typename T>
-static int
-smooth2_t (const DevCSR &A, double b1, double a2, double b2, const PMat &Rin,
-           PMat &D1, PMat &R1, PMat &X, int xadd, PMat *R2)
-{
-  dim3 g (grid_rows64 (A.n), Rin.npanels ());
-  hipLaunchKernelGGL ((sm1_k<TV, T>), g, dim3 (TPB), 0, stream (), A.n, A.rp,
-                      A.ci, (const TV *) A.v, (const TV *) A.dinv, (T) b1,
-                      (const T *) Rin.p, (T *) D1.p, (T *) R1.p);
-  LGH_LAUNCH_CHECK ();
-  T *r2 = R2 ? (T *) R2->p : nullptr;
-#define LGH_SM2(XA, W_)                                                       \
-  hipLaunchKernelGGL ((sm2_k<TV, T, XA, W_>), g, dim3 (TPB), 0, stream (),    \
-                      A.n, A.rp, A.ci, (const TV *) A.v, (const TV *) A.dinv, \
-                      (T) a2, (T) b2, (const T *) D1.p, (const T *) R1.p,      \
-                      (T *) X.p, r2)
-  if (xadd) { if (R2) LGH_SM2 (true, true); else LGH_SM2 (true, false); }
-  else { if (R2) LGH_SM2 (false, true); else LGH_SM2 (false, false); }
-#undef LGH_SM2
-  LGH_LAUNCH_CHECK ();
-  return 0;
-}
-
-int
-k_smooth2 (const DevCSR &A, double b1, double a2, double b2, const PMat &Rin,
-           PMat &D1, PMat &R1, PMat &X, int xadd, PMat *R2)
-{
-  LGH_CHECK (Rin.W == 64 && X.W == 64 && D1.W == 64 && R1.W == 64
-             && Rin.prec == X.prec && A.dinv && Rin.nrows == A.n, 8,
-             "k_smooth2: shape mismatch");
-  const double vec = (double) A.n * 64 * prec_bytes (Rin.prec);
-  const double streams = 3 + (2 + (xadd ? 1 : 0) + 1 + (R2 ? 1 : 0));
-  ProfScope ps (PK_CHEB_JAC, Rin.npanels () * (2 * csr_bytes (A) + streams * vec),
-                Rin.npanels () * (4.0 * (double) A.nnz + 10.0 * A.n) * 64);
-  if (Rin.prec == F64)
-    return A.prec == F64 ? smooth2_t<double, double> (A, b1, a2, b2, Rin, D1, R1, X, xadd, R2)
-                         : smooth2_t<float, double> (A, b1, a2, b2, Rin, D1, R1, X, xadd, R2);
-  return A.prec == F64 ? smooth2_t<double, float> (A, b1, a2, b2, Rin, D1, R1, X, xadd, R2)
-                       : smooth2_t<float, float> (A, b1, a2, b2, Rin, D1, R1, X, xadd, R2);
-}
 
 // ---------------------------------------------------------------------------
 // dense coarse apply X0 = Ainv B0 (Ainv row-major n0 x n0)

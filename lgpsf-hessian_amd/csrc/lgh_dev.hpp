@@ -91,6 +91,8 @@ struct DevCSR {
   void *v = nullptr;              // values, precision `prec`
   int prec = F64;
   void *dinv = nullptr;           // 1/diag (same precision), square only
+  int spmm_np[2] = { 0, 0 };      // autotuned panels per block (W = 64),
+                                  // per vector precision; 0 = not tuned
   ~DevCSR ();
 };
 // upload host CSR (values fp64 on the host) at precision prec
