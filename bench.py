@@ -27,6 +27,11 @@ sys.path.insert(0, PKG_DIR)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 FP32_PEAK_TFS = 157.3
+# profiler kind -> the kernel instantiation it times (rocprofv3 name)
+KERNEL_OF = {"cheb_mixed": "lgh::cheb64m_k<true>",
+             "smooth_pre_fine": "lgh::smx_k<float, float, false>",
+             "smooth_post_fine": "lgh::smx_k<float, float, true>",
+             "resid_fine": "lgh::resid64_k<float, float>"}
 
 
 def load_pkg():
@@ -37,6 +42,23 @@ def load_pkg():
     sys.modules["lgpsf_hessian_amd"] = mod
     spec.loader.exec_module(mod)
     return mod
+
+
+def pmc_traffic(kernel, panels):
+    """HBM bytes per launch of `kernel` at `panels` panels from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel
+    calibration recorded there), or None when it holds no such line."""
+    path = os.path.join(ROOT, "profiles", "cfg2_build_pmc_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                t = line.split()
+                if (len(t) >= 4 and t[0] == "traffic_bytes_per_launch"
+                        and " ".join(t[1:-2]) == kernel and int(t[-2]) == panels):
+                    return float(t[-1])
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -68,7 +90,19 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a 1 GiB device-to-device copy three times (the "
+                         "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calib:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("copied 3 x %d bytes" % (x.numel() * 4))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU "
               "path); use --gpus 1", file=sys.stderr)
@@ -110,28 +144,31 @@ def main():
     lgh.lib().lgh_hip_sync()
     t_mixed = (time.perf_counter() - t0) / args.steps
     lgh.prof_enable(False)
-    prof = {k: lgh.prof_collect(k) for k in lgh.PROF_KINDS}
+    # one kernel instantiation at one launch shape per kind below: the
+    # full-width (all ell columns) launches on the finest level
+    npan = (args.ell + 63) // 64
+    single = ("cheb_mixed", "smooth_pre_fine", "smooth_post_fine", "resid_fine")
+    prof = {k: lgh.prof_collect(k, npan if k in single else -1)
+            for k in lgh.PROF_KINDS}
     log("mixed build %.3fs kept=%d report=%s" % (t_mixed, g.kept, reps[-1]))
     for k, (c, ms, by, fl) in prof.items():
         if c:
-            log("  %-12s launches %6d  %9.2f ms  avg %.3f ms  %7.1f GB/s  %6.2f TF/s"
+            log("  %-16s launches %6d  %9.2f ms  avg %.3f ms  %7.1f GB/s  %6.2f TF/s"
                 % (k, c, ms, ms / c, by / ms / 1e6, fl / ms / 1e9))
 
-    # dominant kernel (by time) -> roofline
-    dom = max(prof, key=lambda k: prof[k][1])
+    # dominant single kernel (by total time) -> roofline
+    dom = max(single, key=lambda k: prof[k][1])
     c, ms, by, fl = prof[dom]
     avg_ms = ms / c
-    if dom in ("gram", "gemm"):
-        ach = fl / c / (avg_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFS,
-                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFS, "traffic": None}
-    else:
-        ach = by / c / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
-    roof["kernel"] = dom
-    roof["avg_launch_ms"] = avg_ms
-    roof["launches"] = c
+    ach = by / c / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+            "kernel": KERNEL_OF[dom], "panels": npan, "avg_launch_ms": avg_ms,
+            "launches": c, "algorithmic_bytes_per_launch": by / c}
+    tr = pmc_traffic(KERNEL_OF[dom], npan)
+    if tr is not None:
+        roof["traffic"] = tr
+        roof["traffic_source"] = "profiles/cfg2_build_pmc_traffic.md"
     sp = prof["spmm_B"]
     spmm_roof = None
     if sp[0]:
@@ -139,6 +176,13 @@ def main():
         spmm_roof = {"kernel": "spmm_B", "achieved": a, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": a / HBM_PEAK_GBS,
                      "avg_launch_ms": sp[1] / sp[0]}
+    mf = {}
+    for k in ("gram", "gemm"):
+        c_, ms_, by_, fl_ = prof[k]
+        if c_:
+            mf[k] = {"achieved_tflops": fl_ / ms_ / 1e9, "peak_fp64_tflops": FP64_PEAK_TFS,
+                     "frac": fl_ / ms_ / 1e9 / FP64_PEAK_TFS}
+    peak_gb = lgh.lib().lgh_hip_bytes_in_use() / 1e9
 
     extra = {}
     if not args.no_plain:
@@ -223,6 +267,8 @@ def main():
                    "n": n, "ell": args.ell, "kept": g.kept},
         "roofline": roof,
         "spmm_roofline": spmm_roof,
+        "mfma": mf,
+        "device_bytes_in_use_gb": peak_gb,
         "cpu_baseline": cpu,
     }
     line.update(extra)

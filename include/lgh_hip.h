@@ -34,11 +34,15 @@ void        lgh_hip_trim (void);
 /* live kernel timing: when enabled, every launch of the engine is bracketed
  * by HIP events on the engine stream and credited with its algorithmic
  * HBM bytes and flops.  kind: 0 B SpMM of the F sweep, 1 other SpMM,
- * 2 fused Chebyshev step (Jacobi), 3 fused Chebyshev step (explicit Z),
- * 4 residual, 5 Gram (MFMA), 6 GEMM (MFMA).  collect synchronizes and
- * resets that kind's totals.  (No reference counterpart: measurement.) */
+ * 2 Jacobi Chebyshev steps and smoothers below the finest AMG level,
+ * 3 fused Chebyshev step (explicit Z), 4 residual below the finest level,
+ * 5 Gram (MFMA), 6 GEMM (MFMA), 7 other, 8 mixed-precision Chebyshev step,
+ * 9 / 10 fused smoother on the finest level (pre: zero guess / post),
+ * 11 residual on the finest level.  collect synchronizes and resets that
+ * kind's totals; np >= 0 counts only launches over np 64-column panels.
+ * (No reference counterpart: measurement.) */
 void lgh_hip_prof_enable (int on);
-int  lgh_hip_prof_collect (int kind, int64_t *count, double *ms,
+int  lgh_hip_prof_collect (int kind, int np, int64_t *count, double *ms,
                            double *bytes, double *flops);
 
 /* the device hashed-Gaussian arithmetic evaluated on the host (no GPU) */

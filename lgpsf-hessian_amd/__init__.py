@@ -157,7 +157,7 @@ _SIGS = {
     "lgh_hip_bytes_in_use": (C.c_int64, []),
     "lgh_hip_trim": (None, []),
     "lgh_hip_prof_enable": (None, [C.c_int]),
-    "lgh_hip_prof_collect": (C.c_int, [C.c_int, C.POINTER(C.c_int64), dp, dp, dp]),
+    "lgh_hip_prof_collect": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int64), dp, dp, dp]),
     "lgh_hip_randn_at_host": (C.c_double, [C.c_uint64, C.c_int64, C.c_int64]),
     "lgh_hip_randn_fill": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.c_int,
                                      C.c_int, C.c_int, C.c_int, dp, C.c_int]),
@@ -477,18 +477,20 @@ def init():
 
 
 PROF_KINDS = {"spmm_B": 0, "spmm": 1, "cheb_jacobi": 2, "cheb_z": 3,
-              "resid": 4, "gram": 5, "gemm": 6}
+              "resid": 4, "gram": 5, "gemm": 6, "other": 7, "cheb_mixed": 8,
+              "smooth_pre_fine": 9, "smooth_post_fine": 10, "resid_fine": 11}
 
 
 def prof_enable(on=True):
     lib().lgh_hip_prof_enable(1 if on else 0)
 
 
-def prof_collect(kind):
-    """(launches, total ms, algorithmic bytes, flops) since the last collect"""
+def prof_collect(kind, np_=-1):
+    """(launches, total ms, algorithmic bytes, flops) since the last collect
+    of that kind; np_ >= 0 counts only launches over np_ panels."""
     c = C.c_int64()
     ms, by, fl = C.c_double(), C.c_double(), C.c_double()
-    _hip_check(lib().lgh_hip_prof_collect(PROF_KINDS.get(kind, kind), C.byref(c),
+    _hip_check(lib().lgh_hip_prof_collect(PROF_KINDS.get(kind, kind), np_, C.byref(c),
                                           C.byref(ms), C.byref(by), C.byref(fl)),
                "lgh_hip_prof_collect")
     return c.value, ms.value, by.value, fl.value

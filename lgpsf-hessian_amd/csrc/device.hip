@@ -64,9 +64,9 @@ dev_sync ()
 }
 
 // ---- live kernel timing -----------------------------------------------------
-struct ProfRec { int kind; hipEvent_t a, b; double bytes, flops; };
+struct ProfRec { int kind, np; hipEvent_t a, b; double bytes, flops; };
 static int g_prof = 0;
-static thread_local int g_btag = 0;
+static thread_local int g_btag = 0, g_ftag = 0;
 static std::vector<ProfRec> g_recs;
 static std::vector<hipEvent_t> g_evpool;
 
@@ -83,9 +83,11 @@ void prof_enable (int on) { g_prof = on; }
 int prof_enabled () { return g_prof; }
 void prof_set_b_tag (int on) { g_btag = on; }
 int prof_b_tag () { return g_btag; }
+void prof_set_fine_tag (int on) { g_ftag = on; }
+int prof_fine_tag () { return g_ftag; }
 
-ProfScope::ProfScope (int k, double by, double fl)
-  : kind (k), bytes (by), flops (fl), active (g_prof), ev0 (nullptr)
+ProfScope::ProfScope (int k, double by, double fl, int np_)
+  : kind (k), np (np_), bytes (by), flops (fl), active (g_prof), ev0 (nullptr)
 {
   if (!active) return;
   hipEvent_t a = ev_get ();
@@ -98,11 +100,12 @@ ProfScope::~ProfScope ()
   if (!active) return;
   hipEvent_t b = ev_get ();
   (void) hipEventRecord (b, g_stream);
-  g_recs.push_back ({ kind, (hipEvent_t) ev0, b, bytes, flops });
+  g_recs.push_back ({ kind, np, (hipEvent_t) ev0, b, bytes, flops });
 }
 
 int
-prof_collect (int kind, long *count, double *ms, double *bytes, double *flops)
+prof_collect (int kind, int np, long *count, double *ms, double *bytes,
+              double *flops)
 {
   LGH_HIP (hipStreamSynchronize (g_stream));
   long c = 0;
@@ -110,9 +113,11 @@ prof_collect (int kind, long *count, double *ms, double *bytes, double *flops)
   std::vector<ProfRec> keep;
   for (auto &r : g_recs) {
     if (r.kind != kind) { keep.push_back (r); continue; }
-    float e = 0.f;
-    LGH_HIP (hipEventElapsedTime (&e, r.a, r.b));
-    c++; t += e; by += r.bytes; fl += r.flops;
+    if (np < 0 || r.np == np) {
+      float e = 0.f;
+      LGH_HIP (hipEventElapsedTime (&e, r.a, r.b));
+      c++; t += e; by += r.bytes; fl += r.flops;
+    }
     g_evpool.push_back (r.a);
     g_evpool.push_back (r.b);
   }

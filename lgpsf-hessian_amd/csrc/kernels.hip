@@ -417,9 +417,12 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
              && Rn.prec == F64 && R.nrows == A.n, 8,
              "k_cheb_step_mixed: shape/precision mismatch");
   const double vec = (double) A.n * 64;
-  ProfScope ps (PK_CHEB_Z, R.npanels () * (csr_bytes (A)
+  // the first step of a solve (no D) is credited to PK_OTHER so that
+  // PK_CHEB_MIXED times one kernel instantiation
+  ProfScope ps (D32 ? PK_CHEB_MIXED : PK_OTHER, R.npanels () * (csr_bytes (A)
                 + vec * (4 + (D32 ? 4 : 0) + 8 + (x_mode ? 8 : 0) + 4 + 8 + 8 + 4)),
-                R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * 64);
+                R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * 64,
+                R.npanels ());
   const dim3 g = grid64 (A.n, R.npanels (), F64);
   if (D32)
     hipLaunchKernelGGL (cheb64m_k<true>, g, dim3 (TPB), 0, stream (), A.n,
@@ -437,16 +440,16 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
   return 0;
 }
 
-// degree-2 correction-form smoother (zsolve_impl.h:385-404 with nu = 2) in
-// two fused passes.  With X = X0 on entry, step 1 gives D1 = b1 dinv R,
-// R1 = R - A D1; step 2 gives D2 = a2 D1 + b2 dinv R1, R2 = R1 - A D2 and
-// the smoothed X = X0 + D1 + D2.  X is touched once (in step 2), D2 is
-// never stored, and R2 only when it is used (pre-smoothing: restriction).
-template <typename TV, typename T>
+// degree-2 correction-form smoother (zsolve_impl.h:385-404 with nu = 2)
+// in one pass.  X2 = X0 + D1 + D2 needs R0 only at
+// radius 1: R1(i) = R0(i) - sum_j A_ij D1(j) with D1(j) = b1 dinv_j R0(j),
+// then D2(i) = a2 D1(i) + b2 dinv_i R1(i).  Same arithmetic as sm1_k +
+// sm2_k; D1, R1 and R2 are never stored.
+template <typename TV, typename T, bool XADD>
 __global__ void __launch_bounds__ (TPB)
-sm1_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
-       const TV *__restrict__ v, const TV *__restrict__ dinv, T b,
-       const T *__restrict__ Rin, T *__restrict__ D1, T *__restrict__ R1)
+smx_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
+       const TV *__restrict__ v, const TV *__restrict__ dinv, T b1, T a2,
+       T b2, const T *__restrict__ R0, T *__restrict__ X)
 {
   LGH_ROWMAP (T)
   T acc[N];
@@ -465,124 +468,60 @@ sm1_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
     for (int u = 0; u < NG; u++) dd[u] = (T) dinv[c[u]];
     T x[NG][N];
 #pragma unroll
-    for (int u = 0; u < NG; u++) vld (Rin + off + (size_t) c[u] * 64, x[u]);
+    for (int u = 0; u < NG; u++) vld (R0 + off + (size_t) c[u] * 64, x[u]);
 #pragma unroll
     for (int u = 0; u < NG; u++)
 #pragma unroll
-      for (int i = 0; i < N; i++) acc[i] += w[u] * (b * (dd[u] * x[u][i]));
+      for (int i = 0; i < N; i++) acc[i] += w[u] * (b1 * (dd[u] * x[u][i]));
   }
   const size_t qi = off + (size_t) row * 64;
   const T dr = (T) dinv[row];
-  T rin[N], d1[N];
-  vld (Rin + qi, rin);
+  T r0[N], x0[N], out[N];
+  vld (R0 + qi, r0);
+  if (XADD) vld (X + qi, x0);
 #pragma unroll
   for (int i = 0; i < N; i++) {
-    d1[i] = b * (dr * rin[i]);
-    rin[i] -= acc[i];
+    const T d1 = b1 * (dr * r0[i]);
+    const T r1 = r0[i] - acc[i];
+    const T bz = b2 * (dr * r1);
+    const T ad = a2 * d1;
+    const T d2 = ad + bz;
+    out[i] = XADD ? x0[i] + d1 + d2 : d1 + d2;
   }
-  vst (D1 + qi, d1);
-  vst (R1 + qi, rin);
-}
-
-template <typename TV, typename T, bool XADD, bool WR>
-__global__ void __launch_bounds__ (TPB)
-sm2_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
-       const TV *__restrict__ v, const TV *__restrict__ dinv, T a, T b,
-       const T *__restrict__ D1, const T *__restrict__ R1,
-       T *__restrict__ X, T *__restrict__ R2)
-{
-  LGH_ROWMAP (T)
-  auto d2 = [&] (size_t q, T dj, T (&o)[N]) {
-    T r[N], d[N];
-    vld (R1 + q, r);
-    vld (D1 + q, d);
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-      const T bz = b * (dj * r[i]);
-      const T ad = a * d[i];
-      o[i] = ad + bz;
-    }
-  };
-  const size_t qi = off + (size_t) row * 64;
-  {
-    T di[N], d1[N], x[N];
-    d2 (qi, (T) dinv[row], di);
-    vld (D1 + qi, d1);
-    if (XADD) vld (X + qi, x);
-#pragma unroll
-    for (int i = 0; i < N; i++) x[i] = XADD ? x[i] + d1[i] + di[i] : d1[i] + di[i];
-    vst (X + qi, x);
-  }
-  if (WR) {
-    T acc[N];
-#pragma unroll
-    for (int i = 0; i < N; i++) acc[i] = 0;
-    for (int kb = k0; kb < k1; kb += NG) {
-      int c[NG];
-      T w[NG], dd[NG];
-#pragma unroll
-      for (int u = 0; u < NG; u++) {
-        const bool in = kb + u < k1;
-        c[u] = in ? ci[kb + u] : row;
-        w[u] = in ? (T) v[kb + u] : (T) 0;
-      }
-#pragma unroll
-      for (int u = 0; u < NG; u++) dd[u] = (T) dinv[c[u]];
-      T y[NG][N];
-#pragma unroll
-      for (int u = 0; u < NG; u++) d2 (off + (size_t) c[u] * 64, dd[u], y[u]);
-#pragma unroll
-      for (int u = 0; u < NG; u++)
-#pragma unroll
-        for (int i = 0; i < N; i++) acc[i] += w[u] * y[u][i];
-    }
-    T r[N];
-    vld (R1 + qi, r);
-#pragma unroll
-    for (int i = 0; i < N; i++) r[i] -= acc[i];
-    vst (R2 + qi, r);
-  }
-}
-
-template <typename TV, typename T>
-static int
-smooth2_t (const DevCSR &A, double b1, double a2, double b2, const PMat &Rin,
-           PMat &D1, PMat &R1, PMat &X, int xadd, PMat *R2)
-{
-  const dim3 g = grid64 (A.n, Rin.npanels (), vt<T>::prec);
-  hipLaunchKernelGGL ((sm1_k<TV, T>), g, dim3 (TPB), 0, stream (), A.n, A.rp,
-                      A.ci, (const TV *) A.v, (const TV *) A.dinv, (T) b1,
-                      (const T *) Rin.p, (T *) D1.p, (T *) R1.p);
-  LGH_LAUNCH_CHECK ();
-  T *r2 = R2 ? (T *) R2->p : nullptr;
-#define LGH_SM2(XA, W_)                                                       \
-  hipLaunchKernelGGL ((sm2_k<TV, T, XA, W_>), g, dim3 (TPB), 0, stream (),    \
-                      A.n, A.rp, A.ci, (const TV *) A.v, (const TV *) A.dinv, \
-                      (T) a2, (T) b2, (const T *) D1.p, (const T *) R1.p,      \
-                      (T *) X.p, r2)
-  if (xadd) { if (R2) LGH_SM2 (true, true); else LGH_SM2 (true, false); }
-  else { if (R2) LGH_SM2 (false, true); else LGH_SM2 (false, false); }
-#undef LGH_SM2
-  LGH_LAUNCH_CHECK ();
-  return 0;
+  vst (X + qi, out);
 }
 
 int
-k_smooth2 (const DevCSR &A, double b1, double a2, double b2, const PMat &Rin,
-           PMat &D1, PMat &R1, PMat &X, int xadd, PMat *R2)
+k_smooth2x (const DevCSR &A, double b1, double a2, double b2, const PMat &R0,
+            PMat &X, int xadd)
 {
-  LGH_CHECK (Rin.W == 64 && X.W == 64 && D1.W == 64 && R1.W == 64
-             && Rin.prec == X.prec && A.dinv && Rin.nrows == A.n, 8,
-             "k_smooth2: shape mismatch");
-  const double vec = (double) A.n * 64 * prec_bytes (Rin.prec);
-  const double streams = 3 + (2 + (xadd ? 1 : 0) + 1 + (R2 ? 1 : 0));
-  ProfScope ps (PK_CHEB_JAC, Rin.npanels () * (2 * csr_bytes (A) + streams * vec),
-                Rin.npanels () * (4.0 * (double) A.nnz + 10.0 * A.n) * 64);
-  if (Rin.prec == F64)
-    return A.prec == F64 ? smooth2_t<double, double> (A, b1, a2, b2, Rin, D1, R1, X, xadd, R2)
-                         : smooth2_t<float, double> (A, b1, a2, b2, Rin, D1, R1, X, xadd, R2);
-  return A.prec == F64 ? smooth2_t<double, float> (A, b1, a2, b2, Rin, D1, R1, X, xadd, R2)
-                       : smooth2_t<float, float> (A, b1, a2, b2, Rin, D1, R1, X, xadd, R2);
+  LGH_CHECK (R0.W == 64 && X.W == 64 && R0.prec == X.prec && A.dinv
+             && R0.nrows == A.n && X.nrows == A.n
+             && R0.npanels () == X.npanels (), 8, "k_smooth2x: shape mismatch");
+  const double vec = (double) A.n * 64 * prec_bytes (R0.prec);
+  const int kind = prof_fine_tag () ? (xadd ? PK_SM_POST : PK_SM_PRE) : PK_CHEB_JAC;
+  ProfScope ps (kind, R0.npanels () * (csr_bytes (A) + (double) A.n * prec_bytes (A.prec)
+                                      + (2 + (xadd ? 1 : 0)) * vec),
+                R0.npanels () * (2.0 * (double) A.nnz + 9.0 * A.n) * 64,
+                R0.npanels ());
+  const dim3 g = grid64 (A.n, R0.npanels (), R0.prec);
+  auto go = [&] (auto tv, auto t) -> int {
+    using TV = decltype (tv);
+    using T = decltype (t);
+    if (xadd)
+      hipLaunchKernelGGL ((smx_k<TV, T, true>), g, dim3 (TPB), 0, stream (),
+                          A.n, A.rp, A.ci, (const TV *) A.v, (const TV *) A.dinv,
+                          (T) b1, (T) a2, (T) b2, (const T *) R0.p, (T *) X.p);
+    else
+      hipLaunchKernelGGL ((smx_k<TV, T, false>), g, dim3 (TPB), 0, stream (),
+                          A.n, A.rp, A.ci, (const TV *) A.v, (const TV *) A.dinv,
+                          (T) b1, (T) a2, (T) b2, (const T *) R0.p, (T *) X.p);
+    LGH_LAUNCH_CHECK ();
+    return 0;
+  };
+  if (R0.prec == F64)
+    return A.prec == F64 ? go (0.0, 0.0) : go (0.f, 0.0);
+  return A.prec == F64 ? go (0.0, 0.f) : go (0.f, 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -643,7 +582,7 @@ k_spmm (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
   ProfScope ps (prof_b_tag () ? PK_SPMM_B : PK_SPMM,
                 np_ * (csr_bytes (A) + (double) A.m * X.W * sx
                        + (double) A.n * X.W * sx * (beta != 0.0 ? 2 : 1)),
-                np_ * 2.0 * (double) A.nnz * X.W);
+                np_ * 2.0 * (double) A.nnz * X.W, (int) np_);
   if (X.prec == F64)
     return A.prec == F64 ? spmm_t<double, double> (A, alpha, X, beta, Y)
                          : spmm_t<float, double> (A, alpha, X, beta, Y);
@@ -703,9 +642,9 @@ k_resid (const DevCSR &A, const PMat &B, const PMat &X, PMat &R)
 {
   LGH_CHECK (X.W == R.W && B.W == R.W && X.prec == R.prec && B.prec == R.prec
              && X.nrows == A.n, 8, "k_resid: shape mismatch");
-  ProfScope ps (PK_RESID, X.npanels () * (csr_bytes (A)
-                + 3.0 * A.n * X.W * prec_bytes (X.prec)),
-                X.npanels () * 2.0 * (double) A.nnz * X.W);
+  ProfScope ps (prof_fine_tag () ? PK_RESID_FINE : PK_RESID,
+                X.npanels () * (csr_bytes (A) + 3.0 * A.n * X.W * prec_bytes (X.prec)),
+                X.npanels () * 2.0 * (double) A.nnz * X.W, X.npanels ());
   if (X.prec == F64)
     return A.prec == F64 ? resid_t<double, double> (A, B, X, R)
                          : resid_t<float, double> (A, B, X, R);
@@ -810,7 +749,8 @@ k_cheb_step (const DevCSR &A, int jacobi, const PMat *Zsrc, double a,
   ProfScope ps (jacobi ? PK_CHEB_JAC : PK_CHEB_Z,
                 R.npanels () * (csr_bytes (A) + streams * vec
                                 + (jacobi ? (double) A.n * prec_bytes (A.prec) : 0.0)),
-                R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * R.W);
+                R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * R.W,
+                R.npanels ());
   if (R.prec == F64)
     return A.prec == F64
         ? cheb_t<double, double> (A, jacobi, Zsrc, a, b, D, R, X, x_mode, Dn, Rn)

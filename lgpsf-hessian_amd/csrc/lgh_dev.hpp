@@ -62,25 +62,34 @@ int dcopy (void *dst, const void *src, size_t bytes);
 enum ProfKind {
   PK_SPMM_B = 0,      // B X of the F sweep
   PK_SPMM = 1,        // other SpMMs (Z apply, P, P^T)
-  PK_CHEB_JAC = 2,    // fused Chebyshev step, Jacobi (smoothers, mode 4)
+  PK_CHEB_JAC = 2,    // Jacobi Chebyshev steps and smoothers below the
+                      // finest AMG level
   PK_CHEB_Z = 3,      // fused Chebyshev step, explicit preconditioned Z
-  PK_RESID = 4,
+  PK_RESID = 4,       // residuals below the finest AMG level
   PK_GRAM = 5,
   PK_GEMM = 6,
   PK_OTHER = 7,
-  PK_N = 8
+  PK_CHEB_MIXED = 8,  // mixed-precision Chebyshev step (FP64 R, X)
+  PK_SM_PRE = 9,      // fused smoother, finest level, zero initial guess
+  PK_SM_POST = 10,    // fused smoother, finest level, X += correction
+  PK_RESID_FINE = 11, // residual on the finest AMG level
+  PK_N = 12
 };
 void prof_enable (int on);
 int  prof_enabled ();
 void prof_set_b_tag (int on);      // next SpMMs are the F sweep's B product
 int  prof_b_tag ();
+void prof_set_fine_tag (int on);   // next launches run on the finest level
+int  prof_fine_tag ();
 struct ProfScope {
-  int kind; double bytes, flops; int active; void *ev0;
-  ProfScope (int kind, double bytes, double flops);
+  int kind, np; double bytes, flops; int active; void *ev0;
+  // np: panels of the launch (recorded so one launch shape can be selected)
+  ProfScope (int kind, double bytes, double flops, int np = 0);
   ~ProfScope ();
 };
-// totals since the last collect (synchronizes the stream)
-int prof_collect (int kind, long *count, double *ms, double *bytes,
+// totals since the last collect of that kind (synchronizes the stream);
+// np >= 0 counts only launches over np panels (the others are dropped too)
+int prof_collect (int kind, int np, long *count, double *ms, double *bytes,
                   double *flops);
 
 // ---- objects ------------------------------------------------------------
@@ -138,12 +147,11 @@ int k_cheb_step (const DevCSR &A, int jacobi, const PMat *Zsrc, double a,
 int k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
                        const PMat *D32, const PMat &R, PMat &X, int x_mode,
                        PMat &Dn32, PMat &Rn, PMat &Rn32);
-// degree-2 Chebyshev-Jacobi smoother in two fused passes (W = 64):
-// D1 = b1 dinv Rin, R1 = Rin - A D1; D2 = a2 D1 + b2 dinv R1;
-// X = (xadd ? X : 0) + D1 + D2; R2 = R1 - A D2 when R2 != null
-int k_smooth2 (const DevCSR &A, double b1, double a2, double b2,
-               const PMat &Rin, PMat &D1, PMat &R1, PMat &X, int xadd,
-               PMat *R2);
+// degree-2 Chebyshev-Jacobi smoother in one pass (W = 64):
+// D1 = b1 dinv R0, R1 = R0 - A D1; D2 = a2 D1 + b2 dinv R1;
+// X = (xadd ? X : 0) + D1 + D2 (D1, R1 recomputed at the neighbours)
+int k_smooth2x (const DevCSR &A, double b1, double a2, double b2,
+                const PMat &R0, PMat &X, int xadd);
 // X0 = Ainv B0 (dense n0 x n0 row-major inverse in X's precision)
 int k_dense_apply (const void *Ainv, int n0, const PMat &B, PMat &X);
 

@@ -156,19 +156,23 @@ mg_run (DevMG &g, int l, const PMat &B, PMat &X)
 {
   if (l == 0) return k_dense_apply (g.cinv, g.n0, B, X);
   if (g.nu[l] == 2 && B.W == 64) {
-    // fused degree-2 smoothers: the same recurrence, fewer passes over X
+    // degree-2 smoother as one pass (k_smooth2x), explicit residuals
     ChebCoef cc (g.slo[l], g.shi[l]);
     double a1, b1, a2, b2;
     cc.step (1, &a1, &b1);
     cc.step (2, &a2, &b2);
-    LGH_TRY (k_smooth2 (*g.A[l], b1, a2, b2, B, g.D1[l], g.R1[l], X, 0,
-                        &g.R2[l]));                                  // pre
-    LGH_TRY (k_spmm (*g.PT[l], 1.0, g.R2[l], 0.0, g.Bl[l - 1]));
+    const int fine = (l == g.nl - 1);
+    prof_set_fine_tag (fine);
+    LGH_TRY (k_smooth2x (*g.A[l], b1, a2, b2, B, X, 0));             // pre
+    LGH_TRY (k_resid (*g.A[l], B, X, g.R1[l]));
+    prof_set_fine_tag (0);
+    LGH_TRY (k_spmm (*g.PT[l], 1.0, g.R1[l], 0.0, g.Bl[l - 1]));
     LGH_TRY (mg_run (g, l - 1, g.Bl[l - 1], g.Xl[l - 1]));
     LGH_TRY (k_spmm (*g.P[l], 1.0, g.Xl[l - 1], 1.0, X));
+    prof_set_fine_tag (fine);
     LGH_TRY (k_resid (*g.A[l], B, X, g.R1[l]));
-    LGH_TRY (k_smooth2 (*g.A[l], b1, a2, b2, g.R1[l], g.D1[l], g.R2[l], X, 1,
-                        nullptr));                                   // post
+    LGH_TRY (k_smooth2x (*g.A[l], b1, a2, b2, g.R1[l], X, 1));       // post
+    prof_set_fine_tag (0);
     return 0;
   }
   // pre-smoothing; the residual B - A X it leaves (the Chebyshev
