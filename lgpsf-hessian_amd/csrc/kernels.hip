@@ -142,8 +142,9 @@ k_randn (PMat &out, uint64_t seed, int64_t row0, int64_t col0)
 // (fp32) or 32 (fp64) lanes and a wavefront works on 4 or 2 rows at once.
 // The CSR entries of a row are taken 8 at a time: their column indices
 // and values first, then the 8 neighbour-row loads all issued before any
-// is used.  Entries past the row's end re-read the row itself with a zero
-// weight, so every load is in bounds and the loop has no branches.  The
+// is used.  Entries past the row's end get weight 0, the row's own index
+// (so every address stays in bounds) and no load: the load is masked off,
+// which matters for 5- and 7-point rows.  The
 // panel index is blockIdx.y: all blocks of one panel run before the next
 // panel's, which keeps the neighbour rows of a row in L2.
 
@@ -166,6 +167,13 @@ __device__ inline void vst (float *p, const double (&x)[2])
 { *(float2 *) p = make_float2 ((float) x[0], (float) x[1]); }
 
 static constexpr int NG = 8;            // CSR entries per load group
+
+template <typename T, int N>
+__device__ inline void vzero (T (&x)[N])
+{
+#pragma unroll
+  for (int i = 0; i < N; i++) x[i] = (T) 0;
+}
 
 #define LGH_ROWMAP(T_)                                                        \
   constexpr int N = VT<T_>::N;                                                \
@@ -211,7 +219,10 @@ spmm64_k (int n, int nx, const int *__restrict__ rp,
     }
     T x[NG][N];
 #pragma unroll
-    for (int u = 0; u < NG; u++) vld (Xp + (size_t) c[u] * 64, x[u]);
+    for (int u = 0; u < NG; u++) {
+      if (kb + u < k1) vld (Xp + (size_t) c[u] * 64, x[u]);
+      else vzero (x[u]);
+    }
 #pragma unroll
     for (int u = 0; u < NG; u++)
 #pragma unroll
@@ -253,7 +264,10 @@ resid64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
     }
     T x[NG][N];
 #pragma unroll
-    for (int u = 0; u < NG; u++) vld (X + off + (size_t) c[u] * 64, x[u]);
+    for (int u = 0; u < NG; u++) {
+      if (kb + u < k1) vld (X + off + (size_t) c[u] * 64, x[u]);
+      else vzero (x[u]);
+    }
 #pragma unroll
     for (int u = 0; u < NG; u++)
 #pragma unroll
@@ -314,7 +328,10 @@ cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
     for (int u = 0; u < NG; u++) dd[u] = JAC ? (T) dinv[c[u]] : (T) 0;
     T y[NG][N];
 #pragma unroll
-    for (int u = 0; u < NG; u++) dnew (off + (size_t) c[u] * 64, dd[u], y[u]);
+    for (int u = 0; u < NG; u++) {
+      if (kb + u < k1) dnew (off + (size_t) c[u] * 64, dd[u], y[u]);
+      else vzero (y[u]);
+    }
 #pragma unroll
     for (int u = 0; u < NG; u++)
 #pragma unroll
@@ -380,29 +397,42 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
     }
     double y[NG][N];
 #pragma unroll
-    for (int u = 0; u < NG; u++) dnew (off + (size_t) c[u] * 64, y[u]);
+    for (int u = 0; u < NG; u++) {
+      if (kb + u < k1) dnew (off + (size_t) c[u] * 64, y[u]);
+      else vzero (y[u]);
+    }
 #pragma unroll
     for (int u = 0; u < NG; u++)
 #pragma unroll
       for (int i = 0; i < N; i++) acc[i] += w[u] * y[u][i];
   }
   const size_t qi = off + (size_t) row * 64;
-  double di[N], r[N], x[N];
-  dnew (qi, di);
-  vld (R + qi, r);
-  if (xmode) {
-    vld (X + qi, x);
+  double di[N], r[N], x[N], dold[N];
+  {
+    double z[N];
+    vld (Zs + qi, z);
+    if (HASD) vld (D + qi, dold);
 #pragma unroll
-    for (int i = 0; i < N; i++) x[i] = x[i] + di[i];
+    for (int i = 0; i < N; i++) {
+      const double bz = b * z[i];
+      di[i] = (double) (float) (HASD ? a * dold[i] + bz : bz);
+    }
   }
-  else {
+  vld (R + qi, r);
+  // X update (MIX_X_*): the deferred forms add the previous direction
+  // first, so X ends bit-identical to updating it at every step
+  if (xmode == MIX_X_ADD || xmode == MIX_X_ADD2) vld (X + qi, x);
 #pragma unroll
-    for (int i = 0; i < N; i++) x[i] = di[i];
+  for (int i = 0; i < N; i++) {
+    if (xmode == MIX_X_SET) x[i] = di[i];
+    else if (xmode == MIX_X_ADD) x[i] = x[i] + di[i];
+    else if (xmode == MIX_X_SET2) x[i] = (HASD ? dold[i] : 0.0) + di[i];
+    else if (xmode == MIX_X_ADD2) x[i] = (x[i] + (HASD ? dold[i] : 0.0)) + di[i];
   }
 #pragma unroll
   for (int i = 0; i < N; i++) r[i] -= acc[i];
   vst (Dn + qi, di);
-  vst (X + qi, x);
+  if (xmode != MIX_X_SKIP) vst (X + qi, x);
   vst (Rn + qi, r);
   vst (Rn32 + qi, r);
 }
@@ -419,8 +449,13 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
   const double vec = (double) A.n * 64;
   // the first step of a solve (no D) is credited to PK_OTHER so that
   // PK_CHEB_MIXED times one kernel instantiation
+  LGH_CHECK (x_mode >= MIX_X_SET && x_mode <= MIX_X_ADD2
+             && (D32 || (x_mode != MIX_X_SET2 && x_mode != MIX_X_ADD2)), 8,
+             "k_cheb_step_mixed: bad X mode");
+  const double xb = (x_mode == MIX_X_SKIP) ? 0 : (x_mode == MIX_X_SET
+                     || x_mode == MIX_X_SET2) ? 8 : 16;
   ProfScope ps (D32 ? PK_CHEB_MIXED : PK_OTHER, R.npanels () * (csr_bytes (A)
-                + vec * (4 + (D32 ? 4 : 0) + 8 + (x_mode ? 8 : 0) + 4 + 8 + 8 + 4)),
+                + vec * (4 + (D32 ? 4 : 0) + 8 + xb + 4 + 8 + 4)),
                 R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * 64,
                 R.npanels ());
   const dim3 g = grid64 (A.n, R.npanels (), F64);
@@ -468,7 +503,10 @@ smx_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
     for (int u = 0; u < NG; u++) dd[u] = (T) dinv[c[u]];
     T x[NG][N];
 #pragma unroll
-    for (int u = 0; u < NG; u++) vld (R0 + off + (size_t) c[u] * 64, x[u]);
+    for (int u = 0; u < NG; u++) {
+      if (kb + u < k1) vld (R0 + off + (size_t) c[u] * 64, x[u]);
+      else vzero (x[u]);
+    }
 #pragma unroll
     for (int u = 0; u < NG; u++)
 #pragma unroll

@@ -5,10 +5,11 @@
 // panels of width W (W in {1,2,4,...,64}); panel p is an nrows x W
 // row-major slab, so element (i, j) lives at
 //     ((j / W) * nrows + i) * W + (j % W).
-// With W = 64 one wavefront owns one row of a panel (one lane per column):
-// every CSR neighbour gather is one coalesced 256 B (fp32) / 512 B (fp64)
-// load, and elementwise work streams contiguously.  Columns past ncols in
-// the last panel are padding and are kept at zero by every kernel.
+// With W = 64 a panel row is 256 B (fp32) / 512 B (fp64) contiguous: 16 or
+// 32 lanes load it as 16 B per lane, so every CSR neighbour gather is one
+// coalesced row load, and elementwise work streams contiguously.  Columns
+// past ncols in the last panel are padding and are kept at zero by every
+// kernel.
 //
 // Sparse matrices are CSR with int32 row pointers / column indices and
 // fp64 or fp32 values.  Small dense matrices (Gram, T, eigenvectors,
@@ -100,8 +101,6 @@ struct DevCSR {
   void *v = nullptr;              // values, precision `prec`
   int prec = F64;
   void *dinv = nullptr;           // 1/diag (same precision), square only
-  int spmm_np[2] = { 0, 0 };      // autotuned panels per block (W = 64),
-                                  // per vector precision; 0 = not tuned
   ~DevCSR ();
 };
 // upload host CSR (values fp64 on the host) at precision prec
@@ -143,7 +142,15 @@ int k_cheb_step (const DevCSR &A, int jacobi, const PMat *Zsrc, double a,
                  double b, const PMat *D, const PMat &R, PMat &X, int x_mode,
                  PMat &Dn, PMat &Rn);
 // mixed step at W = 64: FP64 R, X (A values FP64), FP32 D and Z; also
-// writes the new residual in FP32 (Rn32) for the next preconditioner
+// writes the new residual in FP32 (Rn32) for the next preconditioner.
+// x_mode: how X takes the new direction Dn (D = the previous direction)
+enum {
+  MIX_X_SET = 0,      // X = Dn
+  MIX_X_ADD = 1,      // X = X + Dn
+  MIX_X_SKIP = 2,     // X untouched (Dn is added at the next step)
+  MIX_X_SET2 = 3,     // X = D + Dn            (after a skipped first step)
+  MIX_X_ADD2 = 4      // X = (X + D) + Dn      (after a skipped step)
+};
 int k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
                        const PMat *D32, const PMat &R, PMat &X, int x_mode,
                        PMat &Dn32, PMat &Rn, PMat &Rn32);

@@ -286,8 +286,14 @@ cheb_solve_mixed (const DevCSR &A64, DevMG &mg32, double lmin, double lmax,
     double a, b;
     cc.step (k, &a, &b);
     const PMat *Dcur = (k == 1) ? nullptr : &w.D[di ^ 1];
+    // X takes the directions two at a time: odd steps before the last leave
+    // X alone, the next step adds both (same sums in the same order)
+    int xm;
+    if (k % 2 == 1 && k < nit) xm = MIX_X_SKIP;
+    else if (k % 2 == 0) xm = (k == 2) ? MIX_X_SET2 : MIX_X_ADD2;
+    else xm = (k == 1) ? MIX_X_SET : MIX_X_ADD;
     LGH_TRY (mg_vcycle (mg32, w.R32, w.Z32));
-    LGH_TRY (k_cheb_step_mixed (A64, w.Z32, a, b, Dcur, *Rsrc, X, k > 1,
+    LGH_TRY (k_cheb_step_mixed (A64, w.Z32, a, b, Dcur, *Rsrc, X, xm,
                                 w.D[di], w.R[ri], w.R32));
     Rsrc = &w.R[ri];
     ri ^= 1;
