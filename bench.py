@@ -90,6 +90,10 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--vcycle", choices=("bf16", "f32"), default="bf16",
+                    help="V-cycle storage precision inside the FP64 sweeps")
+    ap.add_argument("--sketch-vcycle", choices=("bf16", "f32"), default="f32",
+                    help="V-cycle storage precision inside the FP32 sweeps")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -122,7 +126,9 @@ def main():
     log("prior create %.1fs %s" % (time.time() - t0, info))
     Bm = lgh.Mat.aij(Bs)
     opts = lgh.glr_opts(ell=args.ell, q_power=1, seed=0)
-    mixed = lgh.xopts(mixed=True)
+    pcode = {"bf16": lgh.PREC_BF16, "f32": lgh.PREC_F32}
+    mixed = lgh.xopts(mixed=True, prec_precond=pcode[args.vcycle],
+                      prec_sketch_precond=pcode[args.sketch_vcycle])
     plain = lgh.xopts(mixed=False)
 
     for w in range(args.warmup):
@@ -264,7 +270,9 @@ def main():
                                "Z^-1 = Chebyshev(%d) around AMG V-cycle"
                                % (args.side, args.side, n, Bs.nnz / n, args.ell,
                                   info["nit"]),
-                   "n": n, "ell": args.ell, "kept": g.kept},
+                   "n": n, "ell": args.ell, "kept": g.kept,
+                   "vcycle_fp64_sweeps": args.vcycle,
+                   "vcycle_fp32_sweeps": args.sketch_vcycle},
         "roofline": roof,
         "spmm_roofline": spmm_roof,
         "mfma": mf,

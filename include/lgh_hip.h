@@ -86,7 +86,8 @@ int  lgh_amg_level (const lgh_amg_t *h, int l, int *n, int *nc,
                     const int **rp, const int **ci, const double **v,
                     const int **prp, const int **pci, const double **pv,
                     double *slo, double *shi, int *nu);
-/* X = V-cycle(B) on the device (prec 0/1), ncols columns */
+/* X = V-cycle(B) on the device, ncols columns; prec 0 FP64, 1 FP32,
+ * 2 BF16 vectors with FP32 matrices and arithmetic (W = 64 only) */
 int  lgh_amg_vcycle (const lgh_amg_t *h, int prec, int W, int ncols,
                      const double *B, int ldb, double *X, int ldx);
 

@@ -11,22 +11,29 @@
 extern "C" {
 #endif
 
-#define LGH_PREC_F64 0
-#define LGH_PREC_F32 1
+#define LGH_PREC_F64  0
+#define LGH_PREC_F32  1
+#define LGH_PREC_BF16 2   /* storage only (fp32 arithmetic); V-cycle only */
 
 /* Precision policy of the build.  All LGH_PREC_F64 reproduces the
  * reference arithmetic ("plain").  The mixed policy (lgh_glr_xopts_mixed)
- * runs the sketch sweep Y = F Omega entirely in FP32 (vectors, B, Z and
- * the V-cycle: its rounding acts like a perturbation of the Gaussian
- * sketch), runs every later sweep with FP64 vectors around an FP32 AMG
- * V-cycle inside an FP64 Chebyshev iteration (FP64-accurate Z solves with
- * the preconditioner work in FP32), keeps CholQR2, T = Q^T F Q, the
- * eigensolve and U in FP64, and runs the residual-gate sweep in FP32. */
+ * runs the sketch sweep Y = F Omega with FP32 vectors, B, Z and V-cycle
+ * (its rounding acts like a perturbation of the Gaussian sketch), runs
+ * every later sweep with FP64 vectors inside an FP64 Chebyshev iteration
+ * whose AMG V-cycle stores its vectors in BF16 (the Chebyshev step rounds
+ * each new direction once and updates X and R with that same value, so the
+ * Z solves stay FP64-accurate), keeps CholQR2, T = Q^T F Q, the
+ * eigensolve and U in FP64, and runs the residual-gate sweep in FP32.
+ * A V-cycle precision above its sweep's vector precision means "the
+ * vectors' own". */
 typedef struct lgh_glr_xopts
 {
-  int prec_sketch;   /* sketch sweep Y = F Omega                           */
-  int prec_precond;  /* V-cycle inside the later sweeps' Z solves          */
-  int prec_check;    /* residual-gate sweep (opts.check)                   */
+  int prec_sketch;          /* sketch sweep Y = F Omega: F64 or F32        */
+  int prec_precond;         /* V-cycle inside FP64-vector sweeps:
+                               F64, F32 or BF16                            */
+  int prec_check;           /* residual-gate sweep (opts.check): F64, F32  */
+  int prec_sketch_precond;  /* V-cycle inside FP32-vector sweeps (sketch,
+                               check): F32 (or F64) or BF16                */
 }
 lgh_glr_xopts_t;
 
@@ -62,7 +69,8 @@ int lgh_prior_get_info (const lgh_prior_t *prior, lgh_prior_info_t *info);
 
 /* Z-solves through the prior's tiers (host column-major blocks):
  * which = 0 blocked tier (what the build uses), 1 vector (PCG) path;
- * prec = 0 FP64, 1 FP32, 2 FP64 vectors around the FP32 V-cycle.        */
+ * prec = 0 FP64, 1 FP32, 2 FP64 vectors around the FP32 V-cycle,
+ * 3 FP64 vectors around the BF16 V-cycle, 4 FP32 around the BF16 V-cycle. */
 int lgh_prior_solve_host (lgh_prior_t *prior, int which, int prec, int m,
                           const double *X, int ldx, double *Y, int ldy);
 

@@ -22,6 +22,7 @@ vp = C.c_void_p
 
 PREC_F64 = 0
 PREC_F32 = 1
+PREC_BF16 = 2          # V-cycle storage only (fp32 arithmetic)
 
 
 class LGHError(RuntimeError):
@@ -66,7 +67,7 @@ class GlrReport(C.Structure):
 
 class GlrXOpts(C.Structure):
     _fields_ = [("prec_sketch", C.c_int), ("prec_precond", C.c_int),
-                ("prec_check", C.c_int)]
+                ("prec_check", C.c_int), ("prec_sketch_precond", C.c_int)]
 
 
 class PriorMatOpts(C.Structure):

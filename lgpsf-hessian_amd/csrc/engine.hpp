@@ -20,7 +20,7 @@ struct lgh_prior {
   lgh_prior_mat_opts_t o {};
   lgh::AmgHierarchy amg;
   int amg_ok = 0;
-  lgh::DevMG *mg[2] = { nullptr, nullptr };
+  lgh::DevMG *mg[3] = { nullptr, nullptr, nullptr };   // per lgh::Prec
   double *d_zdinv = nullptr;            // fp64 1/diag(Z) (Jacobi PCG)
   int nit = 0, bound_its = 0;
   double lmin = 0, lmax = 0;

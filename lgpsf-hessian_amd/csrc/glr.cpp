@@ -41,6 +41,7 @@ lgh_glr_xopts_default (void)
   x.prec_sketch = LGH_PREC_F64;
   x.prec_precond = LGH_PREC_F64;
   x.prec_check = LGH_PREC_F64;
+  x.prec_sketch_precond = LGH_PREC_F64;
   return x;
 }
 
@@ -49,12 +50,24 @@ lgh_glr_xopts_mixed (void)
 {
   lgh_glr_xopts_t x;
   x.prec_sketch = LGH_PREC_F32;
-  x.prec_precond = LGH_PREC_F32;
+  x.prec_precond = LGH_PREC_BF16;
   x.prec_check = LGH_PREC_F32;
+  x.prec_sketch_precond = LGH_PREC_F32;
   return x;
 }
 
-static inline int to_prec (int p) { return p == LGH_PREC_F32 ? F32 : F64; }
+static inline int to_prec (int p)
+{
+  return p == LGH_PREC_F32 ? F32 : p == LGH_PREC_BF16 ? BF16 : F64;
+}
+// V-cycle precision of a sweep whose vectors are in vp: the policy's
+// V-cycle precision for such sweeps when it is lower than vp, else vp
+static inline int pc_for (const lgh_glr_t *g, int vp)
+{
+  auto rank = [] (int q) { return q == F64 ? 2 : q == F32 ? 1 : 0; };
+  const int pp = to_prec (vp == F64 ? g->x.prec_precond : g->x.prec_sketch_precond);
+  return rank (pp) < rank (vp) ? pp : vp;
+}
 
 static inline int npan (int c) { return (c + 63) / 64; }
 
@@ -306,7 +319,7 @@ residual_gate (lgh_glr_t *g, const std::vector<double> &evals,
   }
   const PMat &Uin = (prec == F64) ? g->U : U;
   const double t0 = MPI_Wtime ();
-  if (!rc) rc = apply_F (g, Uin, FU, Wk);
+  if (!rc) rc = apply_F (g, Uin, FU, Wk, pc_for (g, prec));
   std::vector<double> lr ((size_t) Pk * 64, 0.0), r2 ((size_t) Pk * 64, 0.0);
   for (int j = 0; j < k; j++) lr[j] = evals[order[j]];
   if (!rc) rc = h2d (s, lr.data (), sizeof (double) * Pk * 64);
@@ -332,7 +345,8 @@ timed_F (lgh_glr_t *g, const PMat &X, PMat &Y, PMat &Wk, double *t_op,
 }
 
 // sketch sweep Y64 = F Omega(seed, columns col0..col0+nc-1) at precision
-// ps (FP32: vectors, B, Z and V-cycle all in FP32), result in FP64
+// ps (FP32: vectors, B and Z in FP32; the V-cycle in FP32, or BF16 when
+// that is the preconditioner precision), result in FP64
 static int
 sketch_sweep (lgh_glr_t *g, int ps, int64_t col0, int nc, PMat &Y64,
               double *t_op)
@@ -343,7 +357,7 @@ sketch_sweep (lgh_glr_t *g, int ps, int64_t col0, int nc, PMat &Y64,
     LGH_TRY (pmat_alloc (Q, n, nc, 64, F64));
     LGH_TRY (pmat_alloc (Wk, n, nc, 64, F64));
     int rc = k_randn (Q, g->opts.seed, 0, col0);
-    if (!rc) rc = timed_F (g, Q, Y64, Wk, t_op, to_prec (g->x.prec_precond));
+    if (!rc) rc = timed_F (g, Q, Y64, Wk, t_op, pc_for (g, F64));
     pmat_free (Q); pmat_free (Wk);
     return rc;
   }
@@ -352,7 +366,7 @@ sketch_sweep (lgh_glr_t *g, int ps, int64_t col0, int nc, PMat &Y64,
   LGH_TRY (pmat_alloc (Y, n, nc, 64, ps));
   LGH_TRY (pmat_alloc (Wk, n, nc, 64, ps));
   int rc = k_randn (Q, g->opts.seed, 0, col0);
-  if (!rc) rc = timed_F (g, Q, Y, Wk, t_op, ps);
+  if (!rc) rc = timed_F (g, Q, Y, Wk, t_op, pc_for (g, ps));
   if (!rc) rc = k_copy_cols (Y, 0, Y64, 0, nc);
   pmat_free (Q); pmat_free (Y); pmat_free (Wk);
   return rc;
@@ -439,10 +453,17 @@ lgh_glr_compute_x (Mat B, lgh_prior_t *prior, const lgh_glr_opts_t *opts,
               B->n, prior->n);
   PetscCheck (B->assembled, 0, PETSC_ERR_ARG_WRONGSTATE,
               "lgh_glr_compute: B not assembled");
+  const lgh_glr_xopts_t xo = xopts ? *xopts : lgh_glr_xopts_default ();
+  auto vec_ok = [] (int q) { return q == LGH_PREC_F64 || q == LGH_PREC_F32; };
+  auto pc_ok = [&] (int q) { return vec_ok (q) || q == LGH_PREC_BF16; };
+  PetscCheck (vec_ok (xo.prec_sketch) && vec_ok (xo.prec_check)
+              && pc_ok (xo.prec_precond) && pc_ok (xo.prec_sketch_precond),
+              0, PETSC_ERR_ARG_OUTOFRANGE,
+              "lgh_glr_compute_x: precision codes out of range");
   if (dev_init ()) return API_FAIL ("lgh_glr_compute");
   lgh_glr_t *g = new lgh_glr_t ();
   g->opts = o;
-  g->x = xopts ? *xopts : lgh_glr_xopts_default ();
+  g->x = xo;
   g->n = N;
   g->B = B;
   PetscObjectReference ((PetscObject) B);

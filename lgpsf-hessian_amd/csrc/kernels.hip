@@ -1,9 +1,10 @@
 // kernels.hip — gfx950 kernels of the GLR engine and their launchers.
 //
-// Row kernels (SpMM, fused Chebyshev step, residual) map one wavefront to
-// 64/W rows of one panel: lane (r = lane / W, c = lane % W) owns column c of
-// row r.  At W = 64 a wave owns a whole row, so each CSR neighbour costs one
-// coalesced 64-lane load of that neighbour's row.  The tall-skinny products
+// Row kernels (SpMM, fused Chebyshev step, residual) for W < 64 map one
+// wavefront to 64/W rows of one panel: lane (r = lane / W, c = lane % W)
+// owns column c of row r.  At W = 64 (every large block) a lane moves 16 B
+// of a row, so a CSR neighbour costs one coalesced load of that neighbour's
+// row by 8, 16 or 32 lanes (bf16, fp32, fp64 storage).  The tall-skinny products
 // use MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32) on 64 x 64
 // output tiles with split-K partials reduced in a fixed order
 // (deterministic results).
@@ -151,6 +152,7 @@ k_randn (PMat &out, uint64_t seed, int64_t row0, int64_t col0)
 template <typename T> struct VT;
 template <> struct VT<float> { static constexpr int N = 4; };
 template <> struct VT<double> { static constexpr int N = 2; };
+template <> struct VT<bf16_t> { static constexpr int N = 8; };
 
 __device__ inline void vld (const float *p, float (&x)[4])
 { const float4 v = *(const float4 *) p; x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w; }
@@ -165,6 +167,70 @@ __device__ inline void vld (const float *p, double (&x)[2])
 { const float2 v = *(const float2 *) p; x[0] = v.x; x[1] = v.y; }
 __device__ inline void vst (float *p, const double (&x)[2])
 { *(float2 *) p = make_float2 ((float) x[0], (float) x[1]); }
+
+// bf16 storage (V-cycle vectors of the BF16 preconditioner tier): values
+// widen to fp32 exactly; narrowing rounds to nearest even (the gfx950
+// conversion instruction)
+__device__ inline float bf2f (uint32_t h) { return __uint_as_float (h << 16); }
+__device__ inline uint32_t f2bf (float f)
+{
+  return __builtin_bit_cast (uint16_t, (__bf16) f);
+}
+template <typename T, int N>
+__device__ inline void vld (const bf16_t *p, T (&x)[N])
+{
+  static_assert (N == 2 || N == 4 || N == 8, "bf16 vector width");
+  uint32_t w[N / 2];
+  if constexpr (N == 8) {
+    const uint4 v = *(const uint4 *) p;
+    w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+  }
+  else if constexpr (N == 4) {
+    const uint2 v = *(const uint2 *) p;
+    w[0] = v.x; w[1] = v.y;
+  }
+  else w[0] = *(const uint32_t *) p;
+#pragma unroll
+  for (int i = 0; i < N / 2; i++) {
+    x[2 * i] = (T) bf2f (w[i] & 0xffffu);
+    x[2 * i + 1] = (T) bf2f (w[i] >> 16);
+  }
+}
+template <typename T, int N>
+__device__ inline void vst (bf16_t *p, const T (&x)[N])
+{
+  static_assert (N == 2 || N == 4 || N == 8, "bf16 vector width");
+  uint32_t w[N / 2];
+#pragma unroll
+  for (int i = 0; i < N / 2; i++)
+    w[i] = f2bf ((float) x[2 * i]) | (f2bf ((float) x[2 * i + 1]) << 16);
+  if constexpr (N == 8) *(uint4 *) p = make_uint4 (w[0], w[1], w[2], w[3]);
+  else if constexpr (N == 4) *(uint2 *) p = make_uint2 (w[0], w[1]);
+  else *(uint32_t *) p = w[0];
+}
+
+// 16 B of a panel row as loaded (the neighbour loads of a group stay in
+// this raw form, 4 VGPRs each, and are widened to T where they are used)
+template <typename S> struct Raw16;
+template <> struct Raw16<float> { using type = float4; };
+template <> struct Raw16<double> { using type = double2; };
+template <> struct Raw16<bf16_t> { using type = uint4; };
+template <typename S>
+__device__ inline typename Raw16<S>::type rld (const S *p)
+{ return *(const typename Raw16<S>::type *) p; }
+__device__ inline void unpack (const float4 &r, float (&x)[4])
+{ x[0] = r.x; x[1] = r.y; x[2] = r.z; x[3] = r.w; }
+__device__ inline void unpack (const double2 &r, double (&x)[2])
+{ x[0] = r.x; x[1] = r.y; }
+__device__ inline void unpack (const uint4 &r, float (&x)[8])
+{
+  const uint32_t w[4] = { r.x, r.y, r.z, r.w };
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    x[2 * i] = __uint_as_float (w[i] << 16);
+    x[2 * i + 1] = __uint_as_float (w[i] & 0xffff0000u);
+  }
+}
 
 static constexpr int NG = 8;            // CSR entries per load group
 
@@ -184,26 +250,27 @@ __device__ inline void vzero (T (&x)[N])
   if (row >= n) return;                                                       \
   const int k0 = rp[row], k1 = rp[row + 1];
 
+// grid of a W = 64 row kernel whose row map follows storage precision prec
 static inline dim3
 grid64 (int n, int np, int prec)
 {
-  const int rows = (prec == F64) ? TPB / 32 : TPB / 16;
+  const int rows = TPB / (64 / (16 / (int) prec_bytes (prec)));
   return dim3 ((unsigned) ((n + rows - 1) / rows), (unsigned) np);
 }
 
-// Y = alpha A X + beta Y
-template <typename TV, typename T>
+// Y = alpha A X + beta Y (T arithmetic, S storage of X and Y)
+template <typename TV, typename T, typename S>
 __global__ void __launch_bounds__ (TPB)
 spmm64_k (int n, int nx, const int *__restrict__ rp,
           const int *__restrict__ ci, const TV *__restrict__ v, T alpha,
-          const T *__restrict__ X, T beta, T *__restrict__ Y)
+          const S *__restrict__ X, T beta, S *__restrict__ Y)
 {
-  constexpr int N = VT<T>::N;
+  constexpr int N = VT<S>::N;
   constexpr int LPR = 64 / N;
   const int row = blockIdx.x * (TPB / LPR) + (int) threadIdx.x / LPR;
   const int lc = ((int) threadIdx.x % LPR) * N;
   if (row >= n) return;
-  const T *Xp = X + (size_t) blockIdx.y * nx * 64 + lc;
+  const S *Xp = X + (size_t) blockIdx.y * nx * 64 + lc;
   const int k0 = rp[row], k1 = rp[row + 1];
   T acc[N];
 #pragma unroll
@@ -217,18 +284,21 @@ spmm64_k (int n, int nx, const int *__restrict__ rp,
       c[u] = in ? ci[kb + u] : ci[k0];
       w[u] = in ? (T) v[kb + u] : (T) 0;
     }
-    T x[NG][N];
+    typename Raw16<S>::type x[NG];
 #pragma unroll
     for (int u = 0; u < NG; u++) {
-      if (kb + u < k1) vld (Xp + (size_t) c[u] * 64, x[u]);
-      else vzero (x[u]);
+      if (kb + u < k1) x[u] = rld (Xp + (size_t) c[u] * 64);
+      else x[u] = {};
     }
 #pragma unroll
-    for (int u = 0; u < NG; u++)
+    for (int u = 0; u < NG; u++) {
+      T t[N];
+      unpack (x[u], t);
 #pragma unroll
-      for (int i = 0; i < N; i++) acc[i] += w[u] * x[u][i];
+      for (int i = 0; i < N; i++) acc[i] += w[u] * t[i];
+    }
   }
-  T *y = Y + ((size_t) blockIdx.y * n + row) * 64 + lc;
+  S *y = Y + ((size_t) blockIdx.y * n + row) * 64 + lc;
   if (beta == (T) 0) {
 #pragma unroll
     for (int i = 0; i < N; i++) acc[i] *= alpha;
@@ -243,13 +313,13 @@ spmm64_k (int n, int nx, const int *__restrict__ rp,
 }
 
 // R = B - A X
-template <typename TV, typename T>
+template <typename TV, typename T, typename S>
 __global__ void __launch_bounds__ (TPB)
 resid64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
-           const TV *__restrict__ v, const T *__restrict__ B,
-           const T *__restrict__ X, T *__restrict__ R)
+           const TV *__restrict__ v, const S *__restrict__ B,
+           const S *__restrict__ X, S *__restrict__ R)
 {
-  LGH_ROWMAP (T)
+  LGH_ROWMAP (S)
   T acc[N];
 #pragma unroll
   for (int i = 0; i < N; i++) acc[i] = 0;
@@ -262,16 +332,19 @@ resid64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
       c[u] = in ? ci[kb + u] : row;
       w[u] = in ? (T) v[kb + u] : (T) 0;
     }
-    T x[NG][N];
+    typename Raw16<S>::type x[NG];
 #pragma unroll
     for (int u = 0; u < NG; u++) {
-      if (kb + u < k1) vld (X + off + (size_t) c[u] * 64, x[u]);
-      else vzero (x[u]);
+      if (kb + u < k1) x[u] = rld (X + off + (size_t) c[u] * 64);
+      else x[u] = {};
     }
 #pragma unroll
-    for (int u = 0; u < NG; u++)
+    for (int u = 0; u < NG; u++) {
+      T t[N];
+      unpack (x[u], t);
 #pragma unroll
-      for (int i = 0; i < N; i++) acc[i] += w[u] * x[u][i];
+      for (int i = 0; i < N; i++) acc[i] += w[u] * t[i];
+    }
   }
   T b[N];
   vld (B + off + (size_t) row * 64, b);
@@ -283,13 +356,16 @@ resid64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
 // fused Chebyshev step (zsolve_impl.h:104-141): Dn = a D + b Z with
 // Z = dinv R (JAC) or Zs; X (+)= Dn; Rn = R - A Dn, with Dn recomputed at
 // the neighbours from the old state (a pointwise function of it)
-template <typename TV, typename T, bool JAC, bool HASD>
+// (SP: storage of the preconditioned residual Zs and of Rpc, the copy of
+// the new residual that feeds the preconditioner when it runs in a lower
+// precision; Rpc may be null)
+template <typename TV, typename T, bool JAC, bool HASD, typename SP>
 __global__ void __launch_bounds__ (TPB)
 cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
           const TV *__restrict__ v, const TV *__restrict__ dinv,
-          const T *__restrict__ Zs, T a, T b, const T *__restrict__ D,
+          const SP *__restrict__ Zs, T a, T b, const T *__restrict__ D,
           const T *__restrict__ R, T *__restrict__ X, int xmode,
-          T *__restrict__ Dn, T *__restrict__ Rn)
+          T *__restrict__ Dn, T *__restrict__ Rn, SP *__restrict__ Rpc)
 {
   LGH_ROWMAP (T)
   // dnew at one position: (a D + b z), z = dj R (JAC) or Zs
@@ -355,6 +431,7 @@ cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
   vst (Dn + qi, di);
   vst (X + qi, x);
   vst (Rn + qi, r);
+  if (Rpc) vst (Rpc + qi, r);
 }
 
 // mixed-precision fused Chebyshev step: FP64 residual R and accumulator X,
@@ -363,14 +440,14 @@ cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
 // R stays the FP64 residual of X and the iteration reaches FP64 accuracy
 // with an FP32 preconditioner.  Also writes R in FP32 for the next
 // preconditioner application.  Lanes follow the fp64 row map (2 values).
-template <bool HASD>
+template <bool HASD, typename SP>
 __global__ void __launch_bounds__ (TPB)
 cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
-           const double *__restrict__ v, const float *__restrict__ Zs,
+           const double *__restrict__ v, const SP *__restrict__ Zs,
            double a, double b, const float *__restrict__ D,
            const double *__restrict__ R, double *__restrict__ X, int xmode,
            float *__restrict__ Dn, double *__restrict__ Rn,
-           float *__restrict__ Rn32)
+           SP *__restrict__ Rpc)
 {
   LGH_ROWMAP (double)
   auto dnew = [&] (size_t q, double (&o)[N]) {
@@ -434,45 +511,50 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
   vst (Dn + qi, di);
   if (xmode != MIX_X_SKIP) vst (X + qi, x);
   vst (Rn + qi, r);
-  vst (Rn32 + qi, r);
+  vst (Rpc + qi, r);
 }
 
 int
-k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
+k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                    const PMat *D32, const PMat &R, PMat &X, int x_mode,
-                   PMat &Dn32, PMat &Rn, PMat &Rn32)
+                   PMat &Dn32, PMat &Rn, PMat &Rnpc)
 {
   LGH_CHECK (A.prec == F64 && R.prec == F64 && X.prec == F64 && R.W == 64
-             && Z32.prec == F32 && Dn32.prec == F32 && Rn32.prec == F32
-             && Rn.prec == F64 && R.nrows == A.n, 8,
+             && (Zpc.prec == F32 || Zpc.prec == BF16) && Rnpc.prec == Zpc.prec
+             && Dn32.prec == F32 && Rn.prec == F64 && R.nrows == A.n, 8,
              "k_cheb_step_mixed: shape/precision mismatch");
-  const double vec = (double) A.n * 64;
-  // the first step of a solve (no D) is credited to PK_OTHER so that
-  // PK_CHEB_MIXED times one kernel instantiation
   LGH_CHECK (x_mode >= MIX_X_SET && x_mode <= MIX_X_ADD2
              && (D32 || (x_mode != MIX_X_SET2 && x_mode != MIX_X_ADD2)), 8,
              "k_cheb_step_mixed: bad X mode");
+  const double vec = (double) A.n * 64;
   const double xb = (x_mode == MIX_X_SKIP) ? 0 : (x_mode == MIX_X_SET
                      || x_mode == MIX_X_SET2) ? 8 : 16;
+  const double pb = (double) prec_bytes (Zpc.prec);
+  // the first step of a solve (no D) is credited to PK_OTHER so that
+  // PK_CHEB_MIXED times one kernel instantiation
   ProfScope ps (D32 ? PK_CHEB_MIXED : PK_OTHER, R.npanels () * (csr_bytes (A)
-                + vec * (4 + (D32 ? 4 : 0) + 8 + xb + 4 + 8 + 4)),
+                + vec * (pb + (D32 ? 4 : 0) + 8 + xb + 4 + 8 + pb)),
                 R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * 64,
                 R.npanels ());
   const dim3 g = grid64 (A.n, R.npanels (), F64);
-  if (D32)
-    hipLaunchKernelGGL (cheb64m_k<true>, g, dim3 (TPB), 0, stream (), A.n,
-                        A.rp, A.ci, (const double *) A.v, (const float *) Z32.p,
-                        a, b, (const float *) D32->p, (const double *) R.p,
-                        (double *) X.p, x_mode, (float *) Dn32.p,
-                        (double *) Rn.p, (float *) Rn32.p);
-  else
-    hipLaunchKernelGGL (cheb64m_k<false>, g, dim3 (TPB), 0, stream (), A.n,
-                        A.rp, A.ci, (const double *) A.v, (const float *) Z32.p,
-                        a, b, nullptr, (const double *) R.p, (double *) X.p,
-                        x_mode, (float *) Dn32.p, (double *) Rn.p,
-                        (float *) Rn32.p);
-  LGH_LAUNCH_CHECK ();
-  return 0;
+  auto go = [&] (auto sp) -> int {
+    using SP = decltype (sp);
+    if (D32)
+      hipLaunchKernelGGL ((cheb64m_k<true, SP>), g, dim3 (TPB), 0, stream (),
+                          A.n, A.rp, A.ci, (const double *) A.v,
+                          (const SP *) Zpc.p, a, b, (const float *) D32->p,
+                          (const double *) R.p, (double *) X.p, x_mode,
+                          (float *) Dn32.p, (double *) Rn.p, (SP *) Rnpc.p);
+    else
+      hipLaunchKernelGGL ((cheb64m_k<false, SP>), g, dim3 (TPB), 0, stream (),
+                          A.n, A.rp, A.ci, (const double *) A.v,
+                          (const SP *) Zpc.p, a, b, nullptr,
+                          (const double *) R.p, (double *) X.p, x_mode,
+                          (float *) Dn32.p, (double *) Rn.p, (SP *) Rnpc.p);
+    LGH_LAUNCH_CHECK ();
+    return 0;
+  };
+  return Zpc.prec == BF16 ? go (bf16_t ()) : go (0.f);
 }
 
 // degree-2 correction-form smoother (zsolve_impl.h:385-404 with nu = 2)
@@ -480,13 +562,13 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
 // radius 1: R1(i) = R0(i) - sum_j A_ij D1(j) with D1(j) = b1 dinv_j R0(j),
 // then D2(i) = a2 D1(i) + b2 dinv_i R1(i).  Same arithmetic as sm1_k +
 // sm2_k; D1, R1 and R2 are never stored.
-template <typename TV, typename T, bool XADD>
+template <typename TV, typename T, typename S, bool XADD>
 __global__ void __launch_bounds__ (TPB)
 smx_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
        const TV *__restrict__ v, const TV *__restrict__ dinv, T b1, T a2,
-       T b2, const T *__restrict__ R0, T *__restrict__ X)
+       T b2, const S *__restrict__ R0, S *__restrict__ X)
 {
-  LGH_ROWMAP (T)
+  LGH_ROWMAP (S)
   T acc[N];
 #pragma unroll
   for (int i = 0; i < N; i++) acc[i] = 0;
@@ -500,17 +582,21 @@ smx_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
       w[u] = in ? (T) v[kb + u] : (T) 0;
     }
 #pragma unroll
-    for (int u = 0; u < NG; u++) dd[u] = (T) dinv[c[u]];
-    T x[NG][N];
+    for (int u = 0; u < NG; u++) dd[u] = w[u] * (b1 * (T) dinv[c[u]]);
+    typename Raw16<S>::type x[NG];
 #pragma unroll
     for (int u = 0; u < NG; u++) {
-      if (kb + u < k1) vld (R0 + off + (size_t) c[u] * 64, x[u]);
-      else vzero (x[u]);
+      if (kb + u < k1) x[u] = rld (R0 + off + (size_t) c[u] * 64);
+      else x[u] = {};
     }
+    // sum_j A_ij D1(j) with D1(j) = b1 dinv_j R0(j): one FMA per value
 #pragma unroll
-    for (int u = 0; u < NG; u++)
+    for (int u = 0; u < NG; u++) {
+      T t[N];
+      unpack (x[u], t);
 #pragma unroll
-      for (int i = 0; i < N; i++) acc[i] += w[u] * (b1 * (dd[u] * x[u][i]));
+      for (int i = 0; i < N; i++) acc[i] += dd[u] * t[i];
+    }
   }
   const size_t qi = off + (size_t) row * 64;
   const T dr = (T) dinv[row];
@@ -535,7 +621,8 @@ k_smooth2x (const DevCSR &A, double b1, double a2, double b2, const PMat &R0,
 {
   LGH_CHECK (R0.W == 64 && X.W == 64 && R0.prec == X.prec && A.dinv
              && R0.nrows == A.n && X.nrows == A.n
-             && R0.npanels () == X.npanels (), 8, "k_smooth2x: shape mismatch");
+             && R0.npanels () == X.npanels ()
+             && (R0.prec != BF16 || A.prec == F32), 8, "k_smooth2x: shape mismatch");
   const double vec = (double) A.n * 64 * prec_bytes (R0.prec);
   const int kind = prof_fine_tag () ? (xadd ? PK_SM_POST : PK_SM_PRE) : PK_CHEB_JAC;
   ProfScope ps (kind, R0.npanels () * (csr_bytes (A) + (double) A.n * prec_bytes (A.prec)
@@ -543,23 +630,25 @@ k_smooth2x (const DevCSR &A, double b1, double a2, double b2, const PMat &R0,
                 R0.npanels () * (2.0 * (double) A.nnz + 9.0 * A.n) * 64,
                 R0.npanels ());
   const dim3 g = grid64 (A.n, R0.npanels (), R0.prec);
-  auto go = [&] (auto tv, auto t) -> int {
+  auto go = [&] (auto tv, auto t, auto st) -> int {
     using TV = decltype (tv);
     using T = decltype (t);
+    using S = decltype (st);
     if (xadd)
-      hipLaunchKernelGGL ((smx_k<TV, T, true>), g, dim3 (TPB), 0, stream (),
+      hipLaunchKernelGGL ((smx_k<TV, T, S, true>), g, dim3 (TPB), 0, stream (),
                           A.n, A.rp, A.ci, (const TV *) A.v, (const TV *) A.dinv,
-                          (T) b1, (T) a2, (T) b2, (const T *) R0.p, (T *) X.p);
+                          (T) b1, (T) a2, (T) b2, (const S *) R0.p, (S *) X.p);
     else
-      hipLaunchKernelGGL ((smx_k<TV, T, false>), g, dim3 (TPB), 0, stream (),
+      hipLaunchKernelGGL ((smx_k<TV, T, S, false>), g, dim3 (TPB), 0, stream (),
                           A.n, A.rp, A.ci, (const TV *) A.v, (const TV *) A.dinv,
-                          (T) b1, (T) a2, (T) b2, (const T *) R0.p, (T *) X.p);
+                          (T) b1, (T) a2, (T) b2, (const S *) R0.p, (S *) X.p);
     LGH_LAUNCH_CHECK ();
     return 0;
   };
+  if (R0.prec == BF16) return go (0.f, 0.f, bf16_t ());
   if (R0.prec == F64)
-    return A.prec == F64 ? go (0.0, 0.0) : go (0.f, 0.0);
-  return A.prec == F64 ? go (0.0, 0.f) : go (0.f, 0.f);
+    return A.prec == F64 ? go (0.0, 0.0, 0.0) : go (0.f, 0.0, 0.0);
+  return A.prec == F64 ? go (0.0, 0.f, 0.f) : go (0.f, 0.f, 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -586,18 +675,19 @@ spmm_k (int n, int nx, const int *__restrict__ rp, const int *__restrict__ ci,
   *y = (beta == (T) 0) ? alpha * acc : alpha * acc + beta * *y;
 }
 
-template <typename TV, typename T>
+template <typename TV, typename T, typename S = T>
 static int
 spmm_t (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
 {
   if (X.W == 64) {
-    hipLaunchKernelGGL ((spmm64_k<TV, T>), grid64 (A.n, X.npanels (), vt<T>::prec),
+    hipLaunchKernelGGL ((spmm64_k<TV, T, S>), grid64 (A.n, X.npanels (), X.prec),
                         dim3 (TPB), 0, stream (), A.n, A.m, A.rp, A.ci,
-                        (const TV *) A.v, (T) alpha, (const T *) X.p, (T) beta,
-                        (T *) Y.p);
+                        (const TV *) A.v, (T) alpha, (const S *) X.p, (T) beta,
+                        (S *) Y.p);
     LGH_LAUNCH_CHECK ();
     return 0;
   }
+  LGH_CHECK ((std::is_same<S, T>::value), 8, "k_spmm: bf16 storage needs W = 64");
   auto l = [&] (auto wt) -> int {
       constexpr int W = decltype (wt)::value;
       constexpr int rows_per_block = (TPB / 64) * (64 / W);
@@ -621,6 +711,10 @@ k_spmm (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
                 np_ * (csr_bytes (A) + (double) A.m * X.W * sx
                        + (double) A.n * X.W * sx * (beta != 0.0 ? 2 : 1)),
                 np_ * 2.0 * (double) A.nnz * X.W, (int) np_);
+  if (X.prec == BF16) {
+    LGH_CHECK (A.prec == F32, 8, "k_spmm: bf16 vectors need fp32 values");
+    return spmm_t<float, float, bf16_t> (A, alpha, X, beta, Y);
+  }
   if (X.prec == F64)
     return A.prec == F64 ? spmm_t<double, double> (A, alpha, X, beta, Y)
                          : spmm_t<float, double> (A, alpha, X, beta, Y);
@@ -650,18 +744,19 @@ resid_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
   R[off + (size_t) row * W] = B[off + (size_t) row * W] - acc;
 }
 
-template <typename TV, typename T>
+template <typename TV, typename T, typename S = T>
 static int
 resid_t (const DevCSR &A, const PMat &B, const PMat &X, PMat &R)
 {
   if (X.W == 64) {
-    hipLaunchKernelGGL ((resid64_k<TV, T>), grid64 (A.n, X.npanels (), vt<T>::prec),
+    hipLaunchKernelGGL ((resid64_k<TV, T, S>), grid64 (A.n, X.npanels (), X.prec),
                         dim3 (TPB), 0, stream (), A.n, A.rp, A.ci,
-                        (const TV *) A.v, (const T *) B.p, (const T *) X.p,
-                        (T *) R.p);
+                        (const TV *) A.v, (const S *) B.p, (const S *) X.p,
+                        (S *) R.p);
     LGH_LAUNCH_CHECK ();
     return 0;
   }
+  LGH_CHECK ((std::is_same<S, T>::value), 8, "k_resid: bf16 storage needs W = 64");
   auto l = [&] (auto wt) -> int {
       constexpr int W = decltype (wt)::value;
       constexpr int rows_per_block = (TPB / 64) * (64 / W);
@@ -683,6 +778,10 @@ k_resid (const DevCSR &A, const PMat &B, const PMat &X, PMat &R)
   ProfScope ps (prof_fine_tag () ? PK_RESID_FINE : PK_RESID,
                 X.npanels () * (csr_bytes (A) + 3.0 * A.n * X.W * prec_bytes (X.prec)),
                 X.npanels () * 2.0 * (double) A.nnz * X.W, X.npanels ());
+  if (X.prec == BF16) {
+    LGH_CHECK (A.prec == F32, 8, "k_resid: bf16 vectors need fp32 values");
+    return resid_t<float, float, bf16_t> (A, B, X, R);
+  }
   if (X.prec == F64)
     return A.prec == F64 ? resid_t<double, double> (A, B, X, R)
                          : resid_t<float, double> (A, B, X, R);
@@ -731,27 +830,31 @@ cheb_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
   Rn[qi] = R[qi] - acc;
 }
 
-template <typename TV, typename T>
+template <typename TV, typename T, typename SP = T>
 static int
 cheb_t (const DevCSR &A, int jac, const PMat *Zs, double a, double b,
-        const PMat *D, const PMat &R, PMat &X, int xmode, PMat &Dn, PMat &Rn)
+        const PMat *D, const PMat &R, PMat &X, int xmode, PMat &Dn, PMat &Rn,
+        PMat *Rpc)
 {
   if (R.W == 64) {
     const dim3 g = grid64 (A.n, R.npanels (), vt<T>::prec);
-    const T *zp = Zs ? (const T *) Zs->p : nullptr;
+    const SP *zp = Zs ? (const SP *) Zs->p : nullptr;
     const T *dp = D ? (const T *) D->p : nullptr;
+    SP *rpc = Rpc ? (SP *) Rpc->p : nullptr;
 #define LGH_CH64(J, H)                                                        \
-    hipLaunchKernelGGL ((cheb64_k<TV, T, J, H>), g, dim3 (TPB), 0, stream (), \
-                        A.n, A.rp, A.ci, (const TV *) A.v,                     \
+    hipLaunchKernelGGL ((cheb64_k<TV, T, J, H, SP>), g, dim3 (TPB), 0,        \
+                        stream (), A.n, A.rp, A.ci, (const TV *) A.v,          \
                         (const TV *) A.dinv, zp, (T) a, (T) b, dp,             \
                         (const T *) R.p, (T *) X.p, xmode, (T *) Dn.p,         \
-                        (T *) Rn.p)
+                        (T *) Rn.p, rpc)
     if (jac) { if (dp) LGH_CH64 (true, true); else LGH_CH64 (true, false); }
     else { if (dp) LGH_CH64 (false, true); else LGH_CH64 (false, false); }
 #undef LGH_CH64
     LGH_LAUNCH_CHECK ();
     return 0;
   }
+  LGH_CHECK ((std::is_same<SP, T>::value) && !Rpc, 8,
+             "k_cheb_step: a lower-precision preconditioner needs W = 64");
   auto l = [&] (auto wt) -> int {
       constexpr int W = decltype (wt)::value;
       constexpr int rows_per_block = (TPB / 64) * (64 / W);
@@ -776,38 +879,56 @@ cheb_t (const DevCSR &A, int jac, const PMat *Zs, double a, double b,
 int
 k_cheb_step (const DevCSR &A, int jacobi, const PMat *Zsrc, double a,
              double b, const PMat *D, const PMat &R, PMat &X, int x_mode,
-             PMat &Dn, PMat &Rn)
+             PMat &Dn, PMat &Rn, PMat *Rpc)
 {
   LGH_CHECK (R.nrows == A.n && X.W == R.W && Dn.W == R.W && Rn.W == R.W
              && X.prec == R.prec && (!jacobi || A.dinv), 8,
              "k_cheb_step: shape mismatch");
-  // streams: read R, (D), (Zsrc), (X); write Dn, X, Rn
+  const int pc = Zsrc ? Zsrc->prec : R.prec;
+  LGH_CHECK ((pc == R.prec || (pc == BF16 && R.prec == F32 && R.W == 64))
+             && (!Rpc || Rpc->prec == pc), 8,
+             "k_cheb_step: preconditioner precision");
+  // streams: read R, (D), (Zsrc), (X); write Dn, X, Rn, (Rpc)
   const double vec = (double) A.n * R.W * prec_bytes (R.prec);
-  const double streams = 1 + (D ? 1 : 0) + (jacobi ? 0 : 1) + (x_mode ? 1 : 0) + 3;
+  const double vpc = (double) A.n * R.W * prec_bytes (pc);
+  const double streams = 1 + (D ? 1 : 0) + (x_mode ? 1 : 0) + 3;
   ProfScope ps (jacobi ? PK_CHEB_JAC : PK_CHEB_Z,
                 R.npanels () * (csr_bytes (A) + streams * vec
+                                + ((jacobi ? 0 : 1) + (Rpc ? 1 : 0)) * vpc
                                 + (jacobi ? (double) A.n * prec_bytes (A.prec) : 0.0)),
                 R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * R.W,
                 R.npanels ());
+  if (pc == BF16)
+    return A.prec == F64
+        ? cheb_t<double, float, bf16_t> (A, jacobi, Zsrc, a, b, D, R, X, x_mode, Dn, Rn, Rpc)
+        : cheb_t<float, float, bf16_t> (A, jacobi, Zsrc, a, b, D, R, X, x_mode, Dn, Rn, Rpc);
   if (R.prec == F64)
     return A.prec == F64
-        ? cheb_t<double, double> (A, jacobi, Zsrc, a, b, D, R, X, x_mode, Dn, Rn)
-        : cheb_t<float, double> (A, jacobi, Zsrc, a, b, D, R, X, x_mode, Dn, Rn);
+        ? cheb_t<double, double> (A, jacobi, Zsrc, a, b, D, R, X, x_mode, Dn, Rn, Rpc)
+        : cheb_t<float, double> (A, jacobi, Zsrc, a, b, D, R, X, x_mode, Dn, Rn, Rpc);
   return A.prec == F64
-      ? cheb_t<double, float> (A, jacobi, Zsrc, a, b, D, R, X, x_mode, Dn, Rn)
-      : cheb_t<float, float> (A, jacobi, Zsrc, a, b, D, R, X, x_mode, Dn, Rn);
+      ? cheb_t<double, float> (A, jacobi, Zsrc, a, b, D, R, X, x_mode, Dn, Rn, Rpc)
+      : cheb_t<float, float> (A, jacobi, Zsrc, a, b, D, R, X, x_mode, Dn, Rn, Rpc);
 }
 
 
 
 
-// ---------------------------------------------------------------------------
-// dense coarse apply X0 = Ainv B0 (Ainv row-major n0 x n0)
+// scalar element access in any storage precision (value as T)
+template <typename T> __device__ inline T ld_as (const float *p, size_t i) { return (T) p[i]; }
+template <typename T> __device__ inline T ld_as (const double *p, size_t i) { return (T) p[i]; }
+template <typename T> __device__ inline T ld_as (const bf16_t *p, size_t i) { return (T) bf2f (p[i].u); }
+template <typename T> __device__ inline void st_as (float *p, size_t i, T v) { p[i] = (float) v; }
+template <typename T> __device__ inline void st_as (double *p, size_t i, T v) { p[i] = (double) v; }
+template <typename T> __device__ inline void st_as (bf16_t *p, size_t i, T v) { p[i].u = (uint16_t) f2bf ((float) v); }
 
-template <typename T, int W>
+// ---------------------------------------------------------------------------
+// dense coarse apply X0 = Ainv B0 (Ainv row-major n0 x n0; S storage of B, X)
+
+template <typename T, typename S, int W>
 __global__ void __launch_bounds__ (TPB)
-dense_apply_k (int n0, const T *__restrict__ Ainv, const T *__restrict__ B,
-               T *__restrict__ X)
+dense_apply_k (int n0, const T *__restrict__ Ainv, const S *__restrict__ B,
+               S *__restrict__ X)
 {
   constexpr int RPW = 64 / W;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -817,8 +938,8 @@ dense_apply_k (int n0, const T *__restrict__ Ainv, const T *__restrict__ B,
   const size_t off = (size_t) blockIdx.y * n0 * W + c;
   T acc = 0;
   for (int k = 0; k < n0; k++)
-    acc += Ainv[(size_t) row * n0 + k] * B[off + (size_t) k * W];
-  X[off + (size_t) row * W] = acc;
+    acc += Ainv[(size_t) row * n0 + k] * ld_as<T> (B, off + (size_t) k * W);
+  st_as<T> (X, off + (size_t) row * W, acc);
 }
 
 int
@@ -829,18 +950,22 @@ k_dense_apply (const void *Ainv, int n0, const PMat &B, PMat &X)
       constexpr int rows_per_block = (TPB / 64) * (64 / W);
       dim3 g (grid1 (n0, rows_per_block), B.npanels ());
       if (B.prec == F64)
-        hipLaunchKernelGGL ((dense_apply_k<double, W>), g, dim3 (TPB), 0,
+        hipLaunchKernelGGL ((dense_apply_k<double, double, W>), g, dim3 (TPB), 0,
                             stream (), n0, (const double *) Ainv,
                             (const double *) B.p, (double *) X.p);
-      else
-        hipLaunchKernelGGL ((dense_apply_k<float, W>), g, dim3 (TPB), 0,
+      else if (B.prec == F32)
+        hipLaunchKernelGGL ((dense_apply_k<float, float, W>), g, dim3 (TPB), 0,
                             stream (), n0, (const float *) Ainv,
                             (const float *) B.p, (float *) X.p);
+      else
+        hipLaunchKernelGGL ((dense_apply_k<float, bf16_t, W>), g, dim3 (TPB), 0,
+                            stream (), n0, (const float *) Ainv,
+                            (const bf16_t *) B.p, (bf16_t *) X.p);
       LGH_LAUNCH_CHECK ();
       return 0;
   };
-  LGH_CHECK (B.nrows == n0 && X.nrows == n0 && B.W == X.W, 8,
-             "k_dense_apply: shape mismatch");
+  LGH_CHECK (B.nrows == n0 && X.nrows == n0 && B.W == X.W && B.prec == X.prec,
+             8, "k_dense_apply: shape mismatch");
   return with_W (B.W, l);
 }
 
@@ -919,7 +1044,7 @@ copy_cols_k (int nrows, int nc, const TI *X, int xW, int xc0, TO *Y, int yW,
   int xj = xc0 + j, yj = yc0 + j;
   size_t xi = ((size_t) (xj / xW) * nrows + i) * xW + xj % xW;
   size_t yi = ((size_t) (yj / yW) * nrows + i) * yW + yj % yW;
-  Y[yi] = (TO) X[xi];
+  st_as<double> (Y, yi, ld_as<double> (X, xi));
 }
 
 int
@@ -933,10 +1058,16 @@ k_copy_cols (const PMat &X, int xc0, PMat &Y, int yc0, int nc)
   hipLaunchKernelGGL ((copy_cols_k<TI, TO>), dim3 (g), dim3 (TPB), 0,         \
                       stream (), X.nrows, nc, (const TI *) X.p, X.W, xc0,     \
                       (TO *) Y.p, Y.W, yc0)
-  if (X.prec == F64 && Y.prec == F64) LGH_CC (double, double);
-  else if (X.prec == F64) LGH_CC (double, float);
-  else if (Y.prec == F64) LGH_CC (float, double);
-  else LGH_CC (float, float);
+  auto same = [&] (int a, int b) { return X.prec == a && Y.prec == b; };
+  if (same (F64, F64)) LGH_CC (double, double);
+  else if (same (F64, F32)) LGH_CC (double, float);
+  else if (same (F32, F64)) LGH_CC (float, double);
+  else if (same (F32, F32)) LGH_CC (float, float);
+  else if (same (F64, BF16)) LGH_CC (double, bf16_t);
+  else if (same (F32, BF16)) LGH_CC (float, bf16_t);
+  else if (same (BF16, F32)) LGH_CC (bf16_t, float);
+  else if (same (BF16, F64)) LGH_CC (bf16_t, double);
+  else LGH_CC (bf16_t, bf16_t);
 #undef LGH_CC
   LGH_LAUNCH_CHECK ();
   return 0;

@@ -23,8 +23,11 @@
 
 namespace lgh {
 
-enum Prec { F64 = 0, F32 = 1 };
-inline size_t prec_bytes (int p) { return p == F64 ? 8 : 4; }
+// BF16 is a storage precision only (arithmetic in fp32): the vectors of
+// the BF16 V-cycle tier; its matrices are stored in fp32
+enum Prec { F64 = 0, F32 = 1, BF16 = 2 };
+inline size_t prec_bytes (int p) { return p == F64 ? 8 : p == F32 ? 4 : 2; }
+struct bf16_t { uint16_t u; };
 
 // ---- errors -------------------------------------------------------------
 void set_error (const std::string &msg);
@@ -138,11 +141,14 @@ int k_resid (const DevCSR &A, const PMat &B, const PMat &X, PMat &R);
 //   Zj = dinv_j R_j (jacobi) or Zsrc_j;  Dn = a D + b Z;
 //   X = X + Dn (x_mode 1) | X = Dn (x_mode 0);  Rn = R - A Dn
 // (D may be null when a == 0)
+// Zsrc may be BF16 beside FP32 R (W = 64); Rpc (optional, Zsrc's
+// precision) receives a copy of Rn for the next preconditioner application
 int k_cheb_step (const DevCSR &A, int jacobi, const PMat *Zsrc, double a,
                  double b, const PMat *D, const PMat &R, PMat &X, int x_mode,
-                 PMat &Dn, PMat &Rn);
-// mixed step at W = 64: FP64 R, X (A values FP64), FP32 D and Z; also
-// writes the new residual in FP32 (Rn32) for the next preconditioner.
+                 PMat &Dn, PMat &Rn, PMat *Rpc = nullptr);
+// mixed step at W = 64: FP64 R, X (A values FP64), FP32 D; the
+// preconditioned residual Zpc and the copy Rnpc of the new residual (the
+// next preconditioner input) are FP32 or BF16.
 // x_mode: how X takes the new direction Dn (D = the previous direction)
 enum {
   MIX_X_SET = 0,      // X = Dn
@@ -151,9 +157,9 @@ enum {
   MIX_X_SET2 = 3,     // X = D + Dn            (after a skipped first step)
   MIX_X_ADD2 = 4      // X = (X + D) + Dn      (after a skipped step)
 };
-int k_cheb_step_mixed (const DevCSR &A, const PMat &Z32, double a, double b,
+int k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                        const PMat *D32, const PMat &R, PMat &X, int x_mode,
-                       PMat &Dn32, PMat &Rn, PMat &Rn32);
+                       PMat &Dn32, PMat &Rn, PMat &Rnpc);
 // degree-2 Chebyshev-Jacobi smoother in one pass (W = 64):
 // D1 = b1 dinv R0, R1 = R0 - A D1; D2 = a2 D1 + b2 dinv R1;
 // X = (xadd ? X : 0) + D1 + D2 (D1, R1 recomputed at the neighbours)

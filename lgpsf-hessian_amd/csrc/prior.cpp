@@ -208,8 +208,7 @@ extern "C" void
 lgh_prior_destroy (lgh_prior_t *p)
 {
   if (p == NULL || --p->refs > 0) return;
-  delete p->mg[0];
-  delete p->mg[1];
+  for (auto *m : p->mg) delete m;
   dfree (p->d_msqrt); dfree (p->d_minvsqrt); dfree (p->d_minv);
   dfree (p->d_zdinv);
   VecDestroy (&p->msqrt);
@@ -283,10 +282,17 @@ lgh::prior_solve_block (lgh_prior_t *p, int trans, const PMat &X, PMat &Y,
   LGH_TRY (mat_device_csr (p->Z, X.prec, 1, &dz));
   const int mode = p->o.blocked_mode;
   if (mode < 2) return prior_solve_vec (p, trans, X, Y);
-  if (mode <= 3 && pcp == F32 && X.prec == F64 && X.W == 64) {
-    LGH_TRY (prior_setup_device (p, F32));
-    return cheb_solve_mixed (*dz, *p->mg[F32], p->lmin, p->lmax, p->nit, X, Y,
-                             p->cwm);
+  // a V-cycle in a lower precision than the vectors (W = 64): FP32 or BF16
+  // around FP64 vectors, BF16 around FP32 vectors
+  const bool lower = (X.prec == F64 && (pcp == F32 || pcp == BF16))
+                     || (X.prec == F32 && pcp == BF16);
+  if (mode <= 3 && p->amg_ok && lower && X.W == 64) {
+    if (!p->mg[pcp]) LGH_TRY (mg_upload (p->amg, pcp, &p->mg[pcp]));
+    if (X.prec == F64)
+      return cheb_solve_mixed (*dz, *p->mg[pcp], p->lmin, p->lmax, p->nit, X,
+                               Y, p->cwm);
+    return cheb_solve (*dz, p->mg[pcp], p->lmin, p->lmax, p->nit, X, Y,
+                       p->cw[X.prec]);
   }
   return cheb_solve (*dz, mode <= 3 ? p->mg[X.prec] : nullptr, p->lmin,
                      p->lmax, p->nit, X, Y, p->cw[X.prec]);
@@ -374,15 +380,19 @@ extern "C" int
 lgh_prior_solve_host (lgh_prior_t *p, int which, int prec, int m,
                       const double *X, int ldx, double *Y, int ldy)
 {
-  // prec: 0 FP64, 1 FP32, 2 FP64 vectors around the FP32 preconditioner
+  // prec: 0 FP64, 1 FP32, 2 FP64 vectors around the FP32 V-cycle,
+  // 3 FP64 vectors around the BF16 V-cycle, 4 FP32 around the BF16 V-cycle
   const int n = p->n;
-  const int vprec = (prec == 1) ? F32 : F64;
-  const int pcp = (prec == 2) ? F32 : -1;
+  if (prec < 0 || prec > 4)
+    return lgh_petsc_error (PETSC_ERR_ARG_OUTOFRANGE,
+                            "lgh_prior_solve_host: prec must be 0..4");
+  const int vprec = (prec == 1 || prec == 4) ? F32 : F64;
+  const int pcp = (prec == 2) ? F32 : (prec >= 3) ? BF16 : -1;
   std::vector<double> hx ((size_t) n * m);
   for (int j = 0; j < m; j++) memcpy (&hx[(size_t) j * n], X + (size_t) j * ldx, sizeof (double) * n);
   int W = 1;
   while (W < m && W < 64) W *= 2;
-  if (prec == 2) W = 64;
+  if (prec >= 2) W = 64;
   PMat t, x, y;
   int rc = prior_setup_device (p, vprec);
   if (!rc) rc = pmat_alloc (t, n, m, 1, F64);

@@ -29,6 +29,9 @@ mg_upload (const AmgHierarchy &h, int prec, DevMG **out)
 {
   DevMG *g = new DevMG ();
   g->prec = prec;
+  // BF16 is a vector storage precision: the matrices and the coarse inverse
+  // of that tier are fp32
+  const int mprec = (prec == BF16) ? F32 : prec;
   g->nl = (int) h.lev.size ();
   g->A.assign (g->nl, nullptr);
   g->P.assign (g->nl, nullptr);
@@ -41,23 +44,23 @@ mg_upload (const AmgHierarchy &h, int prec, DevMG **out)
     g->shi.push_back (lv.shi);
     g->nu.push_back (lv.nu);
     rc = csr_upload (lv.A.n, lv.A.m, lv.A.rp.data (), lv.A.ci.data (),
-                     lv.A.v.data (), prec, 1, &g->A[l]);
+                     lv.A.v.data (), mprec, 1, &g->A[l]);
     if (!rc && l > 0) {
       rc = csr_upload (lv.P.n, lv.P.m, lv.P.rp.data (), lv.P.ci.data (),
-                       lv.P.v.data (), prec, 0, &g->P[l]);
+                       lv.P.v.data (), mprec, 0, &g->P[l]);
       if (!rc) {
         HostCSR T = host_transpose (lv.P);
         rc = csr_upload (T.n, T.m, T.rp.data (), T.ci.data (), T.v.data (),
-                         prec, 0, &g->PT[l]);
+                         mprec, 0, &g->PT[l]);
       }
     }
   }
   if (!rc) {
     g->n0 = h.lev[0].A.n;
     const size_t cnt = (size_t) g->n0 * g->n0;
-    g->cinv = dmalloc (cnt * prec_bytes (prec));
+    g->cinv = dmalloc (cnt * prec_bytes (mprec));
     if (!g->cinv) rc = 5;
-    else if (prec == F64) rc = h2d (g->cinv, h.coarse_inv.data (), cnt * 8);
+    else if (mprec == F64) rc = h2d (g->cinv, h.coarse_inv.data (), cnt * 8);
     else {
       std::vector<float> f (h.coarse_inv.begin (), h.coarse_inv.end ());
       rc = h2d (g->cinv, f.data (), cnt * 4);
@@ -80,9 +83,11 @@ mg_work (DevMG &g, const PMat &like)
   for (int l = 0; l < g.nl; l++) {
     if (l > 0) {
       LGH_TRY (pmat_alloc (g.R1[l], g.n[l], nc, like.W, g.prec));
-      LGH_TRY (pmat_alloc (g.R2[l], g.n[l], nc, like.W, g.prec));
-      LGH_TRY (pmat_alloc (g.D1[l], g.n[l], nc, like.W, g.prec));
-      LGH_TRY (pmat_alloc (g.D2[l], g.n[l], nc, like.W, g.prec));
+      if (!(g.nu[l] == 2 && like.W == 64)) {   // the per-step smoother path
+        LGH_TRY (pmat_alloc (g.R2[l], g.n[l], nc, like.W, g.prec));
+        LGH_TRY (pmat_alloc (g.D1[l], g.n[l], nc, like.W, g.prec));
+        LGH_TRY (pmat_alloc (g.D2[l], g.n[l], nc, like.W, g.prec));
+      }
     }
     if (l < g.nl - 1) {
       LGH_TRY (pmat_alloc (g.Bl[l], g.n[l], nc, like.W, g.prec));
@@ -199,7 +204,7 @@ mg_vcycle (DevMG &g, const PMat &B, PMat &X)
 ChebWork::~ChebWork ()
 {
   pmat_free (R[0]); pmat_free (R[1]); pmat_free (D[0]); pmat_free (D[1]);
-  pmat_free (Z);
+  pmat_free (Z); pmat_free (Rpc); pmat_free (Zpc);
 }
 
 int
@@ -208,7 +213,7 @@ ChebWork::ensure (const PMat &like)
   if (W == like.W && np == like.npanels () && prec == like.prec
       && nrows == like.nrows) return 0;
   pmat_free (R[0]); pmat_free (R[1]); pmat_free (D[0]); pmat_free (D[1]);
-  pmat_free (Z);
+  pmat_free (Z); pmat_free (Rpc); pmat_free (Zpc);
   const int nc = like.npanels () * like.W;
   LGH_TRY (pmat_alloc (R[0], like.nrows, nc, like.W, like.prec));
   LGH_TRY (pmat_alloc (R[1], like.nrows, nc, like.W, like.prec));
@@ -225,6 +230,19 @@ cheb_solve (const DevCSR &A, DevMG *mg, double lmin, double lmax, int nit,
 {
   LGH_CHECK (nit >= 1, 8, "cheb_solve: nit < 1");
   LGH_TRY (w.ensure (B));
+  // a V-cycle in a lower storage precision (BF16 around FP32 vectors) reads
+  // and writes its own copies: Rpc (written by each step) and Zpc
+  const bool low = mg && mg->prec != B.prec;
+  if (low) {
+    LGH_CHECK (B.W == 64 && B.prec == F32 && mg->prec == BF16, 8,
+               "cheb_solve: unsupported preconditioner precision");
+    if (!w.Rpc.p) {
+      const int nc = B.npanels () * 64;
+      LGH_TRY (pmat_alloc (w.Rpc, B.nrows, nc, 64, mg->prec));
+      LGH_TRY (pmat_alloc (w.Zpc, B.nrows, nc, 64, mg->prec));
+    }
+    LGH_TRY (k_copy_cols (B, 0, w.Rpc, 0, B.npanels () * 64));
+  }
   ChebCoef cc (lmin, lmax);
   const PMat *Rsrc = &B;
   int ri = 0, di = 0;
@@ -232,7 +250,12 @@ cheb_solve (const DevCSR &A, DevMG *mg, double lmin, double lmax, int nit,
     double a, b;
     cc.step (k, &a, &b);
     const PMat *Dcur = (k == 1) ? nullptr : &w.D[di ^ 1];
-    if (mg) {
+    if (low) {
+      LGH_TRY (mg_vcycle (*mg, w.Rpc, w.Zpc));
+      LGH_TRY (k_cheb_step (A, 0, &w.Zpc, a, b, Dcur, *Rsrc, X, k > 1,
+                            w.D[di], w.R[ri], &w.Rpc));
+    }
+    else if (mg) {
       LGH_TRY (mg_vcycle (*mg, *Rsrc, w.Z));
       LGH_TRY (k_cheb_step (A, 0, &w.Z, a, b, Dcur, *Rsrc, X, k > 1, w.D[di],
                             w.R[ri]));
@@ -250,36 +273,38 @@ cheb_solve (const DevCSR &A, DevMG *mg, double lmin, double lmax, int nit,
 ChebMixedWork::~ChebMixedWork ()
 {
   pmat_free (R[0]); pmat_free (R[1]); pmat_free (D[0]); pmat_free (D[1]);
-  pmat_free (R32); pmat_free (Z32);
+  pmat_free (Rpc); pmat_free (Zpc);
 }
 
 int
-ChebMixedWork::ensure (const PMat &like)
+ChebMixedWork::ensure (const PMat &like, int pc)
 {
-  if (np == like.npanels () && nrows == like.nrows) return 0;
+  if (np == like.npanels () && nrows == like.nrows && pprec == pc) return 0;
   pmat_free (R[0]); pmat_free (R[1]); pmat_free (D[0]); pmat_free (D[1]);
-  pmat_free (R32); pmat_free (Z32);
+  pmat_free (Rpc); pmat_free (Zpc);
   const int nc = like.npanels () * 64;
   LGH_TRY (pmat_alloc (R[0], like.nrows, nc, 64, F64));
   LGH_TRY (pmat_alloc (R[1], like.nrows, nc, 64, F64));
   LGH_TRY (pmat_alloc (D[0], like.nrows, nc, 64, F32));
   LGH_TRY (pmat_alloc (D[1], like.nrows, nc, 64, F32));
-  LGH_TRY (pmat_alloc (R32, like.nrows, nc, 64, F32));
-  LGH_TRY (pmat_alloc (Z32, like.nrows, nc, 64, F32));
+  LGH_TRY (pmat_alloc (Rpc, like.nrows, nc, 64, pc));
+  LGH_TRY (pmat_alloc (Zpc, like.nrows, nc, 64, pc));
   np = like.npanels ();
   nrows = like.nrows;
+  pprec = pc;
   return 0;
 }
 
 int
-cheb_solve_mixed (const DevCSR &A64, DevMG &mg32, double lmin, double lmax,
+cheb_solve_mixed (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
                   int nit, const PMat &B, PMat &X, ChebMixedWork &w)
 {
   LGH_CHECK (nit >= 1 && B.W == 64 && B.prec == F64 && X.prec == F64
-             && mg32.prec == F32, 8, "cheb_solve_mixed: shape/precision");
-  LGH_TRY (w.ensure (B));
+             && (mg.prec == F32 || mg.prec == BF16), 8,
+             "cheb_solve_mixed: shape/precision");
+  LGH_TRY (w.ensure (B, mg.prec));
   ChebCoef cc (lmin, lmax);
-  LGH_TRY (k_copy_cols (B, 0, w.R32, 0, B.npanels () * 64));
+  LGH_TRY (k_copy_cols (B, 0, w.Rpc, 0, B.npanels () * 64));
   const PMat *Rsrc = &B;
   int ri = 0, di = 0;
   for (int k = 1; k <= nit; k++) {
@@ -292,9 +317,9 @@ cheb_solve_mixed (const DevCSR &A64, DevMG &mg32, double lmin, double lmax,
     if (k % 2 == 1 && k < nit) xm = MIX_X_SKIP;
     else if (k % 2 == 0) xm = (k == 2) ? MIX_X_SET2 : MIX_X_ADD2;
     else xm = (k == 1) ? MIX_X_SET : MIX_X_ADD;
-    LGH_TRY (mg_vcycle (mg32, w.R32, w.Z32));
-    LGH_TRY (k_cheb_step_mixed (A64, w.Z32, a, b, Dcur, *Rsrc, X, xm,
-                                w.D[di], w.R[ri], w.R32));
+    LGH_TRY (mg_vcycle (mg, w.Rpc, w.Zpc));
+    LGH_TRY (k_cheb_step_mixed (A64, w.Zpc, a, b, Dcur, *Rsrc, X, xm,
+                                w.D[di], w.R[ri], w.Rpc));
     Rsrc = &w.R[ri];
     ri ^= 1;
     di ^= 1;

@@ -29,25 +29,28 @@ int mg_vcycle (DevMG &g, const PMat &B, PMat &X);
 struct ChebWork {
   int W = 0, np = -1, prec = -1, nrows = 0;
   PMat R[2], D[2], Z;
+  PMat Rpc, Zpc;                       // lower-precision V-cycle in / out
   ~ChebWork ();
   int ensure (const PMat &like);
 };
 
 // X = Chebyshev(nit) solve of A X = B, zero initial guess
 // (zsolve_impl.h:104-141 with the blocked-asolve contract of :629-633);
-// preconditioner: the V-cycle when mg, else Jacobi (A.dinv)
+// preconditioner: the V-cycle when mg (in B's precision, or BF16 around
+// FP32 vectors at W = 64), else Jacobi (A.dinv)
 int cheb_solve (const DevCSR &A, DevMG *mg, double lmin, double lmax,
                 int nit, const PMat &B, PMat &X, ChebWork &w);
 
-// FP64-accurate Chebyshev around an FP32 V-cycle (W = 64 panels):
-// X, R in FP64; D, Z and the preconditioner in FP32 (k_cheb_step_mixed)
+// FP64-accurate Chebyshev around an FP32 or BF16 V-cycle (W = 64 panels):
+// X, R in FP64; D in FP32; the V-cycle's input and output (Rpc, Zpc) in
+// its own precision (k_cheb_step_mixed)
 struct ChebMixedWork {
-  int np = -1, nrows = 0;
-  PMat R[2], D[2], R32, Z32;
+  int np = -1, nrows = 0, pprec = -1;
+  PMat R[2], D[2], Rpc, Zpc;
   ~ChebMixedWork ();
-  int ensure (const PMat &like);
+  int ensure (const PMat &like, int pc);
 };
-int cheb_solve_mixed (const DevCSR &A64, DevMG &mg32, double lmin,
+int cheb_solve_mixed (const DevCSR &A64, DevMG &mg, double lmin,
                       double lmax, int nit, const PMat &B, PMat &X,
                       ChebMixedWork &w);
 

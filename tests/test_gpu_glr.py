@@ -149,18 +149,27 @@ def test_grid_blocked_solve_matches_oracle(dev, grid, oracle):
     assert rel(grid["Z"] @ ref, X) < 1e-8
     # FP32 throughout: FP32-level agreement
     assert rel(grid["prior"].solve(X, which=0, prec=1), ref) < 1e-4
-    # FP64 Chebyshev around the FP32 V-cycle: the same FP64-accurate Z^{-1}
-    mixed = grid["prior"].solve(X, which=0, prec=2)
-    assert rel(grid["Z"] @ mixed, X) < 1e-8
-    assert rel(mixed, ref) < 1e-8
+    # FP64 Chebyshev around the FP32 or BF16 V-cycle: the same FP64-accurate
+    # Z^{-1} (each step rounds its direction once and moves X and R by it)
+    for prec in (2, 3):
+        mixed = grid["prior"].solve(X, which=0, prec=prec)
+        assert rel(grid["Z"] @ mixed, X) < 1e-8, prec
+        assert rel(mixed, ref) < 1e-8, prec
+    # FP32 Chebyshev around the BF16 V-cycle: FP32-level agreement
+    assert rel(grid["prior"].solve(X, which=0, prec=4), ref) < 1e-4
 
 
-@pytest.mark.parametrize("mixed", [False, True])
-def test_grid_glr_matches_oracle(dev, grid, oracle, mixed):
+@pytest.mark.parametrize("policy", ["plain", "mixed", "mixed-fp32-vcycle"])
+def test_grid_glr_matches_oracle(dev, grid, oracle, policy):
     ell = 24
     og = oracle.GLR(oracle.CSR.from_scipy(grid["B"]), grid["M"], grid["zs"],
                     ell=ell, q_power=1, trunc_rel=1e-3, seed=0)
+    mixed = policy != "plain"
     x = dev.xopts(mixed=mixed)
+    if policy == "mixed-fp32-vcycle":
+        x.prec_precond = dev.PREC_F32
+    else:
+        assert not mixed or x.prec_precond == dev.PREC_BF16
     g = dev.GLR(grid["B"], grid["prior"], dev.glr_opts(ell=ell, seed=0), x=x)
     assert g.kept == og.kept
     if mixed:
@@ -218,6 +227,42 @@ def test_grid_extend_incremental(dev, grid, oracle):
     # leading (well-converged) modes agree
     k = min(5, g1.kept)
     assert np.max(np.abs(g2.lam[:k] - g1.lam[:k]) / np.abs(g1.lam[:k])) < 1e-6
+
+
+def test_cfg1_full_size_matches_oracle(dev, oracle):
+    """The CPU-runnable benchmark case at full size: 128 x 128 grid
+    (n = 16 384), Z = M + 0.1 K, symmetrized Gaussian PSF B (radius 4h),
+    ell = 64 + 10, seed 0, FP64; one solve and one sample at c = 1e-2 with
+    N(0,1) right-hand sides.  Engine vs the FP64 oracle on the same AMG
+    hierarchy; the mixed build vs the plain one."""
+    Bs, Zs, M = W.cfg1_problem(128)
+    n = Zs.shape[0]
+    prior = dev.Prior.from_mat(Zs, M, blocked_mode=3)
+    info = prior.info()
+    h = dev.AMG(Zs)
+    levels = [{"A": oracle.CSR.from_scipy(l["A"]),
+               "P": oracle.CSR.from_scipy(l["P"]) if l["P"] is not None else None,
+               "slo": l["slo"], "shi": l["shi"], "nu": l["nu"]} for l in h.levels()]
+    mg = oracle.MG(levels)
+    zs = oracle.ZSolver.mat(oracle.CSR.from_scipy(Zs), mode=3, mg=mg,
+                            lmin=info["lmin"], lmax=info["lmax"], nit=info["nit"],
+                            ksp_rtol=1e-12)
+    og = oracle.GLR(oracle.CSR.from_scipy(Bs), M, zs, ell=74, q_power=1, seed=0)
+    g = dev.GLR(Bs, prior, dev.glr_opts(ell=74, seed=0))
+    assert g.kept == og.kept and g.kept > 10
+    assert np.max(np.abs(g.lam - og.lam) / np.abs(og.lam)) < 1e-9
+    for key in ("lam_raw_min", "lam_raw_max", "next_abs"):
+        assert abs(g.report[key] - og.report[key]) <= 1e-9 * abs(og.report[key])
+    c = 1e-2
+    rhs = oracle.randn_fill(1, 0, 0, n, 1)[:, 0]
+    xi = oracle.randn_fill(2, 0, 0, n, 1)[:, 0]
+    assert rel(g.solve(c, rhs), og.solve(c, rhs)) < 1e-9
+    assert rel(g.sample(c, xi), og.sample(c, xi)) < 1e-9
+    assert abs(g.logdet(c) - og.logdet(c)) < 1e-9 * abs(og.logdet(c))
+    gm = dev.GLR(Bs, prior, dev.glr_opts(ell=74, seed=0), x=dev.xopts(mixed=True))
+    assert gm.kept == g.kept
+    assert np.max(np.abs(gm.lam - g.lam)) / g.lam[0] < 1e-6
+    assert rel(gm.solve(c, rhs), g.solve(c, rhs)) < 1e-4
 
 
 def test_c_consumer_of_public_api(tmp_path):

@@ -91,3 +91,5 @@ def test_vcycle_matches_oracle(dev, oracle):
     assert rel(h.vcycle(B), ref) < 1e-12
     assert rel(h.vcycle(B, prec=dev.PREC_F32), ref) < 1e-5
     assert rel(h.vcycle(B, W=4), ref) < 1e-12
+    # BF16 storage (8-bit mantissa, 2^-9 relative rounding per stored value)
+    assert rel(h.vcycle(B, prec=dev.PREC_BF16), ref) < 2e-2
