@@ -27,11 +27,14 @@ sys.path.insert(0, PKG_DIR)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 FP32_PEAK_TFS = 157.3
-# profiler kind -> the kernel instantiation it times (rocprofv3 name)
-KERNEL_OF = {"cheb_mixed": "lgh::cheb64m_k<true>",
-             "smooth_pre_fine": "lgh::smx_k<float, float, false>",
-             "smooth_post_fine": "lgh::smx_k<float, float, true>",
-             "resid_fine": "lgh::resid64_k<float, float>"}
+# profiler kind -> the kernel instantiation it times (rocprofv3 name), for
+# the V-cycle storage type of the FP64 sweeps
+def kernel_of(kind, vc):
+    st = {"bf16": "lgh::bf16_t", "f32": "float"}[vc]
+    return {"cheb_mixed": "lgh::cheb64m_k<true, %s>" % st,
+            "smooth_pre_fine": "lgh::smx_k<float, float, %s, false>" % st,
+            "smooth_post_fine": "lgh::smx_k<float, float, %s, true>" % st,
+            "resid_fine": "lgh::resid64_k<float, float, %s>" % st}[kind]
 
 
 def load_pkg():
@@ -153,8 +156,9 @@ def main():
     # one kernel instantiation at one launch shape per kind below: the
     # full-width (all ell columns) launches on the finest level
     npan = (args.ell + 63) // 64
+    vpc = pcode[args.vcycle]
     single = ("cheb_mixed", "smooth_pre_fine", "smooth_post_fine", "resid_fine")
-    prof = {k: lgh.prof_collect(k, npan if k in single else -1)
+    prof = {k: lgh.prof_collect(k, npan, vpc) if k in single else lgh.prof_collect(k)
             for k in lgh.PROF_KINDS}
     log("mixed build %.3fs kept=%d report=%s" % (t_mixed, g.kept, reps[-1]))
     for k, (c, ms, by, fl) in prof.items():
@@ -169,9 +173,9 @@ def main():
     ach = by / c / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
-            "kernel": KERNEL_OF[dom], "panels": npan, "avg_launch_ms": avg_ms,
+            "kernel": kernel_of(dom, args.vcycle), "panels": npan, "avg_launch_ms": avg_ms,
             "launches": c, "algorithmic_bytes_per_launch": by / c}
-    tr = pmc_traffic(KERNEL_OF[dom], npan)
+    tr = pmc_traffic(kernel_of(dom, args.vcycle), npan)
     if tr is not None:
         roof["traffic"] = tr
         roof["traffic_source"] = "profiles/cfg2_build_pmc_traffic.md"

@@ -39,11 +39,12 @@ void        lgh_hip_trim (void);
  * 5 Gram (MFMA), 6 GEMM (MFMA), 7 other, 8 mixed-precision Chebyshev step,
  * 9 / 10 fused smoother on the finest level (pre: zero guess / post),
  * 11 residual on the finest level.  collect synchronizes and resets that
- * kind's totals; np >= 0 counts only launches over np 64-column panels.
- * (No reference counterpart: measurement.) */
+ * kind's totals; np >= 0 counts only launches over np 64-column panels,
+ * prec >= 0 only launches whose vectors are stored in that precision
+ * (0 FP64, 1 FP32, 2 BF16).  (No reference counterpart: measurement.) */
 void lgh_hip_prof_enable (int on);
-int  lgh_hip_prof_collect (int kind, int np, int64_t *count, double *ms,
-                           double *bytes, double *flops);
+int  lgh_hip_prof_collect (int kind, int np, int prec, int64_t *count,
+                           double *ms, double *bytes, double *flops);
 
 /* the device hashed-Gaussian arithmetic evaluated on the host (no GPU) */
 double lgh_hip_randn_at_host (uint64_t seed, int64_t i, int64_t j);

@@ -158,7 +158,7 @@ _SIGS = {
     "lgh_hip_bytes_in_use": (C.c_int64, []),
     "lgh_hip_trim": (None, []),
     "lgh_hip_prof_enable": (None, [C.c_int]),
-    "lgh_hip_prof_collect": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int64), dp, dp, dp]),
+    "lgh_hip_prof_collect": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64), dp, dp, dp]),
     "lgh_hip_randn_at_host": (C.c_double, [C.c_uint64, C.c_int64, C.c_int64]),
     "lgh_hip_randn_fill": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.c_int,
                                      C.c_int, C.c_int, C.c_int, dp, C.c_int]),
@@ -486,12 +486,13 @@ def prof_enable(on=True):
     lib().lgh_hip_prof_enable(1 if on else 0)
 
 
-def prof_collect(kind, np_=-1):
+def prof_collect(kind, np_=-1, prec=-1):
     """(launches, total ms, algorithmic bytes, flops) since the last collect
-    of that kind; np_ >= 0 counts only launches over np_ panels."""
+    of that kind; np_ >= 0 / prec >= 0 count only launches over np_ panels /
+    with vectors stored in that precision."""
     c = C.c_int64()
     ms, by, fl = C.c_double(), C.c_double(), C.c_double()
-    _hip_check(lib().lgh_hip_prof_collect(PROF_KINDS.get(kind, kind), np_, C.byref(c),
+    _hip_check(lib().lgh_hip_prof_collect(PROF_KINDS.get(kind, kind), np_, prec, C.byref(c),
                                           C.byref(ms), C.byref(by), C.byref(fl)),
                "lgh_hip_prof_collect")
     return c.value, ms.value, by.value, fl.value

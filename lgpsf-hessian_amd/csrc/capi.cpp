@@ -63,12 +63,12 @@ void lgh_hip_trim (void) { dtrim (); }
 void lgh_hip_prof_enable (int on) { prof_enable (on); }
 
 int
-lgh_hip_prof_collect (int kind, int np, int64_t *count, double *ms,
+lgh_hip_prof_collect (int kind, int np, int prec, int64_t *count, double *ms,
                       double *bytes, double *flops)
 {
   LGH_CHECK (kind >= 0 && kind < PK_N, 8, "lgh_hip_prof_collect: bad kind");
   long c = 0;
-  LGH_TRY (prof_collect (kind, np, &c, ms, bytes, flops));
+  LGH_TRY (prof_collect (kind, np, prec, &c, ms, bytes, flops));
   *count = c;
   return 0;
 }

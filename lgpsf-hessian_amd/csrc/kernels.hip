@@ -535,7 +535,7 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
   ProfScope ps (D32 ? PK_CHEB_MIXED : PK_OTHER, R.npanels () * (csr_bytes (A)
                 + vec * (pb + (D32 ? 4 : 0) + 8 + xb + 4 + 8 + pb)),
                 R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * 64,
-                R.npanels ());
+                R.npanels (), Zpc.prec);
   const dim3 g = grid64 (A.n, R.npanels (), F64);
   auto go = [&] (auto sp) -> int {
     using SP = decltype (sp);
@@ -628,7 +628,7 @@ k_smooth2x (const DevCSR &A, double b1, double a2, double b2, const PMat &R0,
   ProfScope ps (kind, R0.npanels () * (csr_bytes (A) + (double) A.n * prec_bytes (A.prec)
                                       + (2 + (xadd ? 1 : 0)) * vec),
                 R0.npanels () * (2.0 * (double) A.nnz + 9.0 * A.n) * 64,
-                R0.npanels ());
+                R0.npanels (), R0.prec);
   const dim3 g = grid64 (A.n, R0.npanels (), R0.prec);
   auto go = [&] (auto tv, auto t, auto st) -> int {
     using TV = decltype (tv);
@@ -710,7 +710,7 @@ k_spmm (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
   ProfScope ps (prof_b_tag () ? PK_SPMM_B : PK_SPMM,
                 np_ * (csr_bytes (A) + (double) A.m * X.W * sx
                        + (double) A.n * X.W * sx * (beta != 0.0 ? 2 : 1)),
-                np_ * 2.0 * (double) A.nnz * X.W, (int) np_);
+                np_ * 2.0 * (double) A.nnz * X.W, (int) np_, X.prec);
   if (X.prec == BF16) {
     LGH_CHECK (A.prec == F32, 8, "k_spmm: bf16 vectors need fp32 values");
     return spmm_t<float, float, bf16_t> (A, alpha, X, beta, Y);
@@ -777,7 +777,7 @@ k_resid (const DevCSR &A, const PMat &B, const PMat &X, PMat &R)
              && X.nrows == A.n, 8, "k_resid: shape mismatch");
   ProfScope ps (prof_fine_tag () ? PK_RESID_FINE : PK_RESID,
                 X.npanels () * (csr_bytes (A) + 3.0 * A.n * X.W * prec_bytes (X.prec)),
-                X.npanels () * 2.0 * (double) A.nnz * X.W, X.npanels ());
+                X.npanels () * 2.0 * (double) A.nnz * X.W, X.npanels (), X.prec);
   if (X.prec == BF16) {
     LGH_CHECK (A.prec == F32, 8, "k_resid: bf16 vectors need fp32 values");
     return resid_t<float, float, bf16_t> (A, B, X, R);
@@ -897,7 +897,7 @@ k_cheb_step (const DevCSR &A, int jacobi, const PMat *Zsrc, double a,
                                 + ((jacobi ? 0 : 1) + (Rpc ? 1 : 0)) * vpc
                                 + (jacobi ? (double) A.n * prec_bytes (A.prec) : 0.0)),
                 R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * R.W,
-                R.npanels ());
+                R.npanels (), R.prec);
   if (pc == BF16)
     return A.prec == F64
         ? cheb_t<double, float, bf16_t> (A, jacobi, Zsrc, a, b, D, R, X, x_mode, Dn, Rn, Rpc)
@@ -1047,17 +1047,44 @@ copy_cols_k (int nrows, int nc, const TI *X, int xW, int xc0, TO *Y, int yW,
   st_as<double> (Y, yi, ld_as<double> (X, xi));
 }
 
+// whole panels of equal width: one contiguous range, 4 elements a thread
+template <typename TI, typename TO>
+__global__ void __launch_bounds__ (TPB)
+convert_k (size_t count, const TI *__restrict__ X, TO *__restrict__ Y)
+{
+  const size_t e = ((size_t) blockIdx.x * TPB + threadIdx.x) * 4;
+  if (e >= count) return;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    if (e + i < count) st_as<double> (Y, e + i, ld_as<double> (X, e + i));
+}
+
 int
 k_copy_cols (const PMat &X, int xc0, PMat &Y, int yc0, int nc)
 {
   LGH_CHECK (X.nrows == Y.nrows && xc0 + nc <= X.npanels () * X.W
              && yc0 + nc <= Y.npanels () * Y.W, 8, "k_copy_cols: range");
   if (nc <= 0) return 0;
+  // same panel width, panel-aligned, and either whole panels or running to
+  // the logical end of both (the rest of the last panel is zero padding in
+  // both): a straight conversion of one contiguous range
+  const bool flat = X.W == Y.W && xc0 % X.W == 0 && yc0 % Y.W == 0
+                    && (nc % X.W == 0 || (xc0 + nc >= X.ncols && yc0 + nc >= Y.ncols));
+  const size_t cnt = (size_t) ((nc + X.W - 1) / X.W) * X.nrows * X.W;
+  const unsigned gf = grid1 ((cnt + 3) / 4, TPB);
   unsigned g = grid1 ((size_t) X.nrows * nc, TPB);
 #define LGH_CC(TI, TO)                                                        \
-  hipLaunchKernelGGL ((copy_cols_k<TI, TO>), dim3 (g), dim3 (TPB), 0,         \
-                      stream (), X.nrows, nc, (const TI *) X.p, X.W, xc0,     \
-                      (TO *) Y.p, Y.W, yc0)
+  do {                                                                        \
+    if (flat)                                                                 \
+      hipLaunchKernelGGL ((convert_k<TI, TO>), dim3 (gf), dim3 (TPB), 0,      \
+                          stream (), cnt,                                     \
+                          (const TI *) X.panel (xc0 / X.W),                   \
+                          (TO *) Y.panel (yc0 / Y.W));                        \
+    else                                                                      \
+      hipLaunchKernelGGL ((copy_cols_k<TI, TO>), dim3 (g), dim3 (TPB), 0,     \
+                          stream (), X.nrows, nc, (const TI *) X.p, X.W, xc0, \
+                          (TO *) Y.p, Y.W, yc0);                              \
+  } while (0)
   auto same = [&] (int a, int b) { return X.prec == a && Y.prec == b; };
   if (same (F64, F64)) LGH_CC (double, double);
   else if (same (F64, F32)) LGH_CC (double, float);

@@ -5,9 +5,10 @@
 // panels of width W (W in {1,2,4,...,64}); panel p is an nrows x W
 // row-major slab, so element (i, j) lives at
 //     ((j / W) * nrows + i) * W + (j % W).
-// With W = 64 a panel row is 256 B (fp32) / 512 B (fp64) contiguous: 16 or
-// 32 lanes load it as 16 B per lane, so every CSR neighbour gather is one
-// coalesced row load, and elementwise work streams contiguously.  Columns
+// With W = 64 a panel row is 128 B (bf16) / 256 B (fp32) / 512 B (fp64)
+// contiguous: 8, 16 or 32 lanes load it as 16 B per lane, so every CSR
+// neighbour gather is one coalesced row load, and elementwise work streams
+// contiguously.  Columns
 // past ncols in the last panel are padding and are kept at zero by every
 // kernel.
 //
@@ -86,15 +87,17 @@ int  prof_b_tag ();
 void prof_set_fine_tag (int on);   // next launches run on the finest level
 int  prof_fine_tag ();
 struct ProfScope {
-  int kind, np; double bytes, flops; int active; void *ev0;
-  // np: panels of the launch (recorded so one launch shape can be selected)
-  ProfScope (int kind, double bytes, double flops, int np = 0);
+  int kind, np, prec; double bytes, flops; int active; void *ev0;
+  // np, prec: panels and vector storage precision of the launch (recorded
+  // so that one kernel instantiation at one shape can be selected)
+  ProfScope (int kind, double bytes, double flops, int np = 0, int prec = -1);
   ~ProfScope ();
 };
 // totals since the last collect of that kind (synchronizes the stream);
-// np >= 0 counts only launches over np panels (the others are dropped too)
-int prof_collect (int kind, int np, long *count, double *ms, double *bytes,
-                  double *flops);
+// np >= 0 / prec >= 0 count only launches over np panels / at that storage
+// precision (the others are dropped too)
+int prof_collect (int kind, int np, int prec, long *count, double *ms,
+                  double *bytes, double *flops);
 
 // ---- objects ------------------------------------------------------------
 struct DevCSR {
