@@ -82,6 +82,90 @@ def cpu_baseline(B, Z, M, lgh, ell, q, kept, budget_s):
     return mg, levels
 
 
+def serve_bench(lgh, g, n, nreq, pool, seed=5):
+    """Request loop on the resident mixed GLR: 60% solve / 35% sample / 5%
+    logdet, c log-uniform in [1e-4, 10], RHS count geometric (mean 8)
+    truncated to [1, 64], entries from a pool of hashed-Gaussian columns.
+    First a burst of nreq/3 requests measures saturation, then nreq
+    requests arrive as a Poisson process at 0.7 x the saturated rate."""
+    import threading
+    rng = np.random.default_rng(seed)
+
+    def make(k):
+        out = []
+        for _ in range(k):
+            u = rng.uniform()
+            op = lgh.SERVE_SOLVE if u < 0.6 else lgh.SERVE_SAMPLE if u < 0.95 else lgh.SERVE_LOGDET
+            m = int(min(64, rng.geometric(1.0 / 8.0)))
+            off = int(rng.integers(0, 64 - m + 1))
+            out.append((op, 10.0 ** rng.uniform(-4, 1), pool[:, off:off + m]))
+        return out
+
+    srv = lgh.Serve(g)
+    free = {}                       # reusable result blocks by width
+    flock = threading.Lock()
+
+    def take(m):
+        with flock:
+            lst = free.get(m)
+            if lst:
+                return lst.pop()
+        return np.empty((n, m), order="F")
+
+    def give(a):
+        if a is not None and not isinstance(a, float):
+            with flock:
+                free.setdefault(a.shape[1], []).append(a)
+
+    def run(reqs, times):
+        """submit reqs at times (s from start) on this thread, collect on
+        another; returns (elapsed s, latencies, RHS columns)"""
+        import queue
+        lat, cols = [None] * len(reqs), 0
+        qq = queue.Queue()
+
+        def waiter():
+            for _ in range(len(reqs)):
+                i, t = qq.get()
+                res, lat[i] = srv.wait(t)
+                give(res)
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        for i, ((op, c, X), ta) in enumerate(zip(reqs, times)):
+            dt = t0 + ta - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            if op == lgh.SERVE_LOGDET:
+                qq.put((i, srv.submit(op, c)))
+            else:
+                cols += X.shape[1]
+                qq.put((i, srv.submit(op, c, X, out=take(X.shape[1]))))
+        th.join()
+        return time.perf_counter() - t0, np.array(lat), cols
+
+    warm = make(8)
+    run(warm, [0.0] * len(warm))
+    burst = make(max(nreq // 3, 16))
+    t_b, lat_b, cols_b = run(burst, [0.0] * len(burst))
+    sat_rps = len(burst) / t_b
+    rate = 0.7 * sat_rps
+    reqs = make(nreq)
+    arr = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
+    t_p, lat_p, cols_p = run(reqs, list(arr))
+    st = srv.stats()
+    srv.close()
+    return {"requests": nreq, "saturation_req_per_s": sat_rps,
+            "saturation_rhs_per_s": cols_b / t_b,
+            "poisson_rate_req_per_s": rate,
+            "rhs_per_s": cols_p / t_p, "req_per_s": nreq / t_p,
+            "latency_p50_s": float(np.percentile(lat_p, 50)),
+            "latency_p99_s": float(np.percentile(lat_p, 99)),
+            "batch_cols": st["batch_cols"], "batches": st["batches"],
+            "mean_batch_cols": st["columns"] / max(st["batches"], 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +181,11 @@ def main():
                     help="V-cycle storage precision inside the FP64 sweeps")
     ap.add_argument("--sketch-vcycle", choices=("bf16", "f32"), default="f32",
                     help="V-cycle storage precision inside the FP32 sweeps")
+    ap.add_argument("--sketch", choices=("f32", "f64"), default=None,
+                    help="vector precision of the sketch sweep (default: the "
+                         "mixed policy's)")
+    ap.add_argument("--serve-requests", type=int, default=300,
+                    help="requests of the Poisson serving run (0: skip)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -132,6 +221,8 @@ def main():
     pcode = {"bf16": lgh.PREC_BF16, "f32": lgh.PREC_F32}
     mixed = lgh.xopts(mixed=True, prec_precond=pcode[args.vcycle],
                       prec_sketch_precond=pcode[args.sketch_vcycle])
+    if args.sketch:
+        mixed.prec_sketch = {"f32": lgh.PREC_F32, "f64": lgh.PREC_F64}[args.sketch]
     plain = lgh.xopts(mixed=False)
 
     for w in range(args.warmup):
@@ -207,11 +298,11 @@ def main():
         log("plain build %.3fs kept=%d; mixed-vs-fp64 spectrum rel err %.2e"
             % (t_plain, gp.kept, spec_err))
 
-    # downstream: batched multi-shift solves (host arrays)
-    import oracle as O
+    # downstream: batched multi-shift solves (host arrays); right-hand sides
+    # from the device hashed Gaussian
     rng = np.random.default_rng(0)
     m = 64
-    R = O.randn_fill(99, 0, 0, n, m)
+    R = lgh.randn_fill(99, 0, 0, n, m)
     cs = 10.0 ** rng.uniform(-4, 1, size=m)
     X = g.solve_block(cs, R)
     t0 = time.perf_counter()
@@ -226,11 +317,25 @@ def main():
                  solve_latency_p50_s=float(np.percentile(lat, 50)),
                  solve_latency_p99_s=float(np.percentile(lat, 99)))
     if not args.no_plain:
+        t0 = time.perf_counter()
         Xp = gp.solve_block(cs, R)
+        extra["plain_solve_rhs_per_s"] = m / (time.perf_counter() - t0)
         extra["mixed_vs_fp64_solve_rel"] = float(
             np.max(np.linalg.norm(X - Xp, axis=0) / np.linalg.norm(Xp, axis=0)))
+        S = g.sample_block(cs, R)
+        Sp = gp.sample_block(cs, R)
+        extra["mixed_vs_fp64_sample_rel"] = float(
+            np.max(np.linalg.norm(S - Sp, axis=0) / np.linalg.norm(Sp, axis=0)))
+        ld = np.array([g.logdet(c) for c in cs])
+        ldp = np.array([gp.logdet(c) for c in cs])
+        extra["mixed_vs_fp64_logdet_rel"] = float(np.max(np.abs(ld - ldp) / np.abs(ldp)))
+        del S, Sp, Xp
     log("solve block m=%d: %.1f RHS/s; single-RHS p50 %.3fs" %
         (m, m / t_blk, extra["solve_latency_p50_s"]))
+    del X
+    if args.serve_requests > 0:
+        extra["serve"] = serve_bench(lgh, g, n, args.serve_requests, R)
+        log("serve:", extra["serve"])
 
     cpu = None
     if not args.no_cpu:

@@ -5,6 +5,8 @@
 #ifndef LGPSF_HESSIAN_LGH_EXT_H
 #define LGPSF_HESSIAN_LGH_EXT_H
 
+#include <stdint.h>
+
 #include "lgpsf_hessian/glr.h"
 
 #ifdef __cplusplus
@@ -17,15 +19,16 @@ extern "C" {
 
 /* Precision policy of the build.  All LGH_PREC_F64 reproduces the
  * reference arithmetic ("plain").  The mixed policy (lgh_glr_xopts_mixed)
- * runs the sketch sweep Y = F Omega with FP32 vectors, B, Z and V-cycle
- * (its rounding acts like a perturbation of the Gaussian sketch), runs
- * every later sweep with FP64 vectors inside an FP64 Chebyshev iteration
- * whose AMG V-cycle stores its vectors in BF16 (the Chebyshev step rounds
- * each new direction once and updates X and R with that same value, so the
- * Z solves stay FP64-accurate), keeps CholQR2, T = Q^T F Q, the
- * eigensolve and U in FP64, and runs the residual-gate sweep in FP32.
- * A V-cycle precision above its sweep's vector precision means "the
- * vectors' own". */
+ * runs every sweep of the build with FP64 vectors and B inside an FP64
+ * Chebyshev iteration whose AMG V-cycle stores its vectors in BF16 (FP32
+ * matrices and arithmetic; the Chebyshev step rounds each new direction
+ * once and updates X and R with that same value, so the Z solves stay
+ * FP64-accurate), keeps CholQR2, T = Q^T F Q, the eigensolve and U in
+ * FP64, runs the residual-gate sweep in FP32, and takes batched
+ * downstream calls of 8 or more columns through the same blocked tier.
+ * prec_sketch = F32 runs the sketch sweep with FP32 vectors, B and Z
+ * instead (its V-cycle in prec_sketch_precond).  A V-cycle precision
+ * above its sweep's vector precision means "the vectors' own". */
 typedef struct lgh_glr_xopts
 {
   int prec_sketch;          /* sketch sweep Y = F Omega: F64 or F32        */
@@ -34,6 +37,12 @@ typedef struct lgh_glr_xopts
   int prec_check;           /* residual-gate sweep (opts.check): F64, F32  */
   int prec_sketch_precond;  /* V-cycle inside FP32-vector sweeps (sketch,
                                check): F32 (or F64) or BF16                */
+  int solve_blocked_min;    /* downstream solve / sample / factor calls on
+                               at least this many columns use the blocked
+                               tier (FP64 Chebyshev around the prec_precond
+                               V-cycle, 64-column panels) for their Z
+                               solves instead of per-column PCG to
+                               ksp_rtol; 0 = never (the reference path)    */
 }
 lgh_glr_xopts_t;
 
@@ -73,6 +82,54 @@ int lgh_prior_get_info (const lgh_prior_t *prior, lgh_prior_info_t *info);
  * 3 FP64 vectors around the BF16 V-cycle, 4 FP32 around the BF16 V-cycle. */
 int lgh_prior_solve_host (lgh_prior_t *prior, int which, int prec, int m,
                           const double *X, int ldx, double *Y, int ldy);
+
+/* Request loop over one resident GLR: callers on any thread submit solve /
+ * sample / logdet requests (shift c, m right-hand sides, column-major host
+ * blocks that stay valid until the wait returns); one worker thread forms
+ * batches of the oldest request's operation from everything queued (each
+ * column with its own shift), so requests that arrive while a batch runs
+ * join the next one.  The batch width is capped by a device workspace
+ * budget; submit blocks while more than queue_cols columns are queued.
+ * A request's result does not depend on what it was batched with.  While a
+ * server exists, the GLR must not be used or destroyed by other calls.
+ * (No reference counterpart: the reference's operations are synchronous,
+ * one Vec at a time.) */
+typedef struct lgh_serve lgh_serve_t;
+
+#define LGH_SERVE_SOLVE  0   /* x = H(c)^{-1} rhs          (lgh_glr_solve)  */
+#define LGH_SERVE_SAMPLE 1   /* x = G(c)^{-T} xi           (lgh_glr_sample) */
+#define LGH_SERVE_LOGDET 2   /* logdet(I + F / c)          (lgh_glr_logdet) */
+
+typedef struct lgh_serve_opts
+{
+  int    max_batch_cols;    /* columns per device batch (multiple of 64)    */
+  double device_budget_gb;  /* workspace budget of one batch; caps the width */
+  int    queue_cols;        /* queued columns before submit blocks          */
+}
+lgh_serve_opts_t;
+
+typedef struct lgh_serve_stats
+{
+  int64_t batches, columns, requests;
+  int     max_batch_cols;   /* widest batch run so far                      */
+  int     batch_cols;       /* the admitted batch width                      */
+  double  busy_s;           /* worker time spent in batches                  */
+}
+lgh_serve_stats_t;
+
+lgh_serve_opts_t lgh_serve_opts_default (void);
+int  lgh_serve_create (lgh_glr_t *glr, const lgh_serve_opts_t *opts,
+                       lgh_serve_t **server);
+/* m, in, ldi, out, ldo are ignored for LGH_SERVE_LOGDET */
+int  lgh_serve_submit (lgh_serve_t *server, int op, double c, int m,
+                       const double *in, int ldi, double *out, int ldo,
+                       int64_t *ticket);
+/* blocks until the request is done; logdet (LGH_SERVE_LOGDET) and the
+ * submit-to-done latency in seconds are returned when non-NULL */
+int  lgh_serve_wait (lgh_serve_t *server, int64_t ticket, double *logdet,
+                     double *latency_s);
+int  lgh_serve_get_stats (lgh_serve_t *server, lgh_serve_stats_t *stats);
+void lgh_serve_destroy (lgh_serve_t *server);
 
 /* copy of the kept U (n x kept, column-major) for tests */
 int lgh_glr_get_U (const lgh_glr_t *glr, double *U, int ldu);

@@ -67,7 +67,22 @@ class GlrReport(C.Structure):
 
 class GlrXOpts(C.Structure):
     _fields_ = [("prec_sketch", C.c_int), ("prec_precond", C.c_int),
-                ("prec_check", C.c_int), ("prec_sketch_precond", C.c_int)]
+                ("prec_check", C.c_int), ("prec_sketch_precond", C.c_int),
+                ("solve_blocked_min", C.c_int)]
+
+
+class ServeOpts(C.Structure):
+    _fields_ = [("max_batch_cols", C.c_int), ("device_budget_gb", C.c_double),
+                ("queue_cols", C.c_int)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("batches", C.c_int64), ("columns", C.c_int64),
+                ("requests", C.c_int64), ("max_batch_cols", C.c_int),
+                ("batch_cols", C.c_int), ("busy_s", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class PriorMatOpts(C.Structure):
@@ -151,6 +166,13 @@ _SIGS = {
     "lgh_prior_solve_host": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp,
                                        C.c_int, dp, C.c_int]),
     "lgh_glr_get_U": (C.c_int, [vp, dp, C.c_int]),
+    "lgh_serve_opts_default": (ServeOpts, []),
+    "lgh_serve_create": (C.c_int, [vp, C.POINTER(ServeOpts), C.POINTER(vp)]),
+    "lgh_serve_submit": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, dp, C.c_int,
+                                   dp, C.c_int, C.POINTER(C.c_int64)]),
+    "lgh_serve_wait": (C.c_int, [vp, C.c_int64, dp, dp]),
+    "lgh_serve_get_stats": (C.c_int, [vp, C.POINTER(ServeStats)]),
+    "lgh_serve_destroy": (None, [vp]),
     # kernel ABI
     "lgh_hip_init": (C.c_int, []),
     "lgh_hip_last_error": (C.c_char_p, []),
@@ -469,6 +491,70 @@ class GLR:
         if getattr(self, "h", None) and _lib is not None:
             _lib.lgh_glr_destroy(self.h)
             self.h = None
+
+
+SERVE_SOLVE, SERVE_SAMPLE, SERVE_LOGDET = 0, 1, 2
+
+
+class Serve:
+    """Request loop over a resident GLR (lgh_serve_*).  submit() returns a
+    ticket; the input block is kept alive here until wait() returns the
+    result (solve / sample: an n x m array; logdet: a float) and the
+    submit-to-done latency in seconds.  Submit and wait may be called from
+    several Python threads (the ctypes calls release the GIL)."""
+
+    def __init__(self, glr, **kw):
+        o = lib().lgh_serve_opts_default()
+        for k, v in kw.items():
+            setattr(o, k, v)
+        h = vp()
+        _petsc_check(lib().lgh_serve_create(glr.h, C.byref(o), C.byref(h)),
+                     "lgh_serve_create")
+        self.h, self.glr, self.n = h, glr, glr.n
+        self._live = {}
+        import threading
+        self._lock = threading.Lock()
+
+    def submit(self, op, c, X=None, out=None):
+        """out: optional preallocated n x m Fortran-ordered float64 result"""
+        t = C.c_int64()
+        if op == SERVE_LOGDET:
+            _petsc_check(lib().lgh_serve_submit(self.h, op, c, 0, None, 0, None, 0,
+                                                C.byref(t)), "lgh_serve_submit")
+            out = None
+        else:
+            X = _fcol(X)
+            if out is None:
+                out = np.empty_like(X, order="F")
+            if out.shape != X.shape or not out.flags.f_contiguous or out.dtype != np.float64:
+                raise ValueError("Serve.submit: out must be an n x m float64 Fortran array")
+            _petsc_check(lib().lgh_serve_submit(self.h, op, c, X.shape[1], _d(X),
+                                                X.shape[0], _d(out), X.shape[0],
+                                                C.byref(t)), "lgh_serve_submit")
+        with self._lock:
+            self._live[t.value] = (X, out)
+        return t.value
+
+    def wait(self, ticket):
+        ld, lat = C.c_double(), C.c_double()
+        _petsc_check(lib().lgh_serve_wait(self.h, ticket, C.byref(ld), C.byref(lat)),
+                     "lgh_serve_wait")
+        with self._lock:
+            X, out = self._live.pop(ticket)
+        return (ld.value if out is None else out), lat.value
+
+    def stats(self):
+        s = ServeStats()
+        _petsc_check(lib().lgh_serve_get_stats(self.h, C.byref(s)), "lgh_serve_get_stats")
+        return s.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.lgh_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 # ---- kernel ABI -------------------------------------------------------------------

@@ -50,11 +50,15 @@ namespace lgh {
 // device-side prior operations on panel matrices (precision = X.prec)
 int prior_setup_device (lgh_prior_t *p, int prec);
 // trans: 1 selects the transposed callbacks (Z symmetric on the Mat path)
-// pcp: preconditioner precision (F32 with FP64 X selects the mixed
-// Chebyshev around the FP32 V-cycle; -1: X's precision)
+// pcp: V-cycle precision when lower than X's (F32 or BF16 around FP64 X:
+// the mixed Chebyshev; BF16 around FP32 X); -1: X's precision
 int prior_solve_block (lgh_prior_t *p, int trans, const PMat &X, PMat &Y,
                        int pcp = -1);
 int prior_solve_vec (lgh_prior_t *p, int trans, const PMat &X, PMat &Y);
 int prior_apply_Z (lgh_prior_t *p, int trans, const PMat &X, PMat &Y);
 void prior_ref (lgh_prior_t *p);
+// batched solve (op 0) or sample (op 3) of m host columns, column j at
+// shift c[j]; tier: -1 by the GLR's policy, 0 per-column PCG, 1 blocked
+int glr_run_block (lgh_glr_t *g, int op, int m, const double *c,
+                   const double *in, int ldi, double *out, int ldo, int tier);
 }

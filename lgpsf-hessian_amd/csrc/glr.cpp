@@ -42,6 +42,7 @@ lgh_glr_xopts_default (void)
   x.prec_precond = LGH_PREC_F64;
   x.prec_check = LGH_PREC_F64;
   x.prec_sketch_precond = LGH_PREC_F64;
+  x.solve_blocked_min = 0;
   return x;
 }
 
@@ -49,10 +50,11 @@ extern "C" lgh_glr_xopts_t
 lgh_glr_xopts_mixed (void)
 {
   lgh_glr_xopts_t x;
-  x.prec_sketch = LGH_PREC_F32;
+  x.prec_sketch = LGH_PREC_F64;
   x.prec_precond = LGH_PREC_BF16;
   x.prec_check = LGH_PREC_F32;
   x.prec_sketch_precond = LGH_PREC_F32;
+  x.solve_blocked_min = 8;
   return x;
 }
 
@@ -457,7 +459,8 @@ lgh_glr_compute_x (Mat B, lgh_prior_t *prior, const lgh_glr_opts_t *opts,
   auto vec_ok = [] (int q) { return q == LGH_PREC_F64 || q == LGH_PREC_F32; };
   auto pc_ok = [&] (int q) { return vec_ok (q) || q == LGH_PREC_BF16; };
   PetscCheck (vec_ok (xo.prec_sketch) && vec_ok (xo.prec_check)
-              && pc_ok (xo.prec_precond) && pc_ok (xo.prec_sketch_precond),
+              && pc_ok (xo.prec_precond) && pc_ok (xo.prec_sketch_precond)
+              && xo.solve_blocked_min >= 0,
               0, PETSC_ERR_ARG_OUTOFRANGE,
               "lgh_glr_compute_x: precision codes out of range");
   if (dev_init ()) return API_FAIL ("lgh_glr_compute");
@@ -700,30 +703,49 @@ coeffs (lgh_glr_t *g, const OpSpec &op, int m, const double *c,
   }
 }
 
+// tier: -1 by the policy (solve_blocked_min), 0 per-column PCG (the
+// reference's vector path), 1 the blocked tier (assembled Z, modes >= 2).
+// An explicit tier also fixes the panel width at 64, so that a column's
+// arithmetic does not depend on how many columns share the call.
 static int
 run_op (lgh_glr_t *g, const OpSpec &op, int m, const double *c,
-        const double *in, int ldi, double *out, int ldo)
+        const double *in, int ldi, double *out, int ldo, int tier = -1)
 {
   lgh_prior_t *p = g->prior;
-  const int n = g->n, Wz = width_for (m);
+  const bool can_block = p->is_mat && p->o.blocked_mode >= 2;
+  const bool fixed_w = tier >= 0;
+  if (tier < 0)
+    tier = (g->x.solve_blocked_min > 0 && m >= g->x.solve_blocked_min) ? 1 : 0;
+  const bool blocked = tier == 1 && can_block;
+  const int n = g->n, Wz = (blocked || fixed_w) ? 64 : width_for (m);
+  auto zsolve = [&] (int trans, const PMat &X, PMat &Y) -> int {
+    if (blocked) return prior_solve_block (p, trans, X, Y, pc_for (g, F64));
+    return prior_solve_vec (p, trans, X, Y);
+  };
   LGH_TRY (prior_setup_device (p, F64));
   std::vector<double> fk, fc;
   coeffs (g, op, m, c, fk, fc);
   PMat H, A, Bm, X64, Y64;
-  std::vector<double> hb ((size_t) n * m);
-  for (int j = 0; j < m; j++) memcpy (&hb[(size_t) j * n], in + (size_t) j * ldi, sizeof (double) * n);
+  // contiguous host blocks (ld == n) move directly; others through hb
+  std::vector<double> hb;
+  const double *hin = in;
+  if (ldi != n || ldo != n) hb.resize ((size_t) n * m);
+  if (ldi != n) {
+    for (int j = 0; j < m; j++) memcpy (&hb[(size_t) j * n], in + (size_t) j * ldi, sizeof (double) * n);
+    hin = hb.data ();
+  }
   int rc = pmat_alloc (H, n, m, 1, F64);
   if (!rc) rc = pmat_alloc (A, n, m, Wz, F64);
   if (!rc) rc = pmat_alloc (Bm, n, m, Wz, F64);
   if (!rc) rc = pmat_alloc (X64, n, m, 64, F64);
   if (!rc) rc = pmat_alloc (Y64, n, m, 64, F64);
-  if (!rc) rc = h2d (H.p, hb.data (), sizeof (double) * hb.size ());
+  if (!rc) rc = h2d (H.p, hin, sizeof (double) * (size_t) n * m);
   if (!rc) rc = k_copy_cols (H, 0, A, 0, m);
   // left part (before the filter): into X64
   if (!rc) {
     switch (op.kind) {
     case 0: case 2:                    // M^{1/2} Z^{-1} in
-      rc = prior_solve_vec (p, 0, A, Bm);
+      rc = zsolve (0, A, Bm);
       if (!rc) rc = k_diag_scale (Bm, p->d_msqrt, Bm);
       if (!rc) rc = k_copy_cols (Bm, 0, X64, 0, m);
       break;
@@ -743,7 +765,7 @@ run_op (lgh_glr_t *g, const OpSpec &op, int m, const double *c,
     switch (op.kind) {
     case 0: case 3:                    // Z^{-T} M^{1/2} (.)
       rc = k_diag_scale (A, p->d_msqrt, A);
-      if (!rc) rc = prior_solve_vec (p, 1, A, Bm);
+      if (!rc) rc = zsolve (1, A, Bm);
       if (!rc) rc = k_copy_cols (Bm, 0, H, 0, m);
       break;
     case 0x10:                         // Z M^{-1/2} (.)
@@ -756,11 +778,21 @@ run_op (lgh_glr_t *g, const OpSpec &op, int m, const double *c,
       break;
     }
   }
-  if (!rc) rc = d2h (hb.data (), H.p, sizeof (double) * hb.size ());
+  double *hout = (ldo == n) ? out : hb.data ();
+  if (!rc) rc = d2h (hout, H.p, sizeof (double) * (size_t) n * m);
   pmat_free (H); pmat_free (A); pmat_free (Bm); pmat_free (X64); pmat_free (Y64);
   LGH_TRY (rc);
-  for (int j = 0; j < m; j++) memcpy (out + (size_t) j * ldo, &hb[(size_t) j * n], sizeof (double) * n);
+  if (ldo != n)
+    for (int j = 0; j < m; j++) memcpy (out + (size_t) j * ldo, &hb[(size_t) j * n], sizeof (double) * n);
   return 0;
+}
+
+int
+lgh::glr_run_block (lgh_glr_t *g, int op, int m, const double *c,
+                    const double *in, int ldi, double *out, int ldo, int tier)
+{
+  LGH_CHECK (op == 0 || op == 3, 8, "glr_run_block: op must be solve or sample");
+  return run_op (g, OpSpec { op, 0, nullptr, 0 }, m, c, in, ldi, out, ldo, tier);
 }
 
 static PetscErrorCode

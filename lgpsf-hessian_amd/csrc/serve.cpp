@@ -1,0 +1,310 @@
+// serve.cpp — request loop over one resident GLR (no reference counterpart:
+// the reference's downstream calls are one synchronous Vec at a time,
+// glr_common_impl.h:1007-1104).
+//
+// Callers submit solve / sample / logdet requests (a shift c and m
+// right-hand sides each) from any thread; one worker thread owns the GLR
+// and the device.  Whenever the device is free it forms the next batch from
+// everything queued: all requests of the oldest request's operation that
+// fit in the batch's columns, each column with its own shift.  Requests
+// that arrive while a batch runs wait for the next one (continuous
+// batching at batch granularity).  The batch width is capped by a device
+// workspace budget (memory-based admission); submit blocks while the
+// queued columns exceed a host-side budget (back-pressure).  Every column
+// of a batch is computed independently of the others (column-wise kernels,
+// per-column shifts and coefficients), so a request's result does not
+// depend on what it was batched with.
+#include "engine.hpp"
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+
+using namespace lgh;
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+struct Req {
+  int64_t id = 0;
+  int op = 0, m = 0;
+  double c = 0.0;
+  const double *in = nullptr;
+  int ldi = 0;
+  double *out = nullptr;
+  int ldo = 0;
+  clk::time_point t_submit, t_done;
+  int done = 0, rc = 0;
+  double logdet = 0.0;
+};
+
+}  // namespace
+
+struct lgh_serve {
+  lgh_glr_t *g = nullptr;
+  lgh_serve_opts_t o;
+  int max_cols = 64;              // columns per device batch (multiple of 64)
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done, cv_space;
+  std::deque<Req *> queue;
+  std::map<int64_t, Req *> reqs;
+  int64_t next_id = 1;
+  int queued_cols = 0;
+  bool stop = false;
+  std::thread worker;
+  double *pin_in = nullptr, *pin_out = nullptr;   // n x max_cols each
+  lgh_serve_stats_t st;
+  int tier = 0;
+};
+
+// copy m columns of n doubles (column j: src[j] -> dst[j]), split over a
+// few host threads when the block is large
+static void
+copy_cols_par (int n, int m, const double *const *src, double *const *dst)
+{
+  const size_t bytes = sizeof (double) * (size_t) n * m;
+  const int nt = bytes > (64u << 20) ? 8 : 1;
+  auto part = [&] (int t) {
+    for (int j = t; j < m; j += nt) memcpy (dst[j], src[j], sizeof (double) * n);
+  };
+  if (nt == 1) { part (0); return; }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; t++) th.emplace_back (part, t);
+  for (auto &x : th) x.join ();
+}
+
+static void
+run_batch (lgh_serve *s, std::vector<Req *> &batch)
+{
+  const int n = s->g->n;
+  int cols = 0;
+  for (Req *r : batch) cols += r->m;
+  std::vector<const double *> src (cols);
+  std::vector<double *> dst (cols);
+  std::vector<double> c (cols);
+  int j = 0;
+  for (Req *r : batch)
+    for (int q = 0; q < r->m; q++, j++) {
+      src[j] = r->in + (size_t) q * r->ldi;
+      dst[j] = s->pin_in + (size_t) j * n;
+      c[j] = r->c;
+    }
+  copy_cols_par (n, cols, src.data (), dst.data ());
+  const int rc = glr_run_block (s->g, batch[0]->op == LGH_SERVE_SOLVE ? 0 : 3,
+                                cols, c.data (), s->pin_in, n, s->pin_out, n,
+                                s->tier);
+  if (!rc) {
+    j = 0;
+    for (Req *r : batch)
+      for (int q = 0; q < r->m; q++, j++) {
+        src[j] = s->pin_out + (size_t) j * n;
+        dst[j] = r->out + (size_t) q * r->ldo;
+      }
+    copy_cols_par (n, cols, src.data (), dst.data ());
+  }
+  const clk::time_point t = clk::now ();
+  std::lock_guard<std::mutex> lk (s->mu);
+  for (Req *r : batch) {
+    r->rc = rc;
+    r->done = 1;
+    r->t_done = t;
+    s->queued_cols -= r->m;
+  }
+  s->st.batches++;
+  s->st.columns += cols;
+  s->st.requests += (int64_t) batch.size ();
+  if (cols > s->st.max_batch_cols) s->st.max_batch_cols = cols;
+  s->cv_done.notify_all ();
+  s->cv_space.notify_all ();
+}
+
+static void
+worker_loop (lgh_serve *s)
+{
+  (void) hipSetDevice (0);
+  for (;;) {
+    std::vector<Req *> batch;
+    {
+      std::unique_lock<std::mutex> lk (s->mu);
+      s->cv_work.wait (lk, [&] { return s->stop || !s->queue.empty (); });
+      if (s->queue.empty ()) return;      // stop requested, nothing queued
+      const int op = s->queue.front ()->op;
+      int cols = 0;
+      for (auto it = s->queue.begin (); it != s->queue.end ();) {
+        Req *r = *it;
+        if (r->op == op && cols + r->m <= s->max_cols) {
+          batch.push_back (r);
+          cols += r->m;
+          it = s->queue.erase (it);
+          if (cols == s->max_cols) break;
+        }
+        else ++it;
+      }
+    }
+    const clk::time_point t0 = clk::now ();
+    run_batch (s, batch);
+    std::lock_guard<std::mutex> lk (s->mu);
+    s->st.busy_s += std::chrono::duration<double> (clk::now () - t0).count ();
+  }
+}
+
+extern "C" lgh_serve_opts_t
+lgh_serve_opts_default (void)
+{
+  lgh_serve_opts_t o;
+  o.max_batch_cols = 256;
+  o.device_budget_gb = 32.0;
+  o.queue_cols = 1024;
+  return o;
+}
+
+// device bytes one batch column needs: the block driver's five n-row
+// panels (host stage, in, out, Woodbury in/out), the mixed Chebyshev state
+// (R x2 FP64, D x2 FP32, V-cycle in/out) and the V-cycle levels
+static double
+bytes_per_col (const lgh_glr_t *g)
+{
+  return (double) g->n * (5 * 8 + 2 * 8 + 2 * 4 + 2 * 4 + 3 * 4);
+}
+
+extern "C" int
+lgh_serve_create (lgh_glr_t *g, const lgh_serve_opts_t *opts, lgh_serve_t **out)
+{
+  PetscCheck (g != NULL && out != NULL, 0, PETSC_ERR_ARG_NULL,
+              "lgh_serve_create: NULL argument");
+  const lgh_serve_opts_t o = opts ? *opts : lgh_serve_opts_default ();
+  PetscCheck (o.max_batch_cols >= 64 && o.device_budget_gb > 0 && o.queue_cols >= 64,
+              0, PETSC_ERR_ARG_OUTOFRANGE,
+              "lgh_serve_create: need max_batch_cols >= 64, a positive device "
+              "budget and queue_cols >= 64");
+  int cols = (o.max_batch_cols / 64) * 64;
+  const int fit = (int) (o.device_budget_gb * 1e9 / bytes_per_col (g)) / 64 * 64;
+  if (fit < cols) cols = fit;
+  PetscCheck (cols >= 64, 0, PETSC_ERR_MEM,
+              "lgh_serve_create: the device budget (%.1f GB) holds no 64-column "
+              "batch at n = %d", o.device_budget_gb, g->n);
+  lgh_serve *s = new lgh_serve ();
+  s->g = g;
+  s->o = o;
+  s->max_cols = cols;
+  memset (&s->st, 0, sizeof (s->st));
+  // the blocked tier when the GLR's policy enables it at all: then every
+  // batch, whatever its width, takes the same path
+  s->tier = g->x.solve_blocked_min > 0 ? 1 : 0;
+  const size_t bytes = sizeof (double) * (size_t) g->n * cols;
+  if (hipHostMalloc ((void **) &s->pin_in, bytes, hipHostMallocDefault) != hipSuccess
+      || hipHostMalloc ((void **) &s->pin_out, bytes, hipHostMallocDefault) != hipSuccess) {
+    if (s->pin_in) (void) hipHostFree (s->pin_in);
+    delete s;
+    return lgh_petsc_error (PETSC_ERR_MEM, "lgh_serve_create: pinned staging "
+                            "(%.1f GB) failed", 2.0 * bytes / 1e9);
+  }
+  s->worker = std::thread (worker_loop, s);
+  *out = s;
+  return 0;
+}
+
+extern "C" int
+lgh_serve_submit (lgh_serve_t *s, int op, double c, int m, const double *in,
+                  int ldi, double *out, int ldo, int64_t *ticket)
+{
+  PetscCheck (s != NULL && ticket != NULL, 0, PETSC_ERR_ARG_NULL,
+              "lgh_serve_submit: NULL argument");
+  PetscCheck (op == LGH_SERVE_SOLVE || op == LGH_SERVE_SAMPLE
+              || op == LGH_SERVE_LOGDET, 0, PETSC_ERR_ARG_OUTOFRANGE,
+              "lgh_serve_submit: unknown operation %d", op);
+  PetscCheck (c > s->g->floor, 0, PETSC_ERR_ARG_OUTOFRANGE,
+              "lgh_serve_submit: shift c must exceed the spectrum floor %g",
+              s->g->floor);
+  Req *r = new Req ();
+  r->op = op;
+  r->c = c;
+  r->t_submit = clk::now ();
+  if (op == LGH_SERVE_LOGDET) {
+    // host-only: the sum over the resident eigenvalues
+    r->logdet = lgh_glr_logdet (s->g, c);
+    r->done = 1;
+    r->t_done = clk::now ();
+    std::lock_guard<std::mutex> lk (s->mu);
+    r->id = s->next_id++;
+    s->reqs[r->id] = r;
+    s->st.requests++;
+    *ticket = r->id;
+    return 0;
+  }
+  if (!(m >= 1 && m <= s->max_cols && in && out && ldi >= s->g->n
+        && ldo >= s->g->n)) {
+    delete r;
+    return lgh_petsc_error (PETSC_ERR_ARG_OUTOFRANGE,
+                            "lgh_serve_submit: need 1 <= m <= %d and full "
+                            "columns (ld >= n)", s->max_cols);
+  }
+  r->m = m;
+  r->in = in; r->ldi = ldi;
+  r->out = out; r->ldo = ldo;
+  std::unique_lock<std::mutex> lk (s->mu);
+  s->cv_space.wait (lk, [&] { return s->queued_cols + m <= s->o.queue_cols
+                                     || s->queued_cols == 0; });
+  r->id = s->next_id++;
+  s->reqs[r->id] = r;
+  s->queue.push_back (r);
+  s->queued_cols += m;
+  *ticket = r->id;
+  s->cv_work.notify_one ();
+  return 0;
+}
+
+extern "C" int
+lgh_serve_wait (lgh_serve_t *s, int64_t ticket, double *logdet,
+                double *latency_s)
+{
+  PetscCheck (s != NULL, 0, PETSC_ERR_ARG_NULL, "lgh_serve_wait: NULL server");
+  std::unique_lock<std::mutex> lk (s->mu);
+  auto it = s->reqs.find (ticket);
+  PetscCheck (it != s->reqs.end (), 0, PETSC_ERR_ARG_WRONG,
+              "lgh_serve_wait: unknown ticket %lld", (long long) ticket);
+  Req *r = it->second;
+  s->cv_done.wait (lk, [&] { return r->done != 0; });
+  s->reqs.erase (it);
+  lk.unlock ();
+  if (logdet) *logdet = r->logdet;
+  if (latency_s)
+    *latency_s = std::chrono::duration<double> (r->t_done - r->t_submit).count ();
+  const int rc = r->rc;
+  delete r;
+  if (rc) return lgh_petsc_error (PETSC_ERR_LIB, "lgh_serve_wait: batch failed: %s",
+                                  lgh::last_error ());
+  return 0;
+}
+
+extern "C" int
+lgh_serve_get_stats (lgh_serve_t *s, lgh_serve_stats_t *st)
+{
+  PetscCheck (s != NULL && st != NULL, 0, PETSC_ERR_ARG_NULL,
+              "lgh_serve_get_stats: NULL argument");
+  std::lock_guard<std::mutex> lk (s->mu);
+  *st = s->st;
+  st->batch_cols = s->max_cols;
+  return 0;
+}
+
+extern "C" void
+lgh_serve_destroy (lgh_serve_t *s)
+{
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk (s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all ();
+  if (s->worker.joinable ()) s->worker.join ();
+  for (auto &kv : s->reqs) delete kv.second;
+  (void) hipHostFree (s->pin_in);
+  (void) hipHostFree (s->pin_out);
+  delete s;
+}

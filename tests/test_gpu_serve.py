@@ -1,0 +1,117 @@
+"""Request loop (lgh_serve_*) and the blocked downstream solve tier on an
+MI355X: a request's result is the same alone and batched with others
+(tolerance 1e-12 relative; the columns of a batch are computed
+independently), batches do form under concurrent submission, bad requests
+are refused, and the blocked tier agrees with per-column PCG."""
+import threading
+
+import numpy as np
+import pytest
+
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def dev(lgh):
+    lgh.init()
+    return lgh
+
+
+@pytest.fixture(scope="module")
+def prob(dev):
+    Bs, Zs, M = W.cfg1_problem(32)
+    prior = dev.Prior.from_mat(Zs, M, blocked_mode=3)
+    return dict(B=Bs, Z=Zs, M=M, prior=prior, n=Zs.shape[0])
+
+
+def _requests(n, seed=0):
+    """(op, c, X) with shifts log-uniform in [1e-4, 10] and 1..17 columns"""
+    rng = np.random.default_rng(seed)
+    ops = [0, 1, 2, 0, 0, 1, 0, 2, 1, 0, 0, 1]
+    out = []
+    for i, op in enumerate(ops):
+        c = 10.0 ** rng.uniform(-4, 1)
+        m = [1, 3, 8, 17][i % 4]
+        X = None if op == 2 else rng.standard_normal((n, m))
+        out.append((op, c, X))
+    return out
+
+
+@pytest.mark.parametrize("mixed", [True, False])
+def test_serve_alone_equals_mix(dev, prob, mixed):
+    g = dev.GLR(prob["B"], prob["prior"], dev.glr_opts(ell=24, seed=0),
+                x=dev.xopts(mixed=mixed))
+    reqs = _requests(prob["n"])
+    srv = dev.Serve(g, max_batch_cols=64)
+    alone = [srv.wait(srv.submit(op, c, X))[0] for op, c, X in reqs]
+    # the same requests from 4 threads at once, interleaved
+    mix = [None] * len(reqs)
+
+    def client(ids):
+        ts = [(i, srv.submit(reqs[i][0], reqs[i][1], reqs[i][2])) for i in ids]
+        for i, t in ts:
+            mix[i] = srv.wait(t)[0]
+
+    th = [threading.Thread(target=client, args=(list(range(k, len(reqs), 4)),))
+          for k in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    st = srv.stats()
+    srv.close()
+    for (op, c, X), a, b in zip(reqs, alone, mix):
+        if op == 2:
+            assert a == b == pytest.approx(g.logdet(c), rel=1e-14)
+        else:
+            assert rel(b, a) <= 1e-12
+    # some batch held several requests (wider than the widest request)
+    assert st["max_batch_cols"] > 17
+    # the loop's results are the library's batched operations (the same
+    # path for the mixed policy; for the plain one solve_block picks a
+    # narrower panel, which changes rounding inside the 1e-12 PCG solves)
+    i = 6
+    op, c, X = reqs[i]
+    assert op == 0 and X.shape[1] == 8
+    assert rel(alone[i], g.solve_block(c, X)) <= (1e-12 if mixed else 1e-9)
+
+
+def test_serve_refusals(dev, prob):
+    g = dev.GLR(prob["B"], prob["prior"], dev.glr_opts(ell=24, seed=0),
+                x=dev.xopts(mixed=True))
+    srv = dev.Serve(g, max_batch_cols=64)
+    X = np.ones((prob["n"], 2))
+    with pytest.raises(dev.LGHError):
+        srv.submit(7, 1e-2, X)                       # unknown operation
+    with pytest.raises(dev.LGHError):
+        srv.submit(0, 0.0, X)                        # c at the floor
+    with pytest.raises(dev.LGHError):
+        srv.submit(0, 1e-2, np.ones((prob["n"], 65)))  # wider than a batch
+    assert srv.stats()["batch_cols"] == 64
+    srv.close()
+    with pytest.raises(dev.LGHError):
+        dev.Serve(g, max_batch_cols=32)
+
+
+def test_blocked_tier_matches_pcg(dev, prob):
+    """The mixed policy's blocked downstream tier (FP64 Chebyshev around the
+    BF16 V-cycle, Chebyshev tolerance 1e-10) against per-column PCG
+    (ksp_rtol 1e-12) on the same GLR arithmetic."""
+    opts = dev.glr_opts(ell=24, seed=0)
+    gb = dev.GLR(prob["B"], prob["prior"], opts, x=dev.xopts(mixed=True))
+    gp = dev.GLR(prob["B"], prob["prior"], opts,
+                 x=dev.xopts(mixed=True, solve_blocked_min=0))
+    assert np.array_equal(gb.lam, gp.lam)
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((prob["n"], 9))
+    cs = 10.0 ** rng.uniform(-4, 1, size=9)
+    assert rel(gb.solve_block(cs, X), gp.solve_block(cs, X)) < 1e-8
+    assert rel(gb.sample_block(cs, X), gp.sample_block(cs, X)) < 1e-8
+    # below the threshold both take the PCG path
+    assert rel(gb.solve_block(cs[:3], X[:, :3]), gp.solve_block(cs[:3], X[:, :3])) <= 1e-14
