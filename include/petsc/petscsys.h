@@ -44,6 +44,7 @@ typedef struct _p_PetscObject *PetscObject;
 #define PETSC_ERR_LIB             76
 #define PETSC_ERR_ARG_NULL        85
 #define PETSC_ERR_ARG_WRONG       62
+#define PETSC_ERR_CONV_FAILED     91
 
 PetscErrorCode PetscInitialize (int *argc, char ***argv, const char *file,
                                 const char *help);

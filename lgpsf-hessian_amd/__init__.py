@@ -112,6 +112,28 @@ class Callbacks(C.Structure):
                 ("solveZt_blocked", MATFN), ("ctx", vp)]
 
 
+HESSFN = C.CFUNCTYPE(None, dp, dp, vp)     # lgh_hessian_fn
+
+
+class CorrectOpts(C.Structure):
+    _fields_ = [("c0", C.c_double), ("rank", C.c_int), ("applies", C.c_int),
+                ("n_qc", C.c_int), ("q_power", C.c_int),
+                ("clamp_eps", C.c_double), ("d_keep_min", C.c_double),
+                ("drop_tol", C.c_double), ("seed", C.c_ulong),
+                ("verbose", C.c_int)]
+
+
+class CorrectReport(C.Structure):
+    _fields_ = [("kept", C.c_int), ("basis", C.c_int), ("clamped", C.c_int),
+                ("hd_applies", C.c_int), ("d_min", C.c_double),
+                ("d_max", C.c_double), ("floor", C.c_double),
+                ("qc_before", C.c_double), ("qc_after", C.c_double),
+                ("t_operator", C.c_double), ("t_dense", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # every exported function: name -> (restype, argtypes)
 _SIGS = {
     # PETSc subset
@@ -152,6 +174,12 @@ _SIGS = {
     "lgh_glr_pow": (C.c_int, [vp, C.c_double, C.c_double, vp, vp]),
     "lgh_glr_filter": (C.c_int, [vp, dp, C.c_double, vp, vp]),
     "lgh_glr_eigs": (C.c_int, [vp, ip, C.POINTER(dp)]),
+    "lgh_glr_correct_opts_default": (CorrectOpts, []),
+    "lgh_glr_correct_probes": (C.c_int, [vp, C.c_int, dp, dp, HESSFN, vp,
+                                         C.POINTER(CorrectOpts),
+                                         C.POINTER(CorrectReport)]),
+    "lgh_glr_correct": (C.c_int, [vp, HESSFN, vp, C.POINTER(CorrectOpts),
+                                  C.POINTER(CorrectReport)]),
     # extensions
     "lgh_glr_xopts_default": (GlrXOpts, []),
     "lgh_glr_xopts_mixed": (GlrXOpts, []),
@@ -417,6 +445,37 @@ class GLR:
         n = C.c_int()
         lib().MatGetSize(self.Bm.h, C.byref(n), None)
         self.n = n.value
+
+    @staticmethod
+    def _hess(fn, n):
+        """wrap fn(x) -> H_d x (numpy, length n) as an lgh_hessian_fn"""
+        def cb(xin, yout, ctx):
+            x = np.ctypeslib.as_array(xin, shape=(n,))
+            np.ctypeslib.as_array(yout, shape=(n,))[:] = fn(x)
+        return HESSFN(cb)
+
+    @staticmethod
+    def _copts(kw):
+        o = lib().lgh_glr_correct_opts_default()
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+    def correct_probes(self, V, HV, hess, **kw):
+        """stage-2.5 value pass (lgh_glr_correct_probes); V, HV: n x k"""
+        V, HV = _fcol(V), _fcol(HV)
+        o, rep, cb = self._copts(kw), CorrectReport(), self._hess(hess, self.n)
+        _petsc_check(lib().lgh_glr_correct_probes(self.h, V.shape[1], _d(V), _d(HV),
+                                                  cb, None, C.byref(o), C.byref(rep)),
+                     "lgh_glr_correct_probes")
+        return rep.as_dict()
+
+    def correct(self, hess, **kw):
+        """stage-2.5 fresh randomized correction (lgh_glr_correct)"""
+        o, rep, cb = self._copts(kw), CorrectReport(), self._hess(hess, self.n)
+        _petsc_check(lib().lgh_glr_correct(self.h, cb, None, C.byref(o), C.byref(rep)),
+                     "lgh_glr_correct")
+        return rep.as_dict()
 
     def extend(self, k_new):
         rep = GlrReport()

@@ -61,4 +61,8 @@ void prior_ref (lgh_prior_t *p);
 // shift c[j]; tier: -1 by the GLR's policy, 0 per-column PCG, 1 blocked
 int glr_run_block (lgh_glr_t *g, int op, int m, const double *c,
                    const double *in, int ldi, double *out, int ldo, int tier);
+// Y = f_c X + U diag(f_lam - f_c) U^T X on m columns of W = 64 FP64 panels
+// (the same filter for every column)
+int glr_filter_panels (lgh_glr_t *g, const double *f_lam, double f_c,
+                       const PMat &X, int m, PMat &Y);
 }

@@ -788,6 +788,17 @@ run_op (lgh_glr_t *g, const OpSpec &op, int m, const double *c,
 }
 
 int
+lgh::glr_filter_panels (lgh_glr_t *g, const double *f_lam, double f_c,
+                        const PMat &X, int m, PMat &Y)
+{
+  const int k = g->kept;
+  std::vector<double> fk ((size_t) std::max (k, 1) * m, 0.0), fc (m, f_c);
+  for (int j = 0; j < m; j++)
+    for (int i = 0; i < k; i++) fk[i + (size_t) j * k] = f_lam[i] - f_c;
+  return woodbury (g, X, m, fk, fc, Y);
+}
+
+int
 lgh::glr_run_block (lgh_glr_t *g, int op, int m, const double *c,
                     const double *in, int ldi, double *out, int ldo, int tier)
 {
@@ -874,6 +885,31 @@ lgh_glr_apply (lgh_glr_t *g, double c, Vec v, Vec y)
   PetscCheck (v->n == g->n && y->n == g->n, 0, PETSC_ERR_ARG_SIZ,
               "lgh_glr_apply: vector size mismatch");
   lgh_prior_t *p = g->prior;
+  if (g->corrected) {
+    // after a correction B alone no longer represents the operator: the
+    // corrected low-rank action Z M^{-1/2} (U lam U^T + cI) M^{-1/2} Z^T v
+    // (glr_common_impl.h:1029-1041)
+    PMat X, T;
+    std::vector<double> f (std::max (g->kept, 1));
+    for (int i = 0; i < g->kept; i++) f[i] = g->lam[i] + c;
+    int rc = prior_setup_device (p, F64);
+    if (!rc) rc = pmat_alloc (X, g->n, 1, 64, F64);
+    if (!rc) rc = pmat_alloc (T, g->n, 1, 64, F64);
+    PMat H;
+    if (!rc) rc = pmat_alloc (H, g->n, 1, 1, F64);
+    if (!rc) rc = h2d (H.p, v->a, sizeof (double) * g->n);
+    if (!rc) rc = k_copy_cols (H, 0, X, 0, 1);
+    if (!rc) rc = prior_apply_Z (p, 1, X, T);
+    if (!rc) rc = k_diag_scale (T, p->d_minvsqrt, T);
+    if (!rc) rc = glr_filter_panels (g, f.data (), c, T, 1, X);
+    if (!rc) rc = k_diag_scale (X, p->d_minvsqrt, X);
+    if (!rc) rc = prior_apply_Z (p, 0, X, T);
+    if (!rc) rc = k_copy_cols (T, 0, H, 0, 1);
+    if (!rc) rc = d2h (y->a, H.p, sizeof (double) * g->n);
+    pmat_free (X); pmat_free (T); pmat_free (H);
+    if (rc) return API_FAIL ("lgh_glr_apply");
+    return 0;
+  }
   PMat X, T, Yb;
   DevCSR *dB = nullptr;
   int rc = prior_setup_device (p, F64);

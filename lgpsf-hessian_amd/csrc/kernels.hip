@@ -138,9 +138,9 @@ k_randn (PMat &out, uint64_t seed, int64_t row0, int64_t col0)
 
 
 // ---------------------------------------------------------------------------
-// W = 64 row kernels.  A row of a 64-column panel is 256 B (fp32) or 512 B
-// (fp64); each lane moves 16 B of it (float4 / double2), so a row takes 16
-// (fp32) or 32 (fp64) lanes and a wavefront works on 4 or 2 rows at once.
+// W = 64 row kernels.  A row of a 64-column panel is 128 B (bf16), 256 B
+// (fp32) or 512 B (fp64); each lane moves 16 B of it, so a row takes 8, 16
+// or 32 lanes and a wavefront works on 8, 4 or 2 rows at once.
 // The CSR entries of a row are taken 8 at a time: their column indices
 // and values first, then the 8 neighbour-row loads all issued before any
 // is used.  Entries past the row's end get weight 0, the row's own index
