@@ -184,6 +184,8 @@ def main():
     ap.add_argument("--sketch", choices=("f32", "f64"), default=None,
                     help="vector precision of the sketch sweep (default: the "
                          "mixed policy's)")
+    ap.add_argument("--extend", type=int, default=200,
+                    help="columns of the timed lgh_glr_extend (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=300,
                     help="requests of the Poisson serving run (0: skip)")
     ap.add_argument("--copy-calib", action="store_true",
@@ -251,7 +253,8 @@ def main():
     single = ("cheb_mixed", "smooth_pre_fine", "smooth_post_fine", "resid_fine")
     prof = {k: lgh.prof_collect(k, npan, vpc) if k in single else lgh.prof_collect(k)
             for k in lgh.PROF_KINDS}
-    log("mixed build %.3fs kept=%d report=%s" % (t_mixed, g.kept, reps[-1]))
+    kept_build = g.kept
+    log("mixed build %.3fs kept=%d report=%s" % (t_mixed, kept_build, reps[-1]))
     for k, (c, ms, by, fl) in prof.items():
         if c:
             log("  %-16s launches %6d  %9.2f ms  avg %.3f ms  %7.1f GB/s  %6.2f TF/s"
@@ -337,10 +340,21 @@ def main():
         extra["serve"] = serve_bench(lgh, g, n, args.serve_requests, R)
         log("serve:", extra["serve"])
 
+    # incremental extension of the resident mixed GLR by 200 columns
+    # (hashed columns ell..ell+199, power pass, BCGS2, bordered projection)
+    if args.extend > 0:
+        lgh.lib().lgh_hip_sync()
+        t0 = time.perf_counter()
+        rep = g.extend(args.extend)
+        lgh.lib().lgh_hip_sync()
+        extra.update(extend_cols=args.extend, extend_s=time.perf_counter() - t0,
+                     kept_after_extend=rep["kept"])
+        log("extend +%d: %.3fs kept=%d" % (args.extend, extra["extend_s"], rep["kept"]))
+
     cpu = None
     if not args.no_cpu:
         import oracle as O
-        mg, levels = cpu_baseline(Bs, Zs, M, lgh, args.ell, 1, g.kept, 0)
+        mg, levels = cpu_baseline(Bs, Zs, M, lgh, args.ell, 1, kept_build, 0)
         zs = O.ZSolver.mat(O.CSR.from_scipy(Zs), mode=3, mg=mg, lmin=info["lmin"],
                            lmax=info["lmax"], nit=info["nit"], ksp_rtol=1e-12)
         msq = np.sqrt(M)
@@ -352,7 +366,7 @@ def main():
         Wk = O.csr_mm(Bc, T)
         T = zs.solve_block(Wk)
         t_cols = time.perf_counter() - t0
-        cols_build = 3 * args.ell + g.kept
+        cols_build = 3 * args.ell + kept_build
         cpu = {"value": t_cols / args.cpu_cols * cols_build, "unit": "s/build",
                "cores": 1, "kind": "port",
                "sample": "%d full F-sweep column(s) of the FP64 oracle (same "
@@ -379,7 +393,7 @@ def main():
                                "Z^-1 = Chebyshev(%d) around AMG V-cycle"
                                % (args.side, args.side, n, Bs.nnz / n, args.ell,
                                   info["nit"]),
-                   "n": n, "ell": args.ell, "kept": g.kept,
+                   "n": n, "ell": args.ell, "kept": kept_build,
                    "vcycle_fp64_sweeps": args.vcycle,
                    "vcycle_fp32_sweeps": args.sketch_vcycle},
         "roofline": roof,

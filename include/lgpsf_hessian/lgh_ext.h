@@ -131,6 +131,16 @@ int  lgh_serve_wait (lgh_serve_t *server, int64_t ticket, double *logdet,
 int  lgh_serve_get_stats (lgh_serve_t *server, lgh_serve_stats_t *stats);
 void lgh_serve_destroy (lgh_serve_t *server);
 
+/* B from the fit stage's row export (the CSR of lgh_fit_get_rows,
+ * reference fit.h:160-165: row pointers, 64-bit global column ids, values;
+ * communicator size 1) as an assembled AIJ Mat in one pass, replacing the
+ * per-entry lgh_fit_get_mat assembly (reference glr.h:274-282).  Column
+ * ids are range-checked, each row sorted by column with duplicates summed;
+ * with sym_tol >= 0 the matrix must equal its transpose to
+ * |b_ij - b_ji| <= sym_tol * max|b| (PETSC_ERR_ARG_WRONG otherwise). */
+int lgh_mat_from_fit_rows (int n, const int *rowptr, const long *colgids,
+                           const double *vals, double sym_tol, Mat *B);
+
 /* copy of the kept U (n x kept, column-major) for tests */
 int lgh_glr_get_U (const lgh_glr_t *glr, double *U, int ldu);
 

@@ -194,6 +194,9 @@ _SIGS = {
     "lgh_prior_solve_host": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp,
                                        C.c_int, dp, C.c_int]),
     "lgh_glr_get_U": (C.c_int, [vp, dp, C.c_int]),
+    "lgh_mat_from_fit_rows": (C.c_int, [C.c_int, ip, C.POINTER(C.c_long), dp,
+                                        C.c_double, C.POINTER(vp)]),
+    "MatSeqAIJGetCSR": (C.c_int, [vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(dp)]),
     "lgh_serve_opts_default": (ServeOpts, []),
     "lgh_serve_create": (C.c_int, [vp, C.POINTER(ServeOpts), C.POINTER(vp)]),
     "lgh_serve_submit": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, dp, C.c_int,
@@ -320,6 +323,33 @@ class Mat:
                                                      C.byref(h)),
                      "MatCreateSeqAIJWithArrays")
         return cls(h)
+
+    @classmethod
+    def from_fit_rows(cls, rowptr, colgids, vals, sym_tol=-1.0):
+        """AIJ Mat from the fit stage's row export (lgh_mat_from_fit_rows):
+        row pointers, 64-bit global column ids, values"""
+        rp = np.ascontiguousarray(rowptr, dtype=np.int32)
+        cg = np.ascontiguousarray(colgids, dtype=np.int64)
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        h = vp()
+        _petsc_check(lib().lgh_mat_from_fit_rows(
+            len(rp) - 1, _i(rp), cg.ctypes.data_as(C.POINTER(C.c_long)), _d(v),
+            sym_tol, C.byref(h)), "lgh_mat_from_fit_rows")
+        return cls(h)
+
+    def csr(self):
+        """scipy CSR copy of an assembled AIJ Mat"""
+        import scipy.sparse as sp
+        i, j, a = ip(), ip(), dp()
+        _petsc_check(lib().MatSeqAIJGetCSR(self.h, C.byref(i), C.byref(j), C.byref(a)),
+                     "MatSeqAIJGetCSR")
+        m, n = C.c_int(), C.c_int()
+        _petsc_check(lib().MatGetSize(self.h, C.byref(m), C.byref(n)), "MatGetSize")
+        rp = np.ctypeslib.as_array(i, shape=(m.value + 1,)).copy()
+        nnz = int(rp[-1])
+        ci = np.ctypeslib.as_array(j, shape=(max(nnz, 1),))[:nnz].copy()
+        v = np.ctypeslib.as_array(a, shape=(max(nnz, 1),))[:nnz].copy()
+        return sp.csr_matrix((v, ci, rp), shape=(m.value, n.value))
 
     @classmethod
     def dense(cls, D):

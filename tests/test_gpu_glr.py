@@ -234,6 +234,41 @@ def test_grid_extend_incremental(dev, grid, oracle):
     assert np.max(np.abs(g2.lam[:k] - g1.lam[:k]) / np.abs(g1.lam[:k])) < 1e-6
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_cfg3_shape_3d_extend_twice(dev, oracle, mixed):
+    """The 3D configuration's shape at 16^3 (n = 4096): Z = M + 0.1 K7
+    (7-point, AMG tier), isotropic PSF B (sigma 1.5h, radius-2h ball,
+    33 nnz/row), ell = 40 then lgh_glr_extend by 20 twice; engine vs the
+    oracle's incremental extension on the same hierarchy."""
+    Bs, Zs, M = W.cfg3_problem(16)
+    assert abs(Bs.nnz / Bs.shape[0] - 33) < 8
+    prior = dev.Prior.from_mat(Zs, M, blocked_mode=3)
+    info = prior.info()
+    h = dev.AMG(Zs)
+    levels = [{"A": oracle.CSR.from_scipy(l["A"]),
+               "P": oracle.CSR.from_scipy(l["P"]) if l["P"] is not None else None,
+               "slo": l["slo"], "shi": l["shi"], "nu": l["nu"]} for l in h.levels()]
+    zs = oracle.ZSolver.mat(oracle.CSR.from_scipy(Zs), mode=3, mg=oracle.MG(levels),
+                            lmin=info["lmin"], lmax=info["lmax"], nit=info["nit"],
+                            ksp_rtol=1e-12)
+    og = oracle.GLR(oracle.CSR.from_scipy(Bs), M, zs, ell=40, q_power=1,
+                    trunc_rel=1e-4, seed=0)
+    g = dev.GLR(Bs, prior, dev.glr_opts(ell=40, seed=0, trunc_rel=1e-4),
+                x=dev.xopts(mixed=mixed))
+    # FP64: the oracle's arithmetic; mixed: FP64-accurate Z solves around the
+    # BF16 V-cycle
+    tol = 1e-7 if mixed else 1e-9
+    for step in range(3):
+        if step:
+            g.extend(20)
+            og.extend(20, incremental=True, panel=64)
+        assert g.kept == og.kept
+        assert np.max(np.abs(g.lam - og.lam) / np.abs(og.lam)) < tol, step
+    c = 1e-2
+    rhs = oracle.randn_fill(3, 0, 0, Zs.shape[0], 1)[:, 0]
+    assert rel(g.solve(c, rhs), og.solve(c, rhs)) < (1e-6 if mixed else 1e-9)
+
+
 def test_cfg1_full_size_matches_oracle(dev, oracle):
     """The CPU-runnable benchmark case at full size: 128 x 128 grid
     (n = 16 384), Z = M + 0.1 K, symmetrized Gaussian PSF B (radius 4h),
