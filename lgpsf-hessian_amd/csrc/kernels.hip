@@ -232,6 +232,35 @@ __device__ inline void unpack (const uint4 &r, float (&x)[8])
   }
 }
 
+// four values of a panel row widened to fp64 (the mixed step's row map:
+// 16 lanes per row); raw forms keep gathered neighbours at 2-4 VGPRs
+__device__ inline void vld (const double *p, double (&x)[4])
+{
+  const double2 a = *(const double2 *) p, b = *(const double2 *) (p + 2);
+  x[0] = a.x; x[1] = a.y; x[2] = b.x; x[3] = b.y;
+}
+__device__ inline void vst (double *p, const double (&x)[4])
+{
+  *(double2 *) p = make_double2 (x[0], x[1]);
+  *(double2 *) (p + 2) = make_double2 (x[2], x[3]);
+}
+__device__ inline void vld (const float *p, double (&x)[4])
+{ const float4 v = *(const float4 *) p; x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w; }
+__device__ inline void vst (float *p, const double (&x)[4])
+{ *(float4 *) p = make_float4 ((float) x[0], (float) x[1], (float) x[2], (float) x[3]); }
+template <typename S> struct Raw4;
+template <> struct Raw4<float> { using type = float4; };
+template <> struct Raw4<bf16_t> { using type = uint2; };
+__device__ inline float4 rld4 (const float *p) { return *(const float4 *) p; }
+__device__ inline uint2 rld4 (const bf16_t *p) { return *(const uint2 *) p; }
+__device__ inline void unpack4 (const float4 &r, double (&x)[4])
+{ x[0] = r.x; x[1] = r.y; x[2] = r.z; x[3] = r.w; }
+__device__ inline void unpack4 (const uint2 &r, double (&x)[4])
+{
+  x[0] = __uint_as_float (r.x << 16); x[1] = __uint_as_float (r.x & 0xffff0000u);
+  x[2] = __uint_as_float (r.y << 16); x[3] = __uint_as_float (r.y & 0xffff0000u);
+}
+
 static constexpr int NG = 8;            // CSR entries per load group
 
 template <typename T, int N>
@@ -434,12 +463,15 @@ cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
   if (Rpc) vst (Rpc + qi, r);
 }
 
-// mixed-precision fused Chebyshev step: FP64 residual R and accumulator X,
-// FP32 direction D and preconditioned residual Z.  The new direction is
-// rounded to FP32 once and that rounded value is what X and R both see, so
-// R stays the FP64 residual of X and the iteration reaches FP64 accuracy
-// with an FP32 preconditioner.  Also writes R in FP32 for the next
-// preconditioner application.  Lanes follow the fp64 row map (2 values).
+// mixed-precision fused Chebyshev step: FP64 residual R and accumulator X;
+// preconditioned residual Z in the preconditioner's precision SP (FP32 or
+// BF16), direction D in FP32.  The new direction is rounded to FP32 once
+// and that stored value is what X and R both see, so R stays the FP64
+// residual of X and the iteration reaches FP64 accuracy with a
+// low-precision preconditioner (the rounding only perturbs the step).  Also
+// writes R in SP for the next preconditioner application.  (A BF16
+// direction saves 4 of ~44 B per element but its 2^-9 rounding, repeated
+// over the 25 steps, costs the solve a factor ~2 in accuracy.)
 template <bool HASD, typename SP>
 __global__ void __launch_bounds__ (TPB)
 cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
@@ -449,39 +481,52 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
            float *__restrict__ Dn, double *__restrict__ Rn,
            SP *__restrict__ Rpc)
 {
-  LGH_ROWMAP (double)
-  auto dnew = [&] (size_t q, double (&o)[N]) {
-    double z[N], d[N];
-    vld (Zs + q, z);
-    if (HASD) vld (D + q, d);
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-      const double bz = b * z[i];
-      o[i] = (double) (float) (HASD ? a * d[i] + bz : bz);
-    }
-  };
+  // 16 lanes per row, 4 values per lane (fewer, wider loads than the fp64
+  // map: the gathered Z and D are 8-16 B per lane)
+  constexpr int N = 4, LPR = 16;
+  const int row = blockIdx.x * (TPB / LPR) + (int) threadIdx.x / LPR;
+  const size_t off = (size_t) blockIdx.y * n * 64
+                     + (size_t) ((int) threadIdx.x % LPR) * N;
+  if (row >= n) return;
+  const int k0 = rp[row], k1 = rp[row + 1];
+  constexpr int NGM = 6;            // entries per group (5- and 7-point Z)
   double acc[N];
 #pragma unroll
   for (int i = 0; i < N; i++) acc[i] = 0;
-  for (int kb = k0; kb < k1; kb += NG) {
-    int c[NG];
-    double w[NG];
+  for (int kb = k0; kb < k1; kb += NGM) {
+    int c[NGM];
+    double w[NGM];
 #pragma unroll
-    for (int u = 0; u < NG; u++) {
+    for (int u = 0; u < NGM; u++) {
       const bool in = kb + u < k1;
       c[u] = in ? ci[kb + u] : row;
       w[u] = in ? v[kb + u] : 0.0;
     }
-    double y[NG][N];
+    typename Raw4<SP>::type zr[NGM];
+    typename Raw4<float>::type dr[NGM];
 #pragma unroll
-    for (int u = 0; u < NG; u++) {
-      if (kb + u < k1) dnew (off + (size_t) c[u] * 64, y[u]);
-      else vzero (y[u]);
+    for (int u = 0; u < NGM; u++) {
+      zr[u] = {};
+      dr[u] = {};
+      if (kb + u < k1) {
+        const size_t q = off + (size_t) c[u] * 64;
+        zr[u] = rld4 (Zs + q);
+        if (HASD) dr[u] = rld4 (D + q);
+      }
     }
+    // the new direction at each neighbour, rounded to FP32 as stored
 #pragma unroll
-    for (int u = 0; u < NG; u++)
+    for (int u = 0; u < NGM; u++) {
+      double z[N], d[N];
+      unpack4 (zr[u], z);
+      unpack4 (dr[u], d);
 #pragma unroll
-      for (int i = 0; i < N; i++) acc[i] += w[u] * y[u][i];
+      for (int i = 0; i < N; i++) {
+        const double bz = b * z[i];
+        const double y = (double) (float) (HASD ? a * d[i] + bz : bz);
+        acc[i] += w[u] * y;
+      }
+    }
   }
   const size_t qi = off + (size_t) row * 64;
   double di[N], r[N], x[N], dold[N];
@@ -521,7 +566,8 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
 {
   LGH_CHECK (A.prec == F64 && R.prec == F64 && X.prec == F64 && R.W == 64
              && (Zpc.prec == F32 || Zpc.prec == BF16) && Rnpc.prec == Zpc.prec
-             && Dn32.prec == F32 && Rn.prec == F64 && R.nrows == A.n, 8,
+             && Dn32.prec == F32 && (!D32 || D32->prec == F32)
+             && Rn.prec == F64 && R.nrows == A.n, 8,
              "k_cheb_step_mixed: shape/precision mismatch");
   LGH_CHECK (x_mode >= MIX_X_SET && x_mode <= MIX_X_ADD2
              && (D32 || (x_mode != MIX_X_SET2 && x_mode != MIX_X_ADD2)), 8,
@@ -536,7 +582,7 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                 + vec * (pb + (D32 ? 4 : 0) + 8 + xb + 4 + 8 + pb)),
                 R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * 64,
                 R.npanels (), Zpc.prec);
-  const dim3 g = grid64 (A.n, R.npanels (), F64);
+  const dim3 g = grid64 (A.n, R.npanels (), F32);     // 16 lanes per row
   auto go = [&] (auto sp) -> int {
     using SP = decltype (sp);
     if (D32)

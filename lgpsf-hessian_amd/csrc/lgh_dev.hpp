@@ -8,9 +8,8 @@
 // With W = 64 a panel row is 128 B (bf16) / 256 B (fp32) / 512 B (fp64)
 // contiguous: 8, 16 or 32 lanes load it as 16 B per lane, so every CSR
 // neighbour gather is one coalesced row load, and elementwise work streams
-// contiguously.  Columns
-// past ncols in the last panel are padding and are kept at zero by every
-// kernel.
+// contiguously.  Columns past ncols in the last panel are padding and are
+// kept at zero by every kernel.
 //
 // Sparse matrices are CSR with int32 row pointers / column indices and
 // fp64 or fp32 values.  Small dense matrices (Gram, T, eigenvectors,
@@ -151,7 +150,8 @@ int k_cheb_step (const DevCSR &A, int jacobi, const PMat *Zsrc, double a,
                  PMat &Dn, PMat &Rn, PMat *Rpc = nullptr);
 // mixed step at W = 64: FP64 R, X (A values FP64), FP32 D; the
 // preconditioned residual Zpc and the copy Rnpc of the new residual (the
-// next preconditioner input) are FP32 or BF16.
+// next preconditioner input) are FP32 or BF16.  The new direction is
+// rounded to FP32 once and X and R move by exactly the stored value.
 // x_mode: how X takes the new direction Dn (D = the previous direction)
 enum {
   MIX_X_SET = 0,      // X = Dn

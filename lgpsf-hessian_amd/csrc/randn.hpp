@@ -7,14 +7,19 @@
 // libm call, log(q), is replaced by a double-double evaluation that is
 // correctly rounded except when the exact value lies within ~2^-100
 // relative of a rounding midpoint:
-//     q = 2^e m, m in [sqrt(1/2), sqrt(2)),  t = (m - 1) / (m + 1),
-//     log q = e ln2 + 2 atanh(t) = e ln2 + 2 t sum_k t^(2k) / (2k + 1),
-// summed by double-double Horner (24 terms, |t| <= 0.1716 so t^48 < 2^-120).
-// A draw therefore equals the host's bit for bit wherever the host libm log
-// is correctly rounded; the tests count the exceptions exhaustively.
+//     q = 2^e m, m in [sqrt(1/2), sqrt(2)),  c = k/64 nearest to m,
+//     t = (m - c) / (m + c),  |t| <= 2^-7.5,
+//     log q = e ln2 + log c + 2 atanh(t) = ... + 2 t sum_k t^(2k) / (2k + 1),
+// with log c from a double-double table (log_table.hpp) and the series to
+// t^14 (t^16 < 2^-120): the terms from t^8 on in double, the first four by
+// double-double Horner.  A draw therefore equals the host's bit for bit
+// wherever the host libm log is correctly rounded; the tests count the
+// exceptions exhaustively.
 #pragma once
 #include <stdint.h>
 #include <math.h>
+
+#include "log_table.hpp"
 
 #if defined(__HIPCC__)
 #define LGH_HD __host__ __device__
@@ -91,23 +96,33 @@ LGH_HD inline double log_dd (double q)
   cv.u = (cv.u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;   // [1,2)
   double m = cv.d;
   if (m > 1.4142135623730951) { m = 0.5 * m; e += 1; }              // exact
-  // t = (m - 1)/(m + 1) in double-double
-  double num = m - 1.0;                                             // exact
-  dd den = dd_two_sum (m, 1.0);
+  // c = k/64 nearest to m (k in 45..91); m - c is exact (Sterbenz)
+  const int k = (int) (m * 64.0 + 0.5);
+  const double c = (double) k * 0.015625;
+  const double num = m - c;
+  dd den = dd_two_sum (m, c);
   double th = num / den.hi;
-  // residual num - th*den exactly enough: num - th*den.hi - th*den.lo
+  // residual num - th * den, exact enough for the low part of t
   double p = th * den.hi;
   double pe = fma (th, den.hi, -p);
   double r = ((num - p) - pe) - th * den.lo;
   dd t = dd_quick (th, r / den.hi);
   dd t2 = dd_mul (t, t);
-  dd S = dd_recip (49.0);
-  for (int k = 23; k >= 0; k--) {
+  // sum_{j=0..7} t^(2j) / (2j + 1): j = 4..7 in double, j = 3..0 in dd
+  const double h = t2.hi;
+  double sd = 1.0 / 15.0;
+  sd = sd * h + 1.0 / 13.0;
+  sd = sd * h + 1.0 / 11.0;
+  sd = sd * h + 1.0 / 9.0;
+  dd S = { sd, 0.0 };
+  for (int j = 3; j >= 0; j--) {
     S = dd_mul (S, t2);
-    S = dd_add (S, dd_recip (2.0 * k + 1.0));
+    S = dd_add (S, dd_recip (2.0 * j + 1.0));
   }
   dd L = dd_mul (t, S);
   L.hi *= 2.0; L.lo *= 2.0;                                         // exact
+  const double *tab = log_table ();
+  const dd Lc = { tab[2 * (k - LOG_TAB_K0)], tab[2 * (k - LOG_TAB_K0) + 1] };
   // e ln2, ln2 = 0x1.62e42fefa39efp-1 + 0x1.abc9e3b39803fp-56
   const double ln2_hi = 6.93147180559945286227e-01;
   const double ln2_lo = 2.31904681384629955842e-17;
@@ -115,7 +130,7 @@ LGH_HD inline double log_dd (double q)
   double a = de * ln2_hi;
   double ae = fma (de, ln2_hi, -a);
   dd E = dd_quick (a, ae + de * ln2_lo);
-  dd res = dd_add (E, L);
+  dd res = dd_add (dd_add (E, Lc), L);
   return res.hi;
 }
 
