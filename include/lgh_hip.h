@@ -70,7 +70,8 @@ int lgh_hip_cheb_jacobi (int n, const int *rp, const int *ci, const double *v,
 /* G (a x b) = X^T Y, X: n x a, Y: n x b */
 int lgh_hip_gram (int prec, int n, int a, const double *X, int ldx, int b,
                   const double *Y, int ldy, double *G, int ldg);
-/* Y (n x b) = X C, X: n x a, C: a x b */
+/* Y (n x b) = X C, X: n x a, C: a x b (a C that is zero below its 64 x 64
+ * block diagonal takes the triangular product, skipping those blocks) */
 int lgh_hip_gemm_nn (int prec, int n, int a, const double *X, int ldx, int b,
                      const double *C, int ldc, double *Y, int ldy);
 

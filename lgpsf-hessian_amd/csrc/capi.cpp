@@ -162,12 +162,18 @@ lgh_hip_gemm_nn (int prec, int n, int a, const double *X, int ldx, int b,
   std::vector<double> hc ((size_t) ld * Pb * 64, 0.0);
   for (int j = 0; j < b; j++)
     for (int i = 0; i < a; i++) hc[i + (size_t) j * ld] = C[i + (size_t) j * ldc];
+  // a C whose 64 x 64 blocks below the block diagonal are all zero takes
+  // the triangular product (the form CholQR uses)
+  int upper = 1;
+  for (int j = 0; j < Pb * 64 && upper; j++)
+    for (int i = (j / 64 + 1) * 64; i < ld; i++)
+      if (hc[i + (size_t) j * ld] != 0.0) { upper = 0; break; }
   double *dC = (double *) dmalloc (sizeof (double) * hc.size ());
   if (!dC) return 5;
   int rc = h2d (dC, hc.data (), sizeof (double) * hc.size ());
   if (!rc) rc = stage_in (X, ldx, n, a, 64, prec, dx);
   if (!rc) rc = pmat_alloc (dy, n, b, 64, prec);
-  if (!rc) rc = k_gemm_nn (dx, 0, Pa, dC, ld, 1.0, 0.0, dy, 0, Pb);
+  if (!rc) rc = k_gemm_nn (dx, 0, Pa, dC, ld, 1.0, 0.0, dy, 0, Pb, upper);
   if (!rc) rc = stage_out (dy, b, Y, ldy);
   pmat_free (dx); pmat_free (dy); dfree (dC);
   return rc;

@@ -132,7 +132,7 @@ cholqr (PMat &X, int ncols, int passes, int *jittered)
     if (!rc) rc = sol_trtri (0, ncols, G, ld, &info);
     if (!rc && info != 0) { set_error ("CholQR2: trtri failed"); rc = 11; }
     if (!rc) rc = k_lower_t_pad (G, ncols, ld, C, ld, ld);
-    if (!rc) rc = k_gemm_nn (X, 0, P, C, ld, 1.0, 0.0, T, 0, P);
+    if (!rc) rc = k_gemm_nn (X, 0, P, C, ld, 1.0, 0.0, T, 0, P, 1);
     if (!rc) std::swap (X.p, T.p);
   }
   pmat_free (T);

@@ -1351,23 +1351,43 @@ template <> struct mfma<float> {
   static __device__ inline int row (int lk, int r) { return 4 * lk + r; }
 };
 
+// Blocks are numbered so that the blocks sharing an input tile land on one
+// XCD back to back: the dispatcher hands block b to XCD b % 8, so block b
+// takes group (b / 8) / nsib * 8 + b % 8 and sibling (b / 8) % nsib.  The
+// first sibling reads the tile from HBM; the others find it in that XCD's L2.
+__device__ inline void
+xcd_split (int nsib, int &group, int &sib)
+{
+  const int b = blockIdx.x, s = b >> 3;
+  group = (s / nsib) * 8 + (b & 7);
+  sib = s % nsib;
+}
+static inline unsigned
+xcd_blocks (int ngroups, int nsib)
+{
+  return (unsigned) (((ngroups + 7) / 8) * 8) * (unsigned) nsib;
+}
+
 // Gram partials: block (chunk, pair) computes the 64 x 64 tile
-// X_pi^T Y_pj over rows [r0, r1).  Wave w owns rows 32*(w>>1).., cols
-// 32*(w&1).. of the tile as 2 x 2 MFMA 16 x 16 tiles.  MFMA 16x16x4
-// operand layout: A(i, k) at lane i + 16 k, B(k, j) at lane j + 16 k;
-// result rows per mfma<T>::row.
+// X_pi^T Y_pj over rows [r0, r1) (the pairs of one chunk are siblings).
+// Wave w owns rows 32*(w>>1).., cols 32*(w&1).. of the tile as 2 x 2 MFMA
+// 16 x 16 tiles.  MFMA 16x16x4 operand layout: A(i, k) at lane i + 16 k,
+// B(k, j) at lane j + 16 k; result rows per mfma<T>::row.
 template <typename T>
 __global__ void __launch_bounds__ (TPB)
-gram_k (int n, const T *__restrict__ X, int xp0, const T *__restrict__ Y,
-        int yp0, int npy, int sym, int rows_per_chunk,
+gram_k (int n, int nchunks, int npairs, const T *__restrict__ X, int xp0,
+        const T *__restrict__ Y, int yp0, int rows_per_chunk,
         const int2 *__restrict__ pairs, double *__restrict__ part)
 {
   typedef typename mfma<T>::acc_t acc_t;
+  int chunk, pair;
+  xcd_split (npairs, chunk, pair);
+  if (chunk >= nchunks) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int2 pr = pairs[blockIdx.y];
+  const int2 pr = pairs[pair];
   const T *Xp = X + (size_t) (xp0 + pr.x) * n * 64;
   const T *Yp = Y + (size_t) (yp0 + pr.y) * n * 64;
-  const int r0 = blockIdx.x * rows_per_chunk;
+  const int r0 = chunk * rows_per_chunk;
   const int r1 = min (n, r0 + rows_per_chunk);
   const int mb = 32 * (wave >> 1), nb = 32 * (wave & 1);
   const int li = lane & 15, lk = lane >> 4;
@@ -1375,21 +1395,28 @@ gram_k (int n, const T *__restrict__ X, int xp0, const T *__restrict__ Y,
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++)
       for (int r = 0; r < 4; r++) acc[a][b][r] = 0;
-  for (int k0 = r0; k0 < r1; k0 += 4) {
-    const int kr = k0 + lk;
-    T av[2], bv[2];
-    if (kr < r1) {
-      const T *xr = Xp + (size_t) kr * 64;
-      const T *yr = Yp + (size_t) kr * 64;
-      av[0] = xr[mb + li]; av[1] = xr[mb + 16 + li];
-      bv[0] = yr[nb + li]; bv[1] = yr[nb + 16 + li];
+  // 8 k-steps (32 rows) per trip: all 32 loads issued before the MFMAs
+  constexpr int KU = 8;
+  for (int k0 = r0; k0 < r1; k0 += 4 * KU) {
+    T av[KU][2], bv[KU][2];
+#pragma unroll
+    for (int u = 0; u < KU; u++) {
+      const int kr = k0 + 4 * u + lk;
+      if (kr < r1) {
+        const T *xr = Xp + (size_t) kr * 64;
+        const T *yr = Yp + (size_t) kr * 64;
+        av[u][0] = xr[mb + li]; av[u][1] = xr[mb + 16 + li];
+        bv[u][0] = yr[nb + li]; bv[u][1] = yr[nb + 16 + li];
+      }
+      else { av[u][0] = av[u][1] = bv[u][0] = bv[u][1] = (T) 0; }
     }
-    else { av[0] = av[1] = bv[0] = bv[1] = (T) 0; }
-    for (int a = 0; a < 2; a++)
-      for (int b = 0; b < 2; b++)
-        acc[a][b] = mfma<T>::op (av[a], bv[b], acc[a][b]);
+#pragma unroll
+    for (int u = 0; u < KU; u++)
+      for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+          acc[a][b] = mfma<T>::op (av[u][a], bv[u][b], acc[a][b]);
   }
-  double *tile = part + ((size_t) blockIdx.x * gridDim.y + blockIdx.y) * 4096;
+  double *tile = part + ((size_t) chunk * npairs + pair) * 4096;
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++)
       for (int r = 0; r < 4; r++) {
@@ -1399,21 +1426,39 @@ gram_k (int n, const T *__restrict__ X, int xp0, const T *__restrict__ Y,
       }
 }
 
+// fixed-order reduction of the chunk partials: block (slice, pair) sums 16
+// consecutive tile entries; thread (g, e) adds chunks g, g + 16, g + 32, ...
+// in order, then thread g = 0 adds the 16 group sums in order (the result
+// depends on the chunk count only, not on the launch)
 __global__ void __launch_bounds__ (TPB)
 gram_reduce_k (int nchunks, int npairs, const int2 *__restrict__ pairs,
                const double *__restrict__ part, double *G, int ldg, int sym)
 {
+  __shared__ double red[16][16];
   const int pair = blockIdx.y;
-  const int e = blockIdx.x * TPB + threadIdx.x;   // 0..4095
-  if (e >= 4096) return;
+  const int el = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const int e = blockIdx.x * 16 + el;                 // 0..4095
   double t = 0.0;
-  for (int q = 0; q < nchunks; q++)
+  int q = g;
+  for (; q + 48 < nchunks; q += 64) {
+    const double a0 = part[((size_t) q * npairs + pair) * 4096 + e];
+    const double a1 = part[((size_t) (q + 16) * npairs + pair) * 4096 + e];
+    const double a2 = part[((size_t) (q + 32) * npairs + pair) * 4096 + e];
+    const double a3 = part[((size_t) (q + 48) * npairs + pair) * 4096 + e];
+    t += a0; t += a1; t += a2; t += a3;
+  }
+  for (; q < nchunks; q += 16)
     t += part[((size_t) q * npairs + pair) * 4096 + e];
+  red[g][el] = t;
+  __syncthreads ();
+  if (g != 0) return;
+  double v = red[0][el];
+  for (int h = 1; h < 16; h++) v += red[h][el];
   const int2 pr = pairs[pair];
   const int row = e % 64, col = e / 64;
-  G[(size_t) (pr.x * 64 + row) + (size_t) (pr.y * 64 + col) * ldg] = t;
+  G[(size_t) (pr.x * 64 + row) + (size_t) (pr.y * 64 + col) * ldg] = v;
   if (sym && pr.x != pr.y)
-    G[(size_t) (pr.y * 64 + col) + (size_t) (pr.x * 64 + row) * ldg] = t;
+    G[(size_t) (pr.y * 64 + col) + (size_t) (pr.x * 64 + row) * ldg] = v;
 }
 
 int
@@ -1444,20 +1489,20 @@ k_gram (const PMat &X, int xp0, int npx, const PMat &Y, int yp0, int npy,
   if (!pd || !part) { dfree (pd); dfree (part); return 5; }
   int rc = h2d (pd, pv.data (), sizeof (int2) * npairs);
   if (!rc) {
-    dim3 g (nch, npairs);
+    const unsigned g = xcd_blocks (nch, npairs);
     if (X.prec == F64)
-      hipLaunchKernelGGL (gram_k<double>, g, dim3 (TPB), 0, stream (), n,
-                          (const double *) X.p, xp0, (const double *) Y.p,
-                          yp0, npy, sym, rpc, pd, part);
+      hipLaunchKernelGGL (gram_k<double>, dim3 (g), dim3 (TPB), 0, stream (),
+                          n, nch, npairs, (const double *) X.p, xp0,
+                          (const double *) Y.p, yp0, rpc, pd, part);
     else
-      hipLaunchKernelGGL (gram_k<float>, g, dim3 (TPB), 0, stream (), n,
-                          (const float *) X.p, xp0, (const float *) Y.p, yp0,
-                          npy, sym, rpc, pd, part);
+      hipLaunchKernelGGL (gram_k<float>, dim3 (g), dim3 (TPB), 0, stream (),
+                          n, nch, npairs, (const float *) X.p, xp0,
+                          (const float *) Y.p, yp0, rpc, pd, part);
     hipError_t e = hipGetLastError ();
     if (e != hipSuccess) rc = hip_fail (e, "gram_k", __FILE__, __LINE__);
   }
   if (!rc) {
-    hipLaunchKernelGGL (gram_reduce_k, dim3 (4096 / TPB, npairs), dim3 (TPB),
+    hipLaunchKernelGGL (gram_reduce_k, dim3 (4096 / 16, npairs), dim3 (TPB),
                         0, stream (), nch, npairs, pd, part, G, ldg, sym);
     hipError_t e = hipGetLastError ();
     if (e != hipSuccess) rc = hip_fail (e, "gram_reduce_k", __FILE__, __LINE__);
@@ -1467,38 +1512,59 @@ k_gram (const PMat &X, int xp0, int npx, const PMat &Y, int yp0, int npy,
   return rc;
 }
 
-// Y panel block = alpha X C + beta Y.  Block (rowtile, out panel q) stages
+// Y panel block = alpha X C + beta Y.  Block (row tile, out panel q) stages
 // the 64 x 64 tile of each input panel and the matching 64 x 64 block of C
 // in LDS; wave w owns output rows 16 w .. 16 w + 15 (4 MFMA tiles across).
+// The out panels of one row tile are siblings (one XCD, back to back), and
+// the next input panel's tile and C block are loaded into registers while
+// the MFMAs of the current one run.  upper: C is block upper triangular
+// (C[p, q] = 0 for p > q), so out panel q reads input panels 0..q only.
 static constexpr int GPAD = 65;
 
 template <typename T>
 __global__ void __launch_bounds__ (TPB)
-gemm_nn_k (int n, const T *__restrict__ X, int xp0, int npx,
-           const double *__restrict__ C, int ldc, T alpha, T beta,
-           T *__restrict__ Y, int yp0)
+gemm_nn_k (int n, int ntiles, int npy, const T *__restrict__ X, int xp0,
+           int npx, const double *__restrict__ C, int ldc, T alpha, T beta,
+           T *__restrict__ Y, int yp0, int upper)
 {
   typedef typename mfma<T>::acc_t acc_t;
   __shared__ T Xs[64 * GPAD];
   __shared__ T Cs[64 * GPAD];
+  int tile, q;
+  xcd_split (npy, tile, q);
+  if (tile >= ntiles) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const int r0 = blockIdx.x * 64;
-  const int q = blockIdx.y;
+  const int r0 = tile * 64;
+  const int pend = upper ? min (npx, q + 1) : npx;
+  // element e = threadIdx.x + 256 i of a 64 x 64 tile: X row rr = e / 64,
+  // col cc = e % 64; C row kk = e % 64, col jj = e / 64
+  const int e64 = (int) threadIdx.x & 63, e6 = (int) threadIdx.x >> 6;
+  constexpr int EPT = 4096 / TPB;
+  T xr[EPT], cr[EPT];
+  auto gload = [&] (int p) {
+    const T *Xp = X + (size_t) (xp0 + p) * n * 64;
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      const int rr = e6 + (TPB / 64) * i;
+      xr[i] = (r0 + rr < n) ? Xp[(size_t) (r0 + rr) * 64 + e64] : (T) 0;
+      cr[i] = (T) C[(size_t) (p * 64 + e64) + (size_t) (q * 64 + rr) * ldc];
+    }
+  };
   acc_t acc[4];
   for (int b = 0; b < 4; b++)
     for (int r = 0; r < 4; r++) acc[b][r] = 0;
-  for (int p = 0; p < npx; p++) {
-    const T *Xp = X + (size_t) (xp0 + p) * n * 64;
+  if (pend > 0) gload (0);
+  for (int p = 0; p < pend; p++) {
     __syncthreads ();
-    for (int e = threadIdx.x; e < 4096; e += TPB) {
-      const int rr = e >> 6, cc = e & 63;
-      Xs[rr * GPAD + cc] = (r0 + rr < n) ? Xp[(size_t) (r0 + rr) * 64 + cc] : (T) 0;
-      // C block (k = p*64 + kk, j = q*64 + jj), stored Cs[kk][jj]
-      const int kk = e & 63, jj = e >> 6;
-      Cs[kk * GPAD + jj] = (T) C[(size_t) (p * 64 + kk) + (size_t) (q * 64 + jj) * ldc];
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      const int rr = e6 + (TPB / 64) * i;
+      Xs[rr * GPAD + e64] = xr[i];
+      Cs[e64 * GPAD + rr] = cr[i];
     }
     __syncthreads ();
+    if (p + 1 < pend) gload (p + 1);
     for (int k0 = 0; k0 < 64; k0 += 4) {
       const T a = Xs[(16 * wave + li) * GPAD + k0 + lk];
       for (int b = 0; b < 4; b++) {
@@ -1522,24 +1588,33 @@ gemm_nn_k (int n, const T *__restrict__ X, int xp0, int npx,
 
 int
 k_gemm_nn (const PMat &X, int xp0, int npx, const double *C, int ldc,
-           double alpha, double beta, PMat &Y, int yp0, int npy)
+           double alpha, double beta, PMat &Y, int yp0, int npy, int upper)
 {
   LGH_CHECK (X.W == 64 && Y.W == 64 && X.prec == Y.prec
              && X.nrows == Y.nrows && xp0 + npx <= X.npanels ()
              && yp0 + npy <= Y.npanels (), 8, "k_gemm_nn: shape mismatch");
+  // output panels may overlap input panels only in the one-panel in-place
+  // form: the block reads its whole input tile before it writes
+  const bool overlap = X.p == Y.p && yp0 < xp0 + npx && xp0 < yp0 + npy;
+  LGH_CHECK (!overlap || (npx == 1 && npy == 1 && xp0 == yp0), 8,
+             "k_gemm_nn: output panels overlap the input panels");
   if (npy == 0) return 0;
+  double pairs = 0;
+  for (int q = 0; q < npy; q++) pairs += upper ? std::min (npx, q + 1) : npx;
   ProfScope ps (PK_GEMM, (double) X.nrows * 64 * prec_bytes (X.prec)
                 * (npx + npy * (beta != 0.0 ? 2 : 1)),
-                2.0 * X.nrows * 4096.0 * npx * npy);
-  dim3 g (grid1 (X.nrows, 64), npy);
+                2.0 * X.nrows * 4096.0 * pairs);
+  const int ntiles = (X.nrows + 63) / 64;
+  const dim3 g (xcd_blocks (ntiles, npy));
   if (X.prec == F64)
     hipLaunchKernelGGL (gemm_nn_k<double>, g, dim3 (TPB), 0, stream (),
-                        X.nrows, (const double *) X.p, xp0, npx, C, ldc,
-                        alpha, beta, (double *) Y.p, yp0);
+                        X.nrows, ntiles, npy, (const double *) X.p, xp0, npx,
+                        C, ldc, alpha, beta, (double *) Y.p, yp0, upper);
   else
     hipLaunchKernelGGL (gemm_nn_k<float>, g, dim3 (TPB), 0, stream (),
-                        X.nrows, (const float *) X.p, xp0, npx, C, ldc,
-                        (float) alpha, (float) beta, (float *) Y.p, yp0);
+                        X.nrows, ntiles, npy, (const float *) X.p, xp0, npx,
+                        C, ldc, (float) alpha, (float) beta, (float *) Y.p,
+                        yp0, upper);
   LGH_LAUNCH_CHECK ();
   return 0;
 }

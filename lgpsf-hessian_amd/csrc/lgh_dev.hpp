@@ -190,8 +190,10 @@ int k_gram (const PMat &X, int xp0, int npx, const PMat &Y, int yp0, int npy,
             double *G, int ldg, int sym);
 // Y[:, panels yp0..yp0+npy) = alpha X[:, panels xp0..xp0+npx) C + beta Y
 //   C: fp64 col-major (npx*64) x (npy*64), ldc
+//   upper: C is block upper triangular (zero blocks p > q are skipped)
 int k_gemm_nn (const PMat &X, int xp0, int npx, const double *C, int ldc,
-               double alpha, double beta, PMat &Y, int yp0, int npy);
+               double alpha, double beta, PMat &Y, int yp0, int npy,
+               int upper = 0);
 
 // ---- small dense helpers on device (fp64) ------------------------------
 int k_symmetrize (double *T, int n, int ld);

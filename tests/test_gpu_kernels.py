@@ -79,6 +79,26 @@ def test_gram_and_gemm(dev, prec, tol):
     assert rel(S, X.T @ X) < tol
 
 
+def test_gram_gemm_many_tiles(dev):
+    """Row counts that leave partial row tiles and partial groups of eight
+    tiles / chunks (the block order groups them per XCD), the symmetric
+    Gram, and a block upper triangular C (the CholQR product, which skips
+    the zero blocks) against numpy."""
+    rng = np.random.default_rng(4)
+    n = 40_037
+    X = rng.standard_normal((n, 300))
+    Y = rng.standard_normal((n, 70))
+    assert rel(dev.gram(X, Y), X.T @ Y) < 1e-13
+    assert rel(dev.gram(X, X), X.T @ X) < 1e-13
+    Cu = np.triu(rng.standard_normal((300, 300)))
+    assert rel(dev.gemm_nn(X, Cu), X @ Cu) < 1e-13
+    Cb = Cu.copy()
+    Cb[200, 10] = 1.0                       # one entry below the blocks
+    assert rel(dev.gemm_nn(X, Cb), X @ Cb) < 1e-13
+    Cr = rng.standard_normal((300, 130))
+    assert rel(dev.gemm_nn(X, Cr), X @ Cr) < 1e-13
+
+
 def test_vcycle_matches_oracle(dev, oracle):
     Z, M = W.prior_Z(40, 2, 1.0, 0.1)
     h = dev.AMG(Z)
