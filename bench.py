@@ -247,8 +247,9 @@ def main():
     t_mixed = (time.perf_counter() - t0) / args.steps
     lgh.prof_enable(False)
     # one kernel instantiation at one launch shape per kind below: the
-    # full-width (all ell columns) launches on the finest level
-    npan = (args.ell + 63) // 64
+    # full-width launches on the finest level (all whole 64-column panels;
+    # a last panel of at most 16 columns runs as its own 16-wide panel)
+    npan = args.ell // 64 if 0 < args.ell % 64 <= 16 else (args.ell + 63) // 64
     vpc = pcode[args.vcycle]
     single = ("cheb_mixed", "smooth_pre_fine", "smooth_post_fine", "resid_fine")
     prof = {k: lgh.prof_collect(k, npan, vpc) if k in single else lgh.prof_collect(k)

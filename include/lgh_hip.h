@@ -89,7 +89,7 @@ int  lgh_amg_level (const lgh_amg_t *h, int l, int *n, int *nc,
                     const int **prp, const int **pci, const double **pv,
                     double *slo, double *shi, int *nu);
 /* X = V-cycle(B) on the device, ncols columns; prec 0 FP64, 1 FP32,
- * 2 BF16 vectors with FP32 matrices and arithmetic (W = 64 only) */
+ * 2 BF16 vectors with FP32 matrices and arithmetic (W = 64 or 16) */
 int  lgh_amg_vcycle (const lgh_amg_t *h, int prec, int W, int ncols,
                      const double *B, int ldb, double *X, int ldx);
 

@@ -64,7 +64,7 @@ dev_sync ()
 }
 
 // ---- live kernel timing -----------------------------------------------------
-struct ProfRec { int kind, np, prec; hipEvent_t a, b; double bytes, flops; };
+struct ProfRec { int kind, np, prec, W; hipEvent_t a, b; double bytes, flops; };
 static int g_prof = 0;
 static thread_local int g_btag = 0, g_ftag = 0;
 static std::vector<ProfRec> g_recs;
@@ -86,8 +86,8 @@ int prof_b_tag () { return g_btag; }
 void prof_set_fine_tag (int on) { g_ftag = on; }
 int prof_fine_tag () { return g_ftag; }
 
-ProfScope::ProfScope (int k, double by, double fl, int np_, int prec_)
-  : kind (k), np (np_), prec (prec_), bytes (by), flops (fl), active (g_prof),
+ProfScope::ProfScope (int k, double by, double fl, int np_, int prec_, int W_)
+  : kind (k), np (np_), prec (prec_), W (W_), bytes (by), flops (fl), active (g_prof),
     ev0 (nullptr)
 {
   if (!active) return;
@@ -101,7 +101,7 @@ ProfScope::~ProfScope ()
   if (!active) return;
   hipEvent_t b = ev_get ();
   (void) hipEventRecord (b, g_stream);
-  g_recs.push_back ({ kind, np, prec, (hipEvent_t) ev0, b, bytes, flops });
+  g_recs.push_back ({ kind, np, prec, W, (hipEvent_t) ev0, b, bytes, flops });
 }
 
 int
@@ -114,7 +114,7 @@ prof_collect (int kind, int np, int prec, long *count, double *ms,
   std::vector<ProfRec> keep;
   for (auto &r : g_recs) {
     if (r.kind != kind) { keep.push_back (r); continue; }
-    if ((np < 0 || r.np == np) && (prec < 0 || r.prec == prec)) {
+    if ((np < 0 || (r.np == np && r.W == 64)) && (prec < 0 || r.prec == prec)) {
       float e = 0.f;
       LGH_HIP (hipEventElapsedTime (&e, r.a, r.b));
       c++; t += e; by += r.bytes; fl += r.flops;

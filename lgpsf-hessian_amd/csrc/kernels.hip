@@ -270,36 +270,49 @@ __device__ inline void vzero (T (&x)[N])
   for (int i = 0; i < N; i++) x[i] = (T) 0;
 }
 
+// row map of a panel of width W (64, or 16 for a narrow last panel): W / N
+// lanes per row, 16 B per lane
 #define LGH_ROWMAP(T_)                                                        \
   constexpr int N = VT<T_>::N;                                                \
-  constexpr int LPR = 64 / N;                                                 \
+  constexpr int LPR = W / N;                                                  \
   const int row = blockIdx.x * (TPB / LPR) + (int) threadIdx.x / LPR;         \
-  const size_t off = (size_t) blockIdx.y * n * 64                             \
+  const size_t off = (size_t) blockIdx.y * n * W                              \
                      + (size_t) ((int) threadIdx.x % LPR) * N;                \
   if (row >= n) return;                                                       \
   const int k0 = rp[row], k1 = rp[row + 1];
 
-// grid of a W = 64 row kernel whose row map follows storage precision prec
+// grid of a row kernel (panel width W) whose row map follows storage
+// precision prec
 static inline dim3
-grid64 (int n, int np, int prec)
+grid64 (int n, int np, int prec, int W = 64)
 {
-  const int rows = TPB / (64 / (16 / (int) prec_bytes (prec)));
+  const int rows = TPB / (W / (16 / (int) prec_bytes (prec)));
   return dim3 ((unsigned) ((n + rows - 1) / rows), (unsigned) np);
+}
+// the row kernels' widths: a full panel, or the narrow last panel of a
+// block whose column count leaves at most 16 in it
+static inline bool row_width (int W) { return W == 64 || W == 16; }
+template <typename F>
+static int
+with_row_W (int W, F &&f)
+{
+  if (W == 64) return f (IC<64> ());
+  return f (IC<16> ());
 }
 
 // Y = alpha A X + beta Y (T arithmetic, S storage of X and Y)
-template <typename TV, typename T, typename S>
+template <typename TV, typename T, typename S, int W = 64>
 __global__ void __launch_bounds__ (TPB)
 spmm64_k (int n, int nx, const int *__restrict__ rp,
           const int *__restrict__ ci, const TV *__restrict__ v, T alpha,
           const S *__restrict__ X, T beta, S *__restrict__ Y)
 {
   constexpr int N = VT<S>::N;
-  constexpr int LPR = 64 / N;
+  constexpr int LPR = W / N;
   const int row = blockIdx.x * (TPB / LPR) + (int) threadIdx.x / LPR;
   const int lc = ((int) threadIdx.x % LPR) * N;
   if (row >= n) return;
-  const S *Xp = X + (size_t) blockIdx.y * nx * 64 + lc;
+  const S *Xp = X + (size_t) blockIdx.y * nx * W + lc;
   const int k0 = rp[row], k1 = rp[row + 1];
   T acc[N];
 #pragma unroll
@@ -316,7 +329,7 @@ spmm64_k (int n, int nx, const int *__restrict__ rp,
     typename Raw16<S>::type x[NG];
 #pragma unroll
     for (int u = 0; u < NG; u++) {
-      if (kb + u < k1) x[u] = rld (Xp + (size_t) c[u] * 64);
+      if (kb + u < k1) x[u] = rld (Xp + (size_t) c[u] * W);
       else x[u] = {};
     }
 #pragma unroll
@@ -327,7 +340,7 @@ spmm64_k (int n, int nx, const int *__restrict__ rp,
       for (int i = 0; i < N; i++) acc[i] += w[u] * t[i];
     }
   }
-  S *y = Y + ((size_t) blockIdx.y * n + row) * 64 + lc;
+  S *y = Y + ((size_t) blockIdx.y * n + row) * W + lc;
   if (beta == (T) 0) {
 #pragma unroll
     for (int i = 0; i < N; i++) acc[i] *= alpha;
@@ -342,7 +355,7 @@ spmm64_k (int n, int nx, const int *__restrict__ rp,
 }
 
 // R = B - A X
-template <typename TV, typename T, typename S>
+template <typename TV, typename T, typename S, int W = 64>
 __global__ void __launch_bounds__ (TPB)
 resid64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
            const TV *__restrict__ v, const S *__restrict__ B,
@@ -364,7 +377,7 @@ resid64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
     typename Raw16<S>::type x[NG];
 #pragma unroll
     for (int u = 0; u < NG; u++) {
-      if (kb + u < k1) x[u] = rld (X + off + (size_t) c[u] * 64);
+      if (kb + u < k1) x[u] = rld (X + off + (size_t) c[u] * W);
       else x[u] = {};
     }
 #pragma unroll
@@ -376,10 +389,10 @@ resid64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
     }
   }
   T b[N];
-  vld (B + off + (size_t) row * 64, b);
+  vld (B + off + (size_t) row * W, b);
 #pragma unroll
   for (int i = 0; i < N; i++) b[i] -= acc[i];
-  vst (R + off + (size_t) row * 64, b);
+  vst (R + off + (size_t) row * W, b);
 }
 
 // fused Chebyshev step (zsolve_impl.h:104-141): Dn = a D + b Z with
@@ -388,7 +401,7 @@ resid64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
 // (SP: storage of the preconditioned residual Zs and of Rpc, the copy of
 // the new residual that feeds the preconditioner when it runs in a lower
 // precision; Rpc may be null)
-template <typename TV, typename T, bool JAC, bool HASD, typename SP>
+template <typename TV, typename T, bool JAC, bool HASD, typename SP, int W = 64>
 __global__ void __launch_bounds__ (TPB)
 cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
           const TV *__restrict__ v, const TV *__restrict__ dinv,
@@ -434,7 +447,7 @@ cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
     T y[NG][N];
 #pragma unroll
     for (int u = 0; u < NG; u++) {
-      if (kb + u < k1) dnew (off + (size_t) c[u] * 64, dd[u], y[u]);
+      if (kb + u < k1) dnew (off + (size_t) c[u] * W, dd[u], y[u]);
       else vzero (y[u]);
     }
 #pragma unroll
@@ -442,7 +455,7 @@ cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
 #pragma unroll
       for (int i = 0; i < N; i++) acc[i] += w[u] * y[u][i];
   }
-  const size_t qi = off + (size_t) row * 64;
+  const size_t qi = off + (size_t) row * W;
   T di[N], r[N], x[N];
   dnew (qi, JAC ? (T) dinv[row] : (T) 0, di);
   vld (R + qi, r);
@@ -472,7 +485,7 @@ cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
 // writes R in SP for the next preconditioner application.  (A BF16
 // direction saves 4 of ~44 B per element but its 2^-9 rounding, repeated
 // over the 25 steps, costs the solve a factor ~2 in accuracy.)
-template <bool HASD, typename SP>
+template <bool HASD, typename SP, int W = 64>
 __global__ void __launch_bounds__ (TPB)
 cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
            const double *__restrict__ v, const SP *__restrict__ Zs,
@@ -483,9 +496,9 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
 {
   // 16 lanes per row, 4 values per lane (fewer, wider loads than the fp64
   // map: the gathered Z and D are 8-16 B per lane)
-  constexpr int N = 4, LPR = 16;
+  constexpr int N = 4, LPR = W / N;
   const int row = blockIdx.x * (TPB / LPR) + (int) threadIdx.x / LPR;
-  const size_t off = (size_t) blockIdx.y * n * 64
+  const size_t off = (size_t) blockIdx.y * n * W
                      + (size_t) ((int) threadIdx.x % LPR) * N;
   if (row >= n) return;
   const int k0 = rp[row], k1 = rp[row + 1];
@@ -509,7 +522,7 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
       zr[u] = {};
       dr[u] = {};
       if (kb + u < k1) {
-        const size_t q = off + (size_t) c[u] * 64;
+        const size_t q = off + (size_t) c[u] * W;
         zr[u] = rld4 (Zs + q);
         if (HASD) dr[u] = rld4 (D + q);
       }
@@ -528,7 +541,7 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
       }
     }
   }
-  const size_t qi = off + (size_t) row * 64;
+  const size_t qi = off + (size_t) row * W;
   double di[N], r[N], x[N], dold[N];
   {
     double z[N];
@@ -564,7 +577,10 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                    const PMat *D32, const PMat &R, PMat &X, int x_mode,
                    PMat &Dn32, PMat &Rn, PMat &Rnpc)
 {
-  LGH_CHECK (A.prec == F64 && R.prec == F64 && X.prec == F64 && R.W == 64
+  const int W = R.W;
+  LGH_CHECK (A.prec == F64 && R.prec == F64 && X.prec == F64 && row_width (W)
+             && X.W == W && Zpc.W == W && Dn32.W == W && Rn.W == W
+             && Rnpc.W == W && (!D32 || D32->W == W)
              && (Zpc.prec == F32 || Zpc.prec == BF16) && Rnpc.prec == Zpc.prec
              && Dn32.prec == F32 && (!D32 || D32->prec == F32)
              && Rn.prec == F64 && R.nrows == A.n, 8,
@@ -572,7 +588,7 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
   LGH_CHECK (x_mode >= MIX_X_SET && x_mode <= MIX_X_ADD2
              && (D32 || (x_mode != MIX_X_SET2 && x_mode != MIX_X_ADD2)), 8,
              "k_cheb_step_mixed: bad X mode");
-  const double vec = (double) A.n * 64;
+  const double vec = (double) A.n * W;
   const double xb = (x_mode == MIX_X_SKIP) ? 0 : (x_mode == MIX_X_SET
                      || x_mode == MIX_X_SET2) ? 8 : 16;
   const double pb = (double) prec_bytes (Zpc.prec);
@@ -580,19 +596,20 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
   // PK_CHEB_MIXED times one kernel instantiation
   ProfScope ps (D32 ? PK_CHEB_MIXED : PK_OTHER, R.npanels () * (csr_bytes (A)
                 + vec * (pb + (D32 ? 4 : 0) + 8 + xb + 4 + 8 + pb)),
-                R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * 64,
-                R.npanels (), Zpc.prec);
-  const dim3 g = grid64 (A.n, R.npanels (), F32);     // 16 lanes per row
-  auto go = [&] (auto sp) -> int {
+                R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * W,
+                R.npanels (), Zpc.prec, W);
+  const dim3 g = grid64 (A.n, R.npanels (), F32, W);   // 4 columns per lane
+  auto go = [&] (auto sp, auto wc) -> int {
     using SP = decltype (sp);
+    constexpr int WW = decltype (wc)::value;
     if (D32)
-      hipLaunchKernelGGL ((cheb64m_k<true, SP>), g, dim3 (TPB), 0, stream (),
+      hipLaunchKernelGGL ((cheb64m_k<true, SP, WW>), g, dim3 (TPB), 0, stream (),
                           A.n, A.rp, A.ci, (const double *) A.v,
                           (const SP *) Zpc.p, a, b, (const float *) D32->p,
                           (const double *) R.p, (double *) X.p, x_mode,
                           (float *) Dn32.p, (double *) Rn.p, (SP *) Rnpc.p);
     else
-      hipLaunchKernelGGL ((cheb64m_k<false, SP>), g, dim3 (TPB), 0, stream (),
+      hipLaunchKernelGGL ((cheb64m_k<false, SP, WW>), g, dim3 (TPB), 0, stream (),
                           A.n, A.rp, A.ci, (const double *) A.v,
                           (const SP *) Zpc.p, a, b, nullptr,
                           (const double *) R.p, (double *) X.p, x_mode,
@@ -600,7 +617,9 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
     LGH_LAUNCH_CHECK ();
     return 0;
   };
-  return Zpc.prec == BF16 ? go (bf16_t ()) : go (0.f);
+  return with_row_W (W, [&] (auto wc) {
+    return Zpc.prec == BF16 ? go (bf16_t (), wc) : go (0.f, wc);
+  });
 }
 
 // degree-2 correction-form smoother (zsolve_impl.h:385-404 with nu = 2)
@@ -608,7 +627,7 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
 // radius 1: R1(i) = R0(i) - sum_j A_ij D1(j) with D1(j) = b1 dinv_j R0(j),
 // then D2(i) = a2 D1(i) + b2 dinv_i R1(i).  Same arithmetic as sm1_k +
 // sm2_k; D1, R1 and R2 are never stored.
-template <typename TV, typename T, typename S, bool XADD>
+template <typename TV, typename T, typename S, bool XADD, int W = 64>
 __global__ void __launch_bounds__ (TPB)
 smx_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
        const TV *__restrict__ v, const TV *__restrict__ dinv, T b1, T a2,
@@ -632,7 +651,7 @@ smx_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
     typename Raw16<S>::type x[NG];
 #pragma unroll
     for (int u = 0; u < NG; u++) {
-      if (kb + u < k1) x[u] = rld (R0 + off + (size_t) c[u] * 64);
+      if (kb + u < k1) x[u] = rld (R0 + off + (size_t) c[u] * W);
       else x[u] = {};
     }
     // sum_j A_ij D1(j) with D1(j) = b1 dinv_j R0(j): one FMA per value
@@ -644,7 +663,7 @@ smx_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
       for (int i = 0; i < N; i++) acc[i] += dd[u] * t[i];
     }
   }
-  const size_t qi = off + (size_t) row * 64;
+  const size_t qi = off + (size_t) row * W;
   const T dr = (T) dinv[row];
   T r0[N], x0[N], out[N];
   vld (R0 + qi, r0);
@@ -665,36 +684,39 @@ int
 k_smooth2x (const DevCSR &A, double b1, double a2, double b2, const PMat &R0,
             PMat &X, int xadd)
 {
-  LGH_CHECK (R0.W == 64 && X.W == 64 && R0.prec == X.prec && A.dinv
+  LGH_CHECK (row_width (R0.W) && X.W == R0.W && R0.prec == X.prec && A.dinv
              && R0.nrows == A.n && X.nrows == A.n
              && R0.npanels () == X.npanels ()
              && (R0.prec != BF16 || A.prec == F32), 8, "k_smooth2x: shape mismatch");
-  const double vec = (double) A.n * 64 * prec_bytes (R0.prec);
+  const double vec = (double) A.n * R0.W * prec_bytes (R0.prec);
   const int kind = prof_fine_tag () ? (xadd ? PK_SM_POST : PK_SM_PRE) : PK_CHEB_JAC;
   ProfScope ps (kind, R0.npanels () * (csr_bytes (A) + (double) A.n * prec_bytes (A.prec)
                                       + (2 + (xadd ? 1 : 0)) * vec),
-                R0.npanels () * (2.0 * (double) A.nnz + 9.0 * A.n) * 64,
-                R0.npanels (), R0.prec);
-  const dim3 g = grid64 (A.n, R0.npanels (), R0.prec);
-  auto go = [&] (auto tv, auto t, auto st) -> int {
+                R0.npanels () * (2.0 * (double) A.nnz + 9.0 * A.n) * R0.W,
+                R0.npanels (), R0.prec, R0.W);
+  const dim3 g = grid64 (A.n, R0.npanels (), R0.prec, R0.W);
+  auto go = [&] (auto tv, auto t, auto st, auto wc) -> int {
     using TV = decltype (tv);
     using T = decltype (t);
     using S = decltype (st);
+    constexpr int W = decltype (wc)::value;
     if (xadd)
-      hipLaunchKernelGGL ((smx_k<TV, T, S, true>), g, dim3 (TPB), 0, stream (),
+      hipLaunchKernelGGL ((smx_k<TV, T, S, true, W>), g, dim3 (TPB), 0, stream (),
                           A.n, A.rp, A.ci, (const TV *) A.v, (const TV *) A.dinv,
                           (T) b1, (T) a2, (T) b2, (const S *) R0.p, (S *) X.p);
     else
-      hipLaunchKernelGGL ((smx_k<TV, T, S, false>), g, dim3 (TPB), 0, stream (),
+      hipLaunchKernelGGL ((smx_k<TV, T, S, false, W>), g, dim3 (TPB), 0, stream (),
                           A.n, A.rp, A.ci, (const TV *) A.v, (const TV *) A.dinv,
                           (T) b1, (T) a2, (T) b2, (const S *) R0.p, (S *) X.p);
     LGH_LAUNCH_CHECK ();
     return 0;
   };
-  if (R0.prec == BF16) return go (0.f, 0.f, bf16_t ());
-  if (R0.prec == F64)
-    return A.prec == F64 ? go (0.0, 0.0, 0.0) : go (0.f, 0.0, 0.0);
-  return A.prec == F64 ? go (0.0, 0.f, 0.f) : go (0.f, 0.f, 0.f);
+  return with_row_W (R0.W, [&] (auto wc) {
+    if (R0.prec == BF16) return go (0.f, 0.f, bf16_t (), wc);
+    if (R0.prec == F64)
+      return A.prec == F64 ? go (0.0, 0.0, 0.0, wc) : go (0.f, 0.0, 0.0, wc);
+    return A.prec == F64 ? go (0.0, 0.f, 0.f, wc) : go (0.f, 0.f, 0.f, wc);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -725,15 +747,18 @@ template <typename TV, typename T, typename S = T>
 static int
 spmm_t (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
 {
-  if (X.W == 64) {
-    hipLaunchKernelGGL ((spmm64_k<TV, T, S>), grid64 (A.n, X.npanels (), X.prec),
-                        dim3 (TPB), 0, stream (), A.n, A.m, A.rp, A.ci,
-                        (const TV *) A.v, (T) alpha, (const S *) X.p, (T) beta,
-                        (S *) Y.p);
-    LGH_LAUNCH_CHECK ();
-    return 0;
-  }
-  LGH_CHECK ((std::is_same<S, T>::value), 8, "k_spmm: bf16 storage needs W = 64");
+  if (row_width (X.W))
+    return with_row_W (X.W, [&] (auto wc) -> int {
+      constexpr int W = decltype (wc)::value;
+      hipLaunchKernelGGL ((spmm64_k<TV, T, S, W>),
+                          grid64 (A.n, X.npanels (), X.prec, W), dim3 (TPB), 0,
+                          stream (), A.n, A.m, A.rp, A.ci, (const TV *) A.v,
+                          (T) alpha, (const S *) X.p, (T) beta, (S *) Y.p);
+      LGH_LAUNCH_CHECK ();
+      return 0;
+    });
+  LGH_CHECK ((std::is_same<S, T>::value), 8,
+             "k_spmm: bf16 storage needs W = 64 or 16");
   auto l = [&] (auto wt) -> int {
       constexpr int W = decltype (wt)::value;
       constexpr int rows_per_block = (TPB / 64) * (64 / W);
@@ -756,7 +781,7 @@ k_spmm (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
   ProfScope ps (prof_b_tag () ? PK_SPMM_B : PK_SPMM,
                 np_ * (csr_bytes (A) + (double) A.m * X.W * sx
                        + (double) A.n * X.W * sx * (beta != 0.0 ? 2 : 1)),
-                np_ * 2.0 * (double) A.nnz * X.W, (int) np_, X.prec);
+                np_ * 2.0 * (double) A.nnz * X.W, (int) np_, X.prec, X.W);
   if (X.prec == BF16) {
     LGH_CHECK (A.prec == F32, 8, "k_spmm: bf16 vectors need fp32 values");
     return spmm_t<float, float, bf16_t> (A, alpha, X, beta, Y);
@@ -794,15 +819,18 @@ template <typename TV, typename T, typename S = T>
 static int
 resid_t (const DevCSR &A, const PMat &B, const PMat &X, PMat &R)
 {
-  if (X.W == 64) {
-    hipLaunchKernelGGL ((resid64_k<TV, T, S>), grid64 (A.n, X.npanels (), X.prec),
-                        dim3 (TPB), 0, stream (), A.n, A.rp, A.ci,
-                        (const TV *) A.v, (const S *) B.p, (const S *) X.p,
-                        (S *) R.p);
-    LGH_LAUNCH_CHECK ();
-    return 0;
-  }
-  LGH_CHECK ((std::is_same<S, T>::value), 8, "k_resid: bf16 storage needs W = 64");
+  if (row_width (X.W))
+    return with_row_W (X.W, [&] (auto wc) -> int {
+      constexpr int W = decltype (wc)::value;
+      hipLaunchKernelGGL ((resid64_k<TV, T, S, W>),
+                          grid64 (A.n, X.npanels (), X.prec, W), dim3 (TPB), 0,
+                          stream (), A.n, A.rp, A.ci, (const TV *) A.v,
+                          (const S *) B.p, (const S *) X.p, (S *) R.p);
+      LGH_LAUNCH_CHECK ();
+      return 0;
+    });
+  LGH_CHECK ((std::is_same<S, T>::value), 8,
+             "k_resid: bf16 storage needs W = 64 or 16");
   auto l = [&] (auto wt) -> int {
       constexpr int W = decltype (wt)::value;
       constexpr int rows_per_block = (TPB / 64) * (64 / W);
@@ -823,7 +851,8 @@ k_resid (const DevCSR &A, const PMat &B, const PMat &X, PMat &R)
              && X.nrows == A.n, 8, "k_resid: shape mismatch");
   ProfScope ps (prof_fine_tag () ? PK_RESID_FINE : PK_RESID,
                 X.npanels () * (csr_bytes (A) + 3.0 * A.n * X.W * prec_bytes (X.prec)),
-                X.npanels () * 2.0 * (double) A.nnz * X.W, X.npanels (), X.prec);
+                X.npanels () * 2.0 * (double) A.nnz * X.W, X.npanels (), X.prec,
+                X.W);
   if (X.prec == BF16) {
     LGH_CHECK (A.prec == F32, 8, "k_resid: bf16 vectors need fp32 values");
     return resid_t<float, float, bf16_t> (A, B, X, R);
@@ -882,25 +911,27 @@ cheb_t (const DevCSR &A, int jac, const PMat *Zs, double a, double b,
         const PMat *D, const PMat &R, PMat &X, int xmode, PMat &Dn, PMat &Rn,
         PMat *Rpc)
 {
-  if (R.W == 64) {
-    const dim3 g = grid64 (A.n, R.npanels (), vt<T>::prec);
-    const SP *zp = Zs ? (const SP *) Zs->p : nullptr;
-    const T *dp = D ? (const T *) D->p : nullptr;
-    SP *rpc = Rpc ? (SP *) Rpc->p : nullptr;
+  if (row_width (R.W))
+    return with_row_W (R.W, [&] (auto wc) -> int {
+      constexpr int W = decltype (wc)::value;
+      const dim3 g = grid64 (A.n, R.npanels (), vt<T>::prec, W);
+      const SP *zp = Zs ? (const SP *) Zs->p : nullptr;
+      const T *dp = D ? (const T *) D->p : nullptr;
+      SP *rpc = Rpc ? (SP *) Rpc->p : nullptr;
 #define LGH_CH64(J, H)                                                        \
-    hipLaunchKernelGGL ((cheb64_k<TV, T, J, H, SP>), g, dim3 (TPB), 0,        \
-                        stream (), A.n, A.rp, A.ci, (const TV *) A.v,          \
-                        (const TV *) A.dinv, zp, (T) a, (T) b, dp,             \
-                        (const T *) R.p, (T *) X.p, xmode, (T *) Dn.p,         \
-                        (T *) Rn.p, rpc)
-    if (jac) { if (dp) LGH_CH64 (true, true); else LGH_CH64 (true, false); }
-    else { if (dp) LGH_CH64 (false, true); else LGH_CH64 (false, false); }
+      hipLaunchKernelGGL ((cheb64_k<TV, T, J, H, SP, W>), g, dim3 (TPB), 0,   \
+                          stream (), A.n, A.rp, A.ci, (const TV *) A.v,        \
+                          (const TV *) A.dinv, zp, (T) a, (T) b, dp,           \
+                          (const T *) R.p, (T *) X.p, xmode, (T *) Dn.p,       \
+                          (T *) Rn.p, rpc)
+      if (jac) { if (dp) LGH_CH64 (true, true); else LGH_CH64 (true, false); }
+      else { if (dp) LGH_CH64 (false, true); else LGH_CH64 (false, false); }
 #undef LGH_CH64
-    LGH_LAUNCH_CHECK ();
-    return 0;
-  }
+      LGH_LAUNCH_CHECK ();
+      return 0;
+    });
   LGH_CHECK ((std::is_same<SP, T>::value) && !Rpc, 8,
-             "k_cheb_step: a lower-precision preconditioner needs W = 64");
+             "k_cheb_step: a lower-precision preconditioner needs W = 64 or 16");
   auto l = [&] (auto wt) -> int {
       constexpr int W = decltype (wt)::value;
       constexpr int rows_per_block = (TPB / 64) * (64 / W);
@@ -931,7 +962,7 @@ k_cheb_step (const DevCSR &A, int jacobi, const PMat *Zsrc, double a,
              && X.prec == R.prec && (!jacobi || A.dinv), 8,
              "k_cheb_step: shape mismatch");
   const int pc = Zsrc ? Zsrc->prec : R.prec;
-  LGH_CHECK ((pc == R.prec || (pc == BF16 && R.prec == F32 && R.W == 64))
+  LGH_CHECK ((pc == R.prec || (pc == BF16 && R.prec == F32 && row_width (R.W)))
              && (!Rpc || Rpc->prec == pc), 8,
              "k_cheb_step: preconditioner precision");
   // streams: read R, (D), (Zsrc), (X); write Dn, X, Rn, (Rpc)
@@ -943,7 +974,7 @@ k_cheb_step (const DevCSR &A, int jacobi, const PMat *Zsrc, double a,
                                 + ((jacobi ? 0 : 1) + (Rpc ? 1 : 0)) * vpc
                                 + (jacobi ? (double) A.n * prec_bytes (A.prec) : 0.0)),
                 R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * R.W,
-                R.npanels (), R.prec);
+                R.npanels (), R.prec, R.W);
   if (pc == BF16)
     return A.prec == F64
         ? cheb_t<double, float, bf16_t> (A, jacobi, Zsrc, a, b, D, R, X, x_mode, Dn, Rn, Rpc)
@@ -1083,10 +1114,12 @@ __global__ void __launch_bounds__ (TPB)
 copy_cols_k (int nrows, int nc, const TI *X, int xW, int xc0, TO *Y, int yW,
              int yc0)
 {
+  // consecutive threads take consecutive columns of one row (contiguous
+  // within a panel row)
   size_t e = (size_t) blockIdx.x * TPB + threadIdx.x;
   if (e >= (size_t) nrows * nc) return;
-  int j = (int) (e / nrows);
-  int i = (int) (e % nrows);
+  int i = (int) (e / nc);
+  int j = (int) (e % nc);
   int xj = xc0 + j, yj = yc0 + j;
   size_t xi = ((size_t) (xj / xW) * nrows + i) * xW + xj % xW;
   size_t yi = ((size_t) (yj / yW) * nrows + i) * yW + yj % yW;

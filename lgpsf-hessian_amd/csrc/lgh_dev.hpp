@@ -86,15 +86,17 @@ int  prof_b_tag ();
 void prof_set_fine_tag (int on);   // next launches run on the finest level
 int  prof_fine_tag ();
 struct ProfScope {
-  int kind, np, prec; double bytes, flops; int active; void *ev0;
-  // np, prec: panels and vector storage precision of the launch (recorded
-  // so that one kernel instantiation at one shape can be selected)
-  ProfScope (int kind, double bytes, double flops, int np = 0, int prec = -1);
+  int kind, np, prec, W; double bytes, flops; int active; void *ev0;
+  // np, prec, W: panels, vector storage precision and panel width of the
+  // launch (recorded so that one kernel instantiation at one shape can be
+  // selected)
+  ProfScope (int kind, double bytes, double flops, int np = 0, int prec = -1,
+             int W = 64);
   ~ProfScope ();
 };
 // totals since the last collect of that kind (synchronizes the stream);
-// np >= 0 / prec >= 0 count only launches over np panels / at that storage
-// precision (the others are dropped too)
+// np >= 0 / prec >= 0 count only launches over np full-width (64) panels /
+// at that storage precision (the others are dropped too)
 int prof_collect (int kind, int np, int prec, long *count, double *ms,
                   double *bytes, double *flops);
 

@@ -16,12 +16,27 @@ DevMG::~DevMG ()
 }
 
 void
-DevMG::free_work ()
+DevMG::Work::free ()
 {
   for (auto *vv : { &R1, &R2, &D1, &D2, &Bl, &Xl })
     for (auto &m : *vv) pmat_free (m);
   R1.clear (); R2.clear (); D1.clear (); D2.clear (); Bl.clear (); Xl.clear ();
-  wnp = -1;
+  np = -1;
+}
+
+void
+DevMG::free_work ()
+{
+  work[0].free ();
+  work[1].free ();
+}
+
+// the one-pass degree-2 smoother and explicit residuals serve the row
+// kernels' widths
+static inline bool
+fused_level (const DevMG &g, int l, int W)
+{
+  return g.nu[l] == 2 && (W == 64 || W == 16);
 }
 
 int
@@ -72,30 +87,32 @@ mg_upload (const AmgHierarchy &h, int prec, DevMG **out)
 }
 
 static int
-mg_work (DevMG &g, const PMat &like)
+mg_work (DevMG &g, const PMat &like, DevMG::Work **out)
 {
   const int np = like.npanels ();
-  if (g.wW == like.W && g.wnp == np) return 0;
-  g.free_work ();
-  g.R1.resize (g.nl); g.R2.resize (g.nl); g.D1.resize (g.nl);
-  g.D2.resize (g.nl); g.Bl.resize (g.nl); g.Xl.resize (g.nl);
+  DevMG::Work &w = g.work[like.W == 64 ? 0 : 1];
+  *out = &w;
+  if (w.W == like.W && w.np == np) return 0;
+  w.free ();
+  w.R1.resize (g.nl); w.R2.resize (g.nl); w.D1.resize (g.nl);
+  w.D2.resize (g.nl); w.Bl.resize (g.nl); w.Xl.resize (g.nl);
   const int nc = np * like.W;
   for (int l = 0; l < g.nl; l++) {
     if (l > 0) {
-      LGH_TRY (pmat_alloc (g.R1[l], g.n[l], nc, like.W, g.prec));
-      if (!(g.nu[l] == 2 && like.W == 64)) {   // the per-step smoother path
-        LGH_TRY (pmat_alloc (g.R2[l], g.n[l], nc, like.W, g.prec));
-        LGH_TRY (pmat_alloc (g.D1[l], g.n[l], nc, like.W, g.prec));
-        LGH_TRY (pmat_alloc (g.D2[l], g.n[l], nc, like.W, g.prec));
+      LGH_TRY (pmat_alloc (w.R1[l], g.n[l], nc, like.W, g.prec));
+      if (!fused_level (g, l, like.W)) {        // the per-step smoother path
+        LGH_TRY (pmat_alloc (w.R2[l], g.n[l], nc, like.W, g.prec));
+        LGH_TRY (pmat_alloc (w.D1[l], g.n[l], nc, like.W, g.prec));
+        LGH_TRY (pmat_alloc (w.D2[l], g.n[l], nc, like.W, g.prec));
       }
     }
     if (l < g.nl - 1) {
-      LGH_TRY (pmat_alloc (g.Bl[l], g.n[l], nc, like.W, g.prec));
-      LGH_TRY (pmat_alloc (g.Xl[l], g.n[l], nc, like.W, g.prec));
+      LGH_TRY (pmat_alloc (w.Bl[l], g.n[l], nc, like.W, g.prec));
+      LGH_TRY (pmat_alloc (w.Xl[l], g.n[l], nc, like.W, g.prec));
     }
   }
-  g.wW = like.W;
-  g.wnp = np;
+  w.W = like.W;
+  w.np = np;
   return 0;
 }
 
@@ -125,20 +142,20 @@ struct ChebCoef {
 // *Rout_final receives the buffer holding the recurrence residual B - A X
 // after the last step.
 static int
-mg_smooth (DevMG &g, int l, const PMat &B, PMat &X, int x_zero,
-           const PMat **Rfinal)
+mg_smooth (DevMG &g, DevMG::Work &w, int l, const PMat &B, PMat &X,
+           int x_zero, const PMat **Rfinal)
 {
   const DevCSR &A = *g.A[l];
   ChebCoef cc (g.slo[l], g.shi[l]);
-  PMat *Rin = &g.R1[l], *Rout = &g.R2[l];
-  PMat *Dcur = nullptr, *Dn = &g.D1[l];
+  PMat *Rin = &w.R1[l], *Rout = &w.R2[l];
+  PMat *Dcur = nullptr, *Dn = &w.D1[l];
   const PMat *R0 = &B;
   if (!x_zero) {
-    LGH_TRY (k_resid (A, B, X, g.R1[l]));
-    R0 = &g.R1[l];
-    Rout = &g.R2[l];
+    LGH_TRY (k_resid (A, B, X, w.R1[l]));
+    R0 = &w.R1[l];
+    Rout = &w.R2[l];
   }
-  else Rout = &g.R1[l];
+  else Rout = &w.R1[l];
   for (int k = 1; k <= g.nu[l]; k++) {
     double a, b;
     cc.step (k, &a, &b);
@@ -147,9 +164,9 @@ mg_smooth (DevMG &g, int l, const PMat &B, PMat &X, int x_zero,
     LGH_TRY (k_cheb_step (A, 1, nullptr, a, b, Dcur, *Rsrc, X, xmode, *Dn, *Rout));
     // rotate: the output residual/direction become the inputs
     Rin = Rout;
-    Rout = (Rin == &g.R1[l]) ? &g.R2[l] : &g.R1[l];
+    Rout = (Rin == &w.R1[l]) ? &w.R2[l] : &w.R1[l];
     Dcur = Dn;
-    Dn = (Dcur == &g.D1[l]) ? &g.D2[l] : &g.D1[l];
+    Dn = (Dcur == &w.D1[l]) ? &w.D2[l] : &w.D1[l];
   }
   if (Rfinal) *Rfinal = (g.nu[l] >= 1) ? Rin : R0;
   return 0;
@@ -157,10 +174,10 @@ mg_smooth (DevMG &g, int l, const PMat &B, PMat &X, int x_zero,
 
 // zsolve_impl.h:406-485 (X overwritten; zero initial guess)
 static int
-mg_run (DevMG &g, int l, const PMat &B, PMat &X)
+mg_run (DevMG &g, DevMG::Work &w, int l, const PMat &B, PMat &X)
 {
   if (l == 0) return k_dense_apply (g.cinv, g.n0, B, X);
-  if (g.nu[l] == 2 && B.W == 64) {
+  if (fused_level (g, l, B.W)) {
     // degree-2 smoother as one pass (k_smooth2x), explicit residuals
     ChebCoef cc (g.slo[l], g.shi[l]);
     double a1, b1, a2, b2;
@@ -169,14 +186,14 @@ mg_run (DevMG &g, int l, const PMat &B, PMat &X)
     const int fine = (l == g.nl - 1);
     prof_set_fine_tag (fine);
     LGH_TRY (k_smooth2x (*g.A[l], b1, a2, b2, B, X, 0));             // pre
-    LGH_TRY (k_resid (*g.A[l], B, X, g.R1[l]));
+    LGH_TRY (k_resid (*g.A[l], B, X, w.R1[l]));
     prof_set_fine_tag (0);
-    LGH_TRY (k_spmm (*g.PT[l], 1.0, g.R1[l], 0.0, g.Bl[l - 1]));
-    LGH_TRY (mg_run (g, l - 1, g.Bl[l - 1], g.Xl[l - 1]));
-    LGH_TRY (k_spmm (*g.P[l], 1.0, g.Xl[l - 1], 1.0, X));
+    LGH_TRY (k_spmm (*g.PT[l], 1.0, w.R1[l], 0.0, w.Bl[l - 1]));
+    LGH_TRY (mg_run (g, w, l - 1, w.Bl[l - 1], w.Xl[l - 1]));
+    LGH_TRY (k_spmm (*g.P[l], 1.0, w.Xl[l - 1], 1.0, X));
     prof_set_fine_tag (fine);
-    LGH_TRY (k_resid (*g.A[l], B, X, g.R1[l]));
-    LGH_TRY (k_smooth2x (*g.A[l], b1, a2, b2, g.R1[l], X, 1));       // post
+    LGH_TRY (k_resid (*g.A[l], B, X, w.R1[l]));
+    LGH_TRY (k_smooth2x (*g.A[l], b1, a2, b2, w.R1[l], X, 1));       // post
     prof_set_fine_tag (0);
     return 0;
   }
@@ -184,11 +201,11 @@ mg_run (DevMG &g, int l, const PMat &B, PMat &X)
   // recurrence keeps it) is restricted directly — the reference forms the
   // same residual again with one more product (zsolve_impl.h:464-468)
   const PMat *R = nullptr;
-  LGH_TRY (mg_smooth (g, l, B, X, 1, &R));
-  LGH_TRY (k_spmm (*g.PT[l], 1.0, *R, 0.0, g.Bl[l - 1]));
-  LGH_TRY (mg_run (g, l - 1, g.Bl[l - 1], g.Xl[l - 1]));
-  LGH_TRY (k_spmm (*g.P[l], 1.0, g.Xl[l - 1], 1.0, X));
-  LGH_TRY (mg_smooth (g, l, B, X, 0, nullptr));                  // post
+  LGH_TRY (mg_smooth (g, w, l, B, X, 1, &R));
+  LGH_TRY (k_spmm (*g.PT[l], 1.0, *R, 0.0, w.Bl[l - 1]));
+  LGH_TRY (mg_run (g, w, l - 1, w.Bl[l - 1], w.Xl[l - 1]));
+  LGH_TRY (k_spmm (*g.P[l], 1.0, w.Xl[l - 1], 1.0, X));
+  LGH_TRY (mg_smooth (g, w, l, B, X, 0, nullptr));               // post
   return 0;
 }
 
@@ -197,14 +214,49 @@ mg_vcycle (DevMG &g, const PMat &B, PMat &X)
 {
   LGH_CHECK (B.prec == g.prec && X.prec == g.prec, 8,
              "mg_vcycle: precision mismatch");
-  LGH_TRY (mg_work (g, B));
-  return mg_run (g, g.nl - 1, B, X);
+  DevMG::Work *w = nullptr;
+  LGH_TRY (mg_work (g, B, &w));
+  return mg_run (g, *w, g.nl - 1, B, X);
 }
 
 ChebWork::~ChebWork ()
 {
   pmat_free (R[0]); pmat_free (R[1]); pmat_free (D[0]); pmat_free (D[1]);
-  pmat_free (Z); pmat_free (Rpc); pmat_free (Zpc);
+  pmat_free (Z); pmat_free (Rpc); pmat_free (Zpc); pmat_free (Bt);
+  pmat_free (Xt);
+  delete tail;
+}
+
+// whole panels and the narrow last panel of B (see zsolve.hpp): calls
+// run (B', X', work) on the whole panels in place and on a 16-wide copy of
+// the last panel's columns; false when B has no such panel
+template <typename WK, typename F>
+static int
+split_tail (const PMat &B, PMat &X, WK &w, bool *done, F &&run)
+{
+  *done = false;
+  const int full = B.ncols / 64, tail = B.ncols - 64 * full;
+  if (B.W != 64 || tail == 0 || tail > 16) return 0;
+  LGH_CHECK (X.W == 64 && X.ncols == B.ncols && X.prec == B.prec, 8,
+             "Z-solve: X shape differs from B");
+  *done = true;
+  if (full > 0) {
+    PMat Bf = B, Xf = X;
+    Bf.ncols = Xf.ncols = 64 * full;
+    LGH_TRY (run (Bf, Xf, w));
+  }
+  if (!w.tail) w.tail = new WK ();
+  if (!w.Bt.p || w.Bt.nrows != B.nrows || w.Bt.prec != B.prec) {
+    pmat_free (w.Bt); pmat_free (w.Xt);
+    LGH_TRY (pmat_alloc (w.Bt, B.nrows, 16, 16, B.prec));
+    LGH_TRY (pmat_alloc (w.Xt, B.nrows, 16, 16, B.prec));
+  }
+  // Bt's columns past `tail` are zero padding (so are Xt's after the solve)
+  LGH_TRY (dzero (w.Bt.p, w.Bt.bytes ()));
+  LGH_TRY (k_copy_cols (B, 64 * full, w.Bt, 0, tail));
+  LGH_TRY (run (w.Bt, w.Xt, *w.tail));
+  LGH_TRY (dzero (X.panel (full), (size_t) X.nrows * 64 * prec_bytes (X.prec)));
+  return k_copy_cols (w.Xt, 0, X, 64 * full, tail);
 }
 
 int
@@ -224,24 +276,23 @@ ChebWork::ensure (const PMat &like)
   return 0;
 }
 
-int
-cheb_solve (const DevCSR &A, DevMG *mg, double lmin, double lmax, int nit,
-            const PMat &B, PMat &X, ChebWork &w)
+static int
+cheb_run (const DevCSR &A, DevMG *mg, double lmin, double lmax, int nit,
+          const PMat &B, PMat &X, ChebWork &w)
 {
-  LGH_CHECK (nit >= 1, 8, "cheb_solve: nit < 1");
   LGH_TRY (w.ensure (B));
   // a V-cycle in a lower storage precision (BF16 around FP32 vectors) reads
   // and writes its own copies: Rpc (written by each step) and Zpc
   const bool low = mg && mg->prec != B.prec;
   if (low) {
-    LGH_CHECK (B.W == 64 && B.prec == F32 && mg->prec == BF16, 8,
+    LGH_CHECK ((B.W == 64 || B.W == 16) && B.prec == F32 && mg->prec == BF16, 8,
                "cheb_solve: unsupported preconditioner precision");
     if (!w.Rpc.p) {
-      const int nc = B.npanels () * 64;
-      LGH_TRY (pmat_alloc (w.Rpc, B.nrows, nc, 64, mg->prec));
-      LGH_TRY (pmat_alloc (w.Zpc, B.nrows, nc, 64, mg->prec));
+      const int nc = B.npanels () * B.W;
+      LGH_TRY (pmat_alloc (w.Rpc, B.nrows, nc, B.W, mg->prec));
+      LGH_TRY (pmat_alloc (w.Zpc, B.nrows, nc, B.W, mg->prec));
     }
-    LGH_TRY (k_copy_cols (B, 0, w.Rpc, 0, B.npanels () * 64));
+    LGH_TRY (k_copy_cols (B, 0, w.Rpc, 0, B.npanels () * B.W));
   }
   ChebCoef cc (lmin, lmax);
   const PMat *Rsrc = &B;
@@ -270,41 +321,54 @@ cheb_solve (const DevCSR &A, DevMG *mg, double lmin, double lmax, int nit,
   return 0;
 }
 
+int
+cheb_solve (const DevCSR &A, DevMG *mg, double lmin, double lmax, int nit,
+            const PMat &B, PMat &X, ChebWork &w)
+{
+  LGH_CHECK (nit >= 1, 8, "cheb_solve: nit < 1");
+  bool done = false;
+  LGH_TRY (split_tail (B, X, w, &done,
+                       [&] (const PMat &Bp, PMat &Xp, ChebWork &wp) {
+                         return cheb_run (A, mg, lmin, lmax, nit, Bp, Xp, wp);
+                       }));
+  return done ? 0 : cheb_run (A, mg, lmin, lmax, nit, B, X, w);
+}
+
 ChebMixedWork::~ChebMixedWork ()
 {
   pmat_free (R[0]); pmat_free (R[1]); pmat_free (D[0]); pmat_free (D[1]);
-  pmat_free (Rpc); pmat_free (Zpc);
+  pmat_free (Rpc); pmat_free (Zpc); pmat_free (Bt); pmat_free (Xt);
+  delete tail;
 }
 
 int
 ChebMixedWork::ensure (const PMat &like, int pc)
 {
-  if (np == like.npanels () && nrows == like.nrows && pprec == pc) return 0;
+  if (W == like.W && np == like.npanels () && nrows == like.nrows
+      && pprec == pc) return 0;
   pmat_free (R[0]); pmat_free (R[1]); pmat_free (D[0]); pmat_free (D[1]);
   pmat_free (Rpc); pmat_free (Zpc);
-  const int nc = like.npanels () * 64;
-  LGH_TRY (pmat_alloc (R[0], like.nrows, nc, 64, F64));
-  LGH_TRY (pmat_alloc (R[1], like.nrows, nc, 64, F64));
-  LGH_TRY (pmat_alloc (D[0], like.nrows, nc, 64, F32));
-  LGH_TRY (pmat_alloc (D[1], like.nrows, nc, 64, F32));
-  LGH_TRY (pmat_alloc (Rpc, like.nrows, nc, 64, pc));
-  LGH_TRY (pmat_alloc (Zpc, like.nrows, nc, 64, pc));
+  const int nc = like.npanels () * like.W;
+  LGH_TRY (pmat_alloc (R[0], like.nrows, nc, like.W, F64));
+  LGH_TRY (pmat_alloc (R[1], like.nrows, nc, like.W, F64));
+  LGH_TRY (pmat_alloc (D[0], like.nrows, nc, like.W, F32));
+  LGH_TRY (pmat_alloc (D[1], like.nrows, nc, like.W, F32));
+  LGH_TRY (pmat_alloc (Rpc, like.nrows, nc, like.W, pc));
+  LGH_TRY (pmat_alloc (Zpc, like.nrows, nc, like.W, pc));
+  W = like.W;
   np = like.npanels ();
   nrows = like.nrows;
   pprec = pc;
   return 0;
 }
 
-int
-cheb_solve_mixed (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
-                  int nit, const PMat &B, PMat &X, ChebMixedWork &w)
+static int
+cheb_mixed_run (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
+                int nit, const PMat &B, PMat &X, ChebMixedWork &w)
 {
-  LGH_CHECK (nit >= 1 && B.W == 64 && B.prec == F64 && X.prec == F64
-             && (mg.prec == F32 || mg.prec == BF16), 8,
-             "cheb_solve_mixed: shape/precision");
   LGH_TRY (w.ensure (B, mg.prec));
   ChebCoef cc (lmin, lmax);
-  LGH_TRY (k_copy_cols (B, 0, w.Rpc, 0, B.npanels () * 64));
+  LGH_TRY (k_copy_cols (B, 0, w.Rpc, 0, B.npanels () * B.W));
   const PMat *Rsrc = &B;
   int ri = 0, di = 0;
   for (int k = 1; k <= nit; k++) {
@@ -325,6 +389,23 @@ cheb_solve_mixed (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
     di ^= 1;
   }
   return 0;
+}
+
+int
+cheb_solve_mixed (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
+                  int nit, const PMat &B, PMat &X, ChebMixedWork &w)
+{
+  LGH_CHECK (nit >= 1 && B.W == 64 && X.W == 64 && B.prec == F64
+             && X.prec == F64 && B.ncols == X.ncols
+             && (mg.prec == F32 || mg.prec == BF16), 8,
+             "cheb_solve_mixed: shape/precision");
+  bool done = false;
+  LGH_TRY (split_tail (B, X, w, &done,
+                       [&] (const PMat &Bp, PMat &Xp, ChebMixedWork &wp) {
+                         return cheb_mixed_run (A64, mg, lmin, lmax, nit, Bp,
+                                                Xp, wp);
+                       }));
+  return done ? 0 : cheb_mixed_run (A64, mg, lmin, lmax, nit, B, X, w);
 }
 
 int

@@ -15,9 +15,14 @@ struct DevMG {
   std::vector<int> nu, n;
   void *cinv = nullptr;                // dense row-major inverse of A_0
   int n0 = 0;
-  // workspace for one (W, npanels) shape
-  int wW = 0, wnp = -1;
-  std::vector<PMat> R1, R2, D1, D2, Bl, Xl;
+  // per-level workspace for one (W, npanels) shape; slot 0 holds full
+  // 64-column panels, slot 1 other widths (the narrow last panel of a
+  // block), so a solve that alternates between the two reallocates nothing
+  struct Work {
+    int W = 0, np = -1;
+    std::vector<PMat> R1, R2, D1, D2, Bl, Xl;
+    void free ();
+  } work[2];
   ~DevMG ();
   void free_work ();
 };
@@ -30,10 +35,18 @@ struct ChebWork {
   int W = 0, np = -1, prec = -1, nrows = 0;
   PMat R[2], D[2], Z;
   PMat Rpc, Zpc;                       // lower-precision V-cycle in / out
+  PMat Bt, Xt;                         // the narrow last panel's B and X
+  ChebWork *tail = nullptr;            // (see ChebMixedWork)
   ~ChebWork ();
   int ensure (const PMat &like);
 };
 
+// Both Chebyshev solves below run a block whose last 64-column panel holds
+// at most 16 columns as its whole panels plus a separate 16-wide panel
+// (tail: its own work and copies of those columns), so the 48 padding
+// columns cost nothing; every column sees the same arithmetic at either
+// width.
+//
 // X = Chebyshev(nit) solve of A X = B, zero initial guess
 // (zsolve_impl.h:104-141 with the blocked-asolve contract of :629-633);
 // preconditioner: the V-cycle when mg (in B's precision, or BF16 around
@@ -45,8 +58,10 @@ int cheb_solve (const DevCSR &A, DevMG *mg, double lmin, double lmax,
 // X, R in FP64; D in FP32; the V-cycle's input and output (Rpc, Zpc) in
 // its own precision (k_cheb_step_mixed)
 struct ChebMixedWork {
-  int np = -1, nrows = 0, pprec = -1;
+  int W = 0, np = -1, nrows = 0, pprec = -1;
   PMat R[2], D[2], Rpc, Zpc;
+  PMat Bt, Xt;                         // the narrow panel's B and X
+  ChebMixedWork *tail = nullptr;
   ~ChebMixedWork ();
   int ensure (const PMat &like, int pc);
 };

@@ -113,3 +113,16 @@ def test_vcycle_matches_oracle(dev, oracle):
     assert rel(h.vcycle(B, W=4), ref) < 1e-12
     # BF16 storage (8-bit mantissa, 2^-9 relative rounding per stored value)
     assert rel(h.vcycle(B, prec=dev.PREC_BF16), ref) < 2e-2
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_vcycle_narrow_panel_bitwise(dev, prec):
+    """A 16-wide panel (the narrow last panel of a block) runs the same
+    per-column arithmetic as a 64-wide one: identical bits, at every
+    storage precision."""
+    Z, M = W.prior_Z(64, 2, 1.0, 0.1)
+    h = dev.AMG(Z)
+    B = np.random.default_rng(5).standard_normal((Z.shape[0], 13))
+    x64 = h.vcycle(B, prec=prec, W=64)
+    x16 = h.vcycle(B, prec=prec, W=16)
+    assert np.array_equal(x64, x16)
