@@ -1127,15 +1127,20 @@ copy_cols_k (int nrows, int nc, const TI *X, int xW, int xc0, TO *Y, int yW,
 }
 
 // whole panels of equal width: one contiguous range, 4 elements a thread
+// (one 8-32 B vector load and store; panel starts are 4-element aligned)
 template <typename TI, typename TO>
 __global__ void __launch_bounds__ (TPB)
 convert_k (size_t count, const TI *__restrict__ X, TO *__restrict__ Y)
 {
   const size_t e = ((size_t) blockIdx.x * TPB + threadIdx.x) * 4;
   if (e >= count) return;
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-    if (e + i < count) st_as<double> (Y, e + i, ld_as<double> (X, e + i));
+  if (e + 4 <= count) {
+    double x[4];
+    vld (X + e, x);
+    vst (Y + e, x);
+    return;
+  }
+  for (size_t i = e; i < count; i++) st_as<double> (Y, i, ld_as<double> (X, i));
 }
 
 int
@@ -1148,6 +1153,7 @@ k_copy_cols (const PMat &X, int xc0, PMat &Y, int yc0, int nc)
   // the logical end of both (the rest of the last panel is zero padding in
   // both): a straight conversion of one contiguous range
   const bool flat = X.W == Y.W && xc0 % X.W == 0 && yc0 % Y.W == 0
+                    && ((size_t) X.nrows * X.W) % 4 == 0      // aligned panels
                     && (nc % X.W == 0 || (xc0 + nc >= X.ncols && yc0 + nc >= Y.ncols));
   const size_t cnt = (size_t) ((nc + X.W - 1) / X.W) * X.nrows * X.W;
   const unsigned gf = grid1 ((cnt + 3) / 4, TPB);

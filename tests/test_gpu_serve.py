@@ -50,7 +50,11 @@ def test_serve_alone_equals_mix(dev, prob, mixed):
     reqs = _requests(prob["n"])
     srv = dev.Serve(g, max_batch_cols=64)
     alone = [srv.wait(srv.submit(op, c, X))[0] for op, c, X in reqs]
-    # the same requests from 4 threads at once, interleaved
+    # the same requests from 4 threads at once, interleaved, queued behind
+    # four full-width solves so that they arrive while batches run
+    rng = np.random.default_rng(9)
+    blockers = [srv.submit(0, 1.0, rng.standard_normal((prob["n"], 64)))
+                for _ in range(4)]
     mix = [None] * len(reqs)
 
     def client(ids):
@@ -64,6 +68,8 @@ def test_serve_alone_equals_mix(dev, prob, mixed):
         t.start()
     for t in th:
         t.join()
+    for t in blockers:
+        srv.wait(t)
     st = srv.stats()
     srv.close()
     for (op, c, X), a, b in zip(reqs, alone, mix):
