@@ -166,6 +166,117 @@ def serve_bench(lgh, g, n, nreq, pool, seed=5):
             "mean_batch_cols": st["columns"] / max(st["batches"], 1)}
 
 
+def dominant_roofline(lgh, args, prof, npan, vpc):
+    """roofline object of the dominant single full-width kernel kind"""
+    single = ("cheb_mixed", "smooth_pre_fine", "smooth_post_fine", "resid_fine")
+    dom = max(single, key=lambda k: prof[k][1])
+    c, ms, by, fl = prof[dom]
+    avg_ms = ms / c
+    ach = by / c / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+            "kernel": kernel_of(dom, args.vcycle), "panels": npan,
+            "avg_launch_ms": avg_ms, "launches": c,
+            "algorithmic_bytes_per_launch": by / c}
+
+
+def cfg3_bench(args, W, lgh):
+    """3D configuration: 128^3 grid (n = 2 097 152), Z = M + 0.1 K7, B an
+    isotropic PSF (sigma 1.5h, radius-2h ball, 33 nnz/row) with FP32
+    values; mixed build at ell = 400, then lgh_glr_extend by 200 twice
+    (ell 800), each timed; the extended spectrum is checked against a
+    one-shot ell = 800 build (extend == one-shot for the hashed sketch)."""
+    side = 128 if args.side == 1024 else args.side
+    t0 = time.time()
+    Bs, Zs, M = W.cfg3_problem(side)
+    n = Zs.shape[0]
+    log("cfg3 problem n=%d nnz(B)=%d (%.1f/row) gen %.1fs"
+        % (n, Bs.nnz, Bs.nnz / n, time.time() - t0))
+    lgh.init()
+    t0 = time.time()
+    prior = lgh.Prior.from_mat(Zs, M, blocked_mode=args.mode)
+    info = prior.info()
+    log("prior create %.1fs %s" % (time.time() - t0, info))
+    Bm = lgh.Mat.aij(Bs)
+    pcode = {"bf16": lgh.PREC_BF16, "f32": lgh.PREC_F32}
+    mixed = lgh.xopts(mixed=True, prec_precond=pcode[args.vcycle],
+                      prec_sketch_precond=pcode[args.sketch_vcycle])
+    ell0, ext = 400, 200
+    opts = lgh.glr_opts(ell=ell0, q_power=1, seed=0)
+    for w in range(args.warmup):
+        t0 = time.time()
+        g = lgh.GLR(Bm, prior, opts, x=mixed)
+        log("warmup build %d: %.2fs kept=%d" % (w, time.time() - t0, g.kept))
+        del g
+    lgh.prof_enable(True)
+    for k in lgh.PROF_KINDS:
+        lgh.prof_collect(k)
+    lgh.lib().lgh_hip_sync()
+    t0 = time.perf_counter()
+    g = None
+    for s in range(args.steps):
+        g = None
+        g = lgh.GLR(Bm, prior, opts, x=mixed)
+    lgh.lib().lgh_hip_sync()
+    t_build = (time.perf_counter() - t0) / args.steps
+    lgh.prof_enable(False)
+    npan = ell0 // 64 if 0 < ell0 % 64 <= 16 else (ell0 + 63) // 64
+    vpc = pcode[args.vcycle]
+    single = ("cheb_mixed", "smooth_pre_fine", "smooth_post_fine", "resid_fine")
+    prof = {k: lgh.prof_collect(k, npan, vpc) if k in single else lgh.prof_collect(k)
+            for k in lgh.PROF_KINDS}
+    log("cfg3 mixed build %.3fs kept=%d report=%s" % (t_build, g.kept, g.report))
+    for k, (c, ms, by, fl) in prof.items():
+        if c:
+            log("  %-16s launches %6d  %9.2f ms  avg %.3f ms  %7.1f GB/s  %6.2f TF/s"
+                % (k, c, ms, ms / c, by / ms / 1e6, fl / ms / 1e9))
+    roof = dominant_roofline(lgh, args, prof, npan, vpc)
+    t_ext = []
+    for e in range(2):
+        lgh.lib().lgh_hip_sync()
+        t0 = time.perf_counter()
+        rep = g.extend(ext)
+        lgh.lib().lgh_hip_sync()
+        t_ext.append(time.perf_counter() - t0)
+        log("extend +%d -> ell %d: %.3fs kept=%d" % (ext, ell0 + ext * (e + 1),
+                                                     t_ext[-1], rep["kept"]))
+    peak_gb = lgh.lib().lgh_hip_bytes_in_use() / 1e9
+    lam_ext = np.array(g.lam)
+    kept_ext = g.kept
+    del g
+    lgh.lib().lgh_hip_trim()
+    t0 = time.perf_counter()
+    g8 = lgh.GLR(Bm, prior, lgh.glr_opts(ell=ell0 + 2 * ext, q_power=1, seed=0),
+                 x=mixed)
+    lgh.lib().lgh_hip_sync()
+    t_one = time.perf_counter() - t0
+    k = min(kept_ext, g8.kept)
+    agree = float(np.max(np.abs(lam_ext[:k] - g8.lam[:k]) / np.abs(g8.lam[:k])))
+    log("one-shot ell=%d build %.3fs kept=%d; extend vs one-shot spectrum rel %.2e"
+        % (ell0 + 2 * ext, t_one, g8.kept, agree))
+    line = {
+        "metric": "GLR build s + solve RHS/s, p99 latency; mixed vs FP64 plain at rel err <=1e-6",
+        "value": t_build, "unit": "s/build", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t_build * 1e3,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "cfg3: 3D %d^3 grid n=%d, isotropic PSF B (sigma "
+                               "1.5h, radius 2h, %.1f nnz/row, FP32 values), "
+                               "Z=M+0.1K7, ell=%d q=1, then extend +%d twice, "
+                               "Z^-1 = Chebyshev(%d) around AMG V-cycle"
+                               % (side, n, Bs.nnz / n, ell0, ext, info["nit"]),
+                   "n": n, "ell": ell0, "kept": kept_ext,
+                   "vcycle_fp64_sweeps": args.vcycle},
+        "roofline": roof,
+        "extend_s": t_ext, "extend_cols": ext, "ell_final": ell0 + 2 * ext,
+        "kept_after_extends": kept_ext, "one_shot_build_s": t_one,
+        "kept_one_shot": g8.kept, "extend_vs_one_shot_spectrum_rel": agree,
+        "device_bytes_in_use_gb": peak_gb,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,6 +299,9 @@ def main():
                     help="columns of the timed lgh_glr_extend (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=300,
                     help="requests of the Poisson serving run (0: skip)")
+    ap.add_argument("--workload", choices=("cfg2", "cfg3"), default="cfg2",
+                    help="cfg2: the 2D 1024^2 build (the headline line); cfg3: "
+                         "the 3D 128^3 build at ell=400, extended by 200 twice")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -208,6 +322,8 @@ def main():
 
     import workloads as W
     lgh = load_pkg()
+    if args.workload == "cfg3":
+        return cfg3_bench(args, W, lgh)
     t0 = time.time()
     Bs, Zs, M = W.cfg2_problem(args.side)
     n = Zs.shape[0]

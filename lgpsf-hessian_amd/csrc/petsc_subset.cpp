@@ -469,8 +469,21 @@ mat_device_csr (Mat A, int prec, int want_dinv, DevCSR **out)
   std::vector<int> rps, cis;
   std::vector<double> vs;
   LGH_TRY (mat_host_csr (A, &rp, &ci, &v, rps, cis, vs));
+  // values that are all exact in FP32 (an operator assembled in single
+  // precision) are stored in FP32 for FP64 sweeps too: the SpMM widens them
+  // exactly, so the product is bit-identical and reads 4 fewer bytes per
+  // entry.  Matrices that need the inverse diagonal (Z, the smoothed
+  // operators) keep their precision.
+  int vprec = prec;
+  if (prec == F64 && !want_dinv) {
+    const int64_t nnz = rp[A->m];
+    bool exact = nnz > 0;
+    for (int64_t k = 0; k < nnz && exact; k++)
+      exact = (double) (float) v[k] == v[k];
+    if (exact) vprec = F32;
+  }
   DevCSR *d = nullptr;
-  LGH_TRY (csr_upload (A->m, A->n, rp, ci, v, prec, want_dinv, &d));
+  LGH_TRY (csr_upload (A->m, A->n, rp, ci, v, vprec, want_dinv, &d));
   A->dev[prec] = d;
   A->dev_state[prec] = A->state;
   *out = d;

@@ -236,12 +236,13 @@ def cfg2_problem(side=1024, dtype=np.float64):
 
 def cfg3_problem(side=128):
     """3D isotropic PSF sigma = 1.5h truncated to a radius-2h ball, Z =
-    M + 0.1 K7 (unscaled 7-point)."""
+    M + 0.1 K7 (unscaled 7-point); B's values are FP32 numbers (the
+    configuration's single-precision operator), held in a float64 CSR."""
     Z, M = prior_Z(side, 3, 1.0, 0.1)
     s = 1.5 / (side - 1)
 
     def val(i, j, d):
         return np.exp(-(d ** 2).sum(1) / (2 * s * s))
 
-    B = psf_B(side, 3, lambda h: _ball_offsets(2.0, 3), val)
-    return B, Z, M
+    B = psf_B(side, 3, lambda h: _ball_offsets(2.0, 3), val, dtype=np.float32)
+    return B.astype(np.float64), Z, M
