@@ -68,7 +68,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(B, Z, M, lgh, ell, q, kept, budget_s):
+def cpu_baseline(B, Z, M, lgh, ell, q, kept, budget_s, gamma=1):
     """FP64 oracle (single thread) on a bounded sample: whole F sweeps of a
     few sketch columns with the same Chebyshev-MG operator, scaled to the
     (q+2) ell + kept columns of one build; the dense stages are not in the
@@ -78,7 +78,7 @@ def cpu_baseline(B, Z, M, lgh, ell, q, kept, budget_s):
     levels = [{"A": O.CSR.from_scipy(l["A"]),
                "P": O.CSR.from_scipy(l["P"]) if l["P"] is not None else None,
                "slo": l["slo"], "shi": l["shi"], "nu": l["nu"]} for l in h.levels()]
-    mg = O.MG(levels)
+    mg = O.MG(levels, gamma=gamma)
     return mg, levels
 
 
@@ -471,7 +471,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         import oracle as O
-        mg, levels = cpu_baseline(Bs, Zs, M, lgh, args.ell, 1, kept_build, 0)
+        mg, levels = cpu_baseline(Bs, Zs, M, lgh, args.ell, 1, kept_build, 0,
+                                  gamma=info["cycle"])
         zs = O.ZSolver.mat(O.CSR.from_scipy(Zs), mode=3, mg=mg, lmin=info["lmin"],
                            lmax=info["lmax"], nit=info["nit"], ksp_rtol=1e-12)
         msq = np.sqrt(M)

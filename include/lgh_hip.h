@@ -88,7 +88,12 @@ int  lgh_amg_level (const lgh_amg_t *h, int l, int *n, int *nc,
                     const int **rp, const int **ci, const double **v,
                     const int **prp, const int **pci, const double **pv,
                     double *slo, double *shi, int *nu);
-/* X = V-cycle(B) on the device, ncols columns; prec 0 FP64, 1 FP32,
+/* cycle index of the device cycle: 1 V-cycle (default), 2 W-cycle (each
+ * level between the finest and the coarsest corrects twice, the second
+ * time from the first result); lgh_prior_create_mat picks it for its own
+ * hierarchy (lgh_prior_info_t.cycle) */
+int  lgh_amg_set_cycle (lgh_amg_t *h, int gamma);
+/* X = cycle(B) on the device, ncols columns; prec 0 FP64, 1 FP32,
  * 2 BF16 vectors with FP32 matrices and arithmetic (W = 64 or 16) */
 int  lgh_amg_vcycle (const lgh_amg_t *h, int prec, int W, int ncols,
                      const double *B, int ldb, double *X, int ldx);

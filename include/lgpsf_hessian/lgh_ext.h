@@ -71,6 +71,7 @@ typedef struct lgh_prior_info
   int    levels;        /* AMG levels (0 when none)                        */
   int    n_coarse;      /* coarsest AMG size                               */
   int    bound_its;     /* iterations of the interval estimate             */
+  int    cycle;         /* AMG cycle index: 1 V-cycle, 2 W-cycle           */
 }
 lgh_prior_info_t;
 

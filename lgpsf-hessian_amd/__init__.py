@@ -96,7 +96,7 @@ class PriorInfo(C.Structure):
     _fields_ = [("is_mat", C.c_int), ("blocked_mode", C.c_int),
                 ("nit", C.c_int), ("lmin", C.c_double), ("lmax", C.c_double),
                 ("levels", C.c_int), ("n_coarse", C.c_int),
-                ("bound_its", C.c_int)]
+                ("bound_its", C.c_int), ("cycle", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -233,6 +233,7 @@ _SIGS = {
                                 C.POINTER(ip), C.POINTER(dp), dp, dp, ip]),
     "lgh_amg_vcycle": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, C.c_int,
                                  dp, C.c_int]),
+    "lgh_amg_set_cycle": (C.c_int, [vp, C.c_int]),
 }
 
 
@@ -772,6 +773,10 @@ class AMG:
             out.append({"A": A, "P": P, "slo": slo.value, "shi": shi.value,
                         "nu": nu.value})
         return out
+
+    def set_cycle(self, gamma):
+        """1: V-cycle, 2: W-cycle (lgh_amg_set_cycle)"""
+        _hip_check(lib().lgh_amg_set_cycle(self.h, gamma), "lgh_amg_set_cycle")
 
     def vcycle(self, B, prec=PREC_F64, W=64):
         B = _fcol(B)

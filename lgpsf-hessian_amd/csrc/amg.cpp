@@ -296,6 +296,30 @@ dense_inverse (const HostCSR &A, std::vector<double> &inv)
   return 0;
 }
 
+double
+amg_cycle_cost (const AmgHierarchy &h, int gamma)
+{
+  // 5 us per launch at 5 TB/s, shared by the 448 columns of a 400-wide
+  // sketch block; a level visit is ~7 launches (smoother, residuals,
+  // transfers)
+  const double launch = 5e-6 * 5e12 / 448.0;
+  double v = 1.0, c = 0.0;
+  for (int l = (int) h.lev.size () - 1; l >= 0; l--) {
+    const AmgLevel &lv = h.lev[l];
+    const double n = lv.A.n, nnz = (double) lv.A.nnz ();
+    if (l == 0) {
+      c += v * (n * n * 4.0 / 64.0 + launch);
+      break;
+    }
+    // 14 BF16 vector passes (smoothers, two residuals, both transfers), A
+    // read 4 times and P twice per 64-column panel
+    c += v * (14.0 * 2.0 * n + (32.0 * nnz + 16.0 * (double) lv.P.nnz ()) / 64.0
+              + 7.0 * launch);
+    if (gamma == 2 && l - 1 >= 1) v *= 2.0;
+  }
+  return c;
+}
+
 int
 amg_setup (const HostCSR &A0, const AmgOpts &o, AmgHierarchy &h)
 {

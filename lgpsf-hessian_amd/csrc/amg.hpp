@@ -34,7 +34,16 @@ struct AmgLevel {
 struct AmgHierarchy {
   std::vector<AmgLevel> lev;  // lev[0] coarsest (PCMG numbering)
   std::vector<double> coarse_inv;   // dense row-major inverse of lev[0].A
+  // cycle index: 1 V-cycle; 2 W-cycle (every level between the finest and
+  // the coarsest corrects its coarse level twice, the second time from
+  // the first correction's result)
+  int gamma = 1;
 };
+// bytes moved per right-hand-side column by one application of the cycle
+// (vectors in BF16, the level CSRs amortized over 64-column panels, and a
+// launch-overhead term per level visit): the cost model that picks the
+// cycle index
+double amg_cycle_cost (const AmgHierarchy &h, int gamma);
 
 // y = A x
 void host_spmv (const HostCSR &A, const double *x, double *y);

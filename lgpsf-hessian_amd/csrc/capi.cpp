@@ -225,6 +225,15 @@ lgh_amg_level (const lgh_amg_t *h, int l, int *n, int *nc, const int **rp,
 }
 
 int
+lgh_amg_set_cycle (lgh_amg_t *h, int gamma)
+{
+  LGH_CHECK (gamma == 1 || gamma == 2, 8, "lgh_amg_set_cycle: gamma must be 1 or 2");
+  h->h.gamma = gamma;
+  for (auto *g : h->mg) if (g) g->gamma = gamma;
+  return 0;
+}
+
+int
 lgh_amg_vcycle (const lgh_amg_t *h, int prec, int W, int ncols,
                 const double *B, int ldb, double *X, int ldx)
 {

@@ -155,40 +155,75 @@ lgh_prior_create_mat (Mat Z, Vec mass, const lgh_prior_mat_opts_t *opts,
                               "lgh_prior_create_mat: AMG setup failed "
                               "(coarsening stalled above 2000 unknowns)");
     }
-    PMat b, x;
-    PcgInfo info;
-    rc = pmat_alloc (b, p->n, 1, 1, F64);
-    if (!rc) rc = pmat_alloc (x, p->n, 1, 1, F64);
-    if (!rc) rc = k_randn (b, 0xB0BD5ULL, 0, 0);
-    DevCSR *dz = nullptr;
-    if (!rc) rc = mat_device_csr (Z, F64, 1, &dz);
-    if (!rc)
-      rc = pcg_solve (*dz, o.blocked_mode <= 3 ? p->mg[F64] : nullptr,
-                      p->d_zdinv, b, x, 1e-12, 200, &info);
-    pmat_free (b);
-    pmat_free (x);
-    if (rc || info.its == 0) {
+    // Lanczos extremes of the preconditioned Z from one PCG run
+    auto estimate = [&] (double *emin, double *emax, int *its) -> int {
+      PMat b, x;
+      PcgInfo info;
+      int rc2 = pmat_alloc (b, p->n, 1, 1, F64);
+      if (!rc2) rc2 = pmat_alloc (x, p->n, 1, 1, F64);
+      if (!rc2) rc2 = k_randn (b, 0xB0BD5ULL, 0, 0);
+      DevCSR *dz = nullptr;
+      if (!rc2) rc2 = mat_device_csr (Z, F64, 1, &dz);
+      if (!rc2)
+        rc2 = pcg_solve (*dz, o.blocked_mode <= 3 ? p->mg[F64] : nullptr,
+                         p->d_zdinv, b, x, 1e-12, 200, &info);
+      pmat_free (b);
+      pmat_free (x);
+      if (rc2) return rc2;
+      if (info.its == 0) { set_error ("no PCG iteration"); return 9; }
+      const int it = info.its;
+      std::vector<double> d (it), e (it);
+      for (int k = 0; k < it; k++) {
+        d[k] = 1.0 / info.alpha[k] + (k > 0 ? info.beta[k - 1] / info.alpha[k - 1] : 0.0);
+        e[k] = std::sqrt (info.beta[k]) / info.alpha[k];
+      }
+      tridiag_extremes (it, d.data (), e.data (), emin, emax);
+      *its = it;
+      return 0;
+    };
+    // zsolve_impl.h:93-102
+    auto cheb_count = [&] (double emin, double emax) {
+      const double kap = (1.1 * emax) / (0.9 * emin);
+      const double rho = (std::sqrt (kap) - 1.) / (std::sqrt (kap) + 1.);
+      const int nit = (int) std::ceil (std::log (2. / o.cheb_rtol) / std::log (1. / rho));
+      return nit < 1 ? 1 : nit;
+    };
+    double emin = 0, emax = 0;
+    int it = 0;
+    rc = estimate (&emin, &emax, &it);
+    // cycle index: the W-cycle when its smaller Chebyshev count pays for
+    // its extra coarse-level work by at least 10 % (bytes per column per
+    // Chebyshev iteration: the fused outer step, ~40 B a row + Z's CSR,
+    // plus one cycle); deterministic, from the hierarchy's sizes only
+    if (!rc && o.blocked_mode <= 3 && p->amg.lev.size () >= 3) {
+      double emin2 = 0, emax2 = 0;
+      int it2 = 0;
+      p->mg[F64]->gamma = 2;
+      rc = estimate (&emin2, &emax2, &it2);
+      p->mg[F64]->gamma = 1;
+      if (!rc) {
+        const double step = 40.0 * p->n + 12.0 * (double) Z->rp[Z->m] / 64.0;
+        const double c1 = cheb_count (emin, emax) * (step + amg_cycle_cost (p->amg, 1));
+        const double c2 = cheb_count (emin2, emax2) * (step + amg_cycle_cost (p->amg, 2));
+        if (o.verbose)
+          PetscPrintf (0, "lgh_prior: V-cycle nit %d cost %.3e, W-cycle nit %d "
+                       "cost %.3e\n", cheb_count (emin, emax), c1,
+                       cheb_count (emin2, emax2), c2);
+        if (c2 < 0.9 * c1) {
+          p->amg.gamma = 2;
+          for (auto *m : p->mg) if (m) m->gamma = 2;
+          emin = emin2; emax = emax2; it = it2;
+        }
+      }
+    }
+    if (rc) {
       lgh_prior_destroy (p);
       return lgh_petsc_error (PETSC_ERR_LIB, "lgh_prior_create_mat: interval "
                               "estimate failed: %s", lgh::last_error ());
     }
-    const int it = info.its;
-    std::vector<double> d (it), e (it);
-    for (int k = 0; k < it; k++) {
-      d[k] = 1.0 / info.alpha[k] + (k > 0 ? info.beta[k - 1] / info.alpha[k - 1] : 0.0);
-      e[k] = std::sqrt (info.beta[k]) / info.alpha[k];
-    }
-    double emin, emax;
-    tridiag_extremes (it, d.data (), e.data (), &emin, &emax);
     p->lmin = 0.9 * emin;
     p->lmax = 1.1 * emax;
-    {
-      // zsolve_impl.h:93-102
-      const double kap = p->lmax / p->lmin;
-      const double rho = (std::sqrt (kap) - 1.) / (std::sqrt (kap) + 1.);
-      p->nit = (int) std::ceil (std::log (2. / o.cheb_rtol) / std::log (1. / rho));
-      if (p->nit < 1) p->nit = 1;
-    }
+    p->nit = cheb_count (emin, emax);
     p->bound_its = it;
     if (o.verbose)
       PetscPrintf (0, "lgh_prior: mode %d bounds [%.3e, %.3e] (%d its) -> "
@@ -373,6 +408,7 @@ lgh_prior_get_info (const lgh_prior_t *p, lgh_prior_info_t *info)
   info->levels = (int) p->amg.lev.size ();
   info->n_coarse = p->amg.lev.empty () ? 0 : p->amg.lev[0].A.n;
   info->bound_its = p->bound_its;
+  info->cycle = p->amg.gamma;
   return 0;
 }
 

@@ -48,6 +48,7 @@ mg_upload (const AmgHierarchy &h, int prec, DevMG **out)
   // of that tier are fp32
   const int mprec = (prec == BF16) ? F32 : prec;
   g->nl = (int) h.lev.size ();
+  g->gamma = h.gamma;
   g->A.assign (g->nl, nullptr);
   g->P.assign (g->nl, nullptr);
   g->PT.assign (g->nl, nullptr);
@@ -172,9 +173,24 @@ mg_smooth (DevMG &g, DevMG::Work &w, int l, const PMat &B, PMat &X,
   return 0;
 }
 
-// zsolve_impl.h:406-485 (X overwritten; zero initial guess)
+// the coarse correction of level l: one cycle on level l - 1, or two
+// (gamma = 2) when level l - 1 is not the coarsest (exact) one
+static int mg_run (DevMG &g, DevMG::Work &w, int l, const PMat &B, PMat &X,
+                   int x_init);
 static int
-mg_run (DevMG &g, DevMG::Work &w, int l, const PMat &B, PMat &X)
+mg_coarse (DevMG &g, DevMG::Work &w, int l)
+{
+  LGH_TRY (mg_run (g, w, l - 1, w.Bl[l - 1], w.Xl[l - 1], 0));
+  if (g.gamma == 2 && l - 1 >= 1)
+    LGH_TRY (mg_run (g, w, l - 1, w.Bl[l - 1], w.Xl[l - 1], 1));
+  return 0;
+}
+
+// zsolve_impl.h:406-485 (X overwritten; zero initial guess).  x_init: X
+// holds an initial guess and the pre-smoother starts from B - A X (the
+// W-cycle's second coarse correction)
+static int
+mg_run (DevMG &g, DevMG::Work &w, int l, const PMat &B, PMat &X, int x_init)
 {
   if (l == 0) return k_dense_apply (g.cinv, g.n0, B, X);
   if (fused_level (g, l, B.W)) {
@@ -185,11 +201,15 @@ mg_run (DevMG &g, DevMG::Work &w, int l, const PMat &B, PMat &X)
     cc.step (2, &a2, &b2);
     const int fine = (l == g.nl - 1);
     prof_set_fine_tag (fine);
-    LGH_TRY (k_smooth2x (*g.A[l], b1, a2, b2, B, X, 0));             // pre
+    if (x_init) {
+      LGH_TRY (k_resid (*g.A[l], B, X, w.R1[l]));
+      LGH_TRY (k_smooth2x (*g.A[l], b1, a2, b2, w.R1[l], X, 1));     // pre
+    }
+    else LGH_TRY (k_smooth2x (*g.A[l], b1, a2, b2, B, X, 0));        // pre
     LGH_TRY (k_resid (*g.A[l], B, X, w.R1[l]));
     prof_set_fine_tag (0);
     LGH_TRY (k_spmm (*g.PT[l], 1.0, w.R1[l], 0.0, w.Bl[l - 1]));
-    LGH_TRY (mg_run (g, w, l - 1, w.Bl[l - 1], w.Xl[l - 1]));
+    LGH_TRY (mg_coarse (g, w, l));
     LGH_TRY (k_spmm (*g.P[l], 1.0, w.Xl[l - 1], 1.0, X));
     prof_set_fine_tag (fine);
     LGH_TRY (k_resid (*g.A[l], B, X, w.R1[l]));
@@ -201,9 +221,9 @@ mg_run (DevMG &g, DevMG::Work &w, int l, const PMat &B, PMat &X)
   // recurrence keeps it) is restricted directly — the reference forms the
   // same residual again with one more product (zsolve_impl.h:464-468)
   const PMat *R = nullptr;
-  LGH_TRY (mg_smooth (g, w, l, B, X, 1, &R));
+  LGH_TRY (mg_smooth (g, w, l, B, X, x_init ? 0 : 1, &R));
   LGH_TRY (k_spmm (*g.PT[l], 1.0, *R, 0.0, w.Bl[l - 1]));
-  LGH_TRY (mg_run (g, w, l - 1, w.Bl[l - 1], w.Xl[l - 1]));
+  LGH_TRY (mg_coarse (g, w, l));
   LGH_TRY (k_spmm (*g.P[l], 1.0, w.Xl[l - 1], 1.0, X));
   LGH_TRY (mg_smooth (g, w, l, B, X, 0, nullptr));               // post
   return 0;
@@ -216,7 +236,7 @@ mg_vcycle (DevMG &g, const PMat &B, PMat &X)
              "mg_vcycle: precision mismatch");
   DevMG::Work *w = nullptr;
   LGH_TRY (mg_work (g, B, &w));
-  return mg_run (g, *w, g.nl - 1, B, X);
+  return mg_run (g, *w, g.nl - 1, B, X, 0);
 }
 
 ChebWork::~ChebWork ()

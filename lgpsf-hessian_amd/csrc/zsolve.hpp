@@ -10,6 +10,7 @@ namespace lgh {
 struct DevMG {
   int prec = F64;
   int nl = 0;
+  int gamma = 1;                       // cycle index (AmgHierarchy::gamma)
   std::vector<DevCSR *> A, P, PT;      // per PCMG level (P, PT null at 0)
   std::vector<double> slo, shi;
   std::vector<int> nu, n;
@@ -28,7 +29,8 @@ struct DevMG {
 };
 
 int mg_upload (const AmgHierarchy &h, int prec, DevMG **out);
-// X = V-cycle(B) at the finest level (zero initial guess)
+// X = cycle(B) at the finest level (zero initial guess; V- or W-cycle by
+// g.gamma)
 int mg_vcycle (DevMG &g, const PMat &B, PMat &X);
 
 struct ChebWork {

@@ -63,6 +63,7 @@ def _declare(L):
                                    C.c_int, ip, ip, dp, C.c_double,
                                    C.c_double, C.c_int]
     L.orc_mg_free.argtypes = [C.c_void_p]
+    L.orc_mg_set_cycle.argtypes = [C.c_void_p, C.c_int]
     L.orc_mg_vcycle.argtypes = [C.c_void_p, C.c_int, dp, C.c_int, dp, C.c_int]
     L.orc_zs_diag.restype = C.c_void_p
     L.orc_zs_diag.argtypes = [C.c_int, dp]
@@ -179,14 +180,16 @@ def csr_mm(A, X):
 
 
 class MG:
-    """Multigrid hierarchy for the oracle V-cycle.
+    """Multigrid hierarchy for the oracle cycle.
 
     levels: list finest-last of dicts {A: CSR, P: CSR or None, slo, shi, nu}
-    (PCMG numbering: levels[0] is the coarsest, solved by dense LU)."""
+    (PCMG numbering: levels[0] is the coarsest, solved by dense LU);
+    gamma: 1 V-cycle, 2 W-cycle."""
 
-    def __init__(self, levels, tile=64):
+    def __init__(self, levels, tile=64, gamma=1):
         L = lib()
         self.h = L.orc_mg_create(len(levels), tile)
+        L.orc_mg_set_cycle(self.h, gamma)
         self._keep = []
         for l, lv in enumerate(levels):
             A = lv["A"]

@@ -536,6 +536,7 @@ typedef struct {
 
 struct orc_mg {
   int nl, tile;
+  int gamma;                    /* cycle index: 1 V-cycle, 2 W-cycle */
   mglev_t *lev;
 };
 
@@ -543,7 +544,7 @@ orc_mg_t *
 orc_mg_create (int nl, int tile)
 {
   orc_mg_t *h = xcalloc (1, sizeof (*h));
-  h->nl = nl; h->tile = tile;
+  h->nl = nl; h->tile = tile; h->gamma = 1;
   h->lev = xcalloc ((size_t) nl, sizeof (mglev_t));
   return h;
 }
@@ -625,7 +626,18 @@ mg_smooth (mglev_t *lv, int ncols, const double *B, double *X, double *R,
              D, AD, n);
 }
 
-/* zsolve_impl.h:406-485; X zeroed on entry; B, X are n_l x ncols, ld n_l */
+void
+orc_mg_set_cycle (orc_mg_t *h, int gamma)
+{
+  h->gamma = gamma;
+}
+
+/* zsolve_impl.h:406-485; B, X are n_l x ncols, ld n_l.  X holds the
+ * initial guess (zero for a plain cycle): the correction-form
+ * pre-smoother starts from B - A X.  With gamma = 2 a level above the
+ * coarsest corrects twice, the second cycle starting from the first
+ * one's result (W-cycle; the cycle index is this build's choice, the
+ * reference's PCMG cycle is the V-cycle) */
 static void
 mg_run (orc_mg_t *h, int l, int ncols, const double *B, double *X)
 {
@@ -654,6 +666,7 @@ mg_run (orc_mg_t *h, int l, int ncols, const double *B, double *X)
     for (size_t q = 0; q < tot; q++) R[q] = B[q] - AD[q];
     csr_tmm (n, nc, lv->prp, lv->pci, lv->pv, ncols, R, n, Bc, nc);
     mg_run (h, l - 1, ncols, Bc, Xc);
+    if (h->gamma == 2 && l - 1 >= 1) mg_run (h, l - 1, ncols, Bc, Xc);
     orc_csr_mm (n, lv->prp, lv->pci, lv->pv, ncols, Xc, nc, AD, n); /* P X_c */
     for (size_t q = 0; q < tot; q++) X[q] += AD[q];
     mg_smooth (lv, ncols, B, X, R, Z, D, AD);                     /* post */

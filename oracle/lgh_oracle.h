@@ -54,6 +54,8 @@ int  orc_mg_set_level (orc_mg_t *h, int l, int n, const int *rp,
                        const int *prp, const int *pci, const double *pv,
                        double slo, double shi, int nu);
 void orc_mg_free (orc_mg_t *h);
+/* 1: V-cycle (default), 2: W-cycle */
+void orc_mg_set_cycle (orc_mg_t *h, int gamma);
 /* X = V-cycle(B) (zero initial guess), ncols columns                    */
 void orc_mg_vcycle (orc_mg_t *h, int ncols, const double *B, int ldb,
                     double *X, int ldx);

@@ -92,6 +92,40 @@ def test_amg_hierarchy_properties(lgh):
     assert lv[0]["A"].shape[0] <= 2000
 
 
+def _oracle_levels(oracle, h):
+    return [{"A": oracle.CSR.from_scipy(l["A"]),
+             "P": oracle.CSR.from_scipy(l["P"]) if l["P"] is not None else None,
+             "slo": l["slo"], "shi": l["shi"], "nu": l["nu"]} for l in h.levels()]
+
+
+def test_oracle_w_cycle_properties(lgh, oracle):
+    """The oracle's W-cycle (cycle index 2) pinned by what the mathematics
+    fixes: it is a symmetric operator (the second coarse correction in
+    correction form keeps the cycle self-adjoint); on a two-level hierarchy
+    (coarse level solved exactly) it equals the V-cycle bit for bit; on
+    three or more levels it reduces the A-norm of the error of one
+    application below the V-cycle's."""
+    rng = np.random.default_rng(11)
+    Z, M = W.prior_Z(48, 2, 1.0, 0.1)
+    h = lgh.AMG(Z)
+    lv = _oracle_levels(oracle, h)
+    assert len(lv) >= 3
+    Wc, Vc = oracle.MG(lv, gamma=2), oracle.MG(lv, gamma=1)
+    x, y = rng.standard_normal((2, Z.shape[0]))
+    a, b = x @ Wc.vcycle(y)[:, 0], y @ Wc.vcycle(x)[:, 0]
+    assert abs(a - b) <= 1e-12 * abs(a)
+    e = rng.standard_normal(Z.shape[0])
+    def anorm(v):
+        return math.sqrt(v @ (Z @ v))
+    ew = e - Wc.vcycle(Z @ e)[:, 0]
+    ev = e - Vc.vcycle(Z @ e)[:, 0]
+    assert anorm(ew) < anorm(ev) < anorm(e)
+    two = lv[-2:]
+    two[0] = dict(two[0], P=None, slo=0.0, shi=0.0, nu=0)
+    B = rng.standard_normal((Z.shape[0], 3))
+    assert np.array_equal(oracle.MG(two, gamma=2).vcycle(B), oracle.MG(two).vcycle(B))
+
+
 def test_api_refusals_without_gpu(lgh):
     """Argument checks that precede any device work (glr_common_impl.h:651-664)."""
     L = lgh.lib()

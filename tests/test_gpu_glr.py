@@ -133,7 +133,7 @@ def grid(dev, oracle):
     levels = [{"A": oracle.CSR.from_scipy(l["A"]),
                "P": oracle.CSR.from_scipy(l["P"]) if l["P"] is not None else None,
                "slo": l["slo"], "shi": l["shi"], "nu": l["nu"]} for l in h.levels()]
-    mg = oracle.MG(levels)
+    mg = oracle.MG(levels, gamma=info["cycle"])
     zs = oracle.ZSolver.mat(oracle.CSR.from_scipy(Zs), mode=3, mg=mg,
                             lmin=info["lmin"], lmax=info["lmax"], nit=info["nit"],
                             ksp_rtol=1e-12)
@@ -248,7 +248,7 @@ def test_cfg3_shape_3d_extend_twice(dev, oracle, mixed):
     levels = [{"A": oracle.CSR.from_scipy(l["A"]),
                "P": oracle.CSR.from_scipy(l["P"]) if l["P"] is not None else None,
                "slo": l["slo"], "shi": l["shi"], "nu": l["nu"]} for l in h.levels()]
-    zs = oracle.ZSolver.mat(oracle.CSR.from_scipy(Zs), mode=3, mg=oracle.MG(levels),
+    zs = oracle.ZSolver.mat(oracle.CSR.from_scipy(Zs), mode=3, mg=oracle.MG(levels, gamma=info["cycle"]),
                             lmin=info["lmin"], lmax=info["lmax"], nit=info["nit"],
                             ksp_rtol=1e-12)
     og = oracle.GLR(oracle.CSR.from_scipy(Bs), M, zs, ell=40, q_power=1,
@@ -283,7 +283,7 @@ def test_cfg1_full_size_matches_oracle(dev, oracle):
     levels = [{"A": oracle.CSR.from_scipy(l["A"]),
                "P": oracle.CSR.from_scipy(l["P"]) if l["P"] is not None else None,
                "slo": l["slo"], "shi": l["shi"], "nu": l["nu"]} for l in h.levels()]
-    mg = oracle.MG(levels)
+    mg = oracle.MG(levels, gamma=info["cycle"])
     zs = oracle.ZSolver.mat(oracle.CSR.from_scipy(Zs), mode=3, mg=mg,
                             lmin=info["lmin"], lmax=info["lmax"], nit=info["nit"],
                             ksp_rtol=1e-12)

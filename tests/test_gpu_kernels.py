@@ -99,13 +99,21 @@ def test_gram_gemm_many_tiles(dev):
     assert rel(dev.gemm_nn(X, Cr), X @ Cr) < 1e-13
 
 
-def test_vcycle_matches_oracle(dev, oracle):
+@pytest.mark.parametrize("gamma", [1, 2])
+def test_vcycle_matches_oracle(dev, oracle, gamma):
+    """V-cycle (gamma 1) and W-cycle (gamma 2) on a hierarchy of at least
+    three levels, against the oracle cycle on the exported hierarchy"""
     Z, M = W.prior_Z(40, 2, 1.0, 0.1)
     h = dev.AMG(Z)
+    h.set_cycle(gamma)
     levels = [{"A": oracle.CSR.from_scipy(l["A"]),
                "P": oracle.CSR.from_scipy(l["P"]) if l["P"] is not None else None,
                "slo": l["slo"], "shi": l["shi"], "nu": l["nu"]} for l in h.levels()]
-    mg = oracle.MG(levels)
+    assert len(levels) >= 3
+    mg = oracle.MG(levels, gamma=gamma)
+    if gamma == 2:      # the two cycles differ on this hierarchy
+        Bt = np.random.default_rng(4).standard_normal((Z.shape[0], 2))
+        assert rel(mg.vcycle(Bt), oracle.MG(levels).vcycle(Bt)) > 1e-6
     B = np.random.default_rng(4).standard_normal((Z.shape[0], 5))
     ref = mg.vcycle(B)
     assert rel(h.vcycle(B), ref) < 1e-12
