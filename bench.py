@@ -277,6 +277,91 @@ def cfg3_bench(args, W, lgh):
     print(json.dumps(line), flush=True)
 
 
+def _cfg5_gen(k):
+    import workloads as W
+    B, Z, M, ell = W.cfg5_instance(k)
+    return B, Z, M, ell
+
+
+def cfg5_bench(args, W, lgh):
+    """Batched configuration: 256 small 2D instances (workloads.cfg5_instance:
+    n = 4k..64k, ell = rank + 10 with rank in {32, 64, 128}), mixed
+    precision.  Inputs are generated on the host (process pool, outside the
+    timed region); timed: the priors (host AMG set-up + interval estimate)
+    and the GLR builds of all instances, one call at a time and as one
+    batch (`value`: the batched builds)."""
+    import multiprocessing as mp
+    t0 = time.time()
+    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
+        inst = pool.map(_cfg5_gen, range(args.instances), chunksize=1)
+    rows = sum(z.shape[0] for _, z, _, _ in inst)
+    nnz = sum(b.nnz for b, _, _, _ in inst)
+    log("cfg5: %d instances, %d rows, %d nnz(B) (%.1f/row), gen %.1fs"
+        % (len(inst), rows, nnz, nnz / rows, time.time() - t0))
+    lgh.init()
+    mats = [(lgh.Mat.aij(b), z, m, ell) for b, z, m, ell in inst]
+    mixed = lgh.xopts(mixed=True)
+    # warm-up: one instance of each rank (kernels loaded, allocator primed)
+    for j in range(min(3, len(mats))):
+        Bm, z, m, ell = mats[j]
+        pr = lgh.Prior.from_mat(z, m, blocked_mode=3)
+        lgh.GLR(Bm, pr, lgh.glr_opts(ell=ell, q_power=1, seed=0), x=mixed)
+    lgh.lib().lgh_hip_sync()
+    t0 = time.perf_counter()
+    priors = [lgh.Prior.from_mat(z, m, blocked_mode=3) for _, z, m, _ in mats]
+    lgh.lib().lgh_hip_sync()
+    t_prior = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    kept = []
+    for (Bm, z, m, ell), pr in zip(mats, priors):
+        g = lgh.GLR(Bm, pr, lgh.glr_opts(ell=ell, q_power=1, seed=0), x=mixed)
+        kept.append(g.kept)
+        del g
+    lgh.lib().lgh_hip_sync()
+    t_seq = time.perf_counter() - t0
+    log("cfg5 sequential: priors %.2fs, builds %.2fs (%.1f instances/s)"
+        % (t_prior, t_seq, len(mats) / t_seq))
+    del priors
+    # the same instances as batches (lgh_prior_create_mat_batch,
+    # lgh_glr_compute_batch): host worker threads, one stream each
+    lgh.lib().lgh_hip_sync()
+    t0 = time.perf_counter()
+    bpri = lgh.Prior.batch([(z, m) for _, z, m, _ in mats],
+                           workers=args.batch_workers, blocked_mode=3)
+    t_bprior = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gs = lgh.GLR.batch([(Bm, pr, lgh.glr_opts(ell=ell, q_power=1, seed=0))
+                        for (Bm, _, _, ell), pr in zip(mats, bpri)],
+                       x=mixed, workers=args.batch_workers)
+    lgh.lib().lgh_hip_sync()
+    t_batch = time.perf_counter() - t0
+    kept_b = [g.kept for g in gs]
+    del gs
+    log("cfg5 batch: priors %.2fs, builds %.2fs (%.1f instances/s), kept %s"
+        % (t_bprior, t_batch, len(mats) / t_batch,
+           "same" if kept_b == kept else "DIFFERENT"))
+    line = {
+        "metric": "GLR build s + solve RHS/s, p99 latency; mixed vs FP64 plain at rel err <=1e-6",
+        "value": t_batch, "unit": "s/batch", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": t_batch * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "cfg5: %d instances of 2D n=%d..%d (sum %d rows), "
+                               "PSF sigma U[1.5h,3h] radius 3 sigma (%.1f nnz/row), "
+                               "Z=M+0.1K, ell=rank+10, rank in {32,64,128}, mixed"
+                               % (len(mats), min(z.shape[0] for _, z, _, _ in inst),
+                                  max(z.shape[0] for _, z, _, _ in inst), rows,
+                                  nnz / rows),
+                   "instances": len(mats)},
+        "instances_per_s": len(mats) / t_batch, "priors_s": t_bprior,
+        "sequential_builds_s": t_seq, "sequential_priors_s": t_prior,
+        "batch_workers": args.batch_workers or 8,
+        "kept_total": int(sum(kept_b)), "kept_same_as_sequential": kept_b == kept,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -299,9 +384,14 @@ def main():
                     help="columns of the timed lgh_glr_extend (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=300,
                     help="requests of the Poisson serving run (0: skip)")
-    ap.add_argument("--workload", choices=("cfg2", "cfg3"), default="cfg2",
+    ap.add_argument("--workload", choices=("cfg2", "cfg3", "cfg5"), default="cfg2",
                     help="cfg2: the 2D 1024^2 build (the headline line); cfg3: "
-                         "the 3D 128^3 build at ell=400, extended by 200 twice")
+                         "the 3D 128^3 build at ell=400, extended by 200 twice; "
+                         "cfg5: 256 small 2D instances built as one batch")
+    ap.add_argument("--instances", type=int, default=256,
+                    help="cfg5: number of instances")
+    ap.add_argument("--batch-workers", type=int, default=0,
+                    help="cfg5: concurrent build workers (0: the library's default)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -324,6 +414,8 @@ def main():
     lgh = load_pkg()
     if args.workload == "cfg3":
         return cfg3_bench(args, W, lgh)
+    if args.workload == "cfg5":
+        return cfg5_bench(args, W, lgh)
     t0 = time.time()
     Bs, Zs, M = W.cfg2_problem(args.side)
     n = Zs.shape[0]

@@ -62,6 +62,23 @@ int lgh_glr_solve_block (lgh_glr_t *glr, int m, const double *c,
 int lgh_glr_sample_block (lgh_glr_t *glr, int m, const double *c,
                           const double *xi, int ldr, double *x, int ldx);
 
+/* Batches of independent instances (no reference counterpart: the
+ * reference builds one object per call).  Instance i is exactly the
+ * single call with argument i (lgh_prior_create_mat, lgh_glr_compute_x);
+ * the calls run on `workers` host threads (<= 0: 8), each driving the GPU
+ * on its own stream, so the kernels of small instances overlap.
+ * Instances that share an object (Z, mass, B or prior) run one after
+ * another on one thread.  opts: one per instance or NULL (defaults);
+ * xopts: shared or NULL.  On failure every output is destroyed and set to
+ * NULL, and the error names the lowest failing instance. */
+int lgh_prior_create_mat_batch (int count, Mat *Z, Vec *mass,
+                                const lgh_prior_mat_opts_t *opts, int workers,
+                                lgh_prior_t **priors);
+int lgh_glr_compute_batch (int count, Mat *B, lgh_prior_t **priors,
+                           const lgh_glr_opts_t *opts,
+                           const lgh_glr_xopts_t *xopts, int workers,
+                           lgh_glr_t **glrs, lgh_glr_report_t *reports);
+
 typedef struct lgh_prior_info
 {
   int    is_mat;        /* 1: assembled Z, 0: callbacks                    */

@@ -186,6 +186,13 @@ _SIGS = {
     "lgh_glr_compute_x": (C.c_int, [vp, vp, C.POINTER(GlrOpts),
                                     C.POINTER(GlrXOpts), C.POINTER(vp),
                                     C.POINTER(GlrReport)]),
+    "lgh_glr_compute_batch": (C.c_int, [C.c_int, C.POINTER(vp), C.POINTER(vp),
+                                        C.POINTER(GlrOpts), C.POINTER(GlrXOpts),
+                                        C.c_int, C.POINTER(vp),
+                                        C.POINTER(GlrReport)]),
+    "lgh_prior_create_mat_batch": (C.c_int, [C.c_int, C.POINTER(vp), C.POINTER(vp),
+                                             C.POINTER(PriorMatOpts), C.c_int,
+                                             C.POINTER(vp)]),
     "lgh_glr_solve_block": (C.c_int, [vp, C.c_int, dp, dp, C.c_int, dp,
                                       C.c_int]),
     "lgh_glr_sample_block": (C.c_int, [vp, C.c_int, dp, dp, C.c_int, dp,
@@ -406,6 +413,20 @@ class Prior:
         return cls(h, keep=(Zm, mv))
 
     @classmethod
+    def batch(cls, pairs, workers=0, **opts):
+        """priors of many (Z, mass) pairs at once (lgh_prior_create_mat_batch)"""
+        mats = [(Z if isinstance(Z, Mat) else Mat.aij(Z),
+                 m if isinstance(m, Vec) else Vec(m)) for Z, m in pairs]
+        k = len(mats)
+        Zs = (vp * k)(*[z.h for z, _ in mats])
+        Ms = (vp * k)(*[m.h for _, m in mats])
+        os_ = (PriorMatOpts * k)(*[prior_mat_opts(**opts) for _ in range(k)])
+        hs = (vp * k)()
+        _petsc_check(lib().lgh_prior_create_mat_batch(k, Zs, Ms, os_, workers, hs),
+                     "lgh_prior_create_mat_batch")
+        return [cls(vp(hs[i]), keep=mats[i]) for i in range(k)]
+
+    @classmethod
     def diagonal_callbacks(cls, zdiag, mass):
         """Prior with Z = diag(zdiag) supplied through the Vec callbacks
         (the exact "fake solver" prior of tests/test_glr.cpp:67-85)."""
@@ -476,6 +497,32 @@ class GLR:
         n = C.c_int()
         lib().MatGetSize(self.Bm.h, C.byref(n), None)
         self.n = n.value
+
+    @classmethod
+    def batch(cls, items, x=None, workers=0):
+        """GLRs of many (B, prior, opts) triples at once
+        (lgh_glr_compute_batch); x: shared extended options or None"""
+        k = len(items)
+        Bms = [B if isinstance(B, Mat) else Mat.aij(B) for B, _, _ in items]
+        Bs = (vp * k)(*[b.h for b in Bms])
+        Ps = (vp * k)(*[p.h for _, p, _ in items])
+        os_ = (GlrOpts * k)(*[o for _, _, o in items])
+        hs = (vp * k)()
+        reps = (GlrReport * k)()
+        _petsc_check(lib().lgh_glr_compute_batch(k, Bs, Ps, os_,
+                                                 C.byref(x) if x is not None else None,
+                                                 workers, hs, reps),
+                     "lgh_glr_compute_batch")
+        out = []
+        for i in range(k):
+            g = cls.__new__(cls)
+            g.Bm, g.prior, g.h = Bms[i], items[i][1], vp(hs[i])
+            g.report = reps[i].as_dict()
+            n = C.c_int()
+            lib().MatGetSize(g.Bm.h, C.byref(n), None)
+            g.n = n.value
+            out.append(g)
+        return out
 
     @staticmethod
     def _hess(fn, n):

@@ -1,6 +1,9 @@
-// device.hip — context, stream, caching allocator, CSR upload, rocSOLVER
-// glue.  One process drives one GPU (device 0) on one non-blocking stream
-// at default priority; every kernel of the engine is enqueued on it.
+// device.hip — context, streams, caching allocator, CSR upload, rocSOLVER
+// glue.  One process drives one GPU (device 0).  Every host thread that
+// calls the engine gets its own non-blocking stream at default priority,
+// rocBLAS handle and info word (created on its first call); a thread's
+// kernels all go to its stream, so the builds of a batch run concurrently
+// from a pool of threads.
 #include "lgh_dev.hpp"
 
 #include <rocblas/rocblas.h>
@@ -28,51 +31,88 @@ hip_fail (hipError_t e, const char *what, const char *file, int line)
 }
 
 static bool g_init = false;
-static hipStream_t g_stream = nullptr;
-static rocblas_handle g_rb = nullptr;
-static int *g_info_dev = nullptr;
 static std::mutex g_mu;
+
+struct ThreadCtx {
+  hipStream_t s = nullptr;
+  rocblas_handle rb = nullptr;
+  int *info = nullptr;
+};
+static thread_local ThreadCtx t_ctx;
+
+static int
+ctx_init (ThreadCtx &c)
+{
+  if (c.s) return 0;
+  LGH_HIP (hipSetDevice (0));
+  LGH_HIP (hipStreamCreateWithFlags (&c.s, hipStreamNonBlocking));
+  if (rocblas_create_handle (&c.rb) != rocblas_status_success) {
+    set_error ("rocblas_create_handle failed");
+    return 4;
+  }
+  rocblas_set_stream (c.rb, c.s);
+  LGH_HIP (hipMalloc (&c.info, sizeof (int) * 4));
+  return 0;
+}
 
 int
 dev_init ()
 {
-  std::lock_guard<std::mutex> lk (g_mu);
-  if (g_init) return 0;
-  int ndev = 0;
-  LGH_HIP (hipGetDeviceCount (&ndev));
-  LGH_CHECK (ndev > 0, 3, "no HIP device visible: the engine needs a GPU");
-  LGH_HIP (hipSetDevice (0));
-  LGH_HIP (hipStreamCreateWithFlags (&g_stream, hipStreamNonBlocking));
-  if (rocblas_create_handle (&g_rb) != rocblas_status_success) {
-    set_error ("rocblas_create_handle failed");
-    return 4;
+  {
+    std::lock_guard<std::mutex> lk (g_mu);
+    if (!g_init) {
+      int ndev = 0;
+      LGH_HIP (hipGetDeviceCount (&ndev));
+      LGH_CHECK (ndev > 0, 3, "no HIP device visible: the engine needs a GPU");
+      LGH_HIP (hipSetDevice (0));
+      g_init = true;
+    }
   }
-  rocblas_set_stream (g_rb, g_stream);
-  LGH_HIP (hipMalloc (&g_info_dev, sizeof (int) * 4));
-  g_init = true;
-  return 0;
+  return ctx_init (t_ctx);
 }
 
-hipStream_t stream () { return g_stream; }
-void *rocblas_h () { return (void *) g_rb; }
+hipStream_t
+stream ()
+{
+  if (!t_ctx.s) (void) dev_init ();
+  return t_ctx.s;
+}
+void *rocblas_h () { (void) stream (); return (void *) t_ctx.rb; }
 
 int
 dev_sync ()
 {
-  LGH_HIP (hipStreamSynchronize (g_stream));
+  LGH_HIP (hipStreamSynchronize (t_ctx.s));
   return 0;
+}
+
+static void free_list_release (hipStream_t s);
+
+void
+dev_thread_exit ()
+{
+  ThreadCtx &c = t_ctx;
+  if (!c.s) return;
+  (void) hipStreamSynchronize (c.s);
+  free_list_release (c.s);
+  (void) hipFree (c.info);
+  rocblas_destroy_handle (c.rb);
+  (void) hipStreamDestroy (c.s);
+  c = ThreadCtx ();
 }
 
 // ---- live kernel timing -----------------------------------------------------
 struct ProfRec { int kind, np, prec, W; hipEvent_t a, b; double bytes, flops; };
 static int g_prof = 0;
 static thread_local int g_btag = 0, g_ftag = 0;
+static std::mutex g_pmu;                 // records and event pool
 static std::vector<ProfRec> g_recs;
 static std::vector<hipEvent_t> g_evpool;
 
 static hipEvent_t
 ev_get ()
 {
+  std::lock_guard<std::mutex> lk (g_pmu);
   if (!g_evpool.empty ()) { hipEvent_t e = g_evpool.back (); g_evpool.pop_back (); return e; }
   hipEvent_t e = nullptr;
   (void) hipEventCreate (&e);
@@ -92,7 +132,7 @@ ProfScope::ProfScope (int k, double by, double fl, int np_, int prec_, int W_)
 {
   if (!active) return;
   hipEvent_t a = ev_get ();
-  (void) hipEventRecord (a, g_stream);
+  (void) hipEventRecord (a, t_ctx.s);
   ev0 = (void *) a;
 }
 
@@ -100,7 +140,8 @@ ProfScope::~ProfScope ()
 {
   if (!active) return;
   hipEvent_t b = ev_get ();
-  (void) hipEventRecord (b, g_stream);
+  (void) hipEventRecord (b, t_ctx.s);
+  std::lock_guard<std::mutex> lk (g_pmu);
   g_recs.push_back ({ kind, np, prec, W, (hipEvent_t) ev0, b, bytes, flops });
 }
 
@@ -108,7 +149,7 @@ int
 prof_collect (int kind, int np, int prec, long *count, double *ms,
               double *bytes, double *flops)
 {
-  LGH_HIP (hipStreamSynchronize (g_stream));
+  std::lock_guard<std::mutex> lk (g_pmu);
   long c = 0;
   double t = 0, by = 0, fl = 0;
   std::vector<ProfRec> keep;
@@ -116,6 +157,7 @@ prof_collect (int kind, int np, int prec, long *count, double *ms,
     if (r.kind != kind) { keep.push_back (r); continue; }
     if ((np < 0 || (r.np == np && r.W == 64)) && (prec < 0 || r.prec == prec)) {
       float e = 0.f;
+      LGH_HIP (hipEventSynchronize (r.b));
       LGH_HIP (hipEventElapsedTime (&e, r.a, r.b));
       c++; t += e; by += r.bytes; fl += r.flops;
     }
@@ -129,8 +171,13 @@ prof_collect (int kind, int np, int prec, long *count, double *ms,
 
 // ---- caching allocator ----------------------------------------------------
 // Blocks are rounded up (256 B granularity below 1 MiB, 2 MiB above) and
-// recycled by exact rounded size.
-static std::multimap<size_t, void *> g_free;
+// recycled by exact rounded size.  A freed block goes to the free list of
+// the freeing thread's stream: work queued on that stream may still use
+// it, and only later work on the same stream is ordered after that use.
+// A thread that leaves the engine (dev_thread_exit) synchronizes its
+// stream and hands its blocks to the shared list (key nullptr), which any
+// stream may take from.
+static std::map<hipStream_t, std::multimap<size_t, void *>> g_free;
 static std::map<void *, size_t> g_live;
 static size_t g_in_use = 0;
 
@@ -142,20 +189,44 @@ round_up (size_t b)
   return (b + g - 1) / g * g;
 }
 
+static bool
+take (hipStream_t key, size_t b, void **p)
+{
+  auto l = g_free.find (key);
+  if (l == g_free.end ()) return false;
+  auto it = l->second.find (b);
+  if (it == l->second.end ()) return false;
+  *p = it->second;
+  l->second.erase (it);
+  return true;
+}
+
+static void
+free_list_release (hipStream_t s)
+{
+  std::lock_guard<std::mutex> lk (g_mu);
+  auto l = g_free.find (s);
+  if (l == g_free.end ()) return;
+  auto &shared = g_free[nullptr];
+  for (auto &kv : l->second) shared.insert (kv);
+  g_free.erase (s);
+}
+
 void *
 dmalloc (size_t bytes)
 {
   if (dev_init ()) return nullptr;
+  const hipStream_t s = t_ctx.s;
   std::lock_guard<std::mutex> lk (g_mu);
   size_t b = round_up (bytes ? bytes : 1);
-  auto it = g_free.find (b);
   void *p = nullptr;
-  if (it != g_free.end ()) { p = it->second; g_free.erase (it); }
-  else {
+  if (!take (s, b, &p) && !take (nullptr, b, &p)) {
     hipError_t e = hipMalloc (&p, b);
     if (e != hipSuccess) {
-      // release the cache and retry once
-      for (auto &kv : g_free) (void) hipFree (kv.second);
+      // release every cached block and retry once (hipFree waits for the
+      // device, so blocks other streams still use are not reused early)
+      for (auto &l : g_free)
+        for (auto &kv : l.second) (void) hipFree (kv.second);
       g_free.clear ();
       e = hipMalloc (&p, b);
       if (e != hipSuccess) {
@@ -176,11 +247,12 @@ void
 dfree (void *p)
 {
   if (!p) return;
+  const hipStream_t s = t_ctx.s;
   std::lock_guard<std::mutex> lk (g_mu);
   auto it = g_live.find (p);
   if (it == g_live.end ()) return;
   g_in_use -= it->second;
-  g_free.insert ({ it->second, p });
+  g_free[s].insert ({ it->second, p });
   g_live.erase (it);
 }
 
@@ -188,8 +260,9 @@ void
 dtrim ()
 {
   std::lock_guard<std::mutex> lk (g_mu);
-  if (g_stream) (void) hipStreamSynchronize (g_stream);
-  for (auto &kv : g_free) (void) hipFree (kv.second);
+  (void) hipDeviceSynchronize ();
+  for (auto &l : g_free)
+    for (auto &kv : l.second) (void) hipFree (kv.second);
   g_free.clear ();
 }
 
@@ -199,8 +272,8 @@ int
 h2d (void *dst, const void *src, size_t bytes)
 {
   if (!bytes) return 0;
-  LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyHostToDevice, g_stream));
-  LGH_HIP (hipStreamSynchronize (g_stream));
+  LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyHostToDevice, stream ()));
+  LGH_HIP (hipStreamSynchronize (t_ctx.s));
   return 0;
 }
 
@@ -208,8 +281,8 @@ int
 d2h (void *dst, const void *src, size_t bytes)
 {
   if (!bytes) return 0;
-  LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
-  LGH_HIP (hipStreamSynchronize (g_stream));
+  LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyDeviceToHost, stream ()));
+  LGH_HIP (hipStreamSynchronize (t_ctx.s));
   return 0;
 }
 
@@ -217,7 +290,7 @@ int
 dcopy (void *dst, const void *src, size_t bytes)
 {
   if (!bytes) return 0;
-  LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
+  LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyDeviceToDevice, stream ()));
   return 0;
 }
 
@@ -289,22 +362,22 @@ pmat_free (PMat &m)
 int
 sol_potrf (int upper, int n, double *A, int lda, int *info_host)
 {
-  rocblas_status s = rocsolver_dpotrf (g_rb, upper ? rocblas_fill_upper
+  rocblas_status s = rocsolver_dpotrf (t_ctx.rb, upper ? rocblas_fill_upper
                                                    : rocblas_fill_lower,
-                                       n, A, lda, g_info_dev);
+                                       n, A, lda, t_ctx.info);
   LGH_CHECK (s == rocblas_status_success, 6, "rocsolver_dpotrf failed");
-  return d2h (info_host, g_info_dev, sizeof (int));
+  return d2h (info_host, t_ctx.info, sizeof (int));
 }
 
 int
 sol_trtri (int upper, int n, double *A, int lda, int *info_host)
 {
-  rocblas_status s = rocsolver_dtrtri (g_rb, upper ? rocblas_fill_upper
+  rocblas_status s = rocsolver_dtrtri (t_ctx.rb, upper ? rocblas_fill_upper
                                                    : rocblas_fill_lower,
                                        rocblas_diagonal_non_unit, n, A, lda,
-                                       g_info_dev);
+                                       t_ctx.info);
   LGH_CHECK (s == rocblas_status_success, 6, "rocsolver_dtrtri failed");
-  return d2h (info_host, g_info_dev, sizeof (int));
+  return d2h (info_host, t_ctx.info, sizeof (int));
 }
 
 int
@@ -312,12 +385,12 @@ sol_syevd (int n, double *A, int lda, double *w_dev, int *info_host)
 {
   double *E = (double *) dmalloc (sizeof (double) * (size_t) (n > 0 ? n : 1));
   if (!E) return 5;
-  rocblas_status s = rocsolver_dsyevd (g_rb, rocblas_evect_original,
+  rocblas_status s = rocsolver_dsyevd (t_ctx.rb, rocblas_evect_original,
                                        rocblas_fill_upper, n, A, lda, w_dev,
-                                       E, g_info_dev);
+                                       E, t_ctx.info);
   int rc = 0;
   if (s != rocblas_status_success) { set_error ("rocsolver_dsyevd failed"); rc = 6; }
-  if (!rc) rc = d2h (info_host, g_info_dev, sizeof (int));
+  if (!rc) rc = d2h (info_host, t_ctx.info, sizeof (int));
   dfree (E);
   return rc;
 }

@@ -44,7 +44,11 @@ int hip_fail (hipError_t e, const char *what, const char *file, int line);
   do { if (!(cond)) { ::lgh::set_error (msg); return (code); } } while (0)
 
 // ---- context ------------------------------------------------------------
-int  dev_init ();                 // idempotent; selects device 0
+int  dev_init ();                 // idempotent; selects device 0 and
+                                  // makes the calling thread's stream
+void dev_thread_exit ();          // a worker thread leaves the engine:
+                                  // waits for its stream, hands its cached
+                                  // blocks to the shared list
 hipStream_t stream ();
 int  dev_sync ();
 void *rocblas_h ();               // rocblas_handle bound to stream()

@@ -132,7 +132,7 @@ worker_loop (lgh_serve *s)
     {
       std::unique_lock<std::mutex> lk (s->mu);
       s->cv_work.wait (lk, [&] { return s->stop || !s->queue.empty (); });
-      if (s->queue.empty ()) return;      // stop requested, nothing queued
+      if (s->queue.empty ()) break;       // stop requested, nothing queued
       const int op = s->queue.front ()->op;
       int cols = 0;
       for (auto it = s->queue.begin (); it != s->queue.end ();) {
@@ -151,6 +151,7 @@ worker_loop (lgh_serve *s)
     std::lock_guard<std::mutex> lk (s->mu);
     s->st.busy_s += std::chrono::duration<double> (clk::now () - t0).count ();
   }
+  dev_thread_exit ();     // the worker's stream and cached blocks
 }
 
 extern "C" lgh_serve_opts_t

@@ -246,3 +246,33 @@ def cfg3_problem(side=128):
 
     B = psf_B(side, 3, lambda h: _ball_offsets(2.0, 3), val, dtype=np.float32)
     return B.astype(np.float64), Z, M
+
+
+def cfg5_instance(k, seed=5):
+    """Instance k of the batched configuration (256 small 2D problems):
+    side s = round(exp(U(ln 64, ln 256))) (n = s^2 in [4096, 65536]), h =
+    1/(s-1), PSF width sigma ~ U[1.5h, 3h] truncated at radius 3 sigma,
+    amplitude field a(x) = 1 + 0.5 sin(2 pi (f1 x1 + p1)) cos(2 pi (f2 x2 +
+    p2)) with f ~ U[0.5, 2], p ~ U[0, 1), B_ij = (a_i + a_j)/2 exp(-|x_i -
+    x_j|^2 / (2 sigma^2)), Z = M + 0.1 K, rank r in {32, 64, 128} (uniform)
+    and ell = r + 10.  Every draw comes from numpy's generator seeded with
+    (seed, k), so instance k is the same whatever the batch.
+    Returns (B, Z, M, ell)."""
+    rng = np.random.default_rng([seed, k])
+    side = int(round(math.exp(rng.uniform(math.log(64), math.log(256)))))
+    h = 1.0 / (side - 1)
+    sig = rng.uniform(1.5, 3.0)                 # in units of h
+    f = rng.uniform(0.5, 2.0, size=2)
+    ph = rng.uniform(0.0, 1.0, size=2)
+    rank = int(rng.choice([32, 64, 128]))
+    Z, M = prior_Z(side, 2, 1.0, 0.1)
+    X, _ = grid_coords(side, 2)
+    a = 1.0 + 0.5 * np.sin(2 * np.pi * (f[0] * X[:, 0] + ph[0])) \
+        * np.cos(2 * np.pi * (f[1] * X[:, 1] + ph[1]))
+    s = sig * h
+
+    def val(i, j, d):
+        return 0.5 * (a[i] + a[j]) * np.exp(-(d ** 2).sum(1) / (2 * s * s))
+
+    B = psf_B(side, 2, lambda hh: _ball_offsets(3.0 * sig, 2), val)
+    return B, Z, M, rank + 10

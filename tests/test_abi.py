@@ -143,3 +143,7 @@ def test_api_refusals_without_gpu(lgh):
     assert L.lgh_glr_compute(B.h, pr.h, C.byref(o), C.byref(g), None) == 63
     o = lgh.glr_opts(ell=5, backend=2)
     assert L.lgh_glr_compute(B.h, pr.h, C.byref(o), C.byref(g), None) == 56
+    # batches: nothing to do, and missing arrays (PETSC_ERR_ARG_NULL)
+    assert L.lgh_glr_compute_batch(0, None, None, None, None, 0, None, None) == 0
+    assert L.lgh_glr_compute_batch(2, None, None, None, None, 0, None, None) == 85
+    assert L.lgh_prior_create_mat_batch(2, None, None, None, 0, None) == 85
