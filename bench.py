@@ -306,22 +306,25 @@ def cfg5_bench(args, W, lgh):
         Bm, z, m, ell = mats[j]
         pr = lgh.Prior.from_mat(z, m, blocked_mode=3)
         lgh.GLR(Bm, pr, lgh.glr_opts(ell=ell, q_power=1, seed=0), x=mixed)
-    lgh.lib().lgh_hip_sync()
-    t0 = time.perf_counter()
-    priors = [lgh.Prior.from_mat(z, m, blocked_mode=3) for _, z, m, _ in mats]
-    lgh.lib().lgh_hip_sync()
-    t_prior = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    kept = []
-    for (Bm, z, m, ell), pr in zip(mats, priors):
-        g = lgh.GLR(Bm, pr, lgh.glr_opts(ell=ell, q_power=1, seed=0), x=mixed)
-        kept.append(g.kept)
-        del g
-    lgh.lib().lgh_hip_sync()
-    t_seq = time.perf_counter() - t0
-    log("cfg5 sequential: priors %.2fs, builds %.2fs (%.1f instances/s)"
-        % (t_prior, t_seq, len(mats) / t_seq))
-    del priors
+    t_prior = t_seq = None
+    kept = None
+    if not args.skip_sequential:
+        lgh.lib().lgh_hip_sync()
+        t0 = time.perf_counter()
+        priors = [lgh.Prior.from_mat(z, m, blocked_mode=3) for _, z, m, _ in mats]
+        lgh.lib().lgh_hip_sync()
+        t_prior = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        kept = []
+        for (Bm, z, m, ell), pr in zip(mats, priors):
+            g = lgh.GLR(Bm, pr, lgh.glr_opts(ell=ell, q_power=1, seed=0), x=mixed)
+            kept.append(g.kept)
+            del g
+        lgh.lib().lgh_hip_sync()
+        t_seq = time.perf_counter() - t0
+        log("cfg5 sequential: priors %.2fs, builds %.2fs (%.1f instances/s)"
+            % (t_prior, t_seq, len(mats) / t_seq))
+        del priors
     # the same instances as batches (lgh_prior_create_mat_batch,
     # lgh_glr_compute_batch): host worker threads, one stream each
     lgh.lib().lgh_hip_sync()
@@ -337,9 +340,9 @@ def cfg5_bench(args, W, lgh):
     t_batch = time.perf_counter() - t0
     kept_b = [g.kept for g in gs]
     del gs
-    log("cfg5 batch: priors %.2fs, builds %.2fs (%.1f instances/s), kept %s"
-        % (t_bprior, t_batch, len(mats) / t_batch,
-           "same" if kept_b == kept else "DIFFERENT"))
+    log("cfg5 batch (%d workers): priors %.2fs, builds %.2fs (%.1f instances/s)%s"
+        % (args.batch_workers or 8, t_bprior, t_batch, len(mats) / t_batch,
+           "" if kept is None else ", kept " + ("same" if kept_b == kept else "DIFFERENT")))
     line = {
         "metric": "GLR build s + solve RHS/s, p99 latency; mixed vs FP64 plain at rel err <=1e-6",
         "value": t_batch, "unit": "s/batch", "n_gpus": 1, "steps": 1,
@@ -356,7 +359,8 @@ def cfg5_bench(args, W, lgh):
         "instances_per_s": len(mats) / t_batch, "priors_s": t_bprior,
         "sequential_builds_s": t_seq, "sequential_priors_s": t_prior,
         "batch_workers": args.batch_workers or 8,
-        "kept_total": int(sum(kept_b)), "kept_same_as_sequential": kept_b == kept,
+        "kept_total": int(sum(kept_b)),
+        "kept_same_as_sequential": None if kept is None else kept_b == kept,
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -392,6 +396,8 @@ def main():
                     help="cfg5: number of instances")
     ap.add_argument("--batch-workers", type=int, default=0,
                     help="cfg5: concurrent build workers (0: the library's default)")
+    ap.add_argument("--skip-sequential", action="store_true",
+                    help="cfg5: time the batches only")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")

@@ -51,6 +51,22 @@ def test_spmm(dev, Wd):
     assert rel(y32, ref) < 1e-5
 
 
+@pytest.mark.parametrize("prec_vals", [0, 1])
+def test_spmm_wide_rows_width_invariant(dev, prec_vals):
+    """A PSF operator with wide rows (cfg-1 style, up to 49 entries a row):
+    the 64-wide row kernel (grouped neighbour loads) is bit-identical to
+    the generic 32-wide kernel (the same sums in the same order) and
+    matches scipy."""
+    B, _, _ = W.cfg1_problem(45)
+    assert B.nnz >= 32 * B.shape[0]
+    X = np.random.default_rng(5).standard_normal((B.shape[0], 150))
+    y64 = dev.spmm(B, X, prec_vals=prec_vals, W=64)
+    y32 = dev.spmm(B, X, prec_vals=prec_vals, W=32)
+    assert np.array_equal(y64, y32)
+    ref = B @ X if prec_vals == 0 else B.astype(np.float32).astype(np.float64) @ X
+    assert rel(y64, ref) < 1e-13
+
+
 def test_cheb_jacobi_matches_oracle(dev, oracle):
     Z = W.tpm_Z()
     n = Z.shape[0]
