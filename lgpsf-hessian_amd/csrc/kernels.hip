@@ -1407,60 +1407,75 @@ xcd_blocks (int ngroups, int nsib)
   return (unsigned) (((ngroups + 7) / 8) * 8) * (unsigned) nsib;
 }
 
-// Gram partials: block (chunk, pair) computes the 64 x 64 tile
-// X_pi^T Y_pj over rows [r0, r1) (the pairs of one chunk are siblings).
-// Wave w owns rows 32*(w>>1).., cols 32*(w&1).. of the tile as 2 x 2 MFMA
-// 16 x 16 tiles.  MFMA 16x16x4 operand layout: A(i, k) at lane i + 16 k,
-// B(k, j) at lane j + 16 k; result rows per mfma<T>::row.
+// Gram partials: block (chunk, group) computes the four 64 x 64 tiles
+// X_px^T Y_py, px in {2 gp, 2 gp + 1}, py in {2 gq, 2 gq + 1}, over rows
+// [r0, r1) (the groups of one chunk are siblings).  Wave w owns tile
+// (2 gp + w / 2, 2 gq + w % 2) as 4 x 4 MFMA 16 x 16 tiles: per k-step of
+// 4 rows it loads 4 A and 4 B fragments and issues 16 MFMAs, and each
+// panel row it reads is shared with one other wave of the block (L1), so
+// a panel chunk is read once per tile group instead of once per tile.
+// Tiles past the panels, and in the symmetric case below the diagonal,
+// are skipped; tile_of maps (px, py) to the pair slot of the reduction.
+// MFMA 16x16x4 operand layout: A(i, k) at lane i + 16 k, B(k, j) at lane
+// j + 16 k; result rows per mfma<T>::row.
 template <typename T>
 __global__ void __launch_bounds__ (TPB)
-gram_k (int n, int nchunks, int npairs, const T *__restrict__ X, int xp0,
-        const T *__restrict__ Y, int yp0, int rows_per_chunk,
-        const int2 *__restrict__ pairs, double *__restrict__ part)
+gram_k (int n, int nchunks, int ngroups, const T *__restrict__ X, int xp0,
+        int npx, const T *__restrict__ Y, int yp0, int npy, int rows_per_chunk,
+        const int2 *__restrict__ groups, const int *__restrict__ tile_of,
+        int npairs, double *__restrict__ part)
 {
   typedef typename mfma<T>::acc_t acc_t;
-  int chunk, pair;
-  xcd_split (npairs, chunk, pair);
+  int chunk, grp;
+  xcd_split (ngroups, chunk, grp);
   if (chunk >= nchunks) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int2 pr = pairs[pair];
-  const T *Xp = X + (size_t) (xp0 + pr.x) * n * 64;
-  const T *Yp = Y + (size_t) (yp0 + pr.y) * n * 64;
+  const int2 gr = groups[grp];
+  const int px = 2 * gr.x + (wave >> 1), py = 2 * gr.y + (wave & 1);
+  if (px >= npx || py >= npy) return;
+  const int slot = tile_of[px * npy + py];
+  if (slot < 0) return;
+  const T *Xp = X + (size_t) (xp0 + px) * n * 64;
+  const T *Yp = Y + (size_t) (yp0 + py) * n * 64;
   const int r0 = chunk * rows_per_chunk;
   const int r1 = min (n, r0 + rows_per_chunk);
-  const int mb = 32 * (wave >> 1), nb = 32 * (wave & 1);
   const int li = lane & 15, lk = lane >> 4;
-  acc_t acc[2][2];
-  for (int a = 0; a < 2; a++)
-    for (int b = 0; b < 2; b++)
+  acc_t acc[4][4];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++)
       for (int r = 0; r < 4; r++) acc[a][b][r] = 0;
-  // 8 k-steps (32 rows) per trip: all 32 loads issued before the MFMAs
-  constexpr int KU = 8;
+  // 4 k-steps (16 rows) per trip: all 32 loads issued before the MFMAs
+  constexpr int KU = 4;
   for (int k0 = r0; k0 < r1; k0 += 4 * KU) {
-    T av[KU][2], bv[KU][2];
+    T av[KU][4], bv[KU][4];
 #pragma unroll
     for (int u = 0; u < KU; u++) {
       const int kr = k0 + 4 * u + lk;
       if (kr < r1) {
         const T *xr = Xp + (size_t) kr * 64;
         const T *yr = Yp + (size_t) kr * 64;
-        av[u][0] = xr[mb + li]; av[u][1] = xr[mb + 16 + li];
-        bv[u][0] = yr[nb + li]; bv[u][1] = yr[nb + 16 + li];
+#pragma unroll
+        for (int q = 0; q < 4; q++) { av[u][q] = xr[16 * q + li]; bv[u][q] = yr[16 * q + li]; }
       }
-      else { av[u][0] = av[u][1] = bv[u][0] = bv[u][1] = (T) 0; }
+      else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) { av[u][q] = (T) 0; bv[u][q] = (T) 0; }
+      }
     }
 #pragma unroll
     for (int u = 0; u < KU; u++)
-      for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
           acc[a][b] = mfma<T>::op (av[u][a], bv[u][b], acc[a][b]);
   }
-  double *tile = part + ((size_t) chunk * npairs + pair) * 4096;
-  for (int a = 0; a < 2; a++)
-    for (int b = 0; b < 2; b++)
+  double *tile = part + ((size_t) chunk * npairs + slot) * 4096;
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++)
       for (int r = 0; r < 4; r++) {
-        const int row = mb + 16 * a + mfma<T>::row (lk, r);
-        const int col = nb + 16 * b + li;
+        const int row = 16 * a + mfma<T>::row (lk, r);
+        const int col = 16 * b + li;
         tile[col * 64 + row] = (double) acc[a][b][r];
       }
 }
@@ -1507,36 +1522,50 @@ k_gram (const PMat &X, int xp0, int npx, const PMat &Y, int yp0, int npy,
   LGH_CHECK (X.W == 64 && Y.W == 64 && X.prec == Y.prec
              && X.nrows == Y.nrows && xp0 + npx <= X.npanels ()
              && yp0 + npy <= Y.npanels (), 8, "k_gram: shape mismatch");
+  // tiles (pairs) in the reduction's order, and their 2 x 2 groups
   std::vector<int2> pv;
+  std::vector<int> tile_of ((size_t) std::max (npx * npy, 1), -1);
   for (int i = 0; i < npx; i++)
-    for (int j = (sym ? i : 0); j < npy; j++) pv.push_back (make_int2 (i, j));
+    for (int j = (sym ? i : 0); j < npy; j++) {
+      tile_of[(size_t) i * npy + j] = (int) pv.size ();
+      pv.push_back (make_int2 (i, j));
+    }
   const int npairs = (int) pv.size ();
   if (npairs == 0) return 0;
+  std::vector<int2> gv;
+  for (int a = 0; a < (npx + 1) / 2; a++)
+    for (int b = (sym ? a : 0); b < (npy + 1) / 2; b++) gv.push_back (make_int2 (a, b));
+  const int ngroups = (int) gv.size ();
   const int n = X.nrows;
   ProfScope ps (PK_GRAM, (double) n * 64 * prec_bytes (X.prec) * (npx + npy),
                 2.0 * n * 4096.0 * npairs);
   // aim for ~4096 blocks; at least 256 rows per chunk
-  int nch = 4096 / npairs;
+  int nch = 4096 / ngroups;
   if (nch < 1) nch = 1;
   int rpc = (n + nch - 1) / nch;
   if (rpc < 256) rpc = 256;
   rpc = (rpc + 3) / 4 * 4;
   nch = (n + rpc - 1) / rpc;
   if (nch < 1) nch = 1;
-  int2 *pd = (int2 *) dmalloc (sizeof (int2) * npairs);
+  int2 *pd = (int2 *) dmalloc (sizeof (int2) * (npairs + ngroups));
+  int *td = (int *) dmalloc (sizeof (int) * tile_of.size ());
   double *part = (double *) dmalloc (sizeof (double) * 4096 * (size_t) nch * npairs);
-  if (!pd || !part) { dfree (pd); dfree (part); return 5; }
+  if (!pd || !td || !part) { dfree (pd); dfree (td); dfree (part); return 5; }
   int rc = h2d (pd, pv.data (), sizeof (int2) * npairs);
+  if (!rc) rc = h2d (pd + npairs, gv.data (), sizeof (int2) * ngroups);
+  if (!rc) rc = h2d (td, tile_of.data (), sizeof (int) * tile_of.size ());
   if (!rc) {
-    const unsigned g = xcd_blocks (nch, npairs);
+    const unsigned g = xcd_blocks (nch, ngroups);
     if (X.prec == F64)
       hipLaunchKernelGGL (gram_k<double>, dim3 (g), dim3 (TPB), 0, stream (),
-                          n, nch, npairs, (const double *) X.p, xp0,
-                          (const double *) Y.p, yp0, rpc, pd, part);
+                          n, nch, ngroups, (const double *) X.p, xp0, npx,
+                          (const double *) Y.p, yp0, npy, rpc, pd + npairs, td,
+                          npairs, part);
     else
       hipLaunchKernelGGL (gram_k<float>, dim3 (g), dim3 (TPB), 0, stream (),
-                          n, nch, npairs, (const float *) X.p, xp0,
-                          (const float *) Y.p, yp0, rpc, pd, part);
+                          n, nch, ngroups, (const float *) X.p, xp0, npx,
+                          (const float *) Y.p, yp0, npy, rpc, pd + npairs, td,
+                          npairs, part);
     hipError_t e = hipGetLastError ();
     if (e != hipSuccess) rc = hip_fail (e, "gram_k", __FILE__, __LINE__);
   }
@@ -1547,6 +1576,7 @@ k_gram (const PMat &X, int xp0, int npx, const PMat &Y, int yp0, int npy,
     if (e != hipSuccess) rc = hip_fail (e, "gram_reduce_k", __FILE__, __LINE__);
   }
   dfree (pd);
+  dfree (td);
   dfree (part);
   return rc;
 }
