@@ -76,6 +76,55 @@ def test_glr_gates(dev, oracle, spd):
     assert np.max(np.abs(g2.lam - lam) / np.abs(lam)) < 1e-10
 
 
+@pytest.mark.parametrize("treat", [0, 1, 2])
+def test_glr_treatments_match_oracle(dev, oracle, treat):
+    """FLIP / RELU / NONE (glr_common_impl.h:549-559) on the indefinite
+    closed-form problem: kept count, treated spectrum, logdet and a solve
+    equal the oracle's; RELU keeps zero-valued modes in kept."""
+    Bd = W.tg_B_exact(spd=False)
+    prior = dev.Prior.diagonal_callbacks(W.tg_z(), W.tg_m())
+    g = dev.GLR(dev.Mat.dense(Bd), prior,
+                dev.glr_opts(ell=40, trunc_rel=1e-8, treat=treat))
+    og = oracle.GLR(oracle.CSR.from_dense(Bd), W.tg_m(),
+                    oracle.ZSolver.diag(W.tg_z()), ell=40, q_power=1,
+                    trunc_rel=1e-8, treat=treat)
+    assert g.kept == og.kept == 10
+    lam, olam = g.lam, og.lam
+    assert np.max(np.abs(lam - olam)) < 1e-12 * np.max(np.abs(olam))
+    if treat == 0:
+        assert np.all(lam > 0)
+    elif treat == 1:
+        assert np.all(lam >= 0) and np.sum(lam == 0) >= 5
+    else:
+        assert np.sum(lam < 0) >= 5
+    c = 0.7 + 2 * max(0.0, -lam.min())
+    assert abs(g.logdet(c) - og.logdet(c)) < 1e-12 * abs(og.logdet(c))
+    v = W.tg_probe()
+    assert rel(g.solve(c, v), og.solve(c, v)) < 1e-12
+
+
+@pytest.mark.parametrize("ell", [1, 120])
+def test_glr_extreme_sketch_widths(dev, oracle, ell):
+    """ell = 1 (one hashed column) and ell = N = 120 (a sketch as wide as
+    the problem; the N = 120 rows also leave a ragged 64-row panel tail):
+    kept modes and spectrum equal the oracle's; at ell = N the rank-10
+    spectrum is the dense one."""
+    Bd = W.tg_B_exact(spd=True)
+    prior = dev.Prior.diagonal_callbacks(W.tg_z(), W.tg_m())
+    g = dev.GLR(dev.Mat.dense(Bd), prior, dev.glr_opts(ell=ell, trunc_rel=1e-8))
+    og = oracle.GLR(oracle.CSR.from_dense(Bd), W.tg_m(),
+                    oracle.ZSolver.diag(W.tg_z()), ell=ell, q_power=1,
+                    trunc_rel=1e-8)
+    assert g.kept == og.kept == min(ell, 10)
+    assert np.max(np.abs(g.lam - og.lam) / np.abs(og.lam)) < 1e-11
+    v = W.tg_probe()
+    assert rel(g.pow(0.7, 1.0, g.pow(0.7, -1.0, v)), v) < 1e-10
+    if ell == 120:      # the model is exact: solve inverts the true apply
+        dense = np.load(os.path.join(GOLDEN, "tg_eigs_spd.npy"))
+        assert np.max(np.abs(g.lam - dense[:10]) / dense[:10]) < 1e-8
+        assert rel(g.solve(0.7, g.apply(0.7, v)), v) < 1e-8
+
+
 def test_glr_refusals(dev):
     prior = dev.Prior.diagonal_callbacks(W.tg_z(), W.tg_m())
     B = dev.Mat.dense(W.tg_B_exact())
