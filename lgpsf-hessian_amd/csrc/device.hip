@@ -37,6 +37,8 @@ struct ThreadCtx {
   hipStream_t s = nullptr;
   rocblas_handle rb = nullptr;
   int *info = nullptr;
+  hipStream_t main = nullptr;     // s outside any StreamScope
+  hipStream_t aux = nullptr;
 };
 static thread_local ThreadCtx t_ctx;
 
@@ -46,6 +48,7 @@ ctx_init (ThreadCtx &c)
   if (c.s) return 0;
   LGH_HIP (hipSetDevice (0));
   LGH_HIP (hipStreamCreateWithFlags (&c.s, hipStreamNonBlocking));
+  c.main = c.s;
   if (rocblas_create_handle (&c.rb) != rocblas_status_success) {
     set_error ("rocblas_create_handle failed");
     return 4;
@@ -79,6 +82,19 @@ stream ()
 }
 void *rocblas_h () { (void) stream (); return (void *) t_ctx.rb; }
 
+hipStream_t
+aux_stream ()
+{
+  (void) stream ();
+  if (!t_ctx.aux
+      && hipStreamCreateWithFlags (&t_ctx.aux, hipStreamNonBlocking) != hipSuccess)
+    t_ctx.aux = nullptr;
+  return t_ctx.aux;
+}
+
+StreamScope::StreamScope (hipStream_t s) : prev (stream ()) { t_ctx.s = s; }
+StreamScope::~StreamScope () { t_ctx.s = prev; }
+
 int
 dev_sync ()
 {
@@ -92,12 +108,17 @@ void
 dev_thread_exit ()
 {
   ThreadCtx &c = t_ctx;
-  if (!c.s) return;
-  (void) hipStreamSynchronize (c.s);
-  free_list_release (c.s);
+  if (!c.main) return;
+  (void) hipStreamSynchronize (c.main);
+  free_list_release (c.main);
+  if (c.aux) {
+    (void) hipStreamSynchronize (c.aux);
+    free_list_release (c.aux);
+    (void) hipStreamDestroy (c.aux);
+  }
   (void) hipFree (c.info);
   rocblas_destroy_handle (c.rb);
-  (void) hipStreamDestroy (c.s);
+  (void) hipStreamDestroy (c.main);
   c = ThreadCtx ();
 }
 

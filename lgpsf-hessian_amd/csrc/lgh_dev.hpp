@@ -50,6 +50,16 @@ void dev_thread_exit ();          // a worker thread leaves the engine:
                                   // waits for its stream, hands its cached
                                   // blocks to the shared list
 hipStream_t stream ();
+// the thread's auxiliary stream (default priority, created on first use):
+// work that may overlap the main stream's, ordered by events
+hipStream_t aux_stream ();
+// while alive, the thread's engine calls enqueue on s instead of its
+// stream (allocator free lists follow: blocks freed here are reused on s)
+struct StreamScope {
+  hipStream_t prev;
+  explicit StreamScope (hipStream_t s);
+  ~StreamScope ();
+};
 int  dev_sync ();
 void *rocblas_h ();               // rocblas_handle bound to stream()
 

@@ -260,11 +260,6 @@ split_tail (const PMat &B, PMat &X, WK &w, bool *done, F &&run)
   LGH_CHECK (X.W == 64 && X.ncols == B.ncols && X.prec == B.prec, 8,
              "Z-solve: X shape differs from B");
   *done = true;
-  if (full > 0) {
-    PMat Bf = B, Xf = X;
-    Bf.ncols = Xf.ncols = 64 * full;
-    LGH_TRY (run (Bf, Xf, w));
-  }
   if (!w.tail) w.tail = new WK ();
   if (!w.Bt.p || w.Bt.nrows != B.nrows || w.Bt.prec != B.prec) {
     pmat_free (w.Bt); pmat_free (w.Xt);
@@ -274,7 +269,39 @@ split_tail (const PMat &B, PMat &X, WK &w, bool *done, F &&run)
   // Bt's columns past `tail` are zero padding (so are Xt's after the solve)
   LGH_TRY (dzero (w.Bt.p, w.Bt.bytes ()));
   LGH_TRY (k_copy_cols (B, 64 * full, w.Bt, 0, tail));
-  LGH_TRY (run (w.Bt, w.Xt, *w.tail));
+  // the narrow panel's solve runs on the auxiliary stream, overlapping the
+  // whole panels' (their coarse levels are latency-bound and leave the
+  // device idle); it touches only its own buffers (tail work, the V-cycle's
+  // second work slot, Bt, Xt)
+  const hipStream_t main = stream ();
+  hipStream_t aux = (full > 0) ? aux_stream () : nullptr;
+  hipEvent_t ready = nullptr, done_ev = nullptr;
+  if (aux && (hipEventCreateWithFlags (&ready, hipEventDisableTiming) != hipSuccess
+              || hipEventCreateWithFlags (&done_ev, hipEventDisableTiming) != hipSuccess
+              || hipEventRecord (ready, main) != hipSuccess
+              || hipStreamWaitEvent (aux, ready, 0) != hipSuccess))
+    aux = nullptr;                       // fall back to one stream
+  int rc = 0;
+  if (aux) {
+    StreamScope on_aux (aux);
+    rc = run (w.Bt, w.Xt, *w.tail);
+    if (!rc && hipEventRecord (done_ev, aux) != hipSuccess)
+      rc = hip_fail (hipGetLastError (), "hipEventRecord", __FILE__, __LINE__);
+  }
+  else rc = run (w.Bt, w.Xt, *w.tail);   // on the main stream, in order
+  if (!rc && full > 0) {                 // the whole panels, in place
+    PMat Bf = B, Xf = X;
+    Bf.ncols = Xf.ncols = 64 * full;
+    rc = run (Bf, Xf, w);
+  }
+  if (aux) {
+    if (!rc && hipStreamWaitEvent (main, done_ev, 0) != hipSuccess)
+      rc = hip_fail (hipGetLastError (), "hipStreamWaitEvent", __FILE__, __LINE__);
+    if (rc) (void) hipStreamSynchronize (aux);
+  }
+  if (ready) (void) hipEventDestroy (ready);
+  if (done_ev) (void) hipEventDestroy (done_ev);
+  LGH_TRY (rc);
   LGH_TRY (dzero (X.panel (full), (size_t) X.nrows * 64 * prec_bytes (X.prec)));
   return k_copy_cols (w.Xt, 0, X, 64 * full, tail);
 }
