@@ -51,7 +51,7 @@ def pmc_traffic(kernel, panels):
     """HBM bytes per launch of `kernel` at `panels` panels from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel
     calibration recorded there), or None when it holds no such line."""
-    path = os.path.join(ROOT, "profiles", "cfg2_build_pmc_traffic.md")
+    path = os.path.join(ROOT, "profiles", "glr_build_grid2d_1024_pmc_traffic.md")
     try:
         with open(path) as f:
             for line in f:
@@ -180,7 +180,7 @@ def dominant_roofline(lgh, args, prof, npan, vpc):
             "algorithmic_bytes_per_launch": by / c}
 
 
-def cfg3_bench(args, W, lgh):
+def grid3d_bench(args, W, lgh):
     """3D configuration: 128^3 grid (n = 2 097 152), Z = M + 0.1 K7, B an
     isotropic PSF (sigma 1.5h, radius-2h ball, 33 nnz/row) with FP32
     values; mixed build at ell = 400, then lgh_glr_extend by 200 twice
@@ -188,9 +188,9 @@ def cfg3_bench(args, W, lgh):
     one-shot ell = 800 build (extend == one-shot for the hashed sketch)."""
     side = 128 if args.side == 1024 else args.side
     t0 = time.time()
-    Bs, Zs, M = W.cfg3_problem(side)
+    Bs, Zs, M = W.psf3d_problem(side)
     n = Zs.shape[0]
-    log("cfg3 problem n=%d nnz(B)=%d (%.1f/row) gen %.1fs"
+    log("3D problem n=%d nnz(B)=%d (%.1f/row) gen %.1fs"
         % (n, Bs.nnz, Bs.nnz / n, time.time() - t0))
     lgh.init()
     t0 = time.time()
@@ -225,7 +225,7 @@ def cfg3_bench(args, W, lgh):
     single = ("cheb_mixed", "smooth_pre_fine", "smooth_post_fine", "resid_fine")
     prof = {k: lgh.prof_collect(k, npan, vpc) if k in single else lgh.prof_collect(k)
             for k in lgh.PROF_KINDS}
-    log("cfg3 mixed build %.3fs kept=%d report=%s" % (t_build, g.kept, g.report))
+    log("3D mixed build %.3fs kept=%d report=%s" % (t_build, g.kept, g.report))
     for k, (c, ms, by, fl) in prof.items():
         if c:
             log("  %-16s launches %6d  %9.2f ms  avg %.3f ms  %7.1f GB/s  %6.2f TF/s"
@@ -260,7 +260,7 @@ def cfg3_bench(args, W, lgh):
         "warmup": args.warmup, "ms_per_step": t_build * 1e3,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": "cfg3: 3D %d^3 grid n=%d, isotropic PSF B (sigma "
+        "config": {"workload": "3D %d^3 grid n=%d, isotropic PSF B (sigma "
                                "1.5h, radius 2h, %.1f nnz/row, FP32 values), "
                                "Z=M+0.1K7, ell=%d q=1, then extend +%d twice, "
                                "Z^-1 = Chebyshev(%d) around AMG V-cycle"
@@ -277,14 +277,14 @@ def cfg3_bench(args, W, lgh):
     print(json.dumps(line), flush=True)
 
 
-def _cfg5_gen(k):
+def _small_instance_gen(k):
     import workloads as W
-    B, Z, M, ell = W.cfg5_instance(k)
+    B, Z, M, ell = W.small_psf2d_instance(k)
     return B, Z, M, ell
 
 
-def cfg5_bench(args, W, lgh):
-    """Batched configuration: 256 small 2D instances (workloads.cfg5_instance:
+def batch_bench(args, W, lgh):
+    """Batched configuration: 256 small 2D instances (workloads.small_psf2d_instance:
     n = 4k..64k, ell = rank + 10 with rank in {32, 64, 128}), mixed
     precision.  Inputs are generated on the host (process pool, outside the
     timed region); timed: the priors (host AMG set-up + interval estimate)
@@ -293,10 +293,10 @@ def cfg5_bench(args, W, lgh):
     import multiprocessing as mp
     t0 = time.time()
     with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
-        inst = pool.map(_cfg5_gen, range(args.instances), chunksize=1)
+        inst = pool.map(_small_instance_gen, range(args.instances), chunksize=1)
     rows = sum(z.shape[0] for _, z, _, _ in inst)
     nnz = sum(b.nnz for b, _, _, _ in inst)
-    log("cfg5: %d instances, %d rows, %d nnz(B) (%.1f/row), gen %.1fs"
+    log("batch: %d instances, %d rows, %d nnz(B) (%.1f/row), gen %.1fs"
         % (len(inst), rows, nnz, nnz / rows, time.time() - t0))
     lgh.init()
     mats = [(lgh.Mat.aij(b), z, m, ell) for b, z, m, ell in inst]
@@ -322,7 +322,7 @@ def cfg5_bench(args, W, lgh):
             del g
         lgh.lib().lgh_hip_sync()
         t_seq = time.perf_counter() - t0
-        log("cfg5 sequential: priors %.2fs, builds %.2fs (%.1f instances/s)"
+        log("batch one at a time: priors %.2fs, builds %.2fs (%.1f instances/s)"
             % (t_prior, t_seq, len(mats) / t_seq))
         del priors
     # the same instances as batches (lgh_prior_create_mat_batch,
@@ -340,7 +340,7 @@ def cfg5_bench(args, W, lgh):
     t_batch = time.perf_counter() - t0
     kept_b = [g.kept for g in gs]
     del gs
-    log("cfg5 batch (%d workers): priors %.2fs, builds %.2fs (%.1f instances/s)%s"
+    log("batch (%d workers): priors %.2fs, builds %.2fs (%.1f instances/s)%s"
         % (args.batch_workers or 8, t_bprior, t_batch, len(mats) / t_batch,
            "" if kept is None else ", kept " + ("same" if kept_b == kept else "DIFFERENT")))
     line = {
@@ -349,7 +349,7 @@ def cfg5_bench(args, W, lgh):
         "warmup": 1, "ms_per_step": t_batch * 1e3, "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "cfg5: %d instances of 2D n=%d..%d (sum %d rows), "
+        "config": {"workload": "%d independent 2D instances n=%d..%d (sum %d rows), "
                                "PSF sigma U[1.5h,3h] radius 3 sigma (%.1f nnz/row), "
                                "Z=M+0.1K, ell=rank+10, rank in {32,64,128}, mixed"
                                % (len(mats), min(z.shape[0] for _, z, _, _ in inst),
@@ -388,16 +388,17 @@ def main():
                     help="columns of the timed lgh_glr_extend (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=300,
                     help="requests of the Poisson serving run (0: skip)")
-    ap.add_argument("--workload", choices=("cfg2", "cfg3", "cfg5"), default="cfg2",
-                    help="cfg2: the 2D 1024^2 build (the headline line); cfg3: "
-                         "the 3D 128^3 build at ell=400, extended by 200 twice; "
-                         "cfg5: 256 small 2D instances built as one batch")
+    ap.add_argument("--workload", choices=("grid2d", "grid3d", "batch"),
+                    default="grid2d",
+                    help="grid2d: the 2D 1024^2 build (the headline line); "
+                         "grid3d: the 3D 128^3 build at ell=400, extended by 200 "
+                         "twice; batch: 256 small 2D instances built as batches")
     ap.add_argument("--instances", type=int, default=256,
-                    help="cfg5: number of instances")
+                    help="batch: number of instances")
     ap.add_argument("--batch-workers", type=int, default=0,
-                    help="cfg5: concurrent build workers (0: the library's default)")
+                    help="batch: concurrent build workers (0: the library's default)")
     ap.add_argument("--skip-sequential", action="store_true",
-                    help="cfg5: time the batches only")
+                    help="batch: time the batches only")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -418,12 +419,12 @@ def main():
 
     import workloads as W
     lgh = load_pkg()
-    if args.workload == "cfg3":
-        return cfg3_bench(args, W, lgh)
-    if args.workload == "cfg5":
-        return cfg5_bench(args, W, lgh)
+    if args.workload == "grid3d":
+        return grid3d_bench(args, W, lgh)
+    if args.workload == "batch":
+        return batch_bench(args, W, lgh)
     t0 = time.time()
-    Bs, Zs, M = W.cfg2_problem(args.side)
+    Bs, Zs, M = W.aniso_psf2d_problem(args.side)
     n = Zs.shape[0]
     log("problem n=%d nnz(B)=%d (%.1f/row) gen %.1fs" % (n, Bs.nnz, Bs.nnz / n,
                                                            time.time() - t0))
@@ -487,7 +488,7 @@ def main():
     tr = pmc_traffic(kernel_of(dom, args.vcycle), npan)
     if tr is not None:
         roof["traffic"] = tr
-        roof["traffic_source"] = "profiles/cfg2_build_pmc_traffic.md"
+        roof["traffic_source"] = "profiles/glr_build_grid2d_1024_pmc_traffic.md"
     sp = prof["spmm_B"]
     spmm_roof = None
     if sp[0]:
@@ -604,7 +605,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "cfg2: 2D %dx%d grid n=%d, anisotropic PSF B "
+        "config": {"workload": "2D %dx%d grid n=%d, anisotropic PSF B "
                                "(%.1f nnz/row), Z=M+0.05K, ell=%d q=1, w=64, "
                                "Z^-1 = Chebyshev(%d) around AMG V-cycle"
                                % (args.side, args.side, n, Bs.nnz / n, args.ell,

@@ -191,7 +191,7 @@ def _ball_offsets(radius_cells, dim):
     return g[(g ** 2).sum(1) <= radius_cells ** 2 + 1e-9]
 
 
-def cfg1_problem(side=128):
+def psf2d_problem(side=128):
     """2D, Z = 1.0 M + 0.1 K, B: symmetrized Gaussian PSF
     a(x) exp(-|x-y|^2 / (2 (2h)^2)), radius 4h, symmetrized as
     (a(x_i) + a(x_j)) / 2, a(x) = 1 + 0.5 sin(2 pi x1) cos(2 pi x2)."""
@@ -208,7 +208,7 @@ def cfg1_problem(side=128):
     return B, Z, M
 
 
-def cfg2_problem(side=1024, dtype=np.float64):
+def aniso_psf2d_problem(side=1024, dtype=np.float64):
     """2D anisotropic PSF: Sigma(x) = R(pi x1) diag((3h)^2, (1.5h)^2) R^T,
     truncated at Mahalanobis distance 3 (of the row's own covariance),
     symmetrized by averaging the two rows' values.  Z = M + 0.05 K."""
@@ -234,7 +234,7 @@ def cfg2_problem(side=1024, dtype=np.float64):
     return B, Z, M
 
 
-def cfg3_problem(side=128):
+def psf3d_problem(side=128):
     """3D isotropic PSF sigma = 1.5h truncated to a radius-2h ball, Z =
     M + 0.1 K7 (unscaled 7-point); B's values are FP32 numbers (the
     configuration's single-precision operator), held in a float64 CSR."""
@@ -248,7 +248,7 @@ def cfg3_problem(side=128):
     return B.astype(np.float64), Z, M
 
 
-def cfg5_instance(k, seed=5):
+def small_psf2d_instance(k, seed=5):
     """Instance k of the batched configuration (256 small 2D problems):
     side s = round(exp(U(ln 64, ln 256))) (n = s^2 in [4096, 65536]), h =
     1/(s-1), PSF width sigma ~ U[1.5h, 3h] truncated at radius 3 sigma,

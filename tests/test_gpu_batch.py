@@ -20,7 +20,7 @@ def dev(lgh):
 def problems():
     out = []
     for k, side in enumerate((16, 20, 24, 28, 22, 18)):
-        Bs, Zs, M = W.cfg1_problem(side)
+        Bs, Zs, M = W.psf2d_problem(side)
         out.append((Bs, Zs, M, 12 + 4 * k))
     return out
 

@@ -178,7 +178,7 @@ def test_grid_correction_reduces_error(dev):
     (n = 1024): the priced values d lie inside the spectrum of the whitened
     error E_w = S^- (F_true - U lam U^T) S^- (Rayleigh bounds), and the
     correction lowers |E_w|_F measured in the same S^- metric."""
-    Bs, Zs, M = W.cfg1_problem(32)
+    Bs, Zs, M = W.psf2d_problem(32)
     n = Zs.shape[0]
     c0 = 1e-2
     prior = dev.Prior.from_mat(Zs, M, blocked_mode=3)

@@ -172,10 +172,10 @@ def test_prior_mat_e2e(dev):
 
 @pytest.fixture(scope="module")
 def grid(dev, oracle):
-    """32 x 32 grid, cfg-1 style B and Z, mode-3 prior; oracle built on the
+    """32 x 32 grid, the 2D isotropic PSF B and Z = M + 0.1 K, mode-3 prior; oracle built on the
     engine's own hierarchy (same Chebyshev-MG operator)."""
     side = 32
-    Bs, Zs, M = W.cfg1_problem(side)
+    Bs, Zs, M = W.psf2d_problem(side)
     prior = dev.Prior.from_mat(Zs, M, blocked_mode=3)
     info = prior.info()
     h = dev.AMG(Zs)
@@ -284,12 +284,12 @@ def test_grid_extend_incremental(dev, grid, oracle):
 
 
 @pytest.mark.parametrize("mixed", [False, True])
-def test_cfg3_shape_3d_extend_twice(dev, oracle, mixed):
+def test_grid3d_16_extend_twice(dev, oracle, mixed):
     """The 3D configuration's shape at 16^3 (n = 4096): Z = M + 0.1 K7
     (7-point, AMG tier), isotropic PSF B (sigma 1.5h, radius-2h ball,
     33 nnz/row), ell = 40 then lgh_glr_extend by 20 twice; engine vs the
     oracle's incremental extension on the same hierarchy."""
-    Bs, Zs, M = W.cfg3_problem(16)
+    Bs, Zs, M = W.psf3d_problem(16)
     assert abs(Bs.nnz / Bs.shape[0] - 33) < 8
     prior = dev.Prior.from_mat(Zs, M, blocked_mode=3)
     info = prior.info()
@@ -318,13 +318,13 @@ def test_cfg3_shape_3d_extend_twice(dev, oracle, mixed):
     assert rel(g.solve(c, rhs), og.solve(c, rhs)) < (1e-6 if mixed else 1e-9)
 
 
-def test_cfg1_full_size_matches_oracle(dev, oracle):
+def test_grid2d_128_full_size_matches_oracle(dev, oracle):
     """The CPU-runnable benchmark case at full size: 128 x 128 grid
     (n = 16 384), Z = M + 0.1 K, symmetrized Gaussian PSF B (radius 4h),
     ell = 64 + 10, seed 0, FP64; one solve and one sample at c = 1e-2 with
     N(0,1) right-hand sides.  Engine vs the FP64 oracle on the same AMG
     hierarchy; the mixed build vs the plain one."""
-    Bs, Zs, M = W.cfg1_problem(128)
+    Bs, Zs, M = W.psf2d_problem(128)
     n = Zs.shape[0]
     prior = dev.Prior.from_mat(Zs, M, blocked_mode=3)
     info = prior.info()

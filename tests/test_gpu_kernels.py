@@ -53,11 +53,11 @@ def test_spmm(dev, Wd):
 
 @pytest.mark.parametrize("prec_vals", [0, 1])
 def test_spmm_wide_rows_width_invariant(dev, prec_vals):
-    """A PSF operator with wide rows (cfg-1 style, up to 49 entries a row):
+    """A PSF operator with wide rows (the 2D isotropic PSF, up to 49 entries a row):
     the 64-wide row kernel (grouped neighbour loads) is bit-identical to
     the generic 32-wide kernel (the same sums in the same order) and
     matches scipy."""
-    B, _, _ = W.cfg1_problem(45)
+    B, _, _ = W.psf2d_problem(45)
     assert B.nnz >= 32 * B.shape[0]
     X = np.random.default_rng(5).standard_normal((B.shape[0], 150))
     y64 = dev.spmm(B, X, prec_vals=prec_vals, W=64)

@@ -25,7 +25,7 @@ def dev(lgh):
 
 @pytest.fixture(scope="module")
 def prob(dev):
-    Bs, Zs, M = W.cfg1_problem(32)
+    Bs, Zs, M = W.psf2d_problem(32)
     prior = dev.Prior.from_mat(Zs, M, blocked_mode=3)
     return dict(B=Bs, Z=Zs, M=M, prior=prior, n=Zs.shape[0])
 

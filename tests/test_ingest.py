@@ -26,7 +26,7 @@ def _rows_from(A, rng, dup=False):
 
 
 def test_rows_to_mat_sorted_and_summed(lgh):
-    B, _, _ = W.cfg1_problem(16)
+    B, _, _ = W.psf2d_problem(16)
     rng = np.random.default_rng(0)
     rp, cg, v = _rows_from(B, rng, dup=True)
     M = lgh.Mat.from_fit_rows(rp, cg, v, sym_tol=1e-14)
@@ -39,7 +39,7 @@ def test_rows_to_mat_sorted_and_summed(lgh):
 
 
 def test_rows_refusals(lgh):
-    B, _, _ = W.cfg1_problem(8)
+    B, _, _ = W.psf2d_problem(8)
     rng = np.random.default_rng(1)
     rp, cg, v = _rows_from(B, rng)
     bad = cg.copy()
