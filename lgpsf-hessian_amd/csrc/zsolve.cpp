@@ -256,19 +256,23 @@ split_tail (const PMat &B, PMat &X, WK &w, bool *done, F &&run)
 {
   *done = false;
   const int full = B.ncols / 64, tail = B.ncols - 64 * full;
-  if (B.W != 64 || tail == 0 || tail > 16) return 0;
+  if (B.W != 64 || tail == 0 || tail > 48) return 0;
   LGH_CHECK (X.W == 64 && X.ncols == B.ncols && X.prec == B.prec, 8,
              "Z-solve: X shape differs from B");
   *done = true;
   if (!w.tail) w.tail = new WK ();
   if (!w.Bt.p || w.Bt.nrows != B.nrows || w.Bt.prec != B.prec) {
     pmat_free (w.Bt); pmat_free (w.Xt);
-    LGH_TRY (pmat_alloc (w.Bt, B.nrows, 16, 16, B.prec));
-    LGH_TRY (pmat_alloc (w.Xt, B.nrows, 16, 16, B.prec));
+    LGH_TRY (pmat_alloc (w.Bt, B.nrows, 48, 16, B.prec));
+    LGH_TRY (pmat_alloc (w.Xt, B.nrows, 48, 16, B.prec));
   }
-  // Bt's columns past `tail` are zero padding (so are Xt's after the solve)
-  LGH_TRY (dzero (w.Bt.p, w.Bt.bytes ()));
-  LGH_TRY (k_copy_cols (B, 64 * full, w.Bt, 0, tail));
+  // the tail's columns as ceil(tail / 16) 16-wide panels (a prefix of the
+  // 48-column buffers); Bt's columns past `tail` are zero padding (so are
+  // Xt's after the solve)
+  PMat Bt = w.Bt, Xt = w.Xt;
+  Bt.ncols = Xt.ncols = tail;
+  LGH_TRY (dzero (Bt.p, Bt.bytes ()));
+  LGH_TRY (k_copy_cols (B, 64 * full, Bt, 0, tail));
   // the narrow panel's solve runs on the auxiliary stream, overlapping the
   // whole panels' (their coarse levels are latency-bound and leave the
   // device idle); it touches only its own buffers (tail work, the V-cycle's
@@ -284,11 +288,11 @@ split_tail (const PMat &B, PMat &X, WK &w, bool *done, F &&run)
   int rc = 0;
   if (aux) {
     StreamScope on_aux (aux);
-    rc = run (w.Bt, w.Xt, *w.tail);
+    rc = run (Bt, Xt, *w.tail);
     if (!rc && hipEventRecord (done_ev, aux) != hipSuccess)
       rc = hip_fail (hipGetLastError (), "hipEventRecord", __FILE__, __LINE__);
   }
-  else rc = run (w.Bt, w.Xt, *w.tail);   // on the main stream, in order
+  else rc = run (Bt, Xt, *w.tail);       // on the main stream, in order
   if (!rc && full > 0) {                 // the whole panels, in place
     PMat Bf = B, Xf = X;
     Bf.ncols = Xf.ncols = 64 * full;
@@ -303,7 +307,7 @@ split_tail (const PMat &B, PMat &X, WK &w, bool *done, F &&run)
   if (done_ev) (void) hipEventDestroy (done_ev);
   LGH_TRY (rc);
   LGH_TRY (dzero (X.panel (full), (size_t) X.nrows * 64 * prec_bytes (X.prec)));
-  return k_copy_cols (w.Xt, 0, X, 64 * full, tail);
+  return k_copy_cols (Xt, 0, X, 64 * full, tail);
 }
 
 int

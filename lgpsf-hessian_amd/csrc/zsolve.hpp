@@ -37,17 +37,17 @@ struct ChebWork {
   int W = 0, np = -1, prec = -1, nrows = 0;
   PMat R[2], D[2], Z;
   PMat Rpc, Zpc;                       // lower-precision V-cycle in / out
-  PMat Bt, Xt;                         // the narrow last panel's B and X
+  PMat Bt, Xt;                         // the narrow panels' B and X (48 columns)
   ChebWork *tail = nullptr;            // (see ChebMixedWork)
   ~ChebWork ();
   int ensure (const PMat &like);
 };
 
 // Both Chebyshev solves below run a block whose last 64-column panel holds
-// at most 16 columns as its whole panels plus a separate 16-wide panel
-// (tail: its own work and copies of those columns), so the 48 padding
-// columns cost nothing; every column sees the same arithmetic at either
-// width.
+// at most 48 columns as its whole panels plus up to three 16-wide panels
+// (tail: its own work and copies of those columns, solved on the thread's
+// auxiliary stream beside the whole panels), so the padding columns cost
+// nothing; every column sees the same arithmetic at either width.
 //
 // X = Chebyshev(nit) solve of A X = B, zero initial guess
 // (zsolve_impl.h:104-141 with the blocked-asolve contract of :629-633);
@@ -62,7 +62,7 @@ int cheb_solve (const DevCSR &A, DevMG *mg, double lmin, double lmax,
 struct ChebMixedWork {
   int W = 0, np = -1, nrows = 0, pprec = -1;
   PMat R[2], D[2], Rpc, Zpc;
-  PMat Bt, Xt;                         // the narrow panel's B and X
+  PMat Bt, Xt;                         // the narrow panels' B and X (48 columns)
   ChebMixedWork *tail = nullptr;
   ~ChebMixedWork ();
   int ensure (const PMat &like, int pc);
