@@ -485,13 +485,17 @@ cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
 // writes R in SP for the next preconditioner application.  (A BF16
 // direction saves 4 of ~44 B per element but its 2^-9 rounding, repeated
 // over the 25 steps, costs the solve a factor ~2 in accuracy.)
-template <bool HASD, typename SP, int W = 64>
+// RI, RO: storage of the input residual and of X and the new residual
+// (double; float in the early phase of a solve, see cheb_mixed_run), the
+// arithmetic is FP64 either way.
+template <bool HASD, typename SP, int W = 64, typename RI = double,
+          typename RO = double>
 __global__ void __launch_bounds__ (TPB)
 cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
            const double *__restrict__ v, const SP *__restrict__ Zs,
            double a, double b, const float *__restrict__ D,
-           const double *__restrict__ R, double *__restrict__ X, int xmode,
-           float *__restrict__ Dn, double *__restrict__ Rn,
+           const RI *__restrict__ R, RO *__restrict__ X, int xmode,
+           float *__restrict__ Dn, RO *__restrict__ Rn,
            SP *__restrict__ Rpc)
 {
   // 16 lanes per row, 4 values per lane (fewer, wider loads than the fp64
@@ -578,47 +582,59 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                    PMat &Dn32, PMat &Rn, PMat &Rnpc)
 {
   const int W = R.W;
-  LGH_CHECK (A.prec == F64 && R.prec == F64 && X.prec == F64 && row_width (W)
+  LGH_CHECK (A.prec == F64 && (R.prec == F64 || R.prec == F32)
+             && (X.prec == F64 || X.prec == F32) && Rn.prec == X.prec
+             && (R.prec == X.prec || R.prec == F64) && row_width (W)
              && X.W == W && Zpc.W == W && Dn32.W == W && Rn.W == W
              && Rnpc.W == W && (!D32 || D32->W == W)
              && (Zpc.prec == F32 || Zpc.prec == BF16) && Rnpc.prec == Zpc.prec
              && Dn32.prec == F32 && (!D32 || D32->prec == F32)
-             && Rn.prec == F64 && R.nrows == A.n, 8,
+             && R.nrows == A.n, 8,
              "k_cheb_step_mixed: shape/precision mismatch");
   LGH_CHECK (x_mode >= MIX_X_SET && x_mode <= MIX_X_ADD2
              && (D32 || (x_mode != MIX_X_SET2 && x_mode != MIX_X_ADD2)), 8,
              "k_cheb_step_mixed: bad X mode");
   const double vec = (double) A.n * W;
+  const double rb = (double) prec_bytes (R.prec), ob = (double) prec_bytes (X.prec);
   const double xb = (x_mode == MIX_X_SKIP) ? 0 : (x_mode == MIX_X_SET
-                     || x_mode == MIX_X_SET2) ? 8 : 16;
+                     || x_mode == MIX_X_SET2) ? ob : 2 * ob;
   const double pb = (double) prec_bytes (Zpc.prec);
-  // the first step of a solve (no D) is credited to PK_OTHER so that
-  // PK_CHEB_MIXED times one kernel instantiation
-  ProfScope ps (D32 ? PK_CHEB_MIXED : PK_OTHER, R.npanels () * (csr_bytes (A)
-                + vec * (pb + (D32 ? 4 : 0) + 8 + xb + 4 + 8 + pb)),
+  // the first step of a solve (no D) and the early-phase steps (FP32 R, X)
+  // are credited to PK_OTHER so that PK_CHEB_MIXED times one kernel
+  // instantiation
+  ProfScope ps ((D32 && X.prec == F64) ? PK_CHEB_MIXED : PK_OTHER,
+                R.npanels () * (csr_bytes (A)
+                + vec * (pb + (D32 ? 4 : 0) + rb + xb + 4 + ob + pb)),
                 R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * W,
                 R.npanels (), Zpc.prec, W);
   const dim3 g = grid64 (A.n, R.npanels (), F32, W);   // 4 columns per lane
-  auto go = [&] (auto sp, auto wc) -> int {
+  auto go = [&] (auto sp, auto wc, auto ri, auto ro) -> int {
     using SP = decltype (sp);
+    using RI = decltype (ri);
+    using RO = decltype (ro);
     constexpr int WW = decltype (wc)::value;
     if (D32)
-      hipLaunchKernelGGL ((cheb64m_k<true, SP, WW>), g, dim3 (TPB), 0, stream (),
-                          A.n, A.rp, A.ci, (const double *) A.v,
+      hipLaunchKernelGGL ((cheb64m_k<true, SP, WW, RI, RO>), g, dim3 (TPB), 0,
+                          stream (), A.n, A.rp, A.ci, (const double *) A.v,
                           (const SP *) Zpc.p, a, b, (const float *) D32->p,
-                          (const double *) R.p, (double *) X.p, x_mode,
-                          (float *) Dn32.p, (double *) Rn.p, (SP *) Rnpc.p);
+                          (const RI *) R.p, (RO *) X.p, x_mode,
+                          (float *) Dn32.p, (RO *) Rn.p, (SP *) Rnpc.p);
     else
-      hipLaunchKernelGGL ((cheb64m_k<false, SP, WW>), g, dim3 (TPB), 0, stream (),
-                          A.n, A.rp, A.ci, (const double *) A.v,
+      hipLaunchKernelGGL ((cheb64m_k<false, SP, WW, RI, RO>), g, dim3 (TPB), 0,
+                          stream (), A.n, A.rp, A.ci, (const double *) A.v,
                           (const SP *) Zpc.p, a, b, nullptr,
-                          (const double *) R.p, (double *) X.p, x_mode,
-                          (float *) Dn32.p, (double *) Rn.p, (SP *) Rnpc.p);
+                          (const RI *) R.p, (RO *) X.p, x_mode,
+                          (float *) Dn32.p, (RO *) Rn.p, (SP *) Rnpc.p);
     LGH_LAUNCH_CHECK ();
     return 0;
   };
   return with_row_W (W, [&] (auto wc) {
-    return Zpc.prec == BF16 ? go (bf16_t (), wc) : go (0.f, wc);
+    auto by_r = [&] (auto sp) {
+      if (X.prec == F64) return go (sp, wc, 0.0, 0.0);
+      if (R.prec == F64) return go (sp, wc, 0.0, 0.f);
+      return go (sp, wc, 0.f, 0.f);
+    };
+    return Zpc.prec == BF16 ? by_r (bf16_t ()) : by_r (0.f);
   });
 }
 

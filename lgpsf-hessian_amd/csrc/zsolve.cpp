@@ -389,6 +389,7 @@ ChebMixedWork::~ChebMixedWork ()
 {
   pmat_free (R[0]); pmat_free (R[1]); pmat_free (D[0]); pmat_free (D[1]);
   pmat_free (Rpc); pmat_free (Zpc); pmat_free (Bt); pmat_free (Xt);
+  pmat_free (R32[0]); pmat_free (R32[1]); pmat_free (X32);
   delete tail;
 }
 
@@ -399,6 +400,7 @@ ChebMixedWork::ensure (const PMat &like, int pc)
       && pprec == pc) return 0;
   pmat_free (R[0]); pmat_free (R[1]); pmat_free (D[0]); pmat_free (D[1]);
   pmat_free (Rpc); pmat_free (Zpc);
+  pmat_free (R32[0]); pmat_free (R32[1]); pmat_free (X32);
   const int nc = like.npanels () * like.W;
   LGH_TRY (pmat_alloc (R[0], like.nrows, nc, like.W, F64));
   LGH_TRY (pmat_alloc (R[1], like.nrows, nc, like.W, F64));
@@ -406,6 +408,9 @@ ChebMixedWork::ensure (const PMat &like, int pc)
   LGH_TRY (pmat_alloc (D[1], like.nrows, nc, like.W, F32));
   LGH_TRY (pmat_alloc (Rpc, like.nrows, nc, like.W, pc));
   LGH_TRY (pmat_alloc (Zpc, like.nrows, nc, like.W, pc));
+  LGH_TRY (pmat_alloc (R32[0], like.nrows, nc, like.W, F32));
+  LGH_TRY (pmat_alloc (R32[1], like.nrows, nc, like.W, F32));
+  LGH_TRY (pmat_alloc (X32, like.nrows, nc, like.W, F32));
   W = like.W;
   np = like.npanels ();
   nrows = like.nrows;
@@ -420,6 +425,9 @@ cheb_mixed_run (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
   LGH_TRY (w.ensure (B, mg.prec));
   ChebCoef cc (lmin, lmax);
   LGH_TRY (k_copy_cols (B, 0, w.Rpc, 0, B.npanels () * B.W));
+  // the early phase: steps 1..k_sw with FP32 R and X (k_sw even, so that X
+  // holds every direction when it ends)
+  const int k_sw = (nit >= 8) ? 2 * (nit / 4) : 0;
   const PMat *Rsrc = &B;
   int ri = 0, di = 0;
   for (int k = 1; k <= nit; k++) {
@@ -432,12 +440,24 @@ cheb_mixed_run (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
     if (k % 2 == 1 && k < nit) xm = MIX_X_SKIP;
     else if (k % 2 == 0) xm = (k == 2) ? MIX_X_SET2 : MIX_X_ADD2;
     else xm = (k == 1) ? MIX_X_SET : MIX_X_ADD;
+    const bool early = k <= k_sw;
+    PMat &Xk = early ? w.X32 : X;
+    PMat &Rn = early ? w.R32[ri] : w.R[ri];
     LGH_TRY (mg_vcycle (mg, w.Rpc, w.Zpc));
-    LGH_TRY (k_cheb_step_mixed (A64, w.Zpc, a, b, Dcur, *Rsrc, X, xm,
-                                w.D[di], w.R[ri], w.Rpc));
-    Rsrc = &w.R[ri];
+    LGH_TRY (k_cheb_step_mixed (A64, w.Zpc, a, b, Dcur, *Rsrc, Xk, xm,
+                                w.D[di], Rn, w.Rpc));
+    Rsrc = &Rn;
     ri ^= 1;
     di ^= 1;
+    if (k == k_sw) {
+      // residual replacement: X in FP64, R = B - A X in FP64, and the next
+      // preconditioner input from it
+      LGH_TRY (k_copy_cols (w.X32, 0, X, 0, X.npanels () * X.W));
+      LGH_TRY (k_resid (A64, B, X, w.R[ri]));
+      LGH_TRY (k_copy_cols (w.R[ri], 0, w.Rpc, 0, B.npanels () * B.W));
+      Rsrc = &w.R[ri];
+      ri ^= 1;
+    }
   }
   return 0;
 }

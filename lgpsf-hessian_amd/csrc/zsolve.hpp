@@ -58,10 +58,14 @@ int cheb_solve (const DevCSR &A, DevMG *mg, double lmin, double lmax,
 
 // FP64-accurate Chebyshev around an FP32 or BF16 V-cycle (W = 64 panels):
 // X, R in FP64; D in FP32; the V-cycle's input and output (Rpc, Zpc) in
-// its own precision (k_cheb_step_mixed)
+// its own precision (k_cheb_step_mixed).  The first steps (an even count,
+// about half) keep R and X in FP32; then X is widened, the residual
+// B - A X is formed again in FP64 and the remaining steps (which damp the
+// early rounding by their convergence factor) run with FP64 R and X.
 struct ChebMixedWork {
   int W = 0, np = -1, nrows = 0, pprec = -1;
   PMat R[2], D[2], Rpc, Zpc;
+  PMat R32[2], X32;                    // the early phase's FP32 R and X
   PMat Bt, Xt;                         // the narrow panels' B and X (48 columns)
   ChebMixedWork *tail = nullptr;
   ~ChebMixedWork ();
