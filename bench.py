@@ -220,7 +220,7 @@ def grid3d_bench(args, W, lgh):
     lgh.lib().lgh_hip_sync()
     t_build = (time.perf_counter() - t0) / args.steps
     lgh.prof_enable(False)
-    npan = ell0 // 64 if 0 < ell0 % 64 <= 16 else (ell0 + 63) // 64
+    npan = ell0 // 64 if 0 < ell0 % 64 <= 48 else (ell0 + 63) // 64
     vpc = pcode[args.vcycle]
     single = ("cheb_mixed", "smooth_pre_fine", "smooth_post_fine", "resid_fine")
     prof = {k: lgh.prof_collect(k, npan, vpc) if k in single else lgh.prof_collect(k)
@@ -463,8 +463,8 @@ def main():
     lgh.prof_enable(False)
     # one kernel instantiation at one launch shape per kind below: the
     # full-width launches on the finest level (all whole 64-column panels;
-    # a last panel of at most 16 columns runs as its own 16-wide panel)
-    npan = args.ell // 64 if 0 < args.ell % 64 <= 16 else (args.ell + 63) // 64
+    # a last panel of at most 48 columns runs as 16-wide panels)
+    npan = args.ell // 64 if 0 < args.ell % 64 <= 48 else (args.ell + 63) // 64
     vpc = pcode[args.vcycle]
     single = ("cheb_mixed", "smooth_pre_fine", "smooth_post_fine", "resid_fine")
     prof = {k: lgh.prof_collect(k, npan, vpc) if k in single else lgh.prof_collect(k)
