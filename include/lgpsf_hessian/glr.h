@@ -24,9 +24,15 @@ typedef struct lgh_glr lgh_glr_t;
 
 typedef enum
 {
-  LGH_GLR_DEFAULT    = 0,
-  LGH_GLR_REPLICATED = 1,
-  LGH_GLR_SCALAPACK  = 2    /* not in this build: PETSC_ERR_SUP */
+  LGH_GLR_DEFAULT    = 0,   /* resolves to REPLICATED                     */
+  LGH_GLR_REPLICATED = 1,   /* explicit U = Q C (n x kept) in HBM          */
+  LGH_GLR_SCALAPACK  = 2    /* the reference's at-scale backend at
+                               communicator size 1 (a 1 x 1 process grid):
+                               implicit U = Q V (Q n x ell, V ell x kept;
+                               U never formed), BCGS2 orthonormalization,
+                               check = eigenvalue cross-check of two
+                               eigensolvers instead of the residual sweep
+                               (glr_scalapack_impl.h:505-688) */
 }
 lgh_glr_backend_t;
 

@@ -43,6 +43,12 @@ typedef struct lgh_glr_xopts
                                V-cycle, 64-column panels) for their Z
                                solves instead of per-column PCG to
                                ksp_rtol; 0 = never (the reference path)    */
+  int sweep_panels;         /* 64-column panels per F sweep chunk (the
+                               Z-solve work of a sweep grows with the panels
+                               solved together); 0 = all panels at once
+                               unless that would not fit in device memory
+                               beside the sketch blocks.  Results do not
+                               depend on it (panels are independent)       */
 }
 lgh_glr_xopts_t;
 

@@ -68,7 +68,7 @@ class GlrReport(C.Structure):
 class GlrXOpts(C.Structure):
     _fields_ = [("prec_sketch", C.c_int), ("prec_precond", C.c_int),
                 ("prec_check", C.c_int), ("prec_sketch_precond", C.c_int),
-                ("solve_blocked_min", C.c_int)]
+                ("solve_blocked_min", C.c_int), ("sweep_panels", C.c_int)]
 
 
 class ServeOpts(C.Structure):

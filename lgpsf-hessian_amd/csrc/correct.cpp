@@ -258,6 +258,9 @@ int
 graft (lgh_glr_t *g, const Blk &Vhat, const std::vector<double> &d,
        const lgh_glr_correct_opts_t &o, lgh_glr_correct_report_t *rep)
 {
+  // an implicit-U object gets its U = Q V formed first
+  // (correct_impl.h:338-345); the sketch state is dropped below anyway
+  LGH_TRY (glr_materialize_U (g));
   const int n = g->n, r = Vhat.m, k0 = g->kept;
   std::vector<double> Gv;
   LGH_TRY (xty (Vhat, Vhat, Gv));

@@ -416,4 +416,45 @@ sol_syevd (int n, double *A, int lda, double *w_dev, int *info_host)
   return rc;
 }
 
+int
+sol_syev_vals (int n, double *A, int lda, double *w_dev, int *info_host)
+{
+  double *E = (double *) dmalloc (sizeof (double) * (size_t) (n > 0 ? n : 1));
+  if (!E) return 5;
+  rocblas_status s = rocsolver_dsyev (t_ctx.rb, rocblas_evect_none,
+                                      rocblas_fill_upper, n, A, lda, w_dev, E,
+                                      t_ctx.info);
+  int rc = 0;
+  if (s != rocblas_status_success) { set_error ("rocsolver_dsyev failed"); rc = 6; }
+  if (!rc) rc = d2h (info_host, t_ctx.info, sizeof (int));
+  dfree (E);
+  return rc;
+}
+
+int
+blas_dgemm (int ta, int tb, int m, int n, int k, double alpha, const double *A,
+            int lda, const double *B, int ldb, double beta, double *C, int ldc)
+{
+  if (m == 0 || n == 0) return 0;
+  (void) stream ();
+  rocblas_status s = rocblas_dgemm (
+      t_ctx.rb, ta ? rocblas_operation_transpose : rocblas_operation_none,
+      tb ? rocblas_operation_transpose : rocblas_operation_none, m, n, k,
+      &alpha, A, lda, B, ldb, &beta, C, ldc);
+  LGH_CHECK (s == rocblas_status_success, 6, "rocblas_dgemm failed");
+  return 0;
+}
+
+size_t
+dmem_total ()
+{
+  static size_t total = 0;
+  if (!total) {
+    size_t fr = 0, tot = 0;
+    (void) stream ();
+    if (hipMemGetInfo (&fr, &tot) == hipSuccess) total = tot;
+  }
+  return total;
+}
+
 }  // namespace lgh

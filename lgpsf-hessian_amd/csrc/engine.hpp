@@ -38,6 +38,12 @@ struct lgh_glr {
   lgh::PMat U;                          // n x kept, fp64, W = 64
   std::vector<double> lam;
   int kept = 0;
+  // implicit representation (backend LGH_GLR_SCALAPACK): U = Q V with V the
+  // kept Ritz vectors (ell x kept, column-major) on the host and, zero-
+  // padded to 64-multiples (ld = npanels(ell) * 64), on the device; U empty
+  int implicit = 0;
+  std::vector<double> V;
+  double *dV = nullptr;
   // incremental-extension state: orthonormal Q (n x ell) and raw Q^T F Q
   lgh::PMat Q;
   std::vector<double> Araw;
@@ -65,4 +71,7 @@ int glr_run_block (lgh_glr_t *g, int op, int m, const double *c,
 // (the same filter for every column)
 int glr_filter_panels (lgh_glr_t *g, const double *f_lam, double f_c,
                        const PMat &X, int m, PMat &Y);
+// an implicit object's U = Q V formed in HBM (the object becomes explicit;
+// no-op for an explicit one)
+int glr_materialize_U (lgh_glr_t *g);
 }

@@ -43,6 +43,7 @@ lgh_glr_xopts_default (void)
   x.prec_check = LGH_PREC_F64;
   x.prec_sketch_precond = LGH_PREC_F64;
   x.solve_blocked_min = 0;
+  x.sweep_panels = 0;
   return x;
 }
 
@@ -55,6 +56,7 @@ lgh_glr_xopts_mixed (void)
   x.prec_check = LGH_PREC_F32;
   x.prec_sketch_precond = LGH_PREC_F32;
   x.solve_blocked_min = 8;
+  x.sweep_panels = 0;
   return x;
 }
 
@@ -235,13 +237,31 @@ select_modes (lgh_glr_t *g, int ell, const std::vector<double> &ev,
   g->kept = kept;
 }
 
+// V (ell x kept) to the device, zero-padded to 64-multiples
+static int
+upload_V (lgh_glr_t *g, int ell)
+{
+  const int k = g->kept, ldl = npan (ell) * 64, ldk = npan (std::max (k, 1)) * 64;
+  dfree (g->dV);
+  g->dV = (double *) dmalloc (sizeof (double) * (size_t) ldl * ldk);
+  if (!g->dV) return 5;
+  std::vector<double> h ((size_t) ldl * ldk, 0.0);
+  for (int j = 0; j < k; j++)
+    for (int i = 0; i < ell; i++) h[i + (size_t) j * ldl] = g->V[i + (size_t) j * ell];
+  return h2d (g->dV, h.data (), sizeof (double) * h.size ());
+}
+
 // symmetrized eig of the raw projection A (ell x ell, host), selection,
-// U = [Qa | Qb] C.  Qb (may be null) holds columns ell_a..ell-1.
+// U = [Qa | Qb] C.  Qb (may be null) holds columns ell_a..ell-1.  An
+// implicit object keeps V = C instead of forming U.  xcheck: the
+// implicit-U backend's check (glr_scalapack_impl.h:633-670): the
+// eigenvalues again by a second method (values-only QR iteration), and
+// rep->resid_max = max |dlam| / max |lam| between the two
 static int
 rayleigh_ritz (lgh_glr_t *g, int ell, const std::vector<double> &Araw,
                const PMat &Qa, int ell_a, const PMat *Qb,
                lgh_glr_report_t *rep, std::vector<double> &evals,
-               std::vector<int> &order, double *t_dense)
+               std::vector<int> &order, double *t_dense, int xcheck = 0)
 {
   std::vector<double> T ((size_t) ell * ell);
   for (int j = 0; j < ell; j++)
@@ -257,7 +277,16 @@ rayleigh_ritz (lgh_glr_t *g, int ell, const std::vector<double> &Araw,
   double *dw = (double *) dmalloc (sizeof (double) * (size_t) ell);
   if (!dT || !dw) { dfree (dT); dfree (dw); return 5; }
   int info = 0;
-  int rc = h2d (dT, T.data (), sizeof (double) * (size_t) ell * ell);
+  std::vector<double> evr;
+  int rc = 0;
+  if (xcheck) {                          // values-only pass on a copy
+    evr.resize (ell);
+    rc = h2d (dT, T.data (), sizeof (double) * (size_t) ell * ell);
+    if (!rc) rc = sol_syev_vals (ell, dT, ell, dw, &info);
+    if (!rc && info != 0) { set_error ("syev failed"); rc = 12; }
+    if (!rc) rc = d2h (evr.data (), dw, sizeof (double) * ell);
+  }
+  if (!rc) rc = h2d (dT, T.data (), sizeof (double) * (size_t) ell * ell);
   if (!rc) rc = sol_syevd (ell, dT, ell, dw, &info);
   if (!rc && info != 0) { set_error ("syevd failed"); rc = 12; }
   evals.resize (ell);
@@ -266,8 +295,24 @@ rayleigh_ritz (lgh_glr_t *g, int ell, const std::vector<double> &Araw,
   dfree (dT); dfree (dw);
   *t_dense += MPI_Wtime () - t0;
   LGH_TRY (rc);
+  if (xcheck) {
+    double dmax = 0., amax = 0.;
+    for (int i = 0; i < ell; i++) {
+      dmax = std::max (dmax, std::fabs (evr[i] - evals[i]));
+      amax = std::max (amax, std::fabs (evr[i]));
+    }
+    rep->resid_max = dmax / std::max (amax, 1e-300);
+  }
   select_modes (g, ell, evals, rep, order);
   const int k = g->kept;
+  if (g->implicit) {
+    pmat_free (g->U);
+    g->V.assign ((size_t) ell * k, 0.0);
+    for (int j = 0; j < k; j++)
+      for (int i = 0; i < ell; i++)
+        g->V[i + (size_t) j * ell] = T[i + (size_t) order[j] * ell];
+    return upload_V (g, ell);
+  }
   pmat_free (g->U);
   LGH_TRY (pmat_alloc (g->U, g->n, std::max (k, 1), 64, F64));
   if (k == 0) return 0;
@@ -346,31 +391,82 @@ timed_F (lgh_glr_t *g, const PMat &X, PMat &Y, PMat &Wk, double *t_op,
   return 0;
 }
 
+// panels q0 .. q0 + np - 1 of X as a matrix of its own (a view: never freed)
+static PMat
+pview (const PMat &X, int q0, int np)
+{
+  PMat v = X;
+  v.p = X.panel (q0);
+  v.ncols = std::min (X.ncols - X.W * q0, X.W * np);
+  return v;
+}
+
+// panels solved together in one F sweep chunk: xopts.sweep_panels, or all
+// P unless their work (the Z-solve buffers, about 64 B per element, and
+// the sweep's scratch) would not fit in 85 % of device memory beside the
+// blocks a build keeps (Q, Y and, for CholQR2, its copy: 24 B per element;
+// 16 B with BCGS2)
+static int
+sweep_chunk (const lgh_glr_t *g, int P)
+{
+  if (g->x.sweep_panels > 0) return std::min (g->x.sweep_panels, P);
+  const double total = (double) dmem_total ();
+  if (total <= 0.0) return P;
+  const double el = (double) g->n * 64.0;        // elements per panel
+  const double keep = (g->implicit ? 16.0 : 24.0) * el * npan (g->ell > 0 ? g->ell : g->opts.ell);
+  const double avail = 0.85 * total - keep, per_panel = 64.0 * el;
+  if (avail >= per_panel * P) return P;
+  return std::max (1, (int) (avail / per_panel));
+}
+
+// Y = F X, the panels of X in chunks (sweep_chunk); panels are independent,
+// so the result does not depend on the chunking
+static int
+sweep_F (lgh_glr_t *g, const PMat &X, PMat &Y, double *t_op, int pcp = -1)
+{
+  const int P = X.npanels (), cp = sweep_chunk (g, P);
+  PMat Wk;
+  LGH_TRY (pmat_alloc (Wk, X.nrows, std::min (X.ncols, 64 * cp), 64, X.prec));
+  int rc = 0;
+  for (int q0 = 0; q0 < P && !rc; q0 += cp) {
+    PMat Yc = pview (Y, q0, cp), Wc = Wk;
+    const PMat Xc = pview (X, q0, cp);
+    Wc.ncols = Xc.ncols;
+    rc = timed_F (g, Xc, Yc, Wc, t_op, pcp);
+  }
+  pmat_free (Wk);
+  return rc;
+}
+
 // sketch sweep Y64 = F Omega(seed, columns col0..col0+nc-1) at precision
 // ps (FP32: vectors, B and Z in FP32; the V-cycle in FP32, or BF16 when
-// that is the preconditioner precision), result in FP64
+// that is the preconditioner precision), result in FP64; Omega is drawn
+// chunk by chunk (sweep_chunk), so only one chunk of it is held
 static int
 sketch_sweep (lgh_glr_t *g, int ps, int64_t col0, int nc, PMat &Y64,
               double *t_op)
 {
-  const int n = g->n;
-  if (ps == F64) {
-    PMat Q, Wk;
-    LGH_TRY (pmat_alloc (Q, n, nc, 64, F64));
-    LGH_TRY (pmat_alloc (Wk, n, nc, 64, F64));
-    int rc = k_randn (Q, g->opts.seed, 0, col0);
-    if (!rc) rc = timed_F (g, Q, Y64, Wk, t_op, pc_for (g, F64));
-    pmat_free (Q); pmat_free (Wk);
-    return rc;
+  const int n = g->n, P = npan (nc), cp = sweep_chunk (g, P);
+  const int cc = std::min (nc, 64 * cp);
+  PMat Om, Wk, Yp;
+  LGH_TRY (pmat_alloc (Om, n, cc, 64, ps));
+  int rc = pmat_alloc (Wk, n, cc, 64, ps);
+  if (!rc && ps != F64) rc = pmat_alloc (Yp, n, cc, 64, ps);
+  for (int q0 = 0; q0 < P && !rc; q0 += cp) {
+    PMat O = Om, W = Wk, Yc = pview (Y64, q0, cp);
+    O.ncols = W.ncols = Yc.ncols;
+    // (k_randn writes the padding columns of a partial last panel as 0)
+    rc = k_randn (O, g->opts.seed, 0, col0 + 64 * (int64_t) q0);
+    if (rc) break;
+    if (ps == F64) rc = timed_F (g, O, Yc, W, t_op, pc_for (g, F64));
+    else {
+      PMat Y = Yp;
+      Y.ncols = O.ncols;
+      rc = timed_F (g, O, Y, W, t_op, pc_for (g, ps));
+      if (!rc) rc = k_copy_cols (Y, 0, Yc, 0, Y.ncols);
+    }
   }
-  PMat Q, Y, Wk;
-  LGH_TRY (pmat_alloc (Q, n, nc, 64, ps));
-  LGH_TRY (pmat_alloc (Y, n, nc, 64, ps));
-  LGH_TRY (pmat_alloc (Wk, n, nc, 64, ps));
-  int rc = k_randn (Q, g->opts.seed, 0, col0);
-  if (!rc) rc = timed_F (g, Q, Y, Wk, t_op, pc_for (g, ps));
-  if (!rc) rc = k_copy_cols (Y, 0, Y64, 0, nc);
-  pmat_free (Q); pmat_free (Y); pmat_free (Wk);
+  pmat_free (Om); pmat_free (Wk); pmat_free (Yp);
   return rc;
 }
 
@@ -384,28 +480,33 @@ build (lgh_glr_t *g, lgh_glr_report_t *rep)
   const int pp = to_prec (g->x.prec_precond);
   const int pc = to_prec (g->x.prec_check);
   memset (rep, 0, sizeof (*rep));
-  PMat Q, Y, Wk;
+  PMat Q, Y;
   LGH_TRY (pmat_alloc (Q, n, ell, 64, F64));
   LGH_TRY (pmat_alloc (Y, n, ell, 64, F64));
-  LGH_TRY (pmat_alloc (Wk, n, ell, 64, F64));
+  // orthonormalization: CholQR2 of the whole block (replicated backend,
+  // glr_common_impl.h:368-407) or BCGS2 in 64-column panels (the
+  // implicit-U backend, glr_scalapack_impl.h:153-195 called at 576 and
+  // 584, in place)
+  auto orth = [&] () {
+    return g->implicit ? bcgs2 (nullptr, 0, Y, ell) : cholqr (Y, ell, 2, nullptr);
+  };
   int rc = sketch_sweep (g, ps, 0, ell, Y, &rep->t_operator);   // F Omega
   for (int p = 0; p < g->opts.q_power && !rc; p++) {
     double t0 = MPI_Wtime ();
-    rc = cholqr (Y, ell, 2, nullptr);
+    rc = orth ();
     if (!rc) rc = dev_sync ();
     rep->t_dense += MPI_Wtime () - t0;
     if (!rc) std::swap (Q.p, Y.p);                              // Q <- Y
-    if (!rc) rc = timed_F (g, Q, Y, Wk, &rep->t_operator, pp);
+    if (!rc) rc = sweep_F (g, Q, Y, &rep->t_operator, pp);
   }
   if (!rc) {
     double t0 = MPI_Wtime ();
-    rc = cholqr (Y, ell, 2, nullptr);
+    rc = orth ();
     if (!rc) std::swap (Q.p, Y.p);
     if (!rc) rc = dev_sync ();
     rep->t_dense += MPI_Wtime () - t0;
   }
-  if (!rc) rc = timed_F (g, Q, Y, Wk, &rep->t_operator, pp);  // Y = F Q
-  pmat_free (Wk);
+  if (!rc) rc = sweep_F (g, Q, Y, &rep->t_operator, pp);      // Y = F Q
   std::vector<double> evals, Araw ((size_t) ell * ell);
   std::vector<int> order;
   if (!rc) {
@@ -422,8 +523,8 @@ build (lgh_glr_t *g, lgh_glr_report_t *rep)
   }
   pmat_free (Y);
   if (!rc) rc = rayleigh_ritz (g, ell, Araw, Q, ell, nullptr, rep, evals, order,
-                               &rep->t_dense);
-  if (!rc) rc = residual_gate (g, evals, order, pc, rep);
+                               &rep->t_dense, g->implicit && g->opts.check);
+  if (!rc && !g->implicit) rc = residual_gate (g, evals, order, pc, rep);
   if (!rc) {
     pmat_free (g->Q);
     g->Q = Q;
@@ -447,9 +548,10 @@ lgh_glr_compute_x (Mat B, lgh_prior_t *prior, const lgh_glr_opts_t *opts,
   // glr_common_impl.h:651-664
   PetscCheck (o.ell > 0 && o.ell <= N, 0, PETSC_ERR_ARG_OUTOFRANGE,
               "lgh_glr_compute: need 0 < ell (%d) <= N (%d)", o.ell, N);
+  // DEFAULT resolves to REPLICATED, as in a reference build without
+  // ScaLAPACK; SCALAPACK selects the implicit-U representation at
+  // communicator size 1 (glr_common_impl.h:673-677)
   if (o.backend == LGH_GLR_DEFAULT) o.backend = LGH_GLR_REPLICATED;
-  PetscCheck (o.backend != LGH_GLR_SCALAPACK, 0, PETSC_ERR_SUP,
-              "lgh_glr_compute: this build has no ScaLAPACK backend");
   PetscCheck (B->m == B->n && B->m == prior->n, 0, PETSC_ERR_ARG_SIZ,
               "lgh_glr_compute: B (%d x %d) and prior (%d) disagree", B->m,
               B->n, prior->n);
@@ -460,7 +562,7 @@ lgh_glr_compute_x (Mat B, lgh_prior_t *prior, const lgh_glr_opts_t *opts,
   auto pc_ok = [&] (int q) { return vec_ok (q) || q == LGH_PREC_BF16; };
   PetscCheck (vec_ok (xo.prec_sketch) && vec_ok (xo.prec_check)
               && pc_ok (xo.prec_precond) && pc_ok (xo.prec_sketch_precond)
-              && xo.solve_blocked_min >= 0,
+              && xo.solve_blocked_min >= 0 && xo.sweep_panels >= 0,
               0, PETSC_ERR_ARG_OUTOFRANGE,
               "lgh_glr_compute_x: precision codes out of range");
   if (dev_init ()) return API_FAIL ("lgh_glr_compute");
@@ -468,6 +570,7 @@ lgh_glr_compute_x (Mat B, lgh_prior_t *prior, const lgh_glr_opts_t *opts,
   g->opts = o;
   g->x = xo;
   g->n = N;
+  g->implicit = (o.backend == LGH_GLR_SCALAPACK);
   g->B = B;
   PetscObjectReference ((PetscObject) B);
   g->prior = prior;
@@ -499,10 +602,9 @@ extend_incr (lgh_glr_t *g, int k_new, lgh_glr_report_t *rep)
   const int pp = to_prec (g->x.prec_precond);
   const int pc = to_prec (g->x.prec_check);
   memset (rep, 0, sizeof (*rep));
-  PMat Qn, Yn, Wn;
+  PMat Qn, Yn;
   LGH_TRY (pmat_alloc (Qn, n, k_new, 64, F64));
   LGH_TRY (pmat_alloc (Yn, n, k_new, 64, F64));
-  LGH_TRY (pmat_alloc (Wn, n, k_new, 64, F64));
   // hashed columns continue the sequence past ell (:726-729)
   int rc = sketch_sweep (g, ps, ell1, k_new, Yn, &rep->t_operator);
   for (int p = 0; p < g->opts.q_power && !rc; p++) {          // :739-747
@@ -511,7 +613,7 @@ extend_incr (lgh_glr_t *g, int k_new, lgh_glr_report_t *rep)
     if (!rc) rc = dev_sync ();
     rep->t_dense += MPI_Wtime () - t0;
     if (!rc) std::swap (Qn.p, Yn.p);
-    if (!rc) rc = timed_F (g, Qn, Yn, Wn, &rep->t_operator, pp);
+    if (!rc) rc = sweep_F (g, Qn, Yn, &rep->t_operator, pp);
   }
   if (!rc) {                                                  // :749-768
     double t0 = MPI_Wtime ();
@@ -520,8 +622,7 @@ extend_incr (lgh_glr_t *g, int k_new, lgh_glr_report_t *rep)
     rep->t_dense += MPI_Wtime () - t0;
     if (!rc) std::swap (Qn.p, Yn.p);
   }
-  if (!rc) rc = timed_F (g, Qn, Yn, Wn, &rep->t_operator, pp);   // :789
-  pmat_free (Wn);
+  if (!rc) rc = sweep_F (g, Qn, Yn, &rep->t_operator, pp);      // :789
   // border Cnew = Q_all^T (F Q_new) (:792-814)
   std::vector<double> A2 ((size_t) ell2 * ell2, 0.0);
   if (!rc) {
@@ -557,7 +658,9 @@ extend_incr (lgh_glr_t *g, int k_new, lgh_glr_report_t *rep)
     rc = rayleigh_ritz (g, ell2, A2, g->Q, ell1, &Qn, rep, evals, order,
                         &rep->t_dense);
   }
-  if (!rc) rc = residual_gate (g, evals, order, pc, rep);
+  // (the implicit-U backend's extension runs no check:
+  // glr_scalapack_impl.h:893-915)
+  if (!rc && !g->implicit) rc = residual_gate (g, evals, order, pc, rep);
   if (!rc) {
     PMat Q2;
     rc = pmat_alloc (Q2, n, ell2, 64, F64);
@@ -600,6 +703,7 @@ lgh_glr_destroy (lgh_glr_t *g)
   if (g == NULL) return;
   pmat_free (g->U);
   pmat_free (g->Q);
+  dfree (g->dV);
   MatDestroy (&g->B);
   lgh_prior_destroy (g->prior);
   delete g;
@@ -634,6 +738,32 @@ woodbury (lgh_glr_t *g, const PMat &X, int m, const std::vector<double> &fk,
   const int k = g->kept;
   if (k == 0) return 0;
   const int Pk = npan (k), ldk = Pk * 64;
+  if (g->implicit) {
+    // U^T X = V^T (Q^T X) and U W = Q (V W) (glr_scalapack_impl.h:923-1027):
+    // the two tall products over Q on the MFMA kernels, the small ones in
+    // rocBLAS, the filter values applied on the host
+    const int Pl = npan (g->ell), ldl = Pl * 64;
+    double *G = (double *) dmalloc (sizeof (double) * (size_t) ldl * ldm);
+    double *W = (double *) dmalloc (sizeof (double) * (size_t) ldk * ldm);
+    rc = (G && W) ? 0 : 5;
+    std::vector<double> hw ((size_t) ldk * ldm);
+    if (!rc) rc = k_gram (g->Q, 0, Pl, X, 0, Pm, G, ldl, 0);
+    if (!rc) rc = blas_dgemm (1, 0, ldk, ldm, ldl, 1.0, g->dV, ldl, G, ldl, 0.0, W, ldk);
+    if (!rc) rc = d2h (hw.data (), W, sizeof (double) * hw.size ());
+    if (!rc) {
+      for (int j = 0; j < ldm; j++)
+        for (int i = 0; i < ldk; i++) {
+          double &w = hw[i + (size_t) j * ldk];
+          w = (i < k && j < m) ? w * fk[i + (size_t) j * k] : 0.0;
+        }
+      rc = h2d (W, hw.data (), sizeof (double) * hw.size ());
+    }
+    if (!rc) rc = blas_dgemm (0, 0, ldl, ldm, ldk, 1.0, g->dV, ldl, W, ldk, 0.0, G, ldl);
+    if (!rc) rc = k_gemm_nn (g->Q, 0, Pl, G, ldl, 1.0, 1.0, Y, 0, Pm);
+    if (!rc) rc = dev_sync ();
+    dfree (G); dfree (W);
+    return rc;
+  }
   double *W = (double *) dmalloc (sizeof (double) * (size_t) ldk * ldm);
   if (!W) return 5;
   std::vector<double> hw ((size_t) ldk * ldm);
@@ -953,17 +1083,47 @@ lgh_glr_sample_block (lgh_glr_t *g, int m, const double *c, const double *xi,
   return 0;
 }
 
+// U = Q V of an implicit object into a new n x kept matrix
+// (glr_scalapack_impl.h:1029-1076)
+static int
+form_U (const lgh_glr_t *g, PMat &U)
+{
+  const int k = g->kept, Pk = npan (std::max (k, 1)), Pl = npan (g->ell);
+  LGH_TRY (pmat_alloc (U, g->n, std::max (k, 1), 64, F64));
+  int rc = (k > 0) ? k_gemm_nn (g->Q, 0, Pl, g->dV, Pl * 64, 1.0, 0.0, U, 0, Pk) : 0;
+  if (!rc) rc = dev_sync ();
+  if (rc) pmat_free (U);
+  return rc;
+}
+
+int
+lgh::glr_materialize_U (lgh_glr_t *g)
+{
+  if (!g->implicit) return 0;
+  PMat U;
+  LGH_TRY (form_U (g, U));
+  pmat_free (g->U);
+  g->U = U;
+  g->implicit = 0;
+  g->V.clear ();
+  dfree (g->dV);
+  g->dV = nullptr;
+  return 0;
+}
+
 extern "C" int
 lgh_glr_get_U (const lgh_glr_t *g, double *U, int ldu)
 {
   const int k = g->kept;
   if (k == 0) return 0;
-  PMat H;
+  PMat H, Uf;
   std::vector<double> h ((size_t) g->n * k);
-  int rc = pmat_alloc (H, g->n, k, 1, F64);
-  if (!rc) rc = k_copy_cols (g->U, 0, H, 0, k);
+  int rc = g->implicit ? form_U (g, Uf) : 0;
+  if (!rc) rc = pmat_alloc (H, g->n, k, 1, F64);
+  if (!rc) rc = k_copy_cols (g->implicit ? Uf : g->U, 0, H, 0, k);
   if (!rc) rc = d2h (h.data (), H.p, sizeof (double) * h.size ());
   pmat_free (H);
+  pmat_free (Uf);
   if (rc) return API_FAIL ("lgh_glr_get_U");
   for (int j = 0; j < k; j++) memcpy (U + (size_t) j * ldu, &h[(size_t) j * g->n], sizeof (double) * g->n);
   return 0;

@@ -225,5 +225,14 @@ int k_trace (const double *A, int n, int ld, double *out_dev);
 int sol_potrf (int upper, int n, double *A, int lda, int *info_host);
 int sol_trtri (int upper, int n, double *A, int lda, int *info_host);
 int sol_syevd (int n, double *A, int lda, double *w_dev, int *info_host);
+// eigenvalues only (QR iteration on the tridiagonal form: a second method
+// beside syevd's divide and conquer); A is destroyed
+int sol_syev_vals (int n, double *A, int lda, double *w_dev, int *info_host);
+// rocBLAS C = alpha op(A) op(B) + beta C on stream(); ta/tb: 1 = transpose
+int blas_dgemm (int ta, int tb, int m, int n, int k, double alpha,
+                const double *A, int lda, const double *B, int ldb,
+                double beta, double *C, int ldc);
+// device memory of the current GPU in bytes (0 if unknown)
+size_t dmem_total ();
 
 }  // namespace lgh

@@ -35,7 +35,8 @@ def lib():
 class GlrOpts(C.Structure):
     _fields_ = [("ell", C.c_int), ("q_power", C.c_int),
                 ("trunc_abs", C.c_double), ("trunc_rel", C.c_double),
-                ("treat", C.c_int), ("seed", C.c_uint64), ("check", C.c_int)]
+                ("treat", C.c_int), ("seed", C.c_uint64), ("check", C.c_int),
+                ("backend", C.c_int)]
 
 
 class GlrReport(C.Structure):
@@ -321,11 +322,12 @@ class GLR:
     """Oracle GLR object over (B CSR, mass, ZSolver)."""
 
     def __init__(self, B, mass, zs, ell=200, q_power=1, trunc_abs=0.0,
-                 trunc_rel=1e-3, treat=0, seed=271828, check=1):
+                 trunc_rel=1e-3, treat=0, seed=271828, check=1, backend=1):
         self.B = B
         self.mass = _f64(mass)
         self.zs = zs
-        o = GlrOpts(ell, q_power, trunc_abs, trunc_rel, treat, seed, check)
+        o = GlrOpts(ell, q_power, trunc_abs, trunc_rel, treat, seed, check,
+                    backend)
         rep = GlrReport()
         self.g = lib().orc_glr_build(*B.args(), _d(self.mass), zs.p,
                                      C.byref(o), C.byref(rep))

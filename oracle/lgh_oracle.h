@@ -119,6 +119,9 @@ typedef struct {
   int      treat;             /* 0 FLIP, 1 RELU, 2 NONE                    */
   uint64_t seed;
   int      check;
+  int      backend;           /* 2: the at-scale backend's build at one
+                                 process (BCGS2 in 64-column panels, no
+                                 residual sweep); otherwise replicated     */
 } orc_glr_opts_t;
 
 typedef struct {

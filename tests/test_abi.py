@@ -141,8 +141,11 @@ def test_api_refusals_without_gpu(lgh):
     o = lgh.glr_opts(ell=11)
     g = C.c_void_p()
     assert L.lgh_glr_compute(B.h, pr.h, C.byref(o), C.byref(g), None) == 63
+    # a negative sweep chunk is refused before any device work
     o = lgh.glr_opts(ell=5, backend=2)
-    assert L.lgh_glr_compute(B.h, pr.h, C.byref(o), C.byref(g), None) == 56
+    x = lgh.xopts(sweep_panels=-1)
+    assert L.lgh_glr_compute_x(B.h, pr.h, C.byref(o), C.byref(x), C.byref(g),
+                               None) == 63
     # batches: nothing to do, and missing arrays (PETSC_ERR_ARG_NULL)
     assert L.lgh_glr_compute_batch(0, None, None, None, None, 0, None, None) == 0
     assert L.lgh_glr_compute_batch(2, None, None, None, None, 0, None, None) == 85

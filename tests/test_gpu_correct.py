@@ -59,11 +59,12 @@ def dev(lgh):
     return lgh
 
 
-def build(dev):
+def build(dev, backend=0):
     prior = dev.Prior.diagonal_callbacks(W.tg_z(N), W.tg_m(N))
     B = dev.Mat.dense(W.tg_B_exact(N, RANK_B, True))
     return dev.GLR(B, prior, dev.glr_opts(ell=40, q_power=1, trunc_abs=0.0,
-                                          trunc_rel=1e-8, nb=8, panel=16, check=0))
+                                          trunc_rel=1e-8, nb=8, panel=16, check=0,
+                                          backend=backend))
 
 
 def identity_battery(g, c, v):
@@ -87,11 +88,14 @@ def check_spectrum(g, oracle, nexpect, tol):
     assert worst < tol, worst
 
 
-def test_value_pass(dev):
-    """Scenario A (tests/test_deflate.cpp:395-478)"""
+@pytest.mark.parametrize("backend", [0, 2])
+def test_value_pass(dev, backend):
+    """Scenario A (tests/test_deflate.cpp:395-478); backend 2: an implicit-U
+    object, whose U is formed when the correction grafts
+    (correct_impl.h:338-345)"""
     c, k, n_qc = 0.7, 30, 6
     H = hd_matrix(1.0)
-    g = build(dev)
+    g = build(dev, backend)
     V = probes(k)
     rep = g.correct_probes(V, H @ V, lambda x: H @ x, c0=1.0, applies=8, n_qc=n_qc)
     assert rep["basis"] == RANK_C

@@ -76,6 +76,63 @@ def test_glr_gates(dev, oracle, spd):
     assert np.max(np.abs(g2.lam - lam) / np.abs(lam)) < 1e-10
 
 
+@pytest.mark.parametrize("spd", [True, False])
+def test_glr_implicit_u_backend(dev, oracle, spd):
+    """LGH_GLR_SCALAPACK at communicator size 1 (glr_scalapack_impl.h:505-688
+    build, 917-1076 implicit-U products): U = Q V is never formed, BCGS2
+    orthonormalizes, and the check compares two eigensolvers.  The
+    test_glr.cpp gates, the oracle's BCGS2 build, and the same operator
+    actions as the replicated object."""
+    c = 0.7
+    Bd = W.tg_B_exact(spd=spd)
+    dense = np.load(os.path.join(GOLDEN, "tg_eigs_%s.npy" % ("spd" if spd else "indef")))
+    prior = dev.Prior.diagonal_callbacks(W.tg_z(), W.tg_m())
+    B = dev.Mat.dense(Bd)
+    o = dict(ell=40, q_power=1, trunc_abs=0.0, trunc_rel=1e-8, check=1)
+    g = dev.GLR(B, prior, dev.glr_opts(backend=2, **o))
+    gr = dev.GLR(B, prior, dev.glr_opts(backend=1, **o))
+    og = oracle.GLR(oracle.CSR.from_dense(Bd), W.tg_m(), oracle.ZSolver.diag(W.tg_z()),
+                    ell=40, q_power=1, trunc_rel=1e-8, backend=2)
+    rep = g.report
+    assert rep["kept"] == og.kept == 10
+    lam = g.lam
+    assert np.max(np.abs(lam - np.abs(dense[:10])) / np.abs(dense[:10])) < 1e-8
+    assert np.max(np.abs(lam - og.lam) / np.abs(og.lam)) < 1e-12
+    assert np.max(np.abs(lam - gr.lam) / np.abs(gr.lam)) < 1e-12
+    assert 0.0 <= rep["resid_max"] < 1e-12       # eigensolver cross-check
+    # (past the exact rank the raw values are rounding noise: absolute scale)
+    for key in ("lam_raw_min", "lam_raw_max", "next_abs"):
+        assert abs(rep[key] - og.report[key]) <= 1e-12 * rep["lam_abs_max"], key
+    # U formed on request equals the oracle's up to column signs
+    U, OU = g.U, og.U
+    U = U * np.sign(np.sum(U * OU, axis=0))
+    assert rel(U, OU) < 1e-10
+    v = W.tg_probe()
+    assert abs(g.logdet(c) - og.logdet(c)) < 1e-12 * abs(og.logdet(c))
+    assert rel(g.pow(c, -0.5, v), og.pow(c, -0.5, v)) < 1e-11
+    if spd:
+        y = g.apply(c, v)
+        for which in range(4):
+            assert rel(g.factor(c, which, v), gr.factor(c, which, v)) < 1e-11
+        assert rel(g.solve(c, y), og.solve(c, y)) < 1e-11
+        assert rel(g.sample(c, v), og.sample(c, v)) < 1e-11
+        cs = np.array([0.1, 0.7, 3.0])
+        R = np.stack([v, np.cos(np.arange(v.size)), v[::-1].copy()], 1)
+        Xb = g.solve_block(cs, R)
+        for j in range(3):
+            assert rel(Xb[:, j], g.solve(cs[j], R[:, j])) < 1e-12
+    # extend stays implicit: == the oracle's incremental extension
+    g2 = dev.GLR(B, prior, dev.glr_opts(backend=2, **dict(o, ell=30)))
+    g2.extend(10)
+    og2 = oracle.GLR(oracle.CSR.from_dense(Bd), W.tg_m(), oracle.ZSolver.diag(W.tg_z()),
+                     ell=30, q_power=1, trunc_rel=1e-8, backend=2)
+    og2.extend(10, incremental=True, panel=64)
+    assert g2.kept == og2.kept == 10
+    assert np.max(np.abs(g2.lam - og2.lam) / np.abs(og2.lam)) < 1e-12
+    assert np.max(np.abs(g2.lam - lam) / np.abs(lam)) < 1e-10
+    assert g2.report["resid_max"] == 0.0          # the extension runs no check
+
+
 @pytest.mark.parametrize("treat", [0, 1, 2])
 def test_glr_treatments_match_oracle(dev, oracle, treat):
     """FLIP / RELU / NONE (glr_common_impl.h:549-559) on the indefinite
@@ -266,6 +323,28 @@ def test_grid_glr_matches_oracle(dev, grid, oracle, policy):
     Xb = g.solve_block(cs, R)
     for j in range(3):
         assert rel(Xb[:, j], g.solve(cs[j], R[:, j])) < 1e-12
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_grid_sweep_chunks_bitwise(dev, grid, mixed):
+    """F sweeps in chunks of 1 or 2 panels (xopts.sweep_panels) give the
+    same object bit for bit as the whole block at once: panels are solved
+    independently (ell = 150: two whole panels and a 22-column tail)."""
+    o = dev.glr_opts(ell=150, seed=0, trunc_rel=1e-6)
+    ref = dev.GLR(grid["B"], grid["prior"], o, x=dev.xopts(mixed=mixed))
+    for cp in (1, 2):
+        g = dev.GLR(grid["B"], grid["prior"], o,
+                    x=dev.xopts(mixed=mixed, sweep_panels=cp))
+        assert g.kept == ref.kept
+        assert np.array_equal(g.lam, ref.lam), cp
+        assert np.array_equal(g.U, ref.U), cp
+    # and on the implicit-U backend, with an extension in chunks too
+    oi = dev.glr_opts(ell=150, seed=0, trunc_rel=1e-6, backend=2)
+    a = dev.GLR(grid["B"], grid["prior"], oi, x=dev.xopts(mixed=mixed))
+    b = dev.GLR(grid["B"], grid["prior"], oi, x=dev.xopts(mixed=mixed, sweep_panels=1))
+    a.extend(70)
+    b.extend(70)
+    assert np.array_equal(a.lam, b.lam)
 
 
 def test_grid_extend_incremental(dev, grid, oracle):

@@ -83,18 +83,27 @@ def test_cheb_count_closed_form(oracle):
 # ---- tests/test_glr.cpp gates ---------------------------------------------
 
 @pytest.mark.parametrize("spd", [True, False])
-def test_glr_gates(oracle, spd):
+@pytest.mark.parametrize("backend", [1, 2])
+def test_glr_gates(oracle, spd, backend):
+    """backend 2: the at-scale backend's build (BCGS2 orthonormalization,
+    no residual sweep) against the same dense closed form."""
     c = 0.7
     B = oracle.CSR.from_dense(W.tg_B_exact(spd=spd))
     dense = np.load(os.path.join(GOLDEN, "tg_eigs_%s.npy" % ("spd" if spd else "indef")))
     zs = oracle.ZSolver.diag(W.tg_z())
-    g = oracle.GLR(B, W.tg_m(), zs, ell=40, q_power=1, trunc_rel=1e-8, check=1)
+    g = oracle.GLR(B, W.tg_m(), zs, ell=40, q_power=1, trunc_rel=1e-8, check=1,
+                   backend=backend)
     rep = g.report
     assert rep["kept"] == 10
     lam = g.lam
     assert np.max(np.abs(lam - np.abs(dense[:10])) / np.abs(dense[:10])) < 1e-8
     assert rep["next_abs"] < 1e-6 * rep["lam_abs_max"]
     assert rep["resid_max"] < 1e-7 * rep["lam_abs_max"]
+    if backend == 2:
+        assert rep["resid_max"] == 0.0
+        # the two orthonormalizations span the same space: the same Ritz values
+        g1 = oracle.GLR(B, W.tg_m(), zs, ell=40, q_power=1, trunc_rel=1e-8)
+        assert np.max(np.abs(lam - g1.lam) / np.abs(g1.lam)) < 1e-12
     if not spd:
         assert rep["n_negative_raw"] >= 5
     ref = np.sum(np.log(np.abs(dense[:10]) + c) - np.log(c))
