@@ -399,6 +399,13 @@ def main():
                     help="batch: concurrent build workers (0: the library's default)")
     ap.add_argument("--skip-sequential", action="store_true",
                     help="batch: time the batches only")
+    ap.add_argument("--backend", choices=("replicated", "implicit"),
+                    default="replicated",
+                    help="grid2d: U explicit (LGH_GLR_REPLICATED) or implicit "
+                         "U = Q V (LGH_GLR_SCALAPACK at one process)")
+    ap.add_argument("--sweep-panels", type=int, default=0,
+                    help="64-column panels per F sweep chunk (0: all at once "
+                         "unless device memory is short)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -434,13 +441,15 @@ def main():
     info = prior.info()
     log("prior create %.1fs %s" % (time.time() - t0, info))
     Bm = lgh.Mat.aij(Bs)
-    opts = lgh.glr_opts(ell=args.ell, q_power=1, seed=0)
+    backend = {"replicated": 1, "implicit": 2}[args.backend]
+    opts = lgh.glr_opts(ell=args.ell, q_power=1, seed=0, backend=backend)
     pcode = {"bf16": lgh.PREC_BF16, "f32": lgh.PREC_F32}
     mixed = lgh.xopts(mixed=True, prec_precond=pcode[args.vcycle],
-                      prec_sketch_precond=pcode[args.sketch_vcycle])
+                      prec_sketch_precond=pcode[args.sketch_vcycle],
+                      sweep_panels=args.sweep_panels)
     if args.sketch:
         mixed.prec_sketch = {"f32": lgh.PREC_F32, "f64": lgh.PREC_F64}[args.sketch]
-    plain = lgh.xopts(mixed=False)
+    plain = lgh.xopts(mixed=False, sweep_panels=args.sweep_panels)
 
     for w in range(args.warmup):
         t0 = time.time()
@@ -465,6 +474,8 @@ def main():
     # full-width launches on the finest level (all whole 64-column panels;
     # a last panel of at most 48 columns runs as 16-wide panels)
     npan = args.ell // 64 if 0 < args.ell % 64 <= 48 else (args.ell + 63) // 64
+    if 0 < args.sweep_panels < npan:      # chunked sweeps: the full chunks
+        npan = args.sweep_panels
     vpc = pcode[args.vcycle]
     single = ("cheb_mixed", "smooth_pre_fine", "smooth_post_fine", "resid_fine")
     prof = {k: lgh.prof_collect(k, npan, vpc) if k in single else lgh.prof_collect(k)
@@ -479,16 +490,21 @@ def main():
     # dominant single kernel (by total time) -> roofline
     dom = max(single, key=lambda k: prof[k][1])
     c, ms, by, fl = prof[dom]
-    avg_ms = ms / c
-    ach = by / c / (avg_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
-            "kernel": kernel_of(dom, args.vcycle), "panels": npan, "avg_launch_ms": avg_ms,
-            "launches": c, "algorithmic_bytes_per_launch": by / c}
-    tr = pmc_traffic(kernel_of(dom, args.vcycle), npan)
-    if tr is not None:
-        roof["traffic"] = tr
-        roof["traffic_source"] = "profiles/glr_build_grid2d_1024_pmc_traffic.md"
+    roof = None
+    if c:
+        avg_ms = ms / c
+        ach = by / c / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                "kernel": kernel_of(dom, args.vcycle), "panels": npan,
+                "avg_launch_ms": avg_ms, "launches": c,
+                "algorithmic_bytes_per_launch": by / c}
+        tr = pmc_traffic(kernel_of(dom, args.vcycle), npan)
+        if tr is not None:
+            roof["traffic"] = tr
+            roof["traffic_source"] = "profiles/glr_build_grid2d_1024_pmc_traffic.md"
+    else:
+        log("no full-width launches at %d panels: roofline not reported" % npan)
     sp = prof["spmm_B"]
     spmm_roof = None
     if sp[0]:
@@ -611,6 +627,7 @@ def main():
                                % (args.side, args.side, n, Bs.nnz / n, args.ell,
                                   info["nit"]),
                    "n": n, "ell": args.ell, "kept": kept_build,
+                   "backend": args.backend, "sweep_panels": args.sweep_panels,
                    "vcycle_fp64_sweeps": args.vcycle,
                    "vcycle_fp32_sweeps": args.sketch_vcycle},
         "roofline": roof,

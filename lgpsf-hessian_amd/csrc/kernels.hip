@@ -1555,9 +1555,11 @@ k_gram (const PMat &X, int xp0, int npx, const PMat &Y, int yp0, int npy,
   const int n = X.nrows;
   ProfScope ps (PK_GRAM, (double) n * 64 * prec_bytes (X.prec) * (npx + npy),
                 2.0 * n * 4096.0 * npairs);
-  // aim for ~4096 blocks; at least 256 rows per chunk
+  // aim for ~4096 blocks; at least 256 rows per chunk, and at least 8
+  // chunks: chunk c of every group runs on XCD c % 8 (xcd_split), so fewer
+  // chunks would leave XCDs idle when there are many groups (wide blocks)
   int nch = 4096 / ngroups;
-  if (nch < 1) nch = 1;
+  if (nch < 8) nch = 8;
   int rpc = (n + nch - 1) / nch;
   if (rpc < 256) rpc = 256;
   rpc = (rpc + 3) / 4 * 4;
