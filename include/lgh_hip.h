@@ -58,6 +58,14 @@ int lgh_hip_randn_fill (uint64_t seed, int64_t row0, int64_t col0, int nr,
 int lgh_hip_spmm (int n, int m, const int *rp, const int *ci, const double *v,
                   int prec_vals, int prec_vec, int W, int ncols,
                   const double *X, int ldx, double *Y, int ldy);
+/* the same through the row-grouped copy of A that the engine keeps for B
+ * (rows in groups of 4, each group's union of column indices walked once;
+ * W = 64; rows with strictly increasing column indices, else the plain
+ * kernel runs): the same bits as lgh_hip_spmm */
+int lgh_hip_spmm_grouped (int n, int m, const int *rp, const int *ci,
+                          const double *v, int prec_vals, int prec_vec,
+                          int W, int ncols, const double *X, int ldx,
+                          double *Y, int ldy);
 
 /* X = Chebyshev(nit) solve of A X = B with the Jacobi preconditioner on
  * the interval [lmin, lmax], zero initial guess, one fused kernel per

@@ -224,6 +224,9 @@ _SIGS = {
                                      C.c_int, C.c_int, C.c_int, dp, C.c_int]),
     "lgh_hip_spmm": (C.c_int, [C.c_int, C.c_int, ip, ip, dp, C.c_int, C.c_int,
                                C.c_int, C.c_int, dp, C.c_int, dp, C.c_int]),
+    "lgh_hip_spmm_grouped": (C.c_int, [C.c_int, C.c_int, ip, ip, dp, C.c_int,
+                                       C.c_int, C.c_int, C.c_int, dp, C.c_int,
+                                       dp, C.c_int]),
     "lgh_hip_cheb_jacobi": (C.c_int, [C.c_int, ip, ip, dp, C.c_int, C.c_int,
                                       C.c_double, C.c_double, C.c_int, C.c_int,
                                       dp, C.c_int, dp, C.c_int]),
@@ -741,13 +744,14 @@ def _csr(A):
             np.ascontiguousarray(A.data, dtype=np.float64))
 
 
-def spmm(A, X, prec_vals=PREC_F64, prec_vec=PREC_F64, W=64):
+def spmm(A, X, prec_vals=PREC_F64, prec_vec=PREC_F64, W=64, grouped=False):
     rp, ci, v = _csr(A)
     X = _fcol(X)
     Y = np.zeros((A.shape[0], X.shape[1]), order="F")
-    _hip_check(lib().lgh_hip_spmm(A.shape[0], A.shape[1], _i(rp), _i(ci), _d(v),
-                                  prec_vals, prec_vec, W, X.shape[1], _d(X),
-                                  X.shape[0], _d(Y), A.shape[0]), "lgh_hip_spmm")
+    fn = lib().lgh_hip_spmm_grouped if grouped else lib().lgh_hip_spmm
+    _hip_check(fn(A.shape[0], A.shape[1], _i(rp), _i(ci), _d(v), prec_vals,
+                  prec_vec, W, X.shape[1], _d(X), X.shape[0], _d(Y), A.shape[0]),
+               "lgh_hip_spmm")
     return Y
 
 

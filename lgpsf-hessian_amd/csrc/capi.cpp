@@ -92,15 +92,16 @@ lgh_hip_randn_fill (uint64_t seed, int64_t row0, int64_t col0, int nr, int nc,
   return rc;
 }
 
-int
-lgh_hip_spmm (int n, int m, const int *rp, const int *ci, const double *v,
-              int prec_vals, int prec_vec, int W, int ncols, const double *X,
-              int ldx, double *Y, int ldy)
+static int
+spmm_host (int n, int m, const int *rp, const int *ci, const double *v,
+           int prec_vals, int prec_vec, int W, int ncols, const double *X,
+           int ldx, double *Y, int ldy, int grouped)
 {
   LGH_TRY (dev_init ());
   DevCSR *A = nullptr;
   LGH_TRY (csr_upload (n, m, rp, ci, v, prec_vals, 0, &A));
   std::unique_ptr<DevCSR> hold (A);
+  if (grouped) LGH_TRY (csr_group_rows (*A, rp, ci, v));
   PMat dx, dy;
   int rc = stage_in (X, ldx, m, ncols, W, prec_vec, dx);
   if (!rc) rc = pmat_alloc (dy, n, ncols, W, prec_vec);
@@ -108,6 +109,24 @@ lgh_hip_spmm (int n, int m, const int *rp, const int *ci, const double *v,
   if (!rc) rc = stage_out (dy, ncols, Y, ldy);
   pmat_free (dx); pmat_free (dy);
   return rc;
+}
+
+int
+lgh_hip_spmm (int n, int m, const int *rp, const int *ci, const double *v,
+              int prec_vals, int prec_vec, int W, int ncols, const double *X,
+              int ldx, double *Y, int ldy)
+{
+  return spmm_host (n, m, rp, ci, v, prec_vals, prec_vec, W, ncols, X, ldx, Y,
+                    ldy, 0);
+}
+
+int
+lgh_hip_spmm_grouped (int n, int m, const int *rp, const int *ci,
+                      const double *v, int prec_vals, int prec_vec, int W,
+                      int ncols, const double *X, int ldx, double *Y, int ldy)
+{
+  return spmm_host (n, m, rp, ci, v, prec_vals, prec_vec, W, ncols, X, ldx, Y,
+                    ldy, 1);
 }
 
 int

@@ -9,8 +9,12 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
+#include <climits>
 #include <map>
 #include <mutex>
+#include <thread>
+#include <vector>
 #include <cstdio>
 #include <cstring>
 
@@ -319,6 +323,85 @@ dcopy (void *dst, const void *src, size_t bytes)
 DevCSR::~DevCSR ()
 {
   dfree (rp); dfree (ci); dfree (v); dfree (dinv);
+  dfree (gp); dfree (gci); dfree (gv);
+}
+
+// Each group's union is the merge of its (sorted) rows; a row's entries
+// keep their order inside it, so a product accumulated over the union in
+// order adds the same terms in the same order as over the row (plus
+// exact zeros).  Two passes over the groups (sizes, then fill), each split
+// over host threads.
+int
+csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v)
+{
+  const int n = A.n, ng = (n + RGROUP - 1) / RGROUP;
+  for (int i = 0; i < n; i++)
+    for (int k = rp[i] + 1; k < rp[i + 1]; k++)
+      if (ci[k] <= ci[k - 1]) return 0;           // unsorted: no copy
+  std::vector<int> gp ((size_t) ng + 1, 0);
+  auto merge = [&] (int g, int *cols, double *vals) -> int {
+    int p[RGROUP], e[RGROUP];
+    for (int r = 0; r < RGROUP; r++) {
+      const int i = g * RGROUP + r;
+      p[r] = (i < n) ? rp[i] : 0;
+      e[r] = (i < n) ? rp[i + 1] : 0;
+    }
+    int cnt = 0;
+    for (;;) {
+      int c = INT_MAX;
+      for (int r = 0; r < RGROUP; r++) if (p[r] < e[r]) c = std::min (c, ci[p[r]]);
+      if (c == INT_MAX) break;
+      for (int r = 0; r < RGROUP; r++) {
+        const bool hit = p[r] < e[r] && ci[p[r]] == c;
+        if (vals) vals[(size_t) cnt * RGROUP + r] = hit ? v[p[r]] : 0.0;
+        if (hit) p[r]++;
+      }
+      if (cols) cols[cnt] = c;
+      cnt++;
+    }
+    return cnt;
+  };
+  const int nt = (int) std::max (1u, std::min (16u, std::thread::hardware_concurrency ()));
+  auto par = [&] (auto &&body) {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++)
+      th.emplace_back ([&, t] {
+        for (int g = (int) ((int64_t) ng * t / nt); g < (int) ((int64_t) ng * (t + 1) / nt); g++)
+          body (g);
+      });
+    for (auto &x : th) x.join ();
+  };
+  par ([&] (int g) { gp[g + 1] = merge (g, nullptr, nullptr); });
+  for (int g = 0; g < ng; g++) gp[g + 1] += gp[g];
+  const int64_t nu = gp[ng];
+  std::vector<int> cols ((size_t) std::max<int64_t> (nu, 1));
+  std::vector<double> vals ((size_t) std::max<int64_t> (nu, 1) * RGROUP);
+  par ([&] (int g) { merge (g, cols.data () + gp[g], vals.data () + (size_t) gp[g] * RGROUP); });
+  const size_t vb = prec_bytes (A.prec);
+  A.gp = (int *) dmalloc (sizeof (int) * ((size_t) ng + 1));
+  A.gci = (int *) dmalloc (sizeof (int) * cols.size ());
+  A.gv = dmalloc (vb * vals.size ());
+  if (!A.gp || !A.gci || !A.gv) {
+    dfree (A.gp); dfree (A.gci); dfree (A.gv);
+    A.gp = A.gci = nullptr; A.gv = nullptr;
+    return 5;
+  }
+  int rc = h2d (A.gp, gp.data (), sizeof (int) * gp.size ());
+  if (!rc) rc = h2d (A.gci, cols.data (), sizeof (int) * cols.size ());
+  if (!rc) {
+    if (A.prec == F64) rc = h2d (A.gv, vals.data (), sizeof (double) * vals.size ());
+    else {
+      std::vector<float> f (vals.begin (), vals.end ());
+      rc = h2d (A.gv, f.data (), sizeof (float) * f.size ());
+    }
+  }
+  if (rc) {
+    dfree (A.gp); dfree (A.gci); dfree (A.gv);
+    A.gp = A.gci = nullptr; A.gv = nullptr;
+    return rc;
+  }
+  A.nunion = nu;
+  return 0;
 }
 
 int

@@ -354,6 +354,88 @@ spmm64_k (int n, int nx, const int *__restrict__ rp,
   vst (y, acc);
 }
 
+// Y = alpha A X + beta Y over the row-grouped copy of A (DevCSR::gp), 64-
+// column panels: one wavefront per group of RGROUP rows and PPW panels,
+// lane = column.  The wave walks the union of the group's column
+// indices, so a neighbour row of X that several rows of the group share is
+// loaded once and used for each of them (a 127-entry PSF row shares ~90 %
+// of its neighbours with the next row: ~1/3 of the row-by-row loads), and
+// each entry's indices and values serve several panels.  The group index is
+// wave-uniform, so the column indices and the RGROUP values of an entry
+// come through scalar loads; an entry costs one vector load per panel.
+// Row r accumulates its own entries in column order, as spmm64_k does,
+// plus exact zeros: the result is the same bits.
+static constexpr int NGR = 8;           // union entries per load group
+#ifndef SPMM_PPW64
+#define SPMM_PPW64 4
+#endif
+#ifndef SPMM_PPW32
+#define SPMM_PPW32 4
+#endif
+template <typename TV, typename T, typename S, int PPW>
+__global__ void __launch_bounds__ (TPB)
+spmm_grp_k (int n, int ngroups, int nx, int np, const int *__restrict__ gp,
+            const int *__restrict__ gci, const TV *__restrict__ gv, T alpha,
+            const S *__restrict__ X, T beta, S *__restrict__ Y)
+{
+  const int lane = (int) threadIdx.x & 63;
+  const int grp = __builtin_amdgcn_readfirstlane (
+      (int) (blockIdx.x * (TPB / 64) + (threadIdx.x >> 6)));
+  if (grp >= ngroups) return;
+  const int r0 = grp * RGROUP;
+  const int p0 = PPW * (int) blockIdx.y;
+  // panels past the last read the first one again (results discarded)
+  const S *Xq[PPW];
+#pragma unroll
+  for (int q = 0; q < PPW; q++)
+    Xq[q] = X + (size_t) (p0 + q < np ? p0 + q : p0) * nx * 64 + lane;
+  const int k0 = gp[grp], k1 = gp[grp + 1];
+  T acc[RGROUP][PPW];
+#pragma unroll
+  for (int r = 0; r < RGROUP; r++)
+#pragma unroll
+    for (int q = 0; q < PPW; q++) acc[r][q] = 0;
+  for (int kb = k0; kb < k1; kb += NGR) {
+    int c[NGR];
+    TV w[NGR][RGROUP];                  // widened at use (FP32 values stay
+                                        // in scalar registers)
+#pragma unroll
+    for (int u = 0; u < NGR; u++) {
+      const bool in = kb + u < k1;
+      c[u] = in ? gci[kb + u] : gci[k0];
+#pragma unroll
+      for (int r = 0; r < RGROUP; r++)
+        w[u][r] = in ? gv[(size_t) (kb + u) * RGROUP + r] : (TV) 0;
+    }
+    T x[NGR][PPW];
+#pragma unroll
+    for (int u = 0; u < NGR; u++) {
+      const bool in = kb + u < k1;
+#pragma unroll
+      for (int q = 0; q < PPW; q++)
+        x[u][q] = in ? (T) Xq[q][(size_t) c[u] * 64] : (T) 0;
+    }
+#pragma unroll
+    for (int u = 0; u < NGR; u++)
+#pragma unroll
+      for (int r = 0; r < RGROUP; r++)
+#pragma unroll
+        for (int q = 0; q < PPW; q++) acc[r][q] += (T) w[u][r] * x[u][q];
+  }
+#pragma unroll
+  for (int q = 0; q < PPW; q++) {
+    if (p0 + q >= np) break;
+#pragma unroll
+    for (int r = 0; r < RGROUP; r++) {
+      const int row = r0 + r;
+      if (row >= n) break;
+      S *y = Y + ((size_t) (p0 + q) * n + row) * 64 + lane;
+      *y = (S) ((beta == (T) 0) ? alpha * acc[r][q]
+                                : alpha * acc[r][q] + beta * (T) *y);
+    }
+  }
+}
+
 // R = B - A X
 template <typename TV, typename T, typename S, int W = 64>
 __global__ void __launch_bounds__ (TPB)
@@ -763,6 +845,20 @@ template <typename TV, typename T, typename S = T>
 static int
 spmm_t (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
 {
+  if constexpr (std::is_same<S, T>::value) {
+    if (A.gp && X.W == 64) {
+      constexpr int PPW = std::is_same<TV, double>::value ? SPMM_PPW64 : SPMM_PPW32;
+      const int ng = (A.n + RGROUP - 1) / RGROUP, np = X.npanels ();
+      const dim3 g ((unsigned) ((ng + TPB / 64 - 1) / (TPB / 64)),
+                    (unsigned) ((np + PPW - 1) / PPW));
+      hipLaunchKernelGGL ((spmm_grp_k<TV, T, S, PPW>), g, dim3 (TPB), 0,
+                          stream (), A.n, ng, A.m, np, A.gp, A.gci,
+                          (const TV *) A.gv, (T) alpha, (const S *) X.p,
+                          (T) beta, (S *) Y.p);
+      LGH_LAUNCH_CHECK ();
+      return 0;
+    }
+  }
   if (row_width (X.W))
     return with_row_W (X.W, [&] (auto wc) -> int {
       constexpr int W = decltype (wc)::value;

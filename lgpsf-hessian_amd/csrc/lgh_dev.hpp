@@ -115,6 +115,9 @@ int prof_collect (int kind, int np, int prec, long *count, double *ms,
                   double *bytes, double *flops);
 
 // ---- objects ------------------------------------------------------------
+// rows per group of the row-grouped copy of a wide-row matrix
+static constexpr int RGROUP = 4;
+
 struct DevCSR {
   int n = 0, m = 0;               // rows, cols
   int64_t nnz = 0;
@@ -122,11 +125,22 @@ struct DevCSR {
   void *v = nullptr;              // values, precision `prec`
   int prec = F64;
   void *dinv = nullptr;           // 1/diag (same precision), square only
+  // row-grouped copy (csr_group_rows): rows in groups of RGROUP; group g
+  // holds the sorted union of its rows' column indices gci[gp[g] ..
+  // gp[g+1]) and, per union entry e, RGROUP values gv[e * RGROUP + r]
+  // (0 where row r of the group has no entry)
+  int *gp = nullptr, *gci = nullptr;
+  void *gv = nullptr;
+  int64_t nunion = 0;
   ~DevCSR ();
 };
 // upload host CSR (values fp64 on the host) at precision prec
 int csr_upload (int n, int m, const int *rp, const int *ci, const double *v,
                 int prec, int want_dinv, DevCSR **out);
+// add the row-grouped copy to an uploaded matrix (same host CSR); rows
+// must hold strictly increasing column indices, else nothing is added
+// (returns 0 either way unless a device call fails)
+int csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v);
 
 struct PMat {
   void *p = nullptr;

@@ -65,6 +65,15 @@ def test_spmm_wide_rows_width_invariant(dev, prec_vals):
     assert np.array_equal(y64, y32)
     ref = B @ X if prec_vals == 0 else B.astype(np.float32).astype(np.float64) @ X
     assert rel(y64, ref) < 1e-13
+    # the row-grouped copy (groups of 4 rows, n = 2025 leaves a last group
+    # of 1): the same sums in the same order plus exact zeros, FP64 and FP32
+    # vectors
+    assert B.shape[0] % 4 == 1
+    assert np.array_equal(dev.spmm(B, X, prec_vals=prec_vals, W=64, grouped=True), y64)
+    for pv in (dev.PREC_F32,):
+        a = dev.spmm(B, X, prec_vals=1, prec_vec=pv, W=64)
+        b = dev.spmm(B, X, prec_vals=1, prec_vec=pv, W=64, grouped=True)
+        assert np.array_equal(a, b)
 
 
 def test_cheb_jacobi_matches_oracle(dev, oracle):
