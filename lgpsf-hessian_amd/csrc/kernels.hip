@@ -248,6 +248,41 @@ __device__ inline void vld (const float *p, double (&x)[4])
 { const float4 v = *(const float4 *) p; x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w; }
 __device__ inline void vst (float *p, const double (&x)[4])
 { *(float4 *) p = make_float4 ((float) x[0], (float) x[1], (float) x[2], (float) x[3]); }
+// streaming forms of the 4-value loads and stores (nontemporal: data that
+// no other row of the launch reads, and that the next reader finds evicted
+// anyway at fine-level sizes, does not displace the gathered neighbours)
+typedef double ntd2 __attribute__ ((ext_vector_type (2)));
+typedef float ntf4 __attribute__ ((ext_vector_type (4)));
+typedef uint32_t ntu2 __attribute__ ((ext_vector_type (2)));
+__device__ inline void ntld (const double *p, double (&x)[4])
+{
+  const ntd2 a = __builtin_nontemporal_load ((const ntd2 *) p);
+  const ntd2 b = __builtin_nontemporal_load ((const ntd2 *) (p + 2));
+  x[0] = a.x; x[1] = a.y; x[2] = b.x; x[3] = b.y;
+}
+__device__ inline void ntld (const float *p, double (&x)[4])
+{
+  const ntf4 v = __builtin_nontemporal_load ((const ntf4 *) p);
+  x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+}
+__device__ inline void ntst (double *p, const double (&x)[4])
+{
+  __builtin_nontemporal_store ((ntd2) { x[0], x[1] }, (ntd2 *) p);
+  __builtin_nontemporal_store ((ntd2) { x[2], x[3] }, (ntd2 *) (p + 2));
+}
+__device__ inline void ntst (float *p, const double (&x)[4])
+{
+  __builtin_nontemporal_store (
+      (ntf4) { (float) x[0], (float) x[1], (float) x[2], (float) x[3] },
+      (ntf4 *) p);
+}
+__device__ inline void ntst (bf16_t *p, const double (&x)[4])
+{
+  const uint32_t w0 = f2bf ((float) x[0]) | (f2bf ((float) x[1]) << 16);
+  const uint32_t w1 = f2bf ((float) x[2]) | (f2bf ((float) x[3]) << 16);
+  __builtin_nontemporal_store ((ntu2) { w0, w1 }, (ntu2 *) p);
+}
+
 template <typename S> struct Raw4;
 template <> struct Raw4<float> { using type = float4; };
 template <> struct Raw4<bf16_t> { using type = uint2; };
@@ -652,10 +687,10 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
   }
 #pragma unroll
   for (int i = 0; i < N; i++) r[i] -= acc[i];
-  vst (Dn + qi, di);
-  if (xmode != MIX_X_SKIP) vst (X + qi, x);
-  vst (Rn + qi, r);
-  vst (Rpc + qi, r);
+  ntst (Dn + qi, di);
+  if (xmode != MIX_X_SKIP) ntst (X + qi, x);
+  ntst (Rn + qi, r);
+  ntst (Rpc + qi, r);
 }
 
 int
