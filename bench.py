@@ -406,6 +406,10 @@ def main():
     ap.add_argument("--sweep-panels", type=int, default=0,
                     help="64-column panels per F sweep chunk (0: all at once "
                          "unless device memory is short)")
+    ap.add_argument("--solve-blocked-min", type=int, default=None,
+                    help="downstream calls on at least this many columns use "
+                         "the blocked Z-solve tier (default: the mixed "
+                         "policy's)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -449,6 +453,8 @@ def main():
                       sweep_panels=args.sweep_panels)
     if args.sketch:
         mixed.prec_sketch = {"f32": lgh.PREC_F32, "f64": lgh.PREC_F64}[args.sketch]
+    if args.solve_blocked_min is not None:
+        mixed.solve_blocked_min = args.solve_blocked_min
     plain = lgh.xopts(mixed=False, sweep_panels=args.sweep_panels)
 
     for w in range(args.warmup):
@@ -628,6 +634,7 @@ def main():
                                   info["nit"]),
                    "n": n, "ell": args.ell, "kept": kept_build,
                    "backend": args.backend, "sweep_panels": args.sweep_panels,
+                   "solve_blocked_min": mixed.solve_blocked_min,
                    "vcycle_fp64_sweeps": args.vcycle,
                    "vcycle_fp32_sweeps": args.sketch_vcycle},
         "roofline": roof,

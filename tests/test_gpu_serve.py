@@ -119,5 +119,10 @@ def test_blocked_tier_matches_pcg(dev, prob):
     cs = 10.0 ** rng.uniform(-4, 1, size=9)
     assert rel(gb.solve_block(cs, X), gp.solve_block(cs, X)) < 1e-8
     assert rel(gb.sample_block(cs, X), gp.sample_block(cs, X)) < 1e-8
-    # below the threshold both take the PCG path
-    assert rel(gb.solve_block(cs[:3], X[:, :3]), gp.solve_block(cs[:3], X[:, :3])) <= 1e-14
+    # the mixed policy takes every call through the blocked tier (one column
+    # too); below an explicit threshold the call takes the PCG path
+    assert rel(gb.solve_block(cs[:1], X[:, :1]), gp.solve_block(cs[:1], X[:, :1])) < 1e-8
+    g8 = dev.GLR(prob["B"], prob["prior"], opts,
+                 x=dev.xopts(mixed=True, solve_blocked_min=8))
+    assert rel(g8.solve_block(cs[:3], X[:, :3]), gp.solve_block(cs[:3], X[:, :3])) <= 1e-14
+    assert rel(g8.solve_block(cs, X), gb.solve_block(cs, X)) <= 1e-14
