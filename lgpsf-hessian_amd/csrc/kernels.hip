@@ -471,6 +471,84 @@ spmm_grp_k (int n, int ngroups, int nx, int np, const int *__restrict__ gp,
   }
 }
 
+// Y = alpha A X + beta Y for matrices with long rows (an aggregation
+// hierarchy's restriction to a small coarse level: hundreds of entries a
+// row over few rows, which one lane group per row walks serially).  One
+// block per (row, panel): the row's entries are cut into LCH chunks of
+// equal length, the block's lane groups sum one chunk each (in order) into
+// shared memory, and the chunk sums are added in chunk order.  The
+// chunking depends only on the row, so the result does not depend on the
+// panel width or the storage's lane map.
+static constexpr int LCH = 16;
+template <typename TV, typename T, typename S, int W>
+__global__ void __launch_bounds__ (TPB)
+spmm_long_k (int n, int nx, const int *__restrict__ rp,
+             const int *__restrict__ ci, const TV *__restrict__ v, T alpha,
+             const S *__restrict__ X, T beta, S *__restrict__ Y)
+{
+  constexpr int N = VT<S>::N;
+  constexpr int LPC = W / N;            // lanes per chunk (one panel row)
+  constexpr int G = TPB / LPC;          // lane groups per block
+  __shared__ T part[LCH][W];
+  const int row = blockIdx.x;
+  const int g = (int) threadIdx.x / LPC;
+  const int lc = ((int) threadIdx.x % LPC) * N;
+  const S *Xp = X + (size_t) blockIdx.y * nx * W + lc;
+  const int k0 = rp[row], k1 = rp[row + 1];
+  const int L = (k1 - k0 + LCH - 1) / LCH;
+  for (int s = g; s < LCH; s += G) {
+    const int e0 = min (k1, k0 + s * L), e1 = min (k1, e0 + L);
+    T acc[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) acc[i] = 0;
+    for (int kb = e0; kb < e1; kb += NG) {
+      int c[NG];
+      T w[NG];
+#pragma unroll
+      for (int u = 0; u < NG; u++) {
+        const bool in = kb + u < e1;
+        c[u] = in ? ci[kb + u] : ci[e0];
+        w[u] = in ? (T) v[kb + u] : (T) 0;
+      }
+      typename Raw16<S>::type x[NG];
+#pragma unroll
+      for (int u = 0; u < NG; u++) {
+        if (kb + u < e1) x[u] = rld (Xp + (size_t) c[u] * W);
+        else x[u] = {};
+      }
+#pragma unroll
+      for (int u = 0; u < NG; u++) {
+        T t[N];
+        unpack (x[u], t);
+#pragma unroll
+        for (int i = 0; i < N; i++) acc[i] += w[u] * t[i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) part[s][lc + i] = acc[i];
+  }
+  __syncthreads ();
+  if (g != 0) return;
+  T acc[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) acc[i] = part[0][lc + i];
+  for (int s = 1; s < LCH; s++)
+#pragma unroll
+    for (int i = 0; i < N; i++) acc[i] += part[s][lc + i];
+  S *y = Y + ((size_t) blockIdx.y * n + row) * W + lc;
+  if (beta == (T) 0) {
+#pragma unroll
+    for (int i = 0; i < N; i++) acc[i] *= alpha;
+  }
+  else {
+    T yo[N];
+    vld (y, yo);
+#pragma unroll
+    for (int i = 0; i < N; i++) acc[i] = alpha * acc[i] + beta * yo[i];
+  }
+  vst (y, acc);
+}
+
 // R = B - A X
 template <typename TV, typename T, typename S, int W = 64>
 __global__ void __launch_bounds__ (TPB)
@@ -894,6 +972,17 @@ spmm_t (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
       return 0;
     }
   }
+  if (row_width (X.W) && A.n > 0 && A.nnz >= 64 * (int64_t) A.n)
+    return with_row_W (X.W, [&] (auto wc) -> int {
+      constexpr int W = decltype (wc)::value;
+      hipLaunchKernelGGL ((spmm_long_k<TV, T, S, W>),
+                          dim3 ((unsigned) A.n, (unsigned) X.npanels ()),
+                          dim3 (TPB), 0, stream (), A.n, A.m, A.rp, A.ci,
+                          (const TV *) A.v, (T) alpha, (const S *) X.p,
+                          (T) beta, (S *) Y.p);
+      LGH_LAUNCH_CHECK ();
+      return 0;
+    });
   if (row_width (X.W))
     return with_row_W (X.W, [&] (auto wc) -> int {
       constexpr int W = decltype (wc)::value;

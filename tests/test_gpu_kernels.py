@@ -76,6 +76,27 @@ def test_spmm_wide_rows_width_invariant(dev, prec_vals):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_spmm_long_rows(dev, prec):
+    """Rows of hundreds of entries (a restriction onto a small coarse level)
+    take the chunked kernel: the same bits at panel widths 64 and 16, and
+    scipy's product; an empty row and a one-entry row included."""
+    A = rand_csr(90, 4000, 0.08, 7).tolil()
+    A[3, :] = 0
+    A[5, :] = 0
+    A[5, 17] = 2.5
+    A = A.tocsr()
+    A.eliminate_zeros()
+    assert A.nnz >= 64 * A.shape[0]
+    X = np.random.default_rng(8).standard_normal((4000, 70))
+    y64 = dev.spmm(A, X, prec_vals=prec, prec_vec=prec, W=64)
+    y16 = dev.spmm(A, X, prec_vals=prec, prec_vec=prec, W=16)
+    assert np.array_equal(y64, y16)
+    ref = A @ X
+    assert rel(y64, ref) < (1e-13 if prec == 0 else 1e-5)
+    assert np.all(y64[3] == 0)
+
+
 def test_cheb_jacobi_matches_oracle(dev, oracle):
     Z = W.tpm_Z()
     n = Z.shape[0]
