@@ -766,8 +766,8 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
 #pragma unroll
   for (int i = 0; i < N; i++) r[i] -= acc[i];
   ntst (Dn + qi, di);
-  if (xmode != MIX_X_SKIP) ntst (X + qi, x);
-  ntst (Rn + qi, r);
+  if (xmode != MIX_X_SKIP) vst (X + qi, x);
+  vst (Rn + qi, r);
   ntst (Rpc + qi, r);
 }
 
