@@ -1644,12 +1644,14 @@ xcd_blocks (int ngroups, int nsib)
 }
 
 // Gram partials: block (chunk, group) computes the four 64 x 64 tiles
-// X_px^T Y_py, px in {2 gp, 2 gp + 1}, py in {2 gq, 2 gq + 1}, over rows
-// [r0, r1) (the groups of one chunk are siblings).  Wave w owns tile
-// (2 gp + w / 2, 2 gq + w % 2) as 4 x 4 MFMA 16 x 16 tiles: per k-step of
-// 4 rows it loads 4 A and 4 B fragments and issues 16 MFMAs, and each
-// panel row it reads is shared with one other wave of the block (L1), so
-// a panel chunk is read once per tile group instead of once per tile.
+// X_px^T Y_py of a gx x gy tile group (2 x 2; 4 x 1 or 1 x 4 when Y or X
+// is a single panel), px in gx gp .. gx gp + gx - 1, py in gy gq .. gy gq
+// + gy - 1, over rows [r0, r1) (the groups of one chunk are siblings).
+// Wave w owns tile (gx gp + w / gy, gy gq + w % gy) as 4 x 4 MFMA 16 x 16
+// tiles: per k-step of 4 rows it loads 4 A and 4 B fragments and issues 16
+// MFMAs, and each panel row it reads is shared with the other waves of the
+// block on the same panel (L1), so a panel chunk is read once per tile
+// group instead of once per tile.
 // Tiles past the panels, and in the symmetric case below the diagonal,
 // are skipped; tile_of maps (px, py) to the pair slot of the reduction.
 // MFMA 16x16x4 operand layout: A(i, k) at lane i + 16 k, B(k, j) at lane
@@ -1659,7 +1661,7 @@ __global__ void __launch_bounds__ (TPB)
 gram_k (int n, int nchunks, int ngroups, const T *__restrict__ X, int xp0,
         int npx, const T *__restrict__ Y, int yp0, int npy, int rows_per_chunk,
         const int2 *__restrict__ groups, const int *__restrict__ tile_of,
-        int npairs, double *__restrict__ part)
+        int npairs, double *__restrict__ part, int gy)
 {
   typedef typename mfma<T>::acc_t acc_t;
   int chunk, grp;
@@ -1667,7 +1669,10 @@ gram_k (int n, int nchunks, int ngroups, const T *__restrict__ X, int xp0,
   if (chunk >= nchunks) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int2 gr = groups[grp];
-  const int px = 2 * gr.x + (wave >> 1), py = 2 * gr.y + (wave & 1);
+  // a group is gx x gy tiles (gx gy = 4): 2 x 2, or 4 x 1 / 1 x 4 when one
+  // side is a single panel (no wave left idle)
+  const int gx = 4 / gy;
+  const int px = gx * gr.x + wave / gy, py = gy * gr.y + wave % gy;
   if (px >= npx || py >= npy) return;
   const int slot = tile_of[px * npy + py];
   if (slot < 0) return;
@@ -1768,9 +1773,13 @@ k_gram (const PMat &X, int xp0, int npx, const PMat &Y, int yp0, int npy,
     }
   const int npairs = (int) pv.size ();
   if (npairs == 0) return 0;
+  // group shape: 2 x 2 tiles, or a column / row of 4 when one side has a
+  // single panel (a symmetric Gram has npx == npy)
+  const int gy = (npx == 1 && !sym) ? 4 : (npy == 1 && !sym) ? 1 : 2;
+  const int gx = 4 / gy;
   std::vector<int2> gv;
-  for (int a = 0; a < (npx + 1) / 2; a++)
-    for (int b = (sym ? a : 0); b < (npy + 1) / 2; b++) gv.push_back (make_int2 (a, b));
+  for (int a = 0; a < (npx + gx - 1) / gx; a++)
+    for (int b = (sym ? a : 0); b < (npy + gy - 1) / gy; b++) gv.push_back (make_int2 (a, b));
   const int ngroups = (int) gv.size ();
   const int n = X.nrows;
   ProfScope ps (PK_GRAM, (double) n * 64 * prec_bytes (X.prec) * (npx + npy),
@@ -1798,12 +1807,12 @@ k_gram (const PMat &X, int xp0, int npx, const PMat &Y, int yp0, int npy,
       hipLaunchKernelGGL (gram_k<double>, dim3 (g), dim3 (TPB), 0, stream (),
                           n, nch, ngroups, (const double *) X.p, xp0, npx,
                           (const double *) Y.p, yp0, npy, rpc, pd + npairs, td,
-                          npairs, part);
+                          npairs, part, gy);
     else
       hipLaunchKernelGGL (gram_k<float>, dim3 (g), dim3 (TPB), 0, stream (),
                           n, nch, ngroups, (const float *) X.p, xp0, npx,
                           (const float *) Y.p, yp0, npy, rpc, pd + npairs, td,
-                          npairs, part);
+                          npairs, part, gy);
     hipError_t e = hipGetLastError ();
     if (e != hipSuccess) rc = hip_fail (e, "gram_k", __FILE__, __LINE__);
   }
