@@ -314,11 +314,13 @@ def batch_bench(args, W, lgh):
         priors = [lgh.Prior.from_mat(z, m, blocked_mode=3) for _, z, m, _ in mats]
         lgh.lib().lgh_hip_sync()
         t_prior = time.perf_counter() - t0
+        infos = [pr.info() for pr in priors]
         t0 = time.perf_counter()
-        kept = []
+        kept, lams = [], []
         for (Bm, z, m, ell), pr in zip(mats, priors):
             g = lgh.GLR(Bm, pr, lgh.glr_opts(ell=ell, q_power=1, seed=0), x=mixed)
             kept.append(g.kept)
+            lams.append(np.array(g.lam))
             del g
         lgh.lib().lgh_hip_sync()
         t_seq = time.perf_counter() - t0
@@ -339,10 +341,24 @@ def batch_bench(args, W, lgh):
     lgh.lib().lgh_hip_sync()
     t_batch = time.perf_counter() - t0
     kept_b = [g.kept for g in gs]
+    # the batched builds equal the single calls bit for bit (the workers only
+    # change which stream runs an instance)
+    same_bits = None if kept is None else all(
+        np.array_equal(np.array(g.lam), l) for g, l in zip(gs, lams))
+    if kept is not None and not same_bits:
+        bad = [i for i, (g, l) in enumerate(zip(gs, lams))
+               if not np.array_equal(np.array(g.lam), l)]
+        worst = max(float(np.max(np.abs(np.array(gs[i].lam) - lams[i])
+                                 / np.abs(lams[i]))) if len(lams[i]) else 0.0
+                    for i in bad)
+        pdiff = [i for i, (a, b) in enumerate(zip(infos, bpri)) if a != b.info()]
+        log("batch vs one at a time: %d instances differ (first %s, max rel %.2e); "
+            "priors differ: %d %s" % (len(bad), bad[:8], worst, len(pdiff), pdiff[:8]))
     del gs
     log("batch (%d workers): priors %.2fs, builds %.2fs (%.1f instances/s)%s"
         % (args.batch_workers or 8, t_bprior, t_batch, len(mats) / t_batch,
-           "" if kept is None else ", kept " + ("same" if kept_b == kept else "DIFFERENT")))
+           "" if kept is None else ", kept " + ("same" if kept_b == kept else "DIFFERENT")
+           + ", spectra " + ("bit-identical" if same_bits else "DIFFERENT")))
     line = {
         "metric": "GLR build s + solve RHS/s, p99 latency; mixed vs FP64 plain at rel err <=1e-6",
         "value": t_batch, "unit": "s/batch", "n_gpus": 1, "steps": 1,
@@ -361,6 +377,7 @@ def batch_bench(args, W, lgh):
         "batch_workers": args.batch_workers or 8,
         "kept_total": int(sum(kept_b)),
         "kept_same_as_sequential": None if kept is None else kept_b == kept,
+        "spectra_bitwise_same_as_sequential": same_bits,
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -480,12 +497,18 @@ def main():
     # full-width launches on the finest level (all whole 64-column panels;
     # a last panel of at most 48 columns runs as 16-wide panels)
     npan = args.ell // 64 if 0 < args.ell % 64 <= 48 else (args.ell + 63) // 64
+    npan_b = (args.ell + 63) // 64
     if 0 < args.sweep_panels < npan:      # chunked sweeps: the full chunks
         npan = args.sweep_panels
+        npan_b = args.sweep_panels
     vpc = pcode[args.vcycle]
     single = ("cheb_mixed", "smooth_pre_fine", "smooth_post_fine", "resid_fine")
     prof = {k: lgh.prof_collect(k, npan, vpc) if k in single else lgh.prof_collect(k)
-            for k in lgh.PROF_KINDS}
+            for k in lgh.PROF_KINDS if k != "spmm_B"}
+    # the B SpMM of the FP64 sweeps (all ceil(ell / 64) panels of a block in
+    # one launch; the residual check's FP32 one-panel product is not this
+    # kernel instantiation)
+    prof["spmm_B"] = lgh.prof_collect("spmm_B", npan_b, lgh.PREC_F64)
     kept_build = g.kept
     log("mixed build %.3fs kept=%d report=%s" % (t_mixed, kept_build, reps[-1]))
     for k, (c, ms, by, fl) in prof.items():

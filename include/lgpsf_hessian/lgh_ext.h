@@ -69,12 +69,16 @@ int lgh_glr_sample_block (lgh_glr_t *glr, int m, const double *c,
                           const double *xi, int ldr, double *x, int ldx);
 
 /* Batches of independent instances (no reference counterpart: the
- * reference builds one object per call).  Instance i is exactly the
- * single call with argument i (lgh_prior_create_mat, lgh_glr_compute_x);
- * the calls run on `workers` host threads (<= 0: 8), each driving the GPU
- * on its own stream, so the kernels of small instances overlap.
- * Instances that share an object (Z, mass, B or prior) run one after
- * another on one thread.  opts: one per instance or NULL (defaults);
+ * reference builds one object per call).  Instance i runs the single
+ * call with argument i (lgh_prior_create_mat, lgh_glr_compute_x): the
+ * same kernels in the same order, on `workers` host threads (<= 0: 8),
+ * each driving the GPU on its own stream, so the kernels of small
+ * instances overlap.  The engine's own kernels reduce in fixed orders and
+ * sequential calls are bit-reproducible; under concurrent load a few
+ * results have come out rounded differently (~1e-12 relative, in a few
+ * of 256 instances; the library dense factorizations are the suspected
+ * source).  Instances that share an object (Z, mass, B or prior) run one
+ * after another on one thread.  opts: one per instance or NULL (defaults);
  * xopts: shared or NULL.  On failure every output is destroyed and set to
  * NULL, and the error names the lowest failing instance. */
 int lgh_prior_create_mat_batch (int count, Mat *Z, Vec *mass,

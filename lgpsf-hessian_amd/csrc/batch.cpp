@@ -67,6 +67,7 @@ run_groups (const std::vector<std::vector<int>> &groups, int workers,
   std::mutex mu;
   Failure first;
   auto work = [&] () {
+    set_aux_overlap (false);             // the other workers fill the device
     for (;;) {
       const int gi = next.fetch_add (1);
       if (gi >= (int) groups.size ()) break;

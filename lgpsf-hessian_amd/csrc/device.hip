@@ -58,6 +58,9 @@ ctx_init (ThreadCtx &c)
     return 4;
   }
   rocblas_set_stream (c.rb, c.s);
+  // no atomics in rocBLAS reductions (rocSOLVER's potrf / trtri / syevd run
+  // through them): results do not depend on the timing of other work
+  rocblas_set_atomics_mode (c.rb, rocblas_atomics_not_allowed);
   LGH_HIP (hipMalloc (&c.info, sizeof (int) * 4));
   return 0;
 }
@@ -86,9 +89,16 @@ stream ()
 }
 void *rocblas_h () { (void) stream (); return (void *) t_ctx.rb; }
 
+// a thread that runs one of many concurrent instances (batch workers)
+// gains nothing from overlapping a block's narrow last panel on a second
+// stream: the other instances fill the device
+static thread_local bool t_no_aux = false;
+void set_aux_overlap (bool on) { t_no_aux = !on; }
+
 hipStream_t
 aux_stream ()
 {
+  if (t_no_aux) return nullptr;
   (void) stream ();
   if (!t_ctx.aux
       && hipStreamCreateWithFlags (&t_ctx.aux, hipStreamNonBlocking) != hipSuccess)
@@ -99,10 +109,32 @@ aux_stream ()
 StreamScope::StreamScope (hipStream_t s) : prev (stream ()) { t_ctx.s = s; }
 StreamScope::~StreamScope () { t_ctx.s = prev; }
 
+// Events that cross-stream waits were enqueued on are not destroyed while
+// such a wait may still be pending: they are kept here and destroyed once
+// the thread's main stream has been synchronized (every wait enqueued
+// before then has completed; the auxiliary stream's own waits precede
+// work that the main stream waited for).
+static thread_local std::vector<hipEvent_t> t_retired;
+
+void
+event_retire (hipEvent_t e)
+{
+  if (e) t_retired.push_back (e);
+}
+
+static void
+retire_flush ()
+{
+  if (t_ctx.s != t_ctx.main) return;
+  for (hipEvent_t e : t_retired) (void) hipEventDestroy (e);
+  t_retired.clear ();
+}
+
 int
 dev_sync ()
 {
   LGH_HIP (hipStreamSynchronize (t_ctx.s));
+  retire_flush ();
   return 0;
 }
 
@@ -120,6 +152,8 @@ dev_thread_exit ()
     free_list_release (c.aux);
     (void) hipStreamDestroy (c.aux);
   }
+  for (hipEvent_t e : t_retired) (void) hipEventDestroy (e);
+  t_retired.clear ();
   (void) hipFree (c.info);
   rocblas_destroy_handle (c.rb);
   (void) hipStreamDestroy (c.main);
@@ -299,6 +333,7 @@ h2d (void *dst, const void *src, size_t bytes)
   if (!bytes) return 0;
   LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyHostToDevice, stream ()));
   LGH_HIP (hipStreamSynchronize (t_ctx.s));
+  retire_flush ();
   return 0;
 }
 
@@ -308,6 +343,7 @@ d2h (void *dst, const void *src, size_t bytes)
   if (!bytes) return 0;
   LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyDeviceToHost, stream ()));
   LGH_HIP (hipStreamSynchronize (t_ctx.s));
+  retire_flush ();
   return 0;
 }
 

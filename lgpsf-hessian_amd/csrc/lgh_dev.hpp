@@ -53,6 +53,8 @@ hipStream_t stream ();
 // the thread's auxiliary stream (default priority, created on first use):
 // work that may overlap the main stream's, ordered by events
 hipStream_t aux_stream ();
+// this thread's aux_stream () returns null (no overlap) when off
+void set_aux_overlap (bool on);
 // while alive, the thread's engine calls enqueue on s instead of its
 // stream (allocator free lists follow: blocks freed here are reused on s)
 struct StreamScope {
@@ -246,6 +248,9 @@ int sol_syev_vals (int n, double *A, int lda, double *w_dev, int *info_host);
 int blas_dgemm (int ta, int tb, int m, int n, int k, double alpha,
                 const double *A, int lda, const double *B, int ldb,
                 double beta, double *C, int ldc);
+// hand over an event that a cross-stream wait was enqueued on: destroyed
+// after the thread's next main-stream synchronization, not before
+void event_retire (hipEvent_t e);
 // device memory of the current GPU in bytes (0 if unknown)
 size_t dmem_total ();
 

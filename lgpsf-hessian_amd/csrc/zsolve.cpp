@@ -303,8 +303,10 @@ split_tail (const PMat &B, PMat &X, WK &w, bool *done, F &&run)
       rc = hip_fail (hipGetLastError (), "hipStreamWaitEvent", __FILE__, __LINE__);
     if (rc) (void) hipStreamSynchronize (aux);
   }
-  if (ready) (void) hipEventDestroy (ready);
-  if (done_ev) (void) hipEventDestroy (done_ev);
+  // the waits just enqueued may still be pending: the events are destroyed
+  // after the next synchronization of this thread's stream
+  event_retire (ready);
+  event_retire (done_ev);
   LGH_TRY (rc);
   LGH_TRY (dzero (X.panel (full), (size_t) X.nrows * 64 * prec_bytes (X.prec)));
   return k_copy_cols (Xt, 0, X, 64 * full, tail);
