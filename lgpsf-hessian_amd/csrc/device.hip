@@ -10,6 +10,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <map>
 #include <mutex>
@@ -567,13 +568,15 @@ blas_dgemm (int ta, int tb, int m, int n, int k, double alpha, const double *A,
 size_t
 dmem_total ()
 {
-  static size_t total = 0;
-  if (!total) {
-    size_t fr = 0, tot = 0;
+  static std::atomic<size_t> total (0);
+  size_t t = total.load ();
+  if (!t) {
+    size_t fr = 0;
     (void) stream ();
-    if (hipMemGetInfo (&fr, &tot) == hipSuccess) total = tot;
+    if (hipMemGetInfo (&fr, &t) != hipSuccess) t = 0;
+    total.store (t);
   }
-  return total;
+  return t;
 }
 
 }  // namespace lgh
