@@ -24,6 +24,10 @@ def problems():
     for k, side in enumerate((16, 20, 24, 28, 22, 18)):
         Bs, Zs, M = W.psf2d_problem(side)
         out.append((Bs, Zs, M, 12 + 4 * k))
+    # a block with a whole panel and a narrow last one (ell = 74: 64 + 10),
+    # which a batch worker solves on its one stream
+    Bs, Zs, M = W.psf2d_problem(30)
+    out.append((Bs, Zs, M, 74))
     return out
 
 
