@@ -360,7 +360,7 @@ dcopy (void *dst, const void *src, size_t bytes)
 DevCSR::~DevCSR ()
 {
   dfree (rp); dfree (ci); dfree (v); dfree (dinv);
-  dfree (gp); dfree (gci); dfree (gv);
+  dfree (gp); dfree (gci); dfree (gv); dfree (vs);
 }
 
 // Each group's union is the merge of its (sorted) rows; a row's entries
@@ -441,6 +441,52 @@ csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v)
   return 0;
 }
 
+// 1 / diagonal of a square host CSR (duplicates summed)
+static std::vector<double>
+host_dinv (int n, const int *rp, const int *ci, const double *v)
+{
+  std::vector<double> d ((size_t) n, 0.0);
+  for (int i = 0; i < n; i++)
+    for (int k = rp[i]; k < rp[i + 1]; k++)
+      if (ci[k] == i) d[i] += v[k];
+  for (int i = 0; i < n; i++) d[i] = 1.0 / d[i];
+  return d;
+}
+
+// The products are formed on the host in the device copy's precision from
+// the values and dinv exactly as uploaded (IEEE products, round to nearest,
+// as the device rounds them), so a kernel that reads vs[k] sees the bits it
+// would compute as v[k] * (b1 * dinv[ci[k]]).
+int
+csr_scale_cols (DevCSR &A, const int *rp, const int *ci, const double *v,
+                double b1)
+{
+  LGH_CHECK (A.dinv && A.n == A.m && (A.prec == F64 || A.prec == F32), 8,
+             "csr_scale_cols: needs a square F64/F32 matrix with dinv");
+  const std::vector<double> d = host_dinv (A.n, rp, ci, v);
+  const int64_t nnz = A.nnz;
+  void *p = dmalloc (prec_bytes (A.prec) * (size_t) (nnz ? nnz : 1));
+  if (!p) return 5;
+  int rc;
+  if (A.prec == F64) {
+    std::vector<double> s ((size_t) nnz);
+    for (int64_t k = 0; k < nnz; k++) s[k] = v[k] * (b1 * d[ci[k]]);
+    rc = h2d (p, s.data (), sizeof (double) * (size_t) nnz);
+  }
+  else {
+    const float bf = (float) b1;
+    std::vector<float> s ((size_t) nnz);
+    for (int64_t k = 0; k < nnz; k++)
+      s[k] = (float) v[k] * (bf * (float) d[ci[k]]);
+    rc = h2d (p, s.data (), sizeof (float) * (size_t) nnz);
+  }
+  if (rc) { dfree (p); return rc; }
+  dfree (A.vs);
+  A.vs = p;
+  A.vs_b1 = b1;
+  return 0;
+}
+
 int
 csr_upload (int n, int m, const int *rp, const int *ci, const double *v,
             int prec, int want_dinv, DevCSR **out)
@@ -465,11 +511,7 @@ csr_upload (int n, int m, const int *rp, const int *ci, const double *v,
     }
   }
   if (!rc && want_dinv && n == m) {
-    std::vector<double> d ((size_t) n, 0.0);
-    for (int i = 0; i < n; i++)
-      for (int k = rp[i]; k < rp[i + 1]; k++)
-        if (ci[k] == i) d[i] += v[k];
-    for (int i = 0; i < n; i++) d[i] = 1.0 / d[i];
+    const std::vector<double> d = host_dinv (n, rp, ci, v);
     A->dinv = dmalloc (vb * (size_t) n);
     if (!A->dinv) rc = 5;
     else if (prec == F64) rc = h2d (A->dinv, d.data (), sizeof (double) * n);

@@ -837,8 +837,11 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
 // in one pass.  X2 = X0 + D1 + D2 needs R0 only at
 // radius 1: R1(i) = R0(i) - sum_j A_ij D1(j) with D1(j) = b1 dinv_j R0(j),
 // then D2(i) = a2 D1(i) + b2 dinv_i R1(i).  Same arithmetic as sm1_k +
-// sm2_k; D1, R1 and R2 are never stored.
-template <typename TV, typename T, typename S, bool XADD, int W = 64>
+// sm2_k; D1, R1 and R2 are never stored.  SC: v holds the level's scaled
+// values A_ij (b1 dinv_j), rounded as the product below rounds them, so
+// the neighbours' dinv is not gathered and the result is the same bits.
+template <typename TV, typename T, typename S, bool XADD, int W = 64,
+          bool SC = false>
 __global__ void __launch_bounds__ (TPB)
 smx_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
        const TV *__restrict__ v, const TV *__restrict__ dinv, T b1, T a2,
@@ -858,7 +861,8 @@ smx_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
       w[u] = in ? (T) v[kb + u] : (T) 0;
     }
 #pragma unroll
-    for (int u = 0; u < NG; u++) dd[u] = w[u] * (b1 * (T) dinv[c[u]]);
+    for (int u = 0; u < NG; u++)
+      dd[u] = SC ? w[u] : w[u] * (b1 * (T) dinv[c[u]]);
     typename Raw16<S>::type x[NG];
 #pragma unroll
     for (int u = 0; u < NG; u++) {
@@ -911,14 +915,20 @@ k_smooth2x (const DevCSR &A, double b1, double a2, double b2, const PMat &R0,
     using T = decltype (t);
     using S = decltype (st);
     constexpr int W = decltype (wc)::value;
-    if (xadd)
-      hipLaunchKernelGGL ((smx_k<TV, T, S, true, W>), g, dim3 (TPB), 0, stream (),
-                          A.n, A.rp, A.ci, (const TV *) A.v, (const TV *) A.dinv,
+    // the scaled values apply when they were made for this b1 and the
+    // arithmetic is in their precision
+    const bool sc = std::is_same<TV, T>::value && A.vs && A.vs_b1 == b1;
+    const TV *vv = (const TV *) (sc ? A.vs : A.v);
+    auto launch = [&] (auto xa, auto scc) {
+      constexpr bool XA = decltype (xa)::value, SCC = decltype (scc)::value;
+      hipLaunchKernelGGL ((smx_k<TV, T, S, XA, W, SCC>), g, dim3 (TPB), 0,
+                          stream (), A.n, A.rp, A.ci, vv, (const TV *) A.dinv,
                           (T) b1, (T) a2, (T) b2, (const S *) R0.p, (S *) X.p);
-    else
-      hipLaunchKernelGGL ((smx_k<TV, T, S, false, W>), g, dim3 (TPB), 0, stream (),
-                          A.n, A.rp, A.ci, (const TV *) A.v, (const TV *) A.dinv,
-                          (T) b1, (T) a2, (T) b2, (const S *) R0.p, (S *) X.p);
+    };
+    using yes = std::true_type;
+    using no = std::false_type;
+    if (xadd) { if (sc) launch (yes (), yes ()); else launch (yes (), no ()); }
+    else { if (sc) launch (no (), yes ()); else launch (no (), no ()); }
     LGH_LAUNCH_CHECK ();
     return 0;
   };

@@ -134,6 +134,10 @@ struct DevCSR {
   int *gp = nullptr, *gci = nullptr;
   void *gv = nullptr;
   int64_t nunion = 0;
+  // scaled values (csr_scale_cols): vs[k] = v[k] * (vs_b1 * dinv[ci[k]]),
+  // each product rounded in precision `prec`
+  void *vs = nullptr;
+  double vs_b1 = 0.0;
   ~DevCSR ();
 };
 // upload host CSR (values fp64 on the host) at precision prec
@@ -143,6 +147,10 @@ int csr_upload (int n, int m, const int *rp, const int *ci, const double *v,
 // must hold strictly increasing column indices, else nothing is added
 // (returns 0 either way unless a device call fails)
 int csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v);
+// add the scaled values for the factor b1 to an uploaded square matrix with
+// its dinv (same host CSR)
+int csr_scale_cols (DevCSR &A, const int *rp, const int *ci, const double *v,
+                    double b1);
 
 struct PMat {
   void *p = nullptr;

@@ -31,6 +31,27 @@ DevMG::free_work ()
   work[1].free ();
 }
 
+// Chebyshev recurrence scalars of zsolve_impl.h:113-138
+struct ChebCoef {
+  double theta, delta, sigma1, rho_old;
+  ChebCoef (double lmin, double lmax)
+  {
+    theta = 0.5 * (lmax + lmin);
+    delta = 0.5 * (lmax - lmin);
+    sigma1 = theta / delta;
+    rho_old = 1.0 / sigma1;
+  }
+  // (a, b) for step k (k = 1: a = 0, b = 1/theta)
+  void step (int k, double *a, double *b)
+  {
+    if (k == 1) { *a = 0.0; *b = 1.0 / theta; return; }
+    const double rho = 1.0 / (2.0 * sigma1 - rho_old);
+    *a = rho * rho_old;
+    *b = 2.0 * rho / delta;
+    rho_old = rho;
+  }
+};
+
 // the one-pass degree-2 smoother and explicit residuals serve the row
 // kernels' widths
 static inline bool
@@ -61,6 +82,13 @@ mg_upload (const AmgHierarchy &h, int prec, DevMG **out)
     g->nu.push_back (lv.nu);
     rc = csr_upload (lv.A.n, lv.A.m, lv.A.rp.data (), lv.A.ci.data (),
                      lv.A.v.data (), mprec, 1, &g->A[l]);
+    if (!rc && lv.nu == 2 && l > 0) {
+      // the one-pass smoother's first-step factor of this level
+      double a1, b1;
+      ChebCoef (lv.slo, lv.shi).step (1, &a1, &b1);
+      rc = csr_scale_cols (*g->A[l], lv.A.rp.data (), lv.A.ci.data (),
+                           lv.A.v.data (), b1);
+    }
     if (!rc && l > 0) {
       rc = csr_upload (lv.P.n, lv.P.m, lv.P.rp.data (), lv.P.ci.data (),
                        lv.P.v.data (), mprec, 0, &g->P[l]);
@@ -116,27 +144,6 @@ mg_work (DevMG &g, const PMat &like, DevMG::Work **out)
   w.np = np;
   return 0;
 }
-
-// Chebyshev recurrence scalars of zsolve_impl.h:113-138
-struct ChebCoef {
-  double theta, delta, sigma1, rho_old;
-  ChebCoef (double lmin, double lmax)
-  {
-    theta = 0.5 * (lmax + lmin);
-    delta = 0.5 * (lmax - lmin);
-    sigma1 = theta / delta;
-    rho_old = 1.0 / sigma1;
-  }
-  // (a, b) for step k (k = 1: a = 0, b = 1/theta)
-  void step (int k, double *a, double *b)
-  {
-    if (k == 1) { *a = 0.0; *b = 1.0 / theta; return; }
-    const double rho = 1.0 / (2.0 * sigma1 - rho_old);
-    *a = rho * rho_old;
-    *b = 2.0 * rho / delta;
-    rho_old = rho;
-  }
-};
 
 // correction-form smoothing of level l (zsolve_impl.h:385-404).
 // x_zero: X is zero on entry (pre-smoothing), so R = B without a product.
