@@ -82,12 +82,13 @@ def cpu_baseline(B, Z, M, lgh, ell, q, kept, budget_s, gamma=1):
     return mg, levels
 
 
-def serve_bench(lgh, g, n, nreq, pool, seed=5):
+def serve_bench(lgh, g, n, nreq, pool, seed=5, rate=None):
     """Request loop on the resident mixed GLR: 60% solve / 35% sample / 5%
     logdet, c log-uniform in [1e-4, 10], RHS count geometric (mean 8)
     truncated to [1, 64], entries from a pool of hashed-Gaussian columns.
     First a burst of nreq/3 requests measures saturation, then nreq
-    requests arrive as a Poisson process at 0.7 x the saturated rate."""
+    requests arrive as a Poisson process at 0.7 x the saturated rate (or
+    at `rate` requests/s when given)."""
     import threading
     rng = np.random.default_rng(seed)
 
@@ -150,7 +151,7 @@ def serve_bench(lgh, g, n, nreq, pool, seed=5):
     burst = make(max(nreq // 3, 16))
     t_b, lat_b, cols_b = run(burst, [0.0] * len(burst))
     sat_rps = len(burst) / t_b
-    rate = 0.7 * sat_rps
+    rate = 0.7 * sat_rps if rate is None else rate
     reqs = make(nreq)
     arr = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
     t_p, lat_p, cols_p = run(reqs, list(arr))
@@ -163,7 +164,8 @@ def serve_bench(lgh, g, n, nreq, pool, seed=5):
             "latency_p50_s": float(np.percentile(lat_p, 50)),
             "latency_p99_s": float(np.percentile(lat_p, 99)),
             "batch_cols": st["batch_cols"], "batches": st["batches"],
-            "mean_batch_cols": st["columns"] / max(st["batches"], 1)}
+            "mean_batch_cols": st["columns"] / max(st["batches"], 1),
+            "worker_busy_s": st["busy_s"], "worker_stage_s": st["stage_s"]}
 
 
 def dominant_roofline(lgh, args, prof, npan, vpc):
@@ -403,6 +405,9 @@ def main():
                          "mixed policy's)")
     ap.add_argument("--extend", type=int, default=200,
                     help="columns of the timed lgh_glr_extend (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=None,
+                    help="Poisson arrival rate of the serving run in requests/s "
+                         "(default 0.7 x the measured saturation)")
     ap.add_argument("--serve-requests", type=int, default=300,
                     help="requests of the Poisson serving run (0: skip)")
     ap.add_argument("--workload", choices=("grid2d", "grid3d", "batch"),
@@ -598,7 +603,8 @@ def main():
         (m, m / t_blk, extra["solve_latency_p50_s"]))
     del X
     if args.serve_requests > 0:
-        extra["serve"] = serve_bench(lgh, g, n, args.serve_requests, R)
+        extra["serve"] = serve_bench(lgh, g, n, args.serve_requests, R,
+                                     rate=args.serve_rate)
         log("serve:", extra["serve"])
 
     # incremental extension of the resident mixed GLR by 200 columns

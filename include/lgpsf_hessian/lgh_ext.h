@@ -113,10 +113,13 @@ int lgh_prior_solve_host (lgh_prior_t *prior, int which, int prec, int m,
 
 /* Request loop over one resident GLR: callers on any thread submit solve /
  * sample / logdet requests (shift c, m right-hand sides, column-major host
- * blocks that stay valid until the wait returns); one worker thread forms
- * batches of the oldest request's operation from everything queued (each
- * column with its own shift), so requests that arrive while a batch runs
- * join the next one.  The batch width is capped by a device workspace
+ * blocks that stay valid until the wait returns); batches of the oldest
+ * request's operation are formed from everything queued (each column with
+ * its own shift) whenever one of two pinned staging blocks is free, so
+ * requests that arrive after a batch is formed join the next one.  One
+ * thread copies a batch's columns in, one runs it on the device, one copies
+ * the results out: the copies of one batch overlap the device work of the
+ * other.  The batch width is capped by a device workspace
  * budget; submit blocks while more than queue_cols columns are queued.
  * A request's result does not depend on what it was batched with.  While a
  * server exists, the GLR must not be used or destroyed by other calls.
@@ -141,7 +144,10 @@ typedef struct lgh_serve_stats
   int64_t batches, columns, requests;
   int     max_batch_cols;   /* widest batch run so far                      */
   int     batch_cols;       /* the admitted batch width                      */
-  double  busy_s;           /* worker time spent in batches                  */
+  double  busy_s;           /* device thread's time in batches              */
+  double  stage_s;          /* host copies between the callers' columns and
+                               the pinned staging blocks (staging threads,
+                               overlapping the device work)                 */
 }
 lgh_serve_stats_t;
 

@@ -79,7 +79,8 @@ class ServeOpts(C.Structure):
 class ServeStats(C.Structure):
     _fields_ = [("batches", C.c_int64), ("columns", C.c_int64),
                 ("requests", C.c_int64), ("max_batch_cols", C.c_int),
-                ("batch_cols", C.c_int), ("busy_s", C.c_double)]
+                ("batch_cols", C.c_int), ("busy_s", C.c_double),
+                ("stage_s", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

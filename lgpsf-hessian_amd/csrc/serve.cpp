@@ -3,19 +3,24 @@
 // glr_common_impl.h:1007-1104).
 //
 // Callers submit solve / sample / logdet requests (a shift c and m
-// right-hand sides each) from any thread; one worker thread owns the GLR
-// and the device.  Whenever the device is free it forms the next batch from
-// everything queued: all requests of the oldest request's operation that
-// fit in the batch's columns, each column with its own shift.  Requests
-// that arrive while a batch runs wait for the next one (continuous
-// batching at batch granularity).  The batch width is capped by a device
-// workspace budget (memory-based admission); submit blocks while the
-// queued columns exceed a host-side budget (back-pressure).  Every column
-// of a batch is computed independently of the others (column-wise kernels,
-// per-column shifts and coefficients), so a request's result does not
-// depend on what it was batched with.
+// right-hand sides each) from any thread.  Three threads serve them as a
+// pipeline over two pinned staging slots: the stager forms a batch from
+// everything queued whenever a slot is free (all requests of the oldest
+// request's operation that fit in the batch's columns, each column with its
+// own shift) and copies its columns into the slot; the device thread, which
+// alone owns the GLR and the device, runs the batch in place in the slot;
+// the finisher copies the results out to the callers and frees the slot.
+// So the host copies of one batch overlap the device work of the other.
+// Requests that arrive after a batch is formed wait for the next one
+// (continuous batching at batch granularity).  The batch width is capped by
+// a device workspace budget (memory-based admission); submit blocks while
+// the queued columns exceed a host-side budget (back-pressure).  Every
+// column of a batch is computed independently of the others (column-wise
+// kernels, per-column shifts and coefficients, a fixed panel width), so a
+// request's result does not depend on what it was batched with.
 #include "engine.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -45,19 +50,34 @@ struct Req {
 
 }  // namespace
 
+// one pinned staging block and the batch it holds
+struct Slot {
+  double *pin = nullptr;          // n x max_cols, inputs then results
+  std::vector<Req *> batch;
+  int cols = 0, rc = 0;
+};
+
 struct lgh_serve {
   lgh_glr_t *g = nullptr;
   lgh_serve_opts_t o;
   int max_cols = 64;              // columns per device batch (multiple of 64)
   std::mutex mu;
-  std::condition_variable cv_work, cv_done, cv_space;
+  std::condition_variable cv_work, cv_done, cv_space, cv_ready, cv_drain;
   std::deque<Req *> queue;
   std::map<int64_t, Req *> reqs;
   int64_t next_id = 1;
   int queued_cols = 0;
   bool stop = false;
-  std::thread worker;
-  double *pin_in = nullptr, *pin_out = nullptr;   // n x max_cols each
+  std::thread stager, device, finisher;
+  Slot slot[2];
+  // slot ids: free, staged (to run), run (to drain); -1 ends a thread
+  std::deque<int> free_slots, ready, drain;
+  // the running batch (start, predicted seconds) and moving averages of
+  // the device seconds per 64-column panel (per operation) and of the
+  // copy-in seconds per column, which time the next batch's formation
+  bool dev_busy = false;
+  clk::time_point dev_t0;
+  double dev_pred = 0.0, ema_panel[2] = { 0.0, 0.0 }, ema_col = 0.0;
   lgh_serve_stats_t st;
   int tier = 0;
 };
@@ -78,80 +98,186 @@ copy_cols_par (int n, int m, const double *const *src, double *const *dst)
   for (auto &x : th) x.join ();
 }
 
-static void
-run_batch (lgh_serve *s, std::vector<Req *> &batch)
+static double
+since (clk::time_point t0)
 {
-  const int n = s->g->n;
-  int cols = 0;
-  for (Req *r : batch) cols += r->m;
-  std::vector<const double *> src (cols);
-  std::vector<double *> dst (cols);
-  std::vector<double> c (cols);
-  int j = 0;
-  for (Req *r : batch)
-    for (int q = 0; q < r->m; q++, j++) {
-      src[j] = r->in + (size_t) q * r->ldi;
-      dst[j] = s->pin_in + (size_t) j * n;
-      c[j] = r->c;
-    }
-  copy_cols_par (n, cols, src.data (), dst.data ());
-  const int rc = glr_run_block (s->g, batch[0]->op == LGH_SERVE_SOLVE ? 0 : 3,
-                                cols, c.data (), s->pin_in, n, s->pin_out, n,
-                                s->tier);
-  if (!rc) {
-    j = 0;
-    for (Req *r : batch)
-      for (int q = 0; q < r->m; q++, j++) {
-        src[j] = s->pin_out + (size_t) j * n;
-        dst[j] = r->out + (size_t) q * r->ldo;
-      }
-    copy_cols_par (n, cols, src.data (), dst.data ());
-  }
-  const clk::time_point t = clk::now ();
-  std::lock_guard<std::mutex> lk (s->mu);
-  for (Req *r : batch) {
-    r->rc = rc;
-    r->done = 1;
-    r->t_done = t;
-    s->queued_cols -= r->m;
-  }
-  s->st.batches++;
-  s->st.columns += cols;
-  s->st.requests += (int64_t) batch.size ();
-  if (cols > s->st.max_batch_cols) s->st.max_batch_cols = cols;
-  s->cv_done.notify_all ();
-  s->cv_space.notify_all ();
+  return std::chrono::duration<double> (clk::now () - t0).count ();
 }
 
 static void
-worker_loop (lgh_serve *s)
+ema (double &e, double x)
 {
-  (void) hipSetDevice (0);
+  e = (e == 0.0) ? x : 0.7 * e + 0.3 * x;
+}
+
+// Forms batches into free slots and copies their inputs in.  A batch is
+// formed when the device is idle, when a full batch is queued, or, while
+// the device runs a batch, once the copy-in of what is queued would end
+// about when that batch does: as late as keeps the device fed, so that
+// requests arriving meanwhile still join it.
+static void
+stager_loop (lgh_serve *s)
+{
+  const int n = s->g->n;
   for (;;) {
-    std::vector<Req *> batch;
+    int id;
+    Slot *sl;
     {
       std::unique_lock<std::mutex> lk (s->mu);
-      s->cv_work.wait (lk, [&] { return s->stop || !s->queue.empty (); });
-      if (s->queue.empty ()) break;       // stop requested, nothing queued
+      for (;;) {
+        if (s->stop && s->queue.empty ()) {   // stop requested, nothing queued
+          s->ready.push_back (-1);
+          s->cv_ready.notify_one ();
+          return;
+        }
+        if (!s->queue.empty () && !s->free_slots.empty () && s->ready.empty ()) {
+          int waiting = 0;
+          for (const Req *r : s->queue) waiting += r->m;
+          if (!s->dev_busy || waiting >= s->max_cols || s->stop) break;
+          if (s->dev_pred > 0.0) {
+            const double lead = s->ema_col * std::min (waiting, s->max_cols);
+            const clk::time_point at = s->dev_t0
+              + std::chrono::duration_cast<clk::duration> (
+                  std::chrono::duration<double> (s->dev_pred - lead));
+            if (clk::now () >= at) break;
+            s->cv_work.wait_until (lk, at);
+            continue;
+          }
+        }
+        s->cv_work.wait (lk);
+      }
+      id = s->free_slots.front ();
+      s->free_slots.pop_front ();
+      sl = &s->slot[id];
+      sl->batch.clear ();
       const int op = s->queue.front ()->op;
       int cols = 0;
       for (auto it = s->queue.begin (); it != s->queue.end ();) {
         Req *r = *it;
         if (r->op == op && cols + r->m <= s->max_cols) {
-          batch.push_back (r);
+          sl->batch.push_back (r);
           cols += r->m;
           it = s->queue.erase (it);
           if (cols == s->max_cols) break;
         }
         else ++it;
       }
+      sl->cols = cols;
     }
+    std::vector<const double *> src (sl->cols);
+    std::vector<double *> dst (sl->cols);
+    int j = 0;
+    for (Req *r : sl->batch)
+      for (int q = 0; q < r->m; q++, j++) {
+        src[j] = r->in + (size_t) q * r->ldi;
+        dst[j] = sl->pin + (size_t) j * n;
+      }
     const clk::time_point t0 = clk::now ();
-    run_batch (s, batch);
+    copy_cols_par (n, sl->cols, src.data (), dst.data ());
+    const double t = since (t0);
     std::lock_guard<std::mutex> lk (s->mu);
-    s->st.busy_s += std::chrono::duration<double> (clk::now () - t0).count ();
+    s->st.stage_s += t;
+    ema (s->ema_col, t / sl->cols);
+    s->ready.push_back (id);
+    s->cv_ready.notify_one ();
   }
-  dev_thread_exit ();     // the worker's stream and cached blocks
+}
+
+// runs the staged batches on the device, in place in their slots
+static void
+device_loop (lgh_serve *s)
+{
+  (void) hipSetDevice (0);
+  const int n = s->g->n;
+  for (;;) {
+    int id;
+    {
+      std::unique_lock<std::mutex> lk (s->mu);
+      s->cv_ready.wait (lk, [&] { return !s->ready.empty (); });
+      id = s->ready.front ();
+      s->ready.pop_front ();
+    }
+    if (id < 0) break;
+    Slot &sl = s->slot[id];
+    std::vector<double> c (sl.cols);
+    int j = 0;
+    for (Req *r : sl.batch)
+      for (int q = 0; q < r->m; q++, j++) c[j] = r->c;
+    const int kind = sl.batch[0]->op == LGH_SERVE_SOLVE ? 0 : 1;
+    const int panels = (sl.cols + 63) / 64;
+    const clk::time_point t0 = clk::now ();
+    {
+      std::lock_guard<std::mutex> lk (s->mu);
+      s->dev_busy = true;
+      s->dev_t0 = t0;
+      s->dev_pred = s->ema_panel[kind] * panels;
+      s->cv_work.notify_one ();
+    }
+    sl.rc = glr_run_block (s->g, kind == 0 ? 0 : 3, sl.cols, c.data (),
+                           sl.pin, n, sl.pin, n, s->tier);
+    const double t = since (t0);
+    std::lock_guard<std::mutex> lk (s->mu);
+    s->st.busy_s += t;
+    if (!sl.rc) ema (s->ema_panel[kind], t / panels);
+    s->dev_busy = false;
+    s->drain.push_back (id);
+    s->cv_drain.notify_one ();
+    s->cv_work.notify_one ();
+  }
+  dev_thread_exit ();     // the device thread's stream and cached blocks
+  std::lock_guard<std::mutex> lk (s->mu);
+  s->drain.push_back (-1);
+  s->cv_drain.notify_one ();
+}
+
+// copies the results out, completes the requests and frees the slots
+static void
+finisher_loop (lgh_serve *s)
+{
+  const int n = s->g->n;
+  for (;;) {
+    int id;
+    {
+      std::unique_lock<std::mutex> lk (s->mu);
+      s->cv_drain.wait (lk, [&] { return !s->drain.empty (); });
+      id = s->drain.front ();
+      s->drain.pop_front ();
+    }
+    if (id < 0) return;
+    Slot &sl = s->slot[id];
+    double t = 0.0;
+    if (!sl.rc) {
+      std::vector<const double *> src (sl.cols);
+      std::vector<double *> dst (sl.cols);
+      int j = 0;
+      for (Req *r : sl.batch)
+        for (int q = 0; q < r->m; q++, j++) {
+          src[j] = sl.pin + (size_t) j * n;
+          dst[j] = r->out + (size_t) q * r->ldo;
+        }
+      const clk::time_point t0 = clk::now ();
+      copy_cols_par (n, sl.cols, src.data (), dst.data ());
+      t = since (t0);
+    }
+    const clk::time_point td = clk::now ();
+    std::lock_guard<std::mutex> lk (s->mu);
+    for (Req *r : sl.batch) {
+      r->rc = sl.rc;
+      r->done = 1;
+      r->t_done = td;
+      s->queued_cols -= r->m;
+    }
+    s->st.stage_s += t;
+    s->st.batches++;
+    s->st.columns += sl.cols;
+    s->st.requests += (int64_t) sl.batch.size ();
+    if (sl.cols > s->st.max_batch_cols) s->st.max_batch_cols = sl.cols;
+    sl.batch.clear ();
+    s->free_slots.push_back (id);
+    s->cv_done.notify_all ();
+    s->cv_space.notify_all ();
+    s->cv_work.notify_one ();
+  }
 }
 
 extern "C" lgh_serve_opts_t
@@ -198,14 +324,17 @@ lgh_serve_create (lgh_glr_t *g, const lgh_serve_opts_t *opts, lgh_serve_t **out)
   // batch, whatever its width, takes the same path
   s->tier = g->x.solve_blocked_min > 0 ? 1 : 0;
   const size_t bytes = sizeof (double) * (size_t) g->n * cols;
-  if (hipHostMalloc ((void **) &s->pin_in, bytes, hipHostMallocDefault) != hipSuccess
-      || hipHostMalloc ((void **) &s->pin_out, bytes, hipHostMallocDefault) != hipSuccess) {
-    if (s->pin_in) (void) hipHostFree (s->pin_in);
+  if (hipHostMalloc ((void **) &s->slot[0].pin, bytes, hipHostMallocDefault) != hipSuccess
+      || hipHostMalloc ((void **) &s->slot[1].pin, bytes, hipHostMallocDefault) != hipSuccess) {
+    if (s->slot[0].pin) (void) hipHostFree (s->slot[0].pin);
     delete s;
     return lgh_petsc_error (PETSC_ERR_MEM, "lgh_serve_create: pinned staging "
                             "(%.1f GB) failed", 2.0 * bytes / 1e9);
   }
-  s->worker = std::thread (worker_loop, s);
+  s->free_slots = { 0, 1 };
+  s->finisher = std::thread (finisher_loop, s);
+  s->device = std::thread (device_loop, s);
+  s->stager = std::thread (stager_loop, s);
   *out = s;
   return 0;
 }
@@ -303,9 +432,13 @@ lgh_serve_destroy (lgh_serve_t *s)
     s->stop = true;
   }
   s->cv_work.notify_all ();
-  if (s->worker.joinable ()) s->worker.join ();
+  // the stager ends once the queue is empty, and its end marker passes
+  // through the device thread to the finisher behind every batch
+  if (s->stager.joinable ()) s->stager.join ();
+  if (s->device.joinable ()) s->device.join ();
+  if (s->finisher.joinable ()) s->finisher.join ();
   for (auto &kv : s->reqs) delete kv.second;
-  (void) hipHostFree (s->pin_in);
-  (void) hipHostFree (s->pin_out);
+  (void) hipHostFree (s->slot[0].pin);
+  (void) hipHostFree (s->slot[1].pin);
   delete s;
 }

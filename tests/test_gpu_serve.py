@@ -79,6 +79,8 @@ def test_serve_alone_equals_mix(dev, prob, mixed):
             assert rel(b, a) <= 1e-12
     # some batch held several requests (wider than the widest request)
     assert st["max_batch_cols"] > 17
+    assert st["busy_s"] > 0 and st["stage_s"] >= 0
+    assert st["requests"] == 2 * len(reqs) + len(blockers)
     # the loop's results are the library's batched operations (the same
     # path for the mixed policy; for the plain one solve_block picks a
     # narrower panel, which changes rounding inside the 1e-12 PCG solves)
