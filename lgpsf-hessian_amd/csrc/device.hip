@@ -140,6 +140,7 @@ dev_sync ()
 }
 
 static void free_list_release (hipStream_t s);
+static void stage_release ();
 
 void
 dev_thread_exit ()
@@ -155,6 +156,7 @@ dev_thread_exit ()
   }
   for (hipEvent_t e : t_retired) (void) hipEventDestroy (e);
   t_retired.clear ();
+  stage_release ();
   (void) hipFree (c.info);
   rocblas_destroy_handle (c.rb);
   (void) hipStreamDestroy (c.main);
@@ -328,12 +330,130 @@ dtrim ()
 
 size_t dbytes_in_use () { return g_in_use; }
 
+// ---- host <-> device transfers -------------------------------------------
+// A pageable block goes through the runtime's own staging at ~31 GB/s in
+// (17 ms for 537 MB) and ~17 GB/s out, 9 GB/s into freshly allocated
+// memory, whose pages fault one at a time; from pinned memory the DMA runs
+// at ~57 GB/s and an 8-thread host copy at ~120 GB/s.  So large pageable
+// blocks are staged through two pinned chunks: the host copy of one chunk
+// (which also takes a fresh destination's page faults on 8 threads)
+// overlaps the DMA of the other.
+static constexpr size_t kStageChunk = size_t (64) << 20;
+
+static std::mutex g_pin_mu;
+static std::vector<std::pair<const char *, size_t>> g_pinned;
+
+void *
+host_pinned_alloc (size_t bytes)
+{
+  void *p = nullptr;
+  if (hipHostMalloc (&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+    (void) hipGetLastError ();
+    set_error ("hipHostMalloc of " + std::to_string (bytes) + " bytes failed");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk (g_pin_mu);
+  g_pinned.emplace_back ((const char *) p, bytes);
+  return p;
+}
+
+void
+host_pinned_free (void *p)
+{
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk (g_pin_mu);
+    for (size_t i = 0; i < g_pinned.size (); i++)
+      if (g_pinned[i].first == (const char *) p) {
+        g_pinned.erase (g_pinned.begin () + (long) i);
+        break;
+      }
+  }
+  (void) hipHostFree (p);
+}
+
+static bool
+own_pinned (const void *p, size_t bytes)
+{
+  const char *c = (const char *) p;
+  std::lock_guard<std::mutex> lk (g_pin_mu);
+  for (const auto &r : g_pinned)
+    if (c >= r.first && c + bytes <= r.first + r.second) return true;
+  return false;
+}
+
+// the calling thread's two pinned chunks and their events
+struct Stage {
+  char *pin[2] = { nullptr, nullptr };
+  hipEvent_t ev[2] = { nullptr, nullptr };
+};
+static thread_local Stage t_stage;
+
+static int
+stage_get (Stage *&st)
+{
+  st = &t_stage;
+  if (st->pin[1]) return 0;
+  for (int b = 0; b < 2; b++) {
+    if (!st->pin[b]
+        && hipHostMalloc ((void **) &st->pin[b], kStageChunk, hipHostMallocDefault)
+           != hipSuccess) {
+      st->pin[b] = nullptr;
+      LGH_HIP (hipGetLastError ());
+    }
+    if (!st->ev[b]) LGH_HIP (hipEventCreateWithFlags (&st->ev[b], hipEventDisableTiming));
+  }
+  return 0;
+}
+
+static void
+stage_release ()
+{
+  for (int b = 0; b < 2; b++) {
+    if (t_stage.pin[b]) (void) hipHostFree (t_stage.pin[b]);
+    if (t_stage.ev[b]) (void) hipEventDestroy (t_stage.ev[b]);
+  }
+  t_stage = Stage ();
+}
+
+// host memcpy split over 8 threads in contiguous parts
+static void
+par_memcpy (char *dst, const char *src, size_t bytes)
+{
+  constexpr int NT = 8;
+  const size_t part = (bytes + NT - 1) / NT;
+  std::thread th[NT - 1];
+  for (int t = 1; t < NT; t++) {
+    const size_t o = std::min (bytes, part * t), e = std::min (bytes, o + part);
+    th[t - 1] = std::thread ([=] { if (e > o) memcpy (dst + o, src + o, e - o); });
+  }
+  memcpy (dst, src, std::min (bytes, part));
+  for (auto &x : th) x.join ();
+}
+
 int
 h2d (void *dst, const void *src, size_t bytes)
 {
   if (!bytes) return 0;
-  LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyHostToDevice, stream ()));
-  LGH_HIP (hipStreamSynchronize (t_ctx.s));
+  hipStream_t s = stream ();
+  if (bytes < kStageChunk || own_pinned (src, bytes))
+    LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyHostToDevice, s));
+  else {
+    Stage *st;
+    LGH_TRY (stage_get (st));
+    bool rec[2] = { false, false };
+    for (size_t off = 0, i = 0; off < bytes; off += kStageChunk, i++) {
+      const size_t len = std::min (kStageChunk, bytes - off);
+      const int b = (int) (i & 1);
+      if (rec[b]) LGH_HIP (hipEventSynchronize (st->ev[b]));   // chunk i - 2 sent
+      par_memcpy (st->pin[b], (const char *) src + off, len);
+      LGH_HIP (hipMemcpyAsync ((char *) dst + off, st->pin[b], len,
+                               hipMemcpyHostToDevice, s));
+      LGH_HIP (hipEventRecord (st->ev[b], s));
+      rec[b] = true;
+    }
+  }
+  LGH_HIP (hipStreamSynchronize (s));
   retire_flush ();
   return 0;
 }
@@ -342,8 +462,30 @@ int
 d2h (void *dst, const void *src, size_t bytes)
 {
   if (!bytes) return 0;
-  LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyDeviceToHost, stream ()));
-  LGH_HIP (hipStreamSynchronize (t_ctx.s));
+  hipStream_t s = stream ();
+  if (bytes < kStageChunk || own_pinned (dst, bytes))
+    LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyDeviceToHost, s));
+  else {
+    Stage *st;
+    LGH_TRY (stage_get (st));
+    const size_t nch = (bytes + kStageChunk - 1) / kStageChunk;
+    auto len = [&] (size_t i) { return std::min (kStageChunk, bytes - i * kStageChunk); };
+    for (size_t i = 0; i <= nch; i++) {
+      // the DMA of chunk i runs while chunk i - 1 is copied out; chunk i's
+      // buffer last held chunk i - 2, copied out in the previous iteration
+      if (i < nch) {
+        LGH_HIP (hipMemcpyAsync (st->pin[i & 1], (const char *) src + i * kStageChunk,
+                                 len (i), hipMemcpyDeviceToHost, s));
+        LGH_HIP (hipEventRecord (st->ev[i & 1], s));
+      }
+      if (i >= 1) {
+        const size_t j = i - 1;
+        LGH_HIP (hipEventSynchronize (st->ev[j & 1]));
+        par_memcpy ((char *) dst + j * kStageChunk, st->pin[j & 1], len (j));
+      }
+    }
+  }
+  LGH_HIP (hipStreamSynchronize (s));
   retire_flush ();
   return 0;
 }

@@ -71,8 +71,15 @@ void  dfree (void *p);
 void  dtrim ();                   // release cached blocks
 size_t dbytes_in_use ();
 
+// synchronous host <-> device copies on the thread's stream; blocks of
+// 64 MiB and more in pageable memory move through two pinned 64 MiB chunks
+// per thread, each chunk's host copy split over 8 threads and overlapping
+// the DMA of the other
 int h2d (void *dst, const void *src, size_t bytes);
 int d2h (void *dst, const void *src, size_t bytes);
+// pinned host blocks that h2d / d2h move directly (no staging)
+void *host_pinned_alloc (size_t bytes);   // nullptr on failure (error set)
+void  host_pinned_free (void *p);
 int dzero (void *dst, size_t bytes);   // fill kernel (graph-safe)
 int dcopy (void *dst, const void *src, size_t bytes);
 

@@ -324,9 +324,10 @@ lgh_serve_create (lgh_glr_t *g, const lgh_serve_opts_t *opts, lgh_serve_t **out)
   // batch, whatever its width, takes the same path
   s->tier = g->x.solve_blocked_min > 0 ? 1 : 0;
   const size_t bytes = sizeof (double) * (size_t) g->n * cols;
-  if (hipHostMalloc ((void **) &s->slot[0].pin, bytes, hipHostMallocDefault) != hipSuccess
-      || hipHostMalloc ((void **) &s->slot[1].pin, bytes, hipHostMallocDefault) != hipSuccess) {
-    if (s->slot[0].pin) (void) hipHostFree (s->slot[0].pin);
+  s->slot[0].pin = (double *) host_pinned_alloc (bytes);
+  s->slot[1].pin = s->slot[0].pin ? (double *) host_pinned_alloc (bytes) : nullptr;
+  if (!s->slot[1].pin) {
+    host_pinned_free (s->slot[0].pin);
     delete s;
     return lgh_petsc_error (PETSC_ERR_MEM, "lgh_serve_create: pinned staging "
                             "(%.1f GB) failed", 2.0 * bytes / 1e9);
@@ -438,7 +439,7 @@ lgh_serve_destroy (lgh_serve_t *s)
   if (s->device.joinable ()) s->device.join ();
   if (s->finisher.joinable ()) s->finisher.join ();
   for (auto &kv : s->reqs) delete kv.second;
-  (void) hipHostFree (s->slot[0].pin);
-  (void) hipHostFree (s->slot[1].pin);
+  host_pinned_free (s->slot[0].pin);
+  host_pinned_free (s->slot[1].pin);
   delete s;
 }
