@@ -102,7 +102,7 @@ int  lgh_amg_level (const lgh_amg_t *h, int l, int *n, int *nc,
  * hierarchy (lgh_prior_info_t.cycle) */
 int  lgh_amg_set_cycle (lgh_amg_t *h, int gamma);
 /* X = cycle(B) on the device, ncols columns; prec 0 FP64, 1 FP32,
- * 2 BF16 vectors with FP32 matrices and arithmetic (W = 64 or 16) */
+ * 2 BF16 vectors with FP32 matrices and arithmetic (W = 64, 16 or 8) */
 int  lgh_amg_vcycle (const lgh_amg_t *h, int prec, int W, int ncols,
                      const double *B, int ldb, double *X, int ldx);
 

@@ -257,8 +257,8 @@ lgh_amg_vcycle (const lgh_amg_t *h, int prec, int W, int ncols,
                 const double *B, int ldb, double *X, int ldx)
 {
   LGH_TRY (dev_init ());
-  LGH_CHECK (prec >= 0 && prec <= 2 && (prec != 2 || W == 64 || W == 16), 8,
-             "lgh_amg_vcycle: prec must be 0, 1 or 2 (BF16 needs W = 64 or 16)");
+  LGH_CHECK (prec >= 0 && prec <= 2 && (prec != 2 || W == 64 || W == 16 || W == 8), 8,
+             "lgh_amg_vcycle: prec must be 0, 1 or 2 (BF16 needs W = 64, 16 or 8)");
   lgh_amg_t *m = (lgh_amg_t *) h;
   if (!m->mg[prec]) LGH_TRY (mg_upload (m->h, prec, &m->mg[prec]));
   const int n = m->h.lev.back ().A.n;

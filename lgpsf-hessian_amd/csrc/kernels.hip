@@ -324,15 +324,18 @@ grid64 (int n, int np, int prec, int W = 64)
   const int rows = TPB / (W / (16 / (int) prec_bytes (prec)));
   return dim3 ((unsigned) ((n + rows - 1) / rows), (unsigned) np);
 }
-// the row kernels' widths: a full panel, or the narrow last panel of a
-// block whose column count leaves at most 16 in it
-static inline bool row_width (int W) { return W == 64 || W == 16; }
+// the row kernels' widths: a full panel, or the narrow last panels of a
+// block whose column count leaves at most 48 (16-wide) or 8 (8-wide) in
+// it.  Every row map fits 8 columns: 16-byte lanes hold 8 BF16, 4 FP32 or
+// 2 FP64 values.
+static inline bool row_width (int W) { return W == 64 || W == 16 || W == 8; }
 template <typename F>
 static int
 with_row_W (int W, F &&f)
 {
   if (W == 64) return f (IC<64> ());
-  return f (IC<16> ());
+  if (W == 16) return f (IC<16> ());
+  return f (IC<8> ());
 }
 
 // Y = alpha A X + beta Y (T arithmetic, S storage of X and Y)

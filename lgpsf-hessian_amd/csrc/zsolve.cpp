@@ -27,8 +27,7 @@ DevMG::Work::free ()
 void
 DevMG::free_work ()
 {
-  work[0].free ();
-  work[1].free ();
+  for (Work &w : work) w.free ();
 }
 
 // Chebyshev recurrence scalars of zsolve_impl.h:113-138
@@ -57,7 +56,7 @@ struct ChebCoef {
 static inline bool
 fused_level (const DevMG &g, int l, int W)
 {
-  return g.nu[l] == 2 && (W == 64 || W == 16);
+  return g.nu[l] == 2 && (W == 64 || W == 16 || W == 8);
 }
 
 int
@@ -119,7 +118,7 @@ static int
 mg_work (DevMG &g, const PMat &like, DevMG::Work **out)
 {
   const int np = like.npanels ();
-  DevMG::Work &w = g.work[like.W == 64 ? 0 : 1];
+  DevMG::Work &w = g.work[like.W == 64 ? 0 : like.W == 16 ? 1 : 2];
   *out = &w;
   if (w.W == like.W && w.np == np) return 0;
   w.free ();
@@ -252,6 +251,7 @@ ChebWork::~ChebWork ()
   pmat_free (Z); pmat_free (Rpc); pmat_free (Zpc); pmat_free (Bt);
   pmat_free (Xt);
   delete tail;
+  delete tail8;
 }
 
 // whole panels and the narrow last panel of B (see zsolve.hpp): calls
@@ -267,23 +267,26 @@ split_tail (const PMat &B, PMat &X, WK &w, bool *done, F &&run)
   LGH_CHECK (X.W == 64 && X.ncols == B.ncols && X.prec == B.prec, 8,
              "Z-solve: X shape differs from B");
   *done = true;
-  if (!w.tail) w.tail = new WK ();
+  const int tw = tail <= 8 ? 8 : 16;
+  WK *&tw_work = (tw == 8) ? w.tail8 : w.tail;
+  if (!tw_work) tw_work = new WK ();
   if (!w.Bt.p || w.Bt.nrows != B.nrows || w.Bt.prec != B.prec) {
     pmat_free (w.Bt); pmat_free (w.Xt);
     LGH_TRY (pmat_alloc (w.Bt, B.nrows, 48, 16, B.prec));
     LGH_TRY (pmat_alloc (w.Xt, B.nrows, 48, 16, B.prec));
   }
-  // the tail's columns as ceil(tail / 16) 16-wide panels (a prefix of the
-  // 48-column buffers); Bt's columns past `tail` are zero padding (so are
-  // Xt's after the solve)
+  // the tail's columns as one 8-wide panel or ceil(tail / 16) 16-wide ones
+  // (a prefix of the 48-column buffers); Bt's columns past `tail` are zero
+  // padding (so are Xt's after the solve)
   PMat Bt = w.Bt, Xt = w.Xt;
   Bt.ncols = Xt.ncols = tail;
+  Bt.W = Xt.W = tw;
   LGH_TRY (dzero (Bt.p, Bt.bytes ()));
   LGH_TRY (k_copy_cols (B, 64 * full, Bt, 0, tail));
   // the narrow panel's solve runs on the auxiliary stream, overlapping the
   // whole panels' (their coarse levels are latency-bound and leave the
   // device idle); it touches only its own buffers (tail work, the V-cycle's
-  // second work slot, Bt, Xt)
+  // narrow-panel work slot, Bt, Xt)
   const hipStream_t main = stream ();
   hipStream_t aux = (full > 0) ? aux_stream () : nullptr;
   hipEvent_t ready = nullptr, done_ev = nullptr;
@@ -295,11 +298,11 @@ split_tail (const PMat &B, PMat &X, WK &w, bool *done, F &&run)
   int rc = 0;
   if (aux) {
     StreamScope on_aux (aux);
-    rc = run (Bt, Xt, *w.tail);
+    rc = run (Bt, Xt, *tw_work);
     if (!rc && hipEventRecord (done_ev, aux) != hipSuccess)
       rc = hip_fail (hipGetLastError (), "hipEventRecord", __FILE__, __LINE__);
   }
-  else rc = run (Bt, Xt, *w.tail);       // on the main stream, in order
+  else rc = run (Bt, Xt, *tw_work);       // on the main stream, in order
   if (!rc && full > 0) {                 // the whole panels, in place
     PMat Bf = B, Xf = X;
     Bf.ncols = Xf.ncols = 64 * full;
@@ -345,7 +348,8 @@ cheb_run (const DevCSR &A, DevMG *mg, double lmin, double lmax, int nit,
   // and writes its own copies: Rpc (written by each step) and Zpc
   const bool low = mg && mg->prec != B.prec;
   if (low) {
-    LGH_CHECK ((B.W == 64 || B.W == 16) && B.prec == F32 && mg->prec == BF16, 8,
+    LGH_CHECK ((B.W == 64 || B.W == 16 || B.W == 8) && B.prec == F32
+               && mg->prec == BF16, 8,
                "cheb_solve: unsupported preconditioner precision");
     if (!w.Rpc.p) {
       const int nc = B.npanels () * B.W;
@@ -400,6 +404,7 @@ ChebMixedWork::~ChebMixedWork ()
   pmat_free (Rpc); pmat_free (Zpc); pmat_free (Bt); pmat_free (Xt);
   pmat_free (R32[0]); pmat_free (R32[1]); pmat_free (X32);
   delete tail;
+  delete tail8;
 }
 
 int

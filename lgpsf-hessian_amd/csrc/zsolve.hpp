@@ -17,13 +17,14 @@ struct DevMG {
   void *cinv = nullptr;                // dense row-major inverse of A_0
   int n0 = 0;
   // per-level workspace for one (W, npanels) shape; slot 0 holds full
-  // 64-column panels, slot 1 other widths (the narrow last panel of a
-  // block), so a solve that alternates between the two reallocates nothing
+  // 64-column panels, slot 1 16-wide ones, slot 2 other widths (the
+  // narrow last panels of blocks), so solves that alternate between them
+  // reallocate nothing
   struct Work {
     int W = 0, np = -1;
     std::vector<PMat> R1, R2, D1, D2, Bl, Xl;
     void free ();
-  } work[2];
+  } work[3];
   ~DevMG ();
   void free_work ();
 };
@@ -38,16 +39,17 @@ struct ChebWork {
   PMat R[2], D[2], Z;
   PMat Rpc, Zpc;                       // lower-precision V-cycle in / out
   PMat Bt, Xt;                         // the narrow panels' B and X (48 columns)
-  ChebWork *tail = nullptr;            // (see ChebMixedWork)
+  ChebWork *tail = nullptr, *tail8 = nullptr;   // (see ChebMixedWork)
   ~ChebWork ();
   int ensure (const PMat &like);
 };
 
 // Both Chebyshev solves below run a block whose last 64-column panel holds
-// at most 48 columns as its whole panels plus up to three 16-wide panels
-// (tail: its own work and copies of those columns, solved on the thread's
-// auxiliary stream beside the whole panels), so the padding columns cost
-// nothing; every column sees the same arithmetic at either width.
+// at most 48 columns as its whole panels plus up to three 16-wide panels,
+// or one 8-wide panel when it holds at most 8 (tail, tail8: their own work
+// and copies of those columns, solved on the thread's auxiliary stream
+// beside the whole panels), so the padding columns cost little; every
+// column sees the same arithmetic at any width.
 //
 // X = Chebyshev(nit) solve of A X = B, zero initial guess
 // (zsolve_impl.h:104-141 with the blocked-asolve contract of :629-633);
@@ -67,7 +69,7 @@ struct ChebMixedWork {
   PMat R[2], D[2], Rpc, Zpc;
   PMat R32[2], X32;                    // the early phase's FP32 R and X
   PMat Bt, Xt;                         // the narrow panels' B and X (48 columns)
-  ChebMixedWork *tail = nullptr;
+  ChebMixedWork *tail = nullptr, *tail8 = nullptr;
   ~ChebMixedWork ();
   int ensure (const PMat &like, int pc);
 };

@@ -323,6 +323,18 @@ def test_grid_glr_matches_oracle(dev, grid, oracle, policy):
     Xb = g.solve_block(cs, R)
     for j in range(3):
         assert rel(Xb[:, j], g.solve(cs[j], R[:, j])) < 1e-12
+    if mixed:
+        # the blocked tier solves a column with the same bits whatever the
+        # width of the panel it lands in: 5 columns run as one 8-wide panel,
+        # 20 as two 16-wide ones, 70 as a whole panel and an 8-wide tail
+        R70 = oracle.randn_fill(13, 0, 0, grid["Z"].shape[0], 70)
+        c70 = 10.0 ** np.linspace(-3, 0, 70)
+        X70 = g.solve_block(c70, R70)
+        assert np.array_equal(g.solve_block(c70[:5], R70[:, :5]), X70[:, :5])
+        assert np.array_equal(g.solve_block(c70[:20], R70[:, :20]), X70[:, :20])
+        assert np.array_equal(g.solve_block(c70[64:], R70[:, 64:]), X70[:, 64:])
+        assert np.array_equal(g.sample_block(c70[:5], R70[:, :5]),
+                              g.sample_block(c70, R70)[:, :5])
 
 
 @pytest.mark.parametrize("mixed", [False, True])

@@ -175,8 +175,8 @@ def test_vcycle_matches_oracle(dev, oracle, gamma, dim):
 
 @pytest.mark.parametrize("prec,dim", [(0, 2), (1, 2), (2, 2), (0, 3), (2, 3)])
 def test_vcycle_narrow_panel_bitwise(dev, prec, dim):
-    """A 16-wide panel (the narrow last panel of a block) runs the same
-    per-column arithmetic as a 64-wide one: identical bits, at every
+    """16- and 8-wide panels (the narrow last panels of a block) run the
+    same per-column arithmetic as a 64-wide one: identical bits, at every
     storage precision (in 3D with the long-row restriction onto the
     coarsest level)."""
     Z, M = W.prior_Z(64 if dim == 2 else 32, dim, 1.0, 0.1)
@@ -184,4 +184,6 @@ def test_vcycle_narrow_panel_bitwise(dev, prec, dim):
     B = np.random.default_rng(5).standard_normal((Z.shape[0], 13 if dim == 2 else 70))
     x64 = h.vcycle(B, prec=prec, W=64)
     x16 = h.vcycle(B, prec=prec, W=16)
+    x8 = h.vcycle(B, prec=prec, W=8)
     assert np.array_equal(x64, x16)
+    assert np.array_equal(x64, x8)
