@@ -335,6 +335,15 @@ def test_grid_glr_matches_oracle(dev, grid, oracle, policy):
         assert np.array_equal(g.solve_block(c70[64:], R70[:, 64:]), X70[:, 64:])
         assert np.array_equal(g.sample_block(c70[:5], R70[:, :5]),
                               g.sample_block(c70, R70)[:, :5])
+        # repeated narrow-panel solves reuse their staging and work buffers:
+        # calls with new inputs, and 16-wide work between them, keep the
+        # first call's bits
+        x1 = g.solve_block(c70[:1], R70[:, :1])
+        for _ in range(4):
+            assert np.array_equal(g.solve_block(c70[:20], R70[:, :20]), X70[:, :20])
+            assert np.array_equal(g.solve_block(c70[:1], R70[:, :1]), x1)
+            assert np.array_equal(g.solve_block(c70[1:4], R70[:, 1:4]), X70[:, 1:4])
+            assert np.array_equal(g.solve(c70[5], R70[:, 5]), X70[:, 5])
 
 
 @pytest.mark.parametrize("mixed", [False, True])
