@@ -386,24 +386,29 @@ own_pinned (const void *p, size_t bytes)
 struct Stage {
   char *pin[2] = { nullptr, nullptr };
   hipEvent_t ev[2] = { nullptr, nullptr };
+  bool failed = false;            // could not be set up: copy directly
 };
 static thread_local Stage t_stage;
 
-static int
-stage_get (Stage *&st)
+// nullptr when the chunks cannot be had (the caller then copies directly)
+static Stage *
+stage_get ()
 {
-  st = &t_stage;
-  if (st->pin[1]) return 0;
+  Stage *st = &t_stage;
+  if (st->ev[1]) return st;
+  if (st->failed) return nullptr;
   for (int b = 0; b < 2; b++) {
-    if (!st->pin[b]
-        && hipHostMalloc ((void **) &st->pin[b], kStageChunk, hipHostMallocDefault)
-           != hipSuccess) {
-      st->pin[b] = nullptr;
-      LGH_HIP (hipGetLastError ());
+    if ((!st->pin[b]
+         && hipHostMalloc ((void **) &st->pin[b], kStageChunk, hipHostMallocDefault)
+            != hipSuccess)
+        || (!st->ev[b]
+            && hipEventCreateWithFlags (&st->ev[b], hipEventDisableTiming) != hipSuccess)) {
+      (void) hipGetLastError ();
+      st->failed = true;
+      return nullptr;
     }
-    if (!st->ev[b]) LGH_HIP (hipEventCreateWithFlags (&st->ev[b], hipEventDisableTiming));
   }
-  return 0;
+  return st;
 }
 
 static void
@@ -436,11 +441,10 @@ h2d (void *dst, const void *src, size_t bytes)
 {
   if (!bytes) return 0;
   hipStream_t s = stream ();
-  if (bytes < kStageChunk || own_pinned (src, bytes))
+  Stage *st = (bytes < kStageChunk || own_pinned (src, bytes)) ? nullptr : stage_get ();
+  if (!st)
     LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyHostToDevice, s));
   else {
-    Stage *st;
-    LGH_TRY (stage_get (st));
     bool rec[2] = { false, false };
     for (size_t off = 0, i = 0; off < bytes; off += kStageChunk, i++) {
       const size_t len = std::min (kStageChunk, bytes - off);
@@ -463,11 +467,10 @@ d2h (void *dst, const void *src, size_t bytes)
 {
   if (!bytes) return 0;
   hipStream_t s = stream ();
-  if (bytes < kStageChunk || own_pinned (dst, bytes))
+  Stage *st = (bytes < kStageChunk || own_pinned (dst, bytes)) ? nullptr : stage_get ();
+  if (!st)
     LGH_HIP (hipMemcpyAsync (dst, src, bytes, hipMemcpyDeviceToHost, s));
   else {
-    Stage *st;
-    LGH_TRY (stage_get (st));
     const size_t nch = (bytes + kStageChunk - 1) / kStageChunk;
     auto len = [&] (size_t i) { return std::min (kStageChunk, bytes - i * kStageChunk); };
     for (size_t i = 0; i <= nch; i++) {
