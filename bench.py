@@ -577,8 +577,12 @@ def main():
     t0 = time.perf_counter()
     X = g.solve_block(cs, R)
     t_blk = time.perf_counter() - t0
+    # single-RHS latency: one untimed call (it sets up the narrow panel's
+    # work buffers, as the untimed block solve above does for 64 columns),
+    # then 20 timed calls
+    g.solve(cs[0], R[:, 0])
     lat = []
-    for j in range(5):
+    for j in range(20):
         t1 = time.perf_counter()
         g.solve(cs[j], R[:, j])
         lat.append(time.perf_counter() - t1)
