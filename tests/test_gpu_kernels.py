@@ -187,3 +187,16 @@ def test_vcycle_narrow_panel_bitwise(dev, prec, dim):
     x8 = h.vcycle(B, prec=prec, W=8)
     assert np.array_equal(x64, x16)
     assert np.array_equal(x64, x8)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,m", [(1_000_003, 13), (1 << 20, 16)])
+def test_large_host_blocks_round_trip(dev, n, m):
+    """Host blocks of 64 MiB and more move through two pinned chunks per
+    thread (104 MB: one whole chunk and a partial one; 128 MiB: exactly
+    two): Y = 2 I X through the SpMM entry point comes back exact."""
+    A = sp.identity(n, format="csr") * 2.0
+    X = np.random.default_rng(n).standard_normal((n, m))
+    assert X.nbytes >= 64 << 20
+    Y = dev.spmm(A, X)
+    assert np.array_equal(Y, 2.0 * X)
