@@ -46,11 +46,21 @@ void lgh_hip_prof_enable (int on);
 int  lgh_hip_prof_collect (int kind, int np, int prec, int64_t *count,
                            double *ms, double *bytes, double *flops);
 
-/* the device hashed-Gaussian arithmetic evaluated on the host (no GPU) */
+/* the device hashed-Gaussian arithmetic evaluated on the host (no GPU):
+ * the draw with the correctly rounded log(q) */
 double lgh_hip_randn_at_host (uint64_t seed, int64_t i, int64_t j);
+/* the same over an nr x nc block (column-major, ld), in precision prec
+ * (0 FP64, 1 FP32, returned as FP64), with flags(r, c) = 1 where the device
+ * fill finishes the draw with the platform log because its value may depend
+ * on that log's rounding (no GPU; no reference counterpart: it lets the
+ * flag window be checked against the platform log on the host) */
+int lgh_hip_randn_host_block (uint64_t seed, int64_t row0, int64_t col0,
+                              int nr, int nc, int prec, double *vals, int ld,
+                              unsigned char *flags, int ldf);
 
-/* out(r, c) = randn_at(seed, row0 + r, col0 + c) generated on the device
- * with panel width W in precision prec, returned as FP64 */
+/* out(r, c) = lgh_randn_at(seed, row0 + r, col0 + c), the reference's draw
+ * bit for bit, generated on the device with panel width W in precision
+ * prec, returned as FP64 */
 int lgh_hip_randn_fill (uint64_t seed, int64_t row0, int64_t col0, int nr,
                         int nc, int W, int prec, double *out, int ld);
 

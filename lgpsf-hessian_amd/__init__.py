@@ -221,6 +221,9 @@ _SIGS = {
     "lgh_hip_prof_enable": (None, [C.c_int]),
     "lgh_hip_prof_collect": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64), dp, dp, dp]),
     "lgh_hip_randn_at_host": (C.c_double, [C.c_uint64, C.c_int64, C.c_int64]),
+    "lgh_hip_randn_host_block": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.c_int,
+                                           C.c_int, C.c_int, dp, C.c_int,
+                                           C.POINTER(C.c_ubyte), C.c_int]),
     "lgh_hip_randn_fill": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.c_int,
                                      C.c_int, C.c_int, C.c_int, dp, C.c_int]),
     "lgh_hip_spmm": (C.c_int, [C.c_int, C.c_int, ip, ip, dp, C.c_int, C.c_int,
@@ -727,6 +730,19 @@ def prof_collect(kind, np_=-1, prec=-1):
 
 def randn_at_host(seed, i, j):
     return lib().lgh_hip_randn_at_host(seed, i, j)
+
+
+def randn_host_block(seed, row0, col0, nr, nc, prec=PREC_F64):
+    """The device draw arithmetic on the host (correctly rounded log) over a
+    block, and the flags of the draws the device fill finishes with the
+    platform log: (values, flags)."""
+    vals = np.zeros((nr, nc), order="F")
+    flags = np.zeros((nr, nc), dtype=np.uint8, order="F")
+    _hip_check(lib().lgh_hip_randn_host_block(
+        seed, row0, col0, nr, nc, prec, _d(vals), max(nr, 1),
+        flags.ctypes.data_as(C.POINTER(C.c_ubyte)), max(nr, 1)),
+        "lgh_hip_randn_host_block")
+    return vals, flags.astype(bool)
 
 
 def randn_fill(seed, row0, col0, nr, nc, W=64, prec=PREC_F64):

@@ -234,7 +234,7 @@ smoothed_prolongator (const HostCSR &A, const std::vector<int> &agg, int na)
   double rho = rs;
   if (n > 1) {
     std::vector<double> rb (n);
-    for (int i = 0; i < n; i++) rb[i] = randn_at (0x5A3C0DEULL, i, 0);
+    for (int i = 0; i < n; i++) rb[i] = randn_at_libm (0x5A3C0DEULL, i, 0);
     double emin = 0.0, emax = 0.0;
     host_cg_lanczos (A, dinv.data (), rb.data (), 1e-12, 20, &emin, &emax);
     if (emax > 0.0) rho = std::min (rs, 1.05 * emax);
@@ -353,7 +353,7 @@ amg_setup (const HostCSR &A0, const AmgOpts &o, AmgHierarchy &h)
     if (l > 0) {
       lv.nu = o.nu_force > 0 ? o.nu_force : 2;
       std::vector<double> rb (lv.A.n);
-      for (int i = 0; i < lv.A.n; i++) rb[i] = randn_at (20260813ULL + (uint64_t) l, i, 0);
+      for (int i = 0; i < lv.A.n; i++) rb[i] = randn_at_libm (20260813ULL + (uint64_t) l, i, 0);
       if (o.smax_krylov) {
         double emin, emax;
         host_cg_lanczos (lv.A, lv.dinv.data (), rb.data (), 1e-12, 40, &emin, &emax);

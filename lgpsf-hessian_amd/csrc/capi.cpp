@@ -6,8 +6,12 @@
 #include "engine.hpp"
 #include "randn.hpp"
 
+#include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <memory>
+#include <thread>
+#include <vector>
 
 using namespace lgh;
 
@@ -77,6 +81,35 @@ double
 lgh_hip_randn_at_host (uint64_t seed, int64_t i, int64_t j)
 {
   return lgh::randn_at (seed, i, j);
+}
+
+int
+lgh_hip_randn_host_block (uint64_t seed, int64_t row0, int64_t col0, int nr,
+                          int nc, int prec, double *vals, int ld,
+                          unsigned char *flags, int ldf)
+{
+  LGH_CHECK (nr >= 0 && nc >= 0 && ld >= std::max (nr, 1)
+             && ldf >= std::max (nr, 1) && (prec == 0 || prec == 1), 63,
+             "lgh_hip_randn_host_block: bad arguments");
+  auto col = [&] (int j) {
+    for (int i = 0; i < nr; i++) {
+      bool f;
+      double u1, q;
+      const double v = prec == 0
+          ? lgh::randn_at_flag<double> (seed, row0 + i, col0 + j, &f, &u1, &q)
+          : (double) lgh::randn_at_flag<float> (seed, row0 + i, col0 + j, &f, &u1, &q);
+      vals[(size_t) j * ld + i] = v;
+      flags[(size_t) j * ldf + i] = f ? 1 : 0;
+    }
+  };
+  const int nt = (int) std::max (1u, std::min (8u, std::thread::hardware_concurrency ()));
+  std::atomic<int> next (0);
+  auto worker = [&] () { for (int j; (j = next++) < nc;) col (j); };
+  std::vector<std::thread> th;
+  for (int t = 1; t < std::min (nt, std::max (nc, 1)); t++) th.emplace_back (worker);
+  worker ();
+  for (auto &t : th) t.join ();
+  return 0;
 }
 
 int

@@ -157,6 +157,7 @@ dev_thread_exit ()
   for (hipEvent_t e : t_retired) (void) hipEventDestroy (e);
   t_retired.clear ();
   stage_release ();
+  randn_host_release ();
   (void) hipFree (c.info);
   rocblas_destroy_handle (c.rb);
   (void) hipStreamDestroy (c.main);

@@ -11,7 +11,10 @@
 #include "lgh_dev.hpp"
 #include "randn.hpp"
 
+#include <algorithm>
+#include <thread>
 #include <type_traits>
+#include <vector>
 
 namespace lgh {
 
@@ -99,43 +102,171 @@ dzero (void *dst, size_t bytes)
 
 // ---------------------------------------------------------------------------
 // hashed Gaussian fill
+//
+// Every draw is computed on the device with the correctly rounded log.  A
+// draw whose value may depend on how the platform log rounds (randn.hpp:
+// about 2.2 % of FP64 draws, ~2^-29 of FP32 ones) is also appended to a
+// list (element offset, q, u1); the host evaluates the platform log of
+// those q and randn_fix_k rewrites the draws, so the panel equals the
+// reference's sketch bit for bit.  The list is gathered per block in LDS
+// and appended with one global atomic per block; its order is arbitrary
+// but every entry rewrites its own element, so the result is not.
+
+static constexpr int RANDN_DPT = 16;        // draws per thread
+static constexpr int RANDN_LDS_CAP = 512;   // list entries a block gathers
 
 template <typename T, int W>
 __global__ void __launch_bounds__ (TPB)
 randn_k (T *out, int nrows, int ncols, int npanels, uint64_t seed,
-         int64_t row0, int64_t col0)
+         int64_t row0, int64_t col0, uint64_t *fe, double *fq, double *fu,
+         unsigned *cnt, unsigned cap)
 {
-  size_t e = (size_t) blockIdx.x * TPB + threadIdx.x;
-  size_t tot = (size_t) npanels * nrows * W;
-  if (e >= tot) return;
-  int c = (int) (e % W);
-  size_t rest = e / W;
-  int i = (int) (rest % nrows);
-  int p = (int) (rest / nrows);
-  int j = p * W + c;
-  out[e] = (j < ncols) ? (T) randn_at (seed, row0 + i, col0 + j) : (T) 0;
+  __shared__ uint64_t se[RANDN_LDS_CAP];
+  __shared__ double sq[RANDN_LDS_CAP], su[RANDN_LDS_CAP];
+  __shared__ unsigned sn, sbase;
+  if (threadIdx.x == 0) sn = 0u;
+  __syncthreads ();
+  const size_t tot = (size_t) npanels * nrows * W;
+  const size_t base = (size_t) blockIdx.x * TPB * RANDN_DPT;
+  for (int k = 0; k < RANDN_DPT; k++) {
+    const size_t e = base + (size_t) k * TPB + threadIdx.x;
+    if (e >= tot) break;
+    const int c = (int) (e % W);
+    const size_t rest = e / W;
+    const int i = (int) (rest % nrows);
+    const int j = (int) (rest / nrows) * W + c;
+    if (j >= ncols) { out[e] = (T) 0; continue; }
+    bool f;
+    double u1, q;
+    out[e] = randn_at_flag<T> (seed, row0 + i, col0 + j, &f, &u1, &q);
+    if (f) {
+      const unsigned sl = atomicAdd (&sn, 1u);
+      if (sl < (unsigned) RANDN_LDS_CAP) { se[sl] = e; sq[sl] = q; su[sl] = u1; }
+      else {
+        const unsigned g = atomicAdd (cnt, 1u);
+        if (g < cap) { fe[g] = e; fq[g] = q; fu[g] = u1; }
+      }
+    }
+  }
+  __syncthreads ();
+  const unsigned m = sn < (unsigned) RANDN_LDS_CAP ? sn : (unsigned) RANDN_LDS_CAP;
+  if (m == 0u) return;
+  if (threadIdx.x == 0) sbase = atomicAdd (cnt, m);
+  __syncthreads ();
+  for (unsigned sl = threadIdx.x; sl < m; sl += TPB) {
+    const unsigned g = sbase + sl;
+    if (g < cap) { fe[g] = se[sl]; fq[g] = sq[sl]; fu[g] = su[sl]; }
+  }
+}
+
+// out[fe[k]] = the draw of (fu[k], fq[k]) with log(fq[k]) = fl[k]
+template <typename T>
+__global__ void __launch_bounds__ (TPB)
+randn_fix_k (T *out, const uint64_t *fe, const double *fq, const double *fu,
+             const double *fl, unsigned n)
+{
+  const unsigned k = blockIdx.x * TPB + threadIdx.x;
+  if (k < n) out[fe[k]] = (T) randn_from (fu[k], fq[k], fl[k]);
+}
+
+// the calling thread's pinned buffer for the listed q (grown on demand)
+struct RandnHost { double *p = nullptr; size_t n = 0; };
+static thread_local RandnHost t_rh;
+
+void
+randn_host_release ()
+{
+  host_pinned_free (t_rh.p);
+  t_rh = RandnHost ();
+}
+
+// x[k] = log(x[k]) with the platform log, over up to 8 threads
+static void
+platform_log_inplace (double *x, size_t n)
+{
+  const int nt = n < 65536 ? 1
+      : (int) std::max (1u, std::min (8u, std::thread::hardware_concurrency ()));
+  auto part = [&] (int t) {
+    const size_t o = n * t / nt, e = n * (t + 1) / nt;
+    for (size_t k = o; k < e; k++) x[k] = ::log (x[k]);
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; t++) th.emplace_back (part, t);
+  part (0);
+  for (auto &t : th) t.join ();
 }
 
 int
 k_randn (PMat &out, uint64_t seed, int64_t row0, int64_t col0)
 {
-  auto l = [&] (auto wt) -> int {
+  const size_t tot = out.elems ();
+  if (!tot) return 0;
+  LGH_CHECK (out.prec == F64 || out.prec == F32, 8, "k_randn: FP64 or FP32 output");
+  // list capacity: 1/32 of the draws (FP64: 2.2 % expected), 1/4096 for
+  // FP32; a fill that lists more runs again with the exact capacity
+  size_t cap = (out.prec == F64 ? tot / 32 : tot / 4096) + 4096;
+  const unsigned nblk = (unsigned) ((tot + (size_t) TPB * RANDN_DPT - 1)
+                                    / ((size_t) TPB * RANDN_DPT));
+  for (int pass = 0; pass < 2; pass++) {
+    cap = std::min (cap, (size_t) UINT32_MAX);
+    char *buf = (char *) dmalloc (cap * 32 + 256);
+    if (!buf) return 55;
+    uint64_t *fe = (uint64_t *) buf;
+    double *fq = (double *) (buf + cap * 8), *fu = (double *) (buf + cap * 16),
+           *fl = (double *) (buf + cap * 24);
+    unsigned *cnt = (unsigned *) (buf + cap * 32);
+    auto l = [&] (auto wt) -> int {
       constexpr int W = decltype (wt)::value;
-      unsigned g = grid1 (out.elems (), TPB);
       if (out.prec == F64)
-        hipLaunchKernelGGL ((randn_k<double, W>), dim3 (g), dim3 (TPB), 0,
+        hipLaunchKernelGGL ((randn_k<double, W>), dim3 (nblk), dim3 (TPB), 0,
                             stream (), (double *) out.p, out.nrows, out.ncols,
-                            out.npanels (), seed, row0, col0);
+                            out.npanels (), seed, row0, col0, fe, fq, fu, cnt,
+                            (unsigned) cap);
       else
-        hipLaunchKernelGGL ((randn_k<float, W>), dim3 (g), dim3 (TPB), 0,
+        hipLaunchKernelGGL ((randn_k<float, W>), dim3 (nblk), dim3 (TPB), 0,
                             stream (), (float *) out.p, out.nrows, out.ncols,
-                            out.npanels (), seed, row0, col0);
+                            out.npanels (), seed, row0, col0, fe, fq, fu, cnt,
+                            (unsigned) cap);
       LGH_LAUNCH_CHECK ();
       return 0;
-  };
-  return with_W (out.W, l);
+    };
+    unsigned n = 0;
+    int rc = dzero (cnt, 4);
+    if (!rc) rc = with_W (out.W, l);
+    if (!rc) rc = d2h (&n, cnt, 4);
+    if (!rc && n > cap && pass == 0) { dfree (buf); cap = n; continue; }
+    if (!rc && n > cap) { set_error ("k_randn: flag list overflow"); rc = 8; }
+    if (!rc && n) {
+      if (n > t_rh.n) {
+        randn_host_release ();
+        const size_t want = std::max<size_t> (n + n / 4, 65536);
+        t_rh.p = (double *) host_pinned_alloc (want * 8);
+        if (!t_rh.p) rc = 55;
+        else t_rh.n = want;
+      }
+      if (!rc) rc = d2h (t_rh.p, fq, (size_t) n * 8);
+      if (!rc) {
+        platform_log_inplace (t_rh.p, n);
+        rc = h2d (fl, t_rh.p, (size_t) n * 8);
+      }
+      if (!rc) {
+        if (out.prec == F64)
+          hipLaunchKernelGGL ((randn_fix_k<double>), dim3 (grid1 (n, TPB)),
+                              dim3 (TPB), 0, stream (), (double *) out.p, fe,
+                              fq, fu, fl, n);
+        else
+          hipLaunchKernelGGL ((randn_fix_k<float>), dim3 (grid1 (n, TPB)),
+                              dim3 (TPB), 0, stream (), (float *) out.p, fe,
+                              fq, fu, fl, n);
+        if (hipGetLastError () != hipSuccess) { set_error ("randn_fix_k launch"); rc = 76; }
+      }
+    }
+    // (a freed block is reused only by later work on this stream)
+    dfree (buf);
+    return rc;
+  }
+  return 0;
 }
-
 
 // ---------------------------------------------------------------------------
 // W = 64 row kernels.  A row of a 64-column panel is 128 B (bf16), 256 B

@@ -172,8 +172,12 @@ int  pmat_alloc (PMat &m, int nrows, int ncols, int W, int prec);  // zeroed
 void pmat_free (PMat &m);
 
 // ---- kernels (all on stream(); asynchronous) ----------------------------
-// hashed Gaussian block: element (i, j) = randn_at(seed, row0 + i, col0 + j)
+// hashed Gaussian block: element (i, j) = the reference's draw
+// lgh_randn_at(seed, row0 + i, col0 + j) bit for bit (device arithmetic,
+// the few draws that depend on the platform log's rounding finished with
+// it on the host; synchronizes the stream when there are any)
 int k_randn (PMat &out, uint64_t seed, int64_t row0, int64_t col0);
+void randn_host_release ();        // the thread's pinned list buffer
 // Y = alpha A X + beta Y over all panels of X/Y (same W, prec of X/Y)
 int k_spmm (const DevCSR &A, double alpha, const PMat &X, double beta,
             PMat &Y);

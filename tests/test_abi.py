@@ -49,9 +49,9 @@ def test_opts_defaults_match_reference(lgh):
 
 
 def test_device_randn_arithmetic_on_host(lgh, oracle):
-    """The device algorithm evaluated on the host: equal to the oracle
-    (glibc log) except where glibc's log is not correctly rounded; there
-    the draw differs by one rounding of log(q)."""
+    """The device draw arithmetic (correctly rounded log) evaluated on the
+    host equals the oracle's glibc-log draw except where glibc's log is not
+    correctly rounded; there the draw differs by one rounding of log(q)."""
     seeds = [0, 271828, 0xB0BD5]
     nbad = tot = 0
     worst = 0.0
@@ -66,6 +66,32 @@ def test_device_randn_arithmetic_on_host(lgh, oracle):
                     worst = max(worst, abs(a - ref[i, j]) / abs(ref[i, j]))
     assert nbad / tot < 2e-3
     assert worst < 4e-16
+
+
+@pytest.mark.parametrize("seed,n,c0,c1", [
+    (0, 16384, 0, 74),              # the 128 x 128 grid's whole sketch (seed 0, l = 74)
+    (271828, 1 << 20, 0, 400),      # the 1024 x 1024 grid's whole sketch (l = 400)
+    (271828, 1 << 21, 400, 416),    # a slice of the 128^3 grid's extension columns
+])
+def test_randn_flag_window_covers_platform_log(lgh, oracle, seed, n, c0, c1):
+    """Every draw of the device arithmetic that differs from the glibc-log
+    draw (the oracle) is flagged for the platform-log finish, in FP64 and in
+    FP32; and the flags stay few (2.2 % of FP64 draws)."""
+    nflag = ndiff = tot = 0
+    for a in range(c0, c1, 16):
+        b = min(a + 16, c1)
+        ref = oracle.randn_fill(seed, 0, a, n, b - a)
+        v, f = lgh.randn_host_block(seed, 0, a, n, b - a, prec=lgh.PREC_F64)
+        d = v != ref
+        assert not (d & ~f).any(), "unflagged FP64 draw differs from the platform log"
+        v32, f32 = lgh.randn_host_block(seed, 0, a, n, b - a, prec=lgh.PREC_F32)
+        d32 = v32 != ref.astype(np.float32).astype(np.float64)
+        assert not (d32 & ~f32).any(), "unflagged FP32 draw differs from the platform log"
+        nflag += int(f.sum())
+        ndiff += int(d.sum())
+        tot += d.size
+    assert ndiff > 0                      # the window is exercised
+    assert nflag / tot < 0.03, nflag / tot
 
 
 def test_amg_hierarchy_properties(lgh):
