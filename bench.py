@@ -669,7 +669,8 @@ def main():
                    "backend": args.backend, "sweep_panels": args.sweep_panels,
                    "solve_blocked_min": mixed.solve_blocked_min,
                    "vcycle_fp64_sweeps": args.vcycle,
-                   "vcycle_fp32_sweeps": args.sketch_vcycle},
+                   "vcycle_fp32_sweeps": args.sketch_vcycle,
+                   "spmm_autotune": [r for r in lgh.spmm_group_report() if r["n"] == n]},
         "roofline": roof,
         "spmm_roofline": spmm_roof,
         "mfma": mf,

@@ -69,13 +69,25 @@ int lgh_hip_spmm (int n, int m, const int *rp, const int *ci, const double *v,
                   int prec_vals, int prec_vec, int W, int ncols,
                   const double *X, int ldx, double *Y, int ldy);
 /* the same through the row-grouped copy of A that the engine keeps for B
- * (rows in groups of 4, each group's union of column indices walked once;
- * W = 64; rows with strictly increasing column indices, else the plain
- * kernel runs): the same bits as lgh_hip_spmm */
+ * (rows in groups of 4 or 8, each group's union of column indices walked
+ * once; W = 64; rows with strictly increasing column indices, else the
+ * plain kernel runs): the same bits as lgh_hip_spmm.  The copy's rows per
+ * group and the panels per wavefront are chosen by the autotuner, which
+ * times the candidates on A on the device (no reference counterpart: the
+ * reference multiplies B column by column, glr_common_impl.h:325-332, and
+ * its only tunable is the prior's tile, prior.h:80) */
 int lgh_hip_spmm_grouped (int n, int m, const int *rp, const int *ci,
                           const double *v, int prec_vals, int prec_vec,
                           int W, int ncols, const double *X, int ldx,
                           double *Y, int ldy);
+/* force the grouped product's configuration for matrices uploaded later
+ * (rg 4 or 8 rows per group, ppw 2 or 4 panels per wavefront with FP64
+ * vectors, twice that with FP32; rg = 0 returns to the autotuner) */
+void lgh_hip_spmm_group_config (int rg, int ppw);
+/* the autotuner's decisions so far as text, one line per matrix shape:
+ * "rows nonzeros value_prec rg ppw_fp64 ppw_fp32 ms_per_panel_fp64
+ * ms_per_panel_fp32"; returns the full length (buf may be null) */
+int lgh_hip_spmm_group_report (char *buf, int len);
 
 /* X = Chebyshev(nit) solve of A X = B with the Jacobi preconditioner on
  * the interval [lmin, lmax], zero initial guess, one fused kernel per

@@ -231,6 +231,8 @@ _SIGS = {
     "lgh_hip_spmm_grouped": (C.c_int, [C.c_int, C.c_int, ip, ip, dp, C.c_int,
                                        C.c_int, C.c_int, C.c_int, dp, C.c_int,
                                        dp, C.c_int]),
+    "lgh_hip_spmm_group_config": (None, [C.c_int, C.c_int]),
+    "lgh_hip_spmm_group_report": (C.c_int, [C.c_char_p, C.c_int]),
     "lgh_hip_cheb_jacobi": (C.c_int, [C.c_int, ip, ip, dp, C.c_int, C.c_int,
                                       C.c_double, C.c_double, C.c_int, C.c_int,
                                       dp, C.c_int, dp, C.c_int]),
@@ -770,6 +772,27 @@ def spmm(A, X, prec_vals=PREC_F64, prec_vec=PREC_F64, W=64, grouped=False):
                   prec_vec, W, X.shape[1], _d(X), X.shape[0], _d(Y), A.shape[0]),
                "lgh_hip_spmm")
     return Y
+
+
+def spmm_group_config(rg=0, ppw=0):
+    """Force the grouped SpMM's rows per group / panels per wave for later
+    uploads (rg = 0: the autotuner decides)."""
+    lib().lgh_hip_spmm_group_config(rg, ppw)
+
+
+def spmm_group_report():
+    """The autotuner's decisions: list of dicts, one per matrix shape."""
+    n = lib().lgh_hip_spmm_group_report(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    lib().lgh_hip_spmm_group_report(buf, n + 1)
+    out = []
+    for line in buf.value.decode().splitlines():
+        f = line.split()
+        out.append({"n": int(f[0]), "nnz": int(f[1]), "value_prec": int(f[2]),
+                    "rows_per_group": int(f[3]), "panels_per_wave_fp64": int(f[4]),
+                    "panels_per_wave_fp32": int(f[5]), "ms_per_panel_fp64": float(f[6]),
+                    "ms_per_panel_fp32": float(f[7])})
+    return out
 
 
 def cheb_jacobi(A, lmin, lmax, nit, B, prec=PREC_F64, W=64):

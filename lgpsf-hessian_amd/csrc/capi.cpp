@@ -144,6 +144,24 @@ spmm_host (int n, int m, const int *rp, const int *ci, const double *v,
   return rc;
 }
 
+void
+lgh_hip_spmm_group_config (int rg, int ppw)
+{
+  spmm_group_force (rg, ppw);
+}
+
+int
+lgh_hip_spmm_group_report (char *buf, int len)
+{
+  const std::string r = spmm_group_report ();
+  if (buf && len > 0) {
+    const size_t k = std::min ((size_t) len - 1, r.size ());
+    memcpy (buf, r.data (), k);
+    buf[k] = 0;
+  }
+  return (int) r.size ();
+}
+
 int
 lgh_hip_spmm (int n, int m, const int *rp, const int *ci, const double *v,
               int prec_vals, int prec_vec, int W, int ncols, const double *X,

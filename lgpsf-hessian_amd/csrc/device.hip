@@ -503,46 +503,67 @@ dcopy (void *dst, const void *src, size_t bytes)
 }
 
 // ---- CSR / panel matrices -------------------------------------------------
+void
+GroupCopy::release ()
+{
+  dfree (gp); dfree (gci); dfree (gv);
+  gp = gci = nullptr;
+  gv = nullptr;
+  rg = 0;
+  nbatch = 0;
+}
+
 DevCSR::~DevCSR ()
 {
   dfree (rp); dfree (ci); dfree (v); dfree (dinv);
-  dfree (gp); dfree (gci); dfree (gv); dfree (vs);
+  grp.release ();
+  dfree (vs);
 }
 
 // Each group's union is the merge of its (sorted) rows; a row's entries
 // keep their order inside it, so a product accumulated over the union in
 // order adds the same terms in the same order as over the row (plus
-// exact zeros).  Two passes over the groups (sizes, then fill), each split
-// over host threads.
+// exact zeros).  The union is padded to whole batches of grp_batch(rg)
+// entries with zero values at the last column (a valid index).  Two
+// passes over the groups (sizes, then fill), each split over host threads.
 int
-csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v)
+csr_group_build (const DevCSR &A, const int *rp, const int *ci,
+                 const double *v, int rg, GroupCopy &out)
 {
-  const int n = A.n, ng = (n + RGROUP - 1) / RGROUP;
+  LGH_CHECK (rg == 4 || rg == 8, 8, "csr_group_build: rows per group 4 or 8");
+  const int n = A.n, ng = (n + rg - 1) / rg, nb = grp_batch (rg);
   for (int i = 0; i < n; i++)
     for (int k = rp[i] + 1; k < rp[i + 1]; k++)
-      if (ci[k] <= ci[k - 1]) return 0;           // unsorted: no copy
+      LGH_CHECK (ci[k] > ci[k - 1], 8, "csr_group_build: unsorted row");
   std::vector<int> gp ((size_t) ng + 1, 0);
+  // entries of group g's union; with cols/vals, also writes them (padded)
   auto merge = [&] (int g, int *cols, double *vals) -> int {
-    int p[RGROUP], e[RGROUP];
-    for (int r = 0; r < RGROUP; r++) {
-      const int i = g * RGROUP + r;
+    int p[8], e[8];
+    for (int r = 0; r < rg; r++) {
+      const int i = g * rg + r;
       p[r] = (i < n) ? rp[i] : 0;
       e[r] = (i < n) ? rp[i + 1] : 0;
     }
-    int cnt = 0;
+    int cnt = 0, last = 0;
     for (;;) {
       int c = INT_MAX;
-      for (int r = 0; r < RGROUP; r++) if (p[r] < e[r]) c = std::min (c, ci[p[r]]);
+      for (int r = 0; r < rg; r++) if (p[r] < e[r]) c = std::min (c, ci[p[r]]);
       if (c == INT_MAX) break;
-      for (int r = 0; r < RGROUP; r++) {
+      for (int r = 0; r < rg; r++) {
         const bool hit = p[r] < e[r] && ci[p[r]] == c;
-        if (vals) vals[(size_t) cnt * RGROUP + r] = hit ? v[p[r]] : 0.0;
+        if (vals) vals[(size_t) cnt * rg + r] = hit ? v[p[r]] : 0.0;
         if (hit) p[r]++;
       }
       if (cols) cols[cnt] = c;
+      last = c;
       cnt++;
     }
-    return cnt;
+    const int padded = (cnt + nb - 1) / nb * nb;
+    for (int k = cnt; k < padded; k++) {
+      if (cols) cols[k] = last;
+      if (vals) for (int r = 0; r < rg; r++) vals[(size_t) k * rg + r] = 0.0;
+    }
+    return padded;
   };
   const int nt = (int) std::max (1u, std::min (16u, std::thread::hardware_concurrency ()));
   auto par = [&] (auto &&body) {
@@ -554,36 +575,35 @@ csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v)
       });
     for (auto &x : th) x.join ();
   };
-  par ([&] (int g) { gp[g + 1] = merge (g, nullptr, nullptr); });
-  for (int g = 0; g < ng; g++) gp[g + 1] += gp[g];
-  const int64_t nu = gp[ng];
+  std::vector<int64_t> off ((size_t) ng + 1, 0);
+  par ([&] (int g) { off[g + 1] = merge (g, nullptr, nullptr); });
+  for (int g = 0; g < ng; g++) off[g + 1] += off[g];
+  const int64_t nu = off[ng];
+  LGH_CHECK (nu / nb < INT_MAX, 8, "csr_group_build: too many batches");
+  for (int g = 0; g <= ng; g++) gp[g] = (int) (off[g] / nb);     // in batches
   std::vector<int> cols ((size_t) std::max<int64_t> (nu, 1));
-  std::vector<double> vals ((size_t) std::max<int64_t> (nu, 1) * RGROUP);
-  par ([&] (int g) { merge (g, cols.data () + gp[g], vals.data () + (size_t) gp[g] * RGROUP); });
+  std::vector<double> vals ((size_t) std::max<int64_t> (nu, 1) * rg);
+  par ([&] (int g) { merge (g, cols.data () + off[g], vals.data () + (size_t) off[g] * rg); });
   const size_t vb = prec_bytes (A.prec);
-  A.gp = (int *) dmalloc (sizeof (int) * ((size_t) ng + 1));
-  A.gci = (int *) dmalloc (sizeof (int) * cols.size ());
-  A.gv = dmalloc (vb * vals.size ());
-  if (!A.gp || !A.gci || !A.gv) {
-    dfree (A.gp); dfree (A.gci); dfree (A.gv);
-    A.gp = A.gci = nullptr; A.gv = nullptr;
-    return 5;
-  }
-  int rc = h2d (A.gp, gp.data (), sizeof (int) * gp.size ());
-  if (!rc) rc = h2d (A.gci, cols.data (), sizeof (int) * cols.size ());
+  GroupCopy c;
+  c.rg = rg;
+  c.nbatch = nu / nb;
+  c.gp = (int *) dmalloc (sizeof (int) * ((size_t) ng + 1));
+  c.gci = (int *) dmalloc (sizeof (int) * cols.size ());
+  c.gv = dmalloc (vb * vals.size ());
+  if (!c.gp || !c.gci || !c.gv) { c.release (); return 5; }
+  int rc = h2d (c.gp, gp.data (), sizeof (int) * gp.size ());
+  if (!rc) rc = h2d (c.gci, cols.data (), sizeof (int) * cols.size ());
   if (!rc) {
-    if (A.prec == F64) rc = h2d (A.gv, vals.data (), sizeof (double) * vals.size ());
+    if (A.prec == F64) rc = h2d (c.gv, vals.data (), sizeof (double) * vals.size ());
     else {
       std::vector<float> f (vals.begin (), vals.end ());
-      rc = h2d (A.gv, f.data (), sizeof (float) * f.size ());
+      rc = h2d (c.gv, f.data (), sizeof (float) * f.size ());
     }
   }
-  if (rc) {
-    dfree (A.gp); dfree (A.gci); dfree (A.gv);
-    A.gp = A.gci = nullptr; A.gv = nullptr;
-    return rc;
-  }
-  A.nunion = nu;
+  if (rc) { c.release (); return rc; }
+  out.release ();
+  out = c;
   return 0;
 }
 

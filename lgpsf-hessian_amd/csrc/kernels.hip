@@ -12,7 +12,11 @@
 #include "randn.hpp"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
 #include <thread>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -523,86 +527,315 @@ spmm64_k (int n, int nx, const int *__restrict__ rp,
   vst (y, acc);
 }
 
-// Y = alpha A X + beta Y over the row-grouped copy of A (DevCSR::gp), 64-
-// column panels: one wavefront per group of RGROUP rows and PPW panels,
-// lane = column.  The wave walks the union of the group's column
-// indices, so a neighbour row of X that several rows of the group share is
-// loaded once and used for each of them (a 127-entry PSF row shares ~90 %
-// of its neighbours with the next row: ~1/3 of the row-by-row loads), and
-// each entry's indices and values serve several panels.  The group index is
-// wave-uniform, so the column indices and the RGROUP values of an entry
-// come through scalar loads; an entry costs one vector load per panel.
+// Y = alpha A X + beta Y over the row-grouped copy of A (DevCSR::grp),
+// 64-column panels: one wavefront per group of RG rows and PPW panels.
+// A lane moves 16 B of a panel row (VPL = 2 FP64 or 4 FP32 columns), so
+// one gather instruction brings one neighbour row of VPL panels (lanes
+// 0-31 panel q, 32-63 panel q + 1 for FP64; four quarters for FP32).  The
+// wave walks the union of the group's column indices in batches of
+// NB = 32 / RG entries, so a neighbour row of X that several rows of the
+// group share is loaded once and used for each of them (a 127-entry PSF
+// row shares ~90 % of its neighbours with the next row), and each entry's
+// index and values serve PPW panels.  The group index is wave-uniform: a
+// batch's NB indices and 32 values arrive by a few wide scalar loads.
 // Row r accumulates its own entries in column order, as spmm64_k does,
-// plus exact zeros: the result is the same bits.
-static constexpr int NGR = 8;           // union entries per load group
-#ifndef SPMM_PPW64
-#define SPMM_PPW64 4
-#endif
-#ifndef SPMM_PPW32
-#define SPMM_PPW32 4
-#endif
-template <typename TV, typename T, typename S, int PPW>
+// plus exact zeros (padding, and the other rows' entries): the result is
+// the same bits.  Panels p0 + PPW * blockIdx.y + k; lanes whose panel is
+// at or past pend read the last valid panel (the same addresses as their
+// neighbours') and store nothing.
+template <typename TV, typename T, typename S, int RG, int PPW>
 __global__ void __launch_bounds__ (TPB)
-spmm_grp_k (int n, int ngroups, int nx, int np, const int *__restrict__ gp,
-            const int *__restrict__ gci, const TV *__restrict__ gv, T alpha,
-            const S *__restrict__ X, T beta, S *__restrict__ Y)
+spmm_grp_k (int n, int ngroups, int nx, int p0, int pend,
+            const int *__restrict__ gp, const int *__restrict__ gci,
+            const TV *__restrict__ gv, T alpha, const S *__restrict__ X,
+            T beta, S *__restrict__ Y)
 {
+  constexpr int NB = 32 / RG;
+  constexpr int VPL = 16 / (int) sizeof (S);      // columns per lane
+  constexpr int LPR = 64 / VPL;                    // lanes per panel row
+  constexpr int NL = PPW / VPL;                    // gathers per entry
+  static_assert (PPW % VPL == 0, "panels per wave: a multiple of 16 B / element");
+  typedef S V __attribute__ ((ext_vector_type (VPL)));
   const int lane = (int) threadIdx.x & 63;
   const int grp = __builtin_amdgcn_readfirstlane (
       (int) (blockIdx.x * (TPB / 64) + (threadIdx.x >> 6)));
   if (grp >= ngroups) return;
-  const int r0 = grp * RGROUP;
-  const int p0 = PPW * (int) blockIdx.y;
-  // panels past the last read the first one again (results discarded)
-  const S *Xq[PPW];
+  const int pb = p0 + PPW * (int) blockIdx.y;
+  const int c0 = (lane % LPR) * VPL;
+  // panel of gather k of this lane, and its element offset from panel pb
+  int pq[NL];
+  size_t off[NL];
 #pragma unroll
-  for (int q = 0; q < PPW; q++)
-    Xq[q] = X + (size_t) (p0 + q < np ? p0 + q : p0) * nx * 64 + lane;
-  const int k0 = gp[grp], k1 = gp[grp + 1];
-  T acc[RGROUP][PPW];
+  for (int k = 0; k < NL; k++) {
+    const int want = pb + k * VPL + lane / LPR;
+    pq[k] = want < pend ? want : pend - 1;
+    off[k] = (size_t) (pq[k] - pb) * nx * 64 + c0;
+  }
+  const S *Xb = X + (size_t) pb * nx * 64;
+  T acc[RG][NL][VPL];
 #pragma unroll
-  for (int r = 0; r < RGROUP; r++)
+  for (int r = 0; r < RG; r++)
 #pragma unroll
-    for (int q = 0; q < PPW; q++) acc[r][q] = 0;
-  for (int kb = k0; kb < k1; kb += NGR) {
-    int c[NGR];
-    TV w[NGR][RGROUP];                  // widened at use (FP32 values stay
-                                        // in scalar registers)
+    for (int k = 0; k < NL; k++)
 #pragma unroll
-    for (int u = 0; u < NGR; u++) {
-      const bool in = kb + u < k1;
-      c[u] = in ? gci[kb + u] : gci[k0];
+      for (int v = 0; v < VPL; v++) acc[r][k][v] = 0;
+  // two-stage pipeline: the gathers of batch b + 1 are issued before the
+  // products of batch b (scheduling barriers keep that order), so a wave
+  // has up to two batches of neighbour rows in flight
+  auto gather = [&] (int b, V (&x)[NB][NL]) {
+    const int *cb = gci + (size_t) b * NB;
 #pragma unroll
-      for (int r = 0; r < RGROUP; r++)
-        w[u][r] = in ? gv[(size_t) (kb + u) * RGROUP + r] : (TV) 0;
+    for (int u = 0; u < NB; u++) {
+      const S *row = Xb + (size_t) cb[u] * 64;
+#pragma unroll
+      for (int k = 0; k < NL; k++) x[u][k] = *(const V *) (row + off[k]);
     }
-    T x[NGR][PPW];
+  };
+  auto product = [&] (int b, const V (&x)[NB][NL]) {
+    const TV *wb = gv + (size_t) b * NB * RG;
 #pragma unroll
-    for (int u = 0; u < NGR; u++) {
-      const bool in = kb + u < k1;
+    for (int u = 0; u < NB; u++)
 #pragma unroll
-      for (int q = 0; q < PPW; q++)
-        x[u][q] = in ? (T) Xq[q][(size_t) c[u] * 64] : (T) 0;
-    }
+      for (int r = 0; r < RG; r++) {
+        const T w = (T) wb[u * RG + r];
 #pragma unroll
-    for (int u = 0; u < NGR; u++)
+        for (int k = 0; k < NL; k++)
 #pragma unroll
-      for (int r = 0; r < RGROUP; r++)
-#pragma unroll
-        for (int q = 0; q < PPW; q++) acc[r][q] += (T) w[u][r] * x[u][q];
+          for (int v = 0; v < VPL; v++) acc[r][k][v] += w * (T) x[u][k][v];
+      }
+  };
+  const int b0 = gp[grp], b1 = gp[grp + 1];
+  V xa[NB][NL], xb[NB][NL];
+  int b = b0;
+  if (b < b1) gather (b, xa);
+  while (b < b1) {
+    if (b + 1 < b1) gather (b + 1, xb);
+    __builtin_amdgcn_sched_barrier (0);
+    product (b, xa);
+    __builtin_amdgcn_sched_barrier (0);
+    if (++b >= b1) break;
+    if (b + 1 < b1) gather (b + 1, xa);
+    __builtin_amdgcn_sched_barrier (0);
+    product (b, xb);
+    __builtin_amdgcn_sched_barrier (0);
+    ++b;
   }
 #pragma unroll
-  for (int q = 0; q < PPW; q++) {
-    if (p0 + q >= np) break;
+  for (int k = 0; k < NL; k++) {
+    if (pb + k * VPL + lane / LPR >= pend) continue;
 #pragma unroll
-    for (int r = 0; r < RGROUP; r++) {
-      const int row = r0 + r;
+    for (int r = 0; r < RG; r++) {
+      const int row = grp * RG + r;
       if (row >= n) break;
-      S *y = Y + ((size_t) (p0 + q) * n + row) * 64 + lane;
-      *y = (S) ((beta == (T) 0) ? alpha * acc[r][q]
-                                : alpha * acc[r][q] + beta * (T) *y);
+      V *y = (V *) (Y + ((size_t) pq[k] * n + row) * 64 + c0);
+      V o;
+      if (beta == (T) 0) {
+#pragma unroll
+        for (int v = 0; v < VPL; v++) o[v] = (S) (alpha * acc[r][k][v]);
+      }
+      else {
+        const V yo = *y;
+#pragma unroll
+        for (int v = 0; v < VPL; v++) o[v] = (S) (alpha * acc[r][k][v] + beta * (T) yo[v]);
+      }
+      *y = o;
     }
   }
+}
+
+// panels [p0, p0 + np) of Y = alpha A X + beta Y through the grouped copy:
+// whole groups of ppw panels, then the remaining panels in one launch of
+// the smallest group width that holds them
+template <typename TV, typename T, typename S>
+static int
+spmm_grp_launch (const DevCSR &A, int ppw, int p0, int np, T alpha,
+                 const S *X, T beta, S *Y)
+{
+  constexpr int VPL = 16 / (int) sizeof (S);
+  const GroupCopy &g = A.grp;
+  const int ng = (A.n + g.rg - 1) / g.rg;
+  const unsigned gx = (unsigned) ((ng + TPB / 64 - 1) / (TPB / 64));
+  const int pend = p0 + np;
+  auto go = [&] (auto rgc, auto pc, int first, int groups) -> int {
+    constexpr int RG = decltype (rgc)::value, PPW = decltype (pc)::value;
+    if (groups <= 0) return 0;
+    hipLaunchKernelGGL ((spmm_grp_k<TV, T, S, RG, PPW>),
+                        dim3 (gx, (unsigned) groups), dim3 (TPB), 0,
+                        stream (), A.n, ng, A.m, first, pend, g.gp, g.gci,
+                        (const TV *) g.gv, alpha, X, beta, Y);
+    LGH_LAUNCH_CHECK ();
+    return 0;
+  };
+  auto by_w = [&] (auto rgc, int w, int first, int groups) -> int {
+    if (w == VPL) return go (rgc, IC<VPL> (), first, groups);
+    if (w == 2 * VPL) return go (rgc, IC<2 * VPL> (), first, groups);
+    set_error ("spmm_grp_launch: panels per wave: 1 or 2 gathers of 16 B");
+    return 8;
+  };
+  auto by_rg = [&] (auto rgc) -> int {
+    const int whole = np / ppw;
+    LGH_TRY (by_w (rgc, ppw, p0, whole));
+    const int rem = np - whole * ppw;
+    if (rem == 0) return 0;
+    return by_w (rgc, rem <= VPL ? VPL : 2 * VPL, p0 + whole * ppw, 1);
+  };
+  if (g.rg == 4) return by_rg (IC<4> ());
+  if (g.rg == 8) return by_rg (IC<8> ());
+  set_error ("spmm_grp_launch: rows per group 4 or 8");
+  return 8;
+}
+
+// ---- autotuner of the grouped SpMM ----------------------------------------
+// At upload of a wide-row matrix, each candidate (4 or 8 rows per group;
+// 2 or 4 panels per wavefront with FP64 vectors, 4 or 8 with FP32) is timed on the
+// matrix itself over 4 zero panels with HIP events (best of 3 launches
+// after one warm-up); the rows per group that run the FP64 product fastest
+// (both products for FP32 values) are kept with the best panels per wave
+// of each vector precision.  Decisions are cached per (rows, nonzeros,
+// value precision), so later uploads of a matrix of that shape build one
+// copy and time nothing.
+
+namespace {
+struct GrpTune { int rg = 0, ppw[2] = { 0, 0 }; double ms[2] = { 0, 0 }; };
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int64_t, int>, GrpTune> g_tune;
+int g_force_rg = 0, g_force_ppw = 0;
+}
+
+void
+spmm_group_force (int rg, int ppw)
+{
+  std::lock_guard<std::mutex> lk (g_tune_mu);
+  g_force_rg = rg;
+  g_force_ppw = ppw;
+}
+
+std::string
+spmm_group_report ()
+{
+  std::lock_guard<std::mutex> lk (g_tune_mu);
+  std::string r;
+  char buf[256];
+  for (const auto &kv : g_tune) {
+    snprintf (buf, sizeof buf, "%d %lld %d %d %d %d %.4f %.4f\n",
+              std::get<0> (kv.first), (long long) std::get<1> (kv.first),
+              std::get<2> (kv.first), kv.second.rg, kv.second.ppw[0],
+              kv.second.ppw[1], kv.second.ms[0], kv.second.ms[1]);
+    r += buf;
+  }
+  return r;
+}
+
+// milliseconds per panel of the grouped product with ppw panels a wave
+template <typename TV, typename T>
+static int
+grp_time (const DevCSR &A, int ppw, const PMat &X, PMat &Y, double *ms)
+{
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  LGH_HIP (hipEventCreate (&e0));
+  if (hipEventCreate (&e1) != hipSuccess) { (void) hipEventDestroy (e0); return 2; }
+  const int np = X.npanels ();
+  float best = 0.f;
+  int rc = spmm_grp_launch<TV, T, T> (A, ppw, 0, np, (T) 1, (const T *) X.p,
+                                      (T) 0, (T *) Y.p);
+  for (int rep = 0; rep < 3 && !rc; rep++) {
+    float t = 0.f;
+    if (hipEventRecord (e0, stream ()) != hipSuccess) rc = 2;
+    if (!rc) rc = spmm_grp_launch<TV, T, T> (A, ppw, 0, np, (T) 1, (const T *) X.p,
+                                             (T) 0, (T *) Y.p);
+    if (!rc && (hipEventRecord (e1, stream ()) != hipSuccess
+                || hipEventSynchronize (e1) != hipSuccess
+                || hipEventElapsedTime (&t, e0, e1) != hipSuccess)) {
+      set_error ("grouped SpMM timing failed");
+      rc = 2;
+    }
+    if (!rc && (rep == 0 || t < best)) best = t;
+  }
+  (void) hipEventDestroy (e0);
+  (void) hipEventDestroy (e1);
+  *ms = (double) best / np;
+  return rc;
+}
+
+template <typename TV>
+static int
+grp_tune_one (const DevCSR &A, PMat (&X)[2], PMat (&Y)[2], GrpTune &t)
+{
+  static const int cand[2][3] = { { 2, 4, 0 }, { 4, 8, 0 } };
+  for (int vp = 0; vp < 2; vp++) {
+    t.ms[vp] = 0.0;
+    for (int k = 0; k < 3 && cand[vp][k]; k++) {
+      double ms = 0.0;
+      const int rc = vp == 0 ? grp_time<TV, double> (A, cand[vp][k], X[0], Y[0], &ms)
+                             : grp_time<TV, float> (A, cand[vp][k], X[1], Y[1], &ms);
+      if (rc) return rc;
+      if (t.ms[vp] == 0.0 || ms < t.ms[vp]) { t.ms[vp] = ms; t.ppw[vp] = cand[vp][k]; }
+    }
+  }
+  return 0;
+}
+
+int
+csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v)
+{
+  for (int i = 0; i < A.n; i++)
+    for (int k = rp[i] + 1; k < rp[i + 1]; k++)
+      if (ci[k] <= ci[k - 1]) return 0;           // unsorted: no copy
+  const auto key = std::make_tuple (A.n, A.nnz, A.prec);
+  int frg, fppw;
+  GrpTune known;
+  {
+    std::lock_guard<std::mutex> lk (g_tune_mu);
+    frg = g_force_rg;
+    fppw = g_force_ppw;
+    auto it = g_tune.find (key);
+    if (it != g_tune.end ()) known = it->second;
+  }
+  if (frg) {
+    LGH_TRY (csr_group_build (A, rp, ci, v, frg, A.grp));
+    A.grp_ppw[0] = fppw == 2 ? 2 : 4;
+    A.grp_ppw[1] = 2 * A.grp_ppw[0];
+    return 0;
+  }
+  if (known.rg) {
+    LGH_TRY (csr_group_build (A, rp, ci, v, known.rg, A.grp));
+    A.grp_ppw[0] = known.ppw[0];
+    A.grp_ppw[1] = known.ppw[1];
+    return 0;
+  }
+  PMat X[2], Y[2];
+  int rc = 0;
+  for (int vp = 0; vp < 2 && !rc; vp++) {
+    rc = pmat_alloc (X[vp], A.m, 256, 64, vp == 0 ? F64 : F32);
+    if (!rc) rc = pmat_alloc (Y[vp], A.n, 256, 64, vp == 0 ? F64 : F32);
+  }
+  GrpTune best;
+  double best_cost = 0.0;
+  for (int rg : { 4, 8 }) {
+    if (rc) break;
+    DevCSR C;                                    // candidate: the copy only
+    C.n = A.n; C.m = A.m; C.nnz = A.nnz; C.prec = A.prec;
+    rc = csr_group_build (A, rp, ci, v, rg, C.grp);
+    GrpTune t;
+    t.rg = rg;
+    if (!rc) rc = A.prec == F64 ? grp_tune_one<double> (C, X, Y, t)
+                                : grp_tune_one<float> (C, X, Y, t);
+    const double cost = A.prec == F64 ? t.ms[0] : t.ms[0] + t.ms[1];
+    if (!rc && (best.rg == 0 || cost < best_cost)) {
+      best = t;
+      best_cost = cost;
+      A.grp.release ();
+      A.grp = C.grp;
+      C.grp = GroupCopy ();                      // now owned by A
+    }
+  }
+  for (int vp = 0; vp < 2; vp++) { pmat_free (X[vp]); pmat_free (Y[vp]); }
+  if (rc) { A.grp.release (); return rc; }
+  A.grp_ppw[0] = best.ppw[0];
+  A.grp_ppw[1] = best.ppw[1];
+  std::lock_guard<std::mutex> lk (g_tune_mu);
+  g_tune[key] = best;
+  return 0;
 }
 
 // Y = alpha A X + beta Y for matrices with long rows (an aggregation
@@ -1103,18 +1336,10 @@ static int
 spmm_t (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
 {
   if constexpr (std::is_same<S, T>::value) {
-    if (A.gp && X.W == 64) {
-      constexpr int PPW = std::is_same<TV, double>::value ? SPMM_PPW64 : SPMM_PPW32;
-      const int ng = (A.n + RGROUP - 1) / RGROUP, np = X.npanels ();
-      const dim3 g ((unsigned) ((ng + TPB / 64 - 1) / (TPB / 64)),
-                    (unsigned) ((np + PPW - 1) / PPW));
-      hipLaunchKernelGGL ((spmm_grp_k<TV, T, S, PPW>), g, dim3 (TPB), 0,
-                          stream (), A.n, ng, A.m, np, A.gp, A.gci,
-                          (const TV *) A.gv, (T) alpha, (const S *) X.p,
-                          (T) beta, (S *) Y.p);
-      LGH_LAUNCH_CHECK ();
-      return 0;
-    }
+    if (A.grp.gp && X.W == 64)
+      return spmm_grp_launch<TV, T, S> (A, A.grp_ppw[std::is_same<T, double>::value ? 0 : 1],
+                                        0, X.npanels (), (T) alpha,
+                                        (const S *) X.p, (T) beta, (S *) Y.p);
   }
   if (row_width (X.W) && A.n > 0 && A.nnz >= 64 * (int64_t) A.n)
     return with_row_W (X.W, [&] (auto wc) -> int {

@@ -124,8 +124,21 @@ int prof_collect (int kind, int np, int prec, long *count, double *ms,
                   double *bytes, double *flops);
 
 // ---- objects ------------------------------------------------------------
-// rows per group of the row-grouped copy of a wide-row matrix
-static constexpr int RGROUP = 4;
+// Row-grouped copy of a wide-row matrix (the B of the F sweep): rows in
+// groups of rg (4 or 8); group g holds the sorted union of its rows' column
+// indices, padded to whole batches of grp_batch(rg) entries (zero values at
+// the last column), batches gp[g] .. gp[g+1]; batch b has the column
+// indices gci[b * nb .. + nb) and, per entry u, rg values
+// gv[(b * nb + u) * rg + r] (0 where row r has no entry).  A batch holds 32
+// values, which the SpMM reads with a few wide scalar loads.
+constexpr int grp_batch (int rg) { return 32 / rg; }
+struct GroupCopy {
+  int rg = 0;
+  int *gp = nullptr, *gci = nullptr;
+  void *gv = nullptr;             // values, precision of the owning matrix
+  int64_t nbatch = 0;
+  void release ();
+};
 
 struct DevCSR {
   int n = 0, m = 0;               // rows, cols
@@ -134,13 +147,10 @@ struct DevCSR {
   void *v = nullptr;              // values, precision `prec`
   int prec = F64;
   void *dinv = nullptr;           // 1/diag (same precision), square only
-  // row-grouped copy (csr_group_rows): rows in groups of RGROUP; group g
-  // holds the sorted union of its rows' column indices gci[gp[g] ..
-  // gp[g+1]) and, per union entry e, RGROUP values gv[e * RGROUP + r]
-  // (0 where row r of the group has no entry)
-  int *gp = nullptr, *gci = nullptr;
-  void *gv = nullptr;
-  int64_t nunion = 0;
+  // row-grouped copy (csr_group_rows) and the panels per wavefront its
+  // SpMM uses with FP64 / FP32 vectors (chosen by measurement)
+  GroupCopy grp;
+  int grp_ppw[2] = { 4, 8 };
   // scaled values (csr_scale_cols): vs[k] = v[k] * (vs_b1 * dinv[ci[k]]),
   // each product rounded in precision `prec`
   void *vs = nullptr;
@@ -150,10 +160,23 @@ struct DevCSR {
 // upload host CSR (values fp64 on the host) at precision prec
 int csr_upload (int n, int m, const int *rp, const int *ci, const double *v,
                 int prec, int want_dinv, DevCSR **out);
-// add the row-grouped copy to an uploaded matrix (same host CSR); rows
-// must hold strictly increasing column indices, else nothing is added
-// (returns 0 either way unless a device call fails)
+// build the row-grouped copy with rg rows per group (same host CSR, rows
+// with strictly increasing column indices) into out
+int csr_group_build (const DevCSR &A, const int *rp, const int *ci,
+                     const double *v, int rg, GroupCopy &out);
+// autotuner: build the row-grouped copies (rg 4 and 8), time their SpMM on
+// the device for each panels-per-wave choice and vector precision, keep
+// the fastest; decisions are cached per (rows, nonzeros, value precision).
+// Rows not strictly sorted: no copy (returns 0).
 int csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v);
+// force the grouped SpMM's configuration for later csr_group_rows calls:
+// rg rows per group, ppw (2 or 4) panels per wave with FP64 vectors and
+// twice that with FP32 (rg 0: autotune); the tests run every candidate
+// through it
+void spmm_group_force (int rg, int ppw);
+// the autotuner's decisions so far, one "n nnz vprec rg ppw64 ppw32 ms64"
+// line per matrix shape
+std::string spmm_group_report ();
 // add the scaled values for the factor b1 to an uploaded square matrix with
 // its dinv (same host CSR)
 int csr_scale_cols (DevCSR &A, const int *rp, const int *ci, const double *v,

@@ -89,15 +89,25 @@ def test_spmm_wide_rows_width_invariant(dev, prec_vals):
     assert np.array_equal(y64, y32)
     ref = B @ X if prec_vals == 0 else B.astype(np.float32).astype(np.float64) @ X
     assert rel(y64, ref) < 1e-13
-    # the row-grouped copy (groups of 4 rows, n = 2025 leaves a last group
-    # of 1): the same sums in the same order plus exact zeros, FP64 and FP32
-    # vectors
-    assert B.shape[0] % 4 == 1
-    assert np.array_equal(dev.spmm(B, X, prec_vals=prec_vals, W=64, grouped=True), y64)
-    for pv in (dev.PREC_F32,):
-        a = dev.spmm(B, X, prec_vals=1, prec_vec=pv, W=64)
-        b = dev.spmm(B, X, prec_vals=1, prec_vec=pv, W=64, grouped=True)
-        assert np.array_equal(a, b)
+    # the row-grouped copy (n = 2025 leaves a last group of 1 row): the same
+    # sums in the same order plus exact zeros, FP64 and FP32 vectors, for
+    # the autotuner's choice and for every candidate it times (4 or 8 rows
+    # a group, 2 or 4 panels a wavefront with FP64 vectors, 4 or 8 with
+    # FP32; 150 columns = 2 whole panels + 22, so the last launch has
+    # panels without data)
+    assert B.shape[0] % 8 == 1
+    a32 = dev.spmm(B, X, prec_vals=1, prec_vec=dev.PREC_F32, W=64)
+    try:
+        for rg, ppw in [(0, 0), (4, 2), (4, 4), (8, 2), (8, 4)]:
+            dev.spmm_group_config(rg, ppw)
+            got = dev.spmm(B, X, prec_vals=prec_vals, W=64, grouped=True)
+            assert np.array_equal(got, y64), (rg, ppw)
+            b32 = dev.spmm(B, X, prec_vals=1, prec_vec=dev.PREC_F32, W=64, grouped=True)
+            assert np.array_equal(a32, b32), (rg, ppw)
+    finally:
+        dev.spmm_group_config(0, 0)
+    rep = [r for r in dev.spmm_group_report() if r["n"] == B.shape[0]]
+    assert rep and rep[0]["rows_per_group"] in (4, 8)
 
 
 @pytest.mark.parametrize("prec", [0, 1])
