@@ -12,12 +12,13 @@
 #include <algorithm>
 #include <atomic>
 #include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <thread>
 #include <vector>
-#include <cstdio>
-#include <cstring>
 
 namespace lgh {
 
@@ -708,25 +709,79 @@ pmat_free (PMat &m)
 }
 
 // ---- rocSOLVER ------------------------------------------------------------
+// Diagnostic trace (environment variable LGH_TRACE_SOLVER = a file name):
+// each factorization appends "name n hash(input) hash(output) info", the
+// hashes (FNV-1a) over the n x n block it reads and writes, so that runs
+// of the same builds can be compared call by call.  Off unless set.
+static std::mutex g_trace_mu;
+
+static const char *
+trace_path ()
+{
+  const char *p = getenv ("LGH_TRACE_SOLVER");
+  return (p && *p) ? p : nullptr;
+}
+
+static uint64_t
+trace_hash (const double *A, int n, int lda, const double *w)
+{
+  std::vector<double> h ((size_t) std::max (lda, 1) * std::max (n, 1));
+  if (n > 0 && d2h (h.data (), A, sizeof (double) * (size_t) lda * n)) return 0;
+  uint64_t x = 1469598103934665603ULL;
+  auto mix = [&] (double v) {
+    uint64_t b;
+    memcpy (&b, &v, 8);
+    for (int k = 0; k < 8; k++) { x ^= (b >> (8 * k)) & 0xff; x *= 1099511628211ULL; }
+  };
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) mix (h[i + (size_t) j * lda]);
+  if (w) {
+    std::vector<double> hw ((size_t) n);
+    if (n > 0 && d2h (hw.data (), w, sizeof (double) * n)) return 0;
+    for (double v : hw) mix (v);
+  }
+  return x;
+}
+
+static void
+trace_line (const char *path, const char *name, int n, uint64_t hin,
+            uint64_t hout, int info)
+{
+  std::lock_guard<std::mutex> lk (g_trace_mu);
+  if (FILE *f = fopen (path, "a")) {
+    fprintf (f, "%s %d %016llx %016llx %d\n", name, n, (unsigned long long) hin,
+             (unsigned long long) hout, info);
+    fclose (f);
+  }
+}
+
+// The Cholesky factor and its inverse use the engine's own blocked kernels
+// (k_potrf, k_trtri): rocSOLVER's dpotrf returned different factors for
+// bit-identical Grams (and now and then a spurious failure) while other
+// streams kept the device busy, which made batched builds differ from
+// one-at-a-time ones.
 int
 sol_potrf (int upper, int n, double *A, int lda, int *info_host)
 {
-  rocblas_status s = rocsolver_dpotrf (t_ctx.rb, upper ? rocblas_fill_upper
-                                                   : rocblas_fill_lower,
-                                       n, A, lda, t_ctx.info);
-  LGH_CHECK (s == rocblas_status_success, 6, "rocsolver_dpotrf failed");
-  return d2h (info_host, t_ctx.info, sizeof (int));
+  const char *tp = trace_path ();
+  const uint64_t hin = tp ? trace_hash (A, n, lda, nullptr) : 0;
+  LGH_TRY (dzero (t_ctx.info, sizeof (int)));
+  LGH_TRY (k_potrf (upper, n, A, lda, t_ctx.info));
+  LGH_TRY (d2h (info_host, t_ctx.info, sizeof (int)));
+  if (tp) trace_line (tp, "potrf", n, hin, trace_hash (A, n, lda, nullptr), *info_host);
+  return 0;
 }
 
 int
 sol_trtri (int upper, int n, double *A, int lda, int *info_host)
 {
-  rocblas_status s = rocsolver_dtrtri (t_ctx.rb, upper ? rocblas_fill_upper
-                                                   : rocblas_fill_lower,
-                                       rocblas_diagonal_non_unit, n, A, lda,
-                                       t_ctx.info);
-  LGH_CHECK (s == rocblas_status_success, 6, "rocsolver_dtrtri failed");
-  return d2h (info_host, t_ctx.info, sizeof (int));
+  const char *tp = trace_path ();
+  const uint64_t hin = tp ? trace_hash (A, n, lda, nullptr) : 0;
+  LGH_TRY (dzero (t_ctx.info, sizeof (int)));
+  LGH_TRY (k_trtri (upper, n, A, lda, t_ctx.info));
+  LGH_TRY (d2h (info_host, t_ctx.info, sizeof (int)));
+  if (tp) trace_line (tp, "trtri", n, hin, trace_hash (A, n, lda, nullptr), *info_host);
+  return 0;
 }
 
 int
@@ -734,12 +789,15 @@ sol_syevd (int n, double *A, int lda, double *w_dev, int *info_host)
 {
   double *E = (double *) dmalloc (sizeof (double) * (size_t) (n > 0 ? n : 1));
   if (!E) return 5;
+  const char *tp = trace_path ();
+  const uint64_t hin = tp ? trace_hash (A, n, lda, nullptr) : 0;
   rocblas_status s = rocsolver_dsyevd (t_ctx.rb, rocblas_evect_original,
                                        rocblas_fill_upper, n, A, lda, w_dev,
                                        E, t_ctx.info);
   int rc = 0;
   if (s != rocblas_status_success) { set_error ("rocsolver_dsyevd failed"); rc = 6; }
   if (!rc) rc = d2h (info_host, t_ctx.info, sizeof (int));
+  if (!rc && tp) trace_line (tp, "syevd", n, hin, trace_hash (A, n, lda, w_dev), *info_host);
   dfree (E);
   return rc;
 }

@@ -685,6 +685,269 @@ spmm_grp_launch (const DevCSR &A, int ppw, int p0, int np, T alpha,
   return 8;
 }
 
+// ---- Cholesky factor and triangular inverse (FP64, deterministic) ---------
+// The CholQR2 steps' potrf / trtri (glr_common_impl.h:387, 402;
+// glr_scalapack_impl.h:128-143) in 64 x 64 blocks: each block is factored,
+// solved or updated by one workgroup in LDS with every sum in a fixed order
+// and no atomics, so a factor depends only on its input (a library potrf
+// gave different factors for identical Grams while other streams kept the
+// device busy).  The upper variants work on the transpose (the lower
+// triangle of A^T is the upper triangle of A); the other triangle is not
+// written.  info: LAPACK's convention (first non-positive pivot / zero
+// diagonal, 1-based), set once.
+
+struct TriView {
+  double *A;
+  int lda, up;
+  __device__ double &at (int i, int j) const
+  { return up ? A[j + (size_t) i * lda] : A[i + (size_t) j * lda]; }
+};
+
+static constexpr int CB = 64;        // block size
+static constexpr int CBP = CB + 1;   // LDS row pitch
+
+// *info = min(*info, v) over the positive values (0: no failure yet)
+__device__ static void
+info_min (int *info, int v)
+{
+  int old = *info;
+  while (old == 0 || old > v) {
+    const int prev = atomicCAS (info, old, v);
+    if (prev == old) break;
+    old = prev;
+  }
+}
+
+// factor the diagonal block k0 .. k0 + b - 1 in place (right-looking)
+__global__ void __launch_bounds__ (TPB)
+chol_diag_k (TriView V, int n, int k0, int *info)
+{
+  __shared__ double s[CB][CBP];
+  const int b = min (CB, n - k0), t = (int) threadIdx.x;
+  for (int e = t; e < CB * CB; e += TPB) {
+    const int i = e % CB, j = e / CB;
+    if (i < b && j <= i) s[i][j] = V.at (k0 + i, k0 + j);
+  }
+  __syncthreads ();
+  for (int j = 0; j < b; j++) {
+    if (t == 0) {
+      const double d = s[j][j];
+      if (!(d > 0.0) && *info == 0) *info = k0 + j + 1;
+      s[j][j] = sqrt (d);
+    }
+    __syncthreads ();
+    const double djj = s[j][j];
+    for (int i = j + 1 + t; i < b; i += TPB) s[i][j] = s[i][j] / djj;
+    __syncthreads ();
+    const int m = b - j - 1;           // trailing (i, c), j < c <= i < b
+    for (int e = t; e < m * m; e += TPB) {
+      const int i = j + 1 + e % m, c = j + 1 + e / m;
+      if (c <= i) s[i][c] -= s[i][j] * s[c][j];
+    }
+    __syncthreads ();
+  }
+  for (int e = t; e < CB * CB; e += TPB) {
+    const int i = e % CB, j = e / CB;
+    if (i < b && j <= i) V.at (k0 + i, k0 + j) = s[i][j];
+  }
+}
+
+// block rows below the diagonal block k0: X = A(r, k) L(k, k)^-T
+__global__ void __launch_bounds__ (TPB)
+chol_panel_k (TriView V, int n, int k0)
+{
+  __shared__ double L[CB][CBP], X[CB][CBP];
+  const int b = min (CB, n - k0), r0 = k0 + CB * (1 + (int) blockIdx.x);
+  const int br = min (CB, n - r0), t = (int) threadIdx.x;
+  for (int e = t; e < CB * CB; e += TPB) {
+    const int i = e % CB, j = e / CB;
+    if (i < b && j <= i) L[i][j] = V.at (k0 + i, k0 + j);
+    if (i < br && j < b) X[i][j] = V.at (r0 + i, k0 + j);
+  }
+  __syncthreads ();
+  if (t < br)
+    for (int c = 0; c < b; c++) {
+      double x = X[t][c];
+      for (int j = 0; j < c; j++) x -= X[t][j] * L[c][j];
+      X[t][c] = x / L[c][c];
+    }
+  __syncthreads ();
+  for (int e = t; e < CB * CB; e += TPB) {
+    const int i = e % CB, j = e / CB;
+    if (i < br && j < b) V.at (r0 + i, k0 + j) = X[i][j];
+  }
+}
+
+// C(bi, bj) -= P(bi) P(bj)^T over the trailing lower blocks, block pair
+// number blockIdx.x in (bi, bj), bj <= bi order; P = the panel of column
+// block k0 (b columns).  A thread owns a 4 x 4 piece of the 64 x 64 block.
+__global__ void __launch_bounds__ (TPB)
+chol_update_k (TriView V, int n, int k0)
+{
+  __shared__ double P[CB][CBP], Q[CB][CBP];
+  int bi = 0, rem = (int) blockIdx.x;
+  while (rem > bi) { rem -= bi + 1; bi++; }
+  const int bj = rem;
+  const int b = min (CB, n - k0);
+  const int r0 = k0 + CB * (1 + bi), c0 = k0 + CB * (1 + bj);
+  const int br = min (CB, n - r0), bc = min (CB, n - c0), t = (int) threadIdx.x;
+  for (int e = t; e < CB * CB; e += TPB) {
+    const int i = e % CB, j = e / CB;
+    P[i][j] = (i < br && j < b) ? V.at (r0 + i, k0 + j) : 0.0;
+    Q[i][j] = (i < bc && j < b) ? V.at (c0 + i, k0 + j) : 0.0;
+  }
+  __syncthreads ();
+  const int ti = (t / 16) * 4, tj = (t % 16) * 4;
+  double acc[4][4] = {};
+  for (int kk = 0; kk < b; kk++)
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc[u][v] += P[ti + u][kk] * Q[tj + v][kk];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int i = ti + u, j = tj + v;
+      if (i < br && j < bc && (bi != bj || j <= i)) V.at (r0 + i, c0 + j) -= acc[u][v];
+    }
+}
+
+// W(blk, blk) = L(blk, blk)^-1 for every diagonal block (one workgroup
+// each; a thread per column)
+__global__ void __launch_bounds__ (TPB)
+trinv_diag_k (TriView L, double *W, int ldw, int n, int *info)
+{
+  __shared__ double s[CB][CBP], w[CB][CBP];
+  const int k0 = CB * (int) blockIdx.x, b = min (CB, n - k0), t = (int) threadIdx.x;
+  for (int e = t; e < CB * CB; e += TPB) {
+    const int i = e % CB, j = e / CB;
+    s[i][j] = (i < b && j <= i) ? L.at (k0 + i, k0 + j) : 0.0;
+  }
+  __syncthreads ();
+  if (t < b) {
+    const int j = t;
+    if (s[j][j] == 0.0) info_min (info, k0 + j + 1);
+    w[j][j] = 1.0 / s[j][j];
+    for (int i = j + 1; i < b; i++) {
+      double x = 0.0;
+      for (int k = j; k < i; k++) x += s[i][k] * w[k][j];
+      w[i][j] = -x / s[i][i];
+    }
+  }
+  __syncthreads ();
+  for (int e = t; e < CB * CB; e += TPB) {
+    const int i = e % CB, j = e / CB;
+    if (i < b && j <= i) W[(k0 + i) + (size_t) (k0 + j) * ldw] = w[i][j];
+  }
+}
+
+// block row bi of the inverse: W(bi, bk) = -W(bi, bi) sum_{bj = bk}^{bi-1}
+// L(bi, bj) W(bj, bk), block column bk = blockIdx.x < bi
+__global__ void __launch_bounds__ (TPB)
+trinv_row_k (TriView L, double *W, int ldw, int n, int bi)
+{
+  __shared__ double P[CB][CBP], Q[CB][CBP];
+  const int bk = (int) blockIdx.x, t = (int) threadIdx.x;
+  const int r0 = CB * bi, c0 = CB * bk, br = min (CB, n - r0);
+  const int ti = (t / 16) * 4, tj = (t % 16) * 4;
+  double acc[4][4] = {};
+  for (int bj = bk; bj < bi; bj++) {
+    const int m0 = CB * bj;
+    __syncthreads ();
+    for (int e = t; e < CB * CB; e += TPB) {
+      const int i = e % CB, j = e / CB;
+      P[i][j] = (i < br) ? L.at (r0 + i, m0 + j) : 0.0;     // L(bi, bj)
+      Q[i][j] = W[(m0 + i) + (size_t) (c0 + j) * ldw];        // W(bj, bk)
+    }
+    __syncthreads ();
+    for (int kk = 0; kk < CB; kk++)
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[u][v] += P[ti + u][kk] * Q[kk][tj + v];
+  }
+  __syncthreads ();
+  for (int e = t; e < CB * CB; e += TPB) {
+    const int i = e % CB, j = e / CB;
+    P[i][j] = (i < br && j <= i) ? W[(r0 + i) + (size_t) (r0 + j) * ldw] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) Q[ti + u][tj + v] = acc[u][v];
+  __syncthreads ();
+  double o[4][4] = {};
+  for (int kk = 0; kk < CB; kk++)
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) o[u][v] += P[ti + u][kk] * Q[kk][tj + v];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+      if (ti + u < br) W[(r0 + ti + u) + (size_t) (c0 + tj + v) * ldw] = -o[u][v];
+}
+
+// A's triangle = W's lower triangle (transposed for the upper variant)
+__global__ void __launch_bounds__ (TPB)
+trinv_store_k (TriView L, const double *W, int ldw, int n)
+{
+  const size_t e = (size_t) blockIdx.x * TPB + threadIdx.x;
+  if (e >= (size_t) n * n) return;
+  const int i = (int) (e % n), j = (int) (e / n);
+  if (j <= i) L.at (i, j) = W[i + (size_t) j * ldw];
+}
+
+int
+k_potrf (int upper, int n, double *A, int lda, int *info_dev)
+{
+  if (n <= 0) return 0;
+  const TriView V { A, lda, upper };
+  const int nb = (n + CB - 1) / CB;
+  for (int kb = 0; kb < nb; kb++) {
+    const int k0 = CB * kb, m = nb - kb - 1;
+    hipLaunchKernelGGL (chol_diag_k, dim3 (1), dim3 (TPB), 0, stream (), V, n, k0, info_dev);
+    LGH_LAUNCH_CHECK ();
+    if (m == 0) break;
+    hipLaunchKernelGGL (chol_panel_k, dim3 ((unsigned) m), dim3 (TPB), 0, stream (), V, n, k0);
+    LGH_LAUNCH_CHECK ();
+    hipLaunchKernelGGL (chol_update_k, dim3 ((unsigned) (m * (m + 1) / 2)), dim3 (TPB), 0,
+                        stream (), V, n, k0);
+    LGH_LAUNCH_CHECK ();
+  }
+  return 0;
+}
+
+int
+k_trtri (int upper, int n, double *A, int lda, int *info_dev)
+{
+  if (n <= 0) return 0;
+  const TriView V { A, lda, upper };
+  const int nb = (n + CB - 1) / CB, ldw = nb * CB;
+  double *W = (double *) dmalloc (sizeof (double) * (size_t) ldw * ldw);
+  if (!W) return 5;
+  int rc = dzero (W, sizeof (double) * (size_t) ldw * ldw);
+  if (!rc) {
+    hipLaunchKernelGGL (trinv_diag_k, dim3 ((unsigned) nb), dim3 (TPB), 0, stream (),
+                        V, W, ldw, n, info_dev);
+    if (hipGetLastError () != hipSuccess) { set_error ("trinv_diag_k launch"); rc = 76; }
+  }
+  for (int bi = 1; bi < nb && !rc; bi++) {
+    hipLaunchKernelGGL (trinv_row_k, dim3 ((unsigned) bi), dim3 (TPB), 0, stream (),
+                        V, W, ldw, n, bi);
+    if (hipGetLastError () != hipSuccess) { set_error ("trinv_row_k launch"); rc = 76; }
+  }
+  if (!rc) {
+    hipLaunchKernelGGL (trinv_store_k, dim3 (grid1 ((size_t) n * n, TPB)), dim3 (TPB), 0,
+                        stream (), V, W, ldw, n);
+    if (hipGetLastError () != hipSuccess) { set_error ("trinv_store_k launch"); rc = 76; }
+  }
+  dfree (W);
+  return rc;
+}
+
 // ---- autotuner of the grouped SpMM ----------------------------------------
 // At upload of a wide-row matrix, each candidate (4 or 8 rows per group;
 // 2 or 4 panels per wavefront with FP64 vectors, 4 or 8 with FP32) is timed on the

@@ -279,7 +279,15 @@ int k_upper_pad (const double *Rinv, int n, int ldr, double *C, int ldc,
                  int ccols);
 int k_trace (const double *A, int n, int ld, double *out_dev);
 
-// rocSOLVER wrappers (device pointers, fp64)
+// own FP64 Cholesky factor (A = L L^T, or U^T U with upper) and triangular
+// inverse of the factor, in place on the triangle, 64 x 64 blocks in fixed
+// order (results depend only on the input); *info_dev (device int, zeroed by
+// the caller) receives LAPACK's info
+int k_potrf (int upper, int n, double *A, int lda, int *info_dev);
+int k_trtri (int upper, int n, double *A, int lda, int *info_dev);
+
+// dense factorizations (device pointers, fp64): potrf / trtri run the own
+// kernels above, the eigensolvers rocSOLVER
 int sol_potrf (int upper, int n, double *A, int lda, int *info_host);
 int sol_trtri (int upper, int n, double *A, int lda, int *info_host);
 int sol_syevd (int n, double *A, int lda, double *w_dev, int *info_host);
