@@ -64,6 +64,19 @@ int lgh_hip_randn_host_block (uint64_t seed, int64_t row0, int64_t col0,
 int lgh_hip_randn_fill (uint64_t seed, int64_t row0, int64_t col0, int nr,
                         int nc, int W, int prec, double *out, int ld);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  The draws of lgh_hip_randn_fill, bit for bit, written into
+ * caller device memory `out` (column-major, leading dimension ld >= nr;
+ * prec 0 FP64, 1 FP32); rows nr .. ld-1 are left as they are.  Nothing is
+ * allocated: the list of draws finished with the platform log lives in a
+ * static 32 MiB device array per GPU (held until the process ends).  At
+ * most 33554432 draws a call, larger requests are refused.  Call
+ * lgh_hip_init on the calling thread first.  Calls that overlap on one GPU
+ * race (they share the array).  Returns after the draws are written. */
+int lgh_hip_randn_fill_noalloc (uint64_t seed, int64_t row0, int64_t col0,
+                                int nr, int nc, int prec, void *out, int ld);
+
 /* Y = A X (A: n x m CSR) on ncols columns */
 int lgh_hip_spmm (int n, int m, const int *rp, const int *ci, const double *v,
                   int prec_vals, int prec_vec, int W, int ncols,

@@ -231,6 +231,8 @@ _SIGS = {
     "lgh_hip_spmm_grouped": (C.c_int, [C.c_int, C.c_int, ip, ip, dp, C.c_int,
                                        C.c_int, C.c_int, C.c_int, dp, C.c_int,
                                        dp, C.c_int]),
+    "lgh_hip_randn_fill_noalloc": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.c_int,
+                                             C.c_int, C.c_int, C.c_void_p, C.c_int]),
     "lgh_hip_spmm_group_config": (None, [C.c_int, C.c_int]),
     "lgh_hip_spmm_group_report": (C.c_int, [C.c_char_p, C.c_int]),
     "lgh_hip_cheb_jacobi": (C.c_int, [C.c_int, ip, ip, dp, C.c_int, C.c_int,

@@ -125,6 +125,13 @@ lgh_hip_randn_fill (uint64_t seed, int64_t row0, int64_t col0, int nr, int nc,
   return rc;
 }
 
+int
+lgh_hip_randn_fill_noalloc (uint64_t seed, int64_t row0, int64_t col0, int nr,
+                            int nc, int prec, void *out, int ld)
+{
+  return randn_fill_noalloc (seed, row0, col0, nr, nc, prec, out, ld);
+}
+
 static int
 spmm_host (int n, int m, const int *rp, const int *ci, const double *v,
            int prec_vals, int prec_vec, int W, int ncols, const double *X,

@@ -77,10 +77,19 @@ dev_init ()
       LGH_HIP (hipGetDeviceCount (&ndev));
       LGH_CHECK (ndev > 0, 3, "no HIP device visible: the engine needs a GPU");
       LGH_HIP (hipSetDevice (0));
+      // the no-allocation entry points' scratch (kernels.hip), with the
+      // device current
+      LGH_TRY (scratch_init ());
       g_init = true;
     }
   }
   return ctx_init (t_ctx);
+}
+
+bool
+dev_ready ()
+{
+  return g_init && t_ctx.s != nullptr;
 }
 
 hipStream_t

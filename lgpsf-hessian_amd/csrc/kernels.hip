@@ -119,26 +119,29 @@ dzero (void *dst, size_t bytes)
 static constexpr int RANDN_DPT = 16;        // draws per thread
 static constexpr int RANDN_LDS_CAP = 512;   // list entries a block gathers
 
+// elements e0 .. e1 - 1 of a panel layout of width W with nrows rows a
+// panel (W = 1: column-major with leading dimension nrows); rows past
+// nvalid are not written, columns past ncols are written as 0
 template <typename T, int W>
 __global__ void __launch_bounds__ (TPB)
-randn_k (T *out, int nrows, int ncols, int npanels, uint64_t seed,
-         int64_t row0, int64_t col0, uint64_t *fe, double *fq, double *fu,
-         unsigned *cnt, unsigned cap)
+randn_k (T *out, int nrows, int nvalid, int ncols, uint64_t seed,
+         int64_t row0, int64_t col0, size_t e0, size_t e1, uint64_t *fe,
+         double *fq, double *fu, unsigned *cnt, unsigned cap)
 {
   __shared__ uint64_t se[RANDN_LDS_CAP];
   __shared__ double sq[RANDN_LDS_CAP], su[RANDN_LDS_CAP];
   __shared__ unsigned sn, sbase;
   if (threadIdx.x == 0) sn = 0u;
   __syncthreads ();
-  const size_t tot = (size_t) npanels * nrows * W;
-  const size_t base = (size_t) blockIdx.x * TPB * RANDN_DPT;
+  const size_t base = e0 + (size_t) blockIdx.x * TPB * RANDN_DPT;
   for (int k = 0; k < RANDN_DPT; k++) {
     const size_t e = base + (size_t) k * TPB + threadIdx.x;
-    if (e >= tot) break;
+    if (e >= e1) break;
     const int c = (int) (e % W);
     const size_t rest = e / W;
     const int i = (int) (rest % nrows);
     const int j = (int) (rest / nrows) * W + c;
+    if (i >= nvalid) continue;
     if (j >= ncols) { out[e] = (T) 0; continue; }
     bool f;
     double u1, q;
@@ -223,14 +226,14 @@ k_randn (PMat &out, uint64_t seed, int64_t row0, int64_t col0)
       constexpr int W = decltype (wt)::value;
       if (out.prec == F64)
         hipLaunchKernelGGL ((randn_k<double, W>), dim3 (nblk), dim3 (TPB), 0,
-                            stream (), (double *) out.p, out.nrows, out.ncols,
-                            out.npanels (), seed, row0, col0, fe, fq, fu, cnt,
-                            (unsigned) cap);
+                            stream (), (double *) out.p, out.nrows, out.nrows,
+                            out.ncols, seed, row0, col0, (size_t) 0, tot, fe,
+                            fq, fu, cnt, (unsigned) cap);
       else
         hipLaunchKernelGGL ((randn_k<float, W>), dim3 (nblk), dim3 (TPB), 0,
-                            stream (), (float *) out.p, out.nrows, out.ncols,
-                            out.npanels (), seed, row0, col0, fe, fq, fu, cnt,
-                            (unsigned) cap);
+                            stream (), (float *) out.p, out.nrows, out.nrows,
+                            out.ncols, seed, row0, col0, (size_t) 0, tot, fe,
+                            fq, fu, cnt, (unsigned) cap);
       LGH_LAUNCH_CHECK ();
       return 0;
     };
@@ -271,6 +274,107 @@ k_randn (PMat &out, uint64_t seed, int64_t row0, int64_t col0)
   }
   return 0;
 }
+// ---- the fill without allocation --------------------------------------------
+// lgh_hip_randn_fill_noalloc: the same draws into caller memory, for callers
+// that cannot allocate device memory.  The flag list of the draws finished
+// with the platform log lives in `scratch`, a file-scope device array of
+// NA_SCRATCH_BYTES (one per GPU, held until the process ends): NA_CAP entries
+// of (offset, q, u1, log q), then the counter.  A call fills at most
+// NA_MAX_DRAWS draws (about 2.2 % of FP64 draws are listed, ~2^-29 of FP32
+// ones); a fill that lists more than NA_CAP runs again in slices of NA_CAP
+// elements, each of which cannot.  The listed q pass through a static host
+// array.  Every call on a GPU shares its array (and the host array):
+// overlapping calls race.
+static constexpr size_t NA_CAP = size_t (1) << 20;
+static constexpr size_t NA_SCRATCH_BYTES = NA_CAP * 32 + 256;
+static constexpr size_t NA_MAX_DRAWS = size_t (1) << 25;
+static __device__ __align__ (256) unsigned char scratch[NA_SCRATCH_BYTES];
+static constexpr int NA_MAXDEV = 64;
+static void *g_scratch_dev[NA_MAXDEV];
+static double g_na_host[NA_CAP];
+
+int
+scratch_init ()
+{
+  int dev = 0;
+  LGH_HIP (hipGetDevice (&dev));
+  LGH_CHECK (dev >= 0 && dev < NA_MAXDEV, 8, "scratch_init: device index out of range");
+  if (g_scratch_dev[dev]) return 0;
+  void *p = nullptr;
+  LGH_HIP (hipGetSymbolAddress (&p, HIP_SYMBOL (scratch)));
+  g_scratch_dev[dev] = p;
+  return 0;
+}
+
+int
+randn_fill_noalloc (uint64_t seed, int64_t row0, int64_t col0, int nr, int nc,
+                    int prec, void *out, int ld)
+{
+  LGH_CHECK (dev_ready (), 73, "lgh_hip_randn_fill_noalloc: call lgh_hip_init on "
+             "this thread first (this entry point creates nothing)");
+  LGH_CHECK (nr >= 0 && nc >= 0 && ld >= std::max (nr, 1) && (prec == 0 || prec == 1)
+             && (out || (size_t) nr * nc == 0), 63,
+             "lgh_hip_randn_fill_noalloc: bad arguments");
+  const size_t tot = (size_t) ld * nc;
+  LGH_CHECK ((size_t) nr * nc <= NA_MAX_DRAWS && tot <= 2 * NA_MAX_DRAWS, 60,
+             "lgh_hip_randn_fill_noalloc: more than 33554432 draws (or a leading "
+             "dimension above twice that per call): the no-allocation limit");
+  if ((size_t) nr * nc == 0) return 0;
+  int dev = 0;
+  LGH_HIP (hipGetDevice (&dev));
+  LGH_CHECK (dev >= 0 && dev < NA_MAXDEV && g_scratch_dev[dev], 73,
+             "lgh_hip_randn_fill_noalloc: no scratch on this device");
+  char *sp = (char *) g_scratch_dev[dev];
+  uint64_t *fe = (uint64_t *) sp;
+  double *fq = (double *) (sp + NA_CAP * 8), *fu = (double *) (sp + NA_CAP * 16),
+         *fl = (double *) (sp + NA_CAP * 24);
+  unsigned *cnt = (unsigned *) (sp + NA_CAP * 32);
+  const hipStream_t s = stream ();
+  // elements [e0, e1) of the column-major block; false: the list overflowed
+  auto pass = [&] (size_t e0, size_t e1, bool *fits) -> int {
+    *fits = true;
+    LGH_TRY (dzero (cnt, 4));
+    const unsigned nblk = (unsigned) ((e1 - e0 + (size_t) TPB * RANDN_DPT - 1)
+                                      / ((size_t) TPB * RANDN_DPT));
+    if (prec == 0)
+      hipLaunchKernelGGL ((randn_k<double, 1>), dim3 (nblk), dim3 (TPB), 0, s,
+                          (double *) out, ld, nr, nc, seed, row0, col0, e0, e1,
+                          fe, fq, fu, cnt, (unsigned) NA_CAP);
+    else
+      hipLaunchKernelGGL ((randn_k<float, 1>), dim3 (nblk), dim3 (TPB), 0, s,
+                          (float *) out, ld, nr, nc, seed, row0, col0, e0, e1,
+                          fe, fq, fu, cnt, (unsigned) NA_CAP);
+    LGH_LAUNCH_CHECK ();
+    unsigned n = 0;
+    LGH_HIP (hipMemcpyAsync (&n, cnt, 4, hipMemcpyDeviceToHost, s));
+    LGH_HIP (hipStreamSynchronize (s));
+    if (n > NA_CAP) { *fits = false; return 0; }
+    if (n == 0) return 0;
+    LGH_HIP (hipMemcpyAsync (g_na_host, fq, (size_t) n * 8, hipMemcpyDeviceToHost, s));
+    LGH_HIP (hipStreamSynchronize (s));
+    for (unsigned k = 0; k < n; k++) g_na_host[k] = ::log (g_na_host[k]);
+    LGH_HIP (hipMemcpyAsync (fl, g_na_host, (size_t) n * 8, hipMemcpyHostToDevice, s));
+    if (prec == 0)
+      hipLaunchKernelGGL ((randn_fix_k<double>), dim3 (grid1 (n, TPB)), dim3 (TPB), 0, s,
+                          (double *) out, fe, fq, fu, fl, n);
+    else
+      hipLaunchKernelGGL ((randn_fix_k<float>), dim3 (grid1 (n, TPB)), dim3 (TPB), 0, s,
+                          (float *) out, fe, fq, fu, fl, n);
+    LGH_LAUNCH_CHECK ();
+    // the host array and the list are reused by the next pass
+    LGH_HIP (hipStreamSynchronize (s));
+    return 0;
+  };
+  bool fits = true;
+  LGH_TRY (pass (0, tot, &fits));
+  for (size_t e0 = 0; !fits && e0 < tot; e0 += NA_CAP) {
+    bool f2 = true;
+    LGH_TRY (pass (e0, std::min (tot, e0 + NA_CAP), &f2));
+    LGH_CHECK (f2, 8, "lgh_hip_randn_fill_noalloc: flag list overflow");
+  }
+  return 0;
+}
+
 
 // ---------------------------------------------------------------------------
 // W = 64 row kernels.  A row of a 64-column panel is 128 B (bf16), 256 B

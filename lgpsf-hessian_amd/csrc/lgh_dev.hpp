@@ -49,6 +49,12 @@ int  dev_init ();                 // idempotent; selects device 0 and
 void dev_thread_exit ();          // a worker thread leaves the engine:
                                   // waits for its stream, hands its cached
                                   // blocks to the shared list
+// the library is initialized and the calling thread has its stream (so an
+// engine call on it creates nothing)
+bool dev_ready ();
+// once per device, with it current (dev_init): the address of the
+// no-allocation entry points' static scratch array
+int  scratch_init ();
 hipStream_t stream ();
 // the thread's auxiliary stream (default priority, created on first use):
 // work that may overlap the main stream's, ordered by events
@@ -201,6 +207,11 @@ void pmat_free (PMat &m);
 // it on the host; synchronizes the stream when there are any)
 int k_randn (PMat &out, uint64_t seed, int64_t row0, int64_t col0);
 void randn_host_release ();        // the thread's pinned list buffer
+// the same fill into caller memory (column-major, leading dimension ld,
+// FP64 or FP32) with no allocation: the flag list lives in the static
+// scratch array (see kernels.hip for the limits)
+int randn_fill_noalloc (uint64_t seed, int64_t row0, int64_t col0, int nr,
+                        int nc, int prec, void *out, int ld);
 // Y = alpha A X + beta Y over all panels of X/Y (same W, prec of X/Y)
 int k_spmm (const DevCSR &A, double alpha, const PMat &X, double beta,
             PMat &Y);
