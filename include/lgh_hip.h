@@ -95,7 +95,9 @@ int lgh_hip_spmm_grouped (int n, int m, const int *rp, const int *ci,
                           double *Y, int ldy);
 /* force the grouped product's configuration for matrices uploaded later
  * (rg 4 or 8 rows per group, ppw 2 or 4 panels per wavefront with FP64
- * vectors, twice that with FP32; rg = 0 returns to the autotuner) */
+ * vectors, twice that with FP32; while forced, every wide-row B of the
+ * engine gets the grouped copy, whatever its size; rg = 0 returns to the
+ * autotuner and the size threshold of 2^24 nonzeros) */
 void lgh_hip_spmm_group_config (int rg, int ppw);
 /* the autotuner's decisions so far as text, one line per matrix shape:
  * "rows nonzeros value_prec rg ppw_fp64 ppw_fp32 ms_per_panel_fp64

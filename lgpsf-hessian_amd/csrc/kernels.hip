@@ -1077,6 +1077,13 @@ spmm_group_force (int rg, int ppw)
   g_force_ppw = ppw;
 }
 
+bool
+spmm_group_forced ()
+{
+  std::lock_guard<std::mutex> lk (g_tune_mu);
+  return g_force_rg != 0;
+}
+
 std::string
 spmm_group_report ()
 {
@@ -1708,7 +1715,11 @@ spmm_t (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
                                         0, X.npanels (), (T) alpha,
                                         (const S *) X.p, (T) beta, (S *) Y.p);
   }
-  if (row_width (X.W) && A.n > 0 && A.nnz >= 64 * (int64_t) A.n)
+  // rows of 64 entries and more on average in a rectangular matrix (a
+  // restriction onto a small coarse level) take the chunked kernel; a
+  // square operator with wide rows (a PSF B) keeps the row-order sums of
+  // spmm64_k, which its row-grouped copy reproduces bit for bit
+  if (row_width (X.W) && A.n > 0 && A.n != A.m && A.nnz >= 64 * (int64_t) A.n)
     return with_row_W (X.W, [&] (auto wc) -> int {
       constexpr int W = decltype (wc)::value;
       hipLaunchKernelGGL ((spmm_long_k<TV, T, S, W>),

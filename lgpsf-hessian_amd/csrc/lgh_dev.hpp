@@ -180,6 +180,8 @@ int csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v);
 // twice that with FP32 (rg 0: autotune); the tests run every candidate
 // through it
 void spmm_group_force (int rg, int ppw);
+// a configuration is forced (wide-row matrices of any size then get the copy)
+bool spmm_group_forced ();
 // the autotuner's decisions so far, one "n nnz vprec rg ppw64 ppw32 ms64"
 // line per matrix shape
 std::string spmm_group_report ();

@@ -487,10 +487,11 @@ mat_device_csr (Mat A, int prec, int want_dinv, DevCSR **out)
   // wide rows (B: a PSF row has tens to hundreds of entries, mostly shared
   // with the neighbouring rows) also get the row-grouped copy its SpMM
   // reads, when the matrix is large enough for the product to matter
-  // (building the copy costs host time of the order of one small build);
-  // stencil operators (Z, the AMG levels) do not
+  // (building the copy costs host time of the order of one small build)
+  // or a configuration is forced (tests of every candidate on small
+  // matrices); stencil operators (Z, the AMG levels) do not
   if (!want_dinv && A->m > 0 && rp[A->m] >= 16 * (int64_t) A->m
-      && rp[A->m] >= ((int64_t) 1 << 24)) {
+      && (rp[A->m] >= ((int64_t) 1 << 24) || spmm_group_forced ())) {
     const int rc = csr_group_rows (*d, rp, ci, v);
     if (rc) { delete d; return rc; }
   }
