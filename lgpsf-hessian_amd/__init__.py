@@ -438,26 +438,50 @@ class Prior:
         return [cls(vp(hs[i]), keep=mats[i]) for i in range(k)]
 
     @classmethod
-    def diagonal_callbacks(cls, zdiag, mass):
+    def diagonal_callbacks(cls, zdiag, mass, blocked=False, calls=None):
         """Prior with Z = diag(zdiag) supplied through the Vec callbacks
-        (the exact "fake solver" prior of tests/test_glr.cpp:67-85)."""
+        (the exact "fake solver" prior of tests/test_glr.cpp:67-85); with
+        blocked=True also through the MATDENSE block callbacks
+        solveZ_blocked / solveZt_blocked (prior.h:53-54).  calls (a dict)
+        counts the invocations of each callback."""
         L = lib()
         zd = np.asarray(zdiag, dtype=np.float64).copy()
         n = len(zd)
+        cnt = calls if calls is not None else {}
 
         def arr(h):
             p = dp()
             L.VecGetArray(h, C.byref(p))
             return np.ctypeslib.as_array(p, shape=(n,))
 
+        def marr(h):
+            rows, cols = C.c_int(), C.c_int()
+            L.MatGetSize(h, C.byref(rows), C.byref(cols))
+            p = dp()
+            L.MatDenseGetArray(h, C.byref(p))
+            return np.ctypeslib.as_array(p, shape=(cols.value, rows.value))
+
         def applyZ(x, y, ctx):
+            cnt["applyZ"] = cnt.get("applyZ", 0) + 1
             arr(y)[:] = arr(x) * zd
 
         def solveZ(x, y, ctx):
+            cnt["solveZ"] = cnt.get("solveZ", 0) + 1
             arr(y)[:] = arr(x) / zd
 
+        def solveZ_blocked(X, Y, ctx):
+            cnt["solveZ_blocked"] = cnt.get("solveZ_blocked", 0) + 1
+            marr(Y)[:] = marr(X) / zd[None, :]
+
+        def solveZt_blocked(X, Y, ctx):
+            cnt["solveZt_blocked"] = cnt.get("solveZt_blocked", 0) + 1
+            marr(Y)[:] = marr(X) / zd[None, :]
+
         cbs = (VECFN(applyZ), VECFN(solveZ))
-        cb = Callbacks(cbs[0], VECFN(), cbs[1], VECFN(), MATFN(), MATFN(), None)
+        mfs = (MATFN(solveZ_blocked), MATFN(solveZt_blocked)) if blocked \
+            else (MATFN(), MATFN())
+        cbs = cbs + mfs
+        cb = Callbacks(cbs[0], VECFN(), cbs[1], VECFN(), mfs[0], mfs[1], None)
         mv = mass if isinstance(mass, Vec) else Vec(mass)
         h = vp()
         _petsc_check(L.lgh_prior_create_callbacks(mv.h, C.byref(cb), C.byref(h)),

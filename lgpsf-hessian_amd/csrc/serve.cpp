@@ -27,6 +27,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <string>
 #include <thread>
 
 using namespace lgh;
@@ -45,6 +46,7 @@ struct Req {
   int ldo = 0;
   clk::time_point t_submit, t_done;
   int done = 0, rc = 0;
+  std::string err;                // the failure's message (device thread's)
   double logdet = 0.0;
 };
 
@@ -55,6 +57,7 @@ struct Slot {
   double *pin = nullptr;          // n x max_cols, inputs then results
   std::vector<Req *> batch;
   int cols = 0, rc = 0;
+  std::string err;                // message of a failed batch
 };
 
 struct lgh_serve {
@@ -215,6 +218,12 @@ device_loop (lgh_serve *s)
     }
     sl.rc = glr_run_block (s->g, kind == 0 ? 0 : 3, sl.cols, c.data (),
                            sl.pin, n, sl.pin, n, s->tier);
+    // the messages are this (device) thread's: keep them with the batch
+    if (sl.rc) {
+      const char *m = lgh_petsc_last_error ();
+      sl.err = (m && *m) ? m : lgh::last_error ();
+    }
+    else sl.err.clear ();
     const double t = since (t0);
     std::lock_guard<std::mutex> lk (s->mu);
     s->st.busy_s += t;
@@ -263,6 +272,7 @@ finisher_loop (lgh_serve *s)
     std::lock_guard<std::mutex> lk (s->mu);
     for (Req *r : sl.batch) {
       r->rc = sl.rc;
+      if (sl.rc) r->err = sl.err;
       r->done = 1;
       r->t_done = td;
       s->queued_cols -= r->m;
@@ -407,9 +417,10 @@ lgh_serve_wait (lgh_serve_t *s, int64_t ticket, double *logdet,
   if (latency_s)
     *latency_s = std::chrono::duration<double> (r->t_done - r->t_submit).count ();
   const int rc = r->rc;
+  const std::string err = r->err;
   delete r;
   if (rc) return lgh_petsc_error (PETSC_ERR_LIB, "lgh_serve_wait: batch failed: %s",
-                                  lgh::last_error ());
+                                  err.c_str ());
   return 0;
 }
 

@@ -76,6 +76,53 @@ def test_glr_gates(dev, oracle, spd):
     assert np.max(np.abs(g2.lam - lam) / np.abs(lam)) < 1e-10
 
 
+def test_glr_blocked_callbacks(dev, oracle):
+    """The callback prior's blocked MATDENSE Z-solves (prior.h:53-54,
+    glr_common_impl.h:206-234): a build through solveZ_blocked /
+    solveZt_blocked takes no Vec solve, matches the oracle and equals the
+    build through the Vec callbacks bit for bit (the same elementwise
+    arithmetic)."""
+    Bd = W.tg_B_exact(spd=True)
+    B = dev.Mat.dense(Bd)
+    opts = dev.glr_opts(ell=40, q_power=1, trunc_abs=0.0, trunc_rel=1e-8)
+    calls = {}
+    pb = dev.Prior.diagonal_callbacks(W.tg_z(), W.tg_m(), blocked=True, calls=calls)
+    gb = dev.GLR(B, pb, opts)
+    assert calls.get("solveZ_blocked", 0) > 0 and calls.get("solveZt_blocked", 0) > 0
+    assert calls.get("solveZ", 0) == 0
+    gv = dev.GLR(B, dev.Prior.diagonal_callbacks(W.tg_z(), W.tg_m()), opts)
+    assert np.array_equal(gb.lam, gv.lam)
+    assert np.array_equal(gb.U, gv.U)
+    og = oracle.GLR(oracle.CSR.from_dense(Bd), W.tg_m(), oracle.ZSolver.diag(W.tg_z()),
+                    ell=40, q_power=1, trunc_rel=1e-8)
+    assert np.max(np.abs(gb.lam - og.lam) / np.abs(og.lam)) < 1e-12
+    v = W.tg_probe()
+    assert rel(gb.solve(0.7, v), og.solve(0.7, v)) < 1e-12
+
+
+def test_glr_from_fit_rows_bitwise(dev):
+    """B ingested from the fit stage's row export (lgh_mat_from_fit_rows:
+    each row's entries shuffled, global column ids) builds the same GLR
+    bit for bit as B assembled as AIJ (fit.h:160-165, glr.h:274-282)."""
+    Bs, Zs, M = W.psf2d_problem(24)
+    rng = np.random.default_rng(3)
+    Bc = sp.csr_matrix(Bs)
+    rp, cg, v = [0], [], []
+    for i in range(Bc.shape[0]):
+        cols = Bc.indices[Bc.indptr[i]:Bc.indptr[i + 1]].astype(np.int64)
+        vals = Bc.data[Bc.indptr[i]:Bc.indptr[i + 1]]
+        perm = rng.permutation(len(cols))
+        cg.extend(cols[perm]); v.extend(vals[perm]); rp.append(len(cg))
+    Bf = dev.Mat.from_fit_rows(np.array(rp), np.array(cg, dtype=np.int64),
+                               np.array(v), sym_tol=1e-14)
+    prior = dev.Prior.from_mat(Zs, M, blocked_mode=3)
+    for mixed in (False, True):
+        x = dev.xopts(mixed=mixed)
+        g1 = dev.GLR(Bf, prior, dev.glr_opts(ell=30, seed=0), x=x)
+        g2 = dev.GLR(dev.Mat.aij(Bs), prior, dev.glr_opts(ell=30, seed=0), x=x)
+        assert np.array_equal(g1.lam, g2.lam) and np.array_equal(g1.U, g2.U)
+
+
 @pytest.mark.parametrize("spd", [True, False])
 def test_glr_implicit_u_backend(dev, oracle, spd):
     """LGH_GLR_SCALAPACK at communicator size 1 (glr_scalapack_impl.h:505-688

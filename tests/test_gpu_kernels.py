@@ -224,11 +224,14 @@ def test_vcycle_narrow_panel_bitwise(dev, prec, dim):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,m", [(1_000_003, 13), (1 << 20, 16)])
+@pytest.mark.parametrize("n,m", [(1_000_003, 13), (1 << 20, 16), (1 << 20, 25)])
 def test_large_host_blocks_round_trip(dev, n, m):
     """Host blocks of 64 MiB and more move through two pinned chunks per
     thread (104 MB: one whole chunk and a partial one; 128 MiB: exactly
-    two): Y = 2 I X through the SpMM entry point comes back exact."""
+    two; 200 MiB: three whole chunks and a partial fourth, so each chunk is
+    reused, the host-to-device side waiting for its earlier copy and the
+    device-to-host side alternating DMA and copy-out): Y = 2 I X through the
+    SpMM entry point comes back exact."""
     A = sp.identity(n, format="csr") * 2.0
     X = np.random.default_rng(n).standard_normal((n, m))
     assert X.nbytes >= 64 << 20
