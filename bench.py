@@ -86,7 +86,7 @@ def serve_bench(lgh, g, n, nreq, pool, seed=5, rate=None):
     """Request loop on the resident mixed GLR: 60% solve / 35% sample / 5%
     logdet, c log-uniform in [1e-4, 10], RHS count geometric (mean 8)
     truncated to [1, 64], entries from a pool of hashed-Gaussian columns.
-    First a burst of nreq/3 requests measures saturation, then nreq
+    First a burst of min(nreq/3, 1000) requests measures saturation, then nreq
     requests arrive as a Poisson process at 0.7 x the saturated rate (or
     at `rate` requests/s when given)."""
     import threading
@@ -148,7 +148,7 @@ def serve_bench(lgh, g, n, nreq, pool, seed=5, rate=None):
 
     warm = make(8)
     run(warm, [0.0] * len(warm))
-    burst = make(max(nreq // 3, 16))
+    burst = make(max(min(nreq // 3, 1000), 16))
     t_b, lat_b, cols_b = run(burst, [0.0] * len(burst))
     sat_rps = len(burst) / t_b
     rate = 0.7 * sat_rps if rate is None else rate
@@ -243,6 +243,14 @@ def grid3d_bench(args, W, lgh):
         log("extend +%d -> ell %d: %.3fs kept=%d" % (ext, ell0 + ext * (e + 1),
                                                      t_ext[-1], rep["kept"]))
     peak_gb = lgh.lib().lgh_hip_bytes_in_use() / 1e9
+    # the achievable HBM rate on this box (a 1 GiB device copy), beside the
+    # nominal 8 TB/s
+    copy_peak = lgh.copy_gbps(1 << 30, 5)
+    for r in (roof, spmm_roof):
+        if r:
+            r["copy_peak_gbs"] = copy_peak
+            r["frac_of_copy_peak"] = r["achieved"] / copy_peak
+    log("device copy: %.0f GB/s" % copy_peak)
     lam_ext = np.array(g.lam)
     kept_ext = g.kept
     del g
@@ -393,7 +401,8 @@ def main():
     ap.add_argument("--side", type=int, default=1024)
     ap.add_argument("--ell", type=int, default=400)
     ap.add_argument("--mode", type=int, default=3)
-    ap.add_argument("--cpu-cols", type=int, default=1)
+    ap.add_argument("--cpu-cols", type=int, default=64,
+                    help="F-sweep columns the CPU baseline measures (one panel)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--vcycle", choices=("bf16", "f32"), default="bf16",
@@ -408,7 +417,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=None,
                     help="Poisson arrival rate of the serving run in requests/s "
                          "(default 0.7 x the measured saturation)")
-    ap.add_argument("--serve-requests", type=int, default=300,
+    ap.add_argument("--serve-requests", type=int, default=10000,
                     help="requests of the Poisson serving run (0: skip)")
     ap.add_argument("--workload", choices=("grid2d", "grid3d", "batch"),
                     default="grid2d",
@@ -553,6 +562,14 @@ def main():
             mf[k] = {"achieved_tflops": fl_ / ms_ / 1e9, "peak_fp64_tflops": FP64_PEAK_TFS,
                      "frac": fl_ / ms_ / 1e9 / FP64_PEAK_TFS}
     peak_gb = lgh.lib().lgh_hip_bytes_in_use() / 1e9
+    # the achievable HBM rate on this box (a 1 GiB device copy), beside the
+    # nominal 8 TB/s
+    copy_peak = lgh.copy_gbps(1 << 30, 5)
+    for r in (roof, spmm_roof):
+        if r:
+            r["copy_peak_gbs"] = copy_peak
+            r["frac_of_copy_peak"] = r["achieved"] / copy_peak
+    log("device copy: %.0f GB/s" % copy_peak)
 
     extra = {}
     if not args.no_plain:
@@ -563,7 +580,9 @@ def main():
         k = min(g.kept, gp.kept)
         spec_err = float(np.max(np.abs(g.lam[:k] - gp.lam[:k]) / np.abs(gp.lam[:k])))
         extra.update(plain_build_s=t_plain, kept_plain=gp.kept,
-                     mixed_vs_fp64_spectrum_rel=spec_err)
+                     mixed_vs_fp64_spectrum_rel=spec_err,
+                     resid_max_plain=gp.report["resid_max"],
+                     resid_max_mixed=reps[-1]["resid_max"])
         log("plain build %.3fs kept=%d; mixed-vs-fp64 spectrum rel err %.2e"
             % (t_plain, gp.kept, spec_err))
 
@@ -631,6 +650,12 @@ def main():
                            lmax=info["lmax"], nit=info["nit"], ksp_rtol=1e-12)
         msq = np.sqrt(M)
         Bc = O.CSR.from_scipy(Bs)
+        # one whole 64-column panel of an F sweep, its columns spread over
+        # the host threads (OMP_NUM_THREADS, else every core): the oracle's
+        # column-parallel sweeps give the single-thread bits
+        threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
+        threads = max(1, min(threads, os.cpu_count() or 1))
+        O.set_threads(threads)
         Xs = O.randn_fill(0, 0, 0, n, args.cpu_cols)
         t0 = time.perf_counter()
         Wk = Xs * msq[:, None]
@@ -638,13 +663,17 @@ def main():
         Wk = O.csr_mm(Bc, T)
         T = zs.solve_block(Wk)
         t_cols = time.perf_counter() - t0
+        O.set_threads(1)
         cols_build = 3 * args.ell + kept_build
         cpu = {"value": t_cols / args.cpu_cols * cols_build, "unit": "s/build",
-               "cores": 1, "kind": "port",
-               "sample": "%d full F-sweep column(s) of the FP64 oracle (same "
-                         "Chebyshev-MG operator) timed %.1fs, scaled to the "
-                         "%d F columns of one build (dense stages excluded)"
-                         % (args.cpu_cols, t_cols, cols_build)}
+               "cores": threads, "kind": "port",
+               "sample": "one full %d-column F-sweep panel (two Chebyshev-MG Z-solves "
+                         "and the B product) of the FP64 oracle on %d threads, "
+                         "measured %.1fs; the build's %d F columns (%.1f panels) "
+                         "extrapolated from it; dense stages excluded"
+                         % (args.cpu_cols, threads, t_cols, cols_build,
+                            cols_build / args.cpu_cols),
+               "measured_s": t_cols, "host_nproc": os.cpu_count()}
         log("cpu baseline:", cpu)
 
     line = {

@@ -46,6 +46,12 @@ void lgh_hip_prof_enable (int on);
 int  lgh_hip_prof_collect (int kind, int np, int prec, int64_t *count,
                            double *ms, double *bytes, double *flops);
 
+/* achieved HBM bandwidth of a device-to-device copy of `bytes` (a 16-byte
+ * vector copy kernel; read plus written bytes per second, best of reps
+ * after one warm-up), the measured reference beside the nominal HBM peak.
+ * (No reference counterpart: measurement.) */
+int lgh_hip_copy_gbps (int64_t bytes, int reps, double *gbps);
+
 /* the device hashed-Gaussian arithmetic evaluated on the host (no GPU):
  * the draw with the correctly rounded log(q) */
 double lgh_hip_randn_at_host (uint64_t seed, int64_t i, int64_t j);

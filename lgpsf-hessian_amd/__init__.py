@@ -220,6 +220,7 @@ _SIGS = {
     "lgh_hip_trim": (None, []),
     "lgh_hip_prof_enable": (None, [C.c_int]),
     "lgh_hip_prof_collect": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64), dp, dp, dp]),
+    "lgh_hip_copy_gbps": (C.c_int, [C.c_int64, C.c_int, dp]),
     "lgh_hip_randn_at_host": (C.c_double, [C.c_uint64, C.c_int64, C.c_int64]),
     "lgh_hip_randn_host_block": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.c_int,
                                            C.c_int, C.c_int, dp, C.c_int,
@@ -758,6 +759,13 @@ def prof_collect(kind, np_=-1, prec=-1):
 
 def randn_at_host(seed, i, j):
     return lib().lgh_hip_randn_at_host(seed, i, j)
+
+
+def copy_gbps(nbytes=1 << 30, reps=3):
+    """achieved device-to-device copy bandwidth (GB/s, read + write)"""
+    g = C.c_double()
+    _hip_check(lib().lgh_hip_copy_gbps(nbytes, reps, C.byref(g)), "lgh_hip_copy_gbps")
+    return g.value
 
 
 def randn_host_block(seed, row0, col0, nr, nc, prec=PREC_F64):

@@ -67,6 +67,13 @@ void lgh_hip_trim (void) { dtrim (); }
 void lgh_hip_prof_enable (int on) { prof_enable (on); }
 
 int
+lgh_hip_copy_gbps (int64_t bytes, int reps, double *gbps)
+{
+  LGH_TRY (dev_init ());
+  return copy_gbps ((size_t) bytes, reps, gbps);
+}
+
+int
 lgh_hip_prof_collect (int kind, int np, int prec, int64_t *count, double *ms,
                       double *bytes, double *flops)
 {

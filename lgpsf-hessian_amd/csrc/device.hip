@@ -262,16 +262,39 @@ round_up (size_t b)
   return (b + g - 1) / g * g;
 }
 
+// Cached (freed, not yet reused) bytes are bounded: a request takes a
+// cached block of at most 1/8 more than it asks for, and a request that
+// the cache cannot serve first releases every cached block when they
+// exceed kCacheFrac of the device's memory.  Without the bound, work of
+// ever-changing shapes (serving batches of 1 .. 256 columns) piled up
+// exact-size blocks until the runtime's own allocations failed.
+static size_t g_cached = 0;
+static constexpr double kCacheFrac = 0.35;
+
 static bool
-take (hipStream_t key, size_t b, void **p)
+take (hipStream_t key, size_t b, void **p, size_t *got)
 {
   auto l = g_free.find (key);
   if (l == g_free.end ()) return false;
-  auto it = l->second.find (b);
-  if (it == l->second.end ()) return false;
+  auto it = l->second.lower_bound (b);
+  if (it == l->second.end () || it->first > b + b / 8) return false;
   *p = it->second;
+  *got = it->first;
+  g_cached -= it->first;
   l->second.erase (it);
   return true;
+}
+
+// release every cached block (g_mu held); the device is idle first, so no
+// stream still uses one
+static void
+flush_cache_locked ()
+{
+  (void) hipDeviceSynchronize ();
+  for (auto &l : g_free)
+    for (auto &kv : l.second) (void) hipFree (kv.second);
+  g_free.clear ();
+  g_cached = 0;
 }
 
 static void
@@ -290,17 +313,16 @@ dmalloc (size_t bytes)
 {
   if (dev_init ()) return nullptr;
   const hipStream_t s = t_ctx.s;
+  const size_t cap = (size_t) (kCacheFrac * (double) dmem_total ());
   std::lock_guard<std::mutex> lk (g_mu);
-  size_t b = round_up (bytes ? bytes : 1);
+  size_t b = round_up (bytes ? bytes : 1), got = 0;
   void *p = nullptr;
-  if (!take (s, b, &p) && !take (nullptr, b, &p)) {
+  if (!take (s, b, &p, &got) && !take (nullptr, b, &p, &got)) {
+    if (cap && g_cached + b > cap) flush_cache_locked ();
     hipError_t e = hipMalloc (&p, b);
     if (e != hipSuccess) {
-      // release every cached block and retry once (hipFree waits for the
-      // device, so blocks other streams still use are not reused early)
-      for (auto &l : g_free)
-        for (auto &kv : l.second) (void) hipFree (kv.second);
-      g_free.clear ();
+      // release every cached block and retry once
+      flush_cache_locked ();
       e = hipMalloc (&p, b);
       if (e != hipSuccess) {
         char buf[256];
@@ -310,9 +332,10 @@ dmalloc (size_t bytes)
         return nullptr;
       }
     }
+    got = b;
   }
-  g_live[p] = b;
-  g_in_use += b;
+  g_live[p] = got;
+  g_in_use += got;
   return p;
 }
 
@@ -325,6 +348,7 @@ dfree (void *p)
   auto it = g_live.find (p);
   if (it == g_live.end ()) return;
   g_in_use -= it->second;
+  g_cached += it->second;
   g_free[s].insert ({ it->second, p });
   g_live.erase (it);
 }
@@ -333,10 +357,7 @@ void
 dtrim ()
 {
   std::lock_guard<std::mutex> lk (g_mu);
-  (void) hipDeviceSynchronize ();
-  for (auto &l : g_free)
-    for (auto &kv : l.second) (void) hipFree (kv.second);
-  g_free.clear ();
+  flush_cache_locked ();
 }
 
 size_t dbytes_in_use () { return g_in_use; }

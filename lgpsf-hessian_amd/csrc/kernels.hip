@@ -274,6 +274,50 @@ k_randn (PMat &out, uint64_t seed, int64_t row0, int64_t col0)
   }
   return 0;
 }
+// ---- copy bandwidth (measurement) ---------------------------------------------
+// y = x over n 16-byte words, grid-stride; the achievable-HBM reference the
+// roofline fractions are also quoted against
+__global__ void __launch_bounds__ (TPB)
+copy16_k (const uint4 *__restrict__ x, uint4 *__restrict__ y, size_t n)
+{
+  const size_t stride = (size_t) gridDim.x * TPB;
+  for (size_t i = (size_t) blockIdx.x * TPB + threadIdx.x; i < n; i += stride)
+    y[i] = x[i];
+}
+
+int
+copy_gbps (size_t bytes, int reps, double *gbps)
+{
+  bytes = bytes / 16 * 16;
+  LGH_CHECK (bytes > 0 && reps > 0, 63, "copy_gbps: bad arguments");
+  void *a = dmalloc (bytes), *b = dmalloc (bytes);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = (a && b) ? 0 : 5;
+  if (!rc) rc = dzero (a, bytes);
+  if (!rc && (hipEventCreate (&e0) != hipSuccess || hipEventCreate (&e1) != hipSuccess)) {
+    set_error ("copy_gbps: event creation failed");
+    rc = 2;
+  }
+  float best = 0.f;
+  for (int r = 0; r <= reps && !rc; r++) {     // the first launch is a warm-up
+    float t = 0.f;
+    (void) hipEventRecord (e0, stream ());
+    hipLaunchKernelGGL (copy16_k, dim3 (256 * 64), dim3 (TPB), 0, stream (),
+                        (const uint4 *) a, (uint4 *) b, bytes / 16);
+    if (hipGetLastError () != hipSuccess) { set_error ("copy16_k launch"); rc = 76; break; }
+    (void) hipEventRecord (e1, stream ());
+    if (hipEventSynchronize (e1) != hipSuccess
+        || hipEventElapsedTime (&t, e0, e1) != hipSuccess) { set_error ("copy timing"); rc = 2; }
+    if (!rc && r > 0 && (best == 0.f || t < best)) best = t;
+  }
+  if (e0) (void) hipEventDestroy (e0);
+  if (e1) (void) hipEventDestroy (e1);
+  dfree (a);
+  dfree (b);
+  if (!rc) *gbps = 2.0 * (double) bytes / (best * 1e-3) / 1e9;
+  return rc;
+}
+
 // ---- the fill without allocation --------------------------------------------
 // lgh_hip_randn_fill_noalloc: the same draws into caller memory, for callers
 // that cannot allocate device memory.  The flag list of the draws finished

@@ -87,6 +87,9 @@ int d2h (void *dst, const void *src, size_t bytes);
 void *host_pinned_alloc (size_t bytes);   // nullptr on failure (error set)
 void  host_pinned_free (void *p);
 int dzero (void *dst, size_t bytes);   // fill kernel (graph-safe)
+// achieved HBM rate of a device copy of `bytes` (read + write counted),
+// best of reps launches after a warm-up
+int copy_gbps (size_t bytes, int reps, double *gbps);
 int dcopy (void *dst, const void *src, size_t bytes);
 
 // ---- live kernel timing (HIP events on the engine stream) ---------------

@@ -55,6 +55,8 @@ def _declare(L):
     L.orc_randn_at.argtypes = [C.c_uint64, C.c_int64, C.c_int64]
     L.orc_randn_fill.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_int64,
                                  C.c_int64, dp, C.c_int64]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_get_threads.restype = C.c_int
     L.orc_csr_mm.argtypes = [C.c_int, ip, ip, dp, C.c_int, dp, C.c_int, dp,
                              C.c_int]
     L.orc_mg_create.restype = C.c_void_p
@@ -161,6 +163,11 @@ class CSR:
 
     def args(self):
         return (self.n, _i(self.rp), _i(self.ci), _d(self.v))
+
+
+def set_threads(nt):
+    """host threads for the column-parallel sweeps (results unchanged)"""
+    lib().orc_set_threads(int(nt))
 
 
 def randn_at(seed, i, j):

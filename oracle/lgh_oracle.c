@@ -18,6 +18,24 @@
 #define MAX(a, b) ((a) > (b) ? (a) : (b))
 #define MIN(a, b) ((a) < (b) ? (a) : (b))
 
+/* Host threads for the column-parallel sweeps (orc_set_threads; default 1).
+ * Every column of a block solve and of a sparse product is computed by the
+ * same operations in the same order whichever thread runs it, so the
+ * thread count never changes a result. */
+static int g_threads = 1;
+
+void
+orc_set_threads (int nt)
+{
+  g_threads = nt > 1 ? nt : 1;
+}
+
+int
+orc_get_threads (void)
+{
+  return g_threads;
+}
+
 static double
 now_s (void)
 {
@@ -80,6 +98,7 @@ void
 orc_csr_mm (int n, const int *rp, const int *ci, const double *v, int ncols,
             const double *X, int ldx, double *Y, int ldy)
 {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(g_threads) if (g_threads > 1 && ncols > 1)
   for (int c = 0; c < ncols; c++) {
     const double *x = X + (size_t) c * ldx;
     double       *y = Y + (size_t) c * ldy;
@@ -857,6 +876,20 @@ orc_zs_solve_block (orc_zs_t *zs, int ncols, const double *X, int ldx,
   if (zs->kind == 0 || zs->mode < 2) {
     for (int c = 0; c < ncols; c++)
       orc_zs_solve_vec (zs, X + (size_t) c * ldx, Y + (size_t) c * ldy);
+    return;
+  }
+  if (g_threads > 1 && ncols > 1) {
+    /* one column per task: the same per-column arithmetic as the block */
+#pragma omp parallel for schedule(dynamic, 1) num_threads(g_threads)
+    for (int c = 0; c < ncols; c++) {
+      if (zs->mode == 4)
+        orc_cheb_jacobi (zs->n, zs->rp, zs->ci, zs->v, zs->dinv, zs->lmin,
+                         zs->lmax, zs->nit, 1, X + (size_t) c * ldx, ldx,
+                         Y + (size_t) c * ldy, ldy);
+      else
+        orc_cheb_mg (zs->h, zs->lmin, zs->lmax, zs->nit, 1, X + (size_t) c * ldx,
+                     ldx, Y + (size_t) c * ldy, ldy);
+    }
     return;
   }
   if (zs->mode == 4)

@@ -40,6 +40,11 @@ void   orc_randn_fill (uint64_t seed, int64_t row0, int64_t col0,
 /* ---- CSR --------------------------------------------------------------- */
 /* Y = A X on column-major blocks (PETSc MatMatMult / per-column MatMult
  * row order: each row sums its nonzeros in stored order).              */
+/* host threads for the column-parallel sweeps (block Z-solves and sparse
+ * products; results do not depend on the count); default 1 */
+void orc_set_threads (int nt);
+int  orc_get_threads (void);
+
 void orc_csr_mm (int n, const int *rp, const int *ci, const double *v,
                  int ncols, const double *X, int ldx, double *Y, int ldy);
 

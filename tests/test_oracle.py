@@ -167,3 +167,27 @@ def test_prior_mat_gates(oracle):
     assert worst < 1e-7
     s1 = zs0.apply(b)[:, 0]
     assert rel(zs0.solve_vec(s1), b) < 1e-8
+
+
+def test_thread_count_does_not_change_results(oracle, lgh):
+    """The column-parallel sweeps (block Chebyshev-MG and Chebyshev-Jacobi
+    Z-solves, sparse products) give the single-thread bits on 4 threads."""
+    Bs, Zs, M = W.psf2d_problem(20)
+    n = Zs.shape[0]
+    h = lgh.AMG(Zs)
+    levels = [{"A": oracle.CSR.from_scipy(l["A"]),
+               "P": oracle.CSR.from_scipy(l["P"]) if l["P"] is not None else None,
+               "slo": l["slo"], "shi": l["shi"], "nu": l["nu"]} for l in h.levels()]
+    mg = oracle.MG(levels)
+    X = oracle.randn_fill(9, 0, 0, n, 7)
+    Zc = oracle.CSR.from_scipy(Zs)
+    solvers = [oracle.ZSolver.mat(Zc, mode=3, mg=mg, lmin=0.5, lmax=1.1, nit=9),
+               oracle.ZSolver.mat(Zc, mode=4, lmin=0.02, lmax=2.0, nit=30)]
+    try:
+        one = [zs.solve_block(X) for zs in solvers] + [oracle.csr_mm(oracle.CSR.from_scipy(Bs), X)]
+        oracle.set_threads(4)
+        four = [zs.solve_block(X) for zs in solvers] + [oracle.csr_mm(oracle.CSR.from_scipy(Bs), X)]
+    finally:
+        oracle.set_threads(1)
+    for a, b in zip(one, four):
+        assert np.array_equal(a, b)
