@@ -246,7 +246,7 @@ def grid3d_bench(args, W, lgh):
     # the achievable HBM rate on this box (a 1 GiB device copy), beside the
     # nominal 8 TB/s
     copy_peak = lgh.copy_gbps(1 << 30, 5)
-    for r in (roof, spmm_roof):
+    for r in (roof,):
         if r:
             r["copy_peak_gbs"] = copy_peak
             r["frac_of_copy_peak"] = r["achieved"] / copy_peak
@@ -276,7 +276,8 @@ def grid3d_bench(args, W, lgh):
                                "Z^-1 = Chebyshev(%d) around AMG V-cycle"
                                % (side, n, Bs.nnz / n, ell0, ext, info["nit"]),
                    "n": n, "ell": ell0, "kept": kept_ext,
-                   "vcycle_fp64_sweeps": args.vcycle},
+                   "vcycle_fp64_sweeps": args.vcycle,
+                   "spmm_autotune": [r for r in lgh.spmm_group_report() if r["n"] == n]},
         "roofline": roof,
         "extend_s": t_ext, "extend_cols": ext, "ell_final": ell0 + 2 * ext,
         "kept_after_extends": kept_ext, "one_shot_build_s": t_one,

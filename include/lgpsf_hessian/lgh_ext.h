@@ -136,6 +136,9 @@ typedef struct lgh_serve_opts
   int    max_batch_cols;    /* columns per device batch (multiple of 64)    */
   double device_budget_gb;  /* workspace budget of one batch; caps the width */
   int    queue_cols;        /* queued columns before submit blocks          */
+  int    device_streams;    /* batches on the device at once, each on its own
+                               device thread, streams and Z-solve buffers
+                               (1 .. 8, default 2; the budget is per batch) */
 }
 lgh_serve_opts_t;
 
@@ -148,6 +151,7 @@ typedef struct lgh_serve_stats
   double  stage_s;          /* host copies between the callers' columns and
                                the pinned staging blocks (staging threads,
                                overlapping the device work)                 */
+  int     max_running;      /* most batches on the device at once so far    */
 }
 lgh_serve_stats_t;
 

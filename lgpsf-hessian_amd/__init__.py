@@ -73,14 +73,14 @@ class GlrXOpts(C.Structure):
 
 class ServeOpts(C.Structure):
     _fields_ = [("max_batch_cols", C.c_int), ("device_budget_gb", C.c_double),
-                ("queue_cols", C.c_int)]
+                ("queue_cols", C.c_int), ("device_streams", C.c_int)]
 
 
 class ServeStats(C.Structure):
     _fields_ = [("batches", C.c_int64), ("columns", C.c_int64),
                 ("requests", C.c_int64), ("max_batch_cols", C.c_int),
                 ("batch_cols", C.c_int), ("busy_s", C.c_double),
-                ("stage_s", C.c_double)]
+                ("stage_s", C.c_double), ("max_running", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

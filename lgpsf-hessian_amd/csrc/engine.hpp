@@ -7,6 +7,7 @@
 #include "zsolve.hpp"
 #include "petsc_impl.hpp"
 
+#include <mutex>
 #include <vector>
 
 struct lgh_prior {
@@ -27,6 +28,7 @@ struct lgh_prior {
   lgh::ChebWork cw[2];
   lgh::ChebMixedWork cwm;
   int n = 0;
+  std::recursive_mutex lazy_mu;         // first-use device set-up
 };
 
 struct lgh_glr {

@@ -3,6 +3,8 @@
 // the Z-solve tiers on the GPU and the AMG hierarchy built on the host.
 #include "engine.hpp"
 
+#include <mutex>
+
 #include <cmath>
 #include <cstring>
 
@@ -82,10 +84,14 @@ lgh_prior_create_ksp (KSP, Vec, const lgh_prior_mat_opts_t *, lgh_prior_t **)
                           "lgh_prior_create_mat or lgh_prior_create_callbacks");
 }
 
-// device mirror of the Mat-path machinery at one precision
+// device mirror of the Mat-path machinery at one precision.  The device
+// copies (vectors, Z, hierarchies) are made on first use; threads that
+// solve with one prior at once (the serving loop's device threads, each
+// with its own SolveWork) make them once, under the prior's lock.
 int
 lgh::prior_setup_device (lgh_prior_t *p, int prec)
 {
+  std::lock_guard<std::recursive_mutex> lk (p->lazy_mu);
   LGH_TRY (dev_init ());
   LGH_TRY (upload_vecs (p));
   if (!p->is_mat) return 0;
@@ -312,34 +318,42 @@ lgh::prior_solve_block (lgh_prior_t *p, int trans, const PMat &X, PMat &Y,
                         int pcp)
 {
   if (!p->is_mat) return cb_block (p, trans ? 1 : 0, X, Y);
-  LGH_TRY (prior_setup_device (p, X.prec));
-  DevCSR *dz = nullptr;
-  LGH_TRY (mat_device_csr (p->Z, X.prec, 1, &dz));
   const int mode = p->o.blocked_mode;
   if (mode < 2) return prior_solve_vec (p, trans, X, Y);
   // a V-cycle in a lower precision than the vectors (W = 64): FP32 or BF16
   // around FP64 vectors, BF16 around FP32 vectors
   const bool lower = (X.prec == F64 && (pcp == F32 || pcp == BF16))
                      || (X.prec == F32 && pcp == BF16);
-  if (mode <= 3 && p->amg_ok && lower && X.W == 64) {
-    if (!p->mg[pcp]) LGH_TRY (mg_upload (p->amg, pcp, &p->mg[pcp]));
+  const bool mixed = mode <= 3 && p->amg_ok && lower && X.W == 64;
+  DevCSR *dz = nullptr;
+  {
+    std::lock_guard<std::recursive_mutex> lk (p->lazy_mu);
+    LGH_TRY (prior_setup_device (p, X.prec));
+    LGH_TRY (mat_device_csr (p->Z, X.prec, 1, &dz));
+    if (mixed && !p->mg[pcp]) LGH_TRY (mg_upload (p->amg, pcp, &p->mg[pcp]));
+  }
+  SolveWork *sw = solve_work ();
+  ChebWork &cw = sw ? sw->cw[X.prec] : p->cw[X.prec];
+  if (mixed) {
     if (X.prec == F64)
       return cheb_solve_mixed (*dz, *p->mg[pcp], p->lmin, p->lmax, p->nit, X,
-                               Y, p->cwm);
-    return cheb_solve (*dz, p->mg[pcp], p->lmin, p->lmax, p->nit, X, Y,
-                       p->cw[X.prec]);
+                               Y, sw ? sw->cwm : p->cwm);
+    return cheb_solve (*dz, p->mg[pcp], p->lmin, p->lmax, p->nit, X, Y, cw);
   }
   return cheb_solve (*dz, mode <= 3 ? p->mg[X.prec] : nullptr, p->lmin,
-                     p->lmax, p->nit, X, Y, p->cw[X.prec]);
+                     p->lmax, p->nit, X, Y, cw);
 }
 
 int
 lgh::prior_solve_vec (lgh_prior_t *p, int trans, const PMat &X, PMat &Y)
 {
   if (!p->is_mat) return cb_block (p, trans ? 1 : 0, X, Y);
-  LGH_TRY (prior_setup_device (p, X.prec));
   DevCSR *dz = nullptr;
-  LGH_TRY (mat_device_csr (p->Z, X.prec, 1, &dz));
+  {
+    std::lock_guard<std::recursive_mutex> lk (p->lazy_mu);
+    LGH_TRY (prior_setup_device (p, X.prec));
+    LGH_TRY (mat_device_csr (p->Z, X.prec, 1, &dz));
+  }
   PcgInfo info;
   if (X.prec == F32) {
     // the PCG path keeps its FP64 accuracy: solve in FP64, convert

@@ -3,14 +3,17 @@
 // glr_common_impl.h:1007-1104).
 //
 // Callers submit solve / sample / logdet requests (a shift c and m
-// right-hand sides each) from any thread.  Three threads serve them as a
-// pipeline over two pinned staging slots: the stager forms a batch from
+// right-hand sides each) from any thread.  Threads serve them as a
+// pipeline over pinned staging slots: the stager forms a batch from
 // everything queued whenever a slot is free (all requests of the oldest
 // request's operation that fit in the batch's columns, each column with its
-// own shift) and copies its columns into the slot; the device thread, which
-// alone owns the GLR and the device, runs the batch in place in the slot;
-// the finisher copies the results out to the callers and frees the slot.
-// So the host copies of one batch overlap the device work of the other.
+// own shift) and copies its columns into the slot; device threads (two by
+// default), each with its own streams and Z-solve work buffers
+// (SolveWork), run batches in place in their slots, so a batch that forms
+// while another runs starts on the device at once and the two overlap
+// there (their coarse multigrid levels leave most of the device idle); the
+// finisher copies the results out to the callers and frees the slot.  The
+// host copies of one batch overlap the device work of the others.
 // Requests that arrive after a batch is formed wait for the next one
 // (continuous batching at batch granularity).  The batch width is capped by
 // a device workspace budget (memory-based admission); submit blocks while
@@ -64,6 +67,7 @@ struct lgh_serve {
   lgh_glr_t *g = nullptr;
   lgh_serve_opts_t o;
   int max_cols = 64;              // columns per device batch (multiple of 64)
+  int ndev = 2;                   // device threads (batches at once)
   std::mutex mu;
   std::condition_variable cv_work, cv_done, cv_space, cv_ready, cv_drain;
   std::deque<Req *> queue;
@@ -71,16 +75,19 @@ struct lgh_serve {
   int64_t next_id = 1;
   int queued_cols = 0;
   bool stop = false;
-  std::thread stager, device, finisher;
-  Slot slot[2];
+  std::thread stager, finisher;
+  std::vector<std::thread> devices;
+  std::vector<Slot> slot;         // 2 x ndev
   // slot ids: free, staged (to run), run (to drain); -1 ends a thread
   std::deque<int> free_slots, ready, drain;
-  // the running batch (start, predicted seconds) and moving averages of
-  // the device seconds per 64-column panel (per operation) and of the
-  // copy-in seconds per column, which time the next batch's formation
-  bool dev_busy = false;
-  clk::time_point dev_t0;
-  double dev_pred = 0.0, ema_panel[2] = { 0.0, 0.0 }, ema_col = 0.0;
+  // the running batches (start, predicted seconds) per device thread and
+  // moving averages of the device seconds per 64-column panel (per
+  // operation) and of the copy-in seconds per column, which time the next
+  // batch's formation
+  std::vector<int> dev_busy;
+  std::vector<clk::time_point> dev_t0;
+  std::vector<double> dev_pred;
+  double ema_panel[2] = { 0.0, 0.0 }, ema_col = 0.0;
   lgh_serve_stats_t st;
   int tier = 0;
 };
@@ -114,10 +121,10 @@ ema (double &e, double x)
 }
 
 // Forms batches into free slots and copies their inputs in.  A batch is
-// formed when the device is idle, when a full batch is queued, or, while
-// the device runs a batch, once the copy-in of what is queued would end
-// about when that batch does: as late as keeps the device fed, so that
-// requests arriving meanwhile still join it.
+// formed when a device thread is idle, when a full batch is queued, or,
+// while every device thread runs a batch, once the copy-in of what is
+// queued would end about when the first of those batches does: as late as
+// keeps the device fed, so that requests arriving meanwhile still join it.
 static void
 stager_loop (lgh_serve *s)
 {
@@ -129,19 +136,29 @@ stager_loop (lgh_serve *s)
       std::unique_lock<std::mutex> lk (s->mu);
       for (;;) {
         if (s->stop && s->queue.empty ()) {   // stop requested, nothing queued
-          s->ready.push_back (-1);
-          s->cv_ready.notify_one ();
+          for (int d = 0; d < s->ndev; d++) s->ready.push_back (-1);
+          s->cv_ready.notify_all ();
           return;
         }
         if (!s->queue.empty () && !s->free_slots.empty () && s->ready.empty ()) {
-          int waiting = 0;
+          int waiting = 0, busy = 0;
           for (const Req *r : s->queue) waiting += r->m;
-          if (!s->dev_busy || waiting >= s->max_cols || s->stop) break;
-          if (s->dev_pred > 0.0) {
-            const double lead = s->ema_col * std::min (waiting, s->max_cols);
-            const clk::time_point at = s->dev_t0
+          for (int d = 0; d < s->ndev; d++) busy += s->dev_busy[d];
+          if (busy < s->ndev || waiting >= s->max_cols || s->stop) break;
+          // every device thread busy: the earliest predicted end
+          bool known = true;
+          clk::time_point first = clk::time_point::max ();
+          for (int d = 0; d < s->ndev; d++) {
+            if (s->dev_pred[d] <= 0.0) { known = false; break; }
+            const clk::time_point e = s->dev_t0[d]
               + std::chrono::duration_cast<clk::duration> (
-                  std::chrono::duration<double> (s->dev_pred - lead));
+                  std::chrono::duration<double> (s->dev_pred[d]));
+            if (e < first) first = e;
+          }
+          if (known) {
+            const double lead = s->ema_col * std::min (waiting, s->max_cols);
+            const clk::time_point at = first
+              - std::chrono::duration_cast<clk::duration> (std::chrono::duration<double> (lead));
             if (clk::now () >= at) break;
             s->cv_work.wait_until (lk, at);
             continue;
@@ -186,54 +203,63 @@ stager_loop (lgh_serve *s)
   }
 }
 
-// runs the staged batches on the device, in place in their slots
+// device thread d: runs staged batches on the device, in place in their
+// slots, with its own streams and Z-solve work buffers
 static void
-device_loop (lgh_serve *s)
+device_loop (lgh_serve *s, int d)
 {
   (void) hipSetDevice (0);
   const int n = s->g->n;
-  for (;;) {
-    int id;
-    {
-      std::unique_lock<std::mutex> lk (s->mu);
-      s->cv_ready.wait (lk, [&] { return !s->ready.empty (); });
-      id = s->ready.front ();
-      s->ready.pop_front ();
-    }
-    if (id < 0) break;
-    Slot &sl = s->slot[id];
-    std::vector<double> c (sl.cols);
-    int j = 0;
-    for (Req *r : sl.batch)
-      for (int q = 0; q < r->m; q++, j++) c[j] = r->c;
-    const int kind = sl.batch[0]->op == LGH_SERVE_SOLVE ? 0 : 1;
-    const int panels = (sl.cols + 63) / 64;
-    const clk::time_point t0 = clk::now ();
-    {
+  {
+    SolveWork work;
+    SolveWorkScope scope (&work);
+    for (;;) {
+      int id;
+      {
+        std::unique_lock<std::mutex> lk (s->mu);
+        s->cv_ready.wait (lk, [&] { return !s->ready.empty (); });
+        id = s->ready.front ();
+        s->ready.pop_front ();
+      }
+      if (id < 0) break;
+      Slot &sl = s->slot[id];
+      std::vector<double> c (sl.cols);
+      int j = 0;
+      for (Req *r : sl.batch)
+        for (int q = 0; q < r->m; q++, j++) c[j] = r->c;
+      const int kind = sl.batch[0]->op == LGH_SERVE_SOLVE ? 0 : 1;
+      const int panels = (sl.cols + 63) / 64;
+      const clk::time_point t0 = clk::now ();
+      {
+        std::lock_guard<std::mutex> lk (s->mu);
+        s->dev_busy[d] = 1;
+        int running = 0;
+        for (int k = 0; k < s->ndev; k++) running += s->dev_busy[k];
+        if (running > s->st.max_running) s->st.max_running = running;
+        s->dev_t0[d] = t0;
+        s->dev_pred[d] = s->ema_panel[kind] * panels;
+        s->cv_work.notify_one ();
+      }
+      sl.rc = glr_run_block (s->g, kind == 0 ? 0 : 3, sl.cols, c.data (),
+                             sl.pin, n, sl.pin, n, s->tier);
+      // the messages are this (device) thread's: keep them with the batch
+      if (sl.rc) {
+        const char *m = lgh_petsc_last_error ();
+        sl.err = (m && *m) ? m : lgh::last_error ();
+      }
+      else sl.err.clear ();
+      const double t = since (t0);
       std::lock_guard<std::mutex> lk (s->mu);
-      s->dev_busy = true;
-      s->dev_t0 = t0;
-      s->dev_pred = s->ema_panel[kind] * panels;
+      s->st.busy_s += t;
+      if (!sl.rc) ema (s->ema_panel[kind], t / panels);
+      s->dev_busy[d] = 0;
+      s->drain.push_back (id);
+      s->cv_drain.notify_one ();
       s->cv_work.notify_one ();
     }
-    sl.rc = glr_run_block (s->g, kind == 0 ? 0 : 3, sl.cols, c.data (),
-                           sl.pin, n, sl.pin, n, s->tier);
-    // the messages are this (device) thread's: keep them with the batch
-    if (sl.rc) {
-      const char *m = lgh_petsc_last_error ();
-      sl.err = (m && *m) ? m : lgh::last_error ();
-    }
-    else sl.err.clear ();
-    const double t = since (t0);
-    std::lock_guard<std::mutex> lk (s->mu);
-    s->st.busy_s += t;
-    if (!sl.rc) ema (s->ema_panel[kind], t / panels);
-    s->dev_busy = false;
-    s->drain.push_back (id);
-    s->cv_drain.notify_one ();
-    s->cv_work.notify_one ();
+    (void) dev_sync ();   // the work buffers are freed with no kernel left on them
   }
-  dev_thread_exit ();     // the device thread's stream and cached blocks
+  dev_thread_exit ();     // the device thread's streams and cached blocks
   std::lock_guard<std::mutex> lk (s->mu);
   s->drain.push_back (-1);
   s->cv_drain.notify_one ();
@@ -244,6 +270,7 @@ static void
 finisher_loop (lgh_serve *s)
 {
   const int n = s->g->n;
+  int ended = 0;                  // device threads that have finished
   for (;;) {
     int id;
     {
@@ -252,7 +279,10 @@ finisher_loop (lgh_serve *s)
       id = s->drain.front ();
       s->drain.pop_front ();
     }
-    if (id < 0) return;
+    if (id < 0) {
+      if (++ended == s->ndev) return;
+      continue;
+    }
     Slot &sl = s->slot[id];
     double t = 0.0;
     if (!sl.rc) {
@@ -297,6 +327,7 @@ lgh_serve_opts_default (void)
   o.max_batch_cols = 256;
   o.device_budget_gb = 32.0;
   o.queue_cols = 1024;
+  o.device_streams = 2;
   return o;
 }
 
@@ -315,10 +346,11 @@ lgh_serve_create (lgh_glr_t *g, const lgh_serve_opts_t *opts, lgh_serve_t **out)
   PetscCheck (g != NULL && out != NULL, 0, PETSC_ERR_ARG_NULL,
               "lgh_serve_create: NULL argument");
   const lgh_serve_opts_t o = opts ? *opts : lgh_serve_opts_default ();
-  PetscCheck (o.max_batch_cols >= 64 && o.device_budget_gb > 0 && o.queue_cols >= 64,
+  PetscCheck (o.max_batch_cols >= 64 && o.device_budget_gb > 0 && o.queue_cols >= 64
+              && o.device_streams >= 1 && o.device_streams <= 8,
               0, PETSC_ERR_ARG_OUTOFRANGE,
               "lgh_serve_create: need max_batch_cols >= 64, a positive device "
-              "budget and queue_cols >= 64");
+              "budget, queue_cols >= 64 and 1 .. 8 device streams");
   int cols = (o.max_batch_cols / 64) * 64;
   const int fit = (int) (o.device_budget_gb * 1e9 / bytes_per_col (g)) / 64 * 64;
   if (fit < cols) cols = fit;
@@ -333,18 +365,25 @@ lgh_serve_create (lgh_glr_t *g, const lgh_serve_opts_t *opts, lgh_serve_t **out)
   // the blocked tier when the GLR's policy enables it at all: then every
   // batch, whatever its width, takes the same path
   s->tier = g->x.solve_blocked_min > 0 ? 1 : 0;
+  s->ndev = o.device_streams;
+  s->slot.resize ((size_t) 2 * s->ndev);
+  s->dev_busy.assign ((size_t) s->ndev, 0);
+  s->dev_t0.assign ((size_t) s->ndev, clk::now ());
+  s->dev_pred.assign ((size_t) s->ndev, 0.0);
   const size_t bytes = sizeof (double) * (size_t) g->n * cols;
-  s->slot[0].pin = (double *) host_pinned_alloc (bytes);
-  s->slot[1].pin = s->slot[0].pin ? (double *) host_pinned_alloc (bytes) : nullptr;
-  if (!s->slot[1].pin) {
-    host_pinned_free (s->slot[0].pin);
-    delete s;
-    return lgh_petsc_error (PETSC_ERR_MEM, "lgh_serve_create: pinned staging "
-                            "(%.1f GB) failed", 2.0 * bytes / 1e9);
+  for (size_t k = 0; k < s->slot.size (); k++) {
+    s->slot[k].pin = (double *) host_pinned_alloc (bytes);
+    if (!s->slot[k].pin) {
+      for (size_t q = 0; q < k; q++) host_pinned_free (s->slot[q].pin);
+      const int ns = (int) s->slot.size ();
+      delete s;
+      return lgh_petsc_error (PETSC_ERR_MEM, "lgh_serve_create: pinned staging "
+                              "(%.1f GB) failed", ns * (double) bytes / 1e9);
+    }
+    s->free_slots.push_back ((int) k);
   }
-  s->free_slots = { 0, 1 };
   s->finisher = std::thread (finisher_loop, s);
-  s->device = std::thread (device_loop, s);
+  for (int d = 0; d < s->ndev; d++) s->devices.emplace_back (device_loop, s, d);
   s->stager = std::thread (stager_loop, s);
   *out = s;
   return 0;
@@ -444,13 +483,14 @@ lgh_serve_destroy (lgh_serve_t *s)
     s->stop = true;
   }
   s->cv_work.notify_all ();
-  // the stager ends once the queue is empty, and its end marker passes
-  // through the device thread to the finisher behind every batch
+  // the stager ends once the queue is empty; its end markers (one per
+  // device thread) pass through the device threads to the finisher behind
+  // every batch
   if (s->stager.joinable ()) s->stager.join ();
-  if (s->device.joinable ()) s->device.join ();
+  for (auto &t : s->devices)
+    if (t.joinable ()) t.join ();
   if (s->finisher.joinable ()) s->finisher.join ();
   for (auto &kv : s->reqs) delete kv.second;
-  host_pinned_free (s->slot[0].pin);
-  host_pinned_free (s->slot[1].pin);
+  for (auto &sl : s->slot) host_pinned_free (sl.pin);
   delete s;
 }

@@ -114,11 +114,36 @@ mg_upload (const AmgHierarchy &h, int prec, DevMG **out)
   return 0;
 }
 
+static thread_local SolveWork *t_solve_work = nullptr;
+
+SolveWork *solve_work () { return t_solve_work; }
+SolveWorkScope::SolveWorkScope (SolveWork *w) : prev (t_solve_work) { t_solve_work = w; }
+SolveWorkScope::~SolveWorkScope () { t_solve_work = prev; }
+
+DevMG::Work *
+SolveWork::mg_slots (const DevMG *g)
+{
+  for (auto &e : mg)
+    if (e.first == g) return e.second;
+  DevMG::Work *w = new DevMG::Work[3];
+  mg.emplace_back (g, w);
+  return w;
+}
+
+SolveWork::~SolveWork ()
+{
+  for (auto &e : mg) {
+    for (int k = 0; k < 3; k++) e.second[k].free ();
+    delete[] e.second;
+  }
+}
+
 static int
 mg_work (DevMG &g, const PMat &like, DevMG::Work **out)
 {
   const int np = like.npanels ();
-  DevMG::Work &w = g.work[like.W == 64 ? 0 : like.W == 16 ? 1 : 2];
+  const int slot = like.W == 64 ? 0 : like.W == 16 ? 1 : 2;
+  DevMG::Work &w = t_solve_work ? t_solve_work->mg_slots (&g)[slot] : g.work[slot];
   *out = &w;
   if (w.W == like.W && w.np == np) return 0;
   w.free ();

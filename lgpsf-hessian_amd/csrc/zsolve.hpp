@@ -77,6 +77,25 @@ int cheb_solve_mixed (const DevCSR &A64, DevMG &mg, double lmin,
                       double lmax, int nit, const PMat &B, PMat &X,
                       ChebMixedWork &w);
 
+// Work buffers of block Z-solves that a thread brings itself, so that
+// several threads can solve with one prior at once (the serving loop's
+// device threads): while a SolveWorkScope is alive on a thread, the
+// prior's block solves and the V-cycles they run take their buffers from
+// it instead of from the prior and the hierarchy.
+struct SolveWork {
+  ChebWork cw[2];
+  ChebMixedWork cwm;
+  std::vector<std::pair<const DevMG *, DevMG::Work *>> mg;   // 3 slots each
+  DevMG::Work *mg_slots (const DevMG *g);
+  ~SolveWork ();
+};
+SolveWork *solve_work ();              // the thread's, or null
+struct SolveWorkScope {
+  SolveWork *prev;
+  explicit SolveWorkScope (SolveWork *w);
+  ~SolveWorkScope ();
+};
+
 struct PcgInfo {
   int its = 0;
   int converged = 0;

@@ -1,8 +1,9 @@
 """Request loop (lgh_serve_*) and the blocked downstream solve tier on an
-MI355X: a request's result is the same alone and batched with others
-(tolerance 1e-12 relative; the columns of a batch are computed
-independently), batches do form under concurrent submission, bad requests
-are refused, and the blocked tier agrees with per-column PCG."""
+MI355X: a request's result is the same alone and batched with others, on
+one device stream or with two batches on the device at once (tolerance
+1e-12 relative; the columns of a batch are computed independently),
+batches do form under concurrent submission, bad requests are refused, and
+the blocked tier agrees with per-column PCG."""
 import threading
 
 import numpy as np
@@ -43,12 +44,12 @@ def _requests(n, seed=0):
     return out
 
 
-@pytest.mark.parametrize("mixed", [True, False])
-def test_serve_alone_equals_mix(dev, prob, mixed):
+@pytest.mark.parametrize("mixed,streams", [(True, 2), (False, 2), (True, 1)])
+def test_serve_alone_equals_mix(dev, prob, mixed, streams):
     g = dev.GLR(prob["B"], prob["prior"], dev.glr_opts(ell=24, seed=0),
                 x=dev.xopts(mixed=mixed))
     reqs = _requests(prob["n"])
-    srv = dev.Serve(g, max_batch_cols=64)
+    srv = dev.Serve(g, max_batch_cols=64, device_streams=streams)
     alone = [srv.wait(srv.submit(op, c, X))[0] for op, c, X in reqs]
     # the same requests from 4 threads at once, interleaved, queued behind
     # four full-width solves so that they arrive while batches run
@@ -77,8 +78,10 @@ def test_serve_alone_equals_mix(dev, prob, mixed):
             assert a == b == pytest.approx(g.logdet(c), rel=1e-14)
         else:
             assert rel(b, a) <= 1e-12
-    # some batch held several requests (wider than the widest request)
+    # some batch held several requests (wider than the widest request), and
+    # with two device streams batches ran on the device at once
     assert st["max_batch_cols"] > 17
+    assert st["max_running"] == streams
     assert st["busy_s"] > 0 and st["stage_s"] >= 0
     assert st["requests"] == 2 * len(reqs) + len(blockers)
     # the loop's results are the library's batched operations (the same
