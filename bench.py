@@ -442,6 +442,10 @@ def main():
                     help="downstream calls on at least this many columns use "
                          "the blocked Z-solve tier (default: the mixed "
                          "policy's)")
+    ap.add_argument("--mg-coarse-rows", type=int, default=-1,
+                    help="multigrid levels of at most this many rows run as one "
+                         "fused launch per coarse visit (0: one launch per "
+                         "operation and level; -1: the library's default)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -462,6 +466,8 @@ def main():
 
     import workloads as W
     lgh = load_pkg()
+    if args.mg_coarse_rows >= 0:
+        lgh.mg_coarse_config(args.mg_coarse_rows, -1)
     if args.workload == "grid3d":
         return grid3d_bench(args, W, lgh)
     if args.workload == "batch":

@@ -105,6 +105,12 @@ int lgh_hip_spmm_grouped (int n, int m, const int *rp, const int *ci,
  * engine gets the grouped copy, whatever its size; rg = 0 returns to the
  * autotuner and the size threshold of 2^24 nonzeros) */
 void lgh_hip_spmm_group_config (int rg, int ppw);
+/* caps of the multigrid levels that run as one fused launch per coarse
+ * visit of a cycle (the small levels above the dense solve, each with at
+ * most max_rows rows and max_nnz nonzeros; negative: the defaults 1024
+ * and 65536; max_rows = 0: one launch per operation and level).  Both
+ * paths give the same bits; this selects the launch shape. */
+void lgh_hip_mg_coarse_config (int max_rows, int max_nnz);
 /* the autotuner's decisions so far as text, one line per matrix shape:
  * "rows nonzeros value_prec rg ppw_fp64 ppw_fp32 ms_per_panel_fp64
  * ms_per_panel_fp32"; returns the full length (buf may be null) */

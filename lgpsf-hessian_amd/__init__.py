@@ -235,6 +235,7 @@ _SIGS = {
     "lgh_hip_randn_fill_noalloc": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.c_int,
                                              C.c_int, C.c_int, C.c_void_p, C.c_int]),
     "lgh_hip_spmm_group_config": (None, [C.c_int, C.c_int]),
+    "lgh_hip_mg_coarse_config": (None, [C.c_int, C.c_int]),
     "lgh_hip_spmm_group_report": (C.c_int, [C.c_char_p, C.c_int]),
     "lgh_hip_cheb_jacobi": (C.c_int, [C.c_int, ip, ip, dp, C.c_int, C.c_int,
                                       C.c_double, C.c_double, C.c_int, C.c_int,
@@ -738,7 +739,8 @@ def init():
 
 PROF_KINDS = {"spmm_B": 0, "spmm": 1, "cheb_jacobi": 2, "cheb_z": 3,
               "resid": 4, "gram": 5, "gemm": 6, "other": 7, "cheb_mixed": 8,
-              "smooth_pre_fine": 9, "smooth_post_fine": 10, "resid_fine": 11}
+              "smooth_pre_fine": 9, "smooth_post_fine": 10, "resid_fine": 11,
+              "mg_coarse": 12}
 
 
 def prof_enable(on=True):
@@ -812,6 +814,13 @@ def spmm_group_config(rg=0, ppw=0):
     """Force the grouped SpMM's rows per group / panels per wave for later
     uploads (rg = 0: the autotuner decides)."""
     lib().lgh_hip_spmm_group_config(rg, ppw)
+
+
+def mg_coarse_config(max_rows=-1, max_nnz=-1):
+    """Caps of the multigrid levels that run as one fused launch per coarse
+    visit (negative: defaults 1024 rows, 65536 nonzeros; max_rows = 0: one
+    launch per operation and level).  Both give the same bits."""
+    lib().lgh_hip_mg_coarse_config(max_rows, max_nnz)
 
 
 def spmm_group_report():

@@ -164,6 +164,12 @@ lgh_hip_spmm_group_config (int rg, int ppw)
   spmm_group_force (rg, ppw);
 }
 
+void
+lgh_hip_mg_coarse_config (int max_rows, int max_nnz)
+{
+  mg_coarse_config (max_rows, max_nnz);
+}
+
 int
 lgh_hip_spmm_group_report (char *buf, int len)
 {

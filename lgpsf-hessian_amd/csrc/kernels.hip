@@ -1618,11 +1618,27 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
   });
 }
 
+// one value of the degree-2 smoother's update: D1 = b1 dr r0, R1 = r0 - acc
+// (acc = sum_j A_ij D1(j)), D2 = a2 D1 + b2 dr R1, out = (x0 +) D1 + D2.
+// Every operation is rounded as written (no fused multiply-add), so the
+// per-level smoother (smx_k) and the fused coarse cycle (mgc_k) round alike.
+template <typename T>
+__device__ __forceinline__ T
+smx_update (T r0, T acc, T x0, T dr, T b1, T a2, T b2, bool xadd)
+{
+#pragma clang fp contract(off)
+  const T d1 = b1 * (dr * r0);
+  const T r1 = r0 - acc;
+  const T bz = b2 * (dr * r1);
+  const T d2 = a2 * d1 + bz;
+  return xadd ? (x0 + d1) + d2 : d1 + d2;
+}
+
 // degree-2 correction-form smoother (zsolve_impl.h:385-404 with nu = 2)
 // in one pass.  X2 = X0 + D1 + D2 needs R0 only at
 // radius 1: R1(i) = R0(i) - sum_j A_ij D1(j) with D1(j) = b1 dinv_j R0(j),
-// then D2(i) = a2 D1(i) + b2 dinv_i R1(i).  Same arithmetic as sm1_k +
-// sm2_k; D1, R1 and R2 are never stored.  SC: v holds the level's scaled
+// then D2(i) = a2 D1(i) + b2 dinv_i R1(i) (smx_update); D1, R1 and R2
+// are never stored.  SC: v holds the level's scaled
 // values A_ij (b1 dinv_j), rounded as the product below rounds them, so
 // the neighbours' dinv is not gathered and the result is the same bits.
 template <typename TV, typename T, typename S, bool XADD, int W = 64,
@@ -1669,14 +1685,8 @@ smx_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
   vld (R0 + qi, r0);
   if (XADD) vld (X + qi, x0);
 #pragma unroll
-  for (int i = 0; i < N; i++) {
-    const T d1 = b1 * (dr * r0[i]);
-    const T r1 = r0[i] - acc[i];
-    const T bz = b2 * (dr * r1);
-    const T ad = a2 * d1;
-    const T d2 = ad + bz;
-    out[i] = XADD ? x0[i] + d1 + d2 : d1 + d2;
-  }
+  for (int i = 0; i < N; i++)
+    out[i] = smx_update (r0[i], acc[i], XADD ? x0[i] : (T) 0, dr, b1, a2, b2, XADD);
   vst (X + qi, out);
 }
 
@@ -2071,6 +2081,290 @@ k_dense_apply (const void *Ainv, int n0, const PMat &B, PMat &X)
   LGH_CHECK (B.nrows == n0 && X.nrows == n0 && B.W == X.W && B.prec == X.prec,
              8, "k_dense_apply: shape mismatch");
   return with_W (B.W, l);
+}
+
+// ---------------------------------------------------------------------------
+// fused coarse cycle (zsolve_impl.h:406-485 on the small levels).  Below a
+// few thousand rows a level's kernels are a few microseconds of work each,
+// and a cycle spends its time between dependent launches.  The columns of a
+// cycle are independent, so one workgroup can run every small level for its
+// own N columns (16 bytes of a panel row) of one panel, ordering the
+// passes by workgroup barriers: the workgroup's stores are visible to its
+// later loads (one CU; the barrier carries a workgroup-scope release and
+// acquire).  One launch per coarse visit replaces six per level.
+
+constexpr int MGC_TPB = 1024;
+
+struct MgcLev {
+  int n, nc;                             // rows; rows of level l - 1
+  int sc;                                // vs holds the scaled values
+  const int *rp, *ci;                    // A
+  const void *v, *vs, *dinv;             // values, the smoother's, 1/diag
+  const int *prp, *pci;                  // P (n x nc)
+  const void *pv;
+  const int *trp, *tci;                  // P^T (nc x n)
+  const void *tv;
+  void *B, *X, *R;                       // the level's panels
+  double b1, a2, b2;
+};
+struct MgcArgs {
+  MgcLev lev[MGC_MAX_LEV];
+  const void *cinv;
+  int n0, nops;
+  unsigned char op[MGC_MAX_OPS];
+};
+
+// sum over one CSR row of w(k, c) X(c) in the row kernels' order: groups of
+// NG entries, the slots past the row's end adding 0 * 0
+template <typename TV, typename T, typename S, int W, typename WF>
+__device__ __forceinline__ void
+mgc_rowsum (int k0, int k1, int pad, const int *ci, const TV *v, const S *Xb,
+            WF wf, T (&acc)[VT<S>::N])
+{
+  constexpr int N = VT<S>::N;
+#pragma unroll
+  for (int i = 0; i < N; i++) acc[i] = 0;
+  for (int kb = k0; kb < k1; kb += NG) {
+    int c[NG];
+    T w[NG], dd[NG];
+#pragma unroll
+    for (int u = 0; u < NG; u++) {
+      const bool in = kb + u < k1;
+      c[u] = in ? ci[kb + u] : pad;
+      w[u] = in ? (T) v[kb + u] : (T) 0;
+    }
+#pragma unroll
+    for (int u = 0; u < NG; u++) dd[u] = wf (w[u], c[u]);
+    typename Raw16<S>::type x[NG];
+#pragma unroll
+    for (int u = 0; u < NG; u++) {
+      if (kb + u < k1) x[u] = rld (Xb + (size_t) c[u] * W);
+      else x[u] = {};
+    }
+#pragma unroll
+    for (int u = 0; u < NG; u++) {
+      T t[N];
+      unpack (x[u], t);
+#pragma unroll
+      for (int i = 0; i < N; i++) acc[i] += dd[u] * t[i];
+    }
+  }
+}
+
+// TV matrix values, T arithmetic, S vector storage, TD the dense inverse
+template <typename TV, typename T, typename S, typename TD, int W>
+__global__ void __launch_bounds__ (MGC_TPB)
+mgc_k (const MgcArgs a)
+{
+  constexpr int N = VT<S>::N;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const size_t col = (size_t) blockIdx.x * N;
+  const size_t pan = blockIdx.y;
+  auto plain = [] (T w, int) { return w; };
+  for (int q = 0; q < a.nops; q++) {
+    const int kind = a.op[q] >> 4, l = a.op[q] & 15;
+    const MgcLev &L = a.lev[l];
+    const int n = L.n;
+    const size_t off = pan * n * W + col;
+    if (kind == MGC_DENSE) {
+      const TD *Ai = (const TD *) a.cinv;
+      const S *B = (const S *) L.B + off;
+      S *X = (S *) L.X + off;
+      for (int row = tid; row < n; row += nt) {
+        TD acc[N], b[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) acc[i] = 0;
+        for (int k = 0; k < n; k++) {
+          const TD ak = Ai[(size_t) row * n + k];
+          vld (B + (size_t) k * W, b);
+#pragma unroll
+          for (int i = 0; i < N; i++) acc[i] += ak * b[i];
+        }
+        vst (X + (size_t) row * W, acc);
+      }
+    }
+    else if (kind == MGC_RESTRICT) {
+      // B(l - 1) = P^T R, rows of level l - 1
+      const MgcLev &C = a.lev[l - 1];
+      const S *R = (const S *) L.R + off;
+      S *Bc = (S *) C.B + pan * L.nc * W + col;
+      for (int row = tid; row < L.nc; row += nt) {
+        const int k0 = L.trp[row], k1 = L.trp[row + 1];
+        T acc[N];
+        mgc_rowsum<TV, T, S, W> (k0, k1, k0 < k1 ? L.tci[k0] : 0, L.tci,
+                                 (const TV *) L.tv, R, plain, acc);
+        vst (Bc + (size_t) row * W, acc);
+      }
+    }
+    else if (kind == MGC_PROLONG) {
+      // X += P X(l - 1)
+      const MgcLev &C = a.lev[l - 1];
+      const S *Xc = (const S *) C.X + pan * L.nc * W + col;
+      S *X = (S *) L.X + off;
+      for (int row = tid; row < n; row += nt) {
+        const int k0 = L.prp[row], k1 = L.prp[row + 1];
+        T acc[N], xo[N];
+        mgc_rowsum<TV, T, S, W> (k0, k1, k0 < k1 ? L.pci[k0] : 0, L.pci,
+                                 (const TV *) L.pv, Xc, plain, acc);
+        vld (X + (size_t) row * W, xo);
+#pragma unroll
+        for (int i = 0; i < N; i++) acc[i] = acc[i] + xo[i];
+        vst (X + (size_t) row * W, acc);
+      }
+    }
+    else if (kind == MGC_RESID) {
+      const S *B = (const S *) L.B + off;
+      const S *X = (const S *) L.X + off;
+      S *R = (S *) L.R + off;
+      for (int row = tid; row < n; row += nt) {
+        T acc[N], b[N];
+        mgc_rowsum<TV, T, S, W> (L.rp[row], L.rp[row + 1], row, L.ci,
+                                 (const TV *) L.v, X, plain, acc);
+        vld (B + (size_t) row * W, b);
+#pragma unroll
+        for (int i = 0; i < N; i++) b[i] -= acc[i];
+        vst (R + (size_t) row * W, b);
+      }
+    }
+    else {
+      // degree-2 smoother (smx_k): MGC_SM0 from B, MGC_SMX adds to X from R
+      const bool xadd = kind == MGC_SMX;
+      const S *R0 = (const S *) (xadd ? L.R : L.B) + off;
+      S *X = (S *) L.X + off;
+      const TV *dinv = (const TV *) L.dinv;
+      const T b1 = (T) L.b1, a2 = (T) L.a2, b2 = (T) L.b2;
+      const bool sc = L.sc;
+      auto scaled = [&] (T w, int c) {
+        return sc ? w : w * (b1 * (T) dinv[c]);
+      };
+      for (int row = tid; row < n; row += nt) {
+        T acc[N], r0[N], x0[N], out[N];
+        mgc_rowsum<TV, T, S, W> (L.rp[row], L.rp[row + 1], row, L.ci,
+                                 (const TV *) L.vs, R0, scaled, acc);
+        const T dr = (T) dinv[row];
+        vld (R0 + (size_t) row * W, r0);
+        if (xadd) vld (X + (size_t) row * W, x0);
+        else vzero (x0);
+#pragma unroll
+        for (int i = 0; i < N; i++)
+          out[i] = smx_update (r0[i], acc[i], x0[i], dr, b1, a2, b2, xadd);
+        vst (X + (size_t) row * W, out);
+      }
+    }
+    __syncthreads ();
+  }
+}
+
+bool
+mgc_supported (int nlev, const MgcLevelIn *lev, int n0)
+{
+  if (nlev < 2 || nlev > MGC_MAX_LEV || !lev[0].B || !lev[0].X) return false;
+  const PMat &v0 = *lev[0].B;
+  if (!row_width (v0.W) || v0.nrows != n0) return false;
+  const int mprec = lev[1].A ? lev[1].A->prec : -1;
+  // (TV, T, S): (double, double, double), (float, float, float),
+  // (float, float, bf16)
+  const bool ok_prec = (v0.prec == F64 && mprec == F64)
+                       || (v0.prec != F64 && mprec == F32);
+  if (!ok_prec) return false;
+  for (int l = 0; l < nlev; l++) {
+    const MgcLevelIn &L = lev[l];
+    if (!L.B || !L.X || (l > 0 && !L.R)) return false;
+    for (const PMat *m : { L.B, L.X, L.R })
+      if (m && (m->W != v0.W || m->prec != v0.prec || m->nrows != L.B->nrows
+                || m->npanels () != v0.npanels ()))
+        return false;
+    if (l == 0) continue;
+    const DevCSR *A = L.A, *P = L.P, *PT = L.PT;
+    if (!A || !P || !PT || !A->dinv || A->prec != mprec || P->prec != mprec
+        || PT->prec != mprec || A->n != A->m || L.B->nrows != A->n
+        || P->n != A->n || PT->m != A->n || P->m != PT->n
+        || P->m != lev[l - 1].B->nrows)
+      return false;
+    // the per-level product's order: spmm64_k (row order), not the long-row
+    // kernel
+    for (const DevCSR *Q : { P, PT })
+      if (Q->n != Q->m && Q->nnz >= 64 * (int64_t) Q->n) return false;
+  }
+  return true;
+}
+
+int
+k_mg_coarse (int nlev, const MgcLevelIn *lev, const void *cinv, int n0,
+             int nops, const unsigned char *ops)
+{
+  LGH_CHECK (nops >= 1 && nops <= MGC_MAX_OPS && mgc_supported (nlev, lev, n0),
+             8, "k_mg_coarse: unsupported levels");
+  MgcArgs a {};
+  a.cinv = cinv;
+  a.n0 = n0;
+  a.nops = nops;
+  double bytes = 0.0, flops = 0.0;
+  const PMat &v0 = *lev[0].B;
+  const double vb = prec_bytes (v0.prec) * (double) v0.W;
+  for (int q = 0; q < nops; q++) {
+    const int kind = ops[q] >> 4, l = ops[q] & 15;
+    LGH_CHECK (l < nlev && kind <= MGC_DENSE && (kind == MGC_DENSE) == (l == 0),
+               8, "k_mg_coarse: bad operation");
+    a.op[q] = ops[q];
+    const MgcLevelIn &L = lev[l];
+    if (kind == MGC_DENSE) {
+      bytes += (double) n0 * n0 * prec_bytes (v0.prec == F64 ? F64 : F32) + 2.0 * n0 * vb;
+      flops += 2.0 * n0 * n0 * v0.W;
+    }
+    else if (kind == MGC_RESTRICT || kind == MGC_PROLONG) {
+      const DevCSR &Q = kind == MGC_RESTRICT ? *L.PT : *L.P;
+      bytes += csr_bytes (Q) + (double) (Q.m + Q.n * (kind == MGC_PROLONG ? 2 : 1)) * vb;
+      flops += 2.0 * (double) Q.nnz * v0.W;
+    }
+    else {
+      bytes += csr_bytes (*L.A) + 3.0 * L.A->n * vb;
+      flops += 2.0 * (double) L.A->nnz * v0.W;
+    }
+  }
+  for (int l = 0; l < nlev; l++) {
+    const MgcLevelIn &L = lev[l];
+    MgcLev &o = a.lev[l];
+    o.n = L.B->nrows;
+    o.B = L.B->p; o.X = L.X->p; o.R = L.R ? L.R->p : nullptr;
+    if (l == 0) continue;
+    o.nc = lev[l - 1].B->nrows;
+    o.sc = L.A->vs && L.A->vs_b1 == L.b1;
+    o.rp = L.A->rp; o.ci = L.A->ci;
+    o.v = L.A->v;
+    o.vs = o.sc ? L.A->vs : L.A->v;
+    o.dinv = L.A->dinv;
+    o.prp = L.P->rp; o.pci = L.P->ci; o.pv = L.P->v;
+    o.trp = L.PT->rp; o.tci = L.PT->ci; o.tv = L.PT->v;
+    o.b1 = L.b1; o.a2 = L.a2; o.b2 = L.b2;
+  }
+  const int np = v0.npanels ();
+  // one thread per row of the largest level (whole wavefronts, at most
+  // MGC_TPB): the small launch fits beside other streams' work
+  int rows = 64;
+  for (int l = 0; l < nlev; l++) rows = std::max (rows, a.lev[l].n);
+  const int tpb = std::min (MGC_TPB, (rows + 63) / 64 * 64);
+  ProfScope ps (PK_MG_COARSE, np * bytes, np * flops, np, v0.prec, v0.W);
+  return with_row_W (v0.W, [&] (auto wc) -> int {
+    constexpr int W = decltype (wc)::value;
+    if (v0.prec == F64) {
+      hipLaunchKernelGGL ((mgc_k<double, double, double, double, W>),
+                          dim3 (W / VT<double>::N, np), dim3 (tpb), 0,
+                          stream (), a);
+    }
+    else if (v0.prec == F32) {
+      hipLaunchKernelGGL ((mgc_k<float, float, float, float, W>),
+                          dim3 (W / VT<float>::N, np), dim3 (tpb), 0,
+                          stream (), a);
+    }
+    else {
+      hipLaunchKernelGGL ((mgc_k<float, float, bf16_t, float, W>),
+                          dim3 (W / VT<bf16_t>::N, np), dim3 (tpb), 0,
+                          stream (), a);
+    }
+    LGH_LAUNCH_CHECK ();
+    return 0;
+  });
 }
 
 // ---------------------------------------------------------------------------

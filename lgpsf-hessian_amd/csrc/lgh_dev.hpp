@@ -109,7 +109,8 @@ enum ProfKind {
   PK_SM_PRE = 9,      // fused smoother, finest level, zero initial guess
   PK_SM_POST = 10,    // fused smoother, finest level, X += correction
   PK_RESID_FINE = 11, // residual on the finest AMG level
-  PK_N = 12
+  PK_MG_COARSE = 12,  // the fused coarse levels of a cycle (one launch)
+  PK_N = 13
 };
 void prof_enable (int on);
 int  prof_enabled ();
@@ -260,6 +261,33 @@ int k_smooth2x (const DevCSR &A, double b1, double a2, double b2,
                 const PMat &R0, PMat &X, int xadd);
 // X0 = Ainv B0 (dense n0 x n0 row-major inverse in X's precision)
 int k_dense_apply (const void *Ainv, int n0, const PMat &B, PMat &X);
+
+// the coarse end of a multigrid cycle in one launch: levels 0 .. nlev - 1
+// (level 0 the dense solve), the operations in ops[] (kind << 4 | level,
+// kinds MGC_*), each a pass over one level's rows by one workgroup per 16
+// bytes of a panel row.  Every row's arithmetic is that of k_smooth2x,
+// k_resid, k_spmm (spmm64_k) and k_dense_apply in the same order, so the
+// result equals the per-level launches bit for bit.
+enum MgcOp {
+  MGC_SM0 = 0,        // X = smoother(B), zero initial guess
+  MGC_SMX = 1,        // X += smoother(R)
+  MGC_RESID = 2,      // R = B - A X
+  MGC_RESTRICT = 3,   // B(l - 1) = P^T R
+  MGC_PROLONG = 4,    // X += P X(l - 1)
+  MGC_DENSE = 5       // X(0) = Ainv B(0)
+};
+constexpr int MGC_MAX_LEV = 6, MGC_MAX_OPS = 256;
+struct MgcLevelIn {
+  const DevCSR *A = nullptr, *P = nullptr, *PT = nullptr;   // P, PT: l >= 1
+  double b1 = 0, a2 = 0, b2 = 0;          // degree-2 smoother coefficients
+  const PMat *B = nullptr, *X = nullptr, *R = nullptr;       // R: l >= 1
+};
+// levels and shapes the fused launch accepts (square CSR with dinv on
+// levels >= 1, vectors of one precision and row width, P / P^T rows the
+// per-level SpMM sums in row order)
+bool mgc_supported (int nlev, const MgcLevelIn *lev, int n0);
+int k_mg_coarse (int nlev, const MgcLevelIn *lev, const void *cinv, int n0,
+                 int nops, const unsigned char *ops);
 
 // column-wise reductions: out[j] = sum_i X(i,j) Y(i,j) (fp64, ncols)
 int k_coldot (const PMat &X, const PMat &Y, double *out_dev);

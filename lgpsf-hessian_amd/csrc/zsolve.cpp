@@ -1,8 +1,10 @@
 // zsolve.cpp — device Z-solve tiers (see zsolve.hpp).
 #include "zsolve.hpp"
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 namespace lgh {
 
@@ -204,6 +206,72 @@ mg_smooth (DevMG &g, DevMG::Work &w, int l, const PMat &B, PMat &X,
   return 0;
 }
 
+// ---- the fused coarse cycle (k_mg_coarse) ----
+// Levels 1 .. top run in one launch when each has at most g_mgc_rows rows
+// and g_mgc_nnz nonzeros (0 rows: never); the finest level never does.
+static std::atomic<int> g_mgc_rows { 1024 }, g_mgc_nnz { 65536 };
+
+void
+mg_coarse_config (int max_rows, int max_nnz)
+{
+  g_mgc_rows = max_rows < 0 ? 1024 : max_rows;
+  g_mgc_nnz = max_nnz < 0 ? 65536 : max_nnz;
+}
+
+// the operations of mg_run (l, x_init) below, in order
+static void
+mgc_prog (const DevMG &g, int l, int x_init, std::vector<unsigned char> &ops)
+{
+  if (l == 0) { ops.push_back ((unsigned char) (MGC_DENSE << 4)); return; }
+  auto op = [&] (int k) { ops.push_back ((unsigned char) (k << 4 | l)); };
+  if (x_init) { op (MGC_RESID); op (MGC_SMX); }
+  else op (MGC_SM0);
+  op (MGC_RESID);
+  op (MGC_RESTRICT);
+  mgc_prog (g, l - 1, 0, ops);
+  if (g.gamma == 2 && l - 1 >= 1) mgc_prog (g, l - 1, 1, ops);
+  op (MGC_PROLONG);
+  op (MGC_RESID);
+  op (MGC_SMX);
+}
+
+// the coarse correction of level l as one fused launch when levels
+// 0 .. l - 1 qualify (*done = 1), else nothing (*done = 0)
+static int
+mg_coarse_fused (DevMG &g, DevMG::Work &w, int l, int *done)
+{
+  *done = 0;
+  const int top = l - 1;
+  if (top < 1 || top >= MGC_MAX_LEV || g_mgc_rows.load () <= 0) return 0;
+  for (int j = 1; j <= top; j++)
+    if (!fused_level (g, j, w.W) || g.n[j] > g_mgc_rows.load ()
+        || g.A[j]->nnz > g_mgc_nnz.load ())
+      return 0;
+  MgcLevelIn lev[MGC_MAX_LEV];
+  for (int j = 0; j <= top; j++) {
+    lev[j].B = &w.Bl[j];
+    lev[j].X = &w.Xl[j];
+    if (j == 0) continue;
+    lev[j].R = &w.R1[j];
+    lev[j].A = g.A[j];
+    lev[j].P = g.P[j];
+    lev[j].PT = g.PT[j];
+    ChebCoef cc (g.slo[j], g.shi[j]);
+    double a1;
+    cc.step (1, &a1, &lev[j].b1);
+    cc.step (2, &lev[j].a2, &lev[j].b2);
+  }
+  if (!mgc_supported (top + 1, lev, g.n0)) return 0;
+  std::vector<unsigned char> ops;
+  mgc_prog (g, top, 0, ops);
+  if (g.gamma == 2) mgc_prog (g, top, 1, ops);
+  if (ops.size () > (size_t) MGC_MAX_OPS) return 0;
+  LGH_TRY (k_mg_coarse (top + 1, lev, g.cinv, g.n0, (int) ops.size (),
+                        ops.data ()));
+  *done = 1;
+  return 0;
+}
+
 // the coarse correction of level l: one cycle on level l - 1, or two
 // (gamma = 2) when level l - 1 is not the coarsest (exact) one
 static int mg_run (DevMG &g, DevMG::Work &w, int l, const PMat &B, PMat &X,
@@ -211,6 +279,9 @@ static int mg_run (DevMG &g, DevMG::Work &w, int l, const PMat &B, PMat &X,
 static int
 mg_coarse (DevMG &g, DevMG::Work &w, int l)
 {
+  int done = 0;
+  LGH_TRY (mg_coarse_fused (g, w, l, &done));
+  if (done) return 0;
   LGH_TRY (mg_run (g, w, l - 1, w.Bl[l - 1], w.Xl[l - 1], 0));
   if (g.gamma == 2 && l - 1 >= 1)
     LGH_TRY (mg_run (g, w, l - 1, w.Bl[l - 1], w.Xl[l - 1], 1));

@@ -33,6 +33,10 @@ int mg_upload (const AmgHierarchy &h, int prec, DevMG **out);
 // X = cycle(B) at the finest level (zero initial guess; V- or W-cycle by
 // g.gamma)
 int mg_vcycle (DevMG &g, const PMat &B, PMat &X);
+// caps of the levels the fused coarse launch (k_mg_coarse) takes: rows and
+// nonzeros per level (negative: the defaults 1024 and 65536; rows 0: the
+// per-level launches everywhere)
+void mg_coarse_config (int max_rows, int max_nnz);
 
 struct ChebWork {
   int W = 0, np = -1, prec = -1, nrows = 0;

@@ -415,6 +415,35 @@ def test_grid_sweep_chunks_bitwise(dev, grid, mixed):
     assert np.array_equal(a.lam, b.lam)
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_grid_fused_coarse_levels_bitwise(dev, grid, mixed):
+    """A build (ell = 150: two whole panels and a 22-column tail) and a
+    solve and sample, with the small AMG levels below the finest as one
+    fused launch per coarse visit (the default), equal the same with one
+    launch per operation and level bit for bit."""
+    assert len(grid["h"].levels()) >= 3
+    o = dev.glr_opts(ell=150, seed=0, trunc_rel=1e-6)
+    x = dev.xopts(mixed=mixed)
+    rhs = np.random.default_rng(8).standard_normal((grid["Z"].shape[0], 3))
+    dev.prof_enable(True)
+    try:
+        dev.prof_collect("mg_coarse")
+        f = dev.GLR(grid["B"], grid["prior"], o, x=x)
+        fs, fx = f.solve_block(1e-2, rhs), f.sample(1e-2, rhs[:, 0])
+        assert dev.prof_collect("mg_coarse")[0] > 0
+        dev.mg_coarse_config(0)
+        p = dev.GLR(grid["B"], grid["prior"], o, x=x)
+        ps, px = p.solve_block(1e-2, rhs), p.sample(1e-2, rhs[:, 0])
+        assert dev.prof_collect("mg_coarse")[0] == 0
+    finally:
+        dev.mg_coarse_config(-1, -1)
+        dev.prof_enable(False)
+    assert f.kept == p.kept
+    assert np.array_equal(f.lam, p.lam)
+    assert np.array_equal(f.U, p.U)
+    assert np.array_equal(fs, ps) and np.array_equal(fx, px)
+
+
 def test_grid_extend_incremental(dev, grid, oracle):
     g1 = dev.GLR(grid["B"], grid["prior"], dev.glr_opts(ell=30, seed=0, trunc_rel=1e-4))
     g2 = dev.GLR(grid["B"], grid["prior"], dev.glr_opts(ell=20, seed=0, trunc_rel=1e-4))

@@ -223,6 +223,43 @@ def test_vcycle_narrow_panel_bitwise(dev, prec, dim):
     assert np.array_equal(x64, x8)
 
 
+@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("gamma", [1, 2])
+def test_vcycle_fused_coarse_bitwise(dev, prec, gamma):
+    """The small levels below the 16 384-row finest level run as one fused
+    launch per coarse visit with the bits of one launch per operation and
+    level: by default the 319-row level and the 38-row dense solve, with a
+    4096-row cap the 2 752-row level too; 64-wide (two panels), 16-wide
+    (two panels) and 8-wide panels, V- and W-cycles, every storage
+    precision."""
+    Z, M = W.prior_Z(128, 2, 1.0, 0.1)
+    h = dev.AMG(Z)
+    sizes = [l["A"].shape[0] for l in h.levels()]
+    assert len(sizes) == 4 and sizes[1] <= 1024 < sizes[2] <= 4096 < sizes[3]
+    h.set_cycle(gamma)
+    rng = np.random.default_rng(11 + prec)
+    for Wd, m in ((64, 100), (16, 30), (8, 5)):
+        B = rng.standard_normal((Z.shape[0], m))
+        dev.prof_enable(True)
+        try:
+            dev.prof_collect("mg_coarse")
+            dev.mg_coarse_config(0)
+            xp = h.vcycle(B, prec=prec, W=Wd)
+            per_level = dev.prof_collect("mg_coarse")[0]
+            for cap in (-1, 4096):
+                dev.mg_coarse_config(cap)
+                xf = h.vcycle(B, prec=prec, W=Wd)
+                fused = dev.prof_collect("mg_coarse")[0]
+                # one launch per visit of the level above the fused ones
+                # (the W-cycle visits the 2 752-row level twice)
+                want = 2 if (gamma == 2 and cap == -1) else 1
+                assert fused == want and per_level == 0, (Wd, cap, fused, per_level)
+                assert np.array_equal(xf, xp), (Wd, cap, np.abs(xf - xp).max())
+        finally:
+            dev.mg_coarse_config(-1, -1)
+            dev.prof_enable(False)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("n,m", [(1_000_003, 13), (1 << 20, 16), (1 << 20, 25)])
 def test_large_host_blocks_round_trip(dev, n, m):
