@@ -408,8 +408,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--vcycle", choices=("bf16", "f32"), default="bf16",
                     help="V-cycle storage precision inside the FP64 sweeps")
-    ap.add_argument("--sketch-vcycle", choices=("bf16", "f32"), default="f32",
-                    help="V-cycle storage precision inside the FP32 sweeps")
+    ap.add_argument("--sketch-vcycle", choices=("bf16", "f32"), default="bf16",
+                    help="V-cycle storage precision inside the FP32 sweeps (the "
+                         "residual-gate sweep; the sketch too with --sketch f32)")
     ap.add_argument("--sketch", choices=("f32", "f64"), default=None,
                     help="vector precision of the sketch sweep (default: the "
                          "mixed policy's)")

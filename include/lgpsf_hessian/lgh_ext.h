@@ -24,7 +24,8 @@ extern "C" {
  * matrices and arithmetic; the Chebyshev step rounds each new direction
  * once and updates X and R with that same value, so the Z solves stay
  * FP64-accurate), keeps CholQR2, T = Q^T F Q, the eigensolve and U in
- * FP64, runs the residual-gate sweep in FP32, and takes downstream
+ * FP64, runs the residual-gate sweep with FP32 vectors around a BF16
+ * V-cycle (a diagnostic at FP32 rounding either way), and takes downstream
  * calls (one column or many) through the same blocked tier.
  * prec_sketch = F32 runs the sketch sweep with FP32 vectors, B and Z
  * instead (its V-cycle in prec_sketch_precond).  A V-cycle precision

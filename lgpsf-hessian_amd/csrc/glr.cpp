@@ -54,7 +54,7 @@ lgh_glr_xopts_mixed (void)
   x.prec_sketch = LGH_PREC_F64;
   x.prec_precond = LGH_PREC_BF16;
   x.prec_check = LGH_PREC_F32;
-  x.prec_sketch_precond = LGH_PREC_F32;
+  x.prec_sketch_precond = LGH_PREC_BF16;
   x.solve_blocked_min = 1;
   x.sweep_panels = 0;
   return x;

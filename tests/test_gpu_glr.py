@@ -324,9 +324,11 @@ def test_grid_glr_matches_oracle(dev, grid, oracle, policy):
         x.prec_precond = dev.PREC_F32
     elif policy == "mixed-fp32-sketch":
         x.prec_sketch = dev.PREC_F32
+        x.prec_sketch_precond = dev.PREC_F32
     else:
         assert not mixed or (x.prec_precond == dev.PREC_BF16
-                             and x.prec_sketch == dev.PREC_F64)
+                             and x.prec_sketch == dev.PREC_F64
+                             and x.prec_sketch_precond == dev.PREC_BF16)
     g = dev.GLR(grid["B"], grid["prior"], dev.glr_opts(ell=ell, seed=0), x=x)
     assert g.kept == og.kept
     if mixed:
