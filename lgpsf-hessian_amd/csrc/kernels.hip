@@ -1557,6 +1557,90 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
   ntst (Rpc + qi, r);
 }
 
+// the mixed solve's residual replacement (zsolve.cpp cheb_mixed_run) in
+// one pass: X = X32 widened, R = B - A X with resid64_k's sums (the FP32
+// values gathered and widened are X's values), and Rpc = R in the
+// V-cycle's storage precision -- in place of a widening copy, an FP64
+// residual that reads X back, and a narrowing copy of R
+template <typename SP, int W = 64>
+__global__ void __launch_bounds__ (TPB)
+mix_switch_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
+              const double *__restrict__ v, const double *__restrict__ B,
+              const float *__restrict__ X32, double *__restrict__ X,
+              double *__restrict__ R, SP *__restrict__ Rpc)
+{
+  LGH_ROWMAP (double)
+  double acc[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) acc[i] = 0;
+  for (int kb = k0; kb < k1; kb += NG) {
+    int c[NG];
+    double w[NG];
+#pragma unroll
+    for (int u = 0; u < NG; u++) {
+      const bool in = kb + u < k1;
+      c[u] = in ? ci[kb + u] : row;
+      w[u] = in ? v[kb + u] : 0.0;
+    }
+    float2 x[NG];
+#pragma unroll
+    for (int u = 0; u < NG; u++) {
+      if (kb + u < k1) x[u] = *(const float2 *) (X32 + off + (size_t) c[u] * W);
+      else x[u] = {};
+    }
+#pragma unroll
+    for (int u = 0; u < NG; u++) {
+      const double t[N] = { (double) x[u].x, (double) x[u].y };
+#pragma unroll
+      for (int i = 0; i < N; i++) acc[i] += w[u] * t[i];
+    }
+  }
+  const size_t q = off + (size_t) row * W;
+  double b[N], xo[N];
+  vld (B + q, b);
+  vld (X32 + q, xo);
+#pragma unroll
+  for (int i = 0; i < N; i++) b[i] -= acc[i];
+  vst (R + q, b);
+  vst (X + q, xo);
+  vst (Rpc + q, b);
+}
+
+int
+k_mix_switch (const DevCSR &A, const PMat &B, const PMat &X32, PMat &X,
+              PMat &R, PMat &Rpc)
+{
+  const int W = B.W;
+  LGH_CHECK (A.prec == F64 && B.prec == F64 && X32.prec == F32 && X.prec == F64
+             && R.prec == F64 && (Rpc.prec == F32 || Rpc.prec == BF16)
+             && row_width (W) && X32.W == W && X.W == W && R.W == W && Rpc.W == W
+             && B.nrows == A.n && A.n == A.m && X32.nrows == A.n && X.nrows == A.n
+             && R.nrows == A.n && Rpc.nrows == A.n
+             && X32.npanels () == B.npanels () && X.npanels () == B.npanels ()
+             && R.npanels () == B.npanels () && Rpc.npanels () == B.npanels (),
+             8, "k_mix_switch: shape/precision mismatch");
+  const double vec = (double) A.n * W;
+  ProfScope ps (PK_OTHER, B.npanels () * (csr_bytes (A)
+                + vec * (8 + 4 + 8 + 8 + prec_bytes (Rpc.prec))),
+                B.npanels () * 2.0 * (double) A.nnz * W, B.npanels (), Rpc.prec, W);
+  const dim3 g = grid64 (A.n, B.npanels (), F64, W);
+  return with_row_W (W, [&] (auto wc) -> int {
+    constexpr int WW = decltype (wc)::value;
+    if (Rpc.prec == BF16)
+      hipLaunchKernelGGL ((mix_switch_k<bf16_t, WW>), g, dim3 (TPB), 0, stream (),
+                          A.n, A.rp, A.ci, (const double *) A.v, (const double *) B.p,
+                          (const float *) X32.p, (double *) X.p, (double *) R.p,
+                          (bf16_t *) Rpc.p);
+    else
+      hipLaunchKernelGGL ((mix_switch_k<float, WW>), g, dim3 (TPB), 0, stream (),
+                          A.n, A.rp, A.ci, (const double *) A.v, (const double *) B.p,
+                          (const float *) X32.p, (double *) X.p, (double *) R.p,
+                          (float *) Rpc.p);
+    LGH_LAUNCH_CHECK ();
+    return 0;
+  });
+}
+
 int
 k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                    const PMat *D32, const PMat &R, PMat &X, int x_mode,

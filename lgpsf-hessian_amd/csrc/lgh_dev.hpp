@@ -254,6 +254,11 @@ enum {
 int k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                        const PMat *D32, const PMat &R, PMat &X, int x_mode,
                        PMat &Dn32, PMat &Rn, PMat &Rnpc);
+// the mixed solve's residual replacement in one pass: X = X32 widened to
+// FP64, R = B - A X (FP64, the sums of k_resid), Rpc = R in Rpc's storage
+// precision (FP32 or BF16, rounded as k_copy_cols rounds)
+int k_mix_switch (const DevCSR &A, const PMat &B, const PMat &X32, PMat &X,
+                  PMat &R, PMat &Rpc);
 // degree-2 Chebyshev-Jacobi smoother in one pass (W = 64):
 // D1 = b1 dinv R0, R1 = R0 - A D1; D2 = a2 D1 + b2 dinv R1;
 // X = (xadd ? X : 0) + D1 + D2 (D1, R1 recomputed at the neighbours)

@@ -563,10 +563,8 @@ cheb_mixed_run (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
     di ^= 1;
     if (k == k_sw) {
       // residual replacement: X in FP64, R = B - A X in FP64, and the next
-      // preconditioner input from it
-      LGH_TRY (k_copy_cols (w.X32, 0, X, 0, X.npanels () * X.W));
-      LGH_TRY (k_resid (A64, B, X, w.R[ri]));
-      LGH_TRY (k_copy_cols (w.R[ri], 0, w.Rpc, 0, B.npanels () * B.W));
+      // preconditioner input from it (one pass)
+      LGH_TRY (k_mix_switch (A64, B, w.X32, X, w.R[ri], w.Rpc));
       Rsrc = &w.R[ri];
       ri ^= 1;
     }
