@@ -634,11 +634,8 @@ def main():
         (m, m / t_blk, extra["solve_latency_p50_s"]))
     del X
     # incremental extension of the resident mixed GLR by 200 columns
-    # (hashed columns ell..ell+199, power pass, BCGS2, bordered projection),
-    # timed before the serving run: the serving run's batches of 1 to 256
-    # columns leave the device block cache in a state that makes the
-    # extension's first allocations cost more (0.68-0.71 s after it, 0.51 s
-    # before it); the served object keeps its 56 modes either way
+    # (hashed columns ell..ell+199, power pass, BCGS2, bordered projection);
+    # the served object below keeps its 56 modes
     if args.extend > 0:
         lgh.lib().lgh_hip_sync()
         t0 = time.perf_counter()

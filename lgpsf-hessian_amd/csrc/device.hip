@@ -149,19 +149,19 @@ dev_sync ()
   return 0;
 }
 
-static void free_list_release (hipStream_t s);
+static void free_list_release (hipStream_t s, bool keep);
 static void stage_release ();
 
 void
-dev_thread_exit ()
+dev_thread_exit (bool keep_blocks)
 {
   ThreadCtx &c = t_ctx;
   if (!c.main) return;
   (void) hipStreamSynchronize (c.main);
-  free_list_release (c.main);
+  free_list_release (c.main, keep_blocks);
   if (c.aux) {
     (void) hipStreamSynchronize (c.aux);
-    free_list_release (c.aux);
+    free_list_release (c.aux, keep_blocks);
     (void) hipStreamDestroy (c.aux);
   }
   for (hipEvent_t e : t_retired) (void) hipEventDestroy (e);
@@ -297,14 +297,24 @@ flush_cache_locked ()
   g_cached = 0;
 }
 
+// a leaving thread's free list (its stream synchronized): to the shared
+// list, or freed
 static void
-free_list_release (hipStream_t s)
+free_list_release (hipStream_t s, bool keep)
 {
   std::lock_guard<std::mutex> lk (g_mu);
   auto l = g_free.find (s);
   if (l == g_free.end ()) return;
-  auto &shared = g_free[nullptr];
-  for (auto &kv : l->second) shared.insert (kv);
+  if (keep) {
+    auto &shared = g_free[nullptr];
+    for (auto &kv : l->second) shared.insert (kv);
+  }
+  else {
+    for (auto &kv : l->second) {
+      (void) hipFree (kv.second);
+      g_cached -= kv.first;
+    }
+  }
   g_free.erase (s);
 }
 

@@ -46,9 +46,12 @@ int hip_fail (hipError_t e, const char *what, const char *file, int line);
 // ---- context ------------------------------------------------------------
 int  dev_init ();                 // idempotent; selects device 0 and
                                   // makes the calling thread's stream
-void dev_thread_exit ();          // a worker thread leaves the engine:
-                                  // waits for its stream, hands its cached
-                                  // blocks to the shared list
+// a worker thread leaves the engine: waits for its streams and hands its
+// cached blocks to the shared list (keep_blocks: a batch worker, whose
+// blocks the next batch reuses) or frees them (a serving thread, whose
+// blocks of ever-changing batch widths nothing after it reuses: left in
+// the cache they would push the next large request into a full flush)
+void dev_thread_exit (bool keep_blocks = true);
 // the library is initialized and the calling thread has its stream (so an
 // engine call on it creates nothing)
 bool dev_ready ();

@@ -259,7 +259,7 @@ device_loop (lgh_serve *s, int d)
     }
     (void) dev_sync ();   // the work buffers are freed with no kernel left on them
   }
-  dev_thread_exit ();     // the device thread's streams and cached blocks
+  dev_thread_exit (false);  // the device thread's streams; its blocks freed
   std::lock_guard<std::mutex> lk (s->mu);
   s->drain.push_back (-1);
   s->cv_drain.notify_one ();
