@@ -2170,13 +2170,15 @@ k_dense_apply (const void *Ainv, int n0, const PMat &B, PMat &X)
 
 // ---------------------------------------------------------------------------
 // fused coarse cycle (zsolve_impl.h:406-485 on the small levels).  Below a
-// few thousand rows a level's kernels are a few microseconds of work each,
-// and a cycle spends its time between dependent launches.  The columns of a
+// thousand rows a level's kernels are a few microseconds of work each, and
+// a cycle spends its time between dependent launches.  The columns of a
 // cycle are independent, so one workgroup can run every small level for its
 // own N columns (16 bytes of a panel row) of one panel, ordering the
-// passes by workgroup barriers: the workgroup's stores are visible to its
-// later loads (one CU; the barrier carries a workgroup-scope release and
-// acquire).  One launch per coarse visit replaces six per level.
+// passes by workgroup barriers.  Every level's slice of B, X and R lives in
+// LDS for the whole launch (16 bytes a row): the top level's B comes in
+// from HBM at the start and its X goes out at the end, and in between a
+// pass reads only the CSR arrays (L2-resident) from memory.  One launch per
+// coarse visit replaces six per level.
 
 constexpr int MGC_TPB = 1024;
 
@@ -2189,19 +2191,30 @@ struct MgcLev {
   const void *pv;
   const int *trp, *tci;                  // P^T (nc x n)
   const void *tv;
-  void *B, *X, *R;                       // the level's panels
+  int ob, ox, orr;                       // LDS offsets of B, X, R (bytes)
   double b1, a2, b2;
 };
 struct MgcArgs {
   MgcLev lev[MGC_MAX_LEV];
   const void *cinv;
-  int n0, nops;
+  const void *Btop;                      // the top level's B, X panels (HBM)
+  void *Xtop;
+  int top, n0, nops;
   unsigned char op[MGC_MAX_OPS];
 };
+// LDS bytes of the levels' slices: B and X per level, R above level 0
+static inline int
+mgc_lds_bytes (int nlev, const int *rows)
+{
+  int b = 0;
+  for (int l = 0; l < nlev; l++) b += 16 * rows[l] * (l > 0 ? 3 : 2);
+  return b;
+}
+constexpr int MGC_MAX_LDS = 64 * 1024;
 
 // sum over one CSR row of w(k, c) X(c) in the row kernels' order: groups of
 // NG entries, the slots past the row's end adding 0 * 0
-template <typename TV, typename T, typename S, int W, typename WF>
+template <typename TV, typename T, typename S, int STRIDE, typename WF>
 __device__ __forceinline__ void
 mgc_rowsum (int k0, int k1, int pad, const int *ci, const TV *v, const S *Xb,
             WF wf, T (&acc)[VT<S>::N])
@@ -2223,7 +2236,7 @@ mgc_rowsum (int k0, int k1, int pad, const int *ci, const TV *v, const S *Xb,
     typename Raw16<S>::type x[NG];
 #pragma unroll
     for (int u = 0; u < NG; u++) {
-      if (kb + u < k1) x[u] = rld (Xb + (size_t) c[u] * W);
+      if (kb + u < k1) x[u] = rld (Xb + (size_t) c[u] * STRIDE);
       else x[u] = {};
     }
 #pragma unroll
@@ -2242,80 +2255,90 @@ __global__ void __launch_bounds__ (MGC_TPB)
 mgc_k (const MgcArgs a)
 {
   constexpr int N = VT<S>::N;
+  typedef typename Raw16<S>::type raw_t;
+  extern __shared__ uint4 mgc_lds[];
+  char *sm = (char *) mgc_lds;
   const int tid = threadIdx.x, nt = blockDim.x;
   const size_t col = (size_t) blockIdx.x * N;
   const size_t pan = blockIdx.y;
   auto plain = [] (T w, int) { return w; };
+  // the level slices: row r of a level's B / X / R at byte 16 r of its area
+  auto vb = [&] (int l) { return (S *) (sm + a.lev[l].ob); };
+  auto vx = [&] (int l) { return (S *) (sm + a.lev[l].ox); };
+  auto vr = [&] (int l) { return (S *) (sm + a.lev[l].orr); };
+  {
+    const MgcLev &L = a.lev[a.top];
+    const S *g = (const S *) a.Btop + pan * L.n * W + col;
+    for (int row = tid; row < L.n; row += nt)
+      *(raw_t *) (vb (a.top) + (size_t) row * N) = rld (g + (size_t) row * W);
+  }
+  __syncthreads ();
   for (int q = 0; q < a.nops; q++) {
     const int kind = a.op[q] >> 4, l = a.op[q] & 15;
     const MgcLev &L = a.lev[l];
     const int n = L.n;
-    const size_t off = pan * n * W + col;
     if (kind == MGC_DENSE) {
       const TD *Ai = (const TD *) a.cinv;
-      const S *B = (const S *) L.B + off;
-      S *X = (S *) L.X + off;
+      const S *B = vb (0);
+      S *X = vx (0);
       for (int row = tid; row < n; row += nt) {
         TD acc[N], b[N];
 #pragma unroll
         for (int i = 0; i < N; i++) acc[i] = 0;
         for (int k = 0; k < n; k++) {
           const TD ak = Ai[(size_t) row * n + k];
-          vld (B + (size_t) k * W, b);
+          vld (B + (size_t) k * N, b);
 #pragma unroll
           for (int i = 0; i < N; i++) acc[i] += ak * b[i];
         }
-        vst (X + (size_t) row * W, acc);
+        vst (X + (size_t) row * N, acc);
       }
     }
     else if (kind == MGC_RESTRICT) {
       // B(l - 1) = P^T R, rows of level l - 1
-      const MgcLev &C = a.lev[l - 1];
-      const S *R = (const S *) L.R + off;
-      S *Bc = (S *) C.B + pan * L.nc * W + col;
+      const S *R = vr (l);
+      S *Bc = vb (l - 1);
       for (int row = tid; row < L.nc; row += nt) {
         const int k0 = L.trp[row], k1 = L.trp[row + 1];
         T acc[N];
-        mgc_rowsum<TV, T, S, W> (k0, k1, k0 < k1 ? L.tci[k0] : 0, L.tci,
+        mgc_rowsum<TV, T, S, N> (k0, k1, k0 < k1 ? L.tci[k0] : 0, L.tci,
                                  (const TV *) L.tv, R, plain, acc);
-        vst (Bc + (size_t) row * W, acc);
+        vst (Bc + (size_t) row * N, acc);
       }
     }
     else if (kind == MGC_PROLONG) {
       // X += P X(l - 1)
-      const MgcLev &C = a.lev[l - 1];
-      const S *Xc = (const S *) C.X + pan * L.nc * W + col;
-      S *X = (S *) L.X + off;
+      const S *Xc = vx (l - 1);
+      S *X = vx (l);
       for (int row = tid; row < n; row += nt) {
         const int k0 = L.prp[row], k1 = L.prp[row + 1];
         T acc[N], xo[N];
-        mgc_rowsum<TV, T, S, W> (k0, k1, k0 < k1 ? L.pci[k0] : 0, L.pci,
+        mgc_rowsum<TV, T, S, N> (k0, k1, k0 < k1 ? L.pci[k0] : 0, L.pci,
                                  (const TV *) L.pv, Xc, plain, acc);
-        vld (X + (size_t) row * W, xo);
+        vld (X + (size_t) row * N, xo);
 #pragma unroll
         for (int i = 0; i < N; i++) acc[i] = acc[i] + xo[i];
-        vst (X + (size_t) row * W, acc);
+        vst (X + (size_t) row * N, acc);
       }
     }
     else if (kind == MGC_RESID) {
-      const S *B = (const S *) L.B + off;
-      const S *X = (const S *) L.X + off;
-      S *R = (S *) L.R + off;
+      const S *B = vb (l), *X = vx (l);
+      S *R = vr (l);
       for (int row = tid; row < n; row += nt) {
         T acc[N], b[N];
-        mgc_rowsum<TV, T, S, W> (L.rp[row], L.rp[row + 1], row, L.ci,
+        mgc_rowsum<TV, T, S, N> (L.rp[row], L.rp[row + 1], row, L.ci,
                                  (const TV *) L.v, X, plain, acc);
-        vld (B + (size_t) row * W, b);
+        vld (B + (size_t) row * N, b);
 #pragma unroll
         for (int i = 0; i < N; i++) b[i] -= acc[i];
-        vst (R + (size_t) row * W, b);
+        vst (R + (size_t) row * N, b);
       }
     }
     else {
       // degree-2 smoother (smx_k): MGC_SM0 from B, MGC_SMX adds to X from R
       const bool xadd = kind == MGC_SMX;
-      const S *R0 = (const S *) (xadd ? L.R : L.B) + off;
-      S *X = (S *) L.X + off;
+      const S *R0 = xadd ? vr (l) : vb (l);
+      S *X = vx (l);
       const TV *dinv = (const TV *) L.dinv;
       const T b1 = (T) L.b1, a2 = (T) L.a2, b2 = (T) L.b2;
       const bool sc = L.sc;
@@ -2324,19 +2347,25 @@ mgc_k (const MgcArgs a)
       };
       for (int row = tid; row < n; row += nt) {
         T acc[N], r0[N], x0[N], out[N];
-        mgc_rowsum<TV, T, S, W> (L.rp[row], L.rp[row + 1], row, L.ci,
+        mgc_rowsum<TV, T, S, N> (L.rp[row], L.rp[row + 1], row, L.ci,
                                  (const TV *) L.vs, R0, scaled, acc);
         const T dr = (T) dinv[row];
-        vld (R0 + (size_t) row * W, r0);
-        if (xadd) vld (X + (size_t) row * W, x0);
+        vld (R0 + (size_t) row * N, r0);
+        if (xadd) vld (X + (size_t) row * N, x0);
         else vzero (x0);
 #pragma unroll
         for (int i = 0; i < N; i++)
           out[i] = smx_update (r0[i], acc[i], x0[i], dr, b1, a2, b2, xadd);
-        vst (X + (size_t) row * W, out);
+        vst (X + (size_t) row * N, out);
       }
     }
     __syncthreads ();
+  }
+  {
+    const MgcLev &L = a.lev[a.top];
+    S *g = (S *) a.Xtop + pan * L.n * W + col;
+    for (int row = tid; row < L.n; row += nt)
+      *(raw_t *) (g + (size_t) row * W) = *(const raw_t *) (vx (a.top) + (size_t) row * N);
   }
 }
 
@@ -2371,7 +2400,9 @@ mgc_supported (int nlev, const MgcLevelIn *lev, int n0)
     for (const DevCSR *Q : { P, PT })
       if (Q->n != Q->m && Q->nnz >= 64 * (int64_t) Q->n) return false;
   }
-  return true;
+  int rows[MGC_MAX_LEV];
+  for (int l = 0; l < nlev; l++) rows[l] = lev[l].B->nrows;
+  return mgc_lds_bytes (nlev, rows) <= MGC_MAX_LDS;
 }
 
 int
@@ -2407,12 +2438,16 @@ k_mg_coarse (int nlev, const MgcLevelIn *lev, const void *cinv, int n0,
       flops += 2.0 * (double) L.A->nnz * v0.W;
     }
   }
+  int lds = 0;
   for (int l = 0; l < nlev; l++) {
     const MgcLevelIn &L = lev[l];
     MgcLev &o = a.lev[l];
     o.n = L.B->nrows;
-    o.B = L.B->p; o.X = L.X->p; o.R = L.R ? L.R->p : nullptr;
+    o.ob = lds; lds += 16 * o.n;
+    o.ox = lds; lds += 16 * o.n;
+    o.orr = -1;
     if (l == 0) continue;
+    o.orr = lds; lds += 16 * o.n;
     o.nc = lev[l - 1].B->nrows;
     o.sc = L.A->vs && L.A->vs_b1 == L.b1;
     o.rp = L.A->rp; o.ci = L.A->ci;
@@ -2423,6 +2458,10 @@ k_mg_coarse (int nlev, const MgcLevelIn *lev, const void *cinv, int n0,
     o.trp = L.PT->rp; o.tci = L.PT->ci; o.tv = L.PT->v;
     o.b1 = L.b1; o.a2 = L.a2; o.b2 = L.b2;
   }
+  a.top = nlev - 1;
+  a.Btop = lev[nlev - 1].B->p;
+  a.Xtop = lev[nlev - 1].X->p;
+  LGH_CHECK (lds <= MGC_MAX_LDS, 8, "k_mg_coarse: levels exceed the LDS bound");
   const int np = v0.npanels ();
   // one thread per row of the largest level (whole wavefronts, at most
   // MGC_TPB): the small launch fits beside other streams' work
@@ -2434,17 +2473,17 @@ k_mg_coarse (int nlev, const MgcLevelIn *lev, const void *cinv, int n0,
     constexpr int W = decltype (wc)::value;
     if (v0.prec == F64) {
       hipLaunchKernelGGL ((mgc_k<double, double, double, double, W>),
-                          dim3 (W / VT<double>::N, np), dim3 (tpb), 0,
+                          dim3 (W / VT<double>::N, np), dim3 (tpb), lds,
                           stream (), a);
     }
     else if (v0.prec == F32) {
       hipLaunchKernelGGL ((mgc_k<float, float, float, float, W>),
-                          dim3 (W / VT<float>::N, np), dim3 (tpb), 0,
+                          dim3 (W / VT<float>::N, np), dim3 (tpb), lds,
                           stream (), a);
     }
     else {
       hipLaunchKernelGGL ((mgc_k<float, float, bf16_t, float, W>),
-                          dim3 (W / VT<bf16_t>::N, np), dim3 (tpb), 0,
+                          dim3 (W / VT<bf16_t>::N, np), dim3 (tpb), lds,
                           stream (), a);
     }
     LGH_LAUNCH_CHECK ();

@@ -228,8 +228,9 @@ def test_vcycle_narrow_panel_bitwise(dev, prec, dim):
 def test_vcycle_fused_coarse_bitwise(dev, prec, gamma):
     """The small levels below the 16 384-row finest level run as one fused
     launch per coarse visit with the bits of one launch per operation and
-    level: by default the 319-row level and the 38-row dense solve, with a
-    4096-row cap the 2 752-row level too; 64-wide (two panels), 16-wide
+    level: the 319-row level and the 38-row dense solve (with a 4096-row
+    cap too: the 2 752-row level's slices would not fit the launch's 64 KB
+    of LDS, so it keeps its own launches); 64-wide (two panels), 16-wide
     (two panels) and 8-wide panels, V- and W-cycles, every storage
     precision."""
     Z, M = W.prior_Z(128, 2, 1.0, 0.1)
@@ -252,7 +253,7 @@ def test_vcycle_fused_coarse_bitwise(dev, prec, gamma):
                 fused = dev.prof_collect("mg_coarse")[0]
                 # one launch per visit of the level above the fused ones
                 # (the W-cycle visits the 2 752-row level twice)
-                want = 2 if (gamma == 2 and cap == -1) else 1
+                want = 2 if gamma == 2 else 1
                 assert fused == want and per_level == 0, (Wd, cap, fused, per_level)
                 assert np.array_equal(xf, xp), (Wd, cap, np.abs(xf - xp).max())
         finally:
