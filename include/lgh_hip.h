@@ -109,7 +109,9 @@ void lgh_hip_spmm_group_config (int rg, int ppw);
  * visit of a cycle (the small levels above the dense solve, each with at
  * most max_rows rows and max_nnz nonzeros; negative: the defaults 1024
  * and 65536; max_rows = 0: one launch per operation and level).  Both
- * paths give the same bits; this selects the launch shape. */
+ * paths give the same bits; this selects the launch shape.  Change it only
+ * while no other thread runs solves (a fused launch keeps a packed copy of
+ * its levels' arrays, rebuilt when the fused depth changes). */
 void lgh_hip_mg_coarse_config (int max_rows, int max_nnz);
 /* the autotuner's decisions so far as text, one line per matrix shape:
  * "rows nonzeros value_prec rg ppw_fp64 ppw_fp32 ms_per_panel_fp64

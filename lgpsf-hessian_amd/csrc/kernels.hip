@@ -2192,14 +2192,19 @@ struct MgcLev {
   const int *trp, *tci;                  // P^T (nc x n)
   const void *tv;
   int ob, ox, orr;                       // LDS offsets of B, X, R (bytes)
+  // LDS offsets of the arrays above when the launch stages them
+  int orp, oci, ov, ovs, odinv, oprp, opci, opv, otrp, otci, otv;
   double b1, a2, b2;
 };
 struct MgcArgs {
   MgcLev lev[MGC_MAX_LEV];
   const void *cinv;
+  int ocinv;
   const void *Btop;                      // the top level's B, X panels (HBM)
   void *Xtop;
   int top, n0, nops;
+  const uint4 *img;                      // the packed arrays (MgcImage), copied
+  int img_dst, img_n16;                  // to LDS byte img_dst, 16 B at a time
   unsigned char op[MGC_MAX_OPS];
 };
 // LDS bytes of the levels' slices: B and X per level, R above level 0
@@ -2250,7 +2255,9 @@ mgc_rowsum (int k0, int k1, int pad, const int *ci, const TV *v, const S *Xb,
 }
 
 // TV matrix values, T arithmetic, S vector storage, TD the dense inverse
-template <typename TV, typename T, typename S, typename TD, int W>
+// CL: the levels' CSR arrays, 1/diag and the dense inverse are staged in
+// LDS too (when they fit beside the slices), so a pass reads no memory
+template <typename TV, typename T, typename S, typename TD, int W, bool CL>
 __global__ void __launch_bounds__ (MGC_TPB)
 mgc_k (const MgcArgs a)
 {
@@ -2266,6 +2273,23 @@ mgc_k (const MgcArgs a)
   auto vb = [&] (int l) { return (S *) (sm + a.lev[l].ob); };
   auto vx = [&] (int l) { return (S *) (sm + a.lev[l].ox); };
   auto vr = [&] (int l) { return (S *) (sm + a.lev[l].orr); };
+  // an array of the levels: its LDS copy (CL) or its memory
+  auto at = [&] (const void *g, int o) -> const void * {
+    return CL ? (const void *) (sm + o) : g;
+  };
+  if (CL) {
+    // one contiguous copy, four 16-byte loads in flight per thread
+    uint4 *dst = (uint4 *) (sm + a.img_dst);
+    for (int w = tid; w < a.img_n16; w += 4 * nt) {
+      uint4 t[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (w + u * nt < a.img_n16) t[u] = a.img[w + u * nt];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (w + u * nt < a.img_n16) dst[w + u * nt] = t[u];
+    }
+  }
   {
     const MgcLev &L = a.lev[a.top];
     const S *g = (const S *) a.Btop + pan * L.n * W + col;
@@ -2277,8 +2301,10 @@ mgc_k (const MgcArgs a)
     const int kind = a.op[q] >> 4, l = a.op[q] & 15;
     const MgcLev &L = a.lev[l];
     const int n = L.n;
+    const int *rp = (const int *) at (L.rp, L.orp);
+    const int *ci = (const int *) at (L.ci, L.oci);
     if (kind == MGC_DENSE) {
-      const TD *Ai = (const TD *) a.cinv;
+      const TD *Ai = (const TD *) at (a.cinv, a.ocinv);
       const S *B = vb (0);
       S *X = vx (0);
       for (int row = tid; row < n; row += nt) {
@@ -2298,11 +2324,14 @@ mgc_k (const MgcArgs a)
       // B(l - 1) = P^T R, rows of level l - 1
       const S *R = vr (l);
       S *Bc = vb (l - 1);
+      const int *trp = (const int *) at (L.trp, L.otrp);
+      const int *tci = (const int *) at (L.tci, L.otci);
+      const TV *tv = (const TV *) at (L.tv, L.otv);
       for (int row = tid; row < L.nc; row += nt) {
-        const int k0 = L.trp[row], k1 = L.trp[row + 1];
+        const int k0 = trp[row], k1 = trp[row + 1];
         T acc[N];
-        mgc_rowsum<TV, T, S, N> (k0, k1, k0 < k1 ? L.tci[k0] : 0, L.tci,
-                                 (const TV *) L.tv, R, plain, acc);
+        mgc_rowsum<TV, T, S, N> (k0, k1, k0 < k1 ? tci[k0] : 0, tci, tv, R,
+                                 plain, acc);
         vst (Bc + (size_t) row * N, acc);
       }
     }
@@ -2310,11 +2339,14 @@ mgc_k (const MgcArgs a)
       // X += P X(l - 1)
       const S *Xc = vx (l - 1);
       S *X = vx (l);
+      const int *prp = (const int *) at (L.prp, L.oprp);
+      const int *pci = (const int *) at (L.pci, L.opci);
+      const TV *pv = (const TV *) at (L.pv, L.opv);
       for (int row = tid; row < n; row += nt) {
-        const int k0 = L.prp[row], k1 = L.prp[row + 1];
+        const int k0 = prp[row], k1 = prp[row + 1];
         T acc[N], xo[N];
-        mgc_rowsum<TV, T, S, N> (k0, k1, k0 < k1 ? L.pci[k0] : 0, L.pci,
-                                 (const TV *) L.pv, Xc, plain, acc);
+        mgc_rowsum<TV, T, S, N> (k0, k1, k0 < k1 ? pci[k0] : 0, pci, pv, Xc,
+                                 plain, acc);
         vld (X + (size_t) row * N, xo);
 #pragma unroll
         for (int i = 0; i < N; i++) acc[i] = acc[i] + xo[i];
@@ -2324,10 +2356,11 @@ mgc_k (const MgcArgs a)
     else if (kind == MGC_RESID) {
       const S *B = vb (l), *X = vx (l);
       S *R = vr (l);
+      const TV *v = (const TV *) at (L.v, L.ov);
       for (int row = tid; row < n; row += nt) {
         T acc[N], b[N];
-        mgc_rowsum<TV, T, S, N> (L.rp[row], L.rp[row + 1], row, L.ci,
-                                 (const TV *) L.v, X, plain, acc);
+        mgc_rowsum<TV, T, S, N> (rp[row], rp[row + 1], row, ci, v, X, plain,
+                                 acc);
         vld (B + (size_t) row * N, b);
 #pragma unroll
         for (int i = 0; i < N; i++) b[i] -= acc[i];
@@ -2339,7 +2372,8 @@ mgc_k (const MgcArgs a)
       const bool xadd = kind == MGC_SMX;
       const S *R0 = xadd ? vr (l) : vb (l);
       S *X = vx (l);
-      const TV *dinv = (const TV *) L.dinv;
+      const TV *dinv = (const TV *) at (L.dinv, L.odinv);
+      const TV *vs = (const TV *) at (L.vs, L.ovs);
       const T b1 = (T) L.b1, a2 = (T) L.a2, b2 = (T) L.b2;
       const bool sc = L.sc;
       auto scaled = [&] (T w, int c) {
@@ -2347,8 +2381,8 @@ mgc_k (const MgcArgs a)
       };
       for (int row = tid; row < n; row += nt) {
         T acc[N], r0[N], x0[N], out[N];
-        mgc_rowsum<TV, T, S, N> (L.rp[row], L.rp[row + 1], row, L.ci,
-                                 (const TV *) L.vs, R0, scaled, acc);
+        mgc_rowsum<TV, T, S, N> (rp[row], rp[row + 1], row, ci, vs, R0,
+                                 scaled, acc);
         const T dr = (T) dinv[row];
         vld (R0 + (size_t) row * N, r0);
         if (xadd) vld (X + (size_t) row * N, x0);
@@ -2405,9 +2439,19 @@ mgc_supported (int nlev, const MgcLevelIn *lev, int n0)
   return mgc_lds_bytes (nlev, rows) <= MGC_MAX_LDS;
 }
 
+void
+MgcImage::release ()
+{
+  dfree (p);
+  p = nullptr;
+  bytes = 0;
+  nlev = 0;
+}
+
 int
 k_mg_coarse (int nlev, const MgcLevelIn *lev, const void *cinv, int n0,
-             int nops, const unsigned char *ops)
+             int nops, const unsigned char *ops, MgcImage *img,
+             std::mutex *img_mu)
 {
   LGH_CHECK (nops >= 1 && nops <= MGC_MAX_OPS && mgc_supported (nlev, lev, n0),
              8, "k_mg_coarse: unsupported levels");
@@ -2462,6 +2506,65 @@ k_mg_coarse (int nlev, const MgcLevelIn *lev, const void *cinv, int n0,
   a.Btop = lev[nlev - 1].B->p;
   a.Xtop = lev[nlev - 1].X->p;
   LGH_CHECK (lds <= MGC_MAX_LDS, 8, "k_mg_coarse: levels exceed the LDS bound");
+  // the CSR arrays, 1/diag and the dense inverse after the slices, each at
+  // a 16-byte boundary, when all of them fit: packed once (MgcImage) in
+  // this layout, so a launch stages them with one contiguous copy
+  bool staged = false;
+  if (img && img_mu) {
+    const int base = lds;
+    int off = base;
+    bool fit = true;
+    std::vector<std::pair<const void *, std::pair<int, size_t>>> parts;
+    auto put = [&] (const void *src, size_t bytes, int *o) {
+      off = (off + 15) / 16 * 16;
+      *o = off;
+      if (off + (int64_t) bytes > MGC_MAX_LDS) { fit = false; return; }
+      parts.push_back ({ src, { off - base, bytes } });
+      off += (int) bytes;
+    };
+    const size_t tdb = v0.prec == F64 ? 8 : 4;
+    put (cinv, (size_t) n0 * n0 * tdb, &a.ocinv);
+    for (int l = 1; l < nlev && fit; l++) {
+      const MgcLevelIn &L = lev[l];
+      MgcLev &o = a.lev[l];
+      const size_t vbs = prec_bytes (L.A->prec);
+      put (L.A->rp, 4 * ((size_t) L.A->n + 1), &o.orp);
+      put (L.A->ci, 4 * (size_t) L.A->nnz, &o.oci);
+      put (L.A->v, vbs * (size_t) L.A->nnz, &o.ov);
+      if (o.sc) put (L.A->vs, vbs * (size_t) L.A->nnz, &o.ovs);
+      else o.ovs = o.ov;
+      put (L.A->dinv, vbs * (size_t) L.A->n, &o.odinv);
+      put (L.P->rp, 4 * ((size_t) L.P->n + 1), &o.oprp);
+      put (L.P->ci, 4 * (size_t) L.P->nnz, &o.opci);
+      put (L.P->v, vbs * (size_t) L.P->nnz, &o.opv);
+      put (L.PT->rp, 4 * ((size_t) L.PT->n + 1), &o.otrp);
+      put (L.PT->ci, 4 * (size_t) L.PT->nnz, &o.otci);
+      put (L.PT->v, vbs * (size_t) L.PT->nnz, &o.otv);
+    }
+    off = (off + 15) / 16 * 16;
+    if (fit) {
+      const size_t ib = (size_t) (off - base);
+      std::lock_guard<std::mutex> lk (*img_mu);
+      if (img->nlev != nlev || img->bytes != ib) {
+        img->release ();
+        img->p = dmalloc (ib);
+        if (!img->p) return 5;
+        int rc = dzero (img->p, ib);
+        for (auto &pt : parts)
+          if (!rc) rc = dcopy ((char *) img->p + pt.second.first, pt.first, pt.second.second);
+        // the image is read by other threads' streams: complete it here
+        if (!rc) rc = dev_sync ();
+        if (rc) { img->release (); return rc; }
+        img->bytes = ib;
+        img->nlev = nlev;
+      }
+      a.img = (const uint4 *) img->p;
+      a.img_dst = base;
+      a.img_n16 = (int) (ib / 16);
+      lds = off;
+      staged = true;
+    }
+  }
   const int np = v0.npanels ();
   // one thread per row of the largest level (whole wavefronts, at most
   // MGC_TPB): the small launch fits beside other streams' work
@@ -2471,21 +2574,23 @@ k_mg_coarse (int nlev, const MgcLevelIn *lev, const void *cinv, int n0,
   ProfScope ps (PK_MG_COARSE, np * bytes, np * flops, np, v0.prec, v0.W);
   return with_row_W (v0.W, [&] (auto wc) -> int {
     constexpr int W = decltype (wc)::value;
-    if (v0.prec == F64) {
-      hipLaunchKernelGGL ((mgc_k<double, double, double, double, W>),
-                          dim3 (W / VT<double>::N, np), dim3 (tpb), lds,
-                          stream (), a);
-    }
-    else if (v0.prec == F32) {
-      hipLaunchKernelGGL ((mgc_k<float, float, float, float, W>),
-                          dim3 (W / VT<float>::N, np), dim3 (tpb), lds,
-                          stream (), a);
-    }
-    else {
-      hipLaunchKernelGGL ((mgc_k<float, float, bf16_t, float, W>),
-                          dim3 (W / VT<bf16_t>::N, np), dim3 (tpb), lds,
-                          stream (), a);
-    }
+    auto go = [&] (auto cl) {
+      constexpr bool CL = decltype (cl)::value;
+      if (v0.prec == F64)
+        hipLaunchKernelGGL ((mgc_k<double, double, double, double, W, CL>),
+                            dim3 (W / VT<double>::N, np), dim3 (tpb), lds,
+                            stream (), a);
+      else if (v0.prec == F32)
+        hipLaunchKernelGGL ((mgc_k<float, float, float, float, W, CL>),
+                            dim3 (W / VT<float>::N, np), dim3 (tpb), lds,
+                            stream (), a);
+      else
+        hipLaunchKernelGGL ((mgc_k<float, float, bf16_t, float, W, CL>),
+                            dim3 (W / VT<bf16_t>::N, np), dim3 (tpb), lds,
+                            stream (), a);
+    };
+    if (staged) go (std::true_type ());
+    else go (std::false_type ());
     LGH_LAUNCH_CHECK ();
     return 0;
   });

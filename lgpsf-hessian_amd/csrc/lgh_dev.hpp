@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -294,8 +295,20 @@ struct MgcLevelIn {
 // levels >= 1, vectors of one precision and row width, P / P^T rows the
 // per-level SpMM sums in row order)
 bool mgc_supported (int nlev, const MgcLevelIn *lev, int n0);
+// the fused levels' CSR arrays, 1/diag and dense inverse packed in the
+// launch's LDS layout (one contiguous copy per launch when they fit beside
+// the vector slices); built on first use for a depth and kept
+struct MgcImage {
+  int nlev = 0;                           // fused levels (0: not built)
+  void *p = nullptr;
+  size_t bytes = 0;
+  void release ();
+};
+// img, img_mu: the hierarchy's image and the lock that builds it (several
+// threads may run cycles on one hierarchy)
 int k_mg_coarse (int nlev, const MgcLevelIn *lev, const void *cinv, int n0,
-                 int nops, const unsigned char *ops);
+                 int nops, const unsigned char *ops, MgcImage *img,
+                 std::mutex *img_mu);
 
 // column-wise reductions: out[j] = sum_i X(i,j) Y(i,j) (fp64, ncols)
 int k_coldot (const PMat &X, const PMat &Y, double *out_dev);

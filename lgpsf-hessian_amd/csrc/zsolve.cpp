@@ -15,6 +15,7 @@ DevMG::~DevMG ()
   for (auto *a : P) delete a;
   for (auto *a : PT) delete a;
   dfree (cinv);
+  mgc_img.release ();
 }
 
 void
@@ -267,7 +268,7 @@ mg_coarse_fused (DevMG &g, DevMG::Work &w, int l, int *done)
   if (g.gamma == 2) mgc_prog (g, top, 1, ops);
   if (ops.size () > (size_t) MGC_MAX_OPS) return 0;
   LGH_TRY (k_mg_coarse (top + 1, lev, g.cinv, g.n0, (int) ops.size (),
-                        ops.data ()));
+                        ops.data (), &g.mgc_img, &g.mgc_mu));
   *done = 1;
   return 0;
 }

@@ -16,6 +16,8 @@ struct DevMG {
   std::vector<int> nu, n;
   void *cinv = nullptr;                // dense row-major inverse of A_0
   int n0 = 0;
+  MgcImage mgc_img;                    // the fused coarse launch's LDS image
+  std::mutex mgc_mu;
   // per-level workspace for one (W, npanels) shape; slot 0 holds full
   // 64-column panels, slot 1 16-wide ones, slot 2 other widths (the
   // narrow last panels of blocks), so solves that alternate between them
