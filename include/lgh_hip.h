@@ -105,6 +105,15 @@ int lgh_hip_spmm_grouped (int n, int m, const int *rp, const int *ci,
  * engine gets the grouped copy, whatever its size; rg = 0 returns to the
  * autotuner and the size threshold of 2^24 nonzeros) */
 void lgh_hip_spmm_group_config (int rg, int ppw);
+/* force the row-tile copy instead (the LDS-staged product: tiles of 64
+ * rows whose union of neighbour rows is staged in LDS chunk by chunk, the
+ * same bits): cand bits 0-1 pick the shape (rows per wavefront x
+ * wavefronts x chunk rows: 4x16x32, 4x16x64, 8x8x32, 8x8x64), bit 2 2D
+ * tiles (16 rows wide, on the matrix's line stride found from its
+ * pattern; 1D when it has none), bit 3 tiles dealt to the XCDs in
+ * contiguous runs; cand < 0 ends the forcing (the autotuner, which times
+ * every tile candidate beside the grouped ones, decides again) */
+void lgh_hip_spmm_tile_config (int cand);
 /* caps of the multigrid levels that run as one fused launch per coarse
  * visit of a cycle (the small levels above the dense solve, each with at
  * most max_rows rows and max_nnz nonzeros; negative: the defaults 1024
@@ -115,7 +124,9 @@ void lgh_hip_spmm_group_config (int rg, int ppw);
 void lgh_hip_mg_coarse_config (int max_rows, int max_nnz);
 /* the autotuner's decisions so far as text, one line per matrix shape:
  * "rows nonzeros value_prec rg ppw_fp64 ppw_fp32 ms_per_panel_fp64
- * ms_per_panel_fp32"; returns the full length (buf may be null) */
+ * ms_per_panel_fp32 tile tile_rg tile_nw tile_ch tile_tx tile_stride
+ * tile_xcd" (tile = 1: the row-tile copy won, its shape follows; the
+ * times are the winner's); returns the full length (buf may be null) */
 int lgh_hip_spmm_group_report (char *buf, int len);
 
 /* X = Chebyshev(nit) solve of A X = B with the Jacobi preconditioner on

@@ -235,6 +235,7 @@ _SIGS = {
     "lgh_hip_randn_fill_noalloc": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.c_int,
                                              C.c_int, C.c_int, C.c_void_p, C.c_int]),
     "lgh_hip_spmm_group_config": (None, [C.c_int, C.c_int]),
+    "lgh_hip_spmm_tile_config": (None, [C.c_int]),
     "lgh_hip_mg_coarse_config": (None, [C.c_int, C.c_int]),
     "lgh_hip_spmm_group_report": (C.c_int, [C.c_char_p, C.c_int]),
     "lgh_hip_cheb_jacobi": (C.c_int, [C.c_int, ip, ip, dp, C.c_int, C.c_int,
@@ -816,6 +817,12 @@ def spmm_group_config(rg=0, ppw=0):
     lib().lgh_hip_spmm_group_config(rg, ppw)
 
 
+def spmm_tile_config(cand=-1):
+    """Force the row-tile (LDS-staged) SpMM copy for later uploads: cand
+    bits 0-1 the shape, bit 2 2D tiles, bit 3 XCD-run order; -1 ends it."""
+    lib().lgh_hip_spmm_tile_config(cand)
+
+
 def mg_coarse_config(max_rows=-1, max_nnz=-1):
     """Caps of the multigrid levels that run as one fused launch per coarse
     visit (negative: defaults 1024 rows, 65536 nonzeros; max_rows = 0: one
@@ -834,7 +841,10 @@ def spmm_group_report():
         out.append({"n": int(f[0]), "nnz": int(f[1]), "value_prec": int(f[2]),
                     "rows_per_group": int(f[3]), "panels_per_wave_fp64": int(f[4]),
                     "panels_per_wave_fp32": int(f[5]), "ms_per_panel_fp64": float(f[6]),
-                    "ms_per_panel_fp32": float(f[7])})
+                    "ms_per_panel_fp32": float(f[7]), "tile": int(f[8]),
+                    "tile_rows_per_wave": int(f[9]), "tile_waves": int(f[10]),
+                    "tile_chunk": int(f[11]), "tile_tx": int(f[12]),
+                    "tile_stride": int(f[13]), "tile_xcd": int(f[14])})
     return out
 
 

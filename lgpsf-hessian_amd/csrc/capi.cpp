@@ -165,6 +165,12 @@ lgh_hip_spmm_group_config (int rg, int ppw)
 }
 
 void
+lgh_hip_spmm_tile_config (int cand)
+{
+  spmm_tile_force (cand);
+}
+
+void
 lgh_hip_mg_coarse_config (int max_rows, int max_nnz)
 {
   mg_coarse_config (max_rows, max_nnz);

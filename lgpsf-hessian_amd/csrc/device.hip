@@ -554,10 +554,18 @@ GroupCopy::release ()
   nbatch = 0;
 }
 
+void
+TileCopy::release ()
+{
+  dfree (tp); dfree (tu); dfree (cb); dfree (es); dfree (trow); dfree (ev);
+  *this = TileCopy ();
+}
+
 DevCSR::~DevCSR ()
 {
   dfree (rp); dfree (ci); dfree (v); dfree (dinv);
   grp.release ();
+  tile.release ();
   dfree (vs);
 }
 
@@ -646,6 +654,271 @@ csr_group_build (const DevCSR &A, const int *rp, const int *ci,
   out.release ();
   out = c;
   return 0;
+}
+
+// the rows of a tile of shape s (-1: none), local index e = w * rg + r.
+// 2D tiles sit on an ntx x nty grid of tiles; with s.xcd = 2 tile t is
+// launched as workgroup t, which the dispatcher hands to XCD t mod 8, and
+// XCD x walks the x-th vertical strip of the tile grid band by band, so
+// the tiles one XCD runs at a time are neighbours and share the rows they
+// stage through its L2 (positions past the grid: empty tiles)
+static inline int
+tile_row (const TileShape &s, int n, int ntx, int nty, int t, int e)
+{
+  const int T = s.rg * s.nw;
+  if (!s.tx) {
+    const int64_t r = (int64_t) t * T + e;
+    return r < n ? (int) r : -1;
+  }
+  int tcol = t % ntx, tband = t / ntx;
+  if (s.xcd == 2) {
+    const int sw = (ntx + 7) / 8, l = t / 8;
+    tcol = (t % 8) * sw + l % sw;
+    tband = l / sw;
+    if (tcol >= ntx || tband >= nty) return -1;
+  }
+  const int ty = T / s.tx, wpl = s.tx / s.rg;     // wavefronts per line
+  const int w = e / s.rg, r = e % s.rg;
+  const int x = tcol * s.tx + (w % wpl) * s.rg + r;
+  const int64_t y = (int64_t) tband * ty + w / wpl;
+  const int64_t row = y * s.stride + x;
+  return (x < s.stride && row < n) ? (int) row : -1;
+}
+
+// sorted union of the column indices of rows[0 .. nr) (skipping -1)
+static void
+rows_union (const int *rp, const int *ci, const int *rows, int nr,
+            std::vector<int> &u)
+{
+  u.clear ();
+  for (int e = 0; e < nr; e++)
+    if (rows[e] >= 0) u.insert (u.end (), ci + rp[rows[e]], ci + rp[rows[e] + 1]);
+  std::sort (u.begin (), u.end ());
+  u.erase (std::unique (u.begin (), u.end ()), u.end ());
+}
+
+// A tile's union is sorted, so a wavefront's merged rows (each in column
+// order) meet its entries chunk by chunk in column order: row r of the
+// wavefront accumulates its own entries in the order of spmm64_k, plus
+// exact zeros.  Two passes over the tiles (sizes, then fill), each split
+// over host threads.
+int
+csr_tile_build (const DevCSR &A, const int *rp, const int *ci,
+                const double *v, const TileShape &s, TileCopy &out)
+{
+  LGH_CHECK ((s.rg == 4 || s.rg == 8) && s.nw >= 1 && s.nw <= 16
+             && s.ch >= 8 && s.ch <= 256, 8, "csr_tile_build: tile shape");
+  const int T = s.rg * s.nw;
+  LGH_CHECK (!s.tx || (s.stride >= s.tx && T % s.tx == 0 && s.tx % s.rg == 0),
+             8, "csr_tile_build: 2D tile shape");
+  const int n = A.n, nb = tile_batch (s.rg), nw = s.nw, ch = s.ch, rg = s.rg;
+  for (int i = 0; i < n; i++)
+    for (int k = rp[i] + 1; k < rp[i + 1]; k++)
+      LGH_CHECK (ci[k] > ci[k - 1], 8, "csr_tile_build: unsorted row");
+  int ntx = 0, nty = 0;
+  int64_t nt64;
+  if (s.tx) {
+    const int64_t lines = ((int64_t) n + s.stride - 1) / s.stride;
+    ntx = (s.stride + s.tx - 1) / s.tx;
+    const int64_t ny = (lines + T / s.tx - 1) / (T / s.tx);
+    LGH_CHECK (ny < INT_MAX / 2, 8, "csr_tile_build: too many tiles");
+    nty = (int) ny;
+    nt64 = s.xcd == 2 ? (int64_t) 8 * ((ntx + 7) / 8) * nty : (int64_t) ntx * nty;
+  }
+  else nt64 = ((int64_t) n + T - 1) / T;
+  LGH_CHECK (nt64 < INT_MAX / 2, 8, "csr_tile_build: too many tiles");
+  const int nt = (int) nt64;
+  // per tile: chunks, and batches per (chunk, wavefront) in a flat list
+  std::vector<int> nch ((size_t) nt, 0);
+  std::vector<std::vector<int>> bcnt ((size_t) nt);
+  // one tile; fill: write its chunks' rows and its batches from the offsets
+  auto work = [&] (int t, bool fill, int *tu, int *es, double *ev,
+                   int64_t bat0) {
+    int rows[128];
+    for (int e = 0; e < T; e++) rows[e] = tile_row (s, n, ntx, nty, t, e);
+    std::vector<int> u;
+    rows_union (rp, ci, rows, T, u);
+    const int nc = (int) ((u.size () + ch - 1) / ch);
+    if (fill)
+      for (int j = 0; j < nc * ch; j++)
+        tu[j] = j < (int) u.size () ? u[j] : u.back ();
+    else {
+      nch[t] = nc;
+      bcnt[t].assign ((size_t) nc * nw, 0);
+    }
+    // per wavefront: its entries (slot, values) chunk by chunk
+    std::vector<int64_t> bpos ((size_t) nc * nw);
+    if (fill) {                       // a wavefront's batches: one run, chunk by chunk
+      int64_t b = bat0;
+      for (int w = 0; w < nw; w++)
+        for (int c = 0; c < nc; c++) { bpos[(size_t) c * nw + w] = b; b += bcnt[t][(size_t) c * nw + w]; }
+    }
+    for (int w = 0; w < nw; w++) {
+      int p[8], e[8];
+      for (int r = 0; r < rg; r++) {
+        const int i = rows[w * rg + r];
+        p[r] = i >= 0 ? rp[i] : 0;
+        e[r] = i >= 0 ? rp[i + 1] : 0;
+      }
+      int j = 0, cur = -1, inchunk = 0, last_slot = 0;
+      int64_t ent = 0;                  // entries written in the current chunk
+      auto close_chunk = [&] () {
+        if (cur < 0) return;
+        const int padded = (inchunk + nb - 1) / nb * nb;
+        if (!fill) bcnt[t][(size_t) cur * nw + w] = padded / nb;
+        else
+          for (int k = inchunk; k < padded; k++) {
+            const int64_t q = bpos[(size_t) cur * nw + w] * nb + k;
+            es[q] = last_slot;
+            for (int r = 0; r < rg; r++) ev[q * rg + r] = 0.0;
+          }
+      };
+      for (;;) {
+        int c = INT_MAX;
+        for (int r = 0; r < rg; r++) if (p[r] < e[r]) c = std::min (c, ci[p[r]]);
+        if (c == INT_MAX) break;
+        while (u[j] < c) j++;
+        const int chunk = j / ch, slot = j % ch;
+        if (chunk != cur) { close_chunk (); cur = chunk; inchunk = 0; ent = 0; }
+        if (fill) {
+          const int64_t q = bpos[(size_t) cur * nw + w] * nb + ent;
+          es[q] = slot;
+          for (int r = 0; r < rg; r++) {
+            const bool hit = p[r] < e[r] && ci[p[r]] == c;
+            ev[q * rg + r] = hit ? v[p[r]] : 0.0;
+          }
+        }
+        for (int r = 0; r < rg; r++) if (p[r] < e[r] && ci[p[r]] == c) p[r]++;
+        last_slot = slot;
+        inchunk++;
+        ent++;
+      }
+      close_chunk ();
+    }
+  };
+  const int nth = (int) std::max (1u, std::min (16u, std::thread::hardware_concurrency ()));
+  auto par = [&] (auto &&body) {
+    std::vector<std::thread> th;
+    for (int k = 0; k < nth; k++)
+      th.emplace_back ([&, k] {
+        for (int t = (int) ((int64_t) nt * k / nth); t < (int) ((int64_t) nt * (k + 1) / nth); t++)
+          body (t);
+      });
+    for (auto &x : th) x.join ();
+  };
+  par ([&] (int t) { work (t, false, nullptr, nullptr, nullptr, 0); });
+  std::vector<int> tp ((size_t) nt + 1, 0);
+  std::vector<int64_t> boff ((size_t) nt + 1, 0);
+  for (int t = 0; t < nt; t++) {
+    LGH_CHECK ((int64_t) tp[t] + nch[t] < INT_MAX / 2, 8, "csr_tile_build: too many chunks");
+    tp[t + 1] = tp[t] + nch[t];
+    int64_t b = 0;
+    for (int x : bcnt[t]) b += x;
+    boff[t + 1] = boff[t] + b;
+  }
+  const int64_t nchunk = tp[nt], nbatch = boff[nt];
+  LGH_CHECK (nbatch < INT_MAX, 8, "csr_tile_build: too many batches");
+  // (first, end) batch of (chunk, wavefront); a wavefront's batches of
+  // one tile are one run (wavefront-major, then chunk)
+  std::vector<int> cb ((size_t) nchunk * nw * 2, 0);
+  for (int t = 0; t < nt; t++) {
+    int64_t b = boff[t];
+    for (int w = 0; w < nw; w++)
+      for (int c = 0; c < nch[t]; c++) {
+        const size_t k = ((size_t) (tp[t] + c) * nw + w) * 2;
+        cb[k] = (int) b;
+        b += bcnt[t][(size_t) c * nw + w];
+        cb[k + 1] = (int) b;
+      }
+  }
+  if (cb.empty ()) cb.assign (2, 0);
+  std::vector<int> tu ((size_t) std::max<int64_t> (nchunk * ch, 1));
+  std::vector<int> es ((size_t) std::max<int64_t> (nbatch * nb, 1));
+  std::vector<double> ev ((size_t) std::max<int64_t> (nbatch * nb * rg, 1));
+  par ([&] (int t) {
+    work (t, true, tu.data () + (size_t) tp[t] * ch, es.data (), ev.data (), boff[t]);
+  });
+  std::vector<int> trow;
+  if (s.tx) {
+    trow.resize ((size_t) nt * T);
+    for (int t = 0; t < nt; t++)
+      for (int e = 0; e < T; e++) trow[(size_t) t * T + e] = tile_row (s, n, ntx, nty, t, e);
+  }
+  const size_t vb = prec_bytes (A.prec);
+  TileCopy c;
+  c.s = s;
+  c.ntiles = nt;
+  c.nchunk = nchunk;
+  c.nbatch = nbatch;
+  c.union_per_row = n ? (double) nchunk * ch / n : 0.0;
+  c.tp = (int *) dmalloc (sizeof (int) * tp.size ());
+  c.tu = (int *) dmalloc (sizeof (int) * tu.size ());
+  c.cb = (int *) dmalloc (sizeof (int) * cb.size ());
+  c.es = (int *) dmalloc (sizeof (int) * es.size ());
+  c.ev = dmalloc (vb * ev.size ());
+  if (s.tx) c.trow = (int *) dmalloc (sizeof (int) * trow.size ());
+  if (!c.tp || !c.tu || !c.cb || !c.es || !c.ev || (s.tx && !c.trow)) { c.release (); return 5; }
+  int rc = h2d (c.tp, tp.data (), sizeof (int) * tp.size ());
+  if (!rc) rc = h2d (c.tu, tu.data (), sizeof (int) * tu.size ());
+  if (!rc) rc = h2d (c.cb, cb.data (), sizeof (int) * cb.size ());
+  if (!rc) rc = h2d (c.es, es.data (), sizeof (int) * es.size ());
+  if (!rc && s.tx) rc = h2d (c.trow, trow.data (), sizeof (int) * trow.size ());
+  if (!rc) {
+    if (A.prec == F64) rc = h2d (c.ev, ev.data (), sizeof (double) * ev.size ());
+    else {
+      std::vector<float> f (ev.begin (), ev.end ());
+      rc = h2d (c.ev, f.data (), sizeof (float) * f.size ());
+    }
+  }
+  if (rc) { c.release (); return rc; }
+  out.release ();
+  out = c;
+  return 0;
+}
+
+int
+csr_tile_stride (int n, const int *rp, const int *ci, int tx, int rows)
+{
+  if (n < 4 * rows || rows % tx) return 0;
+  const int ty = rows / tx;
+  const int i0 = n / 2;
+  std::vector<int> cand;
+  for (int k = rp[i0]; k < rp[i0 + 1]; k++) {
+    const int64_t d = (int64_t) ci[k] - i0;
+    if (d > tx && d * ty < n / 2) cand.push_back ((int) d);
+  }
+  std::sort (cand.begin (), cand.end ());
+  cand.erase (std::unique (cand.begin (), cand.end ()), cand.end ());
+  if (cand.size () > 512) cand.resize (512);
+  // union per row of a few sample tiles (rows r0 + a + s * b)
+  std::vector<int> u, rr ((size_t) rows);
+  auto cost = [&] (int s) -> double {
+    double tot = 0.0;
+    int cnt = 0;
+    for (int q = 1; q <= 4; q++) {
+      const int64_t r0 = (int64_t) n * q / 5;
+      bool ok = true;
+      for (int b = 0; b < ty && ok; b++)
+        for (int a = 0; a < tx; a++) {
+          const int64_t r = s ? r0 + a + (int64_t) s * b : r0 + (int64_t) b * tx + a;
+          if (r >= n) { ok = false; break; }
+          rr[(size_t) b * tx + a] = (int) r;
+        }
+      if (!ok) continue;
+      rows_union (rp, ci, rr.data (), rows, u);
+      tot += (double) u.size () / rows;
+      cnt++;
+    }
+    return cnt ? tot / cnt : 1e300;
+  };
+  const double c1 = cost (0);
+  int best = 0;
+  double bc = 0.9 * c1;
+  for (int s : cand) {
+    const double c = cost (s);
+    if (c < bc) { bc = c; best = s; }
+  }
+  return best;
 }
 
 // 1 / diagonal of a square host CSR (duplicates summed)

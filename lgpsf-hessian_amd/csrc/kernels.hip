@@ -834,6 +834,181 @@ spmm_grp_launch (const DevCSR &A, int ppw, int p0, int np, T alpha,
   return 8;
 }
 
+// Y = alpha A X + beta Y over the row-tile copy of A (DevCSR::tile), one
+// workgroup of NW wavefronts per tile of NW * RG rows and one 16-byte piece
+// of a panel row per lane (2 FP64 / 4 FP32 panels a launch row, so a staged
+// row is 1 KB).  The tile's union of neighbour rows passes through LDS in
+// chunks of CH rows, double-buffered; one barrier a chunk.  Chunk c + 1 is
+// written to the free buffer after the products of chunk c, from registers
+// loaded two chunks earlier (its loads have two chunks of products to
+// land); the same loads bring the wavefront's values of that chunk into L2
+// (one lane a 128-byte line), so that its scalar loads later hit there.  A
+// neighbour row that many rows of the tile share crosses L1 once per tile,
+// not once per row group.  Each wavefront walks its own rows' union, a
+// batch's LDS slots and 16 values by wide scalar loads.  Row r of a
+// wavefront accumulates its own entries in column order (chunks in order,
+// slots in order), plus exact zeros: the bits of spmm64_k.  xcd = 1: tiles
+// dealt to the XCDs (dispatch order b mod 8) in contiguous runs (2D strips
+// per XCD are laid out by the tile builder itself).
+template <typename TV, typename T, typename S, int RG, int NW, int CH>
+__global__ void __launch_bounds__ (NW * 64)
+spmm_tile_k (int n, int nx, int ntiles, int xcd, int p0, int pend,
+             const int *__restrict__ tp, const int *__restrict__ tu,
+             const int *__restrict__ cb, const int *__restrict__ es,
+             const TV *__restrict__ ev, const int *__restrict__ trow, T alpha,
+             const S *__restrict__ X, T beta, S *__restrict__ Y,
+             int *__restrict__ touch_sum)
+{
+  constexpr int NB = 16 / RG;                      // entries a batch
+  constexpr int NV = NB * RG;                      // values a batch
+  constexpr int VPL = 16 / (int) sizeof (S);       // columns per lane = panels a launch row
+  constexpr int LPR = 64 / VPL;                    // lanes per panel row
+  constexpr int ROWB = 1024;                       // bytes of a staged row
+  constexpr int SPW = CH / NW;                     // staged rows per wavefront and chunk
+  static_assert (CH % NW == 0, "chunk rows: a multiple of the wavefronts");
+  typedef S V __attribute__ ((ext_vector_type (VPL)));
+  __shared__ __attribute__ ((aligned (16))) unsigned char lds[2][CH * ROWB];
+  int t = (int) blockIdx.x;
+  if (xcd == 1) {
+    const int per = (ntiles + 7) / 8;
+    t = (t % 8) * per + t / 8;
+  }
+  if (t >= ntiles) return;                         // whole workgroup
+  const int wave = __builtin_amdgcn_readfirstlane ((int) (threadIdx.x >> 6));
+  const int lane = (int) threadIdx.x & 63;
+  const int pb = p0 + VPL * (int) blockIdx.y;
+  const int want = pb + lane / LPR;
+  const int pq = want < pend ? want : pend - 1;
+  const int c0 = (lane % LPR) * VPL;
+  const S *Xl = X + (size_t) pq * nx * 64 + c0;
+  T acc[RG][VPL];
+#pragma unroll
+  for (int r = 0; r < RG; r++)
+#pragma unroll
+    for (int v = 0; v < VPL; v++) acc[r][v] = 0;
+  const int cs = tp[t], ce = tp[t + 1];
+  int sink = 0;
+  if (cs < ce) {
+    V s0[SPW], s1[SPW];
+    auto fetch = [&] (int c, V (&st)[SPW]) {
+      const int *u = tu + (size_t) c * CH + wave;
+#pragma unroll
+      for (int j = 0; j < SPW; j++) st[j] = *(const V *) (Xl + (size_t) u[j * NW] * 64);
+      const int q0 = cb[((size_t) c * NW + wave) * 2], q1 = cb[((size_t) c * NW + wave) * 2 + 1];
+      const size_t b0 = (size_t) q0 * NV * sizeof (TV), b1 = (size_t) q1 * NV * sizeof (TV);
+      const size_t at = (b0 & ~(size_t) 127) + (size_t) lane * 128;
+      if (at < b1) sink += *(const int *) ((const char *) ev + at);
+    };
+    auto put = [&] (const V (&st)[SPW], int buf) {
+#pragma unroll
+      for (int j = 0; j < SPW; j++)
+        *(V *) (&lds[buf][(j * NW + wave) * ROWB + lane * 16]) = st[j];
+    };
+    auto products = [&] (int c, int buf) {
+      const int q0 = cb[((size_t) c * NW + wave) * 2], q1 = cb[((size_t) c * NW + wave) * 2 + 1];
+      const unsigned char *base = &lds[buf][lane * 16];
+      for (int q = q0; q < q1; q++) {
+        const int *sl = es + (size_t) q * NB;
+        const TV *wv = ev + (size_t) q * NV;
+        V x[NB];
+#pragma unroll
+        for (int e = 0; e < NB; e++) x[e] = *(const V *) (base + sl[e] * ROWB);
+#pragma unroll
+        for (int e = 0; e < NB; e++)
+#pragma unroll
+          for (int r = 0; r < RG; r++) {
+            const T w = (T) wv[e * RG + r];
+#pragma unroll
+            for (int v = 0; v < VPL; v++) acc[r][v] += w * (T) x[e][v];
+          }
+      }
+    };
+    // one chunk: barrier (chunk c staged, the other buffer free), its
+    // products, stage chunk c + 1 from st, refill st with chunk c + 3
+    // (unconditionally: past the tile's end its last chunk again, so the
+    // loads in flight are the same on every path and the wait before a
+    // staging write covers only the set it writes)
+    auto step = [&] (int c, int buf, V (&st)[SPW]) {
+      __syncthreads ();
+      products (c, buf);
+      __builtin_amdgcn_sched_barrier (0);
+      if (c + 1 < ce) put (st, buf ^ 1);
+      fetch (min (c + 3, ce - 1), st);
+      __builtin_amdgcn_sched_barrier (0);
+    };
+    fetch (cs, s0);
+    put (s0, 0);
+    fetch (min (cs + 1, ce - 1), s1);
+    fetch (min (cs + 2, ce - 1), s0);
+    for (int c = cs; c < ce; c += 2) {
+      step (c, 0, s1);
+      if (c + 1 >= ce) break;
+      step (c + 1, 1, s0);
+    }
+  }
+  // touch_sum (null from the launcher) takes the L2 touches' sum, so the
+  // touch loads are not dropped as unused
+  if (touch_sum) atomicAdd (touch_sum, sink);
+  if (want >= pend) return;
+#pragma unroll
+  for (int r = 0; r < RG; r++) {
+    const int e = wave * RG + r;
+    const int row = trow ? trow[(size_t) t * (NW * RG) + e]
+                         : ((int64_t) t * (NW * RG) + e < n ? t * (NW * RG) + e : -1);
+    if (row < 0) continue;
+    V *y = (V *) (Y + ((size_t) pq * n + row) * 64 + c0);
+    V o;
+    if (beta == (T) 0) {
+#pragma unroll
+      for (int v = 0; v < VPL; v++) o[v] = (S) (alpha * acc[r][v]);
+    }
+    else {
+      const V yo = *y;
+#pragma unroll
+      for (int v = 0; v < VPL; v++) o[v] = (S) (alpha * acc[r][v] + beta * (T) yo[v]);
+    }
+    *y = o;
+  }
+}
+
+// the tile shapes the autotuner times (rows per wavefront, wavefronts,
+// chunk rows); 2D variants of each are added when the matrix has a line
+// stride (csr_tile_stride)
+struct TileCand { int rg, nw, ch; };
+static const TileCand k_tile_cands[] = {
+  { 4, 16, 32 }, { 4, 16, 64 }, { 8, 8, 32 }, { 8, 8, 64 },
+};
+
+// panels [p0, p0 + np) of Y = alpha A X + beta Y through the tile copy
+template <typename TV, typename T, typename S>
+static int
+spmm_tile_launch (const DevCSR &A, int p0, int np, T alpha, const S *X,
+                  T beta, S *Y)
+{
+  constexpr int VPL = 16 / (int) sizeof (S);
+  const TileCopy &c = A.tile;
+  if (np <= 0) return 0;
+  const unsigned gx = (unsigned) (c.s.xcd == 1 ? (c.ntiles + 7) / 8 * 8 : c.ntiles);
+  const unsigned gy = (unsigned) ((np + VPL - 1) / VPL);
+  auto go = [&] (auto rgc, auto nwc, auto chc) -> int {
+    constexpr int RG = decltype (rgc)::value, NW = decltype (nwc)::value,
+                  CH = decltype (chc)::value;
+    hipLaunchKernelGGL ((spmm_tile_k<TV, T, S, RG, NW, CH>), dim3 (gx, gy),
+                        dim3 (NW * 64), 0, stream (), A.n, A.m, c.ntiles,
+                        c.s.xcd, p0, p0 + np, c.tp, c.tu, c.cb, c.es,
+                        (const TV *) c.ev, c.trow, alpha, X, beta, Y, (int *) nullptr);
+    LGH_LAUNCH_CHECK ();
+    return 0;
+  };
+  const TileShape &s = c.s;
+  if (s.rg == 4 && s.nw == 16 && s.ch == 32) return go (IC<4> (), IC<16> (), IC<32> ());
+  if (s.rg == 4 && s.nw == 16 && s.ch == 64) return go (IC<4> (), IC<16> (), IC<64> ());
+  if (s.rg == 8 && s.nw == 8 && s.ch == 32) return go (IC<8> (), IC<8> (), IC<32> ());
+  if (s.rg == 8 && s.nw == 8 && s.ch == 64) return go (IC<8> (), IC<8> (), IC<64> ());
+  set_error ("spmm_tile_launch: tile shape not compiled");
+  return 8;
+}
+
 // ---- Cholesky factor and triangular inverse (FP64, deterministic) ---------
 // The CholQR2 steps' potrf / trtri (glr_common_impl.h:387, 402;
 // glr_scalapack_impl.h:128-143) in 64 x 64 blocks: each block is factored,
@@ -1108,10 +1283,14 @@ k_trtri (int upper, int n, double *A, int lda, int *info_dev)
 // copy and time nothing.
 
 namespace {
-struct GrpTune { int rg = 0, ppw[2] = { 0, 0 }; double ms[2] = { 0, 0 }; };
+struct GrpTune {
+  int rg = 0, ppw[2] = { 0, 0 }; double ms[2] = { 0, 0 };
+  int tile = 0; TileShape ts;          // tile: the row-tile copy won
+};
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int64_t, int>, GrpTune> g_tune;
 int g_force_rg = 0, g_force_ppw = 0;
+int g_force_tile = -1;
 }
 
 void
@@ -1120,13 +1299,22 @@ spmm_group_force (int rg, int ppw)
   std::lock_guard<std::mutex> lk (g_tune_mu);
   g_force_rg = rg;
   g_force_ppw = ppw;
+  if (rg) g_force_tile = -1;
+}
+
+void
+spmm_tile_force (int cand)
+{
+  std::lock_guard<std::mutex> lk (g_tune_mu);
+  g_force_tile = cand;
+  if (cand >= 0) g_force_rg = 0;
 }
 
 bool
 spmm_group_forced ()
 {
   std::lock_guard<std::mutex> lk (g_tune_mu);
-  return g_force_rg != 0;
+  return g_force_rg != 0 || g_force_tile >= 0;
 }
 
 std::string
@@ -1134,38 +1322,36 @@ spmm_group_report ()
 {
   std::lock_guard<std::mutex> lk (g_tune_mu);
   std::string r;
-  char buf[256];
+  char buf[320];
   for (const auto &kv : g_tune) {
-    snprintf (buf, sizeof buf, "%d %lld %d %d %d %d %.4f %.4f\n",
+    const GrpTune &t = kv.second;
+    snprintf (buf, sizeof buf, "%d %lld %d %d %d %d %.4f %.4f %d %d %d %d %d %d %d\n",
               std::get<0> (kv.first), (long long) std::get<1> (kv.first),
-              std::get<2> (kv.first), kv.second.rg, kv.second.ppw[0],
-              kv.second.ppw[1], kv.second.ms[0], kv.second.ms[1]);
+              std::get<2> (kv.first), t.rg, t.ppw[0], t.ppw[1], t.ms[0], t.ms[1],
+              t.tile, t.ts.rg, t.ts.nw, t.ts.ch, t.ts.tx, t.ts.stride, t.ts.xcd);
     r += buf;
   }
   return r;
 }
 
-// milliseconds per panel of the grouped product with ppw panels a wave
-template <typename TV, typename T>
+// milliseconds per panel of a product (after one warm-up, best of 3)
+template <typename L>
 static int
-grp_time (const DevCSR &A, int ppw, const PMat &X, PMat &Y, double *ms)
+time_panels (int np, L &&launch, double *ms)
 {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   LGH_HIP (hipEventCreate (&e0));
   if (hipEventCreate (&e1) != hipSuccess) { (void) hipEventDestroy (e0); return 2; }
-  const int np = X.npanels ();
   float best = 0.f;
-  int rc = spmm_grp_launch<TV, T, T> (A, ppw, 0, np, (T) 1, (const T *) X.p,
-                                      (T) 0, (T *) Y.p);
+  int rc = launch ();
   for (int rep = 0; rep < 3 && !rc; rep++) {
     float t = 0.f;
     if (hipEventRecord (e0, stream ()) != hipSuccess) rc = 2;
-    if (!rc) rc = spmm_grp_launch<TV, T, T> (A, ppw, 0, np, (T) 1, (const T *) X.p,
-                                             (T) 0, (T *) Y.p);
+    if (!rc) rc = launch ();
     if (!rc && (hipEventRecord (e1, stream ()) != hipSuccess
                 || hipEventSynchronize (e1) != hipSuccess
                 || hipEventElapsedTime (&t, e0, e1) != hipSuccess)) {
-      set_error ("grouped SpMM timing failed");
+      set_error ("SpMM timing failed");
       rc = 2;
     }
     if (!rc && (rep == 0 || t < best)) best = t;
@@ -1174,6 +1360,28 @@ grp_time (const DevCSR &A, int ppw, const PMat &X, PMat &Y, double *ms)
   (void) hipEventDestroy (e1);
   *ms = (double) best / np;
   return rc;
+}
+
+template <typename TV, typename T>
+static int
+grp_time (const DevCSR &A, int ppw, const PMat &X, PMat &Y, double *ms)
+{
+  const int np = X.npanels ();
+  return time_panels (np, [&] {
+    return spmm_grp_launch<TV, T, T> (A, ppw, 0, np, (T) 1, (const T *) X.p,
+                                      (T) 0, (T *) Y.p);
+  }, ms);
+}
+
+template <typename TV, typename T>
+static int
+tile_time (const DevCSR &A, const PMat &X, PMat &Y, double *ms)
+{
+  const int np = X.npanels ();
+  return time_panels (np, [&] {
+    return spmm_tile_launch<TV, T, T> (A, 0, np, (T) 1, (const T *) X.p,
+                                       (T) 0, (T *) Y.p);
+  }, ms);
 }
 
 template <typename TV>
@@ -1194,6 +1402,33 @@ grp_tune_one (const DevCSR &A, PMat (&X)[2], PMat (&Y)[2], GrpTune &t)
   return 0;
 }
 
+template <typename TV>
+static int
+tile_tune_one (const DevCSR &A, PMat (&X)[2], PMat (&Y)[2], double (&ms)[2])
+{
+  int rc = tile_time<TV, double> (A, X[0], Y[0], &ms[0]);
+  if (!rc) rc = tile_time<TV, float> (A, X[1], Y[1], &ms[1]);
+  return rc;
+}
+
+// the tile shape of forced candidate k: k_tile_cands[k & 3], 2D (the
+// matrix's line stride, 1D when it has none) with bit 4, XCD runs with bit 8
+static TileShape
+tile_shape_of (int k, int n, const int *rp, const int *ci)
+{
+  const TileCand &c = k_tile_cands[k & 3];
+  TileShape s;
+  s.rg = c.rg; s.nw = c.nw; s.ch = c.ch;
+  s.xcd = (k & 8) ? 1 : 0;
+  if (k & 4) {
+    const int rows = c.rg * c.nw, tx = 16;
+    s.stride = csr_tile_stride (n, rp, ci, tx, rows);
+    if (s.stride) s.tx = tx;
+    if (s.tx && (k & 8)) s.xcd = 2;     // 2D: strips of the tile grid per XCD
+  }
+  return s;
+}
+
 int
 csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v)
 {
@@ -1201,21 +1436,28 @@ csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v)
     for (int k = rp[i] + 1; k < rp[i + 1]; k++)
       if (ci[k] <= ci[k - 1]) return 0;           // unsorted: no copy
   const auto key = std::make_tuple (A.n, A.nnz, A.prec);
-  int frg, fppw;
+  int frg, fppw, ftile;
   GrpTune known;
   {
     std::lock_guard<std::mutex> lk (g_tune_mu);
     frg = g_force_rg;
     fppw = g_force_ppw;
+    ftile = g_force_tile;
     auto it = g_tune.find (key);
     if (it != g_tune.end ()) known = it->second;
   }
+  if (ftile >= 0) {
+    A.grp.release ();
+    return csr_tile_build (A, rp, ci, v, tile_shape_of (ftile, A.n, rp, ci), A.tile);
+  }
   if (frg) {
+    A.tile.release ();
     LGH_TRY (csr_group_build (A, rp, ci, v, frg, A.grp));
     A.grp_ppw[0] = fppw == 2 ? 2 : 4;
     A.grp_ppw[1] = 2 * A.grp_ppw[0];
     return 0;
   }
+  if (known.tile) return csr_tile_build (A, rp, ci, v, known.ts, A.tile);
   if (known.rg) {
     LGH_TRY (csr_group_build (A, rp, ci, v, known.rg, A.grp));
     A.grp_ppw[0] = known.ppw[0];
@@ -1230,6 +1472,9 @@ csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v)
   }
   GrpTune best;
   double best_cost = 0.0;
+  auto cost_of = [&] (const double (&ms)[2]) {
+    return A.prec == F64 ? ms[0] : ms[0] + ms[1];
+  };
   for (int rg : { 4, 8 }) {
     if (rc) break;
     DevCSR C;                                    // candidate: the copy only
@@ -1239,7 +1484,7 @@ csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v)
     t.rg = rg;
     if (!rc) rc = A.prec == F64 ? grp_tune_one<double> (C, X, Y, t)
                                 : grp_tune_one<float> (C, X, Y, t);
-    const double cost = A.prec == F64 ? t.ms[0] : t.ms[0] + t.ms[1];
+    const double cost = cost_of (t.ms);
     if (!rc && (best.rg == 0 || cost < best_cost)) {
       best = t;
       best_cost = cost;
@@ -1248,10 +1493,40 @@ csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v)
       C.grp = GroupCopy ();                      // now owned by A
     }
   }
+  // the row-tile copy: every shape, 1D and (with a line stride) 2D, tiles
+  // in plain and in XCD-run order
+  int stride = -1;
+  for (int k = 0; k < 16 && !rc; k++) {
+    if ((k & 4) && stride == 0) continue;
+    TileShape s = tile_shape_of (k, A.n, rp, ci);
+    if (k & 4) {
+      stride = s.stride;
+      if (!stride) continue;
+    }
+    DevCSR C;
+    C.n = A.n; C.m = A.m; C.nnz = A.nnz; C.prec = A.prec;
+    rc = csr_tile_build (A, rp, ci, v, s, C.tile);
+    double ms[2] = { 0, 0 };
+    if (!rc) rc = A.prec == F64 ? tile_tune_one<double> (C, X, Y, ms)
+                                : tile_tune_one<float> (C, X, Y, ms);
+    if (!rc && cost_of (ms) < best_cost) {
+      best.tile = 1;
+      best.ts = s;
+      best.ms[0] = ms[0];
+      best.ms[1] = ms[1];
+      best_cost = cost_of (ms);
+      A.tile.release ();
+      A.tile = C.tile;
+      C.tile = TileCopy ();
+    }
+  }
   for (int vp = 0; vp < 2; vp++) { pmat_free (X[vp]); pmat_free (Y[vp]); }
-  if (rc) { A.grp.release (); return rc; }
-  A.grp_ppw[0] = best.ppw[0];
-  A.grp_ppw[1] = best.ppw[1];
+  if (rc) { A.grp.release (); A.tile.release (); return rc; }
+  if (best.tile) A.grp.release ();
+  else {
+    A.grp_ppw[0] = best.ppw[0];
+    A.grp_ppw[1] = best.ppw[1];
+  }
   std::lock_guard<std::mutex> lk (g_tune_mu);
   g_tune[key] = best;
   return 0;
@@ -1849,6 +2124,9 @@ static int
 spmm_t (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
 {
   if constexpr (std::is_same<S, T>::value) {
+    if (A.tile.tp && X.W == 64)
+      return spmm_tile_launch<TV, T, S> (A, 0, X.npanels (), (T) alpha,
+                                         (const S *) X.p, (T) beta, (S *) Y.p);
     if (A.grp.gp && X.W == 64)
       return spmm_grp_launch<TV, T, S> (A, A.grp_ppw[std::is_same<T, double>::value ? 0 : 1],
                                         0, X.npanels (), (T) alpha,

@@ -154,6 +154,34 @@ struct GroupCopy {
   void release ();
 };
 
+// Row-tile copy of a wide-row matrix for the LDS-staged SpMM
+// (spmm_tile_k).  A tile is nw * rg rows (consecutive rows, or a tx-wide
+// block of consecutive rows on successive lines `stride` rows apart, row
+// list trow); wavefront w of the tile's workgroup owns rows w*rg .. +rg of
+// it.  The tile's sorted union of column indices is cut into chunks of ch
+// entries (the last one padded with its last index); chunk c stages the X
+// rows tu[c*ch .. + ch) in LDS.  Wavefront w's own union (the merge of its
+// rows) restricted to chunk c is batches cb[c*nw + w] .. cb[c*nw + w + 1];
+// batch b has tile_batch(rg) entries, entry u at LDS slot es[b*nb + u] with
+// rg values ev[(b*nb + u)*rg + r] (0 where row r has no entry; padding
+// entries: value 0).  Tiles tp[t] .. tp[t+1] in chunks.
+constexpr int tile_batch (int rg) { return 16 / rg; }
+struct TileShape {
+  int rg = 0, nw = 0, ch = 0;     // rows per wavefront, wavefronts, chunk
+  int tx = 0, stride = 0;         // 0: consecutive rows; else a tx x (nw*rg/tx) block
+  int xcd = 0;                    // 1: tiles dealt to XCDs in contiguous runs
+};
+struct TileCopy {
+  TileShape s;
+  int ntiles = 0;
+  int *tp = nullptr, *tu = nullptr, *cb = nullptr, *es = nullptr;
+  int *trow = nullptr;            // null: consecutive rows
+  void *ev = nullptr;             // values, precision of the owning matrix
+  int64_t nchunk = 0, nbatch = 0;
+  double union_per_row = 0.0;     // staged X rows per matrix row
+  void release ();
+};
+
 struct DevCSR {
   int n = 0, m = 0;               // rows, cols
   int64_t nnz = 0;
@@ -165,6 +193,9 @@ struct DevCSR {
   // SpMM uses with FP64 / FP32 vectors (chosen by measurement)
   GroupCopy grp;
   int grp_ppw[2] = { 4, 8 };
+  // row-tile copy (chosen by the same autotuner when it is faster; then
+  // the grouped copy is not kept)
+  TileCopy tile;
   // scaled values (csr_scale_cols): vs[k] = v[k] * (vs_b1 * dinv[ci[k]]),
   // each product rounded in precision `prec`
   void *vs = nullptr;
@@ -178,6 +209,13 @@ int csr_upload (int n, int m, const int *rp, const int *ci, const double *v,
 // with strictly increasing column indices) into out
 int csr_group_build (const DevCSR &A, const int *rp, const int *ci,
                      const double *v, int rg, GroupCopy &out);
+// build the row-tile copy of shape s (same host CSR, sorted rows) into out
+int csr_tile_build (const DevCSR &A, const int *rp, const int *ci,
+                    const double *v, const TileShape &s, TileCopy &out);
+// the line stride of a grid-ordered matrix for 2D tiles of width tx: the
+// offset, among the column offsets of a middle row, whose tx x (rows/tx)
+// tiles have the smallest union per row (0: none smaller than 1D tiles)
+int csr_tile_stride (int n, const int *rp, const int *ci, int tx, int rows);
 // autotuner: build the row-grouped copies (rg 4 and 8), time their SpMM on
 // the device for each panels-per-wave choice and vector precision, keep
 // the fastest; decisions are cached per (rows, nonzeros, value precision).
@@ -188,6 +226,10 @@ int csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v);
 // twice that with FP32 (rg 0: autotune); the tests run every candidate
 // through it
 void spmm_group_force (int rg, int ppw);
+// force the row-tile copy for later csr_group_rows calls: candidate cand
+// (bits 0-1: shape, bit 2: 2D tiles over the matrix's line stride, bit 3:
+// tiles in XCD runs); cand < 0: no tile forced
+void spmm_tile_force (int cand);
 // a configuration is forced (wide-row matrices of any size then get the copy)
 bool spmm_group_forced ();
 // the autotuner's decisions so far, one "n nnz vprec rg ppw64 ppw32 ms64"

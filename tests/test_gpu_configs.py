@@ -74,8 +74,16 @@ def test_aniso_psf_build_matches_oracle(dev, oracle, aniso, mixed):
             gg = dev.GLR(dev.Mat.aij(aniso["B"]), aniso["prior"], opts, x=x)
             assert np.array_equal(gg.lam, g.lam), (rg, ppw)
             assert np.array_equal(gg.U, g.U), (rg, ppw)
+        # and through the row-tile copy (1D 4 x 16 rows; 2D 16 x 4 rows on
+        # the grid's line stride, in XCD-run order)
+        for cand in (0, 4 | 8 | 3):
+            dev.spmm_tile_config(cand)
+            gg = dev.GLR(dev.Mat.aij(aniso["B"]), aniso["prior"], opts, x=x)
+            assert np.array_equal(gg.lam, g.lam), cand
+            assert np.array_equal(gg.U, g.U), cand
     finally:
         dev.spmm_group_config(0, 0)
+        dev.spmm_tile_config(-1)
     assert g.kept == og.kept and g.kept >= 10
     tol = 1e-7 if mixed else 1e-9
     assert np.max(np.abs(g.lam - og.lam) / np.abs(og.lam)) < tol

@@ -104,10 +104,20 @@ def test_spmm_wide_rows_width_invariant(dev, prec_vals):
             assert np.array_equal(got, y64), (rg, ppw)
             b32 = dev.spmm(B, X, prec_vals=1, prec_vec=dev.PREC_F32, W=64, grouped=True)
             assert np.array_equal(a32, b32), (rg, ppw)
+        # the row-tile copy (LDS-staged union of 64 rows): every shape, 1D
+        # and 2D tiles (the 45-wide grid's line stride; partial tiles at
+        # the grid's right edge and bottom), plain and XCD-run tile order
+        for cand in range(16):
+            dev.spmm_tile_config(cand)
+            got = dev.spmm(B, X, prec_vals=prec_vals, W=64, grouped=True)
+            assert np.array_equal(got, y64), cand
+            b32 = dev.spmm(B, X, prec_vals=1, prec_vec=dev.PREC_F32, W=64, grouped=True)
+            assert np.array_equal(a32, b32), cand
     finally:
         dev.spmm_group_config(0, 0)
+        dev.spmm_tile_config(-1)
     rep = [r for r in dev.spmm_group_report() if r["n"] == B.shape[0]]
-    assert rep and rep[0]["rows_per_group"] in (4, 8)
+    assert rep and (rep[0]["rows_per_group"] in (4, 8) or rep[0]["tile"] == 1)
 
 
 @pytest.mark.parametrize("prec", [0, 1])
