@@ -677,10 +677,11 @@ tile_row (const TileShape &s, int n, int ntx, int nty, int t, int e)
     tband = l / sw;
     if (tcol >= ntx || tband >= nty) return -1;
   }
-  const int ty = T / s.tx, wpl = s.tx / s.rg;     // wavefronts per line
+  // wavefront w: a (rg/wy) x wy block of the tile, wpl blocks across
+  const int ty = T / s.tx, wx = s.rg / s.wy, wpl = s.tx / wx;
   const int w = e / s.rg, r = e % s.rg;
-  const int x = tcol * s.tx + (w % wpl) * s.rg + r;
-  const int64_t y = (int64_t) tband * ty + w / wpl;
+  const int x = tcol * s.tx + (w % wpl) * wx + r % wx;
+  const int64_t y = (int64_t) tband * ty + (int64_t) (w / wpl) * s.wy + r / wx;
   const int64_t row = y * s.stride + x;
   return (x < s.stride && row < n) ? (int) row : -1;
 }
@@ -706,10 +707,12 @@ int
 csr_tile_build (const DevCSR &A, const int *rp, const int *ci,
                 const double *v, const TileShape &s, TileCopy &out)
 {
-  LGH_CHECK ((s.rg == 4 || s.rg == 8) && s.nw >= 1 && s.nw <= 16
-             && s.ch >= 8 && s.ch <= 256, 8, "csr_tile_build: tile shape");
+  LGH_CHECK ((s.rg == 4 || s.rg == 8 || s.rg == 16) && s.nw >= 1
+             && s.rg * s.nw <= 128 && s.ch >= 8 && s.ch <= 256 && s.wy >= 1
+             && s.rg % s.wy == 0, 8, "csr_tile_build: tile shape");
   const int T = s.rg * s.nw;
-  LGH_CHECK (!s.tx || (s.stride >= s.tx && T % s.tx == 0 && s.tx % s.rg == 0),
+  LGH_CHECK (!s.tx || (s.stride >= s.tx && T % s.tx == 0 && s.tx % (s.rg / s.wy) == 0
+                       && (T / s.tx) % s.wy == 0),
              8, "csr_tile_build: 2D tile shape");
   const int n = A.n, nb = tile_batch (s.rg), nw = s.nw, ch = s.ch, rg = s.rg;
   for (int i = 0; i < n; i++)
@@ -754,7 +757,7 @@ csr_tile_build (const DevCSR &A, const int *rp, const int *ci,
         for (int c = 0; c < nc; c++) { bpos[(size_t) c * nw + w] = b; b += bcnt[t][(size_t) c * nw + w]; }
     }
     for (int w = 0; w < nw; w++) {
-      int p[8], e[8];
+      int p[16], e[16];
       for (int r = 0; r < rg; r++) {
         const int i = rows[w * rg + r];
         p[r] = i >= 0 ? rp[i] : 0;
@@ -832,9 +835,10 @@ csr_tile_build (const DevCSR &A, const int *rp, const int *ci,
       }
   }
   if (cb.empty ()) cb.assign (2, 0);
+  const int64_t tailb = rg == 16 ? ch / 4 : 0;     // MFMA staging reads ch/4 batches
   std::vector<int> tu ((size_t) std::max<int64_t> (nchunk * ch, 1));
-  std::vector<int> es ((size_t) std::max<int64_t> (nbatch * nb, 1));
-  std::vector<double> ev ((size_t) std::max<int64_t> (nbatch * nb * rg, 1));
+  std::vector<int> es ((size_t) std::max<int64_t> ((nbatch + tailb) * nb, 1), 0);
+  std::vector<double> ev ((size_t) std::max<int64_t> ((nbatch + tailb) * nb * rg, 1), 0.0);
   par ([&] (int t) {
     work (t, true, tu.data () + (size_t) tp[t] * ch, es.data (), ev.data (), boff[t]);
   });

@@ -971,12 +971,168 @@ spmm_tile_k (int n, int nx, int ntiles, int xcd, int p0, int pend,
   }
 }
 
+// Y = alpha A X + beta Y (FP64 vectors) over a row-tile copy with 16 rows
+// a wavefront (rg = 16: 4 x 4 blocks of a 2D tile, or 16 consecutive rows)
+// on the matrix cores.  A wavefront's union of its rows' entries is walked
+// in k-steps of 4 entries: the k-step's 16 x 4 block of values (A, lane
+// i + 16 k holds row i, entry k) and the 4 x 16 column blocks of the 4
+// staged neighbour rows (B, lane j + 16 k holds entry k, column j) feed
+// v_mfma_f64_16x16x4f64, which on gfx950 rounds exactly as the FMA chain
+// over k = 0..3 started from C (probed: every output of 2 000 random
+// products); k-steps in column order therefore give each row the sums of
+// spmm64_k in its order, plus exact zeros: the same bits.  The 128 columns
+// of a launch row (2 panels) are 8 MFMA tiles: 16-column segments split
+// into their even and odd columns, so one 16-byte LDS read gives a lane
+// its column pair (2j, 2j + 1) of a segment of 32.  Everything the k-steps
+// read is staged in LDS per chunk by loads of a fixed count (the chunk's
+// X rows, and each wavefront's at most CH/4 A blocks and slots, which the
+// copy pads past its end), two chunks ahead, double-buffered, one barrier
+// a chunk.
+template <typename TV, int NW, int CH, int PP>
+__global__ void __launch_bounds__ (NW * 64)
+spmm_mt_k (int n, int nx, int ntiles, int xcd, int p0, int pend,
+           const int *__restrict__ tp, const int *__restrict__ tu,
+           const int *__restrict__ cb, const int *__restrict__ es,
+           const TV *__restrict__ ev, const int *__restrict__ trow,
+           double alpha, const double *__restrict__ X, double beta,
+           double *__restrict__ Y)
+{
+  constexpr int KM = CH / 4;                       // k-steps of a wavefront in a chunk, at most
+  constexpr int ROWB = PP * 512;                   // staged row: PP panels x 64 FP64
+  constexpr int SPW = CH / NW;                     // staged X rows per wavefront and chunk
+  constexpr int AL = KM * 64 * (int) sizeof (TV) / 16 / 64;   // 16-byte A loads per lane
+  static_assert (CH % NW == 0 && (KM * 64 * (int) sizeof (TV)) % 1024 == 0, "chunk shape");
+  static_assert (PP == 1 || PP == 2, "one or two panels a launch row");
+  typedef double V __attribute__ ((ext_vector_type (2)));
+  // a lane's piece of a staged row: 16 bytes of panel pb + lane / 32 (two
+  // panels), or 8 bytes of panel pb (one)
+  typedef typename std::conditional<PP == 2, V, double>::type P8;
+  typedef uint4 A16;
+  __shared__ __attribute__ ((aligned (16))) unsigned char ldx[2][CH * ROWB];
+  __shared__ __attribute__ ((aligned (16))) TV lda[2][NW][KM * 64];
+  __shared__ int lds_s[2][NW][KM * 4];
+  int t = (int) blockIdx.x;
+  if (xcd == 1) {
+    const int per = (ntiles + 7) / 8;
+    t = (t % 8) * per + t / 8;
+  }
+  if (t >= ntiles) return;                         // whole workgroup
+  const int wave = __builtin_amdgcn_readfirstlane ((int) (threadIdx.x >> 6));
+  const int lane = (int) threadIdx.x & 63;
+  const int pb = p0 + PP * (int) blockIdx.y;
+  const int sq = PP == 2 ? min (pb + lane / 32, pend - 1) : pb;
+  const double *Xl = X + (size_t) sq * nx * 64 + (PP == 2 ? (lane % 32) * 2 : lane);
+  typedef double d4v __attribute__ ((ext_vector_type (4)));
+  d4v acc[PP][2][2];
+#pragma unroll
+  for (int p = 0; p < PP; p++)
+#pragma unroll
+    for (int g = 0; g < 2; g++)
+#pragma unroll
+      for (int h = 0; h < 2; h++) acc[p][g][h] = (d4v) { 0.0, 0.0, 0.0, 0.0 };
+  const int cs = tp[t], ce = tp[t + 1];
+  if (cs < ce) {
+    struct Stage { P8 x[SPW]; A16 a[AL]; int s; };
+    Stage s0, s1;
+    auto fetch = [&] (int c, Stage &st) {
+      const int *u = tu + (size_t) c * CH + wave;
+#pragma unroll
+      for (int j = 0; j < SPW; j++) st.x[j] = *(const P8 *) (Xl + (size_t) u[j * NW] * 64);
+      const int q0 = cb[((size_t) c * NW + wave) * 2];
+      const A16 *av = (const A16 *) (ev + (size_t) q0 * 64);
+#pragma unroll
+      for (int m = 0; m < AL; m++) st.a[m] = av[m * 64 + lane];
+      st.s = es[(size_t) q0 * 4 + (lane < KM * 4 ? lane : 0)];
+    };
+    auto put = [&] (const Stage &st, int buf) {
+#pragma unroll
+      for (int j = 0; j < SPW; j++)
+        *(P8 *) (&ldx[buf][(j * NW + wave) * ROWB + lane * (int) sizeof (P8)]) = st.x[j];
+#pragma unroll
+      for (int m = 0; m < AL; m++) ((A16 *) lda[buf][wave])[m * 64 + lane] = st.a[m];
+      if (lane < KM * 4) lds_s[buf][wave][lane] = st.s;
+    };
+    auto products = [&] (int c, int buf) {
+      const int nk = cb[((size_t) c * NW + wave) * 2 + 1] - cb[((size_t) c * NW + wave) * 2];
+      const int j = lane % 16, k = lane / 16;
+      for (int ks = 0; ks < nk; ks++) {
+        const double a = (double) lda[buf][wave][ks * 64 + lane];
+        const int sl = lds_s[buf][wave][ks * 4 + k];
+        const unsigned char *row = &ldx[buf][sl * ROWB + j * 16];
+        V b[PP][2];
+#pragma unroll
+        for (int p = 0; p < PP; p++)
+#pragma unroll
+          for (int g = 0; g < 2; g++) b[p][g] = *(const V *) (row + p * 512 + g * 256);
+#pragma unroll
+        for (int p = 0; p < PP; p++)
+#pragma unroll
+          for (int g = 0; g < 2; g++) {
+            acc[p][g][0] = __builtin_amdgcn_mfma_f64_16x16x4f64 (a, b[p][g].x, acc[p][g][0], 0, 0, 0);
+            acc[p][g][1] = __builtin_amdgcn_mfma_f64_16x16x4f64 (a, b[p][g].y, acc[p][g][1], 0, 0, 0);
+          }
+      }
+    };
+    auto step = [&] (int c, int buf, Stage &st) {
+      __syncthreads ();
+      products (c, buf);
+      __builtin_amdgcn_sched_barrier (0);
+      if (c + 1 < ce) put (st, buf ^ 1);
+      fetch (min (c + 3, ce - 1), st);
+      __builtin_amdgcn_sched_barrier (0);
+    };
+    fetch (cs, s0);
+    put (s0, 0);
+    fetch (min (cs + 1, ce - 1), s1);
+    fetch (min (cs + 2, ce - 1), s0);
+    for (int c = cs; c < ce; c += 2) {
+      step (c, 0, s1);
+      if (c + 1 >= ce) break;
+      step (c + 1, 1, s0);
+    }
+  }
+  // D(i, j) of a tile: i = lane / 16 + 4 r, j = lane % 16; tile (p, g, h)
+  // holds column 32 g + 2 j + h of panel pb + p
+  const int j = lane % 16;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int i = lane / 16 + 4 * r;
+    const int e = wave * 16 + i;
+    const int row = trow ? trow[(size_t) t * (NW * 16) + e]
+                         : ((int64_t) t * (NW * 16) + e < n ? t * (NW * 16) + e : -1);
+    if (row < 0) continue;
+#pragma unroll
+    for (int p = 0; p < PP; p++) {
+      if (pb + p >= pend) continue;
+#pragma unroll
+      for (int g = 0; g < 2; g++) {
+        V *y = (V *) (Y + ((size_t) (pb + p) * n + row) * 64 + 32 * g + 2 * j);
+        V o;
+        if (beta == 0.0) {
+          o.x = alpha * acc[p][g][0][r];
+          o.y = alpha * acc[p][g][1][r];
+        }
+        else {
+          const V yo = *y;
+          o.x = alpha * acc[p][g][0][r] + beta * yo.x;
+          o.y = alpha * acc[p][g][1][r] + beta * yo.y;
+        }
+        *y = o;
+      }
+    }
+  }
+}
+
 // the tile shapes the autotuner times (rows per wavefront, wavefronts,
 // chunk rows); 2D variants of each are added when the matrix has a line
 // stride (csr_tile_stride)
 struct TileCand { int rg, nw, ch; };
 static const TileCand k_tile_cands[] = {
   { 4, 16, 32 }, { 4, 16, 64 }, { 8, 8, 32 }, { 8, 8, 64 },
+};
+// the matrix-core kernel's shapes (16 rows a wavefront)
+static const TileCand k_mt_cands[] = {
+  { 16, 4, 16 }, { 16, 8, 16 }, { 16, 4, 32 }, { 16, 8, 32 },
 };
 
 // panels [p0, p0 + np) of Y = alpha A X + beta Y through the tile copy
@@ -1001,6 +1157,36 @@ spmm_tile_launch (const DevCSR &A, int p0, int np, T alpha, const S *X,
     return 0;
   };
   const TileShape &s = c.s;
+  if (s.rg == 16) {
+    if constexpr (std::is_same<T, double>::value && std::is_same<S, double>::value) {
+      // whole pairs of panels, then an odd last panel on its own
+      auto mt = [&] (auto nwc, auto chc) -> int {
+        constexpr int NW = decltype (nwc)::value, CH = decltype (chc)::value;
+        const int pairs = np / 2;
+        if (pairs > 0) {
+          hipLaunchKernelGGL ((spmm_mt_k<TV, NW, CH, 2>), dim3 (gx, (unsigned) pairs),
+                              dim3 (NW * 64), 0, stream (), A.n, A.m, c.ntiles, c.s.xcd,
+                              p0, p0 + 2 * pairs, c.tp, c.tu, c.cb, c.es, (const TV *) c.ev,
+                              c.trow, (double) alpha, X, (double) beta, Y);
+          LGH_LAUNCH_CHECK ();
+        }
+        if (np % 2) {
+          hipLaunchKernelGGL ((spmm_mt_k<TV, NW, CH, 1>), dim3 (gx, 1u),
+                              dim3 (NW * 64), 0, stream (), A.n, A.m, c.ntiles, c.s.xcd,
+                              p0 + np - 1, p0 + np, c.tp, c.tu, c.cb, c.es, (const TV *) c.ev,
+                              c.trow, (double) alpha, X, (double) beta, Y);
+          LGH_LAUNCH_CHECK ();
+        }
+        return 0;
+      };
+      if (s.nw == 4 && s.ch == 16) return mt (IC<4> (), IC<16> ());
+      if (s.nw == 8 && s.ch == 16) return mt (IC<8> (), IC<16> ());
+      if (s.nw == 4 && s.ch == 32) return mt (IC<4> (), IC<32> ());
+      if (s.nw == 8 && s.ch == 32) return mt (IC<8> (), IC<32> ());
+    }
+    set_error ("spmm_tile_launch: matrix-core tile shape not compiled (FP64 vectors only)");
+    return 8;
+  }
   if (s.rg == 4 && s.nw == 16 && s.ch == 32) return go (IC<4> (), IC<16> (), IC<32> ());
   if (s.rg == 4 && s.nw == 16 && s.ch == 64) return go (IC<4> (), IC<16> (), IC<64> ());
   if (s.rg == 8 && s.nw == 8 && s.ch == 32) return go (IC<8> (), IC<8> (), IC<32> ());
@@ -1411,19 +1597,24 @@ tile_tune_one (const DevCSR &A, PMat (&X)[2], PMat (&Y)[2], double (&ms)[2])
   return rc;
 }
 
-// the tile shape of forced candidate k: k_tile_cands[k & 3], 2D (the
-// matrix's line stride, 1D when it has none) with bit 4, XCD runs with bit 8
+// the tile shape of candidate k: bits 0-1 the shape (k_tile_cands, or with
+// bit 4 the matrix-core k_mt_cands), bit 2 2D tiles 16 rows wide on the
+// matrix's line stride (1D when it has none; matrix-core wavefronts then
+// take 4 x 4 blocks), bit 3 XCD order (runs of 1D tiles, strips of 2D)
 static TileShape
 tile_shape_of (int k, int n, const int *rp, const int *ci)
 {
-  const TileCand &c = k_tile_cands[k & 3];
+  const TileCand &c = (k & 16) ? k_mt_cands[k & 3] : k_tile_cands[k & 3];
   TileShape s;
   s.rg = c.rg; s.nw = c.nw; s.ch = c.ch;
   s.xcd = (k & 8) ? 1 : 0;
   if (k & 4) {
     const int rows = c.rg * c.nw, tx = 16;
     s.stride = csr_tile_stride (n, rp, ci, tx, rows);
-    if (s.stride) s.tx = tx;
+    if (s.stride) {
+      s.tx = tx;
+      if (c.rg == 16) s.wy = 4;
+    }
     if (s.tx && (k & 8)) s.xcd = 2;     // 2D: strips of the tile grid per XCD
   }
   return s;
@@ -1494,10 +1685,12 @@ csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v)
     }
   }
   // the row-tile copy: every shape, 1D and (with a line stride) 2D, tiles
-  // in plain and in XCD-run order
+  // in plain and in XCD-run order; the matrix-core shapes (FP64 vectors
+  // only) for FP64 values
   int stride = -1;
-  for (int k = 0; k < 16 && !rc; k++) {
+  for (int k = 0; k < 32 && !rc; k++) {
     if ((k & 4) && stride == 0) continue;
+    if ((k & 16) && A.prec != F64) continue;
     TileShape s = tile_shape_of (k, A.n, rp, ci);
     if (k & 4) {
       stride = s.stride;
@@ -1507,8 +1700,9 @@ csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v)
     C.n = A.n; C.m = A.m; C.nnz = A.nnz; C.prec = A.prec;
     rc = csr_tile_build (A, rp, ci, v, s, C.tile);
     double ms[2] = { 0, 0 };
-    if (!rc) rc = A.prec == F64 ? tile_tune_one<double> (C, X, Y, ms)
-                                : tile_tune_one<float> (C, X, Y, ms);
+    if (!rc && (k & 16)) rc = tile_time<double, double> (C, X[0], Y[0], &ms[0]);
+    else if (!rc) rc = A.prec == F64 ? tile_tune_one<double> (C, X, Y, ms)
+                                     : tile_tune_one<float> (C, X, Y, ms);
     if (!rc && cost_of (ms) < best_cost) {
       best.tile = 1;
       best.ts = s;
@@ -2124,7 +2318,7 @@ static int
 spmm_t (const DevCSR &A, double alpha, const PMat &X, double beta, PMat &Y)
 {
   if constexpr (std::is_same<S, T>::value) {
-    if (A.tile.tp && X.W == 64)
+    if (A.tile.tp && X.W == 64 && (A.tile.s.rg != 16 || std::is_same<T, double>::value))
       return spmm_tile_launch<TV, T, S> (A, 0, X.npanels (), (T) alpha,
                                          (const S *) X.p, (T) beta, (S *) Y.p);
     if (A.grp.gp && X.W == 64)

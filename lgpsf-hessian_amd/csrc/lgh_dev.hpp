@@ -164,12 +164,16 @@ struct GroupCopy {
 // rows) restricted to chunk c is batches cb[c*nw + w] .. cb[c*nw + w + 1];
 // batch b has tile_batch(rg) entries, entry u at LDS slot es[b*nb + u] with
 // rg values ev[(b*nb + u)*rg + r] (0 where row r has no entry; padding
-// entries: value 0).  Tiles tp[t] .. tp[t+1] in chunks.
-constexpr int tile_batch (int rg) { return 16 / rg; }
+// entries: value 0).  Tiles tp[t] .. tp[t+1] in chunks.  With rg = 16 (the
+// MFMA kernel) a batch is one 4-entry k-step and its 64 values are in the
+// MFMA's A-operand lane order (entry k, row i at k*16 + i); es and ev then
+// carry ch/4 batches of zero padding past the last batch.
+constexpr int tile_batch (int rg) { return rg == 16 ? 4 : 16 / rg; }
 struct TileShape {
   int rg = 0, nw = 0, ch = 0;     // rows per wavefront, wavefronts, chunk
   int tx = 0, stride = 0;         // 0: consecutive rows; else a tx x (nw*rg/tx) block
   int xcd = 0;                    // 1: tiles dealt to XCDs in contiguous runs
+  int wy = 1;                     // 2D: lines a wavefront's rows span ((rg/wy) x wy)
 };
 struct TileCopy {
   TileShape s;

@@ -75,8 +75,9 @@ def test_aniso_psf_build_matches_oracle(dev, oracle, aniso, mixed):
             assert np.array_equal(gg.lam, g.lam), (rg, ppw)
             assert np.array_equal(gg.U, g.U), (rg, ppw)
         # and through the row-tile copy (1D 4 x 16 rows; 2D 16 x 4 rows on
-        # the grid's line stride, in XCD-run order)
-        for cand in (0, 4 | 8 | 3):
+        # the grid's line stride, in XCD-run order; the matrix-core kernel
+        # on 2D tiles of 4 x 4-row blocks)
+        for cand in (0, 4 | 8 | 3, 16 | 4 | 1, 16 | 4 | 8 | 2):
             dev.spmm_tile_config(cand)
             gg = dev.GLR(dev.Mat.aij(aniso["B"]), aniso["prior"], opts, x=x)
             assert np.array_equal(gg.lam, g.lam), cand

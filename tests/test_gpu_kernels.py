@@ -106,8 +106,10 @@ def test_spmm_wide_rows_width_invariant(dev, prec_vals):
             assert np.array_equal(a32, b32), (rg, ppw)
         # the row-tile copy (LDS-staged union of 64 rows): every shape, 1D
         # and 2D tiles (the 45-wide grid's line stride; partial tiles at
-        # the grid's right edge and bottom), plain and XCD-run tile order
-        for cand in range(16):
+        # the grid's right edge and bottom), plain and XCD-run tile order;
+        # candidates 16-31 run the FP64 products on the matrix cores (FP32
+        # vectors fall back to the row kernel)
+        for cand in range(32):
             dev.spmm_tile_config(cand)
             got = dev.spmm(B, X, prec_vals=prec_vals, W=64, grouped=True)
             assert np.array_equal(got, y64), cand
