@@ -657,11 +657,8 @@ csr_group_build (const DevCSR &A, const int *rp, const int *ci,
 }
 
 // the rows of a tile of shape s (-1: none), local index e = w * rg + r.
-// 2D tiles sit on an ntx x nty grid of tiles; with s.xcd = 2 tile t is
-// launched as workgroup t, which the dispatcher hands to XCD t mod 8, and
-// XCD x walks the x-th vertical strip of the tile grid band by band, so
-// the tiles one XCD runs at a time are neighbours and share the rows they
-// stage through its L2 (positions past the grid: empty tiles)
+// 2D tiles sit on an ntx x nty grid of tiles, tile t at column t % ntx of
+// band t / ntx (the SpMM launches may deal them to the XCDs in strips)
 static inline int
 tile_row (const TileShape &s, int n, int ntx, int nty, int t, int e)
 {
@@ -670,13 +667,8 @@ tile_row (const TileShape &s, int n, int ntx, int nty, int t, int e)
     const int64_t r = (int64_t) t * T + e;
     return r < n ? (int) r : -1;
   }
-  int tcol = t % ntx, tband = t / ntx;
-  if (s.xcd == 2) {
-    const int sw = (ntx + 7) / 8, l = t / 8;
-    tcol = (t % 8) * sw + l % sw;
-    tband = l / sw;
-    if (tcol >= ntx || tband >= nty) return -1;
-  }
+  (void) nty;
+  const int tcol = t % ntx, tband = t / ntx;
   // wavefront w: a (rg/wy) x wy block of the tile, wpl blocks across
   const int ty = T / s.tx, wx = s.rg / s.wy, wpl = s.tx / wx;
   const int w = e / s.rg, r = e % s.rg;
@@ -726,7 +718,7 @@ csr_tile_build (const DevCSR &A, const int *rp, const int *ci,
     const int64_t ny = (lines + T / s.tx - 1) / (T / s.tx);
     LGH_CHECK (ny < INT_MAX / 2, 8, "csr_tile_build: too many tiles");
     nty = (int) ny;
-    nt64 = s.xcd == 2 ? (int64_t) 8 * ((ntx + 7) / 8) * nty : (int64_t) ntx * nty;
+    nt64 = (int64_t) ntx * nty;
   }
   else nt64 = ((int64_t) n + T - 1) / T;
   LGH_CHECK (nt64 < INT_MAX / 2, 8, "csr_tile_build: too many tiles");
@@ -852,6 +844,7 @@ csr_tile_build (const DevCSR &A, const int *rp, const int *ci,
   TileCopy c;
   c.s = s;
   c.ntiles = nt;
+  c.ntx = ntx;
   c.nchunk = nchunk;
   c.nbatch = nbatch;
   c.union_per_row = n ? (double) nchunk * ch / n : 0.0;

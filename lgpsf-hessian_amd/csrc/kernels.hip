@@ -834,6 +834,27 @@ spmm_grp_launch (const DevCSR &A, int ppw, int p0, int np, T alpha,
   return 8;
 }
 
+// the tile of this workgroup (-1: none) in launch order xcd: 0 the tiles
+// in order; 1 dealt to the XCDs (dispatch order b mod 8) in contiguous
+// runs; 2 (2D tiles, ntx a band) XCD x walks the x-th vertical strip of
+// the tile grid band by band, so the tiles one XCD runs at a time are
+// neighbours and share the rows they stage through its L2
+__device__ inline int
+tile_of_block (int ntiles, int xcd, int ntx)
+{
+  const int b = (int) blockIdx.x;
+  if (xcd == 1) {
+    const int per = (ntiles + 7) / 8, t = (b % 8) * per + b / 8;
+    return t < ntiles ? t : -1;
+  }
+  if (xcd == 2) {
+    const int nty = ntiles / ntx, sw = (ntx + 7) / 8, l = b / 8;
+    const int tcol = (b % 8) * sw + l % sw, tband = l / sw;
+    return (tcol < ntx && tband < nty) ? tband * ntx + tcol : -1;
+  }
+  return b < ntiles ? b : -1;
+}
+
 // Y = alpha A X + beta Y over the row-tile copy of A (DevCSR::tile), one
 // workgroup of NW wavefronts per tile of NW * RG rows and one 16-byte piece
 // of a panel row per lane (2 FP64 / 4 FP32 panels a launch row, so a staged
@@ -852,7 +873,7 @@ spmm_grp_launch (const DevCSR &A, int ppw, int p0, int np, T alpha,
 // per XCD are laid out by the tile builder itself).
 template <typename TV, typename T, typename S, int RG, int NW, int CH>
 __global__ void __launch_bounds__ (NW * 64)
-spmm_tile_k (int n, int nx, int ntiles, int xcd, int p0, int pend,
+spmm_tile_k (int n, int nx, int ntiles, int xcd, int ntx, int p0, int pend,
              const int *__restrict__ tp, const int *__restrict__ tu,
              const int *__restrict__ cb, const int *__restrict__ es,
              const TV *__restrict__ ev, const int *__restrict__ trow, T alpha,
@@ -868,12 +889,8 @@ spmm_tile_k (int n, int nx, int ntiles, int xcd, int p0, int pend,
   static_assert (CH % NW == 0, "chunk rows: a multiple of the wavefronts");
   typedef S V __attribute__ ((ext_vector_type (VPL)));
   __shared__ __attribute__ ((aligned (16))) unsigned char lds[2][CH * ROWB];
-  int t = (int) blockIdx.x;
-  if (xcd == 1) {
-    const int per = (ntiles + 7) / 8;
-    t = (t % 8) * per + t / 8;
-  }
-  if (t >= ntiles) return;                         // whole workgroup
+  const int t = tile_of_block (ntiles, xcd, ntx);
+  if (t < 0) return;                               // whole workgroup
   const int wave = __builtin_amdgcn_readfirstlane ((int) (threadIdx.x >> 6));
   const int lane = (int) threadIdx.x & 63;
   const int pb = p0 + VPL * (int) blockIdx.y;
@@ -990,7 +1007,7 @@ spmm_tile_k (int n, int nx, int ntiles, int xcd, int p0, int pend,
 // a chunk.
 template <typename TV, int NW, int CH, int PP>
 __global__ void __launch_bounds__ (NW * 64)
-spmm_mt_k (int n, int nx, int ntiles, int xcd, int p0, int pend,
+spmm_mt_k (int n, int nx, int ntiles, int xcd, int ntx, int p0, int pend,
            const int *__restrict__ tp, const int *__restrict__ tu,
            const int *__restrict__ cb, const int *__restrict__ es,
            const TV *__restrict__ ev, const int *__restrict__ trow,
@@ -1011,12 +1028,8 @@ spmm_mt_k (int n, int nx, int ntiles, int xcd, int p0, int pend,
   __shared__ __attribute__ ((aligned (16))) unsigned char ldx[2][CH * ROWB];
   __shared__ __attribute__ ((aligned (16))) TV lda[2][NW][KM * 64];
   __shared__ int lds_s[2][NW][KM * 4];
-  int t = (int) blockIdx.x;
-  if (xcd == 1) {
-    const int per = (ntiles + 7) / 8;
-    t = (t % 8) * per + t / 8;
-  }
-  if (t >= ntiles) return;                         // whole workgroup
+  const int t = tile_of_block (ntiles, xcd, ntx);
+  if (t < 0) return;                               // whole workgroup
   const int wave = __builtin_amdgcn_readfirstlane ((int) (threadIdx.x >> 6));
   const int lane = (int) threadIdx.x & 63;
   const int pb = p0 + PP * (int) blockIdx.y;
@@ -1144,14 +1157,16 @@ spmm_tile_launch (const DevCSR &A, int p0, int np, T alpha, const S *X,
   constexpr int VPL = 16 / (int) sizeof (S);
   const TileCopy &c = A.tile;
   if (np <= 0) return 0;
-  const unsigned gx = (unsigned) (c.s.xcd == 1 ? (c.ntiles + 7) / 8 * 8 : c.ntiles);
+  const unsigned gx = (unsigned) (c.s.xcd == 1 ? (c.ntiles + 7) / 8 * 8
+                                  : c.s.xcd == 2 ? 8 * ((c.ntx + 7) / 8) * (c.ntiles / c.ntx)
+                                  : c.ntiles);
   const unsigned gy = (unsigned) ((np + VPL - 1) / VPL);
   auto go = [&] (auto rgc, auto nwc, auto chc) -> int {
     constexpr int RG = decltype (rgc)::value, NW = decltype (nwc)::value,
                   CH = decltype (chc)::value;
     hipLaunchKernelGGL ((spmm_tile_k<TV, T, S, RG, NW, CH>), dim3 (gx, gy),
                         dim3 (NW * 64), 0, stream (), A.n, A.m, c.ntiles,
-                        c.s.xcd, p0, p0 + np, c.tp, c.tu, c.cb, c.es,
+                        c.s.xcd, c.ntx, p0, p0 + np, c.tp, c.tu, c.cb, c.es,
                         (const TV *) c.ev, c.trow, alpha, X, beta, Y, (int *) nullptr);
     LGH_LAUNCH_CHECK ();
     return 0;
@@ -1165,14 +1180,14 @@ spmm_tile_launch (const DevCSR &A, int p0, int np, T alpha, const S *X,
         const int pairs = np / 2;
         if (pairs > 0) {
           hipLaunchKernelGGL ((spmm_mt_k<TV, NW, CH, 2>), dim3 (gx, (unsigned) pairs),
-                              dim3 (NW * 64), 0, stream (), A.n, A.m, c.ntiles, c.s.xcd,
+                              dim3 (NW * 64), 0, stream (), A.n, A.m, c.ntiles, c.s.xcd, c.ntx,
                               p0, p0 + 2 * pairs, c.tp, c.tu, c.cb, c.es, (const TV *) c.ev,
                               c.trow, (double) alpha, X, (double) beta, Y);
           LGH_LAUNCH_CHECK ();
         }
         if (np % 2) {
           hipLaunchKernelGGL ((spmm_mt_k<TV, NW, CH, 1>), dim3 (gx, 1u),
-                              dim3 (NW * 64), 0, stream (), A.n, A.m, c.ntiles, c.s.xcd,
+                              dim3 (NW * 64), 0, stream (), A.n, A.m, c.ntiles, c.s.xcd, c.ntx,
                               p0 + np - 1, p0 + np, c.tp, c.tu, c.cb, c.es, (const TV *) c.ev,
                               c.trow, (double) alpha, X, (double) beta, Y);
           LGH_LAUNCH_CHECK ();
@@ -1684,34 +1699,41 @@ csr_group_rows (DevCSR &A, const int *rp, const int *ci, const double *v)
       C.grp = GroupCopy ();                      // now owned by A
     }
   }
-  // the row-tile copy: every shape, 1D and (with a line stride) 2D, tiles
-  // in plain and in XCD-run order; the matrix-core shapes (FP64 vectors
-  // only) for FP64 values
-  int stride = -1;
+  // the row-tile copies: every shape as 2D tiles when the matrix has a
+  // line stride (1D tiles otherwise; on a grid they never won), the
+  // matrix-core shapes (FP64 vectors only) for FP64 values; each copy is
+  // timed in plain and in XCD order (one copy serves both)
+  const bool two_d = csr_tile_stride (A.n, rp, ci, 16, 64) > 0;
   for (int k = 0; k < 32 && !rc; k++) {
-    if ((k & 4) && stride == 0) continue;
+    if (k & 8) continue;
     if ((k & 16) && A.prec != F64) continue;
-    TileShape s = tile_shape_of (k, A.n, rp, ci);
-    if (k & 4) {
-      stride = s.stride;
-      if (!stride) continue;
-    }
+    if (((k & 4) != 0) != two_d) continue;
+    const TileShape s = tile_shape_of (k, A.n, rp, ci);
+    if ((k & 4) && !s.tx) continue;
     DevCSR C;
     C.n = A.n; C.m = A.m; C.nnz = A.nnz; C.prec = A.prec;
     rc = csr_tile_build (A, rp, ci, v, s, C.tile);
-    double ms[2] = { 0, 0 };
-    if (!rc && (k & 16)) rc = tile_time<double, double> (C, X[0], Y[0], &ms[0]);
-    else if (!rc) rc = A.prec == F64 ? tile_tune_one<double> (C, X, Y, ms)
-                                     : tile_tune_one<float> (C, X, Y, ms);
-    if (!rc && cost_of (ms) < best_cost) {
-      best.tile = 1;
-      best.ts = s;
-      best.ms[0] = ms[0];
-      best.ms[1] = ms[1];
-      best_cost = cost_of (ms);
+    int won = -1;
+    for (int xo = 0; xo < 2 && !rc; xo++) {
+      C.tile.s.xcd = xo ? (s.tx ? 2 : 1) : 0;
+      double ms[2] = { 0, 0 };
+      if (k & 16) rc = tile_time<double, double> (C, X[0], Y[0], &ms[0]);
+      else rc = A.prec == F64 ? tile_tune_one<double> (C, X, Y, ms)
+                              : tile_tune_one<float> (C, X, Y, ms);
+      if (!rc && cost_of (ms) < best_cost) {
+        best.tile = 1;
+        best.ts = C.tile.s;
+        best.ms[0] = ms[0];
+        best.ms[1] = ms[1];
+        best_cost = cost_of (ms);
+        won = C.tile.s.xcd;
+      }
+    }
+    if (!rc && won >= 0) {
+      C.tile.s.xcd = won;
       A.tile.release ();
       A.tile = C.tile;
-      C.tile = TileCopy ();
+      C.tile = TileCopy ();                      // now owned by A
     }
   }
   for (int vp = 0; vp < 2; vp++) { pmat_free (X[vp]); pmat_free (Y[vp]); }

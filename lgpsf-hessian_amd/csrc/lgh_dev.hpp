@@ -172,12 +172,13 @@ constexpr int tile_batch (int rg) { return rg == 16 ? 4 : 16 / rg; }
 struct TileShape {
   int rg = 0, nw = 0, ch = 0;     // rows per wavefront, wavefronts, chunk
   int tx = 0, stride = 0;         // 0: consecutive rows; else a tx x (nw*rg/tx) block
-  int xcd = 0;                    // 1: tiles dealt to XCDs in contiguous runs
+  int xcd = 0;                    // launch order: 1 tiles dealt to the XCDs in
+                                  // contiguous runs, 2 (2D) in strips of the tile grid
   int wy = 1;                     // 2D: lines a wavefront's rows span ((rg/wy) x wy)
 };
 struct TileCopy {
   TileShape s;
-  int ntiles = 0;
+  int ntiles = 0, ntx = 0;        // ntx: tiles a band (2D), 0 for 1D
   int *tp = nullptr, *tu = nullptr, *cb = nullptr, *es = nullptr;
   int *trow = nullptr;            // null: consecutive rows
   void *ev = nullptr;             // values, precision of the owning matrix
