@@ -616,6 +616,23 @@ def main():
     extra.update(solve_rhs_per_s=m / t_blk, solve_block_m=m,
                  solve_latency_p50_s=float(np.percentile(lat, 50)),
                  solve_latency_p99_s=float(np.percentile(lat, 99)))
+    # the same solves on right-hand sides already in HBM (lgh_glr_solve_block_dev:
+    # no host transfer), one untimed call each first
+    Rd = lgh.DeviceBlock.from_host(R)
+    Xd = g.solve_block_device(cs, Rd)
+    t0 = time.perf_counter()
+    Xd = g.solve_block_device(cs, Rd)
+    t_dev = time.perf_counter() - t0
+    same = bool(np.array_equal(Xd.to_host(), X))
+    g.solve_block_device(cs[:1], Rd.columns(0, 1))
+    lat_d = []
+    for j in range(20):
+        t1 = time.perf_counter()
+        g.solve_block_device(cs[j], Rd.columns(j, j + 1))
+        lat_d.append(time.perf_counter() - t1)
+    extra.update(solve_rhs_per_s_resident=m / t_dev, solve_resident_same_bits=same,
+                 solve_latency_resident_p50_s=float(np.percentile(lat_d, 50)))
+    del Rd, Xd
     if not args.no_plain:
         t0 = time.perf_counter()
         Xp = gp.solve_block(cs, R)
@@ -630,8 +647,11 @@ def main():
         ldp = np.array([gp.logdet(c) for c in cs])
         extra["mixed_vs_fp64_logdet_rel"] = float(np.max(np.abs(ld - ldp) / np.abs(ldp)))
         del S, Sp, Xp
-    log("solve block m=%d: %.1f RHS/s; single-RHS p50 %.3fs" %
-        (m, m / t_blk, extra["solve_latency_p50_s"]))
+    log("solve block m=%d: %.1f RHS/s (%.1f resident in HBM, same bits %s); "
+        "single-RHS p50 %.4fs (%.4fs resident)" %
+        (m, m / t_blk, extra["solve_rhs_per_s_resident"],
+         extra["solve_resident_same_bits"], extra["solve_latency_p50_s"],
+         extra["solve_latency_resident_p50_s"]))
     del X
     # incremental extension of the resident mixed GLR by 200 columns
     # (hashed columns ell..ell+199, power pass, BCGS2, bordered projection);

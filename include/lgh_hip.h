@@ -52,6 +52,17 @@ int  lgh_hip_prof_collect (int kind, int np, int prec, int64_t *count,
  * (No reference counterpart: measurement.) */
 int lgh_hip_copy_gbps (int64_t bytes, int reps, double *gbps);
 
+/* plain device memory of the engine's HIP runtime, for callers that hold
+ * no GPU array of their own (the device-resident lgh_glr_*_block_dev
+ * calls take such blocks): alloc returns NULL on failure (message in
+ * lgh_hip_last_error); put / get are synchronous host <-> device copies,
+ * refusing a device side that is not device memory.  (No reference
+ * counterpart: the reference's blocks are host PETSc objects.) */
+void *lgh_hip_dev_alloc (int64_t bytes);
+void  lgh_hip_dev_free (void *p);
+int   lgh_hip_dev_put (void *dst, const void *src, int64_t bytes);
+int   lgh_hip_dev_get (void *dst, const void *src, int64_t bytes);
+
 /* the device hashed-Gaussian arithmetic evaluated on the host (no GPU):
  * the draw with the correctly rounded log(q) */
 double lgh_hip_randn_at_host (uint64_t seed, int64_t i, int64_t j);

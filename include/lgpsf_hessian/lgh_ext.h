@@ -69,6 +69,19 @@ int lgh_glr_solve_block (lgh_glr_t *glr, int m, const double *c,
 int lgh_glr_sample_block (lgh_glr_t *glr, int m, const double *c,
                           const double *xi, int ldr, double *x, int ldx);
 
+/* The same two operations on blocks already in device memory, for callers
+ * whose right-hand sides live in HBM (no host transfer): rhs / xi and x
+ * are device pointers (column-major, leading dimensions >= n; host memory
+ * is refused with PETSC_ERR_ARG_WRONG), c a host array of m shifts.  The
+ * engine reads the input on its own stream, so the work that wrote it must
+ * have finished (synchronize the producing stream first); x is complete
+ * when the call returns.  The same bits as lgh_glr_solve_block /
+ * lgh_glr_sample_block on the same columns. */
+int lgh_glr_solve_block_dev (lgh_glr_t *glr, int m, const double *c,
+                             const double *rhs, int ldr, double *x, int ldx);
+int lgh_glr_sample_block_dev (lgh_glr_t *glr, int m, const double *c,
+                              const double *xi, int ldr, double *x, int ldx);
+
 /* Batches of independent instances (no reference counterpart: the
  * reference builds one object per call).  Instance i runs the single
  * call with argument i (lgh_prior_create_mat, lgh_glr_compute_x): the

@@ -198,6 +198,10 @@ _SIGS = {
                                       C.c_int]),
     "lgh_glr_sample_block": (C.c_int, [vp, C.c_int, dp, dp, C.c_int, dp,
                                        C.c_int]),
+    "lgh_glr_solve_block_dev": (C.c_int, [vp, C.c_int, dp, vp, C.c_int, vp,
+                                          C.c_int]),
+    "lgh_glr_sample_block_dev": (C.c_int, [vp, C.c_int, dp, vp, C.c_int, vp,
+                                           C.c_int]),
     "lgh_prior_get_info": (C.c_int, [vp, C.POINTER(PriorInfo)]),
     "lgh_prior_solve_host": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp,
                                        C.c_int, dp, C.c_int]),
@@ -221,6 +225,10 @@ _SIGS = {
     "lgh_hip_prof_enable": (None, [C.c_int]),
     "lgh_hip_prof_collect": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64), dp, dp, dp]),
     "lgh_hip_copy_gbps": (C.c_int, [C.c_int64, C.c_int, dp]),
+    "lgh_hip_dev_alloc": (vp, [C.c_int64]),
+    "lgh_hip_dev_free": (None, [vp]),
+    "lgh_hip_dev_put": (C.c_int, [vp, vp, C.c_int64]),
+    "lgh_hip_dev_get": (C.c_int, [vp, vp, C.c_int64]),
     "lgh_hip_randn_at_host": (C.c_double, [C.c_uint64, C.c_int64, C.c_int64]),
     "lgh_hip_randn_host_block": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.c_int,
                                            C.c_int, C.c_int, dp, C.c_int,
@@ -498,7 +506,7 @@ class Prior:
 
     def solve(self, X, which=0, prec=PREC_F64):
         X = _fcol(X)
-        Y = np.zeros_like(X, order="F")
+        Y = np.zeros(X.shape, order="F")   # calloc: pages fault in on the copy-out threads
         _petsc_check(lib().lgh_prior_solve_host(self.h, which, prec, X.shape[1],
                                                 _d(X), X.shape[0], _d(Y),
                                                 X.shape[0]),
@@ -514,6 +522,71 @@ class Prior:
         if getattr(self, "h", None) and _lib is not None:
             _lib.lgh_prior_destroy(self.h)
             self.h = None
+
+
+class DeviceBlock:
+    """An n x m FP64 column-major block in device memory (leading dimension
+    ld >= n), allocated with the engine's own HIP runtime
+    (lgh_hip_dev_alloc): the input and output of GLR.solve_block_device /
+    sample_block_device.  columns(j0, j1) is a view of columns j0..j1-1 that
+    keeps its parent alive."""
+
+    def __init__(self, n, m, ld=None, _parent=None, _ptr=None):
+        self.n, self.m = int(n), int(m)
+        self.ld = self.n if ld is None else int(ld)
+        if self.ld < self.n:
+            raise ValueError("DeviceBlock: ld must be >= n")
+        self._parent = _parent
+        if _ptr is not None:
+            self.ptr = _ptr
+        else:
+            p = lib().lgh_hip_dev_alloc(8 * self.ld * max(self.m, 1))
+            if not p:
+                raise LGHError("lgh_hip_dev_alloc failed: %s"
+                               % lib().lgh_hip_last_error().decode())
+            self.ptr = p
+
+    @classmethod
+    def from_host(cls, A, ld=None):
+        A = _fcol(A)
+        b = cls(A.shape[0], A.shape[1], ld)
+        b.put(A)
+        return b
+
+    def put(self, A):
+        A = _fcol(A)
+        if A.shape != (self.n, self.m):
+            raise ValueError("DeviceBlock.put: shape mismatch")
+        for j0, j1 in self._runs():
+            src = np.asfortranarray(A[:, j0:j1]) if self.ld != self.n else A
+            _hip_check(lib().lgh_hip_dev_put(self._col(j0), src.ctypes.data,
+                                             8 * self.n * (j1 - j0)), "lgh_hip_dev_put")
+
+    def to_host(self):
+        A = np.zeros((self.n, self.m), order="F")
+        for j0, j1 in self._runs():
+            part = np.zeros((self.n, j1 - j0), order="F") if self.ld != self.n else A
+            _hip_check(lib().lgh_hip_dev_get(part.ctypes.data, self._col(j0),
+                                             8 * self.n * (j1 - j0)), "lgh_hip_dev_get")
+            if part is not A:
+                A[:, j0:j1] = part
+        return A
+
+    def columns(self, j0, j1):
+        return DeviceBlock(self.n, j1 - j0, self.ld, _parent=self, _ptr=self._col(j0))
+
+    def _col(self, j):
+        return self.ptr + 8 * self.ld * j
+
+    def _runs(self):
+        # contiguous blocks move whole; strided ones column by column
+        return [(0, self.m)] if self.ld == self.n else [(j, j + 1) for j in range(self.m)]
+
+    def __del__(self):
+        if getattr(self, "_parent", 1) is None and getattr(self, "ptr", None) \
+                and _lib is not None:
+            _lib.lgh_hip_dev_free(self.ptr)
+            self.ptr = None
 
 
 class GLR:
@@ -652,7 +725,7 @@ class GLR:
         X = _fcol(X)
         m = X.shape[1]
         cc = np.ascontiguousarray(np.broadcast_to(np.asarray(c, dtype=np.float64), (m,)))
-        Y = np.zeros_like(X, order="F")
+        Y = np.zeros(X.shape, order="F")   # calloc: pages fault in on the copy-out threads
         _petsc_check(fn(self.h, m, _d(cc), _d(X), X.shape[0], _d(Y), X.shape[0]), name)
         return Y
 
@@ -661,6 +734,23 @@ class GLR:
 
     def sample_block(self, c, X):
         return self._block(lib().lgh_glr_sample_block, "lgh_glr_sample_block", c, X)
+
+    def _block_device(self, fn, name, c, X):
+        if not isinstance(X, DeviceBlock) or X.n != self.n:
+            raise ValueError("%s: need a DeviceBlock with n = %d rows" % (name, self.n))
+        m = X.m
+        cc = np.ascontiguousarray(np.broadcast_to(np.asarray(c, dtype=np.float64), (m,)))
+        Y = DeviceBlock(self.n, m)
+        _petsc_check(fn(self.h, m, _d(cc), X.ptr, X.ld, Y.ptr, Y.ld), name)
+        return Y
+
+    def solve_block_device(self, c, X):
+        return self._block_device(lib().lgh_glr_solve_block_dev,
+                                  "lgh_glr_solve_block_dev", c, X)
+
+    def sample_block_device(self, c, X):
+        return self._block_device(lib().lgh_glr_sample_block_dev,
+                                  "lgh_glr_sample_block_dev", c, X)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -851,7 +941,7 @@ def spmm_group_report():
 def cheb_jacobi(A, lmin, lmax, nit, B, prec=PREC_F64, W=64):
     rp, ci, v = _csr(A)
     B = _fcol(B)
-    X = np.zeros_like(B, order="F")
+    X = np.zeros(B.shape, order="F")
     _hip_check(lib().lgh_hip_cheb_jacobi(A.shape[0], _i(rp), _i(ci), _d(v), prec,
                                          W, lmin, lmax, nit, B.shape[1], _d(B),
                                          B.shape[0], _d(X), B.shape[0]),
@@ -924,7 +1014,7 @@ class AMG:
 
     def vcycle(self, B, prec=PREC_F64, W=64):
         B = _fcol(B)
-        X = np.zeros_like(B, order="F")
+        X = np.zeros(B.shape, order="F")
         _hip_check(lib().lgh_amg_vcycle(self.h, prec, W, B.shape[1], _d(B),
                                         B.shape[0], _d(X), B.shape[0]),
                    "lgh_amg_vcycle")

@@ -66,6 +66,31 @@ void lgh_hip_trim (void) { dtrim (); }
 
 void lgh_hip_prof_enable (int on) { prof_enable (on); }
 
+void *
+lgh_hip_dev_alloc (int64_t bytes)
+{
+  void *p = nullptr;
+  return dev_block_alloc (bytes > 0 ? (size_t) bytes : 0, &p) ? nullptr : p;
+}
+
+void lgh_hip_dev_free (void *p) { dev_block_free (p); }
+
+int
+lgh_hip_dev_put (void *dst, const void *src, int64_t bytes)
+{
+  LGH_TRY (dev_init ());
+  LGH_CHECK (bytes >= 0 && is_device_ptr (dst), 8, "lgh_hip_dev_put: dst must be device memory");
+  return h2d (dst, src, (size_t) bytes);
+}
+
+int
+lgh_hip_dev_get (void *dst, const void *src, int64_t bytes)
+{
+  LGH_TRY (dev_init ());
+  LGH_CHECK (bytes >= 0 && is_device_ptr (src), 8, "lgh_hip_dev_get: src must be device memory");
+  return d2h (dst, src, (size_t) bytes);
+}
+
 int
 lgh_hip_copy_gbps (int64_t bytes, int reps, double *gbps)
 {

@@ -543,6 +543,44 @@ dcopy (void *dst, const void *src, size_t bytes)
   return 0;
 }
 
+int
+dcopy2d (void *dst, size_t dpitch, const void *src, size_t spitch,
+         size_t width, size_t height)
+{
+  if (!width || !height) return 0;
+  if (dpitch == width && spitch == width) return dcopy (dst, src, width * height);
+  LGH_HIP (hipMemcpy2DAsync (dst, dpitch, src, spitch, width, height,
+                             hipMemcpyDeviceToDevice, stream ()));
+  return 0;
+}
+
+int
+dev_block_alloc (size_t bytes, void **p)
+{
+  *p = nullptr;
+  LGH_TRY (dev_init ());
+  LGH_CHECK (bytes > 0, 8, "device block: size must be positive");
+  LGH_HIP (hipMalloc (p, bytes));
+  return 0;
+}
+
+void
+dev_block_free (void *p)
+{
+  if (p) (void) hipFree (p);
+}
+
+bool
+is_device_ptr (const void *p)
+{
+  hipPointerAttribute_t a;
+  if (!p || hipPointerGetAttributes (&a, p) != hipSuccess) {
+    (void) hipGetLastError ();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
 // ---- CSR / panel matrices -------------------------------------------------
 void
 GroupCopy::release ()

@@ -839,7 +839,8 @@ coeffs (lgh_glr_t *g, const OpSpec &op, int m, const double *c,
 // arithmetic does not depend on how many columns share the call.
 static int
 run_op (lgh_glr_t *g, const OpSpec &op, int m, const double *c,
-        const double *in, int ldi, double *out, int ldo, int tier = -1)
+        const double *in, int ldi, double *out, int ldo, int tier = -1,
+        bool dev = false)
 {
   lgh_prior_t *p = g->prior;
   const bool can_block = p->is_mat && p->o.blocked_mode >= 2;
@@ -856,20 +857,23 @@ run_op (lgh_glr_t *g, const OpSpec &op, int m, const double *c,
   std::vector<double> fk, fc;
   coeffs (g, op, m, c, fk, fc);
   PMat H, A, Bm, X64, Y64;
-  // contiguous host blocks (ld == n) move directly; others through hb
+  // contiguous host blocks (ld == n) move directly; others through hb;
+  // device blocks (dev) by pitched device copies
   std::vector<double> hb;
   const double *hin = in;
-  if (ldi != n || ldo != n) hb.resize ((size_t) n * m);
-  if (ldi != n) {
+  if (!dev && (ldi != n || ldo != n)) hb.resize ((size_t) n * m);
+  if (!dev && ldi != n) {
     for (int j = 0; j < m; j++) memcpy (&hb[(size_t) j * n], in + (size_t) j * ldi, sizeof (double) * n);
     hin = hb.data ();
   }
+  const size_t colb = sizeof (double) * (size_t) n;
   int rc = pmat_alloc (H, n, m, 1, F64);
   if (!rc) rc = pmat_alloc (A, n, m, Wz, F64);
   if (!rc) rc = pmat_alloc (Bm, n, m, Wz, F64);
   if (!rc) rc = pmat_alloc (X64, n, m, 64, F64);
   if (!rc) rc = pmat_alloc (Y64, n, m, 64, F64);
-  if (!rc) rc = h2d (H.p, hin, sizeof (double) * (size_t) n * m);
+  if (!rc) rc = dev ? dcopy2d (H.p, colb, in, sizeof (double) * (size_t) ldi, colb, m)
+                    : h2d (H.p, hin, colb * m);
   if (!rc) rc = k_copy_cols (H, 0, A, 0, m);
   // left part (before the filter): into X64
   if (!rc) {
@@ -908,11 +912,15 @@ run_op (lgh_glr_t *g, const OpSpec &op, int m, const double *c,
       break;
     }
   }
-  double *hout = (ldo == n) ? out : hb.data ();
-  if (!rc) rc = d2h (hout, H.p, sizeof (double) * (size_t) n * m);
+  double *hout = (ldo == n || dev) ? out : hb.data ();
+  if (!rc && dev) {
+    rc = dcopy2d (out, sizeof (double) * (size_t) ldo, H.p, colb, colb, m);
+    if (!rc) rc = dev_sync ();
+  } else if (!rc)
+    rc = d2h (hout, H.p, colb * m);
   pmat_free (H); pmat_free (A); pmat_free (Bm); pmat_free (X64); pmat_free (Y64);
   LGH_TRY (rc);
-  if (ldo != n)
+  if (!dev && ldo != n)
     for (int j = 0; j < m; j++) memcpy (out + (size_t) j * ldo, &hb[(size_t) j * n], sizeof (double) * n);
   return 0;
 }
@@ -1081,6 +1089,41 @@ lgh_glr_sample_block (lgh_glr_t *g, int m, const double *c, const double *xi,
   if (run_op (g, OpSpec { 3, 0, nullptr, 0 }, m, c, xi, ldr, x, ldx))
     return API_FAIL ("lgh_glr_sample_block");
   return 0;
+}
+
+// the two block operations on device-resident blocks: the same run_op
+// (same kernels, same bits per column) with pitched device copies in place
+// of the host transfers
+static PetscErrorCode
+block_dev (lgh_glr_t *g, int kind, int m, const double *c, const double *in,
+           int ldi, double *out, int ldo, const char *name)
+{
+  PetscCheck (m >= 0 && ldi >= g->n && ldo >= g->n, 0, PETSC_ERR_ARG_SIZ,
+              "%s: need m >= 0 and leading dimensions >= n = %d", name, g->n);
+  if (m == 0) return 0;
+  PetscCheck (c && in && out, 0, PETSC_ERR_ARG_NULL, "%s: null argument", name);
+  PetscCheck (lgh::is_device_ptr (in) && lgh::is_device_ptr (out), 0,
+              PETSC_ERR_ARG_WRONG, "%s: rhs and x must be device memory", name);
+  for (int j = 0; j < m; j++)
+    PetscCheck (c[j] > g->floor, 0, PETSC_ERR_ARG_OUTOFRANGE,
+                "%s: shift c[%d] must exceed %g", name, j, g->floor);
+  if (run_op (g, OpSpec { kind, 0, nullptr, 0 }, m, c, in, ldi, out, ldo, -1, true))
+    return API_FAIL (name);
+  return 0;
+}
+
+extern "C" int
+lgh_glr_solve_block_dev (lgh_glr_t *g, int m, const double *c,
+                         const double *rhs, int ldr, double *x, int ldx)
+{
+  return block_dev (g, 0, m, c, rhs, ldr, x, ldx, "lgh_glr_solve_block_dev");
+}
+
+extern "C" int
+lgh_glr_sample_block_dev (lgh_glr_t *g, int m, const double *c,
+                          const double *xi, int ldr, double *x, int ldx)
+{
+  return block_dev (g, 3, m, c, xi, ldr, x, ldx, "lgh_glr_sample_block_dev");
 }
 
 // U = Q V of an implicit object into a new n x kept matrix

@@ -95,6 +95,16 @@ int dzero (void *dst, size_t bytes);   // fill kernel (graph-safe)
 // best of reps launches after a warm-up
 int copy_gbps (size_t bytes, int reps, double *gbps);
 int dcopy (void *dst, const void *src, size_t bytes);
+// device-to-device copy of height rows of width bytes between pitched
+// blocks (columns of a column-major matrix with leading dimensions)
+int dcopy2d (void *dst, size_t dpitch, const void *src, size_t spitch,
+             size_t width, size_t height);
+// true when p is device (or managed) memory of the HIP runtime
+bool is_device_ptr (const void *p);
+// plain device blocks outside the caching allocator (a caller's own
+// device memory, for the device-resident entry points)
+int  dev_block_alloc (size_t bytes, void **p);
+void dev_block_free (void *p);
 
 // ---- live kernel timing (HIP events on the engine stream) ---------------
 // Launchers bracket their kernel with events when profiling is enabled and

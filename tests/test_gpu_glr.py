@@ -396,6 +396,46 @@ def test_grid_glr_matches_oracle(dev, grid, oracle, policy):
 
 
 @pytest.mark.parametrize("mixed", [False, True])
+def test_grid_block_ops_on_device(dev, grid, oracle, mixed):
+    """lgh_glr_solve_block_dev / lgh_glr_sample_block_dev on blocks already
+    in device memory: the bits of the host-block calls (a whole panel and
+    an 8-wide tail, one column, columns with a leading dimension above n),
+    the plain build's columns against the oracle, host memory refused."""
+    import ctypes as C
+    n = grid["Z"].shape[0]
+    ell = 24
+    g = dev.GLR(grid["B"], grid["prior"], dev.glr_opts(ell=ell, seed=0),
+                x=dev.xopts(mixed=mixed))
+    R = oracle.randn_fill(14, 0, 0, n, 70)
+    cs = 10.0 ** np.linspace(-3, 0, 70)
+    Rd = dev.DeviceBlock.from_host(R)
+    Rs = dev.DeviceBlock.from_host(R, ld=n + 5)
+    assert np.array_equal(Rs.to_host(), R) and np.array_equal(Rd.to_host(), R)
+    for op in ("solve", "sample"):
+        host = getattr(g, op + "_block")(cs, R)
+        dev_op = getattr(g, op + "_block_device")
+        assert np.array_equal(dev_op(cs, Rd).to_host(), host), op
+        assert np.array_equal(dev_op(cs, Rs).to_host(), host), op
+        assert np.array_equal(dev_op(cs[3:4], Rs.columns(3, 4)).to_host(),
+                              getattr(g, op + "_block")(cs[3:4], R[:, 3:4])), op
+    if not mixed:
+        og = oracle.GLR(oracle.CSR.from_scipy(grid["B"]), grid["M"], grid["zs"],
+                        ell=ell, q_power=1, trunc_rel=1e-3, seed=0)
+        X = g.solve_block_device(cs[:3], Rd.columns(0, 3)).to_host()
+        for j in range(3):
+            assert rel(X[:, j], og.solve(cs[j], R[:, j])) < 1e-9
+    # host pointers are refused before any device work
+    Y = np.zeros((n, 2), order="F")
+    cc = np.array([1e-2, 1e-2])
+    rc = dev.lib().lgh_glr_solve_block_dev(
+        g.h, 2, cc.ctypes.data_as(C.POINTER(C.c_double)), R.ctypes.data, n,
+        Y.ctypes.data, n)
+    assert rc != 0 and "device memory" in dev.lib().lgh_petsc_last_error().decode()
+    with pytest.raises(ValueError):
+        g.solve_block_device(cs[:2], R[:, :2])
+
+
+@pytest.mark.parametrize("mixed", [False, True])
 def test_grid_sweep_chunks_bitwise(dev, grid, mixed):
     """F sweeps in chunks of 1 or 2 panels (xopts.sweep_panels) give the
     same object bit for bit as the whole block at once: panels are solved
