@@ -36,7 +36,8 @@ void        lgh_hip_trim (void);
  * HBM bytes and flops.  kind: 0 B SpMM of the F sweep, 1 other SpMM,
  * 2 Jacobi Chebyshev steps and smoothers below the finest AMG level,
  * 3 fused Chebyshev step (explicit Z), 4 residual below the finest level,
- * 5 Gram (MFMA), 6 GEMM (MFMA), 7 other, 8 mixed-precision Chebyshev step,
+ * 5 Gram (MFMA), 6 GEMM (MFMA), 7 other, 8 mixed-precision Chebyshev step
+ * after the residual replacement (FP32 R, FP64 X),
  * 9 / 10 fused smoother on the finest level (pre: zero guess / post),
  * 11 residual on the finest level.  collect synchronizes and resets that
  * kind's totals; np >= 0 counts only launches over np 64-column panels,

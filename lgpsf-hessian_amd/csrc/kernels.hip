@@ -1957,16 +1957,16 @@ cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
 // writes R in SP for the next preconditioner application.  (A BF16
 // direction saves 4 of ~44 B per element but its 2^-9 rounding, repeated
 // over the 25 steps, costs the solve a factor ~2 in accuracy.)
-// RI, RO: storage of the input residual and of X and the new residual
-// (double; float in the early phase of a solve, see cheb_mixed_run), the
-// arithmetic is FP64 either way.
+// RI, RO, XT: storage of the input residual, the new residual and X
+// (double; float in a solve's early phase, and R after its residual
+// replacement, see cheb_mixed_run); the arithmetic is FP64 either way.
 template <bool HASD, typename SP, int W = 64, typename RI = double,
-          typename RO = double>
+          typename RO = double, typename XT = RO>
 __global__ void __launch_bounds__ (TPB)
 cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
            const double *__restrict__ v, const SP *__restrict__ Zs,
            double a, double b, const float *__restrict__ D,
-           const RI *__restrict__ R, RO *__restrict__ X, int xmode,
+           const RI *__restrict__ R, XT *__restrict__ X, int xmode,
            float *__restrict__ Dn, RO *__restrict__ Rn,
            SP *__restrict__ Rpc)
 {
@@ -2053,12 +2053,12 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
 // values gathered and widened are X's values), and Rpc = R in the
 // V-cycle's storage precision -- in place of a widening copy, an FP64
 // residual that reads X back, and a narrowing copy of R
-template <typename SP, int W = 64>
+template <typename SP, int W = 64, typename RT = double>
 __global__ void __launch_bounds__ (TPB)
 mix_switch_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
               const double *__restrict__ v, const double *__restrict__ B,
               const float *__restrict__ X32, double *__restrict__ X,
-              double *__restrict__ R, SP *__restrict__ Rpc)
+              RT *__restrict__ R, SP *__restrict__ Rpc)
 {
   LGH_ROWMAP (double)
   double acc[N];
@@ -2103,7 +2103,7 @@ k_mix_switch (const DevCSR &A, const PMat &B, const PMat &X32, PMat &X,
 {
   const int W = B.W;
   LGH_CHECK (A.prec == F64 && B.prec == F64 && X32.prec == F32 && X.prec == F64
-             && R.prec == F64 && (Rpc.prec == F32 || Rpc.prec == BF16)
+             && (R.prec == F64 || R.prec == F32) && (Rpc.prec == F32 || Rpc.prec == BF16)
              && row_width (W) && X32.W == W && X.W == W && R.W == W && Rpc.W == W
              && B.nrows == A.n && A.n == A.m && X32.nrows == A.n && X.nrows == A.n
              && R.nrows == A.n && Rpc.nrows == A.n
@@ -2112,23 +2112,24 @@ k_mix_switch (const DevCSR &A, const PMat &B, const PMat &X32, PMat &X,
              8, "k_mix_switch: shape/precision mismatch");
   const double vec = (double) A.n * W;
   ProfScope ps (PK_OTHER, B.npanels () * (csr_bytes (A)
-                + vec * (8 + 4 + 8 + 8 + prec_bytes (Rpc.prec))),
+                + vec * (8 + 4 + 8 + prec_bytes (R.prec) + prec_bytes (Rpc.prec))),
                 B.npanels () * 2.0 * (double) A.nnz * W, B.npanels (), Rpc.prec, W);
   const dim3 g = grid64 (A.n, B.npanels (), F64, W);
-  return with_row_W (W, [&] (auto wc) -> int {
+  auto go = [&] (auto sp, auto wc, auto rt) -> int {
+    using SP = decltype (sp);
+    using RT = decltype (rt);
     constexpr int WW = decltype (wc)::value;
-    if (Rpc.prec == BF16)
-      hipLaunchKernelGGL ((mix_switch_k<bf16_t, WW>), g, dim3 (TPB), 0, stream (),
-                          A.n, A.rp, A.ci, (const double *) A.v, (const double *) B.p,
-                          (const float *) X32.p, (double *) X.p, (double *) R.p,
-                          (bf16_t *) Rpc.p);
-    else
-      hipLaunchKernelGGL ((mix_switch_k<float, WW>), g, dim3 (TPB), 0, stream (),
-                          A.n, A.rp, A.ci, (const double *) A.v, (const double *) B.p,
-                          (const float *) X32.p, (double *) X.p, (double *) R.p,
-                          (float *) Rpc.p);
+    hipLaunchKernelGGL ((mix_switch_k<SP, WW, RT>), g, dim3 (TPB), 0, stream (),
+                        A.n, A.rp, A.ci, (const double *) A.v, (const double *) B.p,
+                        (const float *) X32.p, (double *) X.p, (RT *) R.p,
+                        (SP *) Rpc.p);
     LGH_LAUNCH_CHECK ();
     return 0;
+  };
+  return with_row_W (W, [&] (auto wc) -> int {
+    if (Rpc.prec == BF16)
+      return R.prec == F64 ? go (bf16_t (), wc, 0.0) : go (bf16_t (), wc, 0.f);
+    return R.prec == F64 ? go (0.f, wc, 0.0) : go (0.f, wc, 0.f);
   });
 }
 
@@ -2138,9 +2139,13 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                    PMat &Dn32, PMat &Rn, PMat &Rnpc)
 {
   const int W = R.W;
-  LGH_CHECK (A.prec == F64 && (R.prec == F64 || R.prec == F32)
-             && (X.prec == F64 || X.prec == F32) && Rn.prec == X.prec
-             && (R.prec == X.prec || R.prec == F64) && row_width (W)
+  // storage (R, Rn, X): all FP64; FP64, FP32, FP32 (a solve's first
+  // step); all FP32 (its early phase); FP32, FP32, FP64 (after the
+  // residual replacement)
+  const bool combo = (R.prec == F64 && Rn.prec == F64 && X.prec == F64)
+                     || (R.prec == F64 && Rn.prec == F32 && X.prec == F32)
+                     || (R.prec == F32 && Rn.prec == F32);
+  LGH_CHECK (A.prec == F64 && combo && row_width (W)
              && X.W == W && Zpc.W == W && Dn32.W == W && Rn.W == W
              && Rnpc.W == W && (!D32 || D32->W == W)
              && (Zpc.prec == F32 || Zpc.prec == BF16) && Rnpc.prec == Zpc.prec
@@ -2151,44 +2156,48 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
              && (D32 || (x_mode != MIX_X_SET2 && x_mode != MIX_X_ADD2)), 8,
              "k_cheb_step_mixed: bad X mode");
   const double vec = (double) A.n * W;
-  const double rb = (double) prec_bytes (R.prec), ob = (double) prec_bytes (X.prec);
+  const double rb = (double) prec_bytes (R.prec), ob = (double) prec_bytes (Rn.prec);
+  const double xs = (double) prec_bytes (X.prec);
   const double xb = (x_mode == MIX_X_SKIP) ? 0 : (x_mode == MIX_X_SET
-                     || x_mode == MIX_X_SET2) ? ob : 2 * ob;
+                     || x_mode == MIX_X_SET2) ? xs : 2 * xs;
   const double pb = (double) prec_bytes (Zpc.prec);
-  // the first step of a solve (no D) and the early-phase steps (FP32 R, X)
-  // are credited to PK_OTHER so that PK_CHEB_MIXED times one kernel
-  // instantiation
-  ProfScope ps ((D32 && X.prec == F64) ? PK_CHEB_MIXED : PK_OTHER,
+  // PK_CHEB_MIXED times one kernel instantiation: the steps after the
+  // residual replacement (FP32 R, FP64 X); the first step of a solve (no
+  // D), the early phase (FP32 X) and solves of fewer than 8 steps (FP64 R)
+  // are credited to PK_OTHER
+  ProfScope ps ((D32 && R.prec == F32 && X.prec == F64) ? PK_CHEB_MIXED : PK_OTHER,
                 R.npanels () * (csr_bytes (A)
                 + vec * (pb + (D32 ? 4 : 0) + rb + xb + 4 + ob + pb)),
                 R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * W,
                 R.npanels (), Zpc.prec, W);
   const dim3 g = grid64 (A.n, R.npanels (), F32, W);   // 4 columns per lane
-  auto go = [&] (auto sp, auto wc, auto ri, auto ro) -> int {
+  auto go = [&] (auto sp, auto wc, auto ri, auto ro, auto xt) -> int {
     using SP = decltype (sp);
     using RI = decltype (ri);
     using RO = decltype (ro);
+    using XT = decltype (xt);
     constexpr int WW = decltype (wc)::value;
     if (D32)
-      hipLaunchKernelGGL ((cheb64m_k<true, SP, WW, RI, RO>), g, dim3 (TPB), 0,
+      hipLaunchKernelGGL ((cheb64m_k<true, SP, WW, RI, RO, XT>), g, dim3 (TPB), 0,
                           stream (), A.n, A.rp, A.ci, (const double *) A.v,
                           (const SP *) Zpc.p, a, b, (const float *) D32->p,
-                          (const RI *) R.p, (RO *) X.p, x_mode,
+                          (const RI *) R.p, (XT *) X.p, x_mode,
                           (float *) Dn32.p, (RO *) Rn.p, (SP *) Rnpc.p);
     else
-      hipLaunchKernelGGL ((cheb64m_k<false, SP, WW, RI, RO>), g, dim3 (TPB), 0,
+      hipLaunchKernelGGL ((cheb64m_k<false, SP, WW, RI, RO, XT>), g, dim3 (TPB), 0,
                           stream (), A.n, A.rp, A.ci, (const double *) A.v,
                           (const SP *) Zpc.p, a, b, nullptr,
-                          (const RI *) R.p, (RO *) X.p, x_mode,
+                          (const RI *) R.p, (XT *) X.p, x_mode,
                           (float *) Dn32.p, (RO *) Rn.p, (SP *) Rnpc.p);
     LGH_LAUNCH_CHECK ();
     return 0;
   };
   return with_row_W (W, [&] (auto wc) {
     auto by_r = [&] (auto sp) {
-      if (X.prec == F64) return go (sp, wc, 0.0, 0.0);
-      if (R.prec == F64) return go (sp, wc, 0.0, 0.f);
-      return go (sp, wc, 0.f, 0.f);
+      if (R.prec == F64 && X.prec == F64) return go (sp, wc, 0.0, 0.0, 0.0);
+      if (R.prec == F64) return go (sp, wc, 0.0, 0.f, 0.f);
+      if (X.prec == F64) return go (sp, wc, 0.f, 0.f, 0.0);
+      return go (sp, wc, 0.f, 0.f, 0.f);
     };
     return Zpc.prec == BF16 ? by_r (bf16_t ()) : by_r (0.f);
   });

@@ -119,7 +119,8 @@ enum ProfKind {
   PK_GRAM = 5,
   PK_GEMM = 6,
   PK_OTHER = 7,
-  PK_CHEB_MIXED = 8,  // mixed-precision Chebyshev step (FP64 R, X)
+  PK_CHEB_MIXED = 8,  // mixed-precision Chebyshev step after the
+                      // residual replacement (FP32 R, FP64 X)
   PK_SM_PRE = 9,      // fused smoother, finest level, zero initial guess
   PK_SM_POST = 10,    // fused smoother, finest level, X += correction
   PK_RESID_FINE = 11, // residual on the finest AMG level
@@ -300,10 +301,11 @@ int k_resid (const DevCSR &A, const PMat &B, const PMat &X, PMat &R);
 int k_cheb_step (const DevCSR &A, int jacobi, const PMat *Zsrc, double a,
                  double b, const PMat *D, const PMat &R, PMat &X, int x_mode,
                  PMat &Dn, PMat &Rn, PMat *Rpc = nullptr);
-// mixed step at W = 64: FP64 R, X (A values FP64), FP32 D; the
-// preconditioned residual Zpc and the copy Rnpc of the new residual (the
-// next preconditioner input) are FP32 or BF16.  The new direction is
-// rounded to FP32 once and X and R move by exactly the stored value.
+// mixed step: FP64 arithmetic (A values FP64), R and X stored in FP64 or
+// FP32, FP32 D; the preconditioned residual Zpc and the copy Rnpc of the
+// new residual (the next preconditioner input) are FP32 or BF16.  The new
+// direction is rounded to FP32 once and X and R move by exactly the stored
+// value.
 // x_mode: how X takes the new direction Dn (D = the previous direction)
 enum {
   MIX_X_SET = 0,      // X = Dn
@@ -316,8 +318,9 @@ int k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                        const PMat *D32, const PMat &R, PMat &X, int x_mode,
                        PMat &Dn32, PMat &Rn, PMat &Rnpc);
 // the mixed solve's residual replacement in one pass: X = X32 widened to
-// FP64, R = B - A X (FP64, the sums of k_resid), Rpc = R in Rpc's storage
-// precision (FP32 or BF16, rounded as k_copy_cols rounds)
+// FP64, R = B - A X (formed in FP64, the sums of k_resid; stored in R's
+// precision, FP64 or FP32), Rpc = R in Rpc's storage precision (FP32 or
+// BF16, rounded from the FP64 value as k_copy_cols rounds)
 int k_mix_switch (const DevCSR &A, const PMat &B, const PMat &X32, PMat &X,
                   PMat &R, PMat &Rpc);
 // degree-2 Chebyshev-Jacobi smoother in one pass (W = 64):

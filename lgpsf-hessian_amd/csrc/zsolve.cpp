@@ -513,8 +513,6 @@ ChebMixedWork::ensure (const PMat &like, int pc)
   pmat_free (Rpc); pmat_free (Zpc);
   pmat_free (R32[0]); pmat_free (R32[1]); pmat_free (X32);
   const int nc = like.npanels () * like.W;
-  LGH_TRY (pmat_alloc (R[0], like.nrows, nc, like.W, F64));
-  LGH_TRY (pmat_alloc (R[1], like.nrows, nc, like.W, F64));
   LGH_TRY (pmat_alloc (D[0], like.nrows, nc, like.W, F32));
   LGH_TRY (pmat_alloc (D[1], like.nrows, nc, like.W, F32));
   LGH_TRY (pmat_alloc (Rpc, like.nrows, nc, like.W, pc));
@@ -529,6 +527,16 @@ ChebMixedWork::ensure (const PMat &like, int pc)
   return 0;
 }
 
+int
+ChebMixedWork::ensure64 ()
+{
+  if (R[0].p && R[0].W == W && R[0].npanels () == np && R[0].nrows == nrows)
+    return 0;
+  pmat_free (R[0]); pmat_free (R[1]);
+  LGH_TRY (pmat_alloc (R[0], nrows, np * W, W, F64));
+  return pmat_alloc (R[1], nrows, np * W, W, F64);
+}
+
 static int
 cheb_mixed_run (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
                 int nit, const PMat &B, PMat &X, ChebMixedWork &w)
@@ -539,8 +547,18 @@ cheb_mixed_run (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
   // the early phase: steps 1..k_sw with FP32 R and X (k_sw even, so that X
   // holds every direction when it ends).  Half the steps: a longer phase
   // (ending when the Chebyshev bound has shrunk the error by 2^-25, 10 of 16
-  // steps here) moved the smallest kept modes of a 16^3 build by 1e-7
+  // steps here) moved the smallest kept modes of a 16^3 build by 1e-7.
+  // Then the residual replacement: X in FP64 and R = B - A X formed in
+  // FP64; the remaining steps keep X in FP64 and R in FP32.  R's rounding
+  // there is relative to the residual the early phase left (about the
+  // Chebyshev bound at k_sw times B), so the drift it adds to the
+  // recurrence stays far below the solve's tolerance (1024^2 anisotropic
+  // PSF build, 16 steps: spectrum and solves still within 1.4e-11 and
+  // 3e-11 of FP64).  X stays FP64:
+  // an FP32 correction accumulated after the replacement puts its rounding
+  // straight into the solution (a 16^3 build extended twice moved 5x more).
   const int k_sw = (nit >= 8) ? 2 * (nit / 4) : 0;
+  if (!k_sw) LGH_TRY (w.ensure64 ());
   const PMat *Rsrc = &B;
   int ri = 0, di = 0;
   for (int k = 1; k <= nit; k++) {
@@ -555,7 +573,7 @@ cheb_mixed_run (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
     else xm = (k == 1) ? MIX_X_SET : MIX_X_ADD;
     const bool early = k <= k_sw;
     PMat &Xk = early ? w.X32 : X;
-    PMat &Rn = early ? w.R32[ri] : w.R[ri];
+    PMat &Rn = k_sw ? w.R32[ri] : w.R[ri];
     LGH_TRY (mg_vcycle (mg, w.Rpc, w.Zpc));
     LGH_TRY (k_cheb_step_mixed (A64, w.Zpc, a, b, Dcur, *Rsrc, Xk, xm,
                                 w.D[di], Rn, w.Rpc));
@@ -563,10 +581,10 @@ cheb_mixed_run (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
     ri ^= 1;
     di ^= 1;
     if (k == k_sw) {
-      // residual replacement: X in FP64, R = B - A X in FP64, and the next
-      // preconditioner input from it (one pass)
-      LGH_TRY (k_mix_switch (A64, B, w.X32, X, w.R[ri], w.Rpc));
-      Rsrc = &w.R[ri];
+      // residual replacement: X = X32 in FP64, R = B - A X in FP64 (stored
+      // in FP32), and the next preconditioner input from it (one pass)
+      LGH_TRY (k_mix_switch (A64, B, w.X32, X, w.R32[ri], w.Rpc));
+      Rsrc = &w.R32[ri];
       ri ^= 1;
     }
   }

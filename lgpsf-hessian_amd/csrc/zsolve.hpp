@@ -72,12 +72,15 @@ int cheb_solve (const DevCSR &A, DevMG *mg, double lmin, double lmax,
 // early rounding by their convergence factor) run with FP64 R and X.
 struct ChebMixedWork {
   int W = 0, np = -1, nrows = 0, pprec = -1;
-  PMat R[2], D[2], Rpc, Zpc;
-  PMat R32[2], X32;                    // the early phase's FP32 R and X
+  PMat R[2], D[2], Rpc, Zpc;           // R: FP64 residuals (solves of fewer
+                                       // than 8 steps only; made on demand)
+  PMat R32[2], X32;                    // FP32 R (every step of a solve of 8
+                                       // or more) and X (its early phase)
   PMat Bt, Xt;                         // the narrow panels' B and X (48 columns)
   ChebMixedWork *tail = nullptr, *tail8 = nullptr;
   ~ChebMixedWork ();
   int ensure (const PMat &like, int pc);
+  int ensure64 ();                     // R[0], R[1] at the current shape
 };
 int cheb_solve_mixed (const DevCSR &A64, DevMG &mg, double lmin,
                       double lmax, int nit, const PMat &B, PMat &X,
