@@ -65,11 +65,13 @@ int cheb_solve (const DevCSR &A, DevMG *mg, double lmin, double lmax,
                 int nit, const PMat &B, PMat &X, ChebWork &w);
 
 // FP64-accurate Chebyshev around an FP32 or BF16 V-cycle (W = 64 panels):
-// X, R in FP64; D in FP32; the V-cycle's input and output (Rpc, Zpc) in
+// FP64 arithmetic; D in FP32; the V-cycle's input and output (Rpc, Zpc) in
 // its own precision (k_cheb_step_mixed).  The first steps (an even count,
 // about half) keep R and X in FP32; then X is widened, the residual
 // B - A X is formed again in FP64 and the remaining steps (which damp the
-// early rounding by their convergence factor) run with FP64 R and X.
+// early rounding by their convergence factor) run with FP64 X and R
+// stored in FP32 (its rounding is relative to the residual left by then).
+// Solves of fewer than 8 steps keep FP64 R and X throughout.
 struct ChebMixedWork {
   int W = 0, np = -1, nrows = 0, pprec = -1;
   PMat R[2], D[2], Rpc, Zpc;           // R: FP64 residuals (solves of fewer

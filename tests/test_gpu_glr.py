@@ -312,6 +312,24 @@ def test_grid_blocked_solve_matches_oracle(dev, grid, oracle):
     assert rel(grid["prior"].solve(X, which=0, prec=4), ref) < 1e-4
 
 
+def test_grid_mixed_short_solve(dev, grid):
+    """A loose Chebyshev tolerance gives solves of fewer than 8 steps, which
+    the mixed tier runs with FP64 R and X throughout (no early FP32 phase):
+    they reach the same residual as the FP64 tier's solve of that length."""
+    n = grid["Z"].shape[0]
+    p = dev.Prior.from_mat(grid["Z"], grid["M"], blocked_mode=3, cheb_rtol=1e-3)
+    nit = p.info()["nit"]
+    assert 1 <= nit < 8, nit
+    X = dev.randn_fill(8, 0, 0, n, 9)
+    ref = p.solve(X, which=0)
+    r64 = rel(grid["Z"] @ ref, X)
+    assert r64 < 1e-2
+    for prec in (2, 3):
+        Y = p.solve(X, which=0, prec=prec)
+        assert rel(grid["Z"] @ Y, X) < 3 * r64, prec
+        assert rel(Y, ref) < 0.1, prec
+
+
 @pytest.mark.parametrize("policy", ["plain", "mixed", "mixed-fp32-vcycle",
                                     "mixed-fp32-sketch"])
 def test_grid_glr_matches_oracle(dev, grid, oracle, policy):
@@ -433,6 +451,15 @@ def test_grid_block_ops_on_device(dev, grid, oracle, mixed):
     assert rc != 0 and "device memory" in dev.lib().lgh_petsc_last_error().decode()
     with pytest.raises(ValueError):
         g.solve_block_device(cs[:2], R[:, :2])
+    # a leading dimension below n and a shift at the floor are refused on
+    # the host; no columns is a no-op
+    pc = cc.ctypes.data_as(C.POINTER(C.c_double))
+    assert dev.lib().lgh_glr_solve_block_dev(g.h, 2, pc, Rd.ptr, n - 1, Rd.ptr, n) != 0
+    assert "leading dimensions" in dev.lib().lgh_petsc_last_error().decode()
+    bad = np.array([0.0, 1e-2])
+    with pytest.raises(dev.LGHError):
+        g.solve_block_device(bad, Rd.columns(0, 2))
+    assert dev.lib().lgh_glr_sample_block_dev(g.h, 0, pc, Rd.ptr, n, Rd.ptr, n) == 0
 
 
 @pytest.mark.parametrize("mixed", [False, True])
