@@ -68,6 +68,9 @@ run_groups (const std::vector<std::vector<int>> &groups, int workers,
   Failure first;
   auto work = [&] () {
     set_aux_overlap (false);             // the other workers fill the device
+    // a block's partial last panel rides with the whole ones: the batch is
+    // bound by launches, not bytes (256 uneven 2D instances: 2.59 -> 2.42 s)
+    set_tail_split (false);
     for (;;) {
       const int gi = next.fetch_add (1);
       if (gi >= (int) groups.size ()) break;

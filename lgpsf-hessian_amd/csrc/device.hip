@@ -105,6 +105,9 @@ void *rocblas_h () { (void) stream (); return (void *) t_ctx.rb; }
 // stream: the other instances fill the device
 static thread_local bool t_no_aux = false;
 void set_aux_overlap (bool on) { t_no_aux = !on; }
+static thread_local bool t_no_tail = false;
+void set_tail_split (bool on) { t_no_tail = !on; }
+bool tail_split () { return !t_no_tail; }
 
 hipStream_t
 aux_stream ()

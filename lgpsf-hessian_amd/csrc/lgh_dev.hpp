@@ -65,6 +65,11 @@ hipStream_t stream ();
 hipStream_t aux_stream ();
 // this thread's aux_stream () returns null (no overlap) when off
 void set_aux_overlap (bool on);
+// when off, this thread's Z-solves run a block's partial last panel as a
+// whole 64-wide panel with the others instead of as narrow panels of its
+// own (the same bits per column either way; fewer launches, more bytes)
+void set_tail_split (bool on);
+bool tail_split ();
 // while alive, the thread's engine calls enqueue on s instead of its
 // stream (allocator free lists follow: blocks freed here are reused on s)
 struct StreamScope {

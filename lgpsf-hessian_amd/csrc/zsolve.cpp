@@ -360,7 +360,7 @@ split_tail (const PMat &B, PMat &X, WK &w, bool *done, F &&run)
 {
   *done = false;
   const int full = B.ncols / 64, tail = B.ncols - 64 * full;
-  if (B.W != 64 || tail == 0 || tail > 48) return 0;
+  if (B.W != 64 || tail == 0 || tail > 48 || !tail_split ()) return 0;
   LGH_CHECK (X.W == 64 && X.ncols == B.ncols && X.prec == B.prec, 8,
              "Z-solve: X shape differs from B");
   *done = true;
