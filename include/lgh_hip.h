@@ -57,7 +57,8 @@ int lgh_hip_copy_gbps (int64_t bytes, int reps, double *gbps);
  * no GPU array of their own (the device-resident lgh_glr_*_block_dev
  * calls take such blocks): alloc returns NULL on failure (message in
  * lgh_hip_last_error); put / get are synchronous host <-> device copies,
- * refusing a device side that is not device memory.  (No reference
+ * refusing a device side that is not device memory or is shorter than the
+ * copy (one allocation must hold [ptr, ptr + bytes)).  (No reference
  * counterpart: the reference's blocks are host PETSc objects.) */
 void *lgh_hip_dev_alloc (int64_t bytes);
 void  lgh_hip_dev_free (void *p);

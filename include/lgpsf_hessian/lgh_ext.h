@@ -72,7 +72,8 @@ int lgh_glr_sample_block (lgh_glr_t *glr, int m, const double *c,
 /* The same two operations on blocks already in device memory, for callers
  * whose right-hand sides live in HBM (no host transfer): rhs / xi and x
  * are device pointers (column-major, leading dimensions >= n; host memory
- * is refused with PETSC_ERR_ARG_WRONG), c a host array of m shifts.  The
+ * is refused with PETSC_ERR_ARG_WRONG, a block that extends past its
+ * device allocation with PETSC_ERR_ARG_SIZ), c a host array of m shifts.  The
  * engine reads the input on its own stream, so the work that wrote it must
  * have finished (synchronize the producing stream first); x is complete
  * when the call returns.  The same bits as lgh_glr_solve_block /
@@ -87,11 +88,12 @@ int lgh_glr_sample_block_dev (lgh_glr_t *glr, int m, const double *c,
  * call with argument i (lgh_prior_create_mat, lgh_glr_compute_x): the
  * same kernels in the same order, on `workers` host threads (<= 0: 8),
  * each driving the GPU on its own stream, so the kernels of small
- * instances overlap.  The engine's own kernels reduce in fixed orders and
- * sequential calls are bit-reproducible; under concurrent load a few
- * results have come out rounded differently (~1e-12 relative, in a few
- * of 256 instances; the library dense factorizations are the suspected
- * source).  Instances that share an object (Z, mass, B or prior) run one
+ * instances overlap.  Every kernel reduces in a fixed order and the
+ * Cholesky and triangular inverse are the engine's own, so each instance's
+ * result equals the single call's bit for bit (a block's partial last
+ * panel is solved with the whole panels here, as narrow panels in the
+ * single call: a column's bits do not depend on its panel's width).
+ * Instances that share an object (Z, mass, B or prior) run one
  * after another on one thread.  opts: one per instance or NULL (defaults);
  * xopts: shared or NULL.  On failure every output is destroyed and set to
  * NULL, and the error names the lowest failing instance. */
@@ -129,11 +131,12 @@ int lgh_prior_solve_host (lgh_prior_t *prior, int which, int prec, int m,
  * sample / logdet requests (shift c, m right-hand sides, column-major host
  * blocks that stay valid until the wait returns); batches of the oldest
  * request's operation are formed from everything queued (each column with
- * its own shift) whenever one of two pinned staging blocks is free, so
+ * its own shift) into pinned staging blocks (two per device thread), so
  * requests that arrive after a batch is formed join the next one.  One
- * thread copies a batch's columns in, one runs it on the device, one copies
- * the results out: the copies of one batch overlap the device work of the
- * other.  The batch width is capped by a device workspace
+ * thread copies a batch's columns in, device_streams threads run batches
+ * on the device at once (each on its own streams), one copies the results
+ * out: the copies of one batch overlap the device work of the others.  The
+ * batch width is capped by a device workspace
  * budget; submit blocks while more than queue_cols columns are queued.
  * A request's result does not depend on what it was batched with.  While a
  * server exists, the GLR must not be used or destroyed by other calls.

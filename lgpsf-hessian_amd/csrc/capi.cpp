@@ -79,7 +79,8 @@ int
 lgh_hip_dev_put (void *dst, const void *src, int64_t bytes)
 {
   LGH_TRY (dev_init ());
-  LGH_CHECK (bytes >= 0 && is_device_ptr (dst), 8, "lgh_hip_dev_put: dst must be device memory");
+  LGH_CHECK (bytes >= 0 && device_range_ok (dst, (size_t) bytes), 8,
+             "lgh_hip_dev_put: dst must be device memory holding bytes");
   return h2d (dst, src, (size_t) bytes);
 }
 
@@ -87,7 +88,8 @@ int
 lgh_hip_dev_get (void *dst, const void *src, int64_t bytes)
 {
   LGH_TRY (dev_init ());
-  LGH_CHECK (bytes >= 0 && is_device_ptr (src), 8, "lgh_hip_dev_get: src must be device memory");
+  LGH_CHECK (bytes >= 0 && device_range_ok (src, (size_t) bytes), 8,
+             "lgh_hip_dev_get: src must be device memory holding bytes");
   return d2h (dst, src, (size_t) bytes);
 }
 

@@ -584,6 +584,19 @@ is_device_ptr (const void *p)
   return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
 
+bool
+device_range_ok (const void *p, size_t bytes)
+{
+  if (!is_device_ptr (p)) return false;
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange (&base, &size, (hipDeviceptr_t) p) != hipSuccess) {
+    (void) hipGetLastError ();
+    return false;
+  }
+  return (const char *) p + bytes <= (const char *) base + size;
+}
+
 // ---- CSR / panel matrices -------------------------------------------------
 void
 GroupCopy::release ()

@@ -1104,6 +1104,13 @@ block_dev (lgh_glr_t *g, int kind, int m, const double *c, const double *in,
   PetscCheck (c && in && out, 0, PETSC_ERR_ARG_NULL, "%s: null argument", name);
   PetscCheck (lgh::is_device_ptr (in) && lgh::is_device_ptr (out), 0,
               PETSC_ERR_ARG_WRONG, "%s: rhs and x must be device memory", name);
+  // the blocks' last columns end inside their allocations
+  const size_t span_in = sizeof (double) * ((size_t) ldi * (m - 1) + g->n);
+  const size_t span_out = sizeof (double) * ((size_t) ldo * (m - 1) + g->n);
+  PetscCheck (lgh::device_range_ok (in, span_in) && lgh::device_range_ok (out, span_out),
+              0, PETSC_ERR_ARG_SIZ,
+              "%s: an n x %d block with these leading dimensions extends past "
+              "its device allocation", name, m);
   for (int j = 0; j < m; j++)
     PetscCheck (c[j] > g->floor, 0, PETSC_ERR_ARG_OUTOFRANGE,
                 "%s: shift c[%d] must exceed %g", name, j, g->floor);

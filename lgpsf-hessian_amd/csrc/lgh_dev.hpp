@@ -106,6 +106,8 @@ int dcopy2d (void *dst, size_t dpitch, const void *src, size_t spitch,
              size_t width, size_t height);
 // true when p is device (or managed) memory of the HIP runtime
 bool is_device_ptr (const void *p);
+// true when [p, p + bytes) lies inside one device allocation
+bool device_range_ok (const void *p, size_t bytes);
 // plain device blocks outside the caching allocator (a caller's own
 // device memory, for the device-resident entry points)
 int  dev_block_alloc (size_t bytes, void **p);

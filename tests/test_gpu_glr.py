@@ -460,6 +460,13 @@ def test_grid_block_ops_on_device(dev, grid, oracle, mixed):
     with pytest.raises(dev.LGHError):
         g.solve_block_device(bad, Rd.columns(0, 2))
     assert dev.lib().lgh_glr_sample_block_dev(g.h, 0, pc, Rd.ptr, n, Rd.ptr, n) == 0
+    # a block running past its allocation (71 columns of a 70-column one)
+    # is refused before any device work
+    c71 = np.full(71, 1e-2)
+    Y71 = dev.DeviceBlock(n, 71)
+    assert dev.lib().lgh_glr_solve_block_dev(
+        g.h, 71, c71.ctypes.data_as(C.POINTER(C.c_double)), Rd.ptr, n, Y71.ptr, n) != 0
+    assert "extends past" in dev.lib().lgh_petsc_last_error().decode()
 
 
 @pytest.mark.parametrize("mixed", [False, True])
