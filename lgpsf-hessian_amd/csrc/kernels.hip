@@ -1948,12 +1948,13 @@ cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
   if (Rpc) vst (Rpc + qi, r);
 }
 
-// mixed-precision fused Chebyshev step: FP64 residual R and accumulator X;
-// preconditioned residual Z in the preconditioner's precision SP (FP32 or
-// BF16), direction D in FP32.  The new direction is rounded to FP32 once
-// and that stored value is what X and R both see, so R stays the FP64
-// residual of X and the iteration reaches FP64 accuracy with a
-// low-precision preconditioner (the rounding only perturbs the step).  Also
+// mixed-precision fused Chebyshev step: FP64 arithmetic on the residual R
+// and the accumulator X; preconditioned residual Z in the preconditioner's
+// precision SP (FP32 or BF16), direction D in FP32.  The new direction is
+// rounded to FP32 once and that stored value is what X and R both see, so
+// R stays the residual of X (up to R's own storage rounding) and the
+// iteration reaches FP64 accuracy with a low-precision preconditioner (the
+// rounding only perturbs the step).  Also
 // writes R in SP for the next preconditioner application.  (A BF16
 // direction saves 4 of ~44 B per element but its 2^-9 rounding, repeated
 // over the 25 steps, costs the solve a factor ~2 in accuracy.)
