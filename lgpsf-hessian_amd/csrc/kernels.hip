@@ -1960,14 +1960,16 @@ cheb64_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
 // over the 25 steps, costs the solve a factor ~2 in accuracy.)
 // RI, RO, XT: storage of the input residual, the new residual and X
 // (double; float in a solve's early phase, and R after its residual
-// replacement, see cheb_mixed_run); the arithmetic is FP64 either way.
+// replacement, see cheb_mixed_run); XI: storage of the X that is read
+// (Xin: X itself, or the early phase's FP32 X at the first update after
+// the replacement); the arithmetic is FP64 either way.
 template <bool HASD, typename SP, int W = 64, typename RI = double,
-          typename RO = double, typename XT = RO>
+          typename RO = double, typename XT = RO, typename XI = XT>
 __global__ void __launch_bounds__ (TPB)
 cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
            const double *__restrict__ v, const SP *__restrict__ Zs,
            double a, double b, const float *__restrict__ D,
-           const RI *__restrict__ R, XT *__restrict__ X, int xmode,
+           const RI *__restrict__ R, const XI *Xin, XT *X, int xmode,
            float *__restrict__ Dn, RO *__restrict__ Rn,
            SP *__restrict__ Rpc)
 {
@@ -2033,7 +2035,7 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
   vld (R + qi, r);
   // X update (MIX_X_*): the deferred forms add the previous direction
   // first, so X ends bit-identical to updating it at every step
-  if (xmode == MIX_X_ADD || xmode == MIX_X_ADD2) vld (X + qi, x);
+  if (xmode == MIX_X_ADD || xmode == MIX_X_ADD2) vld (Xin + qi, x);
 #pragma unroll
   for (int i = 0; i < N; i++) {
     if (xmode == MIX_X_SET) x[i] = di[i];
@@ -2054,7 +2056,7 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
 // values gathered and widened are X's values), and Rpc = R in the
 // V-cycle's storage precision -- in place of a widening copy, an FP64
 // residual that reads X back, and a narrowing copy of R
-template <typename SP, int W = 64, typename RT = double>
+template <typename SP, int W = 64, typename RT = double, bool WX = true>
 __global__ void __launch_bounds__ (TPB)
 mix_switch_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
               const double *__restrict__ v, const double *__restrict__ B,
@@ -2088,13 +2090,16 @@ mix_switch_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
     }
   }
   const size_t q = off + (size_t) row * W;
-  double b[N], xo[N];
+  double b[N];
   vld (B + q, b);
-  vld (X32 + q, xo);
 #pragma unroll
   for (int i = 0; i < N; i++) b[i] -= acc[i];
   vst (R + q, b);
-  vst (X + q, xo);
+  if (WX) {
+    double xo[N];
+    vld (X32 + q, xo);
+    vst (X + q, xo);
+  }
   vst (Rpc + q, b);
 }
 
@@ -2103,27 +2108,35 @@ k_mix_switch (const DevCSR &A, const PMat &B, const PMat &X32, PMat &X,
               PMat &R, PMat &Rpc)
 {
   const int W = B.W;
-  LGH_CHECK (A.prec == F64 && B.prec == F64 && X32.prec == F32 && X.prec == F64
+  const bool wx = X.p != nullptr;
+  LGH_CHECK (A.prec == F64 && B.prec == F64 && X32.prec == F32
+             && (!wx || (X.prec == F64 && X.W == W && X.nrows == A.n
+                         && X.npanels () == B.npanels ()))
              && (R.prec == F64 || R.prec == F32) && (Rpc.prec == F32 || Rpc.prec == BF16)
-             && row_width (W) && X32.W == W && X.W == W && R.W == W && Rpc.W == W
-             && B.nrows == A.n && A.n == A.m && X32.nrows == A.n && X.nrows == A.n
+             && row_width (W) && X32.W == W && R.W == W && Rpc.W == W
+             && B.nrows == A.n && A.n == A.m && X32.nrows == A.n
              && R.nrows == A.n && Rpc.nrows == A.n
-             && X32.npanels () == B.npanels () && X.npanels () == B.npanels ()
+             && X32.npanels () == B.npanels ()
              && R.npanels () == B.npanels () && Rpc.npanels () == B.npanels (),
              8, "k_mix_switch: shape/precision mismatch");
   const double vec = (double) A.n * W;
   ProfScope ps (PK_OTHER, B.npanels () * (csr_bytes (A)
-                + vec * (8 + 4 + 8 + prec_bytes (R.prec) + prec_bytes (Rpc.prec))),
+                + vec * (8 + 4 + (wx ? 8 : 0) + prec_bytes (R.prec) + prec_bytes (Rpc.prec))),
                 B.npanels () * 2.0 * (double) A.nnz * W, B.npanels (), Rpc.prec, W);
   const dim3 g = grid64 (A.n, B.npanels (), F64, W);
   auto go = [&] (auto sp, auto wc, auto rt) -> int {
     using SP = decltype (sp);
     using RT = decltype (rt);
     constexpr int WW = decltype (wc)::value;
-    hipLaunchKernelGGL ((mix_switch_k<SP, WW, RT>), g, dim3 (TPB), 0, stream (),
-                        A.n, A.rp, A.ci, (const double *) A.v, (const double *) B.p,
-                        (const float *) X32.p, (double *) X.p, (RT *) R.p,
-                        (SP *) Rpc.p);
+    if (wx)
+      hipLaunchKernelGGL ((mix_switch_k<SP, WW, RT, true>), g, dim3 (TPB), 0, stream (),
+                          A.n, A.rp, A.ci, (const double *) A.v, (const double *) B.p,
+                          (const float *) X32.p, (double *) X.p, (RT *) R.p,
+                          (SP *) Rpc.p);
+    else
+      hipLaunchKernelGGL ((mix_switch_k<SP, WW, RT, false>), g, dim3 (TPB), 0, stream (),
+                          A.n, A.rp, A.ci, (const double *) A.v, (const double *) B.p,
+                          (const float *) X32.p, nullptr, (RT *) R.p, (SP *) Rpc.p);
     LGH_LAUNCH_CHECK ();
     return 0;
   };
@@ -2137,8 +2150,13 @@ k_mix_switch (const DevCSR &A, const PMat &B, const PMat &X32, PMat &X,
 int
 k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                    const PMat *D32, const PMat &R, PMat &X, int x_mode,
-                   PMat &Dn32, PMat &Rn, PMat &Rnpc)
+                   PMat &Dn32, PMat &Rn, PMat &Rnpc, const PMat *Xin)
 {
+  const bool xin32 = Xin && Xin->p != X.p;
+  LGH_CHECK (!xin32 || (Xin->prec == F32 && X.prec == F64 && R.prec == F32
+                        && Xin->W == X.W && Xin->nrows == X.nrows
+                        && Xin->npanels () == X.npanels ()), 8,
+             "k_cheb_step_mixed: X input shape/precision");
   const int W = R.W;
   // storage (R, Rn, X): all FP64; FP64, FP32, FP32 (a solve's first
   // step); all FP32 (its early phase); FP32, FP32, FP64 (after the
@@ -2159,8 +2177,9 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
   const double vec = (double) A.n * W;
   const double rb = (double) prec_bytes (R.prec), ob = (double) prec_bytes (Rn.prec);
   const double xs = (double) prec_bytes (X.prec);
+  const double xr = xin32 ? 4.0 : xs;
   const double xb = (x_mode == MIX_X_SKIP) ? 0 : (x_mode == MIX_X_SET
-                     || x_mode == MIX_X_SET2) ? xs : 2 * xs;
+                     || x_mode == MIX_X_SET2) ? xs : xr + xs;
   const double pb = (double) prec_bytes (Zpc.prec);
   // PK_CHEB_MIXED times one kernel instantiation: the steps after the
   // residual replacement (FP32 R, FP64 X); the first step of a solve (no
@@ -2172,33 +2191,36 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                 R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * W,
                 R.npanels (), Zpc.prec, W);
   const dim3 g = grid64 (A.n, R.npanels (), F32, W);   // 4 columns per lane
-  auto go = [&] (auto sp, auto wc, auto ri, auto ro, auto xt) -> int {
+  auto go = [&] (auto sp, auto wc, auto ri, auto ro, auto xt, auto xi) -> int {
     using SP = decltype (sp);
     using RI = decltype (ri);
     using RO = decltype (ro);
     using XT = decltype (xt);
+    using XI = decltype (xi);
     constexpr int WW = decltype (wc)::value;
+    const XI *xin = (const XI *) (xin32 ? Xin->p : X.p);
     if (D32)
-      hipLaunchKernelGGL ((cheb64m_k<true, SP, WW, RI, RO, XT>), g, dim3 (TPB), 0,
+      hipLaunchKernelGGL ((cheb64m_k<true, SP, WW, RI, RO, XT, XI>), g, dim3 (TPB), 0,
                           stream (), A.n, A.rp, A.ci, (const double *) A.v,
                           (const SP *) Zpc.p, a, b, (const float *) D32->p,
-                          (const RI *) R.p, (XT *) X.p, x_mode,
+                          (const RI *) R.p, xin, (XT *) X.p, x_mode,
                           (float *) Dn32.p, (RO *) Rn.p, (SP *) Rnpc.p);
     else
-      hipLaunchKernelGGL ((cheb64m_k<false, SP, WW, RI, RO, XT>), g, dim3 (TPB), 0,
+      hipLaunchKernelGGL ((cheb64m_k<false, SP, WW, RI, RO, XT, XI>), g, dim3 (TPB), 0,
                           stream (), A.n, A.rp, A.ci, (const double *) A.v,
                           (const SP *) Zpc.p, a, b, nullptr,
-                          (const RI *) R.p, (XT *) X.p, x_mode,
+                          (const RI *) R.p, xin, (XT *) X.p, x_mode,
                           (float *) Dn32.p, (RO *) Rn.p, (SP *) Rnpc.p);
     LGH_LAUNCH_CHECK ();
     return 0;
   };
   return with_row_W (W, [&] (auto wc) {
     auto by_r = [&] (auto sp) {
-      if (R.prec == F64 && X.prec == F64) return go (sp, wc, 0.0, 0.0, 0.0);
-      if (R.prec == F64) return go (sp, wc, 0.0, 0.f, 0.f);
-      if (X.prec == F64) return go (sp, wc, 0.f, 0.f, 0.0);
-      return go (sp, wc, 0.f, 0.f, 0.f);
+      if (R.prec == F64 && X.prec == F64) return go (sp, wc, 0.0, 0.0, 0.0, 0.0);
+      if (R.prec == F64) return go (sp, wc, 0.0, 0.f, 0.f, 0.f);
+      if (X.prec == F64)
+        return xin32 ? go (sp, wc, 0.f, 0.f, 0.0, 0.f) : go (sp, wc, 0.f, 0.f, 0.0, 0.0);
+      return go (sp, wc, 0.f, 0.f, 0.f, 0.f);
     };
     return Zpc.prec == BF16 ? by_r (bf16_t ()) : by_r (0.f);
   });

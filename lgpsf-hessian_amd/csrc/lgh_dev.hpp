@@ -321,11 +321,15 @@ enum {
   MIX_X_SET2 = 3,     // X = D + Dn            (after a skipped first step)
   MIX_X_ADD2 = 4      // X = (X + D) + Dn      (after a skipped step)
 };
+// Xin: where an adding mode reads X (null: X itself; the early phase's
+// FP32 X at the first update after the residual replacement, X FP64)
 int k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                        const PMat *D32, const PMat &R, PMat &X, int x_mode,
-                       PMat &Dn32, PMat &Rn, PMat &Rnpc);
+                       PMat &Dn32, PMat &Rn, PMat &Rnpc,
+                       const PMat *Xin = nullptr);
 // the mixed solve's residual replacement in one pass: X = X32 widened to
-// FP64, R = B - A X (formed in FP64, the sums of k_resid; stored in R's
+// FP64 (skipped when X.p is null: the next X update reads X32 itself),
+// R = B - A X (formed in FP64, the sums of k_resid; stored in R's
 // precision, FP64 or FP32), Rpc = R in Rpc's storage precision (FP32 or
 // BF16, rounded from the FP64 value as k_copy_cols rounds)
 int k_mix_switch (const DevCSR &A, const PMat &B, const PMat &X32, PMat &X,

@@ -548,8 +548,9 @@ cheb_mixed_run (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
   // holds every direction when it ends).  Half the steps: a longer phase
   // (ending when the Chebyshev bound has shrunk the error by 2^-25, 10 of 16
   // steps here) moved the smallest kept modes of a 16^3 build by 1e-7.
-  // Then the residual replacement: X in FP64 and R = B - A X formed in
-  // FP64; the remaining steps keep X in FP64 and R in FP32.  R's rounding
+  // Then the residual replacement: R = B - A X formed in FP64 from the
+  // FP32 X (the first X update after it reads that X and writes X in
+  // FP64); the remaining steps keep X in FP64 and R in FP32.  R's rounding
   // there is relative to the residual the early phase left (about the
   // Chebyshev bound at k_sw times B), so the drift it adds to the
   // recurrence stays far below the solve's tolerance (1024^2 anisotropic
@@ -561,6 +562,7 @@ cheb_mixed_run (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
   if (!k_sw) LGH_TRY (w.ensure64 ());
   const PMat *Rsrc = &B;
   int ri = 0, di = 0;
+  bool x_set = false;               // X (FP64) written since the replacement
   for (int k = 1; k <= nit; k++) {
     double a, b;
     cc.step (k, &a, &b);
@@ -574,16 +576,23 @@ cheb_mixed_run (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
     const bool early = k <= k_sw;
     PMat &Xk = early ? w.X32 : X;
     PMat &Rn = k_sw ? w.R32[ri] : w.R[ri];
+    // the first X update after the replacement reads the early phase's X32
+    // (widened exactly) instead of an FP64 copy of it
+    const bool first_x = k_sw && !x_set && !early && xm != MIX_X_SKIP;
     LGH_TRY (mg_vcycle (mg, w.Rpc, w.Zpc));
     LGH_TRY (k_cheb_step_mixed (A64, w.Zpc, a, b, Dcur, *Rsrc, Xk, xm,
-                                w.D[di], Rn, w.Rpc));
+                                w.D[di], Rn, w.Rpc, first_x ? &w.X32 : nullptr));
+    if (first_x) x_set = true;
     Rsrc = &Rn;
     ri ^= 1;
     di ^= 1;
     if (k == k_sw) {
-      // residual replacement: X = X32 in FP64, R = B - A X in FP64 (stored
-      // in FP32), and the next preconditioner input from it (one pass)
-      LGH_TRY (k_mix_switch (A64, B, w.X32, X, w.R32[ri], w.Rpc));
+      // residual replacement: R = B - A X32 in FP64 (stored in FP32), and
+      // the next preconditioner input from it (one pass); X in FP64 is
+      // first written by the next update that adds to it
+      PMat none = X;
+      none.p = nullptr;
+      LGH_TRY (k_mix_switch (A64, B, w.X32, none, w.R32[ri], w.Rpc));
       Rsrc = &w.R32[ri];
       ri ^= 1;
     }
