@@ -2052,16 +2052,17 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
 }
 
 // the mixed solve's residual replacement (zsolve.cpp cheb_mixed_run) in
-// one pass: X = X32 widened, R = B - A X with resid64_k's sums (the FP32
-// values gathered and widened are X's values), and Rpc = R in the
-// V-cycle's storage precision -- in place of a widening copy, an FP64
-// residual that reads X back, and a narrowing copy of R
-template <typename SP, int W = 64, typename RT = double, bool WX = true>
+// one pass: R = B - A X with resid64_k's sums over X32 widened (X in FP64
+// is the FP32 X widened; the solve's next X update reads X32 itself), R
+// stored in RT, and Rpc = R in the V-cycle's storage precision -- in place
+// of a widening copy, an FP64 residual that reads X back, and a narrowing
+// copy of R
+template <typename SP, int W = 64, typename RT = double>
 __global__ void __launch_bounds__ (TPB)
 mix_switch_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
               const double *__restrict__ v, const double *__restrict__ B,
-              const float *__restrict__ X32, double *__restrict__ X,
-              RT *__restrict__ R, SP *__restrict__ Rpc)
+              const float *__restrict__ X32, RT *__restrict__ R,
+              SP *__restrict__ Rpc)
 {
   LGH_ROWMAP (double)
   double acc[N];
@@ -2095,23 +2096,15 @@ mix_switch_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
 #pragma unroll
   for (int i = 0; i < N; i++) b[i] -= acc[i];
   vst (R + q, b);
-  if (WX) {
-    double xo[N];
-    vld (X32 + q, xo);
-    vst (X + q, xo);
-  }
   vst (Rpc + q, b);
 }
 
 int
-k_mix_switch (const DevCSR &A, const PMat &B, const PMat &X32, PMat &X,
-              PMat &R, PMat &Rpc)
+k_mix_switch (const DevCSR &A, const PMat &B, const PMat &X32, PMat &R,
+              PMat &Rpc)
 {
   const int W = B.W;
-  const bool wx = X.p != nullptr;
   LGH_CHECK (A.prec == F64 && B.prec == F64 && X32.prec == F32
-             && (!wx || (X.prec == F64 && X.W == W && X.nrows == A.n
-                         && X.npanels () == B.npanels ()))
              && (R.prec == F64 || R.prec == F32) && (Rpc.prec == F32 || Rpc.prec == BF16)
              && row_width (W) && X32.W == W && R.W == W && Rpc.W == W
              && B.nrows == A.n && A.n == A.m && X32.nrows == A.n
@@ -2121,22 +2114,16 @@ k_mix_switch (const DevCSR &A, const PMat &B, const PMat &X32, PMat &X,
              8, "k_mix_switch: shape/precision mismatch");
   const double vec = (double) A.n * W;
   ProfScope ps (PK_OTHER, B.npanels () * (csr_bytes (A)
-                + vec * (8 + 4 + (wx ? 8 : 0) + prec_bytes (R.prec) + prec_bytes (Rpc.prec))),
+                + vec * (8 + 4 + prec_bytes (R.prec) + prec_bytes (Rpc.prec))),
                 B.npanels () * 2.0 * (double) A.nnz * W, B.npanels (), Rpc.prec, W);
   const dim3 g = grid64 (A.n, B.npanels (), F64, W);
   auto go = [&] (auto sp, auto wc, auto rt) -> int {
     using SP = decltype (sp);
     using RT = decltype (rt);
     constexpr int WW = decltype (wc)::value;
-    if (wx)
-      hipLaunchKernelGGL ((mix_switch_k<SP, WW, RT, true>), g, dim3 (TPB), 0, stream (),
-                          A.n, A.rp, A.ci, (const double *) A.v, (const double *) B.p,
-                          (const float *) X32.p, (double *) X.p, (RT *) R.p,
-                          (SP *) Rpc.p);
-    else
-      hipLaunchKernelGGL ((mix_switch_k<SP, WW, RT, false>), g, dim3 (TPB), 0, stream (),
-                          A.n, A.rp, A.ci, (const double *) A.v, (const double *) B.p,
-                          (const float *) X32.p, nullptr, (RT *) R.p, (SP *) Rpc.p);
+    hipLaunchKernelGGL ((mix_switch_k<SP, WW, RT>), g, dim3 (TPB), 0, stream (),
+                        A.n, A.rp, A.ci, (const double *) A.v, (const double *) B.p,
+                        (const float *) X32.p, (RT *) R.p, (SP *) Rpc.p);
     LGH_LAUNCH_CHECK ();
     return 0;
   };

@@ -327,13 +327,12 @@ int k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                        const PMat *D32, const PMat &R, PMat &X, int x_mode,
                        PMat &Dn32, PMat &Rn, PMat &Rnpc,
                        const PMat *Xin = nullptr);
-// the mixed solve's residual replacement in one pass: X = X32 widened to
-// FP64 (skipped when X.p is null: the next X update reads X32 itself),
-// R = B - A X (formed in FP64, the sums of k_resid; stored in R's
+// the mixed solve's residual replacement in one pass: R = B - A X for X
+// the FP32 X32 widened (formed in FP64, the sums of k_resid; stored in R's
 // precision, FP64 or FP32), Rpc = R in Rpc's storage precision (FP32 or
 // BF16, rounded from the FP64 value as k_copy_cols rounds)
-int k_mix_switch (const DevCSR &A, const PMat &B, const PMat &X32, PMat &X,
-                  PMat &R, PMat &Rpc);
+int k_mix_switch (const DevCSR &A, const PMat &B, const PMat &X32, PMat &R,
+                  PMat &Rpc);
 // degree-2 Chebyshev-Jacobi smoother in one pass (W = 64):
 // D1 = b1 dinv R0, R1 = R0 - A D1; D2 = a2 D1 + b2 dinv R1;
 // X = (xadd ? X : 0) + D1 + D2 (D1, R1 recomputed at the neighbours)

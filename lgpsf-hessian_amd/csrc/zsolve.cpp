@@ -590,9 +590,7 @@ cheb_mixed_run (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
       // residual replacement: R = B - A X32 in FP64 (stored in FP32), and
       // the next preconditioner input from it (one pass); X in FP64 is
       // first written by the next update that adds to it
-      PMat none = X;
-      none.p = nullptr;
-      LGH_TRY (k_mix_switch (A64, B, w.X32, none, w.R32[ri], w.Rpc));
+      LGH_TRY (k_mix_switch (A64, B, w.X32, w.R32[ri], w.Rpc));
       Rsrc = &w.R32[ri];
       ri ^= 1;
     }
