@@ -59,9 +59,12 @@ namespace lgh {
 int prior_setup_device (lgh_prior_t *p, int prec);
 // trans: 1 selects the transposed callbacks (Z symmetric on the Mat path)
 // pcp: V-cycle precision when lower than X's (F32 or BF16 around FP64 X:
-// the mixed Chebyshev; BF16 around FP32 X); -1: X's precision
+// the mixed Chebyshev; BF16 around FP32 X); -1: X's precision.
+// yscale (n values or null): Y is returned as diag(yscale) Z^{-1} X, the
+// bits of k_diag_scale after the solve (the mixed Chebyshev writes it from
+// its last step)
 int prior_solve_block (lgh_prior_t *p, int trans, const PMat &X, PMat &Y,
-                       int pcp = -1);
+                       int pcp = -1, const double *yscale = nullptr);
 int prior_solve_vec (lgh_prior_t *p, int trans, const PMat &X, PMat &Y);
 int prior_apply_Z (lgh_prior_t *p, int trans, const PMat &X, PMat &Y);
 void prior_ref (lgh_prior_t *p);

@@ -90,8 +90,8 @@ apply_F (lgh_glr_t *g, const PMat &X, PMat &Y, PMat &Wk, int pcp = -1)
   int rc = k_spmm (*dB, 1.0, Y, 0.0, Wk);
   prof_set_b_tag (0);
   LGH_TRY (rc);
-  LGH_TRY (prior_solve_block (p, 0, Wk, Y, pcp));
-  LGH_TRY (k_diag_scale (Y, p->d_msqrt, Y));
+  // Y = M^{1/2} Z^{-1} Wk (the scaling from the solve's last step)
+  LGH_TRY (prior_solve_block (p, 0, Wk, Y, pcp, p->d_msqrt));
   return 0;
 }
 

@@ -1971,7 +1971,7 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
            double a, double b, const float *__restrict__ D,
            const RI *__restrict__ R, const XI *Xin, XT *X, int xmode,
            float *__restrict__ Dn, RO *__restrict__ Rn,
-           SP *__restrict__ Rpc)
+           SP *__restrict__ Rpc, const double *__restrict__ xsc)
 {
   // 16 lanes per row, 4 values per lane (fewer, wider loads than the fp64
   // map: the gathered Z and D are 8-16 B per lane)
@@ -2046,7 +2046,16 @@ cheb64m_k (int n, const int *__restrict__ rp, const int *__restrict__ ci,
 #pragma unroll
   for (int i = 0; i < N; i++) r[i] -= acc[i];
   ntst (Dn + qi, di);
-  if (xmode != MIX_X_SKIP) vst (X + qi, x);
+  if (xmode != MIX_X_SKIP) {
+    // the solve's last step may hand X over row-scaled (k_diag_scale's
+    // product, rounded after X's own sum)
+    if (xsc) {
+      const double sr = xsc[row];
+#pragma unroll
+      for (int i = 0; i < N; i++) x[i] = sr * x[i];
+    }
+    vst (X + qi, x);
+  }
   vst (Rn + qi, r);
   ntst (Rpc + qi, r);
 }
@@ -2137,9 +2146,12 @@ k_mix_switch (const DevCSR &A, const PMat &B, const PMat &X32, PMat &R,
 int
 k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                    const PMat *D32, const PMat &R, PMat &X, int x_mode,
-                   PMat &Dn32, PMat &Rn, PMat &Rnpc, const PMat *Xin)
+                   PMat &Dn32, PMat &Rn, PMat &Rnpc, const PMat *Xin,
+                   const double *xscale)
 {
   const bool xin32 = Xin && Xin->p != X.p;
+  LGH_CHECK (!xscale || (X.prec == F64 && x_mode != MIX_X_SKIP), 8,
+             "k_cheb_step_mixed: a row-scaled X is written by an FP64 X update");
   LGH_CHECK (!xin32 || (Xin->prec == F32 && X.prec == F64 && R.prec == F32
                         && Xin->W == X.W && Xin->nrows == X.nrows
                         && Xin->npanels () == X.npanels ()), 8,
@@ -2173,7 +2185,7 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
   // D), the early phase (FP32 X) and solves of fewer than 8 steps (FP64 R)
   // are credited to PK_OTHER
   ProfScope ps ((D32 && R.prec == F32 && X.prec == F64) ? PK_CHEB_MIXED : PK_OTHER,
-                R.npanels () * (csr_bytes (A)
+                R.npanels () * (csr_bytes (A) + (xscale ? 8.0 * A.n : 0.0)
                 + vec * (pb + (D32 ? 4 : 0) + rb + xb + 4 + ob + pb)),
                 R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * W,
                 R.npanels (), Zpc.prec, W);
@@ -2191,13 +2203,13 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                           stream (), A.n, A.rp, A.ci, (const double *) A.v,
                           (const SP *) Zpc.p, a, b, (const float *) D32->p,
                           (const RI *) R.p, xin, (XT *) X.p, x_mode,
-                          (float *) Dn32.p, (RO *) Rn.p, (SP *) Rnpc.p);
+                          (float *) Dn32.p, (RO *) Rn.p, (SP *) Rnpc.p, xscale);
     else
       hipLaunchKernelGGL ((cheb64m_k<false, SP, WW, RI, RO, XT, XI>), g, dim3 (TPB), 0,
                           stream (), A.n, A.rp, A.ci, (const double *) A.v,
                           (const SP *) Zpc.p, a, b, nullptr,
                           (const RI *) R.p, xin, (XT *) X.p, x_mode,
-                          (float *) Dn32.p, (RO *) Rn.p, (SP *) Rnpc.p);
+                          (float *) Dn32.p, (RO *) Rn.p, (SP *) Rnpc.p, xscale);
     LGH_LAUNCH_CHECK ();
     return 0;
   };

@@ -322,11 +322,13 @@ enum {
   MIX_X_ADD2 = 4      // X = (X + D) + Dn      (after a skipped step)
 };
 // Xin: where an adding mode reads X (null: X itself; the early phase's
-// FP32 X at the first update after the residual replacement, X FP64)
+// FP32 X at the first update after the residual replacement, X FP64);
+// xscale: X is written multiplied by xscale[row] (FP64 X; the last step)
 int k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                        const PMat *D32, const PMat &R, PMat &X, int x_mode,
                        PMat &Dn32, PMat &Rn, PMat &Rnpc,
-                       const PMat *Xin = nullptr);
+                       const PMat *Xin = nullptr,
+                       const double *xscale = nullptr);
 // the mixed solve's residual replacement in one pass: R = B - A X for X
 // the FP32 X32 widened (formed in FP64, the sums of k_resid; stored in R's
 // precision, FP64 or FP32), Rpc = R in Rpc's storage precision (FP32 or

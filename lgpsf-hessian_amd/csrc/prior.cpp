@@ -315,11 +315,16 @@ cb_block (lgh_prior_t *p, int kind, const PMat &X, PMat &Y)
 
 int
 lgh::prior_solve_block (lgh_prior_t *p, int trans, const PMat &X, PMat &Y,
-                        int pcp)
+                        int pcp, const double *yscale)
 {
-  if (!p->is_mat) return cb_block (p, trans ? 1 : 0, X, Y);
+  // the other tiers scale after the solve
+  auto scaled = [&] (int rc) -> int {
+    if (rc || !yscale) return rc;
+    return k_diag_scale (Y, yscale, Y);
+  };
+  if (!p->is_mat) return scaled (cb_block (p, trans ? 1 : 0, X, Y));
   const int mode = p->o.blocked_mode;
-  if (mode < 2) return prior_solve_vec (p, trans, X, Y);
+  if (mode < 2) return scaled (prior_solve_vec (p, trans, X, Y));
   // a V-cycle in a lower precision than the vectors (W = 64): FP32 or BF16
   // around FP64 vectors, BF16 around FP32 vectors
   const bool lower = (X.prec == F64 && (pcp == F32 || pcp == BF16))
@@ -337,11 +342,11 @@ lgh::prior_solve_block (lgh_prior_t *p, int trans, const PMat &X, PMat &Y,
   if (mixed) {
     if (X.prec == F64)
       return cheb_solve_mixed (*dz, *p->mg[pcp], p->lmin, p->lmax, p->nit, X,
-                               Y, sw ? sw->cwm : p->cwm);
-    return cheb_solve (*dz, p->mg[pcp], p->lmin, p->lmax, p->nit, X, Y, cw);
+                               Y, sw ? sw->cwm : p->cwm, yscale);
+    return scaled (cheb_solve (*dz, p->mg[pcp], p->lmin, p->lmax, p->nit, X, Y, cw));
   }
-  return cheb_solve (*dz, mode <= 3 ? p->mg[X.prec] : nullptr, p->lmin,
-                     p->lmax, p->nit, X, Y, cw);
+  return scaled (cheb_solve (*dz, mode <= 3 ? p->mg[X.prec] : nullptr, p->lmin,
+                             p->lmax, p->nit, X, Y, cw));
 }
 
 int

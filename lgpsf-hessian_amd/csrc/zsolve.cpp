@@ -539,7 +539,8 @@ ChebMixedWork::ensure64 ()
 
 static int
 cheb_mixed_run (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
-                int nit, const PMat &B, PMat &X, ChebMixedWork &w)
+                int nit, const PMat &B, PMat &X, ChebMixedWork &w,
+                const double *xscale)
 {
   LGH_TRY (w.ensure (B, mg.prec));
   ChebCoef cc (lmin, lmax);
@@ -581,7 +582,8 @@ cheb_mixed_run (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
     const bool first_x = k_sw && !x_set && !early && xm != MIX_X_SKIP;
     LGH_TRY (mg_vcycle (mg, w.Rpc, w.Zpc));
     LGH_TRY (k_cheb_step_mixed (A64, w.Zpc, a, b, Dcur, *Rsrc, Xk, xm,
-                                w.D[di], Rn, w.Rpc, first_x ? &w.X32 : nullptr));
+                                w.D[di], Rn, w.Rpc, first_x ? &w.X32 : nullptr,
+                                k == nit ? xscale : nullptr));
     if (first_x) x_set = true;
     Rsrc = &Rn;
     ri ^= 1;
@@ -600,7 +602,8 @@ cheb_mixed_run (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
 
 int
 cheb_solve_mixed (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
-                  int nit, const PMat &B, PMat &X, ChebMixedWork &w)
+                  int nit, const PMat &B, PMat &X, ChebMixedWork &w,
+                  const double *xscale)
 {
   LGH_CHECK (nit >= 1 && B.W == 64 && X.W == 64 && B.prec == F64
              && X.prec == F64 && B.ncols == X.ncols
@@ -610,9 +613,9 @@ cheb_solve_mixed (const DevCSR &A64, DevMG &mg, double lmin, double lmax,
   LGH_TRY (split_tail (B, X, w, &done,
                        [&] (const PMat &Bp, PMat &Xp, ChebMixedWork &wp) {
                          return cheb_mixed_run (A64, mg, lmin, lmax, nit, Bp,
-                                                Xp, wp);
+                                                Xp, wp, xscale);
                        }));
-  return done ? 0 : cheb_mixed_run (A64, mg, lmin, lmax, nit, B, X, w);
+  return done ? 0 : cheb_mixed_run (A64, mg, lmin, lmax, nit, B, X, w, xscale);
 }
 
 int

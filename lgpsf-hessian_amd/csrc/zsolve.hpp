@@ -84,9 +84,11 @@ struct ChebMixedWork {
   int ensure (const PMat &like, int pc);
   int ensure64 ();                     // R[0], R[1] at the current shape
 };
+// xscale (n FP64 values or null): X is returned row-scaled, diag(xscale) X,
+// with the bits of k_diag_scale applied after the solve
 int cheb_solve_mixed (const DevCSR &A64, DevMG &mg, double lmin,
                       double lmax, int nit, const PMat &B, PMat &X,
-                      ChebMixedWork &w);
+                      ChebMixedWork &w, const double *xscale = nullptr);
 
 // Work buffers of block Z-solves that a thread brings itself, so that
 // several threads can solve with one prior at once (the serving loop's
