@@ -31,9 +31,9 @@ FP32_PEAK_TFS = 157.3
 # the V-cycle storage type of the FP64 sweeps
 def kernel_of(kind, vc):
     st = {"bf16": "lgh::bf16_t", "f32": "float"}[vc]
-    return {"cheb_mixed": "lgh::cheb64m_k<true, %s, 64, float, float, double>" % st,
-            "smooth_pre_fine": "lgh::smx_k<float, float, %s, false, 64>" % st,
-            "smooth_post_fine": "lgh::smx_k<float, float, %s, true, 64>" % st,
+    return {"cheb_mixed": "lgh::cheb64m_k<true, %s, 64, float, float, double, double>" % st,
+            "smooth_pre_fine": "lgh::smx_k<float, float, %s, false, 64, true>" % st,
+            "smooth_post_fine": "lgh::smx_k<float, float, %s, true, 64, true>" % st,
             "resid_fine": "lgh::resid64_k<float, float, %s, 64>" % st}[kind]
 
 

@@ -2181,10 +2181,12 @@ k_cheb_step_mixed (const DevCSR &A, const PMat &Zpc, double a, double b,
                      || x_mode == MIX_X_SET2) ? xs : xr + xs;
   const double pb = (double) prec_bytes (Zpc.prec);
   // PK_CHEB_MIXED times one kernel instantiation: the steps after the
-  // residual replacement (FP32 R, FP64 X); the first step of a solve (no
-  // D), the early phase (FP32 X) and solves of fewer than 8 steps (FP64 R)
+  // residual replacement (FP32 R, FP64 X read and written); the first step
+  // of a solve (no D), the early phase (FP32 X), the first X update after
+  // the replacement (FP32 X read) and solves of fewer than 8 steps (FP64 R)
   // are credited to PK_OTHER
-  ProfScope ps ((D32 && R.prec == F32 && X.prec == F64) ? PK_CHEB_MIXED : PK_OTHER,
+  ProfScope ps ((D32 && R.prec == F32 && X.prec == F64 && !xin32) ? PK_CHEB_MIXED
+                : PK_OTHER,
                 R.npanels () * (csr_bytes (A) + (xscale ? 8.0 * A.n : 0.0)
                 + vec * (pb + (D32 ? 4 : 0) + rb + xb + 4 + ob + pb)),
                 R.npanels () * (2.0 * (double) A.nnz + 6.0 * A.n) * W,
