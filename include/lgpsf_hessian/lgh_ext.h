@@ -155,7 +155,7 @@ typedef struct lgh_serve_opts
   int    queue_cols;        /* queued columns before submit blocks          */
   int    device_streams;    /* batches on the device at once, each on its own
                                device thread, streams and Z-solve buffers
-                               (1 .. 8, default 2; the budget is per batch) */
+                               (1 .. 8, default 3; the budget is per batch) */
 }
 lgh_serve_opts_t;
 

@@ -327,7 +327,10 @@ lgh_serve_opts_default (void)
   o.max_batch_cols = 256;
   o.device_budget_gb = 32.0;
   o.queue_cols = 1024;
-  o.device_streams = 2;
+  // three batches on the device at once: at 125 requests/s of the 10 000-
+  // request mix on the 1024^2 build, p50/p99 78/208 ms against 99/289 ms
+  // with two, and 1 470-1 530 against 1 250-1 370 RHS/s saturated
+  o.device_streams = 3;
   return o;
 }
 
