@@ -1,6 +1,6 @@
 /* lgh_oracle.c — FP64 CPU restatement of the reference GLR path.
  *
- * TEST INFRASTRUCTURE ONLY (the parity checker and the timed CPU baseline).
+ * TEST INFRASTRUCTURE ONLY (the parity reference and the timed CPU baseline).
  * The product library never links or calls it.  Each function cites the
  * reference function it restates (paths relative to the reference repo).
  * Compiled with plain gcc -O2 (no -ffast-math, no FMA contraction), so the

@@ -1,5 +1,5 @@
 /* lgh_oracle.h — FP64 CPU restatement of lgpsf-hessian's GLR stage and its
- * downstream operations, used ONLY as the parity checker and the timed CPU
+ * downstream operations, used ONLY as the parity reference and the timed CPU
  * baseline.  TEST INFRASTRUCTURE: nothing in the product library links,
  * loads or calls this code; only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may.
