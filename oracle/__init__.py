@@ -1,8 +1,8 @@
 """ctypes bindings of the FP64 CPU oracle (oracle/lgh_oracle.c).
 
 TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py import this package.  It is the parity reference and
-the CPU baseline, never part of the product path.
+cpu_baseline leg of bench.py import this package.  It is the parity
+reference and the CPU baseline, never part of the product path.
 """
 import ctypes as C
 import os
