@@ -7,11 +7,11 @@
 // pipeline over pinned staging slots: the stager forms a batch from
 // everything queued whenever a slot is free (all requests of the oldest
 // request's operation that fit in the batch's columns, each column with its
-// own shift) and copies its columns into the slot; device threads (two by
-// default), each with its own streams and Z-solve work buffers
+// own shift) and copies its columns into the slot; device threads (three
+// by default), each with its own streams and Z-solve work buffers
 // (SolveWork), run batches in place in their slots, so a batch that forms
-// while another runs starts on the device at once and the two overlap
-// there (their coarse multigrid levels leave most of the device idle); the
+// while others run starts on the device at once and they overlap there
+// (their coarse multigrid levels leave most of the device idle); the
 // finisher copies the results out to the callers and frees the slot.  The
 // host copies of one batch overlap the device work of the others.
 // Requests that arrive after a batch is formed wait for the next one
