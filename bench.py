@@ -461,7 +461,7 @@ def main():
         log("copied 3 x %d bytes" % (x.numel() * 4))
         return
     if args.gpus != 1:
-        print("bench.py runs on one GPU only (this build has no multi-GPU "
+        print("bench.py runs on one GPU only (the engine has no multi-GPU "
               "path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 

@@ -39,7 +39,7 @@ int lgh_prior_create_callbacks (Vec mass_lumps,
 /* replaces prior.h:76-91.  blocked_mode:
  *   0 — per-column solves (block PCG, V-cycle preconditioner, ksp_rtol);
  *   2, 3 — blocked Chebyshev around the AMG V-cycle (host-built hierarchy,
- *          applied on the GPU; 2 and 3 are the same tier in this build);
+ *          applied on the GPU; 2 and 3 are the same tier in this engine);
  *   4 — blocked Chebyshev with the Jacobi preconditioner, one fused kernel
  *       per iteration (extension; no reference counterpart).
  * Vector solves (solve / sample / factor) always use the PCG path.       */

@@ -1,4 +1,4 @@
-/* petscksp.h — the KSP type is opaque in the PETSc subset: this build has
+/* petscksp.h — the KSP type is opaque in the PETSc subset: the engine has
  * no KSP implementation (lgh_prior_create_ksp reports PETSC_ERR_SUP). */
 #ifndef LGH_PETSCKSP_H
 #define LGH_PETSCKSP_H

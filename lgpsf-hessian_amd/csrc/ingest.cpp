@@ -1,5 +1,5 @@
 // ingest.cpp — B from the fit stage's row export (no reference counterpart
-// in this build's API: it replaces the lgh_fit_get_rows -> lgh_fit_get_mat
+// in the engine's API: it replaces the lgh_fit_get_rows -> lgh_fit_get_mat
 // round trip of fit.h:160-165 and glr.h:274-282 / fit_impl.hpp:893-968,
 // where every entry goes through MatSetValues).  The rows are validated,
 // sorted by column (duplicates summed), optionally checked for symmetry,

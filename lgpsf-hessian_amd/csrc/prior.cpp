@@ -80,7 +80,7 @@ extern "C" int
 lgh_prior_create_ksp (KSP, Vec, const lgh_prior_mat_opts_t *, lgh_prior_t **)
 {
   return lgh_petsc_error (PETSC_ERR_SUP,
-                          "lgh_prior_create_ksp: this build has no KSP; use "
+                          "lgh_prior_create_ksp: the engine has no KSP; use "
                           "lgh_prior_create_mat or lgh_prior_create_callbacks");
 }
 

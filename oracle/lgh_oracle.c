@@ -655,7 +655,7 @@ orc_mg_set_cycle (orc_mg_t *h, int gamma)
  * initial guess (zero for a plain cycle): the correction-form
  * pre-smoother starts from B - A X.  With gamma = 2 a level above the
  * coarsest corrects twice, the second cycle starting from the first
- * one's result (W-cycle; the cycle index is this build's choice, the
+ * one's result (W-cycle; the cycle index is the engine's choice, the
  * reference's PCMG cycle is the V-cycle) */
 static void
 mg_run (orc_mg_t *h, int l, int ncols, const double *B, double *X)
